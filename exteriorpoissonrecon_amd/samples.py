@@ -1,0 +1,162 @@
+"""Synthetic oriented-sample generators for the benchmark workloads.
+
+Each generator returns ``(positions, skews)`` with ``positions`` of shape
+``(n, dim)`` and ``skews`` of shape ``(n, dim, dim)`` holding the full
+skew-symmetric frame matrix ``m[c][r]`` that the PLY properties
+``skew_c_r`` store (reference reader: ``Misha/PlyVertexData.inl:228-234``,
+``ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:282-303``).
+
+For curves in 3D the frame is the Hodge dual of the unit tangent, using the
+reference sample convention ``n((d+1)%3,(d+2)%3) = t[d]``
+(``Include/Samples.h:74-79``).  For surfaces in 4D the frame is the tangent
+2-blade ``(u v^T - v u^T)/2`` of the two tangent vectors.
+"""
+import numpy as np
+
+
+def dual3(t):
+    """Skew matrix n with n[(d+1)%3,(d+2)%3] = t[d] and n antisymmetric."""
+    n = np.zeros(t.shape[:-1] + (3, 3))
+    for d in range(3):
+        a, b = (d + 1) % 3, (d + 2) % 3
+        n[..., a, b] = t[..., d]
+        n[..., b, a] = -t[..., d]
+    return n
+
+
+def blade(u, v):
+    """Skew matrix (u v^T - v u^T)/2 of two vectors (batched)."""
+    return 0.5 * (u[..., :, None] * v[..., None, :] - v[..., :, None] * u[..., None, :])
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def torus_knot(n, p=2, q=3, seed=0):
+    """(p,q) torus knot ((2+cos qt)cos pt, (2+cos qt)sin pt, sin qt), t ~ U[0,2pi)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = rng.uniform(0.0, 2.0 * np.pi, n)
+    r = 2.0 + np.cos(q * t)
+    pos = np.stack([r * np.cos(p * t), r * np.sin(p * t), np.sin(q * t)], axis=-1)
+    dr = -q * np.sin(q * t)
+    tan = np.stack([dr * np.cos(p * t) - r * p * np.sin(p * t),
+                    dr * np.sin(p * t) + r * p * np.cos(p * t),
+                    q * np.cos(q * t)], axis=-1)
+    return pos, dual3(_unit(tan))
+
+
+def _rotation(rng, axis, angle):
+    axis = _unit(axis)
+    k = np.zeros(axis.shape[:-1] + (3, 3))
+    k[..., 0, 1], k[..., 0, 2] = -axis[..., 2], axis[..., 1]
+    k[..., 1, 0], k[..., 1, 2] = axis[..., 2], -axis[..., 0]
+    k[..., 2, 0], k[..., 2, 1] = -axis[..., 1], axis[..., 0]
+    s = np.sin(angle)[..., None, None]
+    c = np.cos(angle)[..., None, None]
+    eye = np.broadcast_to(np.eye(3), k.shape)
+    return eye + s * k + (1 - c) * (k @ k)
+
+
+def borromean(n=16384, pos_sigma=0.005, ang_deg=5.0, spot=0.1, seed=0):
+    """Three mutually orthogonal ellipses (semi-axes 1.0, 0.5) in the xy, yz and
+    zx planes with positional/angular noise and a fraction of uniform spot
+    samples with random skew frames."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    counts = [n // 3 + (1 if i < n % 3 else 0) for i in range(3)]
+    pos_l, tan_l = [], []
+    for k, m in enumerate(counts):
+        t = rng.uniform(0.0, 2.0 * np.pi, m)
+        a, b = np.cos(t), 0.5 * np.sin(t)
+        da, db = -np.sin(t), 0.5 * np.cos(t)
+        p = np.zeros((m, 3)); tt = np.zeros((m, 3))
+        i, j = k, (k + 1) % 3
+        p[:, i], p[:, j] = a, b
+        tt[:, i], tt[:, j] = da, db
+        pos_l.append(p); tan_l.append(tt)
+    pos = np.concatenate(pos_l); tan = _unit(np.concatenate(tan_l))
+    diam = np.linalg.norm(pos.max(0) - pos.min(0))
+    pos = pos + rng.normal(0.0, pos_sigma * diam, pos.shape)
+    axis = rng.normal(size=(n, 3))
+    ang = np.deg2rad(ang_deg) * rng.uniform(0.0, 1.0, n)
+    tan = np.einsum("nij,nj->ni", _rotation(rng, axis, ang), tan)
+    skew = dual3(_unit(tan))
+    ns = int(np.ceil(n * spot))
+    lo, hi = pos.min(0), pos.max(0)
+    spos = rng.uniform(lo, hi, (ns, 3))
+    sv = rng.uniform(-1.0, 1.0, (ns, 3))
+    return np.concatenate([pos, spos]), np.concatenate([skew, dual3(sv)])
+
+
+def clifford_torus(side=256, ang_deg=2.0, seed=0):
+    """(cos u, sin u, cos v, sin v)/sqrt(2), u,v ~ U[0,2pi)^2, side^2 samples,
+    frame = tangent 2-blade of (d/du, d/dv), with small angular noise."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = side * side
+    u = rng.uniform(0.0, 2.0 * np.pi, n)
+    v = rng.uniform(0.0, 2.0 * np.pi, n)
+    s = 1.0 / np.sqrt(2.0)
+    pos = s * np.stack([np.cos(u), np.sin(u), np.cos(v), np.sin(v)], -1)
+    tu = np.stack([-np.sin(u), np.cos(u), 0 * u, 0 * u], -1)
+    tv = np.stack([0 * u, 0 * u, -np.sin(v), np.cos(v)], -1)
+    if ang_deg > 0:
+        eps = np.deg2rad(ang_deg)
+        tu = _unit(tu + eps * rng.normal(size=tu.shape) / 2.0)
+        tv = _unit(tv + eps * rng.normal(size=tv.shape) / 2.0)
+    return pos, blade(tu, tv)
+
+
+def write_ply(path, pos, skew):
+    """Binary little-endian PLY with x,y,z[,w] and skew_c_r (double)."""
+    n, dim = pos.shape
+    names = ["x", "y", "z", "w"][:dim]
+    props = names + [f"skew_{c}_{r}" for c in range(dim) for r in range(dim)]
+    header = "ply\nformat binary_little_endian 1.0\n"
+    header += f"element vertex {n}\n"
+    header += "".join(f"property double {p}\n" for p in props)
+    header += "element face 0\nproperty list uchar int vertex_indices\nend_header\n"
+    data = np.concatenate([pos, skew.reshape(n, dim * dim)], axis=1).astype("<f8")
+    with open(path, "wb") as f:
+        f.write(header.encode("ascii"))
+        f.write(data.tobytes())
+
+
+def read_ply(path):
+    """Reads a PLY written by write_ply (binary little endian or ASCII)."""
+    with open(path, "rb") as f:
+        raw = f.read()
+    end = raw.index(b"end_header") + len(b"end_header")
+    while raw[end:end + 1] in (b"\r", b"\n"):
+        end += 1
+        if raw[end - 1:end] == b"\n":
+            break
+    header = raw[:end].decode("ascii").splitlines()
+    n = 0; props = []; ascii_fmt = False; in_vertex = False
+    for line in header:
+        tok = line.split()
+        if not tok:
+            continue
+        if tok[0] == "format":
+            ascii_fmt = tok[1] == "ascii"
+        elif tok[0] == "element":
+            in_vertex = tok[1] == "vertex"
+            if in_vertex:
+                n = int(tok[2])
+        elif tok[0] == "property" and in_vertex:
+            props.append((tok[-1], tok[1]))
+    types = {"double": "<f8", "float": "<f4", "int": "<i4", "uchar": "u1"}
+    dt = np.dtype([(nm, types[t]) for nm, t in props])
+    if ascii_fmt:
+        body = raw[end:].decode("ascii").split()
+        vals = np.array(body[: n * len(props)], dtype=np.float64).reshape(n, len(props))
+        rec = {nm: vals[:, i] for i, (nm, _) in enumerate(props)}
+    else:
+        arr = np.frombuffer(raw, dtype=dt, count=n, offset=end)
+        rec = {nm: arr[nm].astype(np.float64) for nm, _ in props}
+    dim = 4 if "w" in rec else 3
+    pos = np.stack([rec[c] for c in ["x", "y", "z", "w"][:dim]], -1)
+    skew = np.zeros((n, dim, dim))
+    for c in range(dim):
+        for r in range(dim):
+            skew[:, c, r] = rec[f"skew_{c}_{r}"]
+    return pos, skew

@@ -1,0 +1,86 @@
+/*
+ * CPU oracle for the ExteriorPoissonRecon solve path (dense grid, --aRadius -1).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a from-scratch FP64 restatement of the
+ * reference algorithm, written to reproduce the reference's arithmetic in the
+ * reference's order wherever that order is observable.  Only tests/,
+ * __graft_entry__.smoke() and the CPU-baseline leg of bench.py may load it, and
+ * only as the checker; the product library never links or calls it.
+ *
+ * Every function in epr_oracle.cpp cites the reference file:line it follows.
+ * Parity status of each piece is pinned by tests/test_oracle_golden.py against
+ * vectors written by the reference itself (oracle/ref_dump.cpp).  The one
+ * piece restated with a different method is the Dim x Dim symmetric
+ * eigen-solve in the noise estimate (reference: Eigen 3.3.7
+ * SelfAdjointEigenSolver, Include/GridSamples.inl:94); the oracle uses cyclic
+ * Jacobi, equal to rounding.
+ */
+#ifndef EPR_ORACLE_H
+#define EPR_ORACLE_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct
+{
+	int    depth;          /* --depth            (ExteriorPoissonRecon.cpp:65)  */
+	int    seed;           /* --seed             (:66)                          */
+	int    tRadius;        /* --tRadius          (:67)                          */
+	int    dRadius;        /* --dRadius          (:68)                          */
+	int    minSolveDepth;  /* --minSolveDepth    (:70)                          */
+	int    maxSolveDepth;  /* --maxSolveDepth    (:72), <0 means "= depth"      */
+	int    gsIters;        /* --gsIters          (:71)                          */
+	int    singleLevel;    /* --singleLevel      (:76)                          */
+	double nSigma;         /* --nSigma           (:57)                          */
+	double scale;          /* --scale            (:58)                          */
+	double sWeight;        /* --sWeight          (:59), scales R                */
+	double dWeight;        /* --dWeight          (:60), scales the stiffness    */
+	double iMult;          /* --iMult            (:61)                          */
+} orc_params;
+
+void   orc_default_params( orc_params *p );
+
+/* Builds everything the solve needs (ExteriorPoissonRecon.cpp:163-236).
+ * pos: n*dim world positions; skew: n*dim*dim full frame matrices m[c][r]
+ * (PLY properties skew_c_r).  Returns NULL on error (message via orc_error). */
+void  *orc_create( int dim , const orc_params *p , size_t n , const double *pos , const double *skew );
+void   orc_destroy( void *h );
+const char *orc_error( void );
+
+int    orc_dim( void *h );
+size_t orc_num_functions( void *h , int level );
+size_t orc_num_elements( void *h , int level );
+
+/* Named arrays: unit_pos, skew_packed, unit_to_world, cells, cell_counts,
+ * measure, noise, sample_depth, S_elem, c.  Returns the count (writes at most
+ * cap values when out!=NULL). */
+size_t orc_get( void *h , const char *name , double *out , size_t cap );
+
+/* y = M x for M in {'R','B'} at a level (N_l x N_l), or 'P' (prolongation
+ * from level into level+1). */
+void   orc_matvec( void *h , int level , char which , const double *x , double *y );
+
+/* (E_fit, E_reg) at a level, SystemEnergy.h:486-497 */
+void   orc_energies( void *h , int level , const double *x , const double *y , double out[2] );
+/* wedge square norm ||dx^dy||^2, Hat.products.inl:189-231 */
+double orc_wedge( void *h , int level , const double *x , const double *y );
+/* per-node quadratic fit (q20,q02,q11,q10,q01), SystemEnergy.h:542-643 */
+void   orc_qfit( void *h , int level , const double *x , const double *y , double *q );
+/* paired multicolour Gauss-Seidel, ExteriorPoissonSolvers.h:156-196 */
+void   orc_gs( void *h , int level , double *x , double *y , int iters );
+/* the random initial guess at minSolveDepth, ExteriorPoisson.h:333-341 */
+void   orc_init( void *h , double *x , double *y );
+/* the whole cascadic solve, ExteriorPoisson.h:316-409.  hx/hy receive the
+ * hierarchical coefficients of every level concatenated (level 0 first);
+ * fx/fy (may be NULL) the finest-level non-hierarchical solution; E the final
+ * (E_fit, E_reg). */
+void   orc_solve( void *h , double *hx , double *hy , double *fx , double *fy , double E[2] );
+/* writes <hdr>.tree in the reference's byte layout (ExteriorPoissonRecon.cpp:259-270) */
+int    orc_write_tree( void *h , const char *path , const double *hx , const double *hy , unsigned long long indexerHash );
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,174 @@
+"""ctypes binding of oracle/_build/liboracle.so (test infrastructure only).
+
+The oracle is a CPU restatement of the reference algorithm; see epr_oracle.h.
+"""
+import ctypes
+import os
+import struct
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_int), ("seed", ctypes.c_int), ("tRadius", ctypes.c_int),
+                ("dRadius", ctypes.c_int), ("minSolveDepth", ctypes.c_int),
+                ("maxSolveDepth", ctypes.c_int), ("gsIters", ctypes.c_int),
+                ("singleLevel", ctypes.c_int), ("nSigma", ctypes.c_double),
+                ("scale", ctypes.c_double), ("sWeight", ctypes.c_double),
+                ("dWeight", ctypes.c_double), ("iMult", ctypes.c_double)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        vp, dp, sz = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t
+        L.orc_default_params.argtypes = [ctypes.POINTER(Params)]
+        L.orc_create.restype = vp
+        L.orc_create.argtypes = [ctypes.c_int, ctypes.POINTER(Params), sz, dp, dp]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_error.restype = ctypes.c_char_p
+        L.orc_num_functions.restype = sz
+        L.orc_num_functions.argtypes = [vp, ctypes.c_int]
+        L.orc_num_elements.restype = sz
+        L.orc_num_elements.argtypes = [vp, ctypes.c_int]
+        L.orc_get.restype = sz
+        L.orc_get.argtypes = [vp, ctypes.c_char_p, dp, sz]
+        L.orc_matvec.argtypes = [vp, ctypes.c_int, ctypes.c_char, dp, dp]
+        L.orc_energies.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+        L.orc_wedge.restype = ctypes.c_double
+        L.orc_wedge.argtypes = [vp, ctypes.c_int, dp, dp]
+        L.orc_qfit.argtypes = [vp, ctypes.c_int, dp, dp, dp]
+        L.orc_gs.argtypes = [vp, ctypes.c_int, dp, dp, ctypes.c_int]
+        L.orc_init.argtypes = [vp, dp, dp]
+        L.orc_solve.argtypes = [vp, dp, dp, dp, dp, dp]
+        L.orc_write_tree.argtypes = [vp, ctypes.c_char_p, dp, dp, ctypes.c_ulonglong]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_params(**kw):
+    p = Params()
+    lib().orc_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+class Oracle:
+    """One reconstruction built by the CPU oracle."""
+
+    def __init__(self, pos, skew, **params):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        skew = np.ascontiguousarray(skew, dtype=np.float64)
+        self.dim = pos.shape[1]
+        self.params = default_params(**params)
+        self.h = lib().orc_create(self.dim, ctypes.byref(self.params), pos.shape[0], _p(pos), _p(skew))
+        if not self.h:
+            raise RuntimeError(lib().orc_error().decode())
+        self.depth = self.params.depth
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def nfun(self, level):
+        return lib().orc_num_functions(self.h, level)
+
+    def nelem(self, level):
+        return lib().orc_num_elements(self.h, level)
+
+    def get(self, name):
+        n = lib().orc_get(self.h, name.encode(), None, 0)
+        out = np.zeros(n)
+        lib().orc_get(self.h, name.encode(), _p(out), n)
+        return out
+
+    def matvec(self, level, which, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        n_out = self.nfun(level + 1) if which == "P" else self.nfun(level)
+        y = np.zeros(n_out)
+        lib().orc_matvec(self.h, level, which.encode(), _p(x), _p(y))
+        return y
+
+    def energies(self, level, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64); y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.zeros(2)
+        lib().orc_energies(self.h, level, _p(x), _p(y), _p(out))
+        return out
+
+    def wedge(self, level, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64); y = np.ascontiguousarray(y, dtype=np.float64)
+        return lib().orc_wedge(self.h, level, _p(x), _p(y))
+
+    def qfit(self, level, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64); y = np.ascontiguousarray(y, dtype=np.float64)
+        q = np.zeros(5 * self.nfun(level))
+        lib().orc_qfit(self.h, level, _p(x), _p(y), _p(q))
+        return q.reshape(-1, 5)
+
+    def gs(self, level, x, y, iters=1):
+        x = np.array(x, dtype=np.float64); y = np.array(y, dtype=np.float64)
+        lib().orc_gs(self.h, level, _p(x), _p(y), iters)
+        return x, y
+
+    def init(self):
+        n = self.nfun(min(self.params.minSolveDepth, self.depth))
+        x = np.zeros(n); y = np.zeros(n)
+        lib().orc_init(self.h, _p(x), _p(y))
+        return x, y
+
+    def solve(self):
+        tot = sum(self.nfun(l) for l in range(self.depth + 1))
+        hx = np.zeros(tot); hy = np.zeros(tot)
+        n = self.nfun(self.depth)
+        fx = np.zeros(n); fy = np.zeros(n); E = np.zeros(2)
+        lib().orc_solve(self.h, _p(hx), _p(hy), _p(fx), _p(fy), _p(E))
+        out = {"final_x": fx, "final_y": fy, "final_E": E}
+        o = 0
+        for l in range(self.depth + 1):
+            m = self.nfun(l)
+            out[f"hx{l}"] = hx[o:o + m]; out[f"hy{l}"] = hy[o:o + m]
+            o += m
+        out["_hx"] = hx; out["_hy"] = hy
+        return out
+
+    def write_tree(self, path, hx, hy, indexer_hash):
+        hx = np.ascontiguousarray(hx); hy = np.ascontiguousarray(hy)
+        return lib().orc_write_tree(self.h, path.encode(), _p(hx), _p(hy), indexer_hash)
+
+
+def probe(n, salt):
+    """The probe vectors of oracle/ref_dump.cpp: v[i] = cos(0.7*i + salt)."""
+    return np.cos(0.7 * np.arange(n, dtype=np.float64) + salt)
+
+
+def load_dump(path):
+    """Reads a ref_dump file: {u32 nameLen, name, u64 count, count*f64}*."""
+    d = {}
+    b = open(path, "rb").read()
+    o = 0
+    while o < len(b):
+        l = struct.unpack_from("<I", b, o)[0]; o += 4
+        name = b[o:o + l].decode(); o += l
+        c = struct.unpack_from("<Q", b, o)[0]; o += 8
+        d[name] = np.frombuffer(b, dtype="<f8", count=c, offset=o).copy(); o += 8 * c
+    return d
