@@ -1,0 +1,7 @@
+"""MI355X-native (gfx950 HIP) ExteriorPoissonRecon solve path.
+
+`Reconstructor` mirrors ExteriorPoisson::Reconstructor; the work runs in
+libepr_hip.so (C ABI: include/epr.h).
+"""
+from ._lib import EXPORTS, EprError, LIB_PATH, default_params, load  # noqa: F401
+from .reconstructor import Reconstructor  # noqa: F401

@@ -1,0 +1,91 @@
+"""ctypes binding of libepr_hip.so, the HIP/gfx950 library behind include/epr.h.
+
+The library is loaded from this package directory; a missing or unloadable
+library is an error (there is no CPU fallback).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libepr_hip.so")
+
+EXPORTS = [
+    "epr_default_params", "epr_create", "epr_destroy", "epr_last_error", "epr_set_samples",
+    "epr_build", "epr_init", "epr_gs_sweeps", "epr_prolong_add", "epr_energies", "epr_solve",
+    "epr_num_functions", "epr_get_coeffs", "epr_set_coeffs", "epr_quadratic_fits", "epr_apply",
+    "epr_get_array", "epr_write_tree", "epr_write_grid", "epr_last_seconds",
+]
+
+
+class Params(ctypes.Structure):
+    """epr_params (include/epr.h), the reference's command-line parameters."""
+    _fields_ = [("depth", ctypes.c_int), ("seed", ctypes.c_int), ("tRadius", ctypes.c_int),
+                ("dRadius", ctypes.c_int), ("minSolveDepth", ctypes.c_int),
+                ("maxSolveDepth", ctypes.c_int), ("gsIters", ctypes.c_int),
+                ("singleLevel", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("nSigma", ctypes.c_double), ("scale", ctypes.c_double),
+                ("sWeight", ctypes.c_double), ("dWeight", ctypes.c_double),
+                ("iMult", ctypes.c_double)]
+
+
+class EprError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def load():
+    """Loads libepr_hip.so (raises if it is missing: build it with __graft_entry__.build())."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise EprError(f"{LIB_PATH} is missing; build it with `python -c 'import __graft_entry__ as g; g.build()'`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp, sz, ci = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double), ctypes.c_size_t, ctypes.c_int
+    L.epr_default_params.argtypes = [ctypes.POINTER(Params)]
+    L.epr_create.argtypes = [ci, ctypes.POINTER(Params), ctypes.POINTER(vp)]
+    L.epr_destroy.argtypes = [vp]
+    L.epr_last_error.argtypes = [vp]
+    L.epr_last_error.restype = ctypes.c_char_p
+    L.epr_set_samples.argtypes = [vp, sz, dp, dp]
+    L.epr_build.argtypes = [vp]
+    L.epr_init.argtypes = [vp, ci, dp, dp]
+    L.epr_gs_sweeps.argtypes = [vp, ci, ci]
+    L.epr_prolong_add.argtypes = [vp, ci]
+    L.epr_energies.argtypes = [vp, ci, dp]
+    L.epr_solve.argtypes = [vp, dp]
+    L.epr_num_functions.argtypes = [vp, ci]
+    L.epr_num_functions.restype = sz
+    L.epr_get_coeffs.argtypes = [vp, ci, dp, dp]
+    L.epr_set_coeffs.argtypes = [vp, ci, dp, dp]
+    L.epr_quadratic_fits.argtypes = [vp, ci, dp]
+    L.epr_apply.argtypes = [vp, ci, ctypes.c_char, dp, dp]
+    L.epr_get_array.argtypes = [vp, ctypes.c_char_p, dp, sz, ctypes.POINTER(sz)]
+    L.epr_write_tree.argtypes = [vp, ctypes.c_char_p, ctypes.c_ulonglong]
+    L.epr_write_grid.argtypes = [vp, ctypes.c_char_p]
+    L.epr_last_seconds.argtypes = [vp, ci]
+    L.epr_last_seconds.restype = ctypes.c_double
+    for name in ["epr_create", "epr_set_samples", "epr_build", "epr_init", "epr_gs_sweeps",
+                 "epr_prolong_add", "epr_energies", "epr_solve", "epr_get_coeffs", "epr_set_coeffs",
+                 "epr_quadratic_fits", "epr_apply", "epr_get_array", "epr_write_tree", "epr_write_grid"]:
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def default_params(**kw):
+    p = Params()
+    load().epr_default_params(ctypes.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError(f"unknown parameter {k}")
+        setattr(p, k, v)
+    return p

@@ -1,0 +1,116 @@
+// ExteriorPoissonRecon command line over libepr_hip.so (dense --aRadius -1 path).
+//
+// Same flags, defaults and outputs as ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:52-279:
+// reads <in>.ply, writes <out>.tree (reference byte layout) and, with --grid,
+// a .grid of the finest (phi1,phi2) values.  Errors print "[ERROR] ..." to
+// stderr and exit with status 0, as MK_ERROR_OUT does (Misha/Exceptions.h:101-108).
+#include <bits/hash_bytes.h>
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+
+#include "epr.h"
+#include "ply.h"
+
+static void Error( const std::string &msg )
+{
+	fprintf( stderr , "[ERROR] %s\n" , msg.c_str() );
+	exit( 0 );
+}
+
+// typeid( MishaK::Hat::HierarchicalRegularIndexer< Dim > ).hash_code() of a
+// libstdc++ build of the reference: the hash of the mangled type name.
+static unsigned long long IndexerHash( int dim )
+{
+	std::string name = "N6MishaK3Hat26HierarchicalRegularIndexerILj" + std::to_string( dim ) + "EEE";
+	return (unsigned long long)std::_Hash_bytes( name.c_str() , name.size() , static_cast< size_t >( 0xc70f6907UL ) );
+}
+
+static void Usage( const char *ex )
+{
+	printf( "Usage %s:\n" , ex );
+	printf( "\t --in <input points and frames>\n" );
+	printf( "\t[--out <output voxel grid header>]\n" );
+	printf( "\t[--grid <output .grid of the finest (phi1,phi2)>]\n" );
+	printf( "\t[--depth <grid depth>=5]\n" );
+	printf( "\t[--minSolveDepth <minimum hierarchy solver depth>=0]\n" );
+	printf( "\t[--maxSolveDepth <maximum hierarchy solver depth>]\n" );
+	printf( "\t[--nSigma <noise-to-weight sigma>=0.250000]\n" );
+	printf( "\t[--dRadius <density kernel radius>=1]\n" );
+	printf( "\t[--tRadius <tensor kernel radius>=1]\n" );
+	printf( "\t[--aRadius <adaptive radius>=-1 (only the dense grid is supported)]\n" );
+	printf( "\t[--gsIters <gs iterations>=10]\n" );
+	printf( "\t[--seed <random seed>=0]\n" );
+	printf( "\t[--sWeight <screening weight>=100]\n" );
+	printf( "\t[--dWeight <dirichlet weight>=0.00625]\n" );
+	printf( "\t[--scale <bounding box scale>=1.100000]\n" );
+	printf( "\t[--iMult <iteration multiplier>=2.000000]\n" );
+	printf( "\t[--singleLevel]\n" );
+	printf( "\t[--verbose]\n" );
+}
+
+int main( int argc , char *argv[] )
+{
+	epr_params p;
+	epr_default_params( &p );
+	std::string in , out , grid;
+	long long aRadius = -1;
+	bool project = false , separate = false;
+	double jitter = 0;
+	for( int i=1 ; i<argc ; i++ )
+	{
+		std::string a = argv[i];
+		if( a.rfind( "--" , 0 )!=0 ){ fprintf( stderr , "[WARNING] Expected parameter name: %s\n" , a.c_str() ) ; continue; }
+		a = a.substr( 2 );
+		auto val = [&]() -> std::string { if( i+1>=argc ){ fprintf( stderr , "[WARNING] Missing value for --%s\n" , a.c_str() ) ; return ""; } return argv[++i]; };
+		if     ( a=="in" ) in = val();
+		else if( a=="out" ) out = val();
+		else if( a=="grid" ) grid = val();
+		else if( a=="depth" ) p.depth = atoi( val().c_str() );
+		else if( a=="seed" ) p.seed = atoi( val().c_str() );
+		else if( a=="tRadius" ) p.tRadius = atoi( val().c_str() );
+		else if( a=="dRadius" ) p.dRadius = atoi( val().c_str() );
+		else if( a=="aRadius" ) aRadius = atoll( val().c_str() );
+		else if( a=="minSolveDepth" ) p.minSolveDepth = atoi( val().c_str() );
+		else if( a=="maxSolveDepth" ) p.maxSolveDepth = atoi( val().c_str() );
+		else if( a=="gsIters" ) p.gsIters = atoi( val().c_str() );
+		else if( a=="nSigma" ) p.nSigma = atof( val().c_str() );
+		else if( a=="scale" ) p.scale = atof( val().c_str() );
+		else if( a=="sWeight" ) p.sWeight = atof( val().c_str() );
+		else if( a=="dWeight" ) p.dWeight = atof( val().c_str() );
+		else if( a=="iMult" ) p.iMult = atof( val().c_str() );
+		else if( a=="jitter" ) jitter = atof( val().c_str() );
+		else if( a=="singleLevel" ) p.singleLevel = 1;
+		else if( a=="separate" ) separate = true;
+		else if( a=="project" ) project = true;
+		else if( a=="verbose" ) p.verbose = 1;
+		else fprintf( stderr , "[WARNING] Invalid parameter: --%s\n" , a.c_str() );
+	}
+	if( in.empty() ){ Usage( argv[0] ) ; return EXIT_SUCCESS; }
+	if( aRadius!=-1 && aRadius!=(long long)0xFFFFFFFFLL ) Error( "only the dense grid (--aRadius -1) is supported" );
+	if( project ) Error( "--project is not supported" );
+	if( separate ) Error( "--separate is not supported" );
+	if( jitter>0 ) Error( "--jitter is not supported" );
+	if( p.maxSolveDepth>p.depth ) p.maxSolveDepth = p.depth;
+	if( in.size()<4 || in.substr( in.size()-4 )!=".ply" ) Error( "Only .ply files supported" );
+	std::string err;
+	epr::PlySamples s = epr::ReadPlySamples( in , err );
+	if( !s.dim ) Error( "Invalid sample file: " + err );
+	auto t0 = std::chrono::steady_clock::now();
+	epr_ctx *ctx = nullptr;
+	auto Check = [&]( int rc ){ if( rc!=EPR_OK ) Error( ctx ? epr_last_error( ctx ) : "epr_create failed" ); };
+	Check( epr_create( s.dim , &p , &ctx ) );
+	Check( epr_set_samples( ctx , s.size() , s.pos.data() , s.skew.data() ) );
+	Check( epr_build( ctx ) );
+	if( p.verbose ) printf( "%32s: %.3f (s)\n" , "Reconstructor" , epr_last_seconds( ctx , 0 ) );
+	double E[2];
+	Check( epr_solve( ctx , E ) );
+	printf( "Error: %.17g -> %.17g = %.17g + %.17g\n" , 0. , E[0]+E[1] , E[0] , E[1] );
+	if( !out.empty() ) Check( epr_write_tree( ctx , ( out + ".tree" ).c_str() , IndexerHash( s.dim ) ) );
+	if( !grid.empty() ) Check( epr_write_grid( ctx , grid.c_str() ) );
+	if( p.verbose ) printf( "%32s: %.3f (s)\n" , "Executed" , std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count() );
+	epr_destroy( ctx );
+	return EXIT_SUCCESS;
+}

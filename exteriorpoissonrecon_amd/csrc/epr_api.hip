@@ -1,0 +1,727 @@
+// C ABI and host orchestration of the HIP ExteriorPoissonRecon solve path.
+//
+// Host work is limited to what the reference does on its input before any
+// grid exists: the unit-cube map (ExteriorPoisson.h:100-123) and the ordering
+// of samples into occupied cells (OrderedSamples.inl:32-85, the same
+// std::sort so cells and their sample order are identical), plus the O(1)
+// stencil tables.  Every grid-sized computation runs on the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/epr.h"
+#include "epr_hat.h"
+#include "epr_kernels.h"
+
+namespace epr
+{
+struct HipError { std::string msg; };
+#define HIPCHECK( call ) do { hipError_t _e = ( call ) ; if( _e!=hipSuccess ) throw ::epr::HipError{ std::string( #call ) + ": " + hipGetErrorString( _e ) }; } while( 0 )
+
+template< typename T >
+struct DevBuf
+{
+	T *p = nullptr;
+	size_t n = 0;
+	DevBuf(){}
+	DevBuf( const DevBuf & ) = delete;
+	DevBuf &operator=( const DevBuf & ) = delete;
+	DevBuf( DevBuf &&o ) noexcept : p( o.p ) , n( o.n ) { o.p = nullptr , o.n = 0; }
+	DevBuf &operator=( DevBuf &&o ) noexcept { std::swap( p , o.p ) ; std::swap( n , o.n ) ; return *this; }
+	~DevBuf(){ if( p ) (void)hipFree( p ); }
+	void ensure( size_t count ){ if( count>n ) alloc( count ); }
+	void alloc( size_t count )
+	{
+		if( p ){ (void)hipFree( p ) ; p = nullptr; }
+		n = count;
+		if( count ) HIPCHECK( hipMalloc( (void**)&p , count*sizeof(T) ) );
+	}
+	void zero( hipStream_t s ){ if( n ) HIPCHECK( hipMemsetAsync( p , 0 , n*sizeof(T) , s ) ); }
+	void upload( const T *h , size_t count , hipStream_t s ){ if( count>n ) alloc( count ) ; if( count ) HIPCHECK( hipMemcpyAsync( p , h , count*sizeof(T) , hipMemcpyHostToDevice , s ) ); }
+	void download( T *h , size_t count , hipStream_t s ) const { if( count ) HIPCHECK( hipMemcpyAsync( h , p , count*sizeof(T) , hipMemcpyDeviceToHost , s ) ); }
+};
+
+static inline unsigned int Blocks( long long n , int bs ){ return (unsigned int)( ( n + bs - 1 ) / bs ); }
+
+struct CtxBase
+{
+	virtual ~CtxBase(){}
+	virtual void setSamples( size_t n , const double *pos , const double *skew ) = 0;
+	virtual void build( void ) = 0;
+	virtual void init( int level , const double *x0 , const double *y0 ) = 0;
+	virtual void gs( int level , int iters ) = 0;
+	virtual void prolongAdd( int level ) = 0;
+	virtual void energies( int level , double out[2] ) = 0;
+	virtual void solve( double out[2] ) = 0;
+	virtual size_t numFunctions( int level ) const = 0;
+	virtual void getCoeffs( int level , double *x , double *y ) = 0;
+	virtual void setCoeffs( int level , const double *x , const double *y ) = 0;
+	virtual void qfits( int level , double *q ) = 0;
+	virtual void apply( int level , char which , const double *x , double *y ) = 0;
+	virtual size_t getArray( const std::string &name , double *out , size_t cap ) = 0;
+	virtual void writeTree( const char *path , unsigned long long hash ) = 0;
+	virtual void writeGrid( const char *path ) = 0;
+	double seconds[2] = { 0 , 0 };
+};
+
+template< int Dim >
+struct Ctx : public CtxBase
+{
+	static constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD , NV = DimC< Dim >::NDNI , CoDim = 2;
+	// levels at or below this many nodes run all their iterations in one workgroup
+	static constexpr long long SmallLevel = Dim==3 ? 5000 : 700;
+
+	epr_params prm;
+	int D;
+	hipStream_t stream = nullptr;
+
+	// samples (host, unit cube) and their cell order
+	size_t n = 0;
+	std::vector< double > pos , skewPacked;
+	double unitToWorld[Dim+1][Dim+1];
+	std::vector< int > cellIdx , cellStart , cellSamples;
+	int ncell = 0;
+	bool built = false , solved = false;
+
+	struct Level
+	{
+		int res;
+		long long N;
+		long long stride[Dim];
+		DevBuf< double > x , y , R , B , K;
+	};
+	std::vector< Level > levels;
+
+	// build-phase device data
+	DevBuf< double > dPos , dSkew , dCellP , dCellData , dMeasure , dNoise , dDepth , dRW , dSkewScaled , dS , dS2 , dFS , dScale , dPartials , dTmp , dTmp2 , dDepthScale;
+	DevBuf< int > dCellIdx , dCellStart , dCellSamples , dCellMap;
+	std::vector< std::unique_ptr< DevBuf< double > > > estData , skData;
+	std::vector< std::unique_ptr< DevBuf< unsigned char > > > skKids , estKids;
+	double c_ = 0;
+
+	explicit Ctx( const epr_params &p ) : prm( p ) , D( p.depth )
+	{
+		if( D<0 || D>12 ) throw std::string( "depth must be in [0,12]" );
+		if( prm.tRadius<0 || prm.tRadius>4 ) throw std::string( "Kernel radius must be in range [0,4]: " + std::to_string( prm.tRadius ) );
+		if( prm.dRadius<0 || prm.dRadius>4 ) throw std::string( "Only radii [0,4] supported: " + std::to_string( prm.dRadius ) );
+		HIPCHECK( hipStreamCreate( &stream ) );
+		levels.resize( D+1 );
+		for( int l=0 ; l<=D ; l++ )
+		{
+			Level &L = levels[l];
+			L.res = 1<<l;
+			L.N = 1;
+			for( int d=0 ; d<Dim ; d++ ) L.N *= L.res+1;
+			long long s = 1;
+			for( int d=Dim-1 ; d>=0 ; d-- ){ L.stride[d] = s ; s *= L.res+1; }
+		}
+	}
+	~Ctx(){ if( stream ) (void)hipStreamDestroy( stream ); }
+
+	size_t numFunctions( int l ) const { return (size_t)levels[l].N; }
+	int minD( void ) const { return std::min( std::max( prm.minSolveDepth , 0 ) , D ); }
+	int maxD( void ) const { return prm.maxSolveDepth<0 ? D : std::min( prm.maxSolveDepth , D ); }
+	void checkLevel( int l ) const { if( l<0 || l>D ) throw std::string( "level out of range" ); }
+
+	LevelView< Dim > view( int l )
+	{
+		Level &L = levels[l];
+		LevelView< Dim > v;
+		v.res = L.res , v.N = L.N , v.x = L.x.p , v.y = L.y.p , v.R = L.R.p , v.B = L.B.p , v.K = L.K.p;
+		v.sW = std::pow( (double)L.res , 4.-Dim ) , v.dW = prm.dWeight;
+		for( int d=0 ; d<Dim ; d++ ) v.stride[d] = L.stride[d];
+		return v;
+	}
+
+	// ---------------- samples ----------------
+	void setSamples( size_t _n , const double *wpos , const double *wskew )
+	{
+		if( !_n ) throw std::string( "Expected some samples: 0" );
+		for( size_t i=0 ; i<_n*Dim ; i++ ) if( !std::isfinite( wpos[i] ) ) throw std::string( "non-finite sample position" );
+		n = _n;
+		skewPacked.resize( n*CD );
+		for( size_t s=0 ; s<n ; s++ ) for( int i=0 , idx=0 ; i<Dim ; i++ ) for( int j=0 ; j<i ; j++ , idx++ )
+			skewPacked[s*CD+idx] = ( wskew[ s*Dim*Dim + i*Dim + j ] - wskew[ s*Dim*Dim + j*Dim + i ] ) / 2.;
+		// ToUnitCube (ExteriorPoisson.h:100-123) with the matrix products of Misha/Geometry.inl:97-107, 410-427
+		double bb0[Dim] , bb1[Dim];
+		for( int d=0 ; d<Dim ; d++ ) bb0[d] = bb1[d] = wpos[d];
+		for( size_t i=1 ; i<n ; i++ ) for( int d=0 ; d<Dim ; d++ ){ bb0[d] = std::min( bb0[d] , wpos[i*Dim+d] ) ; bb1[d] = std::max( bb1[d] , wpos[i*Dim+d] ); }
+		double sc = bb1[0]-bb0[0];
+		for( int d=1 ; d<Dim ; d++ ) sc = std::max( sc , bb1[d]-bb0[d] );
+		double t1[Dim+1][Dim+1] , s[Dim+1][Dim+1] , t2[Dim+1][Dim+1] , ts[Dim+1][Dim+1] , M[Dim+1][Dim+1];
+		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) t1[i][j] = s[i][j] = t2[i][j] = ( i==j ? 1. : 0. );
+		for( int d=0 ; d<Dim ; d++ ) t2[Dim][d] = -( bb0[d] + bb1[d] ) / 2.;
+		for( int d=0 ; d<Dim ; d++ ) s[d][d] = 1. / ( sc * prm.scale );
+		for( int d=0 ; d<Dim ; d++ ) t1[Dim][d] = 0.5;
+		auto Mul = []( const double A[Dim+1][Dim+1] , const double B[Dim+1][Dim+1] , double O[Dim+1][Dim+1] )
+		{
+			for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ){ O[i][j] = 0 ; for( int k=0 ; k<=Dim ; k++ ) O[i][j] += B[i][k]*A[k][j]; }
+		};
+		Mul( t1 , s , ts );
+		Mul( ts , t2 , M );
+		pos.resize( n*Dim );
+		for( size_t i=0 ; i<n ; i++ )
+		{
+			double scl = 1;
+			for( int a=0 ; a<Dim ; a++ )
+			{
+				double o = 0;
+				for( int b=0 ; b<Dim ; b++ ) o += wpos[i*Dim+b]*M[b][a];
+				o += M[Dim][a];
+				scl += M[a][Dim];
+				pos[i*Dim+a] = o;
+			}
+			for( int a=0 ; a<Dim ; a++ ) pos[i*Dim+a] /= scl;
+		}
+		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) unitToWorld[i][j] = ( i==j ? 1. : 0. );
+		for( int d=0 ; d<Dim ; d++ ){ unitToWorld[d][d] = 1./M[d][d] ; unitToWorld[Dim][d] = -M[Dim][d]/M[d][d]; }
+		// OrderedSampler: clamped floor binning, lexicographic std::sort, grouping
+		int res = 1<<D;
+		struct IS { size_t idx ; int I[Dim] ; bool operator<( const IS &o ) const { for( int d=0 ; d<Dim ; d++ ){ if( I[d]<o.I[d] ) return true ; if( I[d]>o.I[d] ) return false; } return false; } };
+		std::vector< IS > v( n );
+		for( size_t i=0 ; i<n ; i++ )
+		{
+			v[i].idx = i;
+			for( int d=0 ; d<Dim ; d++ ) v[i].I[d] = std::max< int >( 0 , std::min< int >( (int)floor( pos[i*Dim+d] * res ) , res-1 ) );
+		}
+		std::sort( v.begin() , v.end() );
+		cellIdx.clear() , cellStart.clear() , cellSamples.clear();
+		size_t start = 0;
+		while( start<v.size() )
+		{
+			size_t end = start;
+			auto same = [&]( size_t a , size_t b ){ for( int d=0 ; d<Dim ; d++ ) if( v[a].I[d]!=v[b].I[d] ) return false ; return true; };
+			while( end<v.size() && same( end , start ) ) end++;
+			cellStart.push_back( (int)cellSamples.size() );
+			for( int d=0 ; d<Dim ; d++ ) cellIdx.push_back( v[start].I[d] );
+			for( size_t i=start ; i<end ; i++ ) cellSamples.push_back( (int)v[i].idx );
+			start = end;
+		}
+		cellStart.push_back( (int)cellSamples.size() );
+		ncell = (int)cellStart.size()-1;
+		built = solved = false;
+	}
+
+	std::vector< long long > treeSizes( int nv , bool kids )
+	{
+		std::vector< long long > s( D+2 , 0 );
+		for( int t=1 ; t<=D+1 ; t++ )
+		{
+			long long c = 1;
+			if( kids ){ if( t>D ) continue ; for( int d=0 ; d<Dim ; d++ ) c *= 1LL<<t; }
+			else for( int d=0 ; d<Dim ; d++ ) c *= ( 1LL<<(t-1) ) + 1;
+			s[t] = c*nv;
+		}
+		return s;
+	}
+
+	template< int KR > void launchEstSplat( TreeGrids< Dim > T )
+	{
+		int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= 2*KR+2;
+		long long tot = (long long)ncell*(D+1)*taps;
+		KEstSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p );
+	}
+	template< int KR > void launchSkewSplat( TreeGrids< Dim > T )
+	{
+		int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= 2*KR+2;
+		long long tot = (long long)n*taps;
+		KSkewSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dPos.p , dSkewScaled.p , dDepth.p , (int)n , T );
+	}
+
+	// ---------------- build ----------------
+	void build( void )
+	{
+		if( !n ) throw std::string( "no samples" );
+		auto t0 = std::chrono::steady_clock::now();
+		const int res = 1<<D;
+		dPos.upload( pos.data() , pos.size() , stream );
+		dSkew.upload( skewPacked.data() , skewPacked.size() , stream );
+		dCellIdx.upload( cellIdx.data() , cellIdx.size() , stream );
+		dCellStart.upload( cellStart.data() , cellStart.size() , stream );
+		dCellSamples.upload( cellSamples.data() , cellSamples.size() , stream );
+
+		// ---- tree estimator (GridSamples.inl:740-771)
+		{
+			const int K = prm.dRadius;
+			double scale;
+			{
+				std::vector< double > v( 2*K+2 );
+				// runtime B-spline (GridSamples.inl:50-65)
+				std::function< void( int , double , double* ) > Ev = [&]( int deg , double x , double *val )
+				{
+					if( deg==0 ){ val[0] = 1. ; return; }
+					const double Scale = 1./deg;
+					Ev( deg-1 , x , val+1 );
+					val[0] = val[1] * (1.-x) * Scale;
+					for( int i=1 ; i<deg ; i++ ){ double x1 = ( x-i+deg ) , x2 = ( -x+i+1 ) ; val[i] = ( val[i]*x1 + val[i+1]*x2 ) * Scale; }
+					val[deg] *= x * Scale;
+				};
+				Ev( 2*K+1 , 0.5 , &v[0] );
+				scale = std::pow( 1./v[K+1] , CoDim );
+			}
+			std::vector< double > scales( D+1 );
+			for( int d=0 ; d<=D ; d++ ) scales[d] = scale / std::pow( std::pow( 0.5 , d ) , Dim-CoDim );
+			dDepthScale.upload( scales.data() , scales.size() , stream );
+			dCellP.alloc( (size_t)ncell*Dim ) , dCellData.alloc( (size_t)ncell*NV );
+			KCellStats< Dim ><<< Blocks( ncell , 128 ) , 128 , 0 , stream >>>( dPos.p , dCellStart.p , dCellSamples.p , ncell , dCellP.p , dCellData.p );
+			std::vector< long long > sz = treeSizes( NV , false );
+			estData.clear();
+			TreeGrids< Dim > T;
+			memset( &T , 0 , sizeof(T) );
+			T.maxDepth = D;
+			estData.resize( D+2 );
+			for( int t=1 ; t<=D+1 ; t++ ){ estData[t].reset( new DevBuf< double >() ) ; estData[t]->alloc( sz[t] ) ; estData[t]->zero( stream ) ; T.data[t] = estData[t]->p; }
+			switch( K ){ case 0: launchEstSplat<0>( T ) ; break ; case 1: launchEstSplat<1>( T ) ; break ; case 2: launchEstSplat<2>( T ) ; break ; case 3: launchEstSplat<3>( T ) ; break ; default: launchEstSplat<4>( T ); }
+			HIPCHECK( hipGetLastError() );
+			dMeasure.alloc( n ) , dNoise.alloc( n ) , dDepth.alloc( n ) , dRW.alloc( n ) , dSkewScaled.alloc( n*CD );
+			KEstQuery< Dim ><<< Blocks( (long long)n , 128 ) , 128 , 0 , stream >>>( dPos.p , dSkew.p , (int)n , T , prm.nSigma , dMeasure.p , dNoise.p , dDepth.p , dRW.p , dSkewScaled.p );
+			HIPCHECK( hipGetLastError() );
+		}
+		// ---- skew splat and the target field (ExteriorPoisson.h:175-264, 290-309)
+		{
+			const int K = prm.tRadius;
+			std::vector< long long > sz = treeSizes( CD , false ) , kz = treeSizes( 1 , true );
+			skData.clear() , skKids.clear();
+			skData.resize( D+2 ) , skKids.resize( D+2 );
+			TreeGrids< Dim > T;
+			memset( &T , 0 , sizeof(T) );
+			T.maxDepth = D;
+			for( int t=1 ; t<=D+1 ; t++ )
+			{
+				skData[t].reset( new DevBuf< double >() ) ; skData[t]->alloc( sz[t] ) ; skData[t]->zero( stream ) ; T.data[t] = skData[t]->p;
+				if( t<=D ){ skKids[t].reset( new DevBuf< unsigned char >() ) ; skKids[t]->alloc( kz[t] ) ; skKids[t]->zero( stream ) ; T.kids[t] = skKids[t]->p; }
+			}
+			if( D>=1 ) KTouch< Dim ><<< Blocks( (long long)ncell*D , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , T , K );
+			switch( K ){ case 0: launchSkewSplat<0>( T ) ; break ; case 1: launchSkewSplat<1>( T ) ; break ; case 2: launchSkewSplat<2>( T ) ; break ; case 3: launchSkewSplat<3>( T ) ; break ; default: launchSkewSplat<4>( T ); }
+			const double scl = std::pow( std::pow( 0.5 , D ) , 1.5 * CoDim );
+			for( int t=1 ; t<=D+1 ; t++ ) KScaleArray<<< Blocks( sz[t] , 256 ) , 256 , 0 , stream >>>( T.data[t] , sz[t] , scl );
+			long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= res;
+			dS.alloc( (size_t)ne*CD ) , dS2.alloc( (size_t)ne );
+			KField< Dim ><<< Blocks( ne , 256 ) , 256 , 0 , stream >>>( T , dS.p , dS2.p );
+			HIPCHECK( hipGetLastError() );
+			// c = sum |S(E)|^2 / #elements (SystemEnergy.h:455-461)
+			const int nb = 1024;
+			dPartials.ensure( (size_t)nb*8 );
+			dTmp.ensure( 8 );
+			KPartialSum<<< nb , 256 , 0 , stream >>>( dS2.p , ne , dPartials.p );
+			KSumRows<1><<< 1 , 256 , 0 , stream >>>( dPartials.p , nb , dTmp.p );
+			double csum;
+			dTmp.download( &csum , 1 , stream );
+			HIPCHECK( hipStreamSynchronize( stream ) );
+			c_ = csum / (double)ne;
+		}
+		// ---- finest R and B, then the Galerkin chain
+		{
+			Level &F = levels[D];
+			F.R.alloc( (size_t)F.N*W ) , F.B.alloc( (size_t)F.N*W );
+			// B
+			std::vector< double > fs = hat::FullStiffnessTable< Dim >( res );
+			dFS.upload( fs.data() , fs.size() , stream );
+			KAssembleB< Dim ><<< Blocks( F.N*W , 256 ) , 256 , 0 , stream >>>( dS.p , dFS.p , res , F.B.p );
+			// R
+			if( prm.sWeight>0 )
+			{
+				long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= res;
+				std::vector< int > cmap( ne , -1 );
+				for( int c=0 ; c<ncell ; c++ )
+				{
+					long long e = 0;
+					for( int d=Dim-1 ; d>=0 ; d-- ) e = e*res + cellIdx[ (size_t)c*Dim+d ];
+					cmap[e] = c;
+				}
+				dCellMap.upload( cmap.data() , cmap.size() , stream );
+				const int nb = 1024;
+				dScale.alloc( 2 );
+				KPartialSum<<< nb , 256 , 0 , stream >>>( dMeasure.p , (long long)n , dPartials.p );
+				KMeanMeasure<<< 1 , 1 , 0 , stream >>>( dPartials.p , nb , (long long)n , prm.sWeight , dScale.p );
+				KAssembleR< Dim ><<< Blocks( F.N*W , 256 ) , 256 , 0 , stream >>>( dPos.p , dCellMap.p , dCellStart.p , dCellSamples.p , dRW.p , res , dScale.p , F.R.p );
+			}
+			else F.R.zero( stream );
+			HIPCHECK( hipGetLastError() );
+			for( int l=D-1 ; l>=0 ; l-- )
+			{
+				Level &Cl = levels[l] , &Fl = levels[l+1];
+				Cl.R.alloc( (size_t)Cl.N*W ) , Cl.B.alloc( (size_t)Cl.N*W );
+				KGalerkin< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( Fl.R.p , Cl.R.p , Cl.res );
+				KGalerkin< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( Fl.B.p , Cl.B.p , Cl.res );
+			}
+			HIPCHECK( hipGetLastError() );
+		}
+		// ---- stencil case rows and coefficient storage
+		for( int l=0 ; l<=D ; l++ )
+		{
+			Level &L = levels[l];
+			std::vector< double > k = hat::StiffnessCaseRows< Dim >( L.res );
+			L.K.upload( k.data() , k.size() , stream );
+			L.x.alloc( L.N ) , L.y.alloc( L.N );
+			L.x.zero( stream ) , L.y.zero( stream );
+		}
+		HIPCHECK( hipStreamSynchronize( stream ) );
+		built = true , solved = false;
+		seconds[0] = std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count();
+	}
+
+	void needBuilt( void ) const { if( !built ) throw std::string( "epr_build has not run" ); }
+
+	// ---------------- solve pieces ----------------
+	void gs( int l , int iters )
+	{
+		needBuilt() ; checkLevel( l );
+		if( iters<=0 ) return;
+		LevelView< Dim > v = view( l );
+		if( levels[l].N<=SmallLevel ) KGsSmall< Dim ><<< 1 , 512 , 0 , stream >>>( v , iters );
+		else for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+		{
+			ColourSet< Dim > cs ; cs.set( levels[l].res , c );
+			KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs );
+		}
+		HIPCHECK( hipGetLastError() );
+	}
+
+	void prolong( int l , const double *in , double *out , int mode )
+	{
+		long long nf = levels[l+1].N;
+		KProlong< Dim ><<< Blocks( nf , 256 ) , 256 , 0 , stream >>>( in , out , levels[l].res , mode );
+	}
+
+	void prolongAdd( int l )
+	{
+		needBuilt();
+		if( l<0 || l>=D ) throw std::string( "level out of range" );
+		prolong( l , levels[l].x.p , levels[l+1].x.p , 1 );
+		prolong( l , levels[l].y.p , levels[l+1].y.p , 1 );
+		HIPCHECK( hipGetLastError() );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+	}
+
+	// (W, yBx, R-form, K-form) at a level
+	void energyTerms( int l , double t[4] )
+	{
+		LevelView< Dim > v = view( l );
+		long long threads = levels[l].N*C;
+		const unsigned int nb = Blocks( threads , 256 );
+		dPartials.ensure( (size_t)nb*4 );
+		dTmp.ensure( 8 );
+		KEnergy< Dim ><<< nb , 256 , 0 , stream >>>( v , dPartials.p );
+		KSumRows<4><<< 1 , 256 , 0 , stream >>>( dPartials.p , nb , dTmp.p );
+		dTmp.download( t , 4 , stream );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+	}
+
+	void energies( int l , double out[2] )
+	{
+		needBuilt() ; checkLevel( l );
+		double t[4];
+		energyTerms( l , t );
+		out[0] = t[0] - 2.*t[1] + c_;
+		out[1] = t[2];
+		if( prm.dWeight ) out[1] += t[3]*prm.dWeight;
+	}
+
+	static double Random( void )   // Misha/Geometry.inl:429-438
+	{
+		static const unsigned long long ULL_RAND_MAX = (unsigned long long)( RAND_MAX ) + 1;
+		static const double RAND_MAX_SQUARED = double( ULL_RAND_MAX * ULL_RAND_MAX - 1 );
+		unsigned long long r1 = rand() , r2 = rand();
+		long long foo = r1 * ULL_RAND_MAX + r2;
+		return double( foo ) / RAND_MAX_SQUARED;
+	}
+
+	void init( int l , const double *x0 , const double *y0 )
+	{
+		needBuilt() ; checkLevel( l );
+		Level &L = levels[l];
+		if( x0 && y0 ){ L.x.upload( x0 , L.N , stream ) ; L.y.upload( y0 , L.N , stream ) ; HIPCHECK( hipStreamSynchronize( stream ) ) ; return; }
+		// ExteriorPoisson.h:333-341
+		std::vector< double > hx( L.N ) , hy( L.N );
+		srand( prm.seed );
+		for( long long i=0 ; i<L.N ; i++ ) hx[i] = Random() , hy[i] = Random();
+		L.x.upload( hx.data() , L.N , stream ) , L.y.upload( hy.data() , L.N , stream );
+		double t[4];
+		energyTerms( l , t );
+		double s = std::sqrt( std::sqrt( c_ / t[0] ) );
+		dTmp2.ensure( 1 );
+		dTmp2.upload( &s , 1 , stream );
+		KScale2<<< Blocks( L.N , 256 ) , 256 , 0 , stream >>>( L.x.p , L.y.p , L.N , dTmp2.p );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+	}
+
+	// the cascadic schedule (ExteriorPoisson.h:316-409)
+	void solve( double out[2] )
+	{
+		needBuilt();
+		auto t0 = std::chrono::steady_clock::now();
+		const int lo = minD() , hi = maxD();
+		for( int l=0 ; l<=D ; l++ ) levels[l].x.zero( stream ) , levels[l].y.zero( stream );
+		init( lo , nullptr , nullptr );
+		auto Iters = [&]( int d ){ return (int)std::ceil( std::pow( prm.iMult , D-d ) * prm.gsIters ); };
+		if( prm.singleLevel )
+		{
+			for( int d=lo ; d<hi ; d++ ){ prolong( d , levels[d].x.p , levels[d+1].x.p , 0 ) ; prolong( d , levels[d].y.p , levels[d+1].y.p , 0 ); }
+			for( int d=lo ; d<hi ; d++ ) levels[d].x.zero( stream ) , levels[d].y.zero( stream );
+			gs( hi , prm.gsIters );
+		}
+		else for( int d=lo ; d<=hi ; d++ )
+		{
+			gs( d , Iters( d ) );
+			if( prm.verbose )
+			{
+				double e[2];
+				energies( d , e );
+				printf( "Error[%2d]: %g = %g + %g\n" , 1<<d , e[0]+e[1] , e[0] , e[1] );
+			}
+			if( d<hi )
+			{
+				prolong( d , levels[d].x.p , levels[d+1].x.p , 1 );
+				prolong( d , levels[d].y.p , levels[d+1].y.p , 1 );
+			}
+		}
+		double e[2];
+		energies( hi , e );
+		if( out ) out[0] = e[0] , out[1] = e[1];
+		if( prm.verbose ) printf( "Error: %g -> %g = %g + %g => %g\n" , c_ , e[0]+e[1] , e[0] , e[1] , std::sqrt( ( e[0]+e[1] )/c_ ) );
+		for( int d=hi ; d<D ; d++ ){ prolong( d , levels[d].x.p , levels[d+1].x.p , 0 ) ; prolong( d , levels[d].y.p , levels[d+1].y.p , 0 ); }
+		for( int d=D ; d>0 ; d-- ){ prolong( d-1 , levels[d-1].x.p , levels[d].x.p , 2 ) ; prolong( d-1 , levels[d-1].y.p , levels[d].y.p , 2 ); }
+		HIPCHECK( hipGetLastError() );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+		solved = true;
+		seconds[1] = std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count();
+	}
+
+	void getCoeffs( int l , double *x , double *y )
+	{
+		needBuilt() ; checkLevel( l );
+		if( x ) levels[l].x.download( x , levels[l].N , stream );
+		if( y ) levels[l].y.download( y , levels[l].N , stream );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+	}
+	void setCoeffs( int l , const double *x , const double *y )
+	{
+		needBuilt() ; checkLevel( l );
+		if( x ) levels[l].x.upload( x , levels[l].N , stream );
+		if( y ) levels[l].y.upload( y , levels[l].N , stream );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+	}
+	void qfits( int l , double *q )
+	{
+		needBuilt() ; checkLevel( l );
+		DevBuf< double > dq;
+		dq.alloc( (size_t)levels[l].N*5 );
+		KQFit< Dim ><<< Blocks( levels[l].N*C , 256 ) , 256 , 0 , stream >>>( view( l ) , dq.p );
+		HIPCHECK( hipGetLastError() );
+		dq.download( q , (size_t)levels[l].N*5 , stream );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+	}
+	void apply( int l , char which , const double *x , double *y )
+	{
+		needBuilt() ; checkLevel( l );
+		if( which=='P' )
+		{
+			if( l>=D ) throw std::string( "no finer level" );
+			DevBuf< double > a , b;
+			a.upload( x , levels[l].N , stream );
+			b.alloc( levels[l+1].N );
+			prolong( l , a.p , b.p , 0 );
+			b.download( y , levels[l+1].N , stream );
+			HIPCHECK( hipStreamSynchronize( stream ) );
+			return;
+		}
+		if( which!='R' && which!='B' ) throw std::string( "operator must be R, B or P" );
+		// y = M x, from the columns (host gather; test hook only)
+		const Level &L = levels[l];
+		std::vector< double > M( (size_t)L.N*W );
+		( which=='R' ? L.R : L.B ).download( M.data() , M.size() , stream );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+		for( long long i=0 ; i<L.N ; i++ ) y[i] = 0;
+		for( long long f=0 ; f<L.N ; f++ )
+		{
+			int F[Dim];
+			long long t = f;
+			for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = (int)( t%( L.res+1 ) ) ; t /= L.res+1; }
+			for( int w=0 ; w<W ; w++ )
+			{
+				double v = M[ (size_t)f*W+w ];
+				if( v==0 ) continue;
+				long long r = f;
+				int tw = w;
+				for( int d=Dim-1 ; d>=0 ; d-- ){ int o = tw%3-1 ; tw /= 3 ; r += o*L.stride[d]; }
+				y[r] += v*x[f];
+			}
+		}
+	}
+	size_t getArray( const std::string &name , double *out , size_t cap )
+	{
+		std::vector< double > v;
+		auto dl = [&]( const DevBuf< double > &b ){ v.resize( b.n ) ; b.download( v.data() , b.n , stream ) ; HIPCHECK( hipStreamSynchronize( stream ) ); };
+		if( name=="unit_pos" ) v = pos;
+		else if( name=="skew_packed" ) v = skewPacked;
+		else if( name=="cells" ) v.assign( cellIdx.begin() , cellIdx.end() );
+		else if( name=="cell_counts" ){ for( int c=0 ; c<ncell ; c++ ) v.push_back( cellStart[c+1]-cellStart[c] ); }
+		else if( name=="unit_to_world" ){ for( int c=0 ; c<=Dim ; c++ ) for( int r=0 ; r<=Dim ; r++ ) v.push_back( unitToWorld[c][r] ); }
+		else if( name=="measure" ){ needBuilt() ; dl( dMeasure ); }
+		else if( name=="noise" ){ needBuilt() ; dl( dNoise ); }
+		else if( name=="sample_depth" ){ needBuilt() ; dl( dDepth ); }
+		else if( name=="S_elem" ){ needBuilt() ; dl( dS ); }
+		else if( name=="c" ){ needBuilt() ; v.push_back( c_ ); }
+		else throw std::string( "unknown array: " + name );
+		if( out ) std::copy( v.begin() , v.begin()+std::min( cap , v.size() ) , out );
+		return v.size();
+	}
+
+	void voxelToWorld( double v2w[Dim+1][Dim+1] ) const
+	{
+		double g2c[Dim+1][Dim+1];
+		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) g2c[i][j] = ( i==j ? 1. : 0. );
+		for( int d=0 ; d<Dim ; d++ ) g2c[d][d] = 1./(1<<D);
+		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ){ v2w[i][j] = 0 ; for( int k=0 ; k<=Dim ; k++ ) v2w[i][j] += g2c[i][k]*unitToWorld[k][j]; }
+	}
+
+	// .tree (ExteriorPoissonRecon.cpp:259-270; Misha/Streams.h, Misha/EigenStreams.h)
+	void writeTree( const char *path , unsigned long long hash )
+	{
+		if( !solved ) throw std::string( "epr_solve has not run" );
+		FILE *fp = fopen( path , "wb" );
+		if( !fp ) throw std::string( "cannot open " ) + path;
+		unsigned int dim = Dim;
+		fwrite( &dim , sizeof(dim) , 1 , fp );
+		fwrite( &hash , sizeof(hash) , 1 , fp );
+		double v2w[Dim+1][Dim+1];
+		voxelToWorld( v2w );
+		fwrite( v2w , sizeof(double) , (Dim+1)*(Dim+1) , fp );
+		for( int pass=0 ; pass<2 ; pass++ )
+		{
+			unsigned long long nl = D+1;
+			fwrite( &nl , sizeof(nl) , 1 , fp );
+			for( int d=0 ; d<=D ; d++ )
+			{
+				std::vector< double > h( levels[d].N );
+				( pass==0 ? levels[d].x : levels[d].y ).download( h.data() , h.size() , stream );
+				HIPCHECK( hipStreamSynchronize( stream ) );
+				long long sz = levels[d].N;
+				fwrite( &sz , sizeof(sz) , 1 , fp );
+				fwrite( h.data() , sizeof(double) , h.size() , fp );
+			}
+		}
+		unsigned int maxDepth = D;
+		fwrite( &maxDepth , sizeof(maxDepth) , 1 , fp );
+		fclose( fp );
+	}
+
+	// .grid with 2 DOUBLE values per node (Misha/RegularGrid.inl:440-471), finest regular coefficients
+	void writeGrid( const char *path )
+	{
+		if( !solved ) throw std::string( "epr_solve has not run" );
+		// regular coefficients: sum of prolonged hierarchical levels
+		DevBuf< double > ax , ay , bx , by;
+		ax.alloc( levels[D].N ) , ay.alloc( levels[D].N ) , bx.alloc( levels[D].N ) , by.alloc( levels[D].N );
+		HIPCHECK( hipMemcpyAsync( ax.p , levels[0].x.p , levels[0].N*sizeof(double) , hipMemcpyDeviceToDevice , stream ) );
+		HIPCHECK( hipMemcpyAsync( ay.p , levels[0].y.p , levels[0].N*sizeof(double) , hipMemcpyDeviceToDevice , stream ) );
+		for( int d=1 ; d<=D ; d++ )
+		{
+			prolong( d-1 , ax.p , bx.p , 0 ) , prolong( d-1 , ay.p , by.p , 0 );
+			const long long N = levels[d].N;
+			KAddInto<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( levels[d].x.p , bx.p , ax.p , N );
+			KAddInto<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( levels[d].y.p , by.p , ay.p , N );
+		}
+		HIPCHECK( hipGetLastError() );
+		long long N = levels[D].N;
+		std::vector< double > hx( N ) , hy( N );
+		ax.download( hx.data() , N , stream ) , ay.download( hy.data() , N , stream );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+		FILE *fp = fopen( path , "wb" );
+		if( !fp ) throw std::string( "cannot open " ) + path;
+		fprintf( fp , "G%d\n" , Dim );
+		fprintf( fp , "2 DOUBLE\n" );
+		for( int d=0 ; d<Dim ; d++ ) fprintf( fp , d ? " %d" : "%d" , ( 1<<D )+1 );
+		fprintf( fp , "\n" );
+		double v2w[Dim+1][Dim+1];
+		voxelToWorld( v2w );
+		for( int c=0 ; c<=Dim ; c++ ){ for( int r=0 ; r<=Dim ; r++ ) fprintf( fp , r ? " %f" : "%f" , v2w[c][r] ) ; fprintf( fp , "\n" ); }
+		for( long long i=0 ; i<N ; i++ ){ fwrite( &hx[i] , sizeof(double) , 1 , fp ) ; fwrite( &hy[i] , sizeof(double) , 1 , fp ); }
+		fclose( fp );
+	}
+};
+} // namespace epr
+
+struct epr_ctx
+{
+	int dim;
+	epr_params prm;
+	std::unique_ptr< epr::CtxBase > impl;
+	std::string err;
+};
+
+template< typename F >
+static int Guard( epr_ctx *ctx , F f )
+{
+	if( !ctx ) return EPR_ERR_ARG;
+	try { f() ; ctx->err.clear() ; return EPR_OK; }
+	catch( const epr::HipError &e ){ ctx->err = e.msg ; return EPR_ERR_HIP; }
+	catch( const std::string &s ){ ctx->err = s ; return EPR_ERR_STATE; }
+	catch( const std::exception &e ){ ctx->err = e.what() ; return EPR_ERR_STATE; }
+}
+
+extern "C" {
+
+void epr_default_params( epr_params *p )
+{
+	p->depth = 5 , p->seed = 0 , p->tRadius = 1 , p->dRadius = 1 , p->minSolveDepth = 0 , p->maxSolveDepth = -1 , p->gsIters = 10;
+	p->singleLevel = 0 , p->verbose = 0;
+	p->nSigma = 0.25 , p->scale = 1.1 , p->sWeight = 100. , p->dWeight = 0.00625 , p->iMult = 2.;
+}
+
+int epr_create( int dim , const epr_params *p , epr_ctx **out )
+{
+	if( !out || !p ) return EPR_ERR_ARG;
+	*out = nullptr;
+	if( dim!=3 && dim!=4 ) return EPR_ERR_ARG;
+	epr_ctx *ctx = new epr_ctx;
+	ctx->dim = dim , ctx->prm = *p;
+	int rc = Guard( ctx , [&]()
+	{
+		if( dim==3 ) ctx->impl.reset( new epr::Ctx< 3 >( *p ) );
+		else         ctx->impl.reset( new epr::Ctx< 4 >( *p ) );
+	} );
+	*out = ctx;
+	return rc;
+}
+
+void epr_destroy( epr_ctx *ctx ){ delete ctx; }
+const char *epr_last_error( const epr_ctx *ctx ){ return ctx ? ctx->err.c_str() : "null context"; }
+
+#define IMPL( body ) return Guard( ctx , [&](){ if( !ctx->impl ) throw std::string( "context not initialised" ) ; body; } )
+
+int epr_set_samples( epr_ctx *ctx , size_t n , const double *pos , const double *skew )
+{
+	if( ctx && ( !pos || !skew ) ){ ctx->err = "null sample buffer" ; return EPR_ERR_ARG; }
+	IMPL( ctx->impl->setSamples( n , pos , skew ) );
+}
+int epr_build( epr_ctx *ctx ){ IMPL( ctx->impl->build() ); }
+int epr_init( epr_ctx *ctx , int level , const double *x0 , const double *y0 ){ IMPL( ctx->impl->init( level , x0 , y0 ) ); }
+int epr_gs_sweeps( epr_ctx *ctx , int level , int iters ){ IMPL( ctx->impl->gs( level , iters ) ; HIPCHECK( hipDeviceSynchronize() ) ); }
+int epr_prolong_add( epr_ctx *ctx , int level ){ IMPL( ctx->impl->prolongAdd( level ) ); }
+int epr_energies( epr_ctx *ctx , int level , double out[2] ){ IMPL( ctx->impl->energies( level , out ) ); }
+int epr_solve( epr_ctx *ctx , double out_E[2] ){ IMPL( ctx->impl->solve( out_E ) ); }
+size_t epr_num_functions( const epr_ctx *ctx , int level ){ return ( ctx && ctx->impl && level>=0 && level<=ctx->prm.depth ) ? ctx->impl->numFunctions( level ) : 0; }
+int epr_get_coeffs( epr_ctx *ctx , int level , double *x , double *y ){ IMPL( ctx->impl->getCoeffs( level , x , y ) ); }
+int epr_set_coeffs( epr_ctx *ctx , int level , const double *x , const double *y ){ IMPL( ctx->impl->setCoeffs( level , x , y ) ); }
+int epr_quadratic_fits( epr_ctx *ctx , int level , double *q ){ IMPL( ctx->impl->qfits( level , q ) ); }
+int epr_apply( epr_ctx *ctx , int level , char which , const double *x , double *y ){ IMPL( ctx->impl->apply( level , which , x , y ) ); }
+int epr_get_array( epr_ctx *ctx , const char *name , double *out , size_t cap , size_t *count )
+{
+	IMPL( size_t c = ctx->impl->getArray( name ? name : "" , out , cap ) ; if( count ) *count = c );
+}
+int epr_write_tree( epr_ctx *ctx , const char *path , unsigned long long hash ){ IMPL( ctx->impl->writeTree( path , hash ) ); }
+int epr_write_grid( epr_ctx *ctx , const char *path ){ IMPL( ctx->impl->writeGrid( path ) ); }
+double epr_last_seconds( const epr_ctx *ctx , int which ){ return ( ctx && ctx->impl && ( which==0 || which==1 ) ) ? ctx->impl->seconds[which] : -1.; }
+
+}

@@ -1,0 +1,962 @@
+// HIP kernels of the dense ExteriorPoissonRecon solve path (gfx950).
+//
+// Layout in HBM (one reconstruction, level l has resolution r = 2^l):
+//   * coefficients x,y: N = (r+1)^Dim doubles each, function index with
+//     coordinate 0 most significant (reference Hat.scalars.inl:475-481);
+//   * R and B: N x 3^Dim doubles, column f holds M(f+o, f) for the window
+//     offset o in {-1,0,1}^Dim (window index base 3, first coordinate most
+//     significant, Misha/Window.h:79-83);
+//   * K: 3^Dim boundary cases x 3^Dim dense rows of the stiffness stencil;
+//   * per-element target field S: r^Dim x C(Dim,2) doubles, element index with
+//     coordinate 0 fastest (Hat.scalars.inl:455-473).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <utility>
+#include "epr_wedge_tables.h"
+
+namespace epr
+{
+template< int Dim > struct DimC;
+template<> struct DimC< 3 > { static constexpr int W = 27 , C = 8 , CD = 3 , NDNI = 13; };
+template<> struct DimC< 4 > { static constexpr int W = 81 , C = 16 , CD = 6 , NDNI = 21; };
+
+constexpr int MAXT = 16;     // tree depths supported (depth <= 14)
+
+template< int Dim > __device__ __host__ __forceinline__ int Bit( int c , int d ){ return ( c>>( Dim-1-d ) ) & 1; }
+
+template< int Dim > __device__ __forceinline__ double WT( int j , int p ){ if constexpr( Dim==3 ) return wedge3::WT[j][p]; else return wedge4::WT[j][p]; }
+template< int Dim > __device__ __forceinline__ double WA( int j , int k ){ if constexpr( Dim==3 ) return wedge3::WA[j][k]; else return wedge4::WA[j][k]; }
+template< int Dim > __device__ __forceinline__ int PK( int p ){ if constexpr( Dim==3 ) return wedge3::PK[p]; else return wedge4::PK[p]; }
+template< int Dim > __device__ __forceinline__ int PL( int p ){ if constexpr( Dim==3 ) return wedge3::PL[p]; else return wedge4::PL[p]; }
+
+// One level as the node kernels see it.
+template< int Dim >
+struct LevelView
+{
+	int res;
+	long long N;
+	double *x , *y;
+	const double *R , *B , *K;
+	double sW;      // wedge scale res^(4-Dim)
+	double dW;      // Dirichlet weight
+	long long stride[Dim];
+};
+
+enum { MODE_GS = 0 , MODE_QFIT = 1 , MODE_ENERGY = 2 };
+
+// Compile-time access to the corner-0 wedge tables of a dimension.
+template< int Dim > struct WTab;
+template<> struct WTab< 3 >
+{
+	static constexpr int C = wedge3::C , P = wedge3::P;
+	static constexpr double T( int j , int p ){ return wedge3::WT[j][p]; }
+	static constexpr double A( int j , int k ){ return wedge3::WA[j][k]; }
+	static constexpr int K( int p ){ return wedge3::PK[p]; }
+	static constexpr int L( int p ){ return wedge3::PL[p]; }
+};
+template<> struct WTab< 4 >
+{
+	static constexpr int C = wedge4::C , P = wedge4::P;
+	static constexpr double T( int j , int p ){ return wedge4::WT[j][p]; }
+	static constexpr double A( int j , int k ){ return wedge4::WA[j][k]; }
+	static constexpr int K( int p ){ return wedge4::PK[p]; }
+	static constexpr int L( int p ){ return wedge4::PL[p]; }
+};
+
+// z[j] += T(j,p) * w for the non-zero entries of column p (expanded at compile time)
+template< int Dim , int p , int ... js >
+__device__ __forceinline__ void WedgeColumn( double *z , double w , std::integer_sequence< int , js... > )
+{
+	( [&]{ if constexpr( WTab< Dim >::T( js , p )!=0.0 ) z[js] = fma( WTab< Dim >::T( js , p ) , w , z[js] ); }() , ... );
+}
+template< int Dim , int ... ps >
+__device__ __forceinline__ void WedgeContract( const double *xp , const double *yp , double *z , std::integer_sequence< int , ps... > )
+{
+	( [&]{
+		constexpr int k = WTab< Dim >::K( ps ) , l = WTab< Dim >::L( ps );
+		const double w = xp[k]*yp[l] - xp[l]*yp[k];
+		WedgeColumn< Dim , ps >( z , w , std::make_integer_sequence< int , WTab< Dim >::C-1 >{} );
+	}() , ... );
+}
+// u[j] = sum_k A(j,k) yp[k+1], v[j] = sum_k A(j,k) xp[k+1]
+template< int Dim , int j , int ... ks >
+__device__ __forceinline__ void QuadRow( const double *xp , const double *yp , double &u , double &v , std::integer_sequence< int , ks... > )
+{
+	( [&]{ if constexpr( WTab< Dim >::A( j , ks )!=0.0 ){ u = fma( WTab< Dim >::A( j , ks ) , yp[ks+1] , u ) ; v = fma( WTab< Dim >::A( j , ks ) , xp[ks+1] , v ); } }() , ... );
+}
+template< int Dim , int ... js >
+__device__ __forceinline__ void QuadForms( const double *xp , const double *yp , const double *z , double &a20 , double &a02 , double &a11 , double &b10 , double &b01 , std::integer_sequence< int , js... > )
+{
+	( [&]{
+		double u = 0 , v = 0;
+		QuadRow< Dim , js >( xp , yp , u , v , std::make_integer_sequence< int , WTab< Dim >::C-1 >{} );
+		a20 = fma( yp[js+1] , u , a20 );
+		a02 = fma( xp[js+1] , v , a02 );
+		a11 = fma( xp[js+1] , u , a11 );
+		b10 = fma( yp[js+1] , z[js] , b10 );
+		b01 = fma( xp[js+1] , z[js] , b01 );
+	}() , ... );
+}
+
+// Wedge-energy terms of one element for the node at its corner 0 after the
+// corner relabelling c -> c^beta (the element integrals are invariant under
+// reflections).  xp/yp hold the relabelled corner coefficients.  Returns the
+// unscaled coefficients of E(x+s e_i, y+t e_i) restricted to the element.
+// The quadratic terms follow SystemEnergy.h:623-639, the cubic terms :616-621.
+template< int Dim >
+__device__ __forceinline__ void ElementWedge( const double *xp , const double *yp , double &q20 , double &q02 , double &q11 , double &q10 , double &q01 , double &zy )
+{
+	constexpr int C = 1<<Dim , P = ( C*(C-1) )/2;
+	double z[C-1];
+#pragma unroll
+	for( int j=0 ; j<C-1 ; j++ ) z[j] = 0;
+	WedgeContract< Dim >( xp , yp , z , std::make_integer_sequence< int , P >{} );
+	double a20 = 0 , a02 = 0 , a11 = 0 , b10 = 0 , b01 = 0;
+	QuadForms< Dim >( xp , yp , z , a20 , a02 , a11 , b10 , b01 , std::make_integer_sequence< int , C-1 >{} );
+	q20 = a20 , q02 = a02 , q11 = -2.*a11 , q10 = 2.*b10 , q01 = -2.*b01 , zy = b10;
+}
+
+// The omega=1/2 block-Newton step of ExteriorPoissonSolvers.h:173-181 with the
+// 2x2 inverse of Misha/Geometry.inl:319-331.
+__device__ __forceinline__ bool NodeStep( double q20 , double q02 , double q11 , double q10 , double q01 , double &dx , double &dy )
+{
+	const double H00 = 2*q20 , H11 = 2*q02 , H01 = q11 , H10 = q11;
+	const double det = H00*H11 - H01*H10;
+	if( !( fabs( det )>1e-20 ) ) return false;
+	const double d = 1. / det;
+	const double m00 = -( H11*d ) , m11 = -( H00*d ) , m01 = -( (-H01)*d ) , m10 = -( (-H10)*d );
+	double o0 = q10*m00 , o1 = q10*m01;
+	o0 = o0 + q01*m10 , o1 = o1 + q01*m11;
+	dx = o0*0.5 , dy = o1*0.5;
+	return true;
+}
+
+// Processes node i (multi-index F) with the C lanes of its group: lane = one
+// incident element (wedge terms) + a share of the 3^Dim window (R, K, B terms).
+// All lanes return the group totals.
+template< int Dim , int MODE >
+__device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int *F , long long i , int lane , double acc[5] )
+{
+	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W;
+	double q20 = 0 , q02 = 0 , q11 = 0 , q10 = 0 , q01 = 0;
+	double eW = 0 , eB = 0 , eR = 0 , eK = 0;
+	// ---- the incident element of this lane
+	{
+		bool valid = true;
+		long long sig[Dim];
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			const int b = Bit< Dim >( lane , d );
+			const int e = F[d] - 1 + b;
+			valid = valid && e>=0 && e<L.res;
+			sig[d] = b ? L.stride[d] : -L.stride[d];
+		}
+		if( valid )
+		{
+			double xp[C] , yp[C];
+#pragma unroll
+			for( int c=0 ; c<C ; c++ )
+			{
+				long long o = i;
+#pragma unroll
+				for( int d=0 ; d<Dim ; d++ ) if( Bit< Dim >( c , d ) ) o += sig[d];
+				xp[c] = L.x[o] , yp[c] = L.y[o];
+			}
+			double a20 , a02 , a11 , a10 , a01 , zy;
+			ElementWedge< Dim >( xp , yp , a20 , a02 , a11 , a10 , a01 , zy );
+			q20 = a20*L.sW , q02 = a02*L.sW , q11 = a11*L.sW , q10 = a10*L.sW , q01 = a01*L.sW;
+			if constexpr( MODE==MODE_ENERGY ) eW = xp[0]*zy*L.sW;
+		}
+	}
+	// ---- the lane's share of the 3^Dim window
+	int cse = 0;
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ) cse = cse*3 + ( F[d]<=0 ? 0 : ( F[d]>=L.res ? 2 : 1 ) );
+	const double xi = L.x[i] , yi = L.y[i];
+	for( int w=lane ; w<W ; w+=C )
+	{
+		int t = w;
+		long long r = i;
+		bool in = true;
+#pragma unroll
+		for( int d=Dim-1 ; d>=0 ; d-- )
+		{
+			const int o = t%3 - 1 ; t /= 3;
+			const int g = F[d]+o;
+			in = in && g>=0 && g<=L.res;
+			r += o*L.stride[d];
+		}
+		if( !in ) continue;
+		const double xr = L.x[r] , yr = L.y[r];
+		const double Rv = L.R[ i*W+w ] , Bv = L.B[ i*W+w ] , Kv = L.K[ (long long)cse*W + w ];
+		if constexpr( MODE==MODE_ENERGY )
+		{
+			eR += xi*Rv*xr + yi*Rv*yr;
+			eK += xi*Kv*xr + yi*Kv*yr;
+			eB += xi*Bv*yr;
+		}
+		else
+		{
+			const double Dv = Kv*L.dW;
+			if( w==W/2 ){ q20 += Rv + Dv ; q02 += Rv + Dv; }
+			q10 += 2.*Rv*xr + 2.*Dv*xr;
+			q01 += 2.*Rv*yr + 2.*Dv*yr;
+			if( w!=W/2 ){ q10 -= 2.*Bv*yr ; q01 += 2.*Bv*xr; }
+		}
+	}
+	if constexpr( MODE==MODE_ENERGY ){ q20 = eW , q02 = eB , q11 = eR , q10 = eK , q01 = 0; }
+	// ---- group reduction (xor butterfly inside the C-lane group)
+#pragma unroll
+	for( int m=1 ; m<C ; m<<=1 )
+	{
+		q20 += __shfl_xor( q20 , m , C );
+		q02 += __shfl_xor( q02 , m , C );
+		q11 += __shfl_xor( q11 , m , C );
+		q10 += __shfl_xor( q10 , m , C );
+		q01 += __shfl_xor( q01 , m , C );
+	}
+	acc[0] = q20 , acc[1] = q02 , acc[2] = q11 , acc[3] = q10 , acc[4] = q01;
+}
+
+// Per-dimension counts of the nodes of a colour and the k-th node's index.
+template< int Dim >
+struct ColourSet
+{
+	int cnt[Dim] , par[Dim];
+	long long total;
+	__device__ __host__ void set( int res , int colour )
+	{
+		total = 1;
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			par[d] = ( colour>>d ) & 1;   // bit d of the colour is the parity of coordinate d (ExteriorPoisson.h:161-169)
+			cnt[d] = ( res - par[d] )/2 + 1;
+			total *= cnt[d];
+		}
+	}
+	__device__ __forceinline__ void node( long long k , int *F ) const
+	{
+#pragma unroll
+		for( int d=Dim-1 ; d>=0 ; d-- ){ const int kd = (int)( k % cnt[d] ) ; k /= cnt[d] ; F[d] = 2*kd + par[d]; }
+	}
+};
+
+template< int Dim > __device__ __forceinline__ long long FlatIndex( const int *F , const long long *stride ){ long long i = 0 ; for( int d=0 ; d<Dim ; d++ ) i += F[d]*stride[d] ; return i; }
+template< int Dim > __device__ __forceinline__ void Unflatten( long long i , int res , int *F ){ for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = (int)( i % (res+1) ) ; i /= res+1; } }
+
+// One colour phase of the paired Gauss-Seidel sweep (one launch per colour).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSet< Dim > cs )
+{
+	constexpr int C = DimC< Dim >::C;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	const long long k = g / C;
+	const int lane = (int)( g % C );
+	if( k>=cs.total ) return;
+	int F[Dim];
+	cs.node( k , F );
+	const long long i = FlatIndex< Dim >( F , L.stride );
+	double q[5];
+	NodeGroup< Dim , MODE_GS >( L , F , i , lane , q );
+	if( lane==0 )
+	{
+		double dx , dy;
+		if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
+	}
+}
+
+// All iterations of a small level in one workgroup (barrier between colours).
+template< int Dim >
+__global__ void __launch_bounds__( 512 ) KGsSmall( LevelView< Dim > L , int iters )
+{
+	constexpr int C = DimC< Dim >::C;
+	for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+	{
+		ColourSet< Dim > cs ; cs.set( L.res , c );
+		for( long long g=threadIdx.x ; g<cs.total*C ; g+=blockDim.x )
+		{
+			const long long k = g / C;
+			const int lane = (int)( g % C );
+			int F[Dim];
+			cs.node( k , F );
+			const long long i = FlatIndex< Dim >( F , L.stride );
+			double q[5];
+			NodeGroup< Dim , MODE_GS >( L , F , i , lane , q );
+			if( lane==0 )
+			{
+				double dx , dy;
+				if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
+			}
+		}
+		__syncthreads();
+	}
+}
+
+// Per-node quadratic fits of all nodes (test hook).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KQFit( LevelView< Dim > L , double *q )
+{
+	constexpr int C = DimC< Dim >::C;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	const long long i = g / C;
+	const int lane = (int)( g % C );
+	if( i>=L.N ) return;
+	int F[Dim];
+	Unflatten< Dim >( i , L.res , F );
+	double a[5];
+	NodeGroup< Dim , MODE_QFIT >( L , F , i , lane , a );
+	if( lane==0 ) for( int k=0 ; k<5 ; k++ ) q[5*i+k] = a[k];
+}
+
+// Energy terms: per block partial sums of (W, y^T B x, x^T R x + y^T R y, x^T K x + y^T K y).
+// W = 1/2 sum_i x_i dW/dx_i (W is homogeneous of degree 2 in x).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KEnergy( LevelView< Dim > L , double *partials )
+{
+	constexpr int C = DimC< Dim >::C , G = 256/C;
+	__shared__ double sh[G][4];
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	const long long i = g / C;
+	const int lane = (int)( g % C );
+	double a[5] = { 0 , 0 , 0 , 0 , 0 };
+	if( i<L.N )
+	{
+		int F[Dim];
+		Unflatten< Dim >( i , L.res , F );
+		NodeGroup< Dim , MODE_ENERGY >( L , F , i , lane , a );
+	}
+	if( lane==0 ) for( int k=0 ; k<4 ; k++ ) sh[ threadIdx.x/C ][k] = a[k];
+	__syncthreads();
+	if( threadIdx.x==0 )
+	{
+		double s[4] = { 0 , 0 , 0 , 0 };
+		for( int gidx=0 ; gidx<G ; gidx++ ) for( int k=0 ; k<4 ; k++ ) s[k] += sh[gidx][k];
+		for( int k=0 ; k<4 ; k++ ) partials[ (long long)blockIdx.x*4 + k ] = s[k];
+	}
+}
+
+// Deterministic sum of nb rows of nv values (one block).
+template< int NV >
+__global__ void __launch_bounds__( 256 ) KSumRows( const double *in , long long nb , double *out )
+{
+	__shared__ double sh[256][NV];
+	double s[NV];
+	for( int k=0 ; k<NV ; k++ ) s[k] = 0;
+	for( long long r=threadIdx.x ; r<nb ; r+=blockDim.x ) for( int k=0 ; k<NV ; k++ ) s[k] += in[ r*NV + k ];
+	for( int k=0 ; k<NV ; k++ ) sh[threadIdx.x][k] = s[k];
+	__syncthreads();
+	for( int h=128 ; h>0 ; h>>=1 )
+	{
+		if( threadIdx.x<h ) for( int k=0 ; k<NV ; k++ ) sh[threadIdx.x][k] += sh[threadIdx.x+h][k];
+		__syncthreads();
+	}
+	if( threadIdx.x==0 ) for( int k=0 ; k<NV ; k++ ) out[k] = sh[0][k];
+}
+
+// Block partial sums of a strided array (first pass of a deterministic reduction).
+__global__ void __launch_bounds__( 256 ) KPartialSum( const double *in , long long n , double *partials )
+{
+	__shared__ double sh[256];
+	double s = 0;
+	const long long per = ( n + gridDim.x - 1 ) / gridDim.x;
+	const long long b = (long long)blockIdx.x*per , e = b+per<n ? b+per : n;
+	for( long long r=b+threadIdx.x ; r<e ; r+=blockDim.x ) s += in[r];
+	sh[threadIdx.x] = s;
+	__syncthreads();
+	for( int h=128 ; h>0 ; h>>=1 ){ if( threadIdx.x<h ) sh[threadIdx.x] += sh[threadIdx.x+h] ; __syncthreads(); }
+	if( threadIdx.x==0 ) partials[blockIdx.x] = sh[0];
+}
+
+// x_f (=, +=, -=) sum_c P(f,c) x_c, summed over coarse c in increasing index
+// order (Eigen's column-major SpMV of SystemEnergy.h:162; weights of
+// Hat.scalars.inl:373-386).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf , int cres , int mode )
+{
+	const int fres = 2*cres;
+	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= fres+1;
+	const long long f = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( f>=nf ) return;
+	int F[Dim];
+	Unflatten< Dim >( f , fres , F );
+	int nodd = 0 , oddDims[Dim];
+	for( int d=0 ; d<Dim ; d++ ) if( F[d]&1 ) oddDims[nodd++] = d;
+	double s = 0;
+	for( int m=0 ; m<(1<<nodd) ; m++ )
+	{
+		long long c = 0;
+		double w = 1;
+		int oi = 0;
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			int cd = F[d]>>1;
+			if( F[d]&1 ){ cd += ( m>>( nodd-1-oi ) ) & 1 ; oi++ ; w *= 0.5; }
+			c = c*(cres+1) + cd;
+		}
+		s += w*xc[c];
+	}
+	if( mode==0 ) xf[f] = s;
+	else if( mode==1 ) xf[f] = xf[f] + s;
+	else xf[f] = xf[f] - s;
+}
+
+// out = a + b
+__global__ void KAddInto( const double *a , const double *b , double *out , long long n )
+{
+	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( i<n ) out[i] = a[i] + b[i];
+}
+
+__global__ void KScale2( double *x , double *y , long long n , const double *s )
+{
+	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( i<n ){ x[i] *= s[0] ; y[i] *= s[0]; }
+}
+
+// Galerkin coarsening Mc = (P^T Mf) P with the accumulation order of Eigen's
+// conservative sparse product (SystemEnergy.h:472-473): for coarse (i,j),
+// sum over fine k of P(.,j) increasing of [ sum over fine m increasing of
+// P(m,i) Mf(m,k) ] * P(k,j); zero terms skipped.
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGalerkin( const double *Mf , double *Mc , int cres )
+{
+	constexpr int W = DimC< Dim >::W;
+	const int fres = 2*cres;
+	long long nc = 1 ; for( int d=0 ; d<Dim ; d++ ) nc *= cres+1;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=nc*W ) return;
+	const long long j = g / W;
+	const int wi = (int)( g % W );
+	int J[Dim] , I[Dim];
+	Unflatten< Dim >( j , cres , J );
+	{
+		int t = wi;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ I[d] = J[d] + t%3 - 1 ; t /= 3; }
+	}
+	for( int d=0 ; d<Dim ; d++ ) if( I[d]<0 || I[d]>cres ){ Mc[g] = 0 ; return; }
+	long long fstride[Dim];
+	{
+		long long s = 1;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ fstride[d] = s ; s *= fres+1; }
+	}
+	int klo[Dim] , khi[Dim];
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		klo[d] = max( max( 2*J[d]-1 , 2*I[d]-2 ) , 0 );
+		khi[d] = min( min( 2*J[d]+1 , 2*I[d]+2 ) , fres );
+		if( klo[d]>khi[d] ){ Mc[g] = 0 ; return; }
+	}
+	double acc = 0;
+	bool any = false;
+	int K[Dim];
+	for( int d=0 ; d<Dim ; d++ ) K[d] = klo[d];
+	while( true )
+	{
+		double pkj = 1;
+		long long k = 0;
+		for( int d=0 ; d<Dim ; d++ ){ if( K[d]!=2*J[d] ) pkj *= 0.5 ; k += K[d]*fstride[d]; }
+		int mlo[Dim] , mhi[Dim] , M[Dim];
+		bool ok = true;
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			mlo[d] = max( max( 2*I[d]-1 , K[d]-1 ) , 0 );
+			mhi[d] = min( min( 2*I[d]+1 , K[d]+1 ) , fres );
+			if( mlo[d]>mhi[d] ) ok = false;
+			M[d] = mlo[d];
+		}
+		if( ok )
+		{
+			double t = 0;
+			bool tany = false;
+			while( true )
+			{
+				double pmi = 1;
+				int wm = 0;
+				for( int d=0 ; d<Dim ; d++ ){ if( M[d]!=2*I[d] ) pmi *= 0.5 ; wm = wm*3 + ( M[d]-K[d]+1 ); }
+				const double mv = Mf[ k*W + wm ];
+				if( mv!=0 ){ if( tany ) t += pmi*mv ; else t = pmi*mv , tany = true; }
+				int d = Dim-1;
+				for( ; d>=0 ; d-- ){ if( ++M[d]<=mhi[d] ) break ; M[d] = mlo[d]; }
+				if( d<0 ) break;
+			}
+			if( tany ){ if( any ) acc += t*pkj ; else acc = t*pkj , any = true; }
+		}
+		int d = Dim-1;
+		for( ; d>=0 ; d-- ){ if( ++K[d]<=khi[d] ) break ; K[d] = klo[d]; }
+		if( d<0 ) break;
+	}
+	Mc[g] = acc;
+}
+
+//////////////////////////////////////////////////////////////////////////
+// Build phase
+//////////////////////////////////////////////////////////////////////////
+
+// Uniform B-spline weights of degree Deg at x in [0,1] (GridSamples.inl:32-48).
+template< int Deg >
+__device__ __forceinline__ void BSpline( double x , double *v )
+{
+	if constexpr( Deg==0 ) v[0] = 1.;
+	else
+	{
+		const double Scale = 1./Deg;
+		BSpline< Deg-1 >( x , v+1 );
+		v[0] = v[1] * (1.-x) * Scale;
+#pragma unroll
+		for( int i=1 ; i<Deg ; i++ )
+		{
+			const double x1 = ( x-i+Deg ) , x2 = ( -x+i+1 );
+			v[i] = ( v[i]*x1 + v[i+1]*x2 ) * Scale;
+		}
+		v[Deg] *= x * Scale;
+	}
+}
+
+template< int Dim >
+struct TreeGrids
+{
+	double *data[MAXT];            // data[t]: (2^(t-1)+1)^Dim corners x NV values
+	unsigned char *kids[MAXT];     // kids[t]: (2^t)^Dim flags, t<=maxDepth
+	int maxDepth;                  // D (tree depths 1..D+1)
+};
+
+template< int Dim > __device__ __forceinline__ long long CornerIndex( int t , const int *o ){ const int r = 1<<(t-1) ; long long i = 0 ; for( int d=0 ; d<Dim ; d++ ) i = i*(r+1) + o[d] ; return i; }
+template< int Dim > __device__ __forceinline__ long long TreeNodeIndex( int t , const int *o ){ long long i = 0 ; for( int d=0 ; d<Dim ; d++ ) i = i*( 1LL<<t ) + o[d] ; return i; }
+
+// Per occupied cell: the mean position and the summed density statistics of its
+// samples, in the cell's sample order (GridSamples.inl:381-387).
+template< int Dim >
+__global__ void KCellStats( const double *pos , const int *cellStart , const int *cellSamples , int ncell , double *cellP , double *cellData )
+{
+	constexpr int NV = DimC< Dim >::NDNI;
+	const int c = blockIdx.x*blockDim.x + threadIdx.x;
+	if( c>=ncell ) return;
+	double p[Dim] , dat[NV];
+	for( int d=0 ; d<Dim ; d++ ) p[d] = 0;
+	for( int k=0 ; k<NV ; k++ ) dat[k] = 0;
+	for( int s=cellStart[c] ; s<cellStart[c+1] ; s++ )
+	{
+		const double *q = pos + (long long)cellSamples[s]*Dim;
+		for( int d=0 ; d<Dim ; d++ ) p[d] += q[d];
+		dat[0] += 1.;
+		for( int d=0 ; d<Dim ; d++ ) dat[1+d] += q[d]*1.;
+		for( int a=0 ; a<Dim ; a++ ) for( int b=0 ; b<Dim ; b++ ) dat[1+Dim+a*Dim+b] += q[a]*q[b]*1.;
+	}
+	const double inv = 1. / (double)( cellStart[c+1]-cellStart[c] );
+	for( int d=0 ; d<Dim ; d++ ) cellP[ (long long)c*Dim+d ] = p[d]*inv;
+	for( int k=0 ; k<NV ; k++ ) cellData[ (long long)c*NV+k ] = dat[k];
+}
+
+// Node creation of the tree splatters (Misha/RegularTree.inl:762-800): the
+// parents (C_t+i)>>1, i in [-K,K+1], of every occupied cell's ancestors get
+// children.  One thread per (cell, depth).
+template< int Dim >
+__global__ void KTouch( const int *cellIdx , int ncell , TreeGrids< Dim > T , int K )
+{
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	const int nt = T.maxDepth;   // depths t = 2 .. D+1
+	if( g>=(long long)ncell*nt ) return;
+	const int c = (int)( g / nt ) , t = (int)( g % nt ) + 2;
+	int lo[Dim] , hi[Dim] , P[Dim];
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		const int Cd = cellIdx[ (long long)c*Dim+d ]>>( T.maxDepth+1-t );
+		lo[d] = max( 0 , ( Cd-K )>>1 );
+		hi[d] = min( (1<<(t-1))-1 , ( Cd+K+1 )>>1 );
+		P[d] = lo[d];
+	}
+	while( true )
+	{
+		T.kids[t-1][ TreeNodeIndex< Dim >( t-1 , P ) ] = 1;
+		int d = Dim-1;
+		for( ; d>=0 ; d-- ){ if( ++P[d]<=hi[d] ) break ; P[d] = lo[d]; }
+		if( d<0 ) break;
+	}
+}
+
+// Estimator splat (GridSamples.inl:436-491, merged cells): one thread per
+// (cell, tree depth, tap); density statistics added with FP64 atomics.
+template< int Dim , int KR >
+__global__ void KEstSplat( const double *cellP , const double *cellData , int ncell , TreeGrids< Dim > T , const double *depthScale )
+{
+	constexpr int NV = DimC< Dim >::NDNI , Deg = 2*KR+1 , NT = Deg+1;
+	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
+	const int nt = T.maxDepth+1;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=(long long)ncell*nt*taps ) return;
+	const int tap = (int)( g % taps );
+	const int t = (int)( ( g / taps ) % nt ) + 1;
+	const int c = (int)( g / ( (long long)taps*nt ) );
+	const int res = 1<<(t-1);
+	const double *p = cellP + (long long)c*Dim;
+	int o[Dim];
+	double scl = 1.;
+	int tt = tap;
+	int ii[Dim];
+	for( int d=Dim-1 ; d>=0 ; d-- ){ ii[d] = tt%NT ; tt /= NT; }
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		const int E = (int)floor( p[d]*res );
+		double v[NT];
+		BSpline< Deg >( p[d]*res - E , v );
+		scl = scl * v[ ii[d] ];
+		o[d] = E - KR + ii[d];
+		if( o[d]<0 || o[d]>res ) return;
+	}
+	double *node = T.data[t] + CornerIndex< Dim >( t , o )*NV;
+	const double dw = depthScale[t-1];
+	for( int k=0 ; k<NV ; k++ ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
+}
+
+// Linear interpolation of a tree depth at p (GridSamples.inl:774-809).
+template< int Dim , int NV >
+__device__ __forceinline__ void TreeEval( const TreeGrids< Dim > &T , int t , const double *p , double *out , int nv )
+{
+	const int res = 1<<(t-1);
+	int e[Dim];
+	double v[Dim][2];
+	for( int d=0 ; d<Dim ; d++ ){ e[d] = (int)floor( p[d]*res ) ; BSpline< 1 >( p[d]*res - e[d] , v[d] ); }
+	for( int k=0 ; k<nv ; k++ ) out[k] = 0;
+	for( int f=0 ; f<(1<<Dim) ; f++ )
+	{
+		int o[Dim];
+		bool in = true;
+		double scale = 1.;
+		for( int d=0 ; d<Dim ; d++ ){ const int b = Bit< Dim >( f , d ) ; o[d] = e[d]+b ; in = in && o[d]<=res ; scale *= v[d][b]; }
+		if( !in ) continue;
+		const double *node = T.data[t] + CornerIndex< Dim >( t , o )*NV;
+		for( int k=0 ; k<nv ; k++ ) out[k] += node[k]*scale;
+	}
+}
+
+// Symmetric eigenvalues by cyclic Jacobi, ascending.
+template< int Dim >
+__device__ void SymEig( double A[Dim][Dim] , double *ev )
+{
+	for( int sweep=0 ; sweep<64 ; sweep++ )
+	{
+		double off = 0;
+		for( int i=0 ; i<Dim ; i++ ) for( int j=i+1 ; j<Dim ; j++ ) off += A[i][j]*A[i][j];
+		if( off==0 ) break;
+		for( int p=0 ; p<Dim ; p++ ) for( int q=p+1 ; q<Dim ; q++ )
+		{
+			if( A[p][q]==0 ) continue;
+			const double theta = ( A[q][q]-A[p][p] ) / ( 2*A[p][q] );
+			const double t = ( theta>=0 ? 1. : -1. ) / ( fabs( theta ) + sqrt( theta*theta+1 ) );
+			const double c = 1./sqrt( t*t+1 ) , s = t*c;
+			for( int k=0 ; k<Dim ; k++ ){ const double akp = A[k][p] , akq = A[k][q] ; A[k][p] = c*akp - s*akq ; A[k][q] = s*akp + c*akq; }
+			for( int k=0 ; k<Dim ; k++ ){ const double apk = A[p][k] , aqk = A[q][k] ; A[p][k] = c*apk - s*aqk ; A[q][k] = s*apk + c*aqk; }
+		}
+	}
+	for( int i=0 ; i<Dim ; i++ ) ev[i] = A[i][i];
+	for( int i=1 ; i<Dim ; i++ ) for( int j=i ; j>0 && ev[j-1]>ev[j] ; j-- ){ const double t = ev[j] ; ev[j] = ev[j-1] ; ev[j-1] = t; }
+}
+
+// Per sample: fractional depth, measure and noise of the tree estimator
+// (GridSamples.inl:811-896, 82-105), the R weight exp(-n^2/(2 sigma^2))
+// (ExteriorPoissonRecon.cpp:222) and the scaled frame that is splatted
+// (ExteriorPoisson.h:181-213).
+template< int Dim >
+__global__ void KEstQuery( const double *pos , const double *skewPacked , int n , TreeGrids< Dim > T , double nSigma ,
+                           double *measure , double *noise , double *depth , double *rweight , double *skewScaled )
+{
+	constexpr int NV = DimC< Dim >::NDNI , CD = DimC< Dim >::CD , CoDim = 2;
+	const int s = blockIdx.x*blockDim.x + threadIdx.x;
+	if( s>=n ) return;
+	const double *p = pos + (long long)s*Dim;
+	const double target = (double)( 1<<(Dim-CoDim) );
+	const int maxT = T.maxDepth+1;
+	auto spc = [&]( int t ){ double w ; TreeEval< Dim , NV >( T , t , p , &w , 1 ) ; return w / (double)( 1<<( (t-1)*(Dim-CoDim) ) ); };
+	double dep;
+	{
+		int d1 = maxT , d2 = maxT+1;
+		while( d2>1 && spc( d1 )<target ) d1-- , d2--;
+		if( d2==1 || d2==maxT+1 )
+		{
+			const int d = d2==1 ? 1 : maxT;
+			dep = d + log( spc( d )/target ) / log( 2. );
+		}
+		else
+		{
+			const double s1 = spc( d1 ) , s2 = spc( d2 );
+			dep = !s2 ? (double)d1 : d1 + log( target/s1 ) / log( s2/s1 );
+		}
+	}
+	double info[NV];
+	if( !( dep<maxT ) ) TreeEval< Dim , NV >( T , maxT , p , info , NV );      // also catches NaN
+	else if( !( dep>=1 ) ) TreeEval< Dim , NV >( T , 1 , p , info , NV );
+	else
+	{
+		const int d1 = min( max( (int)floor( dep ) , 1 ) , maxT-1 ) , d2 = d1+1;
+		const double dd = dep-d1;
+		double a[NV] , b[NV];
+		TreeEval< Dim , NV >( T , d1 , p , a , NV );
+		TreeEval< Dim , NV >( T , d2 , p , b , NV );
+		for( int k=0 ; k<NV ; k++ ) info[k] = a[k]*(1.-dd) + b[k]*dd;
+	}
+	const double w = info[0];
+	double A[Dim][Dim] , ev[Dim];
+	for( int i=0 ; i<Dim ; i++ ) for( int j=0 ; j<Dim ; j++ ) A[i][j] = info[1+Dim+i*Dim+j]/w - info[1+i]/w * info[1+j]/w;
+	SymEig< Dim >( A , ev );
+	double sum = 0 , low = 0;
+	for( int d=0 ; d<Dim ; d++ ){ sum += ev[d] ; if( d<CoDim ) low += ev[d]; }
+	const double nz = ( low*Dim ) / ( sum*CoDim );
+	const double m = 1./w;
+	measure[s] = m , noise[s] = nz , depth[s] = dep - 1.;
+	rweight[s] = exp( -nz*nz/( 2.*nSigma*nSigma ) );
+	// frame: S / |S|_F * measure * 2^(depth*CoDim)
+	double sq = 0;
+	{
+		const double *q = skewPacked + (long long)s*CD;
+		double M[Dim][Dim];
+		for( int i=0 ; i<Dim ; i++ ) for( int j=0 ; j<Dim ; j++ ) M[i][j] = 0;
+		for( int i=0 , idx=0 ; i<Dim ; i++ ) for( int j=0 ; j<i ; j++ , idx++ ) M[i][j] = q[idx] , M[j][i] = -q[idx];
+		for( int i=0 ; i<Dim ; i++ ) for( int j=0 ; j<Dim ; j++ ) sq += M[i][j]*M[i][j];
+		const double inv = 1./sqrt( sq );
+		const double lv = (double)( 1<<( T.maxDepth*CoDim ) );
+		for( int c=0 ; c<CD ; c++ ) skewScaled[ (long long)s*CD + c ] = ( ( q[c]*inv )*m )*lv;
+	}
+}
+
+// Skew splat at the fractional sample depth (GridSamples.inl:393-491): one
+// thread per (sample, tap), up to two tree depths, FP64 atomics.
+template< int Dim , int KR >
+__global__ void KSkewSplat( const double *pos , const double *skewScaled , const double *depth , int n , TreeGrids< Dim > T )
+{
+	constexpr int CD = DimC< Dim >::CD , Deg = 2*KR+1 , NT = Deg+1;
+	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=(long long)n*taps ) return;
+	const int s = (int)( g / taps ) , tap = (int)( g % taps );
+	const double *p = pos + (long long)s*Dim;
+	const int D = T.maxDepth;
+	const double dep = depth[s] + 1.;
+	int ts[2] ; double fw[2] ; int nts;
+	auto DW = [&]( int d ){ return (double)( 1<<( d*Dim ) ); };
+	if( !( dep<D+1 ) ){ ts[0] = D+1 , fw[0] = -1 , nts = 1; }    // also catches NaN
+	else if( dep<=1 ){ ts[0] = 1 , fw[0] = -1 , nts = 1; }
+	else
+	{
+		const int d1 = min( max( (int)floor( dep ) , 1 ) , D );
+		const double dd = dep-d1;
+		ts[0] = d1 , fw[0] = 1.-dd , ts[1] = d1+1 , fw[1] = dd , nts = 2;
+	}
+	int ii[Dim];
+	{
+		int tt = tap;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ ii[d] = tt%NT ; tt /= NT; }
+	}
+	for( int k=0 ; k<nts ; k++ )
+	{
+		const int t = ts[k] , res = 1<<(t-1);
+		int o[Dim];
+		double scl = 1.;
+		bool in = true;
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			const int E = (int)floor( p[d]*res );
+			double v[NT];
+			BSpline< Deg >( p[d]*res - E , v );
+			scl = scl * v[ ii[d] ];
+			o[d] = E - KR + ii[d];
+			in = in && o[d]>=0 && o[d]<=res;
+		}
+		if( !in ) continue;
+		double *node = T.data[t] + CornerIndex< Dim >( t , o )*CD;
+		for( int c=0 ; c<CD ; c++ )
+		{
+			const double v = skewScaled[ (long long)s*CD + c ];
+			const double dv = fw[k]<0 ? v*DW( t-1 ) : ( v*fw[k] )*DW( t-1 );
+			atomicAdd( node+c , dv*scl );
+		}
+	}
+}
+
+__global__ void KScaleArray( double *a , long long n , double s )
+{
+	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( i<n ) a[i] *= s;
+}
+
+// Target field at every finest element centre: sum over tree depths up to the
+// leaf containing the centre of the multilinear interpolation of the splatted
+// frames (ExteriorPoisson.h:294-307, GridSamples.inl:600-651).  Also writes
+// |S|^2 (full-matrix Frobenius) per element.
+template< int Dim >
+__global__ void KField( TreeGrids< Dim > T , double *S , double *S2 )
+{
+	constexpr int CD = DimC< Dim >::CD;
+	const int D = T.maxDepth , res = 1<<D;
+	long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= res;
+	const long long e = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( e>=ne ) return;
+	int I[Dim];
+	{
+		long long t = e;
+		for( int d=0 ; d<Dim ; d++ ){ I[d] = (int)( t%res ) ; t /= res; }
+	}
+	double p[Dim];
+	for( int d=0 ; d<Dim ; d++ ) p[d] = ( I[d] + 0.5 ) / res;
+	int L = 1;
+	while( L<=D )
+	{
+		int o[Dim];
+		for( int d=0 ; d<Dim ; d++ ) o[d] = I[d]>>( D+1-L );
+		if( !T.kids[L][ TreeNodeIndex< Dim >( L , o ) ] ) break;
+		L++;
+	}
+	double s[CD];
+	for( int c=0 ; c<CD ; c++ ) s[c] = 0;
+	for( int t=1 ; t<=L ; t++ )
+	{
+		const int r = 1<<(t-1);
+		int eo[Dim];
+		double v[Dim][2];
+		for( int d=0 ; d<Dim ; d++ ){ eo[d] = (int)floor( p[d]*r ) ; BSpline< 1 >( p[d]*r - eo[d] , v[d] ); }
+		for( int f=0 ; f<(1<<Dim) ; f++ )
+		{
+			int o[Dim];
+			double scl = 1.;
+			bool in = true;
+			for( int d=0 ; d<Dim ; d++ ){ const int b = Bit< Dim >( f , d ) ; scl = scl * v[d][b] ; o[d] = eo[d]+b ; in = in && o[d]<=r; }
+			if( !in ) continue;
+			const double *node = T.data[t] + CornerIndex< Dim >( t , o )*CD;
+			for( int c=0 ; c<CD ; c++ ) s[c] = s[c] + node[c]*scl;
+		}
+	}
+	double sq = 0;
+	{
+		double M[Dim][Dim];
+		for( int i=0 ; i<Dim ; i++ ) for( int j=0 ; j<Dim ; j++ ) M[i][j] = 0;
+		for( int i=0 , idx=0 ; i<Dim ; i++ ) for( int j=0 ; j<i ; j++ , idx++ ) M[i][j] = s[idx] , M[j][i] = -s[idx];
+		for( int i=0 ; i<Dim ; i++ ) for( int j=0 ; j<Dim ; j++ ) sq += M[i][j]*M[i][j];
+	}
+	for( int c=0 ; c<CD ; c++ ) S[ e*CD + c ] = s[c];
+	S2[e] = sq;
+}
+
+// B(f2,f1) = sum over common elements E of <S(E), FS[i1][i2]> (Hat.scalars.inl:1637-1686).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KAssembleB( const double *S , const double *FS , int res , double *B )
+{
+	constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD;
+	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= res+1;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=nf*W ) return;
+	const long long f1 = g / W;
+	const int w = (int)( g % W );
+	int F1[Dim] , F2[Dim];
+	Unflatten< Dim >( f1 , res , F1 );
+	{
+		int t = w;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ F2[d] = F1[d] + t%3 - 1 ; t /= 3; }
+	}
+	int lo[Dim] , hi[Dim] , E[Dim];
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		if( F2[d]<0 || F2[d]>res ){ B[g] = 0 ; return; }
+		lo[d] = max( 0 , max( F1[d] , F2[d] )-1 );
+		hi[d] = min( res , min( F1[d] , F2[d] )+1 );
+		if( lo[d]>=hi[d] ){ B[g] = 0 ; return; }
+		E[d] = lo[d];
+	}
+	double dot = 0;
+	while( true )
+	{
+		long long e = 0;
+		int i1 = 0 , i2 = 0;
+		for( int d=Dim-1 ; d>=0 ; d-- ) e = e*res + E[d];
+		for( int d=0 ; d<Dim ; d++ ) i1 = i1*2 + ( F1[d]-E[d] ) , i2 = i2*2 + ( F2[d]-E[d] );
+		double M[Dim][Dim];
+		for( int a=0 ; a<Dim ; a++ ) for( int b=0 ; b<Dim ; b++ ) M[a][b] = 0;
+		const double *s = S + e*CD;
+		for( int a=0 , idx=0 ; a<Dim ; a++ ) for( int b=0 ; b<a ; b++ , idx++ ) M[a][b] = s[idx] , M[b][a] = -s[idx];
+		const double *st = FS + (long long)( i1*C+i2 )*Dim*Dim;
+		double dd = 0;
+		for( int a=0 ; a<Dim ; a++ ) for( int b=0 ; b<Dim ; b++ ) dd += st[a*Dim+b]*M[a][b];
+		dot += dd;
+		int d = Dim-1;
+		for( ; d>=0 ; d-- ){ if( ++E[d]<hi[d] ) break ; E[d] = lo[d]; }
+		if( d<0 ) break;
+	}
+	B[g] = dot;
+}
+
+// R = (E + E^T) * sWeight * mean(measure), E the sample-weighted delta mass of
+// Hat.scalars.inl:1006-1062: for the pair (f,g) summed over the occupied cells
+// adjacent to both in cell order, within a cell over its samples in order.
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const int *cellMap , const int *cellStart , const int *cellSamples ,
+                                                      const double *rweight , int res , const double *scale , double *R )
+{
+	constexpr int W = DimC< Dim >::W;
+	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= res+1;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=nf*W ) return;
+	const long long f = g / W;
+	const int w = (int)( g % W );
+	int F[Dim] , G[Dim];
+	Unflatten< Dim >( f , res , F );
+	{
+		int t = w;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ G[d] = F[d] + t%3 - 1 ; t /= 3; }
+	}
+	int lo[Dim] , hi[Dim] , E[Dim];
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		if( G[d]<0 || G[d]>res ){ R[g] = 0 ; return; }
+		lo[d] = max( 0 , max( F[d] , G[d] )-1 );
+		hi[d] = min( res , min( F[d] , G[d] )+1 );
+		E[d] = lo[d];
+	}
+	// lexicographic order of the pair: a = min(F,G), b = max(F,G)
+	bool FleG = true;
+	for( int d=0 ; d<Dim ; d++ ){ if( F[d]<G[d] ){ FleG = true ; break; } if( F[d]>G[d] ){ FleG = false ; break; } }
+	const int *A = FleG ? F : G , *Bn = FleG ? G : F;
+	const bool diag = ( w==W/2 );
+	double acc = 0;
+	bool any = false;
+	while( true )
+	{
+		long long e = 0;
+		for( int d=Dim-1 ; d>=0 ; d-- ) e = e*res + E[d];
+		const int c = cellMap[e];
+		if( c>=0 )
+		{
+			double value = 0;
+			for( int k=cellStart[c] ; k<cellStart[c+1] ; k++ )
+			{
+				const int s = cellSamples[k];
+				double va = 1. , vb = 1.;
+				for( int d=0 ; d<Dim ; d++ )
+				{
+					double lp = pos[ (long long)s*Dim+d ] * res;
+					lp -= E[d];
+					va *= ( A[d]-E[d] ) ? lp : (1.-lp);
+					vb *= ( Bn[d]-E[d] ) ? lp : (1.-lp);
+				}
+				value += va * vb * rweight[s];
+			}
+			if( diag ) value /= 2.;
+			if( any ) acc = acc + value ; else acc = value , any = true;
+		}
+		int d = Dim-1;
+		for( ; d>=0 ; d-- ){ if( ++E[d]<hi[d] ) break ; E[d] = lo[d]; }
+		if( d<0 ) break;
+	}
+	if( !any ){ R[g] = 0 ; return; }
+	const double v = diag ? acc+acc : acc;
+	R[g] = v * scale[0] * scale[1];
+}
+
+// mean of the per-sample measures (deterministic): out[1] = sum / n, out[0] = sWeight
+__global__ void KMeanMeasure( const double *partials , int nb , long long n , double sWeight , double *out )
+{
+	if( threadIdx.x || blockIdx.x ) return;
+	double s = 0;
+	for( int b=0 ; b<nb ; b++ ) s += partials[b];
+	out[0] = sWeight , out[1] = s / (double)n;
+}
+} // namespace epr

@@ -1,0 +1,127 @@
+"""Host-side mirror of the reference's ExteriorPoisson::Reconstructor.
+
+Reference interface: ExteriorPoissonRecon/ExteriorPoisson.h:56-94 (construct
+from samples, solve(), x(depth)/y(depth) per-level coefficients) and the
+energy methods of Include/SystemEnergy.h:60-125.  All work happens in
+libepr_hip.so on the current HIP device.
+"""
+import ctypes
+
+import numpy as np
+
+from ._lib import EprError, _p, default_params, load
+
+
+class Reconstructor:
+    """One dense-grid reconstruction on the GPU.
+
+    Parameters follow the reference command line (depth, seed, sWeight,
+    dWeight, gsIters, iMult, nSigma, scale, tRadius, dRadius, minSolveDepth,
+    maxSolveDepth, singleLevel, verbose)."""
+
+    def __init__(self, positions, skews, **params):
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        sk = np.ascontiguousarray(skews, dtype=np.float64)
+        if pos.ndim != 2 or pos.shape[1] not in (3, 4):
+            raise ValueError("positions must be n x 3 or n x 4")
+        n, dim = pos.shape
+        if sk.shape != (n, dim, dim):
+            raise ValueError("skews must be n x dim x dim")
+        self.dim = dim
+        self.params = default_params(**params)
+        self.depth = self.params.depth
+        L = load()
+        h = ctypes.c_void_p()
+        rc = L.epr_create(dim, ctypes.byref(self.params), ctypes.byref(h))
+        self._h = h
+        self._check(rc)
+        self._check(L.epr_set_samples(self._h, n, _p(pos), _p(sk)))
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = load().epr_last_error(self._h) if self._h else b"create failed"
+            raise EprError(f"epr error {rc}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().epr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    # -- phases ---------------------------------------------------------------
+    def build(self):
+        self._check(load().epr_build(self._h))
+        return self
+
+    def solve(self):
+        E = np.zeros(2)
+        self._check(load().epr_solve(self._h, _p(E)))
+        return E
+
+    def init(self, level, x=None, y=None):
+        if x is None:
+            self._check(load().epr_init(self._h, level, None, None))
+        else:
+            x = np.ascontiguousarray(x, dtype=np.float64); y = np.ascontiguousarray(y, dtype=np.float64)
+            self._check(load().epr_init(self._h, level, _p(x), _p(y)))
+
+    def gauss_seidel(self, level, iters):
+        self._check(load().epr_gs_sweeps(self._h, level, iters))
+
+    def prolong_add(self, level):
+        self._check(load().epr_prolong_add(self._h, level))
+
+    # -- queries --------------------------------------------------------------
+    def num_functions(self, level):
+        return load().epr_num_functions(self._h, level)
+
+    def coefficients(self, level):
+        n = self.num_functions(level)
+        x = np.zeros(n); y = np.zeros(n)
+        self._check(load().epr_get_coeffs(self._h, level, _p(x), _p(y)))
+        return x, y
+
+    def x(self, level):
+        return self.coefficients(level)[0]
+
+    def y(self, level):
+        return self.coefficients(level)[1]
+
+    def set_coefficients(self, level, x, y):
+        x = np.ascontiguousarray(x, dtype=np.float64); y = np.ascontiguousarray(y, dtype=np.float64)
+        self._check(load().epr_set_coeffs(self._h, level, _p(x), _p(y)))
+
+    def energies(self, level):
+        out = np.zeros(2)
+        self._check(load().epr_energies(self._h, level, _p(out)))
+        return out
+
+    def quadratic_fits(self, level):
+        q = np.zeros(5 * self.num_functions(level))
+        self._check(load().epr_quadratic_fits(self._h, level, _p(q)))
+        return q.reshape(-1, 5)
+
+    def apply(self, level, which, v):
+        v = np.ascontiguousarray(v, dtype=np.float64)
+        n = self.num_functions(level + 1) if which == "P" else self.num_functions(level)
+        out = np.zeros(n)
+        self._check(load().epr_apply(self._h, level, which.encode(), _p(v), _p(out)))
+        return out
+
+    def array(self, name):
+        cnt = ctypes.c_size_t()
+        self._check(load().epr_get_array(self._h, name.encode(), None, 0, ctypes.byref(cnt)))
+        out = np.zeros(cnt.value)
+        self._check(load().epr_get_array(self._h, name.encode(), _p(out), cnt.value, ctypes.byref(cnt)))
+        return out
+
+    def write_tree(self, path, indexer_hash):
+        self._check(load().epr_write_tree(self._h, path.encode(), indexer_hash))
+
+    def write_grid(self, path):
+        self._check(load().epr_write_grid(self._h, path.encode()))
+
+    def seconds(self, which):
+        return load().epr_last_seconds(self._h, 0 if which == "build" else 1)

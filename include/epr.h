@@ -1,0 +1,133 @@
+/*
+ * epr.h -- C ABI of the MI355X (gfx950) ExteriorPoissonRecon solve path.
+ *
+ * One context is one reconstruction on the current HIP device.  Contexts are
+ * not thread-safe; several may coexist.  All buffers in these signatures are
+ * host pointers; device memory stays inside the context.  Every call returns
+ * EPR_OK (0) or a negative status, with a message in epr_last_error().
+ *
+ * The entry points replace the reference's C++ call boundaries for the dense
+ * (--aRadius -1) path; each declaration names the reference interface it
+ * replaces (paths relative to the reference tree).
+ */
+#ifndef EPR_H
+#define EPR_H
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define EPR_OK             0
+#define EPR_ERR_ARG       -1
+#define EPR_ERR_STATE     -2
+#define EPR_ERR_HIP       -3
+#define EPR_ERR_INPUT     -4
+
+typedef struct epr_ctx epr_ctx;
+
+/* Command-line parameters of ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:52-78
+ * (same names, same defaults; aRadius is implied: this library is the dense
+ * --aRadius -1 path). */
+typedef struct
+{
+	int    depth;          /* --depth 5 */
+	int    seed;           /* --seed 0 */
+	int    tRadius;        /* --tRadius 1 */
+	int    dRadius;        /* --dRadius 1 */
+	int    minSolveDepth;  /* --minSolveDepth 0 */
+	int    maxSolveDepth;  /* --maxSolveDepth (<0: = depth) */
+	int    gsIters;        /* --gsIters 10 */
+	int    singleLevel;    /* --singleLevel */
+	int    verbose;        /* --verbose */
+	double nSigma;         /* --nSigma 0.25 */
+	double scale;          /* --scale 1.1 */
+	double sWeight;        /* --sWeight 100 (screening, scales R) */
+	double dWeight;        /* --dWeight 0.00625 (Dirichlet, scales the stiffness) */
+	double iMult;          /* --iMult 2 */
+} epr_params;
+
+/* Fills the reference defaults (ExteriorPoissonRecon.cpp:56-72). */
+void epr_default_params( epr_params *p );
+
+/* Replaces the construction of ExteriorPoisson::Reconstructor<Dim,Energy>
+ * (ExteriorPoisson/ExteriorPoisson.h:56-72, 126-313); dim is 3 or 4. */
+int  epr_create( int dim , const epr_params *p , epr_ctx **out );
+void epr_destroy( epr_ctx *ctx );
+const char *epr_last_error( const epr_ctx *ctx );
+
+/* Hands over the samples read by ReadSamples<Dim>
+ * (ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:282-303): pos is n x dim
+ * world positions, skew is n x dim x dim full frame matrices m[c][r] (PLY
+ * properties skew_c_r).  The unit-cube map of ExteriorPoisson.h:100-123 is
+ * applied here. */
+int  epr_set_samples( epr_ctx *ctx , size_t n , const double *pos , const double *skew );
+
+/* Everything the reference does between reading the samples and solving:
+ * OrderedSampler (Include/OrderedSamples.inl:32-85), TreeEstimator
+ * (Include/GridSamples.inl:740-896), the skew splat and target field
+ * (ExteriorPoisson.h:175-264, 290-309), R = deltaMass (Hat.scalars.inl:976-1063),
+ * B = stiffness(S) (Hat.scalars.inl:1584-1690) and the Galerkin chain
+ * (Include/SystemEnergy.h:464-477). */
+int  epr_build( epr_ctx *ctx );
+
+/* Sets the coefficients of one level; x0/y0 NULL selects the reference's
+ * random initial guess at minSolveDepth (ExteriorPoisson.h:333-341, glibc
+ * srand/rand with the seed). */
+int  epr_init( epr_ctx *ctx , int level , const double *x0 , const double *y0 );
+
+/* iters paired multicolour Gauss-Seidel iterations at one level
+ * (ExteriorPoisson/ExteriorPoissonSolvers.h:156-196). */
+int  epr_gs_sweeps( epr_ctx *ctx , int level , int iters );
+
+/* x[level+1] += P x[level], y likewise (ExteriorPoisson.h:386). */
+int  epr_prolong_add( epr_ctx *ctx , int level );
+
+/* (E_fit, E_reg) of the current coefficients of a level
+ * (Include/SystemEnergy.h:486-497). */
+int  epr_energies( epr_ctx *ctx , int level , double out[2] );
+
+/* The whole cascadic schedule with the random initial guess
+ * (ExteriorPoisson.h:316-409); afterwards the coefficients are hierarchical
+ * as in the reference's .tree output. out_E (may be NULL) receives the final
+ * (E_fit, E_reg) at maxSolveDepth. */
+int  epr_solve( epr_ctx *ctx , double out_E[2] );
+
+/* Number of hat functions (2^level+1)^dim of a level. */
+size_t epr_num_functions( const epr_ctx *ctx , int level );
+
+/* Copies the coefficients of one level to/from the host (Reconstructor::x/y,
+ * ExteriorPoisson.h:83-86). */
+int  epr_get_coeffs( epr_ctx *ctx , int level , double *x , double *y );
+int  epr_set_coeffs( epr_ctx *ctx , int level , const double *x , const double *y );
+
+/* Per-node quadratic fits (q20,q02,q11,q10,q01) of a level at its current
+ * coefficients (Include/SystemEnergy.h:542-643); q holds 5*N doubles. */
+int  epr_quadratic_fits( epr_ctx *ctx , int level , double *q );
+
+/* y = M x for M = 'R' or 'B' of a level, or 'P' (level -> level+1)
+ * (SystemEnergy.h:160-176 operators). */
+int  epr_apply( epr_ctx *ctx , int level , char which , const double *x , double *y );
+
+/* Named build products: unit_pos, cells, cell_counts, measure, noise,
+ * sample_depth, S_elem, c, unit_to_world.  Writes min(count,cap) values when
+ * out is not NULL and returns the count in *count. */
+int  epr_get_array( epr_ctx *ctx , const char *name , double *out , size_t cap , size_t *count );
+
+/* Writes <path> in the reference's .tree layout
+ * (ExteriorPoissonRecon.cpp:259-270) from the hierarchical coefficients left
+ * by epr_solve; indexer_hash is the reference build's
+ * typeid(HierarchicalRegularIndexer<Dim>).hash_code(). */
+int  epr_write_tree( epr_ctx *ctx , const char *path , unsigned long long indexer_hash );
+
+/* Writes the finest-level (phi1,phi2) values as a .grid file
+ * (Misha/RegularGrid.inl:440-471 layout, 2 DOUBLE values per node). */
+int  epr_write_grid( epr_ctx *ctx , const char *path );
+
+/* Wall-clock seconds of the last epr_build / epr_solve (device-synchronised). */
+double epr_last_seconds( const epr_ctx *ctx , int which /* 0 build, 1 solve */ );
+
+#ifdef __cplusplus
+}
+#endif
+#endif

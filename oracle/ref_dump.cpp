@@ -185,7 +185,14 @@ void Execute( std::vector< std::pair< Point< double , Dim > , Hat::SkewSymmetric
 		auto DepthWeight = []( unsigned int d ){ return (double)( 1<<(d*Dim) ); };
 		auto __F = [&]( size_t idx ){ return std::pair< Point< double , Dim > , SkewSymmetricMatrix< double , Dim > >( std::get< 0 >( samples[idx] ) , std::get< 1 >( samples[idx] ) ); };
 		auto __D = [&]( size_t idx ){ return std::get< 2 >( samples[idx] ); };
-		treeSplatter.template addSamples< 1 >( __F , orderedSampler , __D , DepthWeight , false );
+		switch( TensorKernelRadius.value )
+		{
+		case 0: treeSplatter.template addSamples< 0 >( __F , orderedSampler , __D , DepthWeight , false ) ; break;
+		case 1: treeSplatter.template addSamples< 1 >( __F , orderedSampler , __D , DepthWeight , false ) ; break;
+		case 2: treeSplatter.template addSamples< 2 >( __F , orderedSampler , __D , DepthWeight , false ) ; break;
+		case 3: treeSplatter.template addSamples< 3 >( __F , orderedSampler , __D , DepthWeight , false ) ; break;
+		default: treeSplatter.template addSamples< 4 >( __F , orderedSampler , __D , DepthWeight , false ) ; break;
+		}
 		double scale = pow( pow( 0.5 , depth ) , 1.5 * CoDim );
 		treeSplatter.process( [&]( unsigned int , Hat::Index< Dim > , SkewSymmetricMatrix< double , Dim > &s ){ s *= scale; } );
 		unsigned int res = 1<<depth;

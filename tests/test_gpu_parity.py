@@ -1,0 +1,167 @@
+"""Parity of the HIP library (through its C ABI) with the reference.
+
+Small cases are compared with the reference's own vectors (tests/golden);
+larger cases with the CPU oracle on the same inputs.  Tolerances: the GPU
+sums in a different order (element-wise wedge terms, FP64 atomics in the
+splats, device libm), so values agree to rounding, amplified along the
+non-convex Gauss-Seidel trajectory.  The acceptance bar for the solve is the
+north-star's 1e-6 relative on (E, phi1, phi2); observed errors are far below.
+"""
+import os
+import subprocess
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import CASES, ROOT, case_args, load_golden, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_SOLVE = 1e-6     # north-star bar on E and (phi1, phi2)
+
+
+def _strided(g, key, arr, width=1):
+    s = g.get(key + "_stride")
+    if s is None:
+        return arr
+    s = int(s)
+    return arr.reshape(-1, width)[::s].ravel() if width > 1 else arr[::s]
+
+
+@pytest.fixture(scope="module", params=CASES)
+def built(request):
+    from exteriorpoissonrecon_amd import Reconstructor
+    g = load_golden(request.param)
+    print(f"[gpu] case {request.param}", flush=True)
+    r = Reconstructor(g["in_pos"], g["in_skew"], **case_args(g)).build()
+    return request.param, g, r
+
+
+def test_build_products(built):
+    name, g, r = built
+    np.testing.assert_array_equal(r.array("unit_pos"), g["unit_pos"])
+    np.testing.assert_array_equal(r.array("cells"), g["cells"])
+    assert rel_err(r.array("measure"), g["measure"]) < 1e-12
+    assert rel_err(r.array("sample_depth"), g["sample_depth"]) < 1e-12
+    assert rel_err(r.array("noise"), g["noise"]) < 1e-9
+    cd = 3 if r.dim == 3 else 6
+    assert rel_err(_strided(g, "S_elem", r.array("S_elem"), cd), g["S_elem"]) < 1e-12
+    assert rel_err(r.array("c"), g["c"]) < 1e-12
+
+
+def test_operators(built):
+    from oracle.pyoracle import probe
+    name, g, r = built
+    for d in range(r.depth + 1):
+        if f"R{d}_v" not in g:
+            continue
+        v = probe(r.num_functions(d), 0.25 + d)
+        assert rel_err(_strided(g, f"R{d}_v", r.apply(d, "R", v)), g[f"R{d}_v"]) < 1e-11, d
+        assert rel_err(_strided(g, f"B{d}_v", r.apply(d, "B", v)), g[f"B{d}_v"]) < 1e-11, d
+        if f"P{d}_v" in g:
+            np.testing.assert_array_equal(_strided(g, f"P{d}_v", r.apply(d, "P", v)), g[f"P{d}_v"])
+
+
+def test_fits_energies_and_one_sweep(built):
+    from oracle.pyoracle import probe
+    name, g, r = built
+    for d in range(r.depth + 1):
+        if f"E{d}" not in g:
+            continue
+        N = r.num_functions(d)
+        x = probe(N, 1.5 + d) * 0.01
+        y = probe(N, 2.75 + 0.5 * d) * 0.01
+        r.set_coefficients(d, x, y)
+        assert rel_err(r.energies(d), g[f"E{d}"]) < 1e-10, d
+        if f"Q{d}" in g:
+            q = r.quadratic_fits(d).ravel()
+            assert rel_err(_strided(g, f"Q{d}", q, 5), g[f"Q{d}"]) < 1e-9, d
+            r.gauss_seidel(d, 1)
+            gx, gy = r.coefficients(d)
+            assert rel_err(_strided(g, f"gs{d}_x", gx), g[f"gs{d}_x"]) < 1e-9, d
+            assert rel_err(_strided(g, f"gs{d}_y", gy), g[f"gs{d}_y"]) < 1e-9, d
+
+
+def test_full_solve(built):
+    name, g, r = built
+    E = r.solve()
+    assert rel_err(E, g["final_E"]) < TOL_SOLVE
+    for d in range(r.depth + 1):
+        if f"hx{d}" in g:
+            hx, hy = r.coefficients(d)
+            scale = max(np.abs(g["final_x"]).max(), np.abs(g["final_y"]).max())
+            assert np.abs(_strided(g, f"hx{d}", hx) - g[f"hx{d}"]).max() < TOL_SOLVE * scale, d
+            assert np.abs(_strided(g, f"hy{d}", hy) - g[f"hy{d}"]).max() < TOL_SOLVE * scale, d
+    print(f"[gpu] {name}: E rel err {rel_err(E, g['final_E']):.3e}", flush=True)
+
+
+@pytest.mark.parametrize("dim,depth", [(3, 6), (4, 4)])
+def test_solve_matches_oracle_larger(dim, depth, oracle_lib):
+    from exteriorpoissonrecon_amd import Reconstructor, samples
+    if dim == 3:
+        pos, skew = samples.borromean(n=4096, seed=2)
+    else:
+        pos, skew = samples.clifford_torus(side=64, seed=2)
+    print(f"[gpu] oracle vs gpu dim={dim} depth={depth}", flush=True)
+    o = oracle_lib.Oracle(pos, skew, depth=depth)
+    so = o.solve()
+    r = Reconstructor(pos, skew, depth=depth).build()
+    E = r.solve()
+    assert rel_err(E, so["final_E"]) < TOL_SOLVE
+    for d in range(depth + 1):
+        hx, hy = r.coefficients(d)
+        scale = np.abs(so["final_x"]).max() + np.abs(so["final_y"]).max()
+        assert np.abs(hx - so[f"hx{d}"]).max() < TOL_SOLVE * scale
+        assert np.abs(hy - so[f"hy{d}"]).max() < TOL_SOLVE * scale
+
+
+def test_colour_phases_match_single_workgroup_path():
+    """The same level solved by the per-colour launches and by the one-workgroup
+    kernel gives the same iterates (colour independence, ExteriorPoisson.h:161-169)."""
+    from exteriorpoissonrecon_amd import Reconstructor, samples
+    pos, skew = samples.torus_knot(1024)
+    r = Reconstructor(pos, skew, depth=5).build()
+    rng = np.random.default_rng(0)
+    # level 4 (4913 nodes) runs in one workgroup, level 5 (35937) with per-colour launches;
+    # compare level 5 against the oracle instead of itself: one iteration from random data
+    for d in (4, 5):
+        N = r.num_functions(d)
+        x, y = rng.normal(size=N) * 0.05, rng.normal(size=N) * 0.05
+        r.set_coefficients(d, x, y)
+        r.gauss_seidel(d, 2)
+        gx, gy = r.coefficients(d)
+        from oracle.pyoracle import Oracle
+        o = Oracle(pos, skew, depth=5)
+        ox, oy = o.gs(d, x, y, 2)
+        assert rel_err(gx, ox) < 1e-9 and rel_err(gy, oy) < 1e-9, d
+
+
+def test_cli_tree_output(tmp_path, oracle_lib):
+    from exteriorpoissonrecon_amd import samples
+    g = load_golden("knot3_d4")
+    ply = str(tmp_path / "knot.ply")
+    samples.write_ply(ply, g["in_pos"], g["in_skew"])
+    cli = os.path.join(ROOT, "exteriorpoissonrecon_amd", "bin", "ExteriorPoissonRecon")
+    out = str(tmp_path / "o")
+    res = subprocess.run([cli, "--in", ply, "--out", out, "--depth", "4", "--aRadius", "-1",
+                          "--grid", str(tmp_path / "o.grid")], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0 and "[ERROR]" not in res.stderr, res.stderr
+    b = open(out + ".tree", "rb").read()
+    assert b[:12] == g["tree_head"].tobytes()          # dim + reference indexer hash
+    o = 12 + 8 * 16
+    hx = []
+    for pass_ in range(2):
+        nl = struct.unpack_from("<Q", b, o)[0]; o += 8
+        assert nl == 5
+        for d in range(nl):
+            n = struct.unpack_from("<q", b, o)[0]; o += 8
+            v = np.frombuffer(b, dtype="<f8", count=n, offset=o); o += 8 * n
+            if pass_ == 0:
+                hx.append(v)
+    assert struct.unpack_from("<I", b, o)[0] == 4 and o + 4 == len(b)
+    scale = max(np.abs(g["final_x"]).max(), 1.0)
+    for d in range(5):
+        assert np.abs(hx[d] - g[f"hx{d}"]).max() < TOL_SOLVE * scale
+    grid = open(str(tmp_path / "o.grid"), "rb").read()
+    assert grid.startswith(b"G3\n2 DOUBLE\n17 17 17\n")
