@@ -1,0 +1,172 @@
+"""Benchmark of the HIP ExteriorPoissonRecon solve path (one MI355X).
+
+A step is one whole reconstruction from samples resident on the device side of
+the C ABI: epr_build (estimator, splats, target field, R/B assembly, Galerkin
+chain) + epr_solve (the reference's cascadic paired Gauss-Seidel schedule with
+its random initial guess).  The metric is time-to-energy: the wall time of a
+step, which ends at the reference schedule's final energy; the step's final
+FP64 energy is checked against the reference's within 1e-6 relative.
+
+Workloads (synthetic samples, exteriorpoissonrecon_amd.samples):
+  borromean3d_d7  3D Borromean ellipses, 16384 samples + 10% spot noise, depth 7 (default)
+  clifford4d_d6   4D Clifford torus, 65536 samples, 2 deg angular noise, depth 6
+  knot3d_d5       3D (2,3) torus knot, 1024 samples, depth 5
+
+Prints one JSON line (stdout); progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "time-to-energy (1e-6 rel vs FP64) at depth 7 3D / 6 4D; GS sweep HBM GB/s vs peak"
+FP64_PEAK_TFLOPS = 78.6
+HBM_PEAK_GBS = 8000.0
+# Algorithmic FP64 flops of one node update (reference stencil formulation):
+# 5 per wedge entry + 8 per 1-ring R/K/B term + 20 for the 2x2 step.
+F_NODE = {3: 5 * 1358 + 8 * 27 + 20, 4: 5 * 22952 + 8 * 81 + 20}
+# Algorithmic HBM bytes of one node update: x,y read+write (32) + R,B columns.
+B_NODE = {3: 32 + 2 * 27 * 8, 4: 32 + 2 * 81 * 8}
+
+WORKLOADS = {
+    "borromean3d_d7": dict(dim=3, depth=7, gen=lambda s: s.borromean(16384)),
+    "clifford4d_d6": dict(dim=4, depth=6, gen=lambda s: s.clifford_torus(256, ang_deg=2.0)),
+    "knot3d_d5": dict(dim=3, depth=5, gen=lambda s: s.torus_knot(1024)),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def reference_energy(name):
+    path = os.path.join(ROOT, "tests", "golden", "bench_reference.json")
+    if not os.path.exists(path):
+        return None
+    return json.load(open(path)).get(name)
+
+
+def cpu_baseline(name, wl, pos, skew):
+    """The reference build (oracle/_ref/ExteriorPoissonRecon, compiled from the
+    reference sources) timed on this host's cores on the same samples."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ExteriorPoissonRecon")
+    from exteriorpoissonrecon_amd import samples
+    cores = os.cpu_count()
+    if not os.path.exists(exe):
+        return None
+    with tempfile.TemporaryDirectory() as td:
+        ply = os.path.join(td, "in.ply")
+        samples.write_ply(ply, pos, skew)
+        depth = wl["depth"]
+        # bounded sample: the full workload for 3D; for 4D depth 6 one level
+        # coarser (the CPU cost of the full 4D solve is many minutes)
+        if wl["dim"] == 4 and depth > 5:
+            depth = 5
+        t0 = time.perf_counter()
+        r = subprocess.run([exe, "--in", ply, "--depth", str(depth), "--aRadius", "-1"],
+                           capture_output=True, text=True, timeout=1800)
+        dt = time.perf_counter() - t0
+        if r.returncode != 0:
+            return None
+    sample = f"{name} samples, reference CLI --depth {depth} --aRadius -1 (full build+solve), {cores} threads"
+    return {"value": dt, "unit": "s", "cores": cores, "kind": "reference", "sample": sample}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="borromean3d_d7", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
+              "the solve path is single-device, see DESIGN.md for how it would shard", file=sys.stderr)
+        sys.exit(2)
+
+    from exteriorpoissonrecon_amd import Reconstructor, samples
+    wl = WORKLOADS[args.config]
+    pos, skew = wl["gen"](samples)
+    dim, depth = wl["dim"], wl["depth"]
+    log(f"[bench] {args.config}: {pos.shape[0]} samples, dim {dim}, depth {depth}")
+    rec = Reconstructor(pos, skew, depth=depth)
+
+    E = None
+    for i in range(args.warmup):
+        rec.build(); E = rec.solve()
+        log(f"[bench] warmup {i}: build {rec.seconds('build'):.3f}s solve {rec.seconds('solve'):.3f}s E={E.tolist()}")
+    times, builds, solves = [], [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        rec.build()
+        E = rec.solve()
+        dt = time.perf_counter() - t0
+        times.append(dt); builds.append(rec.seconds("build")); solves.append(rec.seconds("solve"))
+        log(f"[bench] step {i}: {dt:.4f}s (build {builds[-1]:.4f} solve {solves[-1]:.4f}) E={E.tolist()}")
+    t = float(np.mean(times))
+
+    Eref = reference_energy(args.config)
+    rel = None
+    if Eref is not None:
+        rel = abs(float(E.sum()) - float(sum(Eref))) / abs(float(sum(Eref)))
+        log(f"[bench] E={E.sum():.17g} reference {sum(Eref):.17g} rel {rel:.3e}")
+
+    # dominant kernel: one colour phase of the finest-level sweep
+    iters = int(np.ceil(10))
+    N = rec.num_functions(depth)
+    ms, launches = rec.time_gauss_seidel(depth, iters)
+    per_launch_s = ms * 1e-3 / launches
+    nodes_per_launch = N / (1 << dim)
+    flops_per_launch = F_NODE[dim] * nodes_per_launch
+    bytes_per_launch = B_NODE[dim] * nodes_per_launch
+    achieved = flops_per_launch / per_launch_s / 1e12
+    log(f"[bench] finest sweep: {ms / iters:.3f} ms/iteration, {launches} launches, "
+        f"{per_launch_s * 1e6:.1f} us/launch, {achieved:.2f} TFLOP/s algorithmic, "
+        f"{bytes_per_launch / per_launch_s / 1e9:.1f} GB/s algorithmic")
+    if args.detail:
+        for d in range(depth + 1):
+            it = int(np.ceil(2.0 ** (depth - d) * 10))
+            lms, ln = rec.time_gauss_seidel(d, it)
+            log(f"[bench]   level {d}: N={rec.num_functions(d)} iters={it} {lms:.3f} ms ({ln} launches)")
+
+    cpu = None if args.no_cpu else cpu_baseline(args.config, wl, pos, skew)
+    if cpu:
+        log(f"[bench] cpu baseline {cpu['value']:.3f}s ({cpu['sample']})")
+
+    out = {
+        "metric": METRIC,
+        "value": t,
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": args.config, "dim": dim, "depth": depth, "samples": int(pos.shape[0]),
+                   "grid_nodes": int(N), "build_s": float(np.mean(builds)), "solve_s": float(np.mean(solves)),
+                   "final_energy": float(E.sum()), "energy_rel_err_vs_reference": rel},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": f"KGsPhase<{dim}> (finest level, one colour)",
+                     "hbm_gbs_algorithmic": bytes_per_launch / per_launch_s / 1e9},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

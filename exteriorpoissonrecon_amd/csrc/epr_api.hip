@@ -59,6 +59,7 @@ struct CtxBase
 	virtual void build( void ) = 0;
 	virtual void init( int level , const double *x0 , const double *y0 ) = 0;
 	virtual void gs( int level , int iters ) = 0;
+	virtual void timeGs( int level , int iters , double *ms , int *launches ) = 0;
 	virtual void prolongAdd( int level ) = 0;
 	virtual void energies( int level , double out[2] ) = 0;
 	virtual void solve( double out[2] ) = 0;
@@ -387,6 +388,22 @@ struct Ctx : public CtxBase
 		HIPCHECK( hipGetLastError() );
 	}
 
+	void timeGs( int l , int iters , double *ms , int *launches )
+	{
+		needBuilt() ; checkLevel( l );
+		hipEvent_t e0 , e1;
+		HIPCHECK( hipEventCreate( &e0 ) ) ; HIPCHECK( hipEventCreate( &e1 ) );
+		HIPCHECK( hipEventRecord( e0 , stream ) );
+		gs( l , iters );
+		HIPCHECK( hipEventRecord( e1 , stream ) );
+		HIPCHECK( hipEventSynchronize( e1 ) );
+		float t = 0;
+		HIPCHECK( hipEventElapsedTime( &t , e0 , e1 ) );
+		HIPCHECK( hipEventDestroy( e0 ) ) ; HIPCHECK( hipEventDestroy( e1 ) );
+		if( ms ) *ms = t;
+		if( launches ) *launches = levels[l].N<=SmallLevel ? 1 : iters*C;
+	}
+
 	void prolong( int l , const double *in , double *out , int mode )
 	{
 		long long nf = levels[l+1].N;
@@ -708,6 +725,7 @@ int epr_set_samples( epr_ctx *ctx , size_t n , const double *pos , const double 
 int epr_build( epr_ctx *ctx ){ IMPL( ctx->impl->build() ); }
 int epr_init( epr_ctx *ctx , int level , const double *x0 , const double *y0 ){ IMPL( ctx->impl->init( level , x0 , y0 ) ); }
 int epr_gs_sweeps( epr_ctx *ctx , int level , int iters ){ IMPL( ctx->impl->gs( level , iters ) ; HIPCHECK( hipDeviceSynchronize() ) ); }
+int epr_time_gs_sweeps( epr_ctx *ctx , int level , int iters , double *ms , int *launches ){ IMPL( ctx->impl->timeGs( level , iters , ms , launches ) ); }
 int epr_prolong_add( epr_ctx *ctx , int level ){ IMPL( ctx->impl->prolongAdd( level ) ); }
 int epr_energies( epr_ctx *ctx , int level , double out[2] ){ IMPL( ctx->impl->energies( level , out ) ); }
 int epr_solve( epr_ctx *ctx , double out_E[2] ){ IMPL( ctx->impl->solve( out_E ) ); }

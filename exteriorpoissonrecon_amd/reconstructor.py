@@ -70,6 +70,12 @@ class Reconstructor:
     def gauss_seidel(self, level, iters):
         self._check(load().epr_gs_sweeps(self._h, level, iters))
 
+    def time_gauss_seidel(self, level, iters):
+        """(device milliseconds, kernel launches) of iters iterations at a level (HIP events)."""
+        ms = ctypes.c_double(); n = ctypes.c_int()
+        self._check(load().epr_time_gs_sweeps(self._h, level, iters, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def prolong_add(self, level):
         self._check(load().epr_prolong_add(self._h, level))
 

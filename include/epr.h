@@ -124,6 +124,11 @@ int  epr_write_tree( epr_ctx *ctx , const char *path , unsigned long long indexe
  * (Misha/RegularGrid.inl:440-471 layout, 2 DOUBLE values per node). */
 int  epr_write_grid( epr_ctx *ctx , const char *path );
 
+/* Measurement hook (no reference counterpart): runs iters Gauss-Seidel
+ * iterations at one level bracketed by HIP events on the context's stream and
+ * returns the elapsed device milliseconds and the number of kernel launches. */
+int  epr_time_gs_sweeps( epr_ctx *ctx , int level , int iters , double *ms , int *launches );
+
 /* Wall-clock seconds of the last epr_build / epr_solve (device-synchronised). */
 double epr_last_seconds( const epr_ctx *ctx , int which /* 0 build, 1 solve */ );
 
