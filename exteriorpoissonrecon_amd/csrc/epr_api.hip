@@ -79,7 +79,9 @@ struct Ctx : public CtxBase
 {
 	static constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD , NV = DimC< Dim >::NDNI , CoDim = 2;
 	// levels at or below this many nodes run all their iterations in one workgroup
-	static constexpr long long SmallLevel = Dim==3 ? 5000 : 700;
+	static constexpr long long SmallLevel = Dim==3 ? 800 : 700;
+	static constexpr size_t LdsLimit = 160*1024;   // gfx950 LDS per workgroup
+	bool ldsAttrSet = false;
 
 	epr_params prm;
 	int D;
@@ -99,11 +101,23 @@ struct Ctx : public CtxBase
 		long long N;
 		long long stride[Dim];
 		DevBuf< double > x , y , R , B , K;
+		hipGraphExec_t graph = nullptr;    // captured Gauss-Seidel launches of this level
+		int graphIters = 0;
+		const double *graphX = nullptr , *graphR = nullptr;
+		Level( void ){}
+		Level( Level &&o ) noexcept : res( o.res ) , N( o.N ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , graph( o.graph ) , graphIters( o.graphIters )
+		{
+			for( int d=0 ; d<Dim ; d++ ) stride[d] = o.stride[d];
+			o.graph = nullptr;
+		}
+		~Level(){ if( graph ) (void)hipGraphExecDestroy( graph ); }
+		// drops the captured launches when the buffers they point at move
+		void checkGraph( void ){ if( graph && ( graphX!=x.p || graphR!=R.p ) ){ (void)hipGraphExecDestroy( graph ) ; graph = nullptr; } }
 	};
 	std::vector< Level > levels;
 
 	// build-phase device data
-	DevBuf< double > dPos , dSkew , dCellP , dCellData , dMeasure , dNoise , dDepth , dRW , dSkewScaled , dS , dS2 , dFS , dScale , dPartials , dTmp , dTmp2 , dDepthScale;
+	DevBuf< double > dGalT , dPos , dSkew , dCellP , dCellData , dMeasure , dNoise , dDepth , dRW , dSkewScaled , dS , dS2 , dFS , dScale , dPartials , dTmp , dTmp2 , dDepthScale;
 	DevBuf< int > dCellIdx , dCellStart , dCellSamples , dCellMap;
 	std::vector< std::unique_ptr< DevBuf< double > > > estData , skData;
 	std::vector< std::unique_ptr< DevBuf< unsigned char > > > skKids , estKids;
@@ -272,18 +286,17 @@ struct Ctx : public CtxBase
 			std::vector< double > scales( D+1 );
 			for( int d=0 ; d<=D ; d++ ) scales[d] = scale / std::pow( std::pow( 0.5 , d ) , Dim-CoDim );
 			dDepthScale.upload( scales.data() , scales.size() , stream );
-			dCellP.alloc( (size_t)ncell*Dim ) , dCellData.alloc( (size_t)ncell*NV );
+			dCellP.ensure( (size_t)ncell*Dim ) , dCellData.ensure( (size_t)ncell*NV );
 			KCellStats< Dim ><<< Blocks( ncell , 128 ) , 128 , 0 , stream >>>( dPos.p , dCellStart.p , dCellSamples.p , ncell , dCellP.p , dCellData.p );
 			std::vector< long long > sz = treeSizes( NV , false );
-			estData.clear();
 			TreeGrids< Dim > T;
 			memset( &T , 0 , sizeof(T) );
 			T.maxDepth = D;
 			estData.resize( D+2 );
-			for( int t=1 ; t<=D+1 ; t++ ){ estData[t].reset( new DevBuf< double >() ) ; estData[t]->alloc( sz[t] ) ; estData[t]->zero( stream ) ; T.data[t] = estData[t]->p; }
+			for( int t=1 ; t<=D+1 ; t++ ){ if( !estData[t] ) estData[t].reset( new DevBuf< double >() ) ; estData[t]->ensure( sz[t] ) ; estData[t]->zero( stream ) ; T.data[t] = estData[t]->p; }
 			switch( K ){ case 0: launchEstSplat<0>( T ) ; break ; case 1: launchEstSplat<1>( T ) ; break ; case 2: launchEstSplat<2>( T ) ; break ; case 3: launchEstSplat<3>( T ) ; break ; default: launchEstSplat<4>( T ); }
 			HIPCHECK( hipGetLastError() );
-			dMeasure.alloc( n ) , dNoise.alloc( n ) , dDepth.alloc( n ) , dRW.alloc( n ) , dSkewScaled.alloc( n*CD );
+			dMeasure.ensure( n ) , dNoise.ensure( n ) , dDepth.ensure( n ) , dRW.ensure( n ) , dSkewScaled.ensure( n*CD );
 			KEstQuery< Dim ><<< Blocks( (long long)n , 128 ) , 128 , 0 , stream >>>( dPos.p , dSkew.p , (int)n , T , prm.nSigma , dMeasure.p , dNoise.p , dDepth.p , dRW.p , dSkewScaled.p );
 			HIPCHECK( hipGetLastError() );
 		}
@@ -291,22 +304,22 @@ struct Ctx : public CtxBase
 		{
 			const int K = prm.tRadius;
 			std::vector< long long > sz = treeSizes( CD , false ) , kz = treeSizes( 1 , true );
-			skData.clear() , skKids.clear();
 			skData.resize( D+2 ) , skKids.resize( D+2 );
 			TreeGrids< Dim > T;
 			memset( &T , 0 , sizeof(T) );
 			T.maxDepth = D;
 			for( int t=1 ; t<=D+1 ; t++ )
 			{
-				skData[t].reset( new DevBuf< double >() ) ; skData[t]->alloc( sz[t] ) ; skData[t]->zero( stream ) ; T.data[t] = skData[t]->p;
-				if( t<=D ){ skKids[t].reset( new DevBuf< unsigned char >() ) ; skKids[t]->alloc( kz[t] ) ; skKids[t]->zero( stream ) ; T.kids[t] = skKids[t]->p; }
+				if( !skData[t] ) skData[t].reset( new DevBuf< double >() );
+				skData[t]->ensure( sz[t] ) ; skData[t]->zero( stream ) ; T.data[t] = skData[t]->p;
+				if( t<=D ){ if( !skKids[t] ) skKids[t].reset( new DevBuf< unsigned char >() ) ; skKids[t]->ensure( kz[t] ) ; skKids[t]->zero( stream ) ; T.kids[t] = skKids[t]->p; }
 			}
 			if( D>=1 ) KTouch< Dim ><<< Blocks( (long long)ncell*D , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , T , K );
 			switch( K ){ case 0: launchSkewSplat<0>( T ) ; break ; case 1: launchSkewSplat<1>( T ) ; break ; case 2: launchSkewSplat<2>( T ) ; break ; case 3: launchSkewSplat<3>( T ) ; break ; default: launchSkewSplat<4>( T ); }
 			const double scl = std::pow( std::pow( 0.5 , D ) , 1.5 * CoDim );
 			for( int t=1 ; t<=D+1 ; t++ ) KScaleArray<<< Blocks( sz[t] , 256 ) , 256 , 0 , stream >>>( T.data[t] , sz[t] , scl );
 			long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= res;
-			dS.alloc( (size_t)ne*CD ) , dS2.alloc( (size_t)ne );
+			dS.ensure( (size_t)ne*CD ) , dS2.ensure( (size_t)ne );
 			KField< Dim ><<< Blocks( ne , 256 ) , 256 , 0 , stream >>>( T , dS.p , dS2.p );
 			HIPCHECK( hipGetLastError() );
 			// c = sum |S(E)|^2 / #elements (SystemEnergy.h:455-461)
@@ -323,11 +336,9 @@ struct Ctx : public CtxBase
 		// ---- finest R and B, then the Galerkin chain
 		{
 			Level &F = levels[D];
-			F.R.alloc( (size_t)F.N*W ) , F.B.alloc( (size_t)F.N*W );
+			F.R.ensure( (size_t)F.N*W ) , F.B.ensure( (size_t)F.N*W );
 			// B
-			std::vector< double > fs = hat::FullStiffnessTable< Dim >( res );
-			dFS.upload( fs.data() , fs.size() , stream );
-			KAssembleB< Dim ><<< Blocks( F.N*W , 256 ) , 256 , 0 , stream >>>( dS.p , dFS.p , res , F.B.p );
+			KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p );
 			// R
 			if( prm.sWeight>0 )
 			{
@@ -341,19 +352,22 @@ struct Ctx : public CtxBase
 				}
 				dCellMap.upload( cmap.data() , cmap.size() , stream );
 				const int nb = 1024;
-				dScale.alloc( 2 );
+				dScale.ensure( 2 );
 				KPartialSum<<< nb , 256 , 0 , stream >>>( dMeasure.p , (long long)n , dPartials.p );
 				KMeanMeasure<<< 1 , 1 , 0 , stream >>>( dPartials.p , nb , (long long)n , prm.sWeight , dScale.p );
 				KAssembleR< Dim ><<< Blocks( F.N*W , 256 ) , 256 , 0 , stream >>>( dPos.p , dCellMap.p , dCellStart.p , dCellSamples.p , dRW.p , res , dScale.p , F.R.p );
 			}
 			else F.R.zero( stream );
 			HIPCHECK( hipGetLastError() );
+			if( D>0 ) dGalT.ensure( (size_t)levels[D].N*W );
 			for( int l=D-1 ; l>=0 ; l-- )
 			{
 				Level &Cl = levels[l] , &Fl = levels[l+1];
-				Cl.R.alloc( (size_t)Cl.N*W ) , Cl.B.alloc( (size_t)Cl.N*W );
-				KGalerkin< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( Fl.R.p , Cl.R.p , Cl.res );
-				KGalerkin< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( Fl.B.p , Cl.B.p , Cl.res );
+				Cl.R.ensure( (size_t)Cl.N*W ) , Cl.B.ensure( (size_t)Cl.N*W );
+				KGalerkinT< Dim ><<< Blocks( Fl.N*W , 256 ) , 256 , 0 , stream >>>( Fl.R.p , dGalT.p , Cl.res );
+				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.R.p , Cl.res );
+				KGalerkinT< Dim ><<< Blocks( Fl.N*W , 256 ) , 256 , 0 , stream >>>( Fl.B.p , dGalT.p , Cl.res );
+				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.B.p , Cl.res );
 			}
 			HIPCHECK( hipGetLastError() );
 		}
@@ -363,8 +377,9 @@ struct Ctx : public CtxBase
 			Level &L = levels[l];
 			std::vector< double > k = hat::StiffnessCaseRows< Dim >( L.res );
 			L.K.upload( k.data() , k.size() , stream );
-			L.x.alloc( L.N ) , L.y.alloc( L.N );
+			L.x.ensure( L.N ) , L.y.ensure( L.N );
 			L.x.zero( stream ) , L.y.zero( stream );
+			L.checkGraph();
 		}
 		HIPCHECK( hipStreamSynchronize( stream ) );
 		built = true , solved = false;
@@ -379,11 +394,46 @@ struct Ctx : public CtxBase
 		needBuilt() ; checkLevel( l );
 		if( iters<=0 ) return;
 		LevelView< Dim > v = view( l );
-		if( levels[l].N<=SmallLevel ) KGsSmall< Dim ><<< 1 , 512 , 0 , stream >>>( v , iters );
-		else for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+		const long long N = levels[l].N;
+		if( N<=SmallLevel )
 		{
-			ColourSet< Dim > cs ; cs.set( levels[l].res , c );
-			KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs );
+			// one workgroup sized to the largest colour, coefficients in LDS (and R, B when they fit in 64 KiB)
+			long long maxColour = 0;
+			for( int c=0 ; c<C ; c++ ){ ColourSet< Dim > cs ; cs.set( levels[l].res , c ) ; maxColour = std::max( maxColour , cs.total ); }
+			int threads = (int)std::min< long long >( Dim==3 ? 1024 : 512 , ( maxColour*C + 63 )/64*64 );
+			size_t lds = ( 2*N + W*W )*sizeof(double);
+			int rb = lds + 2*N*W*sizeof(double) <= LdsLimit ? 1 : 0;
+			if( rb ) lds += 2*N*W*sizeof(double);
+			if( lds<=LdsLimit )
+			{
+				if( !ldsAttrSet ){ HIPCHECK( hipFuncSetAttribute( (const void*)KGsSmallLds< Dim > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) ) ; ldsAttrSet = true; }
+				KGsSmallLds< Dim ><<< 1 , threads , lds , stream >>>( v , iters , rb );
+			}
+			else KGsSmall< Dim ><<< 1 , 512 , 0 , stream >>>( v , iters );
+		}
+		else
+		{
+			// the iters*2^Dim colour launches of a level are captured once into a
+			// HIP graph and replayed (the level's buffers are stable between builds
+			// of the same size)
+			Level &L = levels[l];
+			if( !L.graph || L.graphIters!=iters )
+			{
+				if( L.graph ){ (void)hipGraphExecDestroy( L.graph ) ; L.graph = nullptr; }
+				hipGraph_t g;
+				HIPCHECK( hipStreamBeginCapture( stream , hipStreamCaptureModeThreadLocal ) );
+				for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+				{
+					ColourSet< Dim > cs ; cs.set( L.res , c );
+					KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs );
+				}
+				HIPCHECK( hipStreamEndCapture( stream , &g ) );
+				HIPCHECK( hipGraphInstantiate( &L.graph , g , nullptr , nullptr , 0 ) );
+				HIPCHECK( hipGraphDestroy( g ) );
+				L.graphIters = iters;
+				L.graphX = L.x.p , L.graphR = L.R.p;
+			}
+			HIPCHECK( hipGraphLaunch( L.graph , stream ) );
 		}
 		HIPCHECK( hipGetLastError() );
 	}

@@ -23,7 +23,7 @@ template<> struct DimC< 4 > { static constexpr int W = 81 , C = 16 , CD = 6 , ND
 
 constexpr int MAXT = 16;     // tree depths supported (depth <= 14)
 
-template< int Dim > __device__ __host__ __forceinline__ int Bit( int c , int d ){ return ( c>>( Dim-1-d ) ) & 1; }
+template< int Dim > __device__ __host__ constexpr int Bit( int c , int d ){ return ( c>>( Dim-1-d ) ) & 1; }
 
 template< int Dim > __device__ __forceinline__ double WT( int j , int p ){ if constexpr( Dim==3 ) return wedge3::WT[j][p]; else return wedge4::WT[j][p]; }
 template< int Dim > __device__ __forceinline__ double WA( int j , int k ){ if constexpr( Dim==3 ) return wedge3::WA[j][k]; else return wedge4::WA[j][k]; }
@@ -135,12 +135,41 @@ __device__ __forceinline__ bool NodeStep( double q20 , double q02 , double q11 ,
 // Processes node i (multi-index F) with the C lanes of its group: lane = one
 // incident element (wedge terms) + a share of the 3^Dim window (R, K, B terms).
 // All lanes return the group totals.
-template< int Dim , int MODE >
+template< int Dim , int MODE , bool PREFETCH = true >
 __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int *F , long long i , int lane , double acc[5] )
 {
 	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W;
 	double q20 = 0 , q02 = 0 , q11 = 0 , q10 = 0 , q01 = 0;
 	double eW = 0 , eB = 0 , eR = 0 , eK = 0;
+	// ---- the lane's share of the 3^Dim window: issue its loads first
+	constexpr int NWL = ( W + C - 1 ) / C;
+	int cse = 0;
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ) cse = cse*3 + ( F[d]<=0 ? 0 : ( F[d]>=L.res ? 2 : 1 ) );
+	double wx[NWL] , wy[NWL] , wR[NWL] , wB[NWL] , wK[NWL];
+#define EPR_LOAD_WINDOW( m ) \
+	{ \
+		const int w_ = lane + (m)*C; \
+		int t_ = w_; \
+		long long r_ = i; \
+		bool in_ = w_<W; \
+		_Pragma( "unroll" ) \
+		for( int d=Dim-1 ; d>=0 ; d-- ) \
+		{ \
+			const int o_ = t_%3 - 1 ; t_ /= 3; \
+			const int g_ = F[d]+o_; \
+			in_ = in_ && g_>=0 && g_<=L.res; \
+			r_ += o_*L.stride[d]; \
+		} \
+		if( in_ ) wx[m] = L.x[r_] , wy[m] = L.y[r_] , wR[m] = L.R[ i*W+w_ ] , wB[m] = L.B[ i*W+w_ ] , wK[m] = L.K[ cse*W + w_ ]; \
+		else      wx[m] = wy[m] = wR[m] = wB[m] = wK[m] = 0; \
+	}
+	if constexpr( PREFETCH )
+	{
+#pragma unroll
+		for( int m=0 ; m<NWL ; m++ ) EPR_LOAD_WINDOW( m )
+	}
+	const double xi = L.x[i] , yi = L.y[i];
 	// ---- the incident element of this lane
 	{
 		bool valid = true;
@@ -170,27 +199,12 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int
 			if constexpr( MODE==MODE_ENERGY ) eW = xp[0]*zy*L.sW;
 		}
 	}
-	// ---- the lane's share of the 3^Dim window
-	int cse = 0;
 #pragma unroll
-	for( int d=0 ; d<Dim ; d++ ) cse = cse*3 + ( F[d]<=0 ? 0 : ( F[d]>=L.res ? 2 : 1 ) );
-	const double xi = L.x[i] , yi = L.y[i];
-	for( int w=lane ; w<W ; w+=C )
+	for( int m=0 ; m<NWL ; m++ )
 	{
-		int t = w;
-		long long r = i;
-		bool in = true;
-#pragma unroll
-		for( int d=Dim-1 ; d>=0 ; d-- )
-		{
-			const int o = t%3 - 1 ; t /= 3;
-			const int g = F[d]+o;
-			in = in && g>=0 && g<=L.res;
-			r += o*L.stride[d];
-		}
-		if( !in ) continue;
-		const double xr = L.x[r] , yr = L.y[r];
-		const double Rv = L.R[ i*W+w ] , Bv = L.B[ i*W+w ] , Kv = L.K[ (long long)cse*W + w ];
+		if constexpr( !PREFETCH ) EPR_LOAD_WINDOW( m )
+		const int w = lane + m*C;
+		const double xr = wx[m] , yr = wy[m] , Rv = wR[m] , Bv = wB[m] , Kv = wK[m];
 		if constexpr( MODE==MODE_ENERGY )
 		{
 			eR += xi*Rv*xr + yi*Rv*yr;
@@ -206,6 +220,7 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int
 			if( w!=W/2 ){ q10 -= 2.*Bv*yr ; q01 += 2.*Bv*xr; }
 		}
 	}
+#undef EPR_LOAD_WINDOW
 	if constexpr( MODE==MODE_ENERGY ){ q20 = eW , q02 = eB , q11 = eR , q10 = eK , q01 = 0; }
 	// ---- group reduction (xor butterfly inside the C-lane group)
 #pragma unroll
@@ -236,10 +251,11 @@ struct ColourSet
 			total *= cnt[d];
 		}
 	}
-	__device__ __forceinline__ void node( long long k , int *F ) const
+	__device__ __forceinline__ void node( long long k64 , int *F ) const
 	{
+		unsigned int k = (unsigned int)k64;   // colour sets hold fewer than 2^32 nodes
 #pragma unroll
-		for( int d=Dim-1 ; d>=0 ; d-- ){ const int kd = (int)( k % cnt[d] ) ; k /= cnt[d] ; F[d] = 2*kd + par[d]; }
+		for( int d=Dim-1 ; d>=0 ; d-- ){ const unsigned int c = (unsigned int)cnt[d] , q = k / c ; F[d] = 2*(int)( k - q*c ) + par[d] ; k = q; }
 	}
 };
 
@@ -259,7 +275,7 @@ __global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSe
 	cs.node( k , F );
 	const long long i = FlatIndex< Dim >( F , L.stride );
 	double q[5];
-	NodeGroup< Dim , MODE_GS >( L , F , i , lane , q );
+	NodeGroup< Dim , MODE_GS , Dim==3 >( L , F , i , lane , q );
 	if( lane==0 )
 	{
 		double dx , dy;
@@ -306,7 +322,7 @@ __global__ void __launch_bounds__( 256 ) KQFit( LevelView< Dim > L , double *q )
 	int F[Dim];
 	Unflatten< Dim >( i , L.res , F );
 	double a[5];
-	NodeGroup< Dim , MODE_QFIT >( L , F , i , lane , a );
+	NodeGroup< Dim , MODE_QFIT , Dim==3 >( L , F , i , lane , a );
 	if( lane==0 ) for( int k=0 ; k<5 ; k++ ) q[5*i+k] = a[k];
 }
 
@@ -325,7 +341,7 @@ __global__ void __launch_bounds__( 256 ) KEnergy( LevelView< Dim > L , double *p
 	{
 		int F[Dim];
 		Unflatten< Dim >( i , L.res , F );
-		NodeGroup< Dim , MODE_ENERGY >( L , F , i , lane , a );
+		NodeGroup< Dim , MODE_ENERGY , Dim==3 >( L , F , i , lane , a );
 	}
 	if( lane==0 ) for( int k=0 ; k<4 ; k++ ) sh[ threadIdx.x/C ][k] = a[k];
 	__syncthreads();
@@ -949,6 +965,227 @@ __global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const i
 	if( !any ){ R[g] = 0 ; return; }
 	const double v = diag ? acc+acc : acc;
 	R[g] = v * scale[0] * scale[1];
+}
+
+// ---- B assembly, one thread per finest node f1 and all 3^Dim outputs of its
+// column (Hat.scalars.inl:1637-1686).  The 2^Dim incident elements' fields are
+// loaded once; every output (f2,f1) sums its common elements in lexicographic
+// order, each element's <S(E), FS[i1][i2]> in (a,b) order without contraction,
+// with the unit-resolution table FS scaled by res^(2-Dim) (an exact power of
+// two), so the values equal the reference's.
+template< int Dim > struct FSTab;
+template<> struct FSTab< 3 >{ static constexpr double v( int i1 , int i2 , int a , int b ){ return wedge3::FS[i1][i2][a][b]; } };
+template<> struct FSTab< 4 >{ static constexpr double v( int i1 , int i2 , int a , int b ){ return wedge4::FS[i1][i2][a][b]; } };
+
+template< int Dim > constexpr int WOff( int w , int d ){ for( int k=Dim-1 ; k>d ; k-- ) w /= 3 ; return w%3 - 1; }
+template< int Dim > constexpr int PackedIndex( int a , int b ){ int idx = 0 ; for( int i=0 ; i<Dim ; i++ ) for( int j=0 ; j<i ; j++ , idx++ ) if( i==a && j==b ) return idx ; return -1; }
+
+// dot of FS[i1][i2] (elements incident to f1 indexed by e-bits, i1/i2 relative corners) with S(E)
+template< int Dim , int i1 , int i2 , int ab >
+__device__ __forceinline__ void FSDotTerm( const double *s , double &dd )
+{
+	constexpr int a = ab / Dim , b = ab % Dim;
+	if constexpr( a!=b && FSTab< Dim >::v( i1 , i2 , a , b )!=0.0 )
+	{
+		constexpr double st = FSTab< Dim >::v( i1 , i2 , a , b );
+		// M(a,b) = p for a>b, -p for a<b
+		const double m = a>b ? s[ PackedIndex< Dim >( a , b ) ] : -s[ PackedIndex< Dim >( b , a ) ];
+		dd = __dadd_rn( dd , __dmul_rn( st , m ) );
+	}
+}
+template< int Dim , int i1 , int i2 , int ... abs >
+__device__ __forceinline__ double FSDot( const double *s , std::integer_sequence< int , abs... > )
+{
+	double dd = 0;
+	( FSDotTerm< Dim , i1 , i2 , abs >( s , dd ) , ... );
+	return dd;
+}
+// output w of node f1: elements b (bits, coordinate 0 most significant) with E = F1-1+b common to f1 and f2 = f1+off(w)
+template< int Dim , int w , int b >
+__device__ __forceinline__ void BElement( const double (&S)[1<<Dim][DimC< Dim >::CD] , double &dot )
+{
+	constexpr bool common = [](){ for( int d=0 ; d<Dim ; d++ ){ const int e = -1 + Bit< Dim >( b , d ) , o = WOff< Dim >( w , d ) ; if( o<e || o>e+1 ) return false; } return true; }();
+	if constexpr( common )
+	{
+		constexpr int i1 = [](){ int i = 0 ; for( int d=0 ; d<Dim ; d++ ) i = i*2 + ( 1 - Bit< Dim >( b , d ) ) ; return i; }();
+		constexpr int i2 = [](){ int i = 0 ; for( int d=0 ; d<Dim ; d++ ) i = i*2 + ( WOff< Dim >( w , d ) + 1 - Bit< Dim >( b , d ) ) ; return i; }();
+		dot = __dadd_rn( dot , FSDot< Dim , i1 , i2 >( S[b] , std::make_integer_sequence< int , Dim*Dim >{} ) );
+	}
+}
+template< int Dim , int w , int ... bs >
+__device__ __forceinline__ double BOutput( const double (&S)[1<<Dim][DimC< Dim >::CD] , std::integer_sequence< int , bs... > )
+{
+	double dot = 0;
+	( BElement< Dim , w , bs >( S , dot ) , ... );
+	return dot;
+}
+template< int Dim , int ... ws >
+__device__ __forceinline__ void BOutputs( const double (&S)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , double *col , std::integer_sequence< int , ws... > )
+{
+	( [&]{
+		bool in = true;
+		for( int d=0 ; d<Dim ; d++ ){ const int g = F[d] + WOff< Dim >( ws , d ) ; in = in && g>=0 && g<=res; }
+		col[ws] = in ? __dmul_rn( BOutput< Dim , ws >( S , std::make_integer_sequence< int , 1<<Dim >{} ) , scale ) : 0.;
+	}() , ... );
+}
+
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int res , double scale , double *B )
+{
+	constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD;
+	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= res+1;
+	const long long f1 = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( f1>=nf ) return;
+	int F[Dim];
+	Unflatten< Dim >( f1 , res , F );
+	double Sv[C][CD];
+#pragma unroll
+	for( int b=0 ; b<C ; b++ )
+	{
+		bool in = true;
+		long long e = 0;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ const int E = F[d]-1+Bit< Dim >( b , d ) ; in = in && E>=0 && E<res ; e = e*res + E; }
+#pragma unroll
+		for( int c=0 ; c<CD ; c++ ) Sv[b][c] = in ? S[ e*CD + c ] : 0.;
+	}
+	BOutputs< Dim >( Sv , F , res , scale , B + f1*W , std::make_integer_sequence< int , W >{} );
+}
+
+// ---- Galerkin coarsening in two passes (same accumulation order as KGalerkin):
+// T(i,k) = sum over fine m increasing of P(m,i) Mf(m,k), stored per fine column
+// k for the 3^Dim coarse i = (k>>1)+o; then Mc(i,j) = sum over children k of j
+// increasing of T(i,k) P(k,j).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGalerkinT( const double *Mf , double *T , int cres )
+{
+	constexpr int W = DimC< Dim >::W;
+	const int fres = 2*cres;
+	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= fres+1;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=nf*W ) return;
+	const long long k = g / W;
+	const int wo = (int)( g % W );
+	int K[Dim] , I[Dim] , lo[Dim] , hi[Dim] , M[Dim];
+	Unflatten< Dim >( k , fres , K );
+	{
+		int t = wo;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ I[d] = ( K[d]>>1 ) + t%3 - 1 ; t /= 3; }
+	}
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		if( I[d]<0 || I[d]>cres ){ T[g] = 0 ; return; }
+		lo[d] = max( max( 2*I[d]-1 , K[d]-1 ) , 0 );
+		hi[d] = min( min( 2*I[d]+1 , K[d]+1 ) , fres );
+		if( lo[d]>hi[d] ){ T[g] = 0 ; return; }
+		M[d] = lo[d];
+	}
+	double t = 0;
+	bool any = false;
+	const double *col = Mf + k*W;
+	while( true )
+	{
+		double pmi = 1;
+		int wm = 0;
+		for( int d=0 ; d<Dim ; d++ ){ if( M[d]!=2*I[d] ) pmi *= 0.5 ; wm = wm*3 + ( M[d]-K[d]+1 ); }
+		const double mv = col[wm];
+		if( mv!=0 ){ const double pr = __dmul_rn( pmi , mv ) ; t = any ? __dadd_rn( t , pr ) : pr ; any = true; }
+		int d = Dim-1;
+		for( ; d>=0 ; d-- ){ if( ++M[d]<=hi[d] ) break ; M[d] = lo[d]; }
+		if( d<0 ) break;
+	}
+	T[g] = t;
+}
+
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *Mc , int cres )
+{
+	constexpr int W = DimC< Dim >::W;
+	const int fres = 2*cres;
+	long long nc = 1 ; for( int d=0 ; d<Dim ; d++ ) nc *= cres+1;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=nc*W ) return;
+	const long long j = g / W;
+	const int wi = (int)( g % W );
+	int J[Dim] , I[Dim] , lo[Dim] , hi[Dim] , K[Dim];
+	Unflatten< Dim >( j , cres , J );
+	{
+		int t = wi;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ I[d] = J[d] + t%3 - 1 ; t /= 3; }
+	}
+	long long fstride[Dim];
+	{
+		long long s = 1;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ fstride[d] = s ; s *= fres+1; }
+	}
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		if( I[d]<0 || I[d]>cres ){ Mc[g] = 0 ; return; }
+		lo[d] = max( max( 2*J[d]-1 , 2*I[d]-2 ) , 0 );
+		hi[d] = min( min( 2*J[d]+1 , 2*I[d]+2 ) , fres );
+		if( lo[d]>hi[d] ){ Mc[g] = 0 ; return; }
+		K[d] = lo[d];
+	}
+	double acc = 0;
+	bool any = false;
+	while( true )
+	{
+		double pkj = 1;
+		long long k = 0;
+		int wo = 0;
+		for( int d=0 ; d<Dim ; d++ ){ if( K[d]!=2*J[d] ) pkj *= 0.5 ; k += K[d]*fstride[d] ; wo = wo*3 + ( I[d] - ( K[d]>>1 ) + 1 ); }
+		const double t = T[ k*W + wo ];
+		if( t!=0 ){ const double pr = __dmul_rn( t , pkj ) ; acc = any ? __dadd_rn( acc , pr ) : pr ; any = true; }
+		int d = Dim-1;
+		for( ; d>=0 ; d-- ){ if( ++K[d]<=hi[d] ) break ; K[d] = lo[d]; }
+		if( d<0 ) break;
+	}
+	Mc[g] = acc;
+}
+
+// All iterations of a small level in one workgroup with the coefficients (and,
+// when they fit, the R and B columns) resident in LDS; barrier between colours.
+template< int Dim >
+__global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLds( LevelView< Dim > L , int iters , int rbInLds )
+{
+	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W;
+	extern __shared__ double sm[];
+	double *sx = sm , *sy = sm + L.N;
+	for( long long i=threadIdx.x ; i<L.N ; i+=blockDim.x ) sx[i] = L.x[i] , sy[i] = L.y[i];
+	LevelView< Dim > Ls = L;
+	Ls.x = sx , Ls.y = sy;
+	{
+		double *sK = sm + 2*L.N;
+		for( int i=threadIdx.x ; i<W*W ; i+=blockDim.x ) sK[i] = L.K[i];
+		Ls.K = sK;
+	}
+	if( rbInLds )
+	{
+		double *sR = sm + 2*L.N + W*W , *sB = sR + L.N*W;
+		for( long long i=threadIdx.x ; i<L.N*W ; i+=blockDim.x ) sR[i] = L.R[i] , sB[i] = L.B[i];
+		Ls.R = sR , Ls.B = sB;
+	}
+	__syncthreads();
+	for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+	{
+		ColourSet< Dim > cs ; cs.set( L.res , c );
+		for( long long g=threadIdx.x ; g<cs.total*C ; g+=blockDim.x )
+		{
+			const long long k = g / C;
+			const int lane = (int)( g % C );
+			int F[Dim];
+			cs.node( k , F );
+			const long long i = FlatIndex< Dim >( F , L.stride );
+			double q[5];
+			NodeGroup< Dim , MODE_GS >( Ls , F , i , lane , q );
+			if( lane==0 )
+			{
+				double dx , dy;
+				if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) sx[i] += dx , sy[i] += dy;
+			}
+		}
+		__syncthreads();
+	}
+	for( long long i=threadIdx.x ; i<L.N ; i+=blockDim.x ) L.x[i] = sx[i] , L.y[i] = sy[i];
 }
 
 // mean of the per-sample measures (deterministic): out[1] = sum / n, out[0] = sWeight

@@ -17,6 +17,15 @@ Gauss-Seidel fit of a node only needs the tables of corner 0:
     WT[j][kl] = M(0, j, k, l)      j in [1,2^Dim), k<l   (cubic terms)
     WA[j][k]  = M(0, j, 0, k)      j,k in [1,2^Dim)      (quadratic terms)
 
+It also emits the full-stiffness element table used to assemble B
+(Include/Hat.scalars.inl:550-565) at unit resolution,
+
+    FS[i1][i2][a][b] = int d_b phi_i1 * d_a phi_i2,
+
+evaluated with the reference's floating-point operation order
+(Include/Hat.inl:188-194, ElementFunction::integral), so that the table
+scaled by r^(2-Dim) (exact: r is a power of two) equals the reference's.
+
 Run: python gen_wedge_tables.py > epr_wedge_tables.h
 """
 from fractions import Fraction
@@ -82,6 +91,50 @@ def tables(dim):
     return kl, WT, WA
 
 
+def basis_integral_float(p, q):
+    """_Basis::Integral in double precision, Include/Hat.inl:121-131."""
+    def prod(first, last):
+        f = 1.0
+        for i in range(first, last + 1):
+            f *= i
+        return f
+    if p > q:
+        return prod(1, q) / prod(p + 1, p + q + 1)
+    return prod(1, p) / prod(q + 1, p + q + 1)
+
+
+def full_stiffness_float(dim):
+    C = 1 << dim
+
+    def bit(c, d):
+        return (c >> (dim - 1 - d)) & 1
+
+    def deriv(c, a):
+        # (coefficient, p[d], q[d], dd[d]) of d/dt_a of the hat of corner c
+        p = [1 if bit(c, d) else 0 for d in range(dim)]
+        q = [0 if bit(c, d) else 1 for d in range(dim)]
+        dd = [0] * dim
+        coef = 1.0
+        if p[a]:
+            coef *= p[a]; p[a] -= 1; dd[a] = 1
+        elif q[a]:
+            coef *= -q[a]; q[a] -= 1; dd[a] = 1
+        return coef, p, q, dd
+
+    fs = [[[[0.0] * dim for _ in range(dim)] for _ in range(C)] for _ in range(C)]
+    for i1 in range(C):
+        for i2 in range(C):
+            for d1 in range(dim):
+                for d2 in range(dim):
+                    c1, p1, q1, e1 = deriv(i1, d1)
+                    c2, p2, q2, e2 = deriv(i2, d2)
+                    I = c1 * c2
+                    for d in range(dim):
+                        I *= basis_integral_float(p1[d] + p2[d], q1[d] + q2[d]) * 1.0 / 1.0
+                    fs[i1][i2][d2][d1] = 0.0 + I
+    return fs
+
+
 def fmt(v):
     return repr(float(v))
 
@@ -107,6 +160,11 @@ def main():
         out.append(f"constexpr double WA[{C - 1}][{C - 1}] = {{")
         for row in WA:
             out.append("  {" + ", ".join(fmt(v) for v in row) + "},")
+        out.append("};")
+        fs = full_stiffness_float(dim)
+        out.append(f"constexpr double FS[{C}][{C}][{dim}][{dim}] = {{")
+        for i1 in range(C):
+            out.append("  {" + ", ".join("{" + ", ".join("{" + ", ".join(repr(v) for v in row) + "}" for row in fs[i1][i2]) + "}" for i2 in range(C)) + "},")
         out.append("};")
         out.append(f"}} // namespace wedge{dim}")
     print("\n".join(out))
