@@ -30,9 +30,16 @@ sys.path.insert(0, ROOT)
 METRIC = "time-to-energy (1e-6 rel vs FP64) at depth 7 3D / 6 4D; GS sweep HBM GB/s vs peak"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-# Algorithmic FP64 flops of one node update (reference stencil formulation):
-# 5 per wedge entry + 8 per 1-ring R/K/B term + 20 for the 2x2 step.
-F_NODE = {3: 5 * 1358 + 8 * 27 + 20, 4: 5 * 22952 + 8 * 81 + 20}
+# Algorithmic FP64 flops of one node update in this build's element-local
+# formulation: per incident element 3 per corner pair (wedge products), 2 per
+# nonzero of the cubic table (130 / 1481), 4 per nonzero of the quadratic table
+# (37 / 225), 20 per table row for the dot products; 12 per window term; 20 for
+# the 2x2 step.
+F_NODE = {3: 8 * (28 * 3 + 130 * 2 + 37 * 4 + 7 * 10) + 27 * 12 + 20,
+          4: 16 * (120 * 3 + 1481 * 2 + 225 * 4 + 15 * 10) + 81 * 12 + 20}
+# The reference's stencil formulation does the same update in 5 flops per wedge
+# stencil entry (1358 / 22952 entries) + 8 per window term + 20.
+F_NODE_REF = {3: 5 * 1358 + 8 * 27 + 20, 4: 5 * 22952 + 8 * 81 + 20}
 # Algorithmic HBM bytes of one node update: x,y read+write (32) + R,B columns.
 B_NODE = {3: 32 + 2 * 27 * 8, 4: 32 + 2 * 81 * 8}
 
@@ -130,8 +137,10 @@ def main():
     flops_per_launch = F_NODE[dim] * nodes_per_launch
     bytes_per_launch = B_NODE[dim] * nodes_per_launch
     achieved = flops_per_launch / per_launch_s / 1e12
+    achieved_ref = F_NODE_REF[dim] * nodes_per_launch / per_launch_s / 1e12
     log(f"[bench] finest sweep: {ms / iters:.3f} ms/iteration, {launches} launches, "
-        f"{per_launch_s * 1e6:.1f} us/launch, {achieved:.2f} TFLOP/s algorithmic, "
+        f"{per_launch_s * 1e6:.1f} us/launch, {achieved:.2f} TFLOP/s algorithmic "
+        f"({achieved_ref:.2f} reference-equivalent), "
         f"{bytes_per_launch / per_launch_s / 1e9:.1f} GB/s algorithmic")
     if args.detail:
         for d in range(depth + 1):
@@ -162,7 +171,8 @@ def main():
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
                      "kernel": f"KGsPhase<{dim}> (finest level, one colour)",
-                     "hbm_gbs_algorithmic": bytes_per_launch / per_launch_s / 1e9},
+                     "hbm_gbs_algorithmic": bytes_per_launch / per_launch_s / 1e9,
+                     "flops_per_node": F_NODE[dim], "reference_equivalent_tflops": achieved_ref},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
