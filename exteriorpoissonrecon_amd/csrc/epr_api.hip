@@ -81,7 +81,9 @@ struct Ctx : public CtxBase
 	// levels at or below this many nodes run all their iterations in one workgroup
 	static constexpr long long SmallLevel = Dim==3 ? 800 : 700;
 	static constexpr size_t LdsLimit = 160*1024;   // gfx950 LDS per workgroup
-	bool ldsAttrSet = false;
+	bool ldsAttrSet = false , tinyAttrSet = false;
+	// levels with at most this many elements use KGsTiny (EPR_TINY_ELEMENTS overrides, for tuning)
+	long long tinyElements = getenv( "EPR_TINY_ELEMENTS" ) ? atoll( getenv( "EPR_TINY_ELEMENTS" ) ) : ( Dim==3 ? 8 : 16 );
 
 	epr_params prm;
 	int D;
@@ -395,11 +397,28 @@ struct Ctx : public CtxBase
 		if( iters<=0 ) return;
 		LevelView< Dim > v = view( l );
 		const long long N = levels[l].N;
-		if( N<=SmallLevel )
+		long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= levels[l].res;
+		long long maxColour = 0;
+		for( int c=0 ; c<C ; c++ ){ ColourSet< Dim > cs ; cs.set( levels[l].res , c ) ; maxColour = std::max( maxColour , cs.total ); }
+		// tiny levels: one wavefront per element, a 16-lane row per node, all nodes of a colour at once
+		const int maxWaves = TinyMaxThreads< Dim >()/64;
+		const int tinyWaves = (int)std::min< long long >( maxWaves , std::max( ne , ( maxColour+3 )/4 ) );
+		const bool tinyRB = TinyLds< Dim >::Bytes( N , ne , tinyWaves , true ) <= LdsLimit;
+		if( ne<=tinyElements && maxColour<=4*maxWaves && TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB )<=LdsLimit )
 		{
-			// one workgroup sized to the largest colour, coefficients in LDS (and R, B when they fit in 64 KiB)
-			long long maxColour = 0;
-			for( int c=0 ; c<C ; c++ ){ ColourSet< Dim > cs ; cs.set( levels[l].res , c ) ; maxColour = std::max( maxColour , cs.total ); }
+			const size_t lds = TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB );
+			if( !tinyAttrSet )
+			{
+				HIPCHECK( hipFuncSetAttribute( (const void*)KGsTiny< Dim , true > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) );
+				HIPCHECK( hipFuncSetAttribute( (const void*)KGsTiny< Dim , false > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) );
+				tinyAttrSet = true;
+			}
+			if( tinyRB ) KGsTiny< Dim , true  ><<< 1 , 64*tinyWaves , lds , stream >>>( v , iters );
+			else         KGsTiny< Dim , false ><<< 1 , 64*tinyWaves , lds , stream >>>( v , iters );
+		}
+		else if( N<=SmallLevel )
+		{
+			// one workgroup sized to the largest colour, coefficients in LDS (and R, B when they fit)
 			int threads = (int)std::min< long long >( Dim==3 ? 1024 : 512 , ( maxColour*C + 63 )/64*64 );
 			size_t lds = ( 2*N + W*W )*sizeof(double);
 			int rb = lds + 2*N*W*sizeof(double) <= LdsLimit ? 1 : 0;

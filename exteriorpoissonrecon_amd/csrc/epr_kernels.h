@@ -1188,6 +1188,305 @@ __global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLds( LevelView<
 	for( long long i=threadIdx.x ; i<L.N ; i+=blockDim.x ) L.x[i] = sx[i] , L.y[i] = sy[i];
 }
 
+// ---- Tiny levels (a few elements per colour phase), one workgroup for all
+// iterations.  Every element has exactly one corner of each colour, so a colour
+// phase is: element stage (each element's wedge terms for the node at its
+// corner of that colour, one wavefront per element) -> barrier -> node stage
+// (a 16-lane row per node: its elements' contributions + its 3^Dim window
+// terms + the step) -> barrier.  Coefficients, the Dirichlet rows (and R/B when
+// they fit) and a colour-sorted node table live in LDS for the whole launch;
+// all reductions are DPP / permlane-swap butterflies (no LDS round trips).
+//
+// Element stage, per element: lanes 0..2C-1 fetch the corner values relabelled
+// so that the node is corner 0; lane (j, ch) contracts row j of the cubic table
+// with pair chunk ch (its table slice is held in registers) over the pair
+// wedges w_kl = x_k y_l - x_l y_k; the quadratic table rows and the five sums
+// over j finish on lanes ch = 0, 1, 2.
+template< int Dim > struct TinyShape;
+template<> struct TinyShape< 3 >{ static constexpr int NCH = 8 , QC = 4; };    // 7 rows x 8 chunks of 4 pairs (28 pairs padded to 32)
+template<> struct TinyShape< 4 >{ static constexpr int NCH = 4 , QC = 30; };   // 15 rows x 4 chunks of 30 pairs
+template< int Dim > constexpr int TinyMaxThreads( void ){ return Dim==3 ? 1024 : 512; }   // 4D: 256 VGPRs for the 60 of the table slice
+
+// v + v[lane'] for the DPP lane pattern CTRL (the whole wavefront must be active)
+template< int CTRL > __device__ __forceinline__ double DppAdd( double v ){ return v + __builtin_amdgcn_update_dpp( 0.0 , v , CTRL , 0xF , 0xF , false ); }
+enum { DPP_XOR1 = 0xB1 , DPP_XOR2 = 0x4E , DPP_HALF_MIRROR = 0x141 , DPP_MIRROR = 0x140 , DPP_ROR4 = 0x124 , DPP_ROR8 = 0x128 };
+// sum over the 16 lanes of each row
+__device__ __forceinline__ double RowSum( double v ){ return DppAdd< DPP_MIRROR >( DppAdd< DPP_HALF_MIRROR >( DppAdd< DPP_XOR2 >( DppAdd< DPP_XOR1 >( v ) ) ) ); }
+// v[l] + v[l^16] (ROWS=16) or v[l] + v[l^32] (ROWS=32) with the gfx950 permlane swaps
+template< int ROWS >
+__device__ __forceinline__ double SwapAdd( double v )
+{
+	const unsigned long long b = (unsigned long long)__double_as_longlong( v );
+	const unsigned lo = (unsigned)b , hi = (unsigned)( b>>32 );
+	unsigned l0 , l1 , h0 , h1;
+	if constexpr( ROWS==16 )
+	{
+		const auto l = __builtin_amdgcn_permlane16_swap( lo , lo , false , false ) , h = __builtin_amdgcn_permlane16_swap( hi , hi , false , false );
+		l0 = l[0] , l1 = l[1] , h0 = h[0] , h1 = h[1];
+	}
+	else
+	{
+		const auto l = __builtin_amdgcn_permlane32_swap( lo , lo , false , false ) , h = __builtin_amdgcn_permlane32_swap( hi , hi , false , false );
+		l0 = l[0] , l1 = l[1] , h0 = h[0] , h1 = h[1];
+	}
+	return __longlong_as_double( (long long)( ( (unsigned long long)h0<<32 ) | l0 ) ) + __longlong_as_double( (long long)( ( (unsigned long long)h1<<32 ) | l1 ) );
+}
+// Orders a wavefront's LDS accesses on either side (a wave's own LDS
+// operations complete in order, so this only constrains the compiler).
+__device__ __forceinline__ void WaveSync( void )
+{
+	__builtin_amdgcn_fence( __ATOMIC_RELEASE , "wavefront" );
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence( __ATOMIC_ACQUIRE , "wavefront" );
+}
+
+// The step of NodeStep with the reciprocal of det from v_rcp_f64 refined by
+// two Newton iterations r <- r + r (1 - det r) (instead of the IEEE division
+// sequence; used on the latency-bound tiny levels).
+__device__ __forceinline__ bool NodeStepFast( double q20 , double q02 , double q11 , double q10 , double q01 , double &dx , double &dy )
+{
+	const double H00 = 2*q20 , H11 = 2*q02 , H01 = q11;
+	const double det = H00*H11 - H01*H01;
+	double r = __builtin_amdgcn_rcp( det );
+	r = fma( r , fma( -det , r , 1. ) , r );
+	r = fma( r , fma( -det , r , 1. ) , r );
+	dx = -0.5*( ( H11*q10 - H01*q01 )*r );
+	dy = -0.5*( ( H00*q01 - H01*q10 )*r );
+	return fabs( det )>1e-20;
+}
+
+template< int Dim > struct TinyLds
+{
+	static constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W , P = ( C*(C-1) )/2;
+	static constexpr bool Direct = Dim==3;                                     // 3D lanes form their pair wedges from the corners
+	static constexpr int WB = Direct ? 0 : TinyShape< Dim >::NCH * TinyShape< Dim >::QC;
+	static constexpr int PerWave = 2*C + WB;
+	static constexpr int PairSlots = ( P + 63 ) / 64;
+	// doubles before the node table: x, y, quadratic table, contributions,
+	// per-wave scratch, then (R + dWeight K, B) per node when RB, else the
+	// Dirichlet rows; rounded up to an even count
+	__host__ __device__ static size_t Doubles( long long N , long long ne , int waves , bool rb )
+	{
+		const size_t d = 2*N + (C-1)*(C-1) + 5*ne + (size_t)waves*PerWave + ( rb ? 2*N*W : W*W );
+		return ( d+1 ) & ~(size_t)1;
+	}
+	// bytes of dynamic LDS: the doubles, the node table (4 ints per node) and the colour starts
+	static size_t Bytes( long long N , long long ne , int waves , bool rb ){ return Doubles( N , ne , waves , rb )*sizeof(double) + ( 4*N + C + 1 )*sizeof(int); }
+};
+
+// Requires: res a power of two, at most 4 * (blockDim/64) nodes per colour.
+template< int Dim , bool RB >
+__global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView< Dim > L , int iters )
+{
+	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W , J = C-1 , P = ( C*(C-1) )/2;
+	constexpr int NCH = TinyShape< Dim >::NCH , QC = TinyShape< Dim >::QC;
+	constexpr int PS = TinyLds< Dim >::PairSlots , WM = ( W + 15 )/16;
+	constexpr int MC = (W/2)/16 , RC = (W/2)%16;   // slot and row lane of the centre window term
+	constexpr bool DIRECT = TinyLds< Dim >::Direct;
+	const int lane = threadIdx.x & 63 , wave = __builtin_amdgcn_readfirstlane( threadIdx.x >> 6 ) , nWaves = blockDim.x >> 6;
+	const int row = lane >> 4 , rl = lane & 15;
+	const int res = L.res , lres = __builtin_ctz( (unsigned)res );
+	const int N = (int)L.N , ne = 1 << ( lres*Dim );
+	int strideI[Dim];
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ) strideI[d] = (int)L.stride[d];
+
+	extern __shared__ double sm[];
+	double *sx = sm , *sy = sx + N , *sA = sy + N , *contrib = sA + J*J , *scr = contrib + 5*ne , *tail = scr + nWaves*TinyLds< Dim >::PerWave;
+	double *sRK = tail , *sB = tail + N*W , *sK = tail;
+	int *nodeInfo = (int *)( sm + TinyLds< Dim >::Doubles( N , ne , nWaves , RB ) ) , *colStart = nodeInfo + 4*N;
+	double *cb = scr + wave*TinyLds< Dim >::PerWave , *wb = cb + 2*C;
+	for( int i=threadIdx.x ; i<N ; i+=blockDim.x ) sx[i] = L.x[i] , sy[i] = L.y[i];
+	for( int i=threadIdx.x ; i<J*J ; i+=blockDim.x ) sA[i] = WA< Dim >( i/J , i%J );
+	if constexpr( RB )
+		for( int k=threadIdx.x ; k<N*W ; k+=blockDim.x )
+		{
+			int t = k/W , cse = 0 , F[Dim];
+			for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = t % (res+1) ; t /= res+1; }
+			for( int d=0 ; d<Dim ; d++ ) cse = cse*3 + ( F[d]==0 ? 0 : ( F[d]==res ? 2 : 1 ) );
+			sRK[k] = L.R[k] + L.K[ cse*W + k%W ]*L.dW , sB[k] = L.B[k];
+		}
+	else for( int i=threadIdx.x ; i<W*W ; i+=blockDim.x ) sK[i] = L.K[i];
+	// node table, sorted by colour: (flat index, boundary case, first incident
+	// element, low/high boundary bits), bit Dim-1-d for coordinate d
+	{
+		auto start = [&]( int c ){ int s = 0 ; for( int c2=0 ; c2<c ; c2++ ){ ColourSet< Dim > cs ; cs.set( res , c2 ) ; s += (int)cs.total; } return s; };
+		if( threadIdx.x<=C ) colStart[ threadIdx.x ] = start( threadIdx.x );
+		for( int n=threadIdx.x ; n<N ; n+=blockDim.x )
+		{
+			int F[Dim] , t = n;
+			for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = t % (res+1) ; t /= res+1; }
+			int colour = 0 , cse = 0 , e0 = 0 , low = 0 , high = 0;
+			for( int d=0 ; d<Dim ; d++ )
+			{
+				colour |= ( F[d]&1 )<<d;
+				cse = cse*3 + ( F[d]==0 ? 0 : ( F[d]==res ? 2 : 1 ) );
+				e0 += ( F[d]-1 ) * ( 1<<( lres*d ) );
+				if( F[d]==0 ) low |= 1<<( Dim-1-d );
+				if( F[d]==res ) high |= 1<<( Dim-1-d );
+			}
+			ColourSet< Dim > cs ; cs.set( res , colour );
+			int k = 0;
+			for( int d=0 ; d<Dim ; d++ ) k = k*cs.cnt[d] + ( F[d]>>1 );
+			int *o = nodeInfo + 4*( start( colour ) + k );
+			o[0] = n , o[1] = cse , o[2] = e0 , o[3] = low | ( high<<8 );
+		}
+	}
+	if constexpr( !DIRECT ) for( int p=P+lane ; p<TinyLds< Dim >::WB ; p+=64 ) wb[p] = 0;   // padding pairs stay zero
+
+	// element stage: lane (j, ch) holds row j of the cubic table on pair chunk ch
+	const int j = lane / NCH , ch = lane % NCH;
+	const bool rowLane = j<J;
+	double Treg[QC];
+#pragma unroll
+	for( int q=0 ; q<QC ; q++ ){ const int pp = ch*QC+q ; Treg[q] = ( rowLane && pp<P ) ? WT< Dim >( j , pp ) : 0.; }
+	// ch 0 forms u_j = sum_k A(j,k) y_k (from the y corners), ch 1 v_j = sum_k A(j,k) x_k;
+	// the quadratic table rows stay in registers in 3D and are read from LDS in 4D
+	constexpr int JR = Dim==3 ? J : 1;
+	double Areg[JR];
+#pragma unroll
+	for( int k=0 ; k<JR ; k++ ) Areg[k] = rowLane ? WA< Dim >( j , k ) : 0.;
+	const double *aRow = sA + ( rowLane ? j : 0 )*J;
+	const int dotBase = ( ch==0 ? C : 0 ) + 1;
+	// contribution slots: lanes 0..2 store their first sum (a20, a02, b01), lanes 0..1 their second (a11, b10)
+	const double f0 = ( lane==2 ? -2. : 1. )*L.sW , f1 = ( lane==0 ? -2. : 2. )*L.sW;
+	// pairs: DIRECT, the lane's own chunk; otherwise the pairs lane + 64 s it stages
+	constexpr int NPK = DIRECT ? QC : PS;
+	int pk[NPK] , pl[NPK];
+#pragma unroll
+	for( int s=0 ; s<NPK ; s++ )
+	{
+		const int p = DIRECT ? ch*QC+s : lane + 64*s;
+		pk[s] = p<P ? PK< Dim >( p ) : 0 , pl[s] = p<P ? PL< Dim >( p ) : 0;
+	}
+	// node stage: row lane rl takes incident element rl (rl<C) and window terms rl + 16 m
+	int eoff = 0;
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ) eoff += Bit< Dim >( rl , d ) * ( 1<<( lres*d ) );
+	int woff[WM] , wneg[WM] , wpos[WM];
+#pragma unroll
+	for( int m=0 ; m<WM ; m++ )
+	{
+		int t = rl + 16*m;
+		woff[m] = 0 , wneg[m] = 0 , wpos[m] = 0;
+#pragma unroll
+		for( int d=Dim-1 ; d>=0 ; d-- )
+		{
+			const int o = t%3 - 1 ; t /= 3;
+			woff[m] += o*strideI[d];
+			if( o<0 ) wneg[m] |= 1<<( Dim-1-d );
+			if( o>0 ) wpos[m] |= 1<<( Dim-1-d );
+		}
+	}
+	__syncthreads();
+
+	for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+	{
+		// ---- this wave's nodes (one per 16-lane row): table entry and window
+		// values are read first, their latency hides behind the element stage
+		const int c1 = colStart[c+1] , n = colStart[c] + 4*wave + row;
+		const bool valid = n<c1;
+		const int nn = valid ? n : 0;
+		const int i = nodeInfo[4*nn] , cse = nodeInfo[4*nn+1] , e0 = nodeInfo[4*nn+2] , lh = nodeInfo[4*nn+3];
+		const int low = valid ? lh & 0xFF : 0xFF , high = valid ? lh>>8 : 0xFF;   // an invalid row sees no neighbours and no elements
+		double wx[WM] , wy[WM] , wR[WM] , wB[WM];
+#pragma unroll
+		for( int m=0 ; m<WM ; m++ )
+		{
+			const int w = rl + 16*m;
+			const bool ok = valid && w<W && !( wneg[m] & low ) && !( wpos[m] & high );
+			const int r = ok ? i + woff[m] : 0 , k = ok ? i*W + w : 0;
+			const double xr = sx[r] , yr = sy[r];
+			double Rv , Bv;
+			if constexpr( RB ) Rv = sRK[k] , Bv = sB[k];
+			else Rv = L.R[k] + sK[ ok ? cse*W+w : 0 ]*L.dW , Bv = L.B[k];
+			wx[m] = ok ? xr : 0. , wy[m] = ok ? yr : 0. , wR[m] = ok ? Rv : 0. , wB[m] = ok ? Bv : 0.;
+		}
+		// ---- element stage
+		for( int e=wave ; e<ne ; e+=nWaves )
+		{
+			int E[Dim] , bb = 0;
+#pragma unroll
+			for( int d=0 ; d<Dim ; d++ ){ E[d] = ( e>>( lres*d ) ) & ( res-1 ) ; bb |= ( ( ( c>>d ) ^ E[d] ) & 1 )<<( Dim-1-d ); }
+			{
+				const int a = ( lane & (C-1) ) ^ bb;
+				int o = 0;
+#pragma unroll
+				for( int d=0 ; d<Dim ; d++ ) o += ( E[d] + Bit< Dim >( a , d ) ) * strideI[d];
+				const double v = ( lane & C ) ? sy[o] : sx[o];
+				if( lane<2*C ) cb[lane] = v;
+			}
+			WaveSync();
+			double z0 = 0 , z1 = 0;
+			if constexpr( DIRECT )
+			{
+#pragma unroll
+				for( int q=0 ; q<QC ; q+=2 )
+				{
+					z0 = fma( Treg[q  ] , cb[ pk[q  ] ]*cb[ C+pl[q  ] ] - cb[ pl[q  ] ]*cb[ C+pk[q  ] ] , z0 );
+					z1 = fma( Treg[q+1] , cb[ pk[q+1] ]*cb[ C+pl[q+1] ] - cb[ pl[q+1] ]*cb[ C+pk[q+1] ] , z1 );
+				}
+			}
+			else
+			{
+#pragma unroll
+				for( int s=0 ; s<PS ; s++ ) if( lane+64*s<P ) wb[ lane+64*s ] = cb[ pk[s] ]*cb[ C+pl[s] ] - cb[ pl[s] ]*cb[ C+pk[s] ];
+				WaveSync();
+#pragma unroll
+				for( int q=0 ; q<QC ; q+=2 ){ z0 = fma( Treg[q] , wb[ ch*QC+q ] , z0 ) ; z1 = fma( Treg[q+1] , wb[ ch*QC+q+1 ] , z1 ); }
+			}
+			// sum over the NCH chunk lanes of row j
+			double z = DppAdd< DPP_XOR2 >( DppAdd< DPP_XOR1 >( z0 + z1 ) );
+			if constexpr( NCH==8 ) z = DppAdd< DPP_HALF_MIRROR >( z );
+			double d0 = 0 , d1 = 0;
+#pragma unroll
+			for( int k=0 ; k<J ; k+=2 )
+			{
+				d0 = fma( Dim==3 ? Areg[ k<JR ? k : 0 ] : aRow[k] , cb[ dotBase+k ] , d0 );
+				if( k+1<J ) d1 = fma( Dim==3 ? Areg[ k+1<JR ? k+1 : 0 ] : aRow[k+1] , cb[ dotBase+k+1 ] , d1 );
+			}
+			const double dot = d0 + d1;
+			const double xj = rowLane ? cb[ j+1 ] : 0. , yj = rowLane ? cb[ C+j+1 ] : 0.;
+			// ch 0: (y.u , x.u) = (a20 , a11); ch 1: (x.v , y.z) = (a02 , b10); ch 2: (x.z , 0) = (b01 , 0)
+			double s0 = ch==0 ? yj*dot : ( ch==1 ? xj*dot : ( ch==2 ? xj*z : 0. ) );
+			double s1 = ch==0 ? xj*dot : ( ch==1 ? yj*z : 0. );
+			// sum over j: inside a 16-lane row the rows j sit NCH lanes apart, then across the four rows
+			if constexpr( NCH==8 ) s0 = DppAdd< DPP_ROR8 >( s0 ) , s1 = DppAdd< DPP_ROR8 >( s1 );
+			else s0 = DppAdd< DPP_ROR8 >( DppAdd< DPP_ROR4 >( s0 ) ) , s1 = DppAdd< DPP_ROR8 >( DppAdd< DPP_ROR4 >( s1 ) );
+			s0 = SwapAdd< 32 >( SwapAdd< 16 >( s0 ) ) , s1 = SwapAdd< 32 >( SwapAdd< 16 >( s1 ) );
+			// contrib[5e + 0..4] = (q20, q02, q01, q11, q10) of the element's colour-c node
+			if( lane<3 ) contrib[ 5*e+lane ] = s0*f0;
+			if( lane<2 ) contrib[ 5*e+3+lane ] = s1*f1;
+			WaveSync();
+		}
+		// ---- window terms of the wave's nodes
+		double q20 = 0 , q02 = 0 , q11 = 0 , q10 = 0 , q01 = 0;
+#pragma unroll
+		for( int m=0 ; m<WM ; m++ )
+		{
+			const double Rv = wR[m] , Bv = wB[m];
+			if( m==MC && rl==RC ){ q20 += Rv ; q02 += Rv; }
+			q10 += 2.*Rv*wx[m];
+			q01 += 2.*Rv*wy[m];
+			if( !( m==MC && rl==RC ) ){ q10 -= 2.*Bv*wy[m] ; q01 += 2.*Bv*wx[m]; }
+		}
+		__syncthreads();
+		// ---- the contributions of the nodes' elements, the step
+		{
+			const bool okE = rl<C && !( ( ( ~rl & low ) | ( rl & high ) ) & (C-1) );
+			const double *o = contrib + 5*( okE ? e0 + eoff : 0 );
+			const double o0 = o[0] , o1 = o[1] , o2 = o[2] , o3 = o[3] , o4 = o[4];
+			if( okE ) q20 += o0 , q02 += o1 , q01 += o2 , q11 += o3 , q10 += o4;
+		}
+		q20 = RowSum( q20 ) , q02 = RowSum( q02 ) , q11 = RowSum( q11 ) , q10 = RowSum( q10 ) , q01 = RowSum( q01 );
+		double dx , dy;
+		const bool step = NodeStepFast( q20 , q02 , q11 , q10 , q01 , dx , dy );
+		if( valid && rl==RC && step ) sx[i] = wx[MC] + dx , sy[i] = wy[MC] + dy;
+		__syncthreads();
+	}
+	for( int i=threadIdx.x ; i<N ; i+=blockDim.x ) L.x[i] = sx[i] , L.y[i] = sy[i];
+}
+
 // mean of the per-sample measures (deterministic): out[1] = sum / n, out[0] = sWeight
 __global__ void KMeanMeasure( const double *partials , int nb , long long n , double sWeight , double *out )
 {
