@@ -61,6 +61,31 @@ def reference_energy(name):
     return json.load(open(path)).get(name)
 
 
+COPY_BYTES = 1 << 30      # calibration copy: 1 GiB read + 1 GiB written per dispatch
+
+
+def measured_traffic(name):
+    """HBM bytes per launch of the finest-sweep kernel from the committed
+    rocprofv3 --pmc summary of this workload (None when there is none)."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{name}.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def copy_bandwidth():
+    """GB/s of the library's plain copy kernel (epr_time_copy), 2 x COPY_BYTES per copy."""
+    import ctypes
+    from exteriorpoissonrecon_amd._lib import load
+    ms = ctypes.c_double()
+    if load().epr_time_copy(COPY_BYTES, 5, ctypes.byref(ms)) != 0:
+        return None
+    return 2 * COPY_BYTES / (ms.value * 1e-3) / 1e9
+
+
 def cpu_baseline(name, wl, pos, skew):
     """The reference build (oracle/_ref/ExteriorPoissonRecon, compiled from the
     reference sources) timed on this host's cores on the same samples."""
@@ -148,6 +173,14 @@ def main():
             lms, ln = rec.time_gauss_seidel(d, it)
             log(f"[bench]   level {d}: N={rec.num_functions(d)} iters={it} {lms:.3f} ms ({ln} launches)")
 
+    copy_gbs = copy_bandwidth()
+    if copy_gbs:
+        log(f"[bench] copy kernel: {copy_gbs:.1f} GB/s ({2 * COPY_BYTES} bytes per copy)")
+    traffic = measured_traffic(args.config)
+    if traffic:
+        log(f"[bench] measured HBM traffic of the finest-sweep kernel: {traffic / 1e6:.2f} MB per launch "
+            f"({traffic / per_launch_s / 1e9:.1f} GB/s at this launch time)")
+
     cpu = None if args.no_cpu else cpu_baseline(args.config, wl, pos, skew)
     if cpu:
         log(f"[bench] cpu baseline {cpu['value']:.3f}s ({cpu['sample']})")
@@ -169,10 +202,12 @@ def main():
                    "grid_nodes": int(N), "build_s": float(np.mean(builds)), "solve_s": float(np.mean(solves)),
                    "final_energy": float(E.sum()), "energy_rel_err_vs_reference": rel},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": f"KGsPhase<{dim}> (finest level, one colour)",
                      "hbm_gbs_algorithmic": bytes_per_launch / per_launch_s / 1e9,
-                     "flops_per_node": F_NODE[dim], "reference_equivalent_tflops": achieved_ref},
+                     "flops_per_node": F_NODE[dim], "reference_equivalent_tflops": achieved_ref,
+                     "us_per_launch": per_launch_s * 1e6, "copy_kernel_gbs": copy_gbs,
+                     "traffic_gbs": traffic / per_launch_s / 1e9 if traffic else None},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

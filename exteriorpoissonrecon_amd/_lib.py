@@ -16,6 +16,7 @@ EXPORTS = [
     "epr_build", "epr_init", "epr_gs_sweeps", "epr_prolong_add", "epr_energies", "epr_solve",
     "epr_num_functions", "epr_get_coeffs", "epr_set_coeffs", "epr_quadratic_fits", "epr_apply",
     "epr_get_array", "epr_write_tree", "epr_write_grid", "epr_last_seconds", "epr_time_gs_sweeps",
+    "epr_time_copy",
 ]
 
 
@@ -69,6 +70,8 @@ def load():
     L.epr_write_grid.argtypes = [vp, ctypes.c_char_p]
     L.epr_time_gs_sweeps.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
     L.epr_time_gs_sweeps.restype = ctypes.c_int
+    L.epr_time_copy.argtypes = [sz, ci, dp]
+    L.epr_time_copy.restype = ctypes.c_int
     L.epr_last_seconds.argtypes = [vp, ci]
     L.epr_last_seconds.restype = ctypes.c_double
     for name in ["epr_create", "epr_set_samples", "epr_build", "epr_init", "epr_gs_sweeps",

@@ -809,6 +809,30 @@ int epr_get_array( epr_ctx *ctx , const char *name , double *out , size_t cap , 
 }
 int epr_write_tree( epr_ctx *ctx , const char *path , unsigned long long hash ){ IMPL( ctx->impl->writeTree( path , hash ) ); }
 int epr_write_grid( epr_ctx *ctx , const char *path ){ IMPL( ctx->impl->writeGrid( path ) ); }
+int epr_time_copy( size_t bytes , int reps , double *ms )
+{
+	if( !ms || reps<=0 || bytes<16 ) return EPR_ERR_ARG;
+	const long long n = (long long)( bytes/16 );
+	double2 *a = nullptr , *b = nullptr;
+	hipEvent_t e0 = nullptr , e1 = nullptr;
+	int rc = EPR_OK;
+	auto ok = [&]( hipError_t e ){ if( e!=hipSuccess && rc==EPR_OK ) rc = EPR_ERR_HIP ; return rc==EPR_OK; };
+	if( ok( hipMalloc( &a , n*16 ) ) && ok( hipMalloc( &b , n*16 ) ) && ok( hipMemset( a , 0 , n*16 ) ) && ok( hipEventCreate( &e0 ) ) && ok( hipEventCreate( &e1 ) ) )
+	{
+		const unsigned int blocks = (unsigned int)std::min< long long >( ( n + 255 )/256 , 256LL*64 );
+		epr::KCopy<<< blocks , 256 >>>( a , b , n );
+		ok( hipEventRecord( e0 , nullptr ) );
+		for( int r=0 ; r<reps ; r++ ) epr::KCopy<<< blocks , 256 >>>( a , b , n );
+		ok( hipEventRecord( e1 , nullptr ) );
+		float t = 0;
+		if( ok( hipEventSynchronize( e1 ) ) && ok( hipEventElapsedTime( &t , e0 , e1 ) ) ) *ms = t / reps;
+	}
+	if( e0 ) (void)hipEventDestroy( e0 );
+	if( e1 ) (void)hipEventDestroy( e1 );
+	if( a ) (void)hipFree( a );
+	if( b ) (void)hipFree( b );
+	return rc;
+}
 double epr_last_seconds( const epr_ctx *ctx , int which ){ return ( ctx && ctx->impl && ( which==0 || which==1 ) ) ? ctx->impl->seconds[which] : -1.; }
 
 }

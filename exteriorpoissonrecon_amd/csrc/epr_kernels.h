@@ -419,6 +419,12 @@ __global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf
 }
 
 // out = a + b
+// Plain device copy (16 B per lane per iteration), the calibration kernel for HBM counters.
+__global__ void __launch_bounds__( 256 ) KCopy( const double2 *__restrict__ a , double2 *__restrict__ b , long long n )
+{
+	for( long long k=(long long)blockIdx.x*blockDim.x + threadIdx.x ; k<n ; k+=(long long)gridDim.x*blockDim.x ) b[k] = a[k];
+}
+
 __global__ void KAddInto( const double *a , const double *b , double *out , long long n )
 {
 	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;

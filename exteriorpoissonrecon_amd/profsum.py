@@ -1,0 +1,110 @@
+"""Summaries of rocprofv3 CSV output for the profiles/ folder.
+
+    python -m exteriorpoissonrecon_amd.profsum stats <dir> [--top N]
+        per-kernel table from a `rocprofv3 --kernel-trace --stats --output-format csv` run
+    python -m exteriorpoissonrecon_amd.profsum traffic <fetch_dir> <write_dir> --kernel NAME --copy-bytes B
+        HBM bytes per dispatch of the largest-grid dispatches of kernel NAME from
+        two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs, scaled by what
+        the same counters read for the KCopy calibration kernel (B bytes read and
+        B bytes written per dispatch, launched by bench.py through epr_time_copy)
+
+Counter values are KiB per dispatch.  The scale factor of a counter is
+(known KiB of the copy) / (counter value of the copy); DESIGN.md records it.
+"""
+import argparse
+import csv
+import glob
+import os
+import sys
+from collections import defaultdict
+
+
+def _files(d, suffix):
+    return sorted(glob.glob(os.path.join(d, "**", f"*{suffix}"), recursive=True))
+
+
+def _short(name, width=90):
+    name = name.replace("void ", "")
+    return name if len(name) <= width else name[:width - 3] + "..."
+
+
+def stats(d, top=25):
+    rows = []
+    for f in _files(d, "kernel_stats.csv"):
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    if not rows:
+        raise SystemExit(f"no kernel_stats.csv under {d}")
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    for r in rows[:top]:
+        out.append(f"| `{_short(r['Name'])}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e6:.3f} | "
+                   f"{float(r['AverageNs']) / 1e3:.1f} | {float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | "
+                   f"{100 * float(r['TotalDurationNs']) / total:.1f} |")
+    out.append(f"\nTotal kernel time {total / 1e6:.3f} ms over {sum(int(r['Calls']) for r in rows)} dispatches.")
+    return "\n".join(out)
+
+
+def _counters(d, counter):
+    """(kernel name, grid size) -> list of counter values (one per dispatch)."""
+    vals = defaultdict(list)
+    for f in _files(d, "counter_collection.csv"):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") != counter:
+                    continue
+                vals[(r["Kernel_Name"], int(r["Grid_Size"]))].append(float(r["Counter_Value"]))
+    if not vals:
+        raise SystemExit(f"no {counter} rows under {d}")
+    return vals
+
+
+def _select(vals, kernel):
+    keys = [k for k in vals if kernel in k[0]]
+    if not keys:
+        raise SystemExit(f"no dispatches of {kernel}")
+    gmax = max(k[1] for k in keys)
+    big = [k for k in keys if k[1] * 4 >= gmax]          # the largest-grid dispatches (the finest level)
+    v = [x for k in big for x in vals[k]]
+    return sum(v) / len(v), len(v), sorted({k[1] for k in big})
+
+
+def traffic(fetch_dir, write_dir, kernel, copy_bytes):
+    fv, wv = _counters(fetch_dir, "FETCH_SIZE"), _counters(write_dir, "WRITE_SIZE")
+    cf, _, _ = _select(fv, "KCopy")
+    cw, _, _ = _select(wv, "KCopy")
+    known_kib = copy_bytes / 1024.0
+    sf, sw = known_kib / cf, known_kib / cw
+    kf, nf, grids = _select(fv, kernel)
+    kw, nw, _ = _select(wv, kernel)
+    bytes_per = (kf * sf + kw * sw) * 1024.0
+    out = [
+        f"calibration KCopy ({copy_bytes} B read + {copy_bytes} B written per dispatch): "
+        f"FETCH_SIZE {cf:.1f} KiB -> factor {sf:.4f}; WRITE_SIZE {cw:.1f} KiB -> factor {sw:.4f}",
+        f"{kernel}: {nf} dispatches (grid sizes {grids}), mean FETCH_SIZE {kf:.1f} KiB, "
+        f"mean WRITE_SIZE {kw:.1f} KiB",
+        f"traffic_bytes_per_launch: {bytes_per:.6g}",
+        f"fetch_bytes_per_launch: {kf * sf * 1024.0:.6g}",
+        f"write_bytes_per_launch: {kw * sw * 1024.0:.6g}",
+    ]
+    return "\n".join(out)
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    s.add_argument("--top", type=int, default=25)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--kernel", required=True)
+    t.add_argument("--copy-bytes", type=int, required=True)
+    a = ap.parse_args(argv)
+    print(stats(a.dir, a.top) if a.cmd == "stats" else traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_bytes))
+
+
+if __name__ == "__main__":
+    sys.exit(main())

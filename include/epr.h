@@ -129,6 +129,13 @@ int  epr_write_grid( epr_ctx *ctx , const char *path );
  * returns the elapsed device milliseconds and the number of kernel launches. */
 int  epr_time_gs_sweeps( epr_ctx *ctx , int level , int iters , double *ms , int *launches );
 
+/* Measurement hook (no reference counterpart): a plain device-to-device copy
+ * kernel over two buffers of `bytes` bytes on the current device, run reps
+ * times after one warm-up launch; returns the mean milliseconds per copy.  It
+ * calibrates the HBM byte counters (FETCH_SIZE, WRITE_SIZE) against a kernel
+ * of known traffic and gives the achievable copy bandwidth. */
+int  epr_time_copy( size_t bytes , int reps , double *ms );
+
 /* Wall-clock seconds of the last epr_build / epr_solve (device-synchronised). */
 double epr_last_seconds( const epr_ctx *ctx , int which /* 0 build, 1 solve */ );
 
