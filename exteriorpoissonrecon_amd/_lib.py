@@ -16,7 +16,8 @@ EXPORTS = [
     "epr_build", "epr_init", "epr_gs_sweeps", "epr_prolong_add", "epr_energies", "epr_solve",
     "epr_num_functions", "epr_get_coeffs", "epr_set_coeffs", "epr_quadratic_fits", "epr_apply",
     "epr_get_array", "epr_write_tree", "epr_write_grid", "epr_last_seconds", "epr_time_gs_sweeps",
-    "epr_time_copy",
+    "epr_time_copy", "epr_subgrid_triangulation", "epr_level_set", "epr_mesh_info", "epr_mesh_copy",
+    "epr_mesh_destroy",
 ]
 
 
@@ -70,6 +71,16 @@ def load():
     L.epr_write_grid.argtypes = [vp, ctypes.c_char_p]
     L.epr_time_gs_sweeps.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
     L.epr_time_gs_sweeps.restype = ctypes.c_int
+    up = ctypes.POINTER(ctypes.c_uint)
+    L.epr_subgrid_triangulation.argtypes = [ci, sz, up, ci, ctypes.POINTER(vp)]
+    L.epr_subgrid_triangulation.restype = ctypes.c_int
+    L.epr_level_set.argtypes = [ci, ci, sz, dp, sz, up, dp, dp, ctypes.POINTER(vp)]
+    L.epr_level_set.restype = ctypes.c_int
+    L.epr_mesh_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(ci), ctypes.POINTER(sz), ctypes.POINTER(ci)]
+    L.epr_mesh_info.restype = ctypes.c_int
+    L.epr_mesh_copy.argtypes = [vp, dp, up]
+    L.epr_mesh_copy.restype = ctypes.c_int
+    L.epr_mesh_destroy.argtypes = [vp]
     L.epr_time_copy.argtypes = [sz, ci, dp]
     L.epr_time_copy.restype = ctypes.c_int
     L.epr_last_seconds.argtypes = [vp, ci]

@@ -19,39 +19,12 @@
 #include <vector>
 
 #include "../../include/epr.h"
+#include "epr_dev.h"
 #include "epr_hat.h"
 #include "epr_kernels.h"
 
 namespace epr
 {
-struct HipError { std::string msg; };
-#define HIPCHECK( call ) do { hipError_t _e = ( call ) ; if( _e!=hipSuccess ) throw ::epr::HipError{ std::string( #call ) + ": " + hipGetErrorString( _e ) }; } while( 0 )
-
-template< typename T >
-struct DevBuf
-{
-	T *p = nullptr;
-	size_t n = 0;
-	DevBuf(){}
-	DevBuf( const DevBuf & ) = delete;
-	DevBuf &operator=( const DevBuf & ) = delete;
-	DevBuf( DevBuf &&o ) noexcept : p( o.p ) , n( o.n ) { o.p = nullptr , o.n = 0; }
-	DevBuf &operator=( DevBuf &&o ) noexcept { std::swap( p , o.p ) ; std::swap( n , o.n ) ; return *this; }
-	~DevBuf(){ if( p ) (void)hipFree( p ); }
-	void ensure( size_t count ){ if( count>n ) alloc( count ); }
-	void alloc( size_t count )
-	{
-		if( p ){ (void)hipFree( p ) ; p = nullptr; }
-		n = count;
-		if( count ) HIPCHECK( hipMalloc( (void**)&p , count*sizeof(T) ) );
-	}
-	void zero( hipStream_t s ){ if( n ) HIPCHECK( hipMemsetAsync( p , 0 , n*sizeof(T) , s ) ); }
-	void upload( const T *h , size_t count , hipStream_t s ){ if( count>n ) alloc( count ) ; if( count ) HIPCHECK( hipMemcpyAsync( p , h , count*sizeof(T) , hipMemcpyHostToDevice , s ) ); }
-	void download( T *h , size_t count , hipStream_t s ) const { if( count ) HIPCHECK( hipMemcpyAsync( h , p , count*sizeof(T) , hipMemcpyDeviceToHost , s ) ); }
-};
-
-static inline unsigned int Blocks( long long n , int bs ){ return (unsigned int)( ( n + bs - 1 ) / bs ); }
-
 struct CtxBase
 {
 	virtual ~CtxBase(){}
@@ -746,6 +719,8 @@ struct epr_ctx
 	std::string err;
 };
 
+const char *EprMeshLastError( void );   // epr_levelset.hip: the calling thread's last context-free error
+
 template< typename F >
 static int Guard( epr_ctx *ctx , F f )
 {
@@ -782,7 +757,7 @@ int epr_create( int dim , const epr_params *p , epr_ctx **out )
 }
 
 void epr_destroy( epr_ctx *ctx ){ delete ctx; }
-const char *epr_last_error( const epr_ctx *ctx ){ return ctx ? ctx->err.c_str() : "null context"; }
+const char *epr_last_error( const epr_ctx *ctx ){ return ctx ? ctx->err.c_str() : EprMeshLastError(); }
 
 #define IMPL( body ) return Guard( ctx , [&](){ if( !ctx->impl ) throw std::string( "context not initialised" ) ; body; } )
 

@@ -54,7 +54,7 @@ void epr_default_params( epr_params *p );
  * (ExteriorPoisson/ExteriorPoisson.h:56-72, 126-313); dim is 3 or 4. */
 int  epr_create( int dim , const epr_params *p , epr_ctx **out );
 void epr_destroy( epr_ctx *ctx );
-const char *epr_last_error( const epr_ctx *ctx );
+const char *epr_last_error( const epr_ctx *ctx );   /* NULL: the calling thread's last context-free (mesh) error */
 
 /* Hands over the samples read by ReadSamples<Dim>
  * (ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:282-303): pos is n x dim
@@ -128,6 +128,29 @@ int  epr_write_grid( epr_ctx *ctx , const char *path );
  * iterations at one level bracketed by HIP events on the context's stream and
  * returns the elapsed device milliseconds and the number of kernel launches. */
 int  epr_time_gs_sweeps( epr_ctx *ctx , int level , int iters , double *ms , int *launches );
+
+/* ---- Simplicial extraction (the reference's Extract, Extract/Extract.cpp:161-208) ----
+ * A mesh handle holds vertices (nVertices x vertexDim doubles) and simplices
+ * (nSimplices x simplexVertices indices) in host memory. */
+typedef struct epr_mesh epr_mesh;
+int  epr_mesh_info( const epr_mesh *m , size_t *nVertices , int *vertexDim , size_t *nSimplices , int *simplexVertices );
+int  epr_mesh_copy( const epr_mesh *m , double *vertices , unsigned *simplices );
+void epr_mesh_destroy( epr_mesh *m );
+
+/* Replaces MarchingSimplices::RegularSubGridTriangulation< dim >( nCells , F ,
+ * orient , false ) (Misha/MarchingSimplices.h:622-706): the dim! simplices of
+ * every listed cell (cells: nCells x dim unsigned offsets), vertices numbered
+ * in order of first appearance, positions as doubles. */
+int  epr_subgrid_triangulation( int dim , size_t nCells , const unsigned *cells , int orient , epr_mesh **out );
+
+/* Replaces MarchingSimplices::LevelSet< double , 2 , dim >( mesh , PF , iso )
+ * (Misha/MarchingSimplices.h:410-576): the (dim-2)-simplices of the joint
+ * zero set of two values per vertex (values: nV x 2; iso NULL = (0,0)) over a
+ * mesh of dim-simplices (simplices: nS x (dim+1)); output vertices carry the
+ * barycentric interpolation of the input vertex data (nV x vdim), numbered
+ * and ordered as the reference numbers and orders them. */
+int  epr_level_set( int dim , int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *simplices ,
+                    const double *values , const double *iso , epr_mesh **out );
 
 /* Measurement hook (no reference counterpart): a plain device-to-device copy
  * kernel over two buffers of `bytes` bytes on the current device, run reps

@@ -80,6 +80,22 @@ void   orc_solve( void *h , double *hx , double *hy , double *fx , double *fy , 
 /* writes <hdr>.tree in the reference's byte layout (ExteriorPoissonRecon.cpp:259-270) */
 int    orc_write_tree( void *h , const char *path , const double *hx , const double *hy , unsigned long long indexerHash );
 
+/* ---- simplicial extraction (epr_levelset.cpp) ---- */
+/* the canonical cube's Dim! simplices, (dim+1) vertex bit-indices each
+ * (Misha/HyperCubeSimplices.h:47-170); returns the simplex count */
+int    orc_cube_simplices( int dim , int *out );
+/* MarchingSimplices::RegularSubGridTriangulation (orient=true/false,
+ * addCenters=false), Misha/MarchingSimplices.h:622-706: V (nV x dim) and
+ * S (nS x (dim+1)) are malloc'ed, release with orc_free */
+int    orc_subgrid_triangulation( int dim , size_t nCells , const unsigned *cells , int orient ,
+                                  size_t *nV , double **V , size_t *nS , unsigned **S );
+/* MarchingSimplices::LevelSet (codim 1 or 2), Misha/MarchingSimplices.h:410-576:
+ * values are nV x codim, vertex data nV x vdim; LV (nLV x vdim) and LS
+ * (nLS x (dim-codim+1)) are malloc'ed */
+int    orc_level_set( int dim , int codim , int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *S ,
+                      const double *values , const double *iso , size_t *nLV , double **LV , size_t *nLS , unsigned **LS );
+void   orc_free( void *p );
+
 #ifdef __cplusplus
 }
 #endif

@@ -56,6 +56,13 @@ def lib():
         L.orc_init.argtypes = [vp, dp, dp]
         L.orc_solve.argtypes = [vp, dp, dp, dp, dp, dp]
         L.orc_write_tree.argtypes = [vp, ctypes.c_char_p, dp, dp, ctypes.c_ulonglong]
+        up = ctypes.POINTER(ctypes.c_uint)
+        L.orc_cube_simplices.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)]
+        L.orc_subgrid_triangulation.argtypes = [ctypes.c_int, sz, up, ctypes.c_int, ctypes.POINTER(sz),
+                                                ctypes.POINTER(dp), ctypes.POINTER(sz), ctypes.POINTER(up)]
+        L.orc_level_set.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, sz, dp, sz, up, dp, dp,
+                                    ctypes.POINTER(sz), ctypes.POINTER(dp), ctypes.POINTER(sz), ctypes.POINTER(up)]
+        L.orc_free.argtypes = [ctypes.c_void_p]
         _lib = L
     return _lib
 
@@ -172,3 +179,47 @@ def load_dump(path):
         c = struct.unpack_from("<Q", b, o)[0]; o += 8
         d[name] = np.frombuffer(b, dtype="<f8", count=c, offset=o).copy(); o += 8 * c
     return d
+
+
+# ---- simplicial extraction ------------------------------------------------
+def _take(ptr, n, dtype):
+    out = np.ctypeslib.as_array(ptr, shape=(max(n, 1),))[:n].astype(dtype, copy=True)
+    lib().orc_free(ctypes.cast(ptr, ctypes.c_void_p))
+    return out
+
+
+def cube_simplices(dim):
+    """The canonical cube's simplices as vertex bit-indices, (dim!, dim+1)."""
+    out = (ctypes.c_int * (24 * 5))()
+    n = lib().orc_cube_simplices(dim, out)
+    return np.array(out[:n * (dim + 1)]).reshape(n, dim + 1)
+
+
+def subgrid_triangulation(dim, cells, orient=True):
+    """(vertices nV x dim, simplices nS x (dim+1)) of the reference's RegularSubGridTriangulation."""
+    c = np.ascontiguousarray(cells, dtype=np.uint32)
+    nV, nS = ctypes.c_size_t(), ctypes.c_size_t()
+    V, S = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_uint)()
+    rc = lib().orc_subgrid_triangulation(dim, len(c), c.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), int(orient),
+                                         ctypes.byref(nV), ctypes.byref(V), ctypes.byref(nS), ctypes.byref(S))
+    if rc:
+        raise ValueError("bad triangulation input")
+    return _take(V, nV.value * dim, np.float64).reshape(-1, dim), _take(S, nS.value * (dim + 1), np.uint32).reshape(-1, dim + 1)
+
+
+def level_set(vertices, simplices, values, iso=None):
+    """(vertices, simplices) of the reference's MarchingSimplices::LevelSet; values nV x codim."""
+    V = np.ascontiguousarray(vertices, dtype=np.float64)
+    S = np.ascontiguousarray(simplices, dtype=np.uint32)
+    val = np.ascontiguousarray(values, dtype=np.float64)
+    codim = 1 if val.ndim == 1 else val.shape[1]
+    dim, vdim = S.shape[1] - 1, V.shape[1]
+    iso = np.zeros(codim) if iso is None else np.ascontiguousarray(iso, dtype=np.float64)
+    nLV, nLS = ctypes.c_size_t(), ctypes.c_size_t()
+    LV, LS = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_uint)()
+    rc = lib().orc_level_set(dim, codim, vdim, len(V), _p(V), len(S), S.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)),
+                             _p(val), _p(iso), ctypes.byref(nLV), ctypes.byref(LV), ctypes.byref(nLS), ctypes.byref(LS))
+    if rc:
+        raise ValueError("bad level-set input")
+    k = dim - codim + 1
+    return _take(LV, nLV.value * vdim, np.float64).reshape(-1, vdim), _take(LS, nLS.value * k, np.uint32).reshape(-1, k)
