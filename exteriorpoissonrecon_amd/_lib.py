@@ -17,7 +17,8 @@ EXPORTS = [
     "epr_num_functions", "epr_get_coeffs", "epr_set_coeffs", "epr_quadratic_fits", "epr_apply",
     "epr_get_array", "epr_write_tree", "epr_write_grid", "epr_last_seconds", "epr_time_gs_sweeps",
     "epr_time_copy", "epr_subgrid_triangulation", "epr_level_set", "epr_mesh_info", "epr_mesh_copy",
-    "epr_mesh_destroy",
+    "epr_mesh_destroy", "epr_mg_create", "epr_mg_num_levels", "epr_mg_level_size", "epr_mg_set_colours",
+    "epr_mg_vcycle", "epr_mg_relax", "epr_mg_destroy",
 ]
 
 
@@ -81,6 +82,20 @@ def load():
     L.epr_mesh_copy.argtypes = [vp, dp, up]
     L.epr_mesh_copy.restype = ctypes.c_int
     L.epr_mesh_destroy.argtypes = [vp]
+    L.epr_mg_create.argtypes = [sz, up, up, dp, ci, ctypes.POINTER(sz), ctypes.POINTER(sz), ctypes.POINTER(up),
+                                ctypes.POINTER(up), ctypes.POINTER(dp), ctypes.POINTER(vp)]
+    L.epr_mg_create.restype = ctypes.c_int
+    L.epr_mg_num_levels.argtypes = [vp]
+    L.epr_mg_num_levels.restype = ctypes.c_int
+    L.epr_mg_level_size.argtypes = [vp, ci]
+    L.epr_mg_level_size.restype = sz
+    L.epr_mg_set_colours.argtypes = [vp, ci, ci, up, up]
+    L.epr_mg_set_colours.restype = ctypes.c_int
+    L.epr_mg_vcycle.argtypes = [vp, dp, dp, ctypes.c_uint]
+    L.epr_mg_vcycle.restype = ctypes.c_int
+    L.epr_mg_relax.argtypes = [vp, ci, dp, dp, ctypes.c_uint]
+    L.epr_mg_relax.restype = ctypes.c_int
+    L.epr_mg_destroy.argtypes = [vp]
     L.epr_time_copy.argtypes = [sz, ci, dp]
     L.epr_time_copy.restype = ctypes.c_int
     L.epr_last_seconds.argtypes = [vp, ci]

@@ -1,6 +1,7 @@
 // Device-memory helpers shared by the library's translation units.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <exception>
 #include <string>
 #include <utility>
 
@@ -34,4 +35,16 @@ struct DevBuf
 
 static inline unsigned int Blocks( long long n , int bs ){ return (unsigned int)( ( n + bs - 1 ) / bs ); }
 
+// the calling thread's last error of a context-free entry point (epr_last_error(NULL))
+std::string &ThreadError( void );
+
+// runs f, mapping exceptions to status codes and messages (EPR_* of epr.h)
+template< typename F >
+int ContextFree( F f )
+{
+	try { f() ; ThreadError().clear() ; return 0; }
+	catch( const HipError &e ){ ThreadError() = e.msg ; return -3; }
+	catch( const std::string &s ){ ThreadError() = s ; return -4; }
+	catch( const std::exception &e ){ ThreadError() = e.what() ; return -2; }
+}
 } // namespace epr

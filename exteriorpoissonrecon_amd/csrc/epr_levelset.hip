@@ -661,35 +661,28 @@ static epr_mesh *LevelSet2( int vdim , size_t nV , const double *vdata , size_t 
 } // namespace ls
 } // namespace epr
 
-static thread_local std::string g_meshError;
-
-template< typename F >
-static int MeshGuard( F f )
+namespace epr
 {
-	try { f() ; g_meshError.clear() ; return EPR_OK; }
-	catch( const epr::HipError &e ){ g_meshError = e.msg ; return EPR_ERR_HIP; }
-	catch( const std::string &s ){ g_meshError = s ; return EPR_ERR_INPUT; }
-	catch( const std::exception &e ){ g_meshError = e.what() ; return EPR_ERR_STATE; }
+std::string &ThreadError( void ){ static thread_local std::string e ; return e; }
 }
-
-const char *EprMeshLastError( void ){ return g_meshError.c_str(); }
+const char *EprMeshLastError( void ){ return epr::ThreadError().c_str(); }
 
 extern "C" {
 
 int epr_subgrid_triangulation( int dim , size_t nCells , const unsigned *cells , int orient , epr_mesh **out )
 {
-	if( !out || ( dim!=3 && dim!=4 ) || ( nCells && !cells ) ){ g_meshError = "bad argument" ; return EPR_ERR_ARG; }
+	if( !out || ( dim!=3 && dim!=4 ) || ( nCells && !cells ) ){ epr::ThreadError() = "bad argument" ; return EPR_ERR_ARG; }
 	*out = nullptr;
-	return MeshGuard( [&](){ *out = epr::ls::Triangulate( dim , nCells , cells , orient!=0 ); } );
+	return epr::ContextFree( [&](){ *out = epr::ls::Triangulate( dim , nCells , cells , orient!=0 ); } );
 }
 
 int epr_level_set( int dim , int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *simplices , const double *values , const double *iso , epr_mesh **out )
 {
-	if( !out || ( dim!=3 && dim!=4 ) || vdim<1 || vdim>16 || ( nS && ( !simplices || !values || !vdata ) ) ){ g_meshError = "bad argument" ; return EPR_ERR_ARG; }
+	if( !out || ( dim!=3 && dim!=4 ) || vdim<1 || vdim>16 || ( nS && ( !simplices || !values || !vdata ) ) ){ epr::ThreadError() = "bad argument" ; return EPR_ERR_ARG; }
 	*out = nullptr;
 	const double zero[2] = { 0 , 0 };
 	if( !iso ) iso = zero;
-	return MeshGuard( [&]()
+	return epr::ContextFree( [&]()
 	{
 		if( dim==3 ) *out = epr::ls::LevelSet2< 3 >( vdim , nV , vdata , nS , simplices , values , iso );
 		else         *out = epr::ls::LevelSet2< 4 >( vdim , nV , vdata , nS , simplices , values , iso );

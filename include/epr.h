@@ -152,6 +152,33 @@ int  epr_subgrid_triangulation( int dim , size_t nCells , const unsigned *cells 
 int  epr_level_set( int dim , int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *simplices ,
                     const double *values , const double *iso , epr_mesh **out );
 
+/* ---- Multigrid V-cycle (the reference's Include/MG.h) ----
+ * Sparse matrices are CSC with sorted row indices (Eigen's storage). */
+typedef struct epr_mg epr_mg;
+
+/* Replaces MGSolver( M , Ps , pNum ) (Include/MG.h:83-108): M is n x n;
+ * prolongation i (i = 0..pNum-1) is pRows[i] x pCols[i] and maps level i to
+ * level i+1, level pNum being M's.  The Galerkin matrices
+ * M_i = P_i^T M_{i+1} P_i are formed on the device; the coarsest (at most
+ * 4096 unknowns, symmetric positive definite) is inverted there. */
+int  epr_mg_create( size_t n , const unsigned *colptr , const unsigned *rowidx , const double *values ,
+                    int pNum , const size_t *pRows , const size_t *pCols ,
+                    const unsigned *const *pColptr , const unsigned *const *pRowidx , const double *const *pValues , epr_mg **out );
+int  epr_mg_num_levels( const epr_mg *mg );
+size_t epr_mg_level_size( const epr_mg *mg , int level );
+
+/* The colour lists MCI(level) of MGSolver::vCycle's multicolour form
+ * (MG.h:146-186) for a level 1..pNum: colour c holds
+ * indices[starts[c] .. starts[c+1]). */
+int  epr_mg_set_colours( epr_mg *mg , int level , int nColours , const unsigned *starts , const unsigned *indices );
+
+/* MGSolver::vCycle( b , x , iters , MCI ) (MG.h:146-186); x is updated in place. */
+int  epr_mg_vcycle( epr_mg *mg , const double *b , double *x , unsigned iters );
+
+/* iters multicolour GSRelaxer sweeps (MG.h:62-79) of a level's matrix. */
+int  epr_mg_relax( epr_mg *mg , int level , const double *b , double *x , unsigned iters );
+void epr_mg_destroy( epr_mg *mg );
+
 /* Measurement hook (no reference counterpart): a plain device-to-device copy
  * kernel over two buffers of `bytes` bytes on the current device, run reps
  * times after one warm-up launch; returns the mean milliseconds per copy.  It
