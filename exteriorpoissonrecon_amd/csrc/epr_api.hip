@@ -52,7 +52,11 @@ struct Ctx : public CtxBase
 {
 	static constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD , NV = DimC< Dim >::NDNI , CoDim = 2;
 	// levels at or below this many nodes run all their iterations in one workgroup
-	static constexpr long long SmallLevel = Dim==3 ? 800 : 700;
+	// levels up to this many nodes (and not tiny) run all their iterations in one
+	// workgroup; larger ones use graph-replayed colour launches (EPR_SMALL_LEVEL
+	// overrides, for tuning).  Measured on MI355X: the one-workgroup kernel wins
+	// for 125 nodes (3D level 2), colour launches for 729 (3D) and 625 (4D).
+	long long SmallLevel = getenv( "EPR_SMALL_LEVEL" ) ? atoll( getenv( "EPR_SMALL_LEVEL" ) ) : ( Dim==3 ? 200 : 100 );
 	static constexpr size_t LdsLimit = 160*1024;   // gfx950 LDS per workgroup
 	bool ldsAttrSet = false , tinyAttrSet = false;
 	// levels with at most this many elements use KGsTiny (EPR_TINY_ELEMENTS overrides, for tuning)
@@ -214,11 +218,35 @@ struct Ctx : public CtxBase
 		return s;
 	}
 
+	// T = P^T M of a fine level into dGalT, one launch per parity class
+	template< int CLS > void galerkinTClass( const double *Mf , int fres )
+	{
+		ColourSet< Dim > cs ; cs.set( fres , CLS );
+		KGalerkinTCls< Dim , CLS ><<< Blocks( cs.total , 64 ) , 64 , 0 , stream >>>( Mf , dGalT.p , fres/2 , cs );
+	}
+	template< int ... Cs > void galerkinTAll( const double *Mf , int fres , std::integer_sequence< int , Cs... > ){ ( galerkinTClass< Cs >( Mf , fres ) , ... ); }
+	void galerkinT( const double *Mf , int fres ){ galerkinTAll( Mf , fres , std::make_integer_sequence< int , C >{} ); }
+
 	template< int KR > void launchEstSplat( TreeGrids< Dim > T )
 	{
 		int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= 2*KR+2;
-		long long tot = (long long)ncell*(D+1)*taps;
-		KEstSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p );
+		// depths whose corner values fit in LDS are summed per workgroup first
+		int t = 1;
+		for( ; t<=D+1 ; t++ )
+		{
+			long long corners = 1 ; for( int d=0 ; d<Dim ; d++ ) corners *= ( 1LL<<(t-1) ) + 1;
+			const size_t lds = (size_t)corners*NV*sizeof(double);
+			if( lds>LdsLimit ) break;
+			if( lds>64*1024 ) HIPCHECK( hipFuncSetAttribute( (const void*)KEstSplatLds< Dim , KR > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) );
+			const long long items = (long long)ncell*taps;
+			const unsigned blocks = (unsigned)std::max< long long >( 1 , std::min< long long >( 256 , items/4096 ) );
+			KEstSplatLds< Dim , KR ><<< blocks , 256 , lds , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p , t );
+		}
+		if( t<=D+1 )
+		{
+			long long tot = (long long)ncell*(D+2-t)*taps;
+			KEstSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p , t );
+		}
 	}
 	template< int KR > void launchSkewSplat( TreeGrids< Dim > T )
 	{
@@ -339,9 +367,9 @@ struct Ctx : public CtxBase
 			{
 				Level &Cl = levels[l] , &Fl = levels[l+1];
 				Cl.R.ensure( (size_t)Cl.N*W ) , Cl.B.ensure( (size_t)Cl.N*W );
-				KGalerkinT< Dim ><<< Blocks( Fl.N*W , 256 ) , 256 , 0 , stream >>>( Fl.R.p , dGalT.p , Cl.res );
+				galerkinT( Fl.R.p , Fl.res );
 				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.R.p , Cl.res );
-				KGalerkinT< Dim ><<< Blocks( Fl.N*W , 256 ) , 256 , 0 , stream >>>( Fl.B.p , dGalT.p , Cl.res );
+				galerkinT( Fl.B.p , Fl.res );
 				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.B.p , Cl.res );
 			}
 			HIPCHECK( hipGetLastError() );

@@ -600,23 +600,15 @@ __global__ void KTouch( const int *cellIdx , int ncell , TreeGrids< Dim > T , in
 
 // Estimator splat (GridSamples.inl:436-491, merged cells): one thread per
 // (cell, tree depth, tap); density statistics added with FP64 atomics.
+// Tap `tap` of cell c at depth t: the corner offset and the B-spline weight;
+// false when the corner lies outside the depth's grid.
 template< int Dim , int KR >
-__global__ void KEstSplat( const double *cellP , const double *cellData , int ncell , TreeGrids< Dim > T , const double *depthScale )
+__device__ __forceinline__ bool EstTap( const double *p , int t , int tap , int *o , double &scl )
 {
-	constexpr int NV = DimC< Dim >::NDNI , Deg = 2*KR+1 , NT = Deg+1;
-	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
-	const int nt = T.maxDepth+1;
-	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-	if( g>=(long long)ncell*nt*taps ) return;
-	const int tap = (int)( g % taps );
-	const int t = (int)( ( g / taps ) % nt ) + 1;
-	const int c = (int)( g / ( (long long)taps*nt ) );
+	constexpr int Deg = 2*KR+1 , NT = Deg+1;
 	const int res = 1<<(t-1);
-	const double *p = cellP + (long long)c*Dim;
-	int o[Dim];
-	double scl = 1.;
-	int tt = tap;
-	int ii[Dim];
+	scl = 1.;
+	int tt = tap , ii[Dim];
 	for( int d=Dim-1 ; d>=0 ; d-- ){ ii[d] = tt%NT ; tt /= NT; }
 	for( int d=0 ; d<Dim ; d++ )
 	{
@@ -625,8 +617,52 @@ __global__ void KEstSplat( const double *cellP , const double *cellData , int nc
 		BSpline< Deg >( p[d]*res - E , v );
 		scl = scl * v[ ii[d] ];
 		o[d] = E - KR + ii[d];
-		if( o[d]<0 || o[d]>res ) return;
+		if( o[d]<0 || o[d]>res ) return false;
 	}
+	return true;
+}
+
+// The coarse depths (few corners, every cell hits them): per-workgroup sums in
+// LDS, one global atomic per corner value and workgroup.
+template< int Dim , int KR >
+__global__ void __launch_bounds__( 256 ) KEstSplatLds( const double *cellP , const double *cellData , int ncell , TreeGrids< Dim > T , const double *depthScale , int t )
+{
+	constexpr int NV = DimC< Dim >::NDNI , NT = 2*KR+2;
+	extern __shared__ double acc[];
+	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
+	int corners = 1 ; for( int d=0 ; d<Dim ; d++ ) corners *= ( 1<<(t-1) ) + 1;
+	for( int i=threadIdx.x ; i<corners*NV ; i+=blockDim.x ) acc[i] = 0;
+	__syncthreads();
+	const double dw = depthScale[t-1];
+	const long long items = (long long)ncell*taps;
+	for( long long g=(long long)blockIdx.x*blockDim.x + threadIdx.x ; g<items ; g+=(long long)gridDim.x*blockDim.x )
+	{
+		const int c = (int)( g / taps ) , tap = (int)( g % taps );
+		int o[Dim];
+		double scl;
+		if( !EstTap< Dim , KR >( cellP + (long long)c*Dim , t , tap , o , scl ) ) continue;
+		double *node = acc + CornerIndex< Dim >( t , o )*NV;
+		for( int k=0 ; k<NV ; k++ ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
+	}
+	__syncthreads();
+	for( int i=threadIdx.x ; i<corners*NV ; i+=blockDim.x ) if( acc[i]!=0 ) atomicAdd( T.data[t]+i , acc[i] );
+}
+
+// Depths tMin..maxDepth+1: one thread per (cell, depth, tap), FP64 atomics.
+template< int Dim , int KR >
+__global__ void KEstSplat( const double *cellP , const double *cellData , int ncell , TreeGrids< Dim > T , const double *depthScale , int tMin )
+{
+	constexpr int NV = DimC< Dim >::NDNI , NT = 2*KR+2;
+	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
+	const int nt = T.maxDepth+2-tMin;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=(long long)ncell*nt*taps ) return;
+	const int tap = (int)( g % taps );
+	const int t = (int)( ( g / taps ) % nt ) + tMin;
+	const int c = (int)( g / ( (long long)taps*nt ) );
+	int o[Dim];
+	double scl;
+	if( !EstTap< Dim , KR >( cellP + (long long)c*Dim , t , tap , o , scl ) ) return;
 	double *node = T.data[t] + CornerIndex< Dim >( t , o )*NV;
 	const double dw = depthScale[t-1];
 	for( int k=0 ; k<NV ; k++ ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
@@ -1100,6 +1136,105 @@ __global__ void __launch_bounds__( 256 ) KGalerkinT( const double *Mf , double *
 		if( d<0 ) break;
 	}
 	T[g] = t;
+}
+
+// ---- T = P^T M_f by fine node, one launch per parity class of the fine node.
+// For a fine node K of parity class CLS (bit d = K_d & 1) every entry m = K + delta
+// of its column has its coarse parents i at compile-time window offsets
+// wo_d = i_d - (K_d>>1) + 1:
+//   K_d even: delta -1 -> {0,1} (1/2 each), 0 -> {1} (1), +1 -> {1,2} (1/2 each)
+//   K_d odd : delta -1 -> {1} (1), 0 -> {1,2} (1/2 each), +1 -> {2} (1)
+// so each thread scatters its column into W register accumulators; the terms
+// of every output arrive in increasing m (the Eigen product order of
+// KGalerkinT's gather) and the values are identical.
+template< int Dim > struct GalerkinShape
+{
+	static constexpr int W = DimC< Dim >::W;
+	static constexpr int Pow3( int e ){ return e==0 ? 1 : 3*Pow3( e-1 ); }
+	static constexpr int Digit( int w , int d ){ return ( w / Pow3( Dim-1-d ) ) % 3; }
+	// number of parents of the entry in dimension d
+	static constexpr int Opts( int cls , int wm , int d ){ const int p = ( cls>>d )&1 , dl = Digit( wm , d ) - 1 ; return p==0 ? ( dl==0 ? 1 : 2 ) : ( dl==0 ? 2 : 1 ); }
+	// window offset of option `opt` in dimension d
+	static constexpr int Off( int cls , int wm , int d , int opt )
+	{
+		const int p = ( cls>>d )&1 , dl = Digit( wm , d ) - 1;
+		if( p==0 ) return dl==-1 ? opt : ( dl==0 ? 1 : 1+opt );
+		return dl==-1 ? 1 : ( dl==0 ? 1+opt : 2 );
+	}
+	static constexpr int NumCombos( int cls , int wm ){ int n = 1 ; for( int d=0 ; d<Dim ; d++ ) n *= Opts( cls , wm , d ) ; return n; }
+	// option of dimension d in combination `combo` (mixed radix, first dimension most significant)
+	static constexpr int OptOf( int cls , int wm , int combo , int d ){ int r = 1 ; for( int e=Dim-1 ; e>d ; e-- ) r *= Opts( cls , wm , e ) ; return ( combo / r ) % Opts( cls , wm , d ); }
+	static constexpr int Wo( int cls , int wm , int combo ){ int w = 0 ; for( int d=0 ; d<Dim ; d++ ) w = w*3 + Off( cls , wm , d , OptOf( cls , wm , combo , d ) ) ; return w; }
+	// the weight of an entry with two parents in a dimension is 1/2 there; an
+	// entry with one parent has weight 1 (m even, m = 2i)
+	static constexpr double Weight( int cls , int wm ){ double w = 1 ; for( int d=0 ; d<Dim ; d++ ) if( Opts( cls , wm , d )==2 ) w *= 0.5 ; return w; }
+};
+
+template< int Dim , int CLS , int WM , int ... Cs >
+__device__ __forceinline__ void GalerkinScatter( double mv , const int *K , int cres , double *acc , std::integer_sequence< int , Cs... > )
+{
+	typedef GalerkinShape< Dim > G;
+	constexpr double pmi = G::Weight( CLS , WM );
+	( [&]{
+		constexpr int wo = G::Wo( CLS , WM , Cs );
+		bool in = true;
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			const int off = G::Off( CLS , WM , d , G::OptOf( CLS , WM , Cs , d ) );
+			const int i = ( K[d]>>1 ) + off - 1;
+			in = in && i>=0 && i<=cres;
+		}
+		if( in ) acc[wo] = __dadd_rn( acc[wo] , __dmul_rn( pmi , mv ) );
+	}() , ... );
+}
+
+template< int Dim , int CLS , int ... WMs >
+__device__ __forceinline__ void GalerkinColumn( const double *col , const int *K , int fres , int cres , double *acc , std::integer_sequence< int , WMs... > )
+{
+	typedef GalerkinShape< Dim > G;
+	( [&]{
+		bool in = true;
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ ){ const int m = K[d] + G::Digit( WMs , d ) - 1 ; in = in && m>=0 && m<=fres; }
+		if( !in ) return;
+		const double mv = col[WMs];
+		if( mv!=0 ) GalerkinScatter< Dim , CLS , WMs >( mv , K , cres , acc , std::make_integer_sequence< int , G::NumCombos( CLS , WMs ) >{} );
+	}() , ... );
+}
+
+// 64 fine nodes per workgroup: their columns are loaded into LDS with
+// contiguous reads, scattered from LDS into register accumulators, and the
+// results leave through LDS with contiguous writes (node-major T[k*W + wo]).
+template< int Dim , int CLS >
+__global__ void __launch_bounds__( 64 ) KGalerkinTCls( const double *Mf , double *T , int cres , ColourSet< Dim > cs )
+{
+	constexpr int W = DimC< Dim >::W;
+	__shared__ double buf[ 64*W ];
+	__shared__ long long knode[64];
+	const int lane = threadIdx.x , fres = 2*cres;
+	const long long g = (long long)blockIdx.x*64 + lane;
+	int K[Dim];
+	long long k = -1;
+	if( g<cs.total )
+	{
+		cs.node( g , K );
+		k = 0;
+		for( int d=0 ; d<Dim ; d++ ) k = k*( fres+1 ) + K[d];
+	}
+	knode[lane] = k;
+	__syncthreads();
+	for( int e=lane ; e<64*W ; e+=64 ){ const long long kn = knode[ e/W ] ; buf[e] = kn>=0 ? Mf[ kn*W + e%W ] : 0.; }
+	__syncthreads();
+	double acc[W];
+#pragma unroll
+	for( int w=0 ; w<W ; w++ ) acc[w] = 0;
+	if( k>=0 ) GalerkinColumn< Dim , CLS >( buf + lane*W , K , fres , cres , acc , std::make_integer_sequence< int , W >{} );
+	__syncthreads();
+#pragma unroll
+	for( int w=0 ; w<W ; w++ ) buf[ lane*W + w ] = acc[w];
+	__syncthreads();
+	for( int e=lane ; e<64*W ; e+=64 ){ const long long kn = knode[ e/W ] ; if( kn>=0 ) T[ kn*W + e%W ] = buf[e]; }
 }
 
 template< int Dim >
