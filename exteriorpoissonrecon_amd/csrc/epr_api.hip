@@ -51,7 +51,6 @@ template< int Dim >
 struct Ctx : public CtxBase
 {
 	static constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD , NV = DimC< Dim >::NDNI , CoDim = 2;
-	// levels at or below this many nodes run all their iterations in one workgroup
 	// levels up to this many nodes (and not tiny) run all their iterations in one
 	// workgroup; larger ones use graph-replayed colour launches (EPR_SMALL_LEVEL
 	// overrides, for tuning).  Measured on MI355X: the one-workgroup kernel wins
@@ -79,12 +78,13 @@ struct Ctx : public CtxBase
 		int res;
 		long long N;
 		long long stride[Dim];
+		ColLayout< Dim > lay;              // colour-grouped column order of R and B
 		DevBuf< double > x , y , R , B , K;
 		hipGraphExec_t graph = nullptr;    // captured Gauss-Seidel launches of this level
 		int graphIters = 0;
 		const double *graphX = nullptr , *graphR = nullptr;
 		Level( void ){}
-		Level( Level &&o ) noexcept : res( o.res ) , N( o.N ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , graph( o.graph ) , graphIters( o.graphIters )
+		Level( Level &&o ) noexcept : res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , graph( o.graph ) , graphIters( o.graphIters )
 		{
 			for( int d=0 ; d<Dim ; d++ ) stride[d] = o.stride[d];
 			o.graph = nullptr;
@@ -113,6 +113,7 @@ struct Ctx : public CtxBase
 		{
 			Level &L = levels[l];
 			L.res = 1<<l;
+			L.lay.set( L.res );
 			L.N = 1;
 			for( int d=0 ; d<Dim ; d++ ) L.N *= L.res+1;
 			long long s = 1;
@@ -133,6 +134,7 @@ struct Ctx : public CtxBase
 		v.res = L.res , v.N = L.N , v.x = L.x.p , v.y = L.y.p , v.R = L.R.p , v.B = L.B.p , v.K = L.K.p;
 		v.sW = std::pow( (double)L.res , 4.-Dim ) , v.dW = prm.dWeight;
 		for( int d=0 ; d<Dim ; d++ ) v.stride[d] = L.stride[d];
+		v.lay = L.lay;
 		return v;
 	}
 
@@ -219,13 +221,13 @@ struct Ctx : public CtxBase
 	}
 
 	// T = P^T M of a fine level into dGalT, one launch per parity class
-	template< int CLS > void galerkinTClass( const double *Mf , int fres )
+	template< int CLS > void galerkinTClass( const double *Mf , const Level &Fl )
 	{
-		ColourSet< Dim > cs ; cs.set( fres , CLS );
-		KGalerkinTCls< Dim , CLS ><<< Blocks( cs.total , 64 ) , 64 , 0 , stream >>>( Mf , dGalT.p , fres/2 , cs );
+		ColourSet< Dim > cs ; cs.set( Fl.res , CLS );
+		KGalerkinTCls< Dim , CLS ><<< Blocks( cs.total , 64 ) , 64 , 0 , stream >>>( Mf , dGalT.p , Fl.res/2 , cs , Fl.lay.cstart[CLS] );
 	}
-	template< int ... Cs > void galerkinTAll( const double *Mf , int fres , std::integer_sequence< int , Cs... > ){ ( galerkinTClass< Cs >( Mf , fres ) , ... ); }
-	void galerkinT( const double *Mf , int fres ){ galerkinTAll( Mf , fres , std::make_integer_sequence< int , C >{} ); }
+	template< int ... Cs > void galerkinTAll( const double *Mf , const Level &Fl , std::integer_sequence< int , Cs... > ){ ( galerkinTClass< Cs >( Mf , Fl ) , ... ); }
+	void galerkinT( const double *Mf , const Level &Fl ){ galerkinTAll( Mf , Fl , std::make_integer_sequence< int , C >{} ); }
 
 	template< int KR > void launchEstSplat( TreeGrids< Dim > T )
 	{
@@ -341,7 +343,7 @@ struct Ctx : public CtxBase
 			Level &F = levels[D];
 			F.R.ensure( (size_t)F.N*W ) , F.B.ensure( (size_t)F.N*W );
 			// B
-			KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p );
+			KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay );
 			// R
 			if( prm.sWeight>0 )
 			{
@@ -358,7 +360,7 @@ struct Ctx : public CtxBase
 				dScale.ensure( 2 );
 				KPartialSum<<< nb , 256 , 0 , stream >>>( dMeasure.p , (long long)n , dPartials.p );
 				KMeanMeasure<<< 1 , 1 , 0 , stream >>>( dPartials.p , nb , (long long)n , prm.sWeight , dScale.p );
-				KAssembleR< Dim ><<< Blocks( F.N*W , 256 ) , 256 , 0 , stream >>>( dPos.p , dCellMap.p , dCellStart.p , dCellSamples.p , dRW.p , res , dScale.p , F.R.p );
+				KAssembleR< Dim ><<< Blocks( F.N*W , 256 ) , 256 , 0 , stream >>>( dPos.p , dCellMap.p , dCellStart.p , dCellSamples.p , dRW.p , res , dScale.p , F.R.p , F.lay );
 			}
 			else F.R.zero( stream );
 			HIPCHECK( hipGetLastError() );
@@ -367,10 +369,10 @@ struct Ctx : public CtxBase
 			{
 				Level &Cl = levels[l] , &Fl = levels[l+1];
 				Cl.R.ensure( (size_t)Cl.N*W ) , Cl.B.ensure( (size_t)Cl.N*W );
-				galerkinT( Fl.R.p , Fl.res );
-				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.R.p , Cl.res );
-				galerkinT( Fl.B.p , Fl.res );
-				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.B.p , Cl.res );
+				galerkinT( Fl.R.p , Fl );
+				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.R.p , Cl.res , Cl.lay );
+				galerkinT( Fl.B.p , Fl );
+				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.B.p , Cl.res , Cl.lay );
 			}
 			HIPCHECK( hipGetLastError() );
 		}
@@ -445,7 +447,7 @@ struct Ctx : public CtxBase
 				for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
 				{
 					ColourSet< Dim > cs ; cs.set( L.res , c );
-					KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs );
+					KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , v.lay.cstart[c] );
 				}
 				HIPCHECK( hipStreamEndCapture( stream , &g ) );
 				HIPCHECK( hipGraphInstantiate( &L.graph , g , nullptr , nullptr , 0 ) );
@@ -623,27 +625,13 @@ struct Ctx : public CtxBase
 			return;
 		}
 		if( which!='R' && which!='B' ) throw std::string( "operator must be R, B or P" );
-		// y = M x, from the columns (host gather; test hook only)
 		const Level &L = levels[l];
-		std::vector< double > M( (size_t)L.N*W );
-		( which=='R' ? L.R : L.B ).download( M.data() , M.size() , stream );
+		DevBuf< double > a , b;
+		a.upload( x , L.N , stream );
+		b.alloc( L.N );
+		KApplyCols< Dim ><<< Blocks( L.N , 256 ) , 256 , 0 , stream >>>( which=='R' ? L.R.p : L.B.p , a.p , b.p , view( l ) );
+		b.download( y , L.N , stream );
 		HIPCHECK( hipStreamSynchronize( stream ) );
-		for( long long i=0 ; i<L.N ; i++ ) y[i] = 0;
-		for( long long f=0 ; f<L.N ; f++ )
-		{
-			int F[Dim];
-			long long t = f;
-			for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = (int)( t%( L.res+1 ) ) ; t /= L.res+1; }
-			for( int w=0 ; w<W ; w++ )
-			{
-				double v = M[ (size_t)f*W+w ];
-				if( v==0 ) continue;
-				long long r = f;
-				int tw = w;
-				for( int d=Dim-1 ; d>=0 ; d-- ){ int o = tw%3-1 ; tw /= 3 ; r += o*L.stride[d]; }
-				y[r] += v*x[f];
-			}
-		}
 	}
 	size_t getArray( const std::string &name , double *out , size_t cap )
 	{

@@ -30,6 +30,35 @@ template< int Dim > __device__ __forceinline__ double WA( int j , int k ){ if co
 template< int Dim > __device__ __forceinline__ int PK( int p ){ if constexpr( Dim==3 ) return wedge3::PK[p]; else return wedge4::PK[p]; }
 template< int Dim > __device__ __forceinline__ int PL( int p ){ if constexpr( Dim==3 ) return wedge3::PL[p]; else return wedge4::PL[p]; }
 
+// Column layout of R and B: the columns (3^Dim values each) of the nodes of
+// one colour (parity class, ExteriorPoisson.h:161-169) are contiguous, colour
+// after colour, in ColourSet order within a colour -- a colour phase of the
+// sweep reads one contiguous block.
+template< int Dim >
+struct ColLayout
+{
+	int res;
+	long long cstart[ (1<<Dim)+1 ];
+	__device__ __host__ long long col( const int *F ) const
+	{
+		int c = 0;
+		long long k = 0;
+		for( int d=0 ; d<Dim ; d++ ) c |= ( F[d]&1 )<<d;
+		for( int d=0 ; d<Dim ; d++ ){ const int cnt = ( res - ( ( c>>d )&1 ) )/2 + 1 ; k = k*cnt + ( F[d]>>1 ); }
+		return cstart[c] + k;
+	}
+	void set( int r )
+	{
+		res = r , cstart[0] = 0;
+		for( int c=0 ; c<(1<<Dim) ; c++ )
+		{
+			long long t = 1;
+			for( int d=0 ; d<Dim ; d++ ) t *= ( r - ( ( c>>d )&1 ) )/2 + 1;
+			cstart[c+1] = cstart[c] + t;
+		}
+	}
+};
+
 // One level as the node kernels see it.
 template< int Dim >
 struct LevelView
@@ -41,6 +70,7 @@ struct LevelView
 	double sW;      // wedge scale res^(4-Dim)
 	double dW;      // Dirichlet weight
 	long long stride[Dim];
+	ColLayout< Dim > lay;   // column position of a node in R and B
 };
 
 enum { MODE_GS = 0 , MODE_QFIT = 1 , MODE_ENERGY = 2 };
@@ -136,7 +166,7 @@ __device__ __forceinline__ bool NodeStep( double q20 , double q02 , double q11 ,
 // incident element (wedge terms) + a share of the 3^Dim window (R, K, B terms).
 // All lanes return the group totals.
 template< int Dim , int MODE , bool PREFETCH = true >
-__device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int *F , long long i , int lane , double acc[5] )
+__device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int *F , long long i , long long col , int lane , double acc[5] )
 {
 	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W;
 	double q20 = 0 , q02 = 0 , q11 = 0 , q10 = 0 , q01 = 0;
@@ -161,7 +191,7 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int
 			in_ = in_ && g_>=0 && g_<=L.res; \
 			r_ += o_*L.stride[d]; \
 		} \
-		if( in_ ) wx[m] = L.x[r_] , wy[m] = L.y[r_] , wR[m] = L.R[ i*W+w_ ] , wB[m] = L.B[ i*W+w_ ] , wK[m] = L.K[ cse*W + w_ ]; \
+		if( in_ ) wx[m] = L.x[r_] , wy[m] = L.y[r_] , wR[m] = L.R[ col*W+w_ ] , wB[m] = L.B[ col*W+w_ ] , wK[m] = L.K[ cse*W + w_ ]; \
 		else      wx[m] = wy[m] = wR[m] = wB[m] = wK[m] = 0; \
 	}
 	if constexpr( PREFETCH )
@@ -235,6 +265,17 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int
 	acc[0] = q20 , acc[1] = q02 , acc[2] = q11 , acc[3] = q10 , acc[4] = q01;
 }
 
+// Workgroups are dispatched to the 8 XCDs round robin (workgroup b to XCD
+// b mod 8), each XCD with its own L2.  This maps the workgroups of one XCD to
+// one contiguous range of the work, so neighbouring rows and planes -- which
+// share their x, y cache lines -- stay in the same L2.  A bijection of [0, n).
+__device__ __forceinline__ unsigned XcdBlock( unsigned b , unsigned n )
+{
+	constexpr unsigned X = 8;
+	const unsigned q = n / X , r = n % X , xcd = b % X , local = b / X;
+	return xcd<r ? xcd*( q+1 ) + local : r*( q+1 ) + ( xcd-r )*q + local;
+}
+
 // Per-dimension counts of the nodes of a colour and the k-th node's index.
 template< int Dim >
 struct ColourSet
@@ -264,10 +305,10 @@ template< int Dim > __device__ __forceinline__ void Unflatten( long long i , int
 
 // One colour phase of the paired Gauss-Seidel sweep (one launch per colour).
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSet< Dim > cs )
+__global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSet< Dim > cs , long long colBase )
 {
 	constexpr int C = DimC< Dim >::C;
-	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	const long long g = (long long)XcdBlock( blockIdx.x , gridDim.x )*blockDim.x + threadIdx.x;
 	const long long k = g / C;
 	const int lane = (int)( g % C );
 	if( k>=cs.total ) return;
@@ -275,7 +316,7 @@ __global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSe
 	cs.node( k , F );
 	const long long i = FlatIndex< Dim >( F , L.stride );
 	double q[5];
-	NodeGroup< Dim , MODE_GS , Dim==3 >( L , F , i , lane , q );
+	NodeGroup< Dim , MODE_GS , Dim==3 >( L , F , i , colBase+k , lane , q );
 	if( lane==0 )
 	{
 		double dx , dy;
@@ -299,7 +340,7 @@ __global__ void __launch_bounds__( 512 ) KGsSmall( LevelView< Dim > L , int iter
 			cs.node( k , F );
 			const long long i = FlatIndex< Dim >( F , L.stride );
 			double q[5];
-			NodeGroup< Dim , MODE_GS >( L , F , i , lane , q );
+			NodeGroup< Dim , MODE_GS >( L , F , i , L.lay.cstart[c]+k , lane , q );
 			if( lane==0 )
 			{
 				double dx , dy;
@@ -322,7 +363,7 @@ __global__ void __launch_bounds__( 256 ) KQFit( LevelView< Dim > L , double *q )
 	int F[Dim];
 	Unflatten< Dim >( i , L.res , F );
 	double a[5];
-	NodeGroup< Dim , MODE_QFIT , Dim==3 >( L , F , i , lane , a );
+	NodeGroup< Dim , MODE_QFIT , Dim==3 >( L , F , i , L.lay.col( F ) , lane , a );
 	if( lane==0 ) for( int k=0 ; k<5 ; k++ ) q[5*i+k] = a[k];
 }
 
@@ -341,7 +382,7 @@ __global__ void __launch_bounds__( 256 ) KEnergy( LevelView< Dim > L , double *p
 	{
 		int F[Dim];
 		Unflatten< Dim >( i , L.res , F );
-		NodeGroup< Dim , MODE_ENERGY , Dim==3 >( L , F , i , lane , a );
+		NodeGroup< Dim , MODE_ENERGY , Dim==3 >( L , F , i , L.lay.col( F ) , lane , a );
 	}
 	if( lane==0 ) for( int k=0 ; k<4 ; k++ ) sh[ threadIdx.x/C ][k] = a[k];
 	__syncthreads();
@@ -419,6 +460,31 @@ __global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf
 }
 
 // out = a + b
+// y = M x for a level operator stored by columns (column f holds M(f+o_w, f)):
+// y_r = sum over the window of r of M(r, f) x_f with f = r - o_w, taken in
+// increasing f (the accumulation order of a column-major sparse product).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KApplyCols( const double *M , const double *x , double *y , LevelView< Dim > L )
+{
+	constexpr int W = DimC< Dim >::W;
+	const long long r = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( r>=L.N ) return;
+	int F[Dim];
+	Unflatten< Dim >( r , L.res , F );
+	double s = 0;
+	for( int w=W-1 ; w>=0 ; w-- )
+	{
+		int t = w , Ff[Dim];
+		long long f = r;
+		bool in = true;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ const int o = t%3 - 1 ; t /= 3 ; Ff[d] = F[d]-o ; in = in && Ff[d]>=0 && Ff[d]<=L.res ; f -= o*L.stride[d]; }
+		if( !in ) continue;
+		const double v = M[ L.lay.col( Ff )*W + w ];
+		if( v!=0 ) s = __dadd_rn( s , __dmul_rn( v , x[f] ) );
+	}
+	y[r] = s;
+}
+
 // Plain device copy (16 B per lane per iteration), the calibration kernel for HBM counters.
 __global__ void __launch_bounds__( 256 ) KCopy( const double2 *__restrict__ a , double2 *__restrict__ b , long long n )
 {
@@ -437,80 +503,6 @@ __global__ void KScale2( double *x , double *y , long long n , const double *s )
 	if( i<n ){ x[i] *= s[0] ; y[i] *= s[0]; }
 }
 
-// Galerkin coarsening Mc = (P^T Mf) P with the accumulation order of Eigen's
-// conservative sparse product (SystemEnergy.h:472-473): for coarse (i,j),
-// sum over fine k of P(.,j) increasing of [ sum over fine m increasing of
-// P(m,i) Mf(m,k) ] * P(k,j); zero terms skipped.
-template< int Dim >
-__global__ void __launch_bounds__( 256 ) KGalerkin( const double *Mf , double *Mc , int cres )
-{
-	constexpr int W = DimC< Dim >::W;
-	const int fres = 2*cres;
-	long long nc = 1 ; for( int d=0 ; d<Dim ; d++ ) nc *= cres+1;
-	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-	if( g>=nc*W ) return;
-	const long long j = g / W;
-	const int wi = (int)( g % W );
-	int J[Dim] , I[Dim];
-	Unflatten< Dim >( j , cres , J );
-	{
-		int t = wi;
-		for( int d=Dim-1 ; d>=0 ; d-- ){ I[d] = J[d] + t%3 - 1 ; t /= 3; }
-	}
-	for( int d=0 ; d<Dim ; d++ ) if( I[d]<0 || I[d]>cres ){ Mc[g] = 0 ; return; }
-	long long fstride[Dim];
-	{
-		long long s = 1;
-		for( int d=Dim-1 ; d>=0 ; d-- ){ fstride[d] = s ; s *= fres+1; }
-	}
-	int klo[Dim] , khi[Dim];
-	for( int d=0 ; d<Dim ; d++ )
-	{
-		klo[d] = max( max( 2*J[d]-1 , 2*I[d]-2 ) , 0 );
-		khi[d] = min( min( 2*J[d]+1 , 2*I[d]+2 ) , fres );
-		if( klo[d]>khi[d] ){ Mc[g] = 0 ; return; }
-	}
-	double acc = 0;
-	bool any = false;
-	int K[Dim];
-	for( int d=0 ; d<Dim ; d++ ) K[d] = klo[d];
-	while( true )
-	{
-		double pkj = 1;
-		long long k = 0;
-		for( int d=0 ; d<Dim ; d++ ){ if( K[d]!=2*J[d] ) pkj *= 0.5 ; k += K[d]*fstride[d]; }
-		int mlo[Dim] , mhi[Dim] , M[Dim];
-		bool ok = true;
-		for( int d=0 ; d<Dim ; d++ )
-		{
-			mlo[d] = max( max( 2*I[d]-1 , K[d]-1 ) , 0 );
-			mhi[d] = min( min( 2*I[d]+1 , K[d]+1 ) , fres );
-			if( mlo[d]>mhi[d] ) ok = false;
-			M[d] = mlo[d];
-		}
-		if( ok )
-		{
-			double t = 0;
-			bool tany = false;
-			while( true )
-			{
-				double pmi = 1;
-				int wm = 0;
-				for( int d=0 ; d<Dim ; d++ ){ if( M[d]!=2*I[d] ) pmi *= 0.5 ; wm = wm*3 + ( M[d]-K[d]+1 ); }
-				const double mv = Mf[ k*W + wm ];
-				if( mv!=0 ){ if( tany ) t += pmi*mv ; else t = pmi*mv , tany = true; }
-				int d = Dim-1;
-				for( ; d>=0 ; d-- ){ if( ++M[d]<=mhi[d] ) break ; M[d] = mlo[d]; }
-				if( d<0 ) break;
-			}
-			if( tany ){ if( any ) acc += t*pkj ; else acc = t*pkj , any = true; }
-		}
-		int d = Dim-1;
-		for( ; d>=0 ; d-- ){ if( ++K[d]<=khi[d] ) break ; K[d] = klo[d]; }
-		if( d<0 ) break;
-	}
-	Mc[g] = acc;
-}
 
 //////////////////////////////////////////////////////////////////////////
 // Build phase
@@ -947,7 +939,7 @@ __global__ void __launch_bounds__( 256 ) KAssembleB( const double *S , const dou
 // adjacent to both in cell order, within a cell over its samples in order.
 template< int Dim >
 __global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const int *cellMap , const int *cellStart , const int *cellSamples ,
-                                                      const double *rweight , int res , const double *scale , double *R )
+                                                      const double *rweight , int res , const double *scale , double *R , ColLayout< Dim > lay )
 {
 	constexpr int W = DimC< Dim >::W;
 	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= res+1;
@@ -957,6 +949,7 @@ __global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const i
 	const int w = (int)( g % W );
 	int F[Dim] , G[Dim];
 	Unflatten< Dim >( f , res , F );
+	const long long out = lay.col( F )*W + w;
 	{
 		int t = w;
 		for( int d=Dim-1 ; d>=0 ; d-- ){ G[d] = F[d] + t%3 - 1 ; t /= 3; }
@@ -964,7 +957,7 @@ __global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const i
 	int lo[Dim] , hi[Dim] , E[Dim];
 	for( int d=0 ; d<Dim ; d++ )
 	{
-		if( G[d]<0 || G[d]>res ){ R[g] = 0 ; return; }
+		if( G[d]<0 || G[d]>res ){ R[out] = 0 ; return; }
 		lo[d] = max( 0 , max( F[d] , G[d] )-1 );
 		hi[d] = min( res , min( F[d] , G[d] )+1 );
 		E[d] = lo[d];
@@ -1004,9 +997,9 @@ __global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const i
 		for( ; d>=0 ; d-- ){ if( ++E[d]<hi[d] ) break ; E[d] = lo[d]; }
 		if( d<0 ) break;
 	}
-	if( !any ){ R[g] = 0 ; return; }
+	if( !any ){ R[out] = 0 ; return; }
 	const double v = diag ? acc+acc : acc;
-	R[g] = v * scale[0] * scale[1];
+	R[out] = v * scale[0] * scale[1];
 }
 
 // ---- B assembly, one thread per finest node f1 and all 3^Dim outputs of its
@@ -1072,7 +1065,7 @@ __device__ __forceinline__ void BOutputs( const double (&S)[1<<Dim][DimC< Dim >:
 }
 
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int res , double scale , double *B )
+__global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int res , double scale , double *B , ColLayout< Dim > lay )
 {
 	constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD;
 	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= res+1;
@@ -1090,53 +1083,9 @@ __global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int r
 #pragma unroll
 		for( int c=0 ; c<CD ; c++ ) Sv[b][c] = in ? S[ e*CD + c ] : 0.;
 	}
-	BOutputs< Dim >( Sv , F , res , scale , B + f1*W , std::make_integer_sequence< int , W >{} );
+	BOutputs< Dim >( Sv , F , res , scale , B + lay.col( F )*W , std::make_integer_sequence< int , W >{} );
 }
 
-// ---- Galerkin coarsening in two passes (same accumulation order as KGalerkin):
-// T(i,k) = sum over fine m increasing of P(m,i) Mf(m,k), stored per fine column
-// k for the 3^Dim coarse i = (k>>1)+o; then Mc(i,j) = sum over children k of j
-// increasing of T(i,k) P(k,j).
-template< int Dim >
-__global__ void __launch_bounds__( 256 ) KGalerkinT( const double *Mf , double *T , int cres )
-{
-	constexpr int W = DimC< Dim >::W;
-	const int fres = 2*cres;
-	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= fres+1;
-	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-	if( g>=nf*W ) return;
-	const long long k = g / W;
-	const int wo = (int)( g % W );
-	int K[Dim] , I[Dim] , lo[Dim] , hi[Dim] , M[Dim];
-	Unflatten< Dim >( k , fres , K );
-	{
-		int t = wo;
-		for( int d=Dim-1 ; d>=0 ; d-- ){ I[d] = ( K[d]>>1 ) + t%3 - 1 ; t /= 3; }
-	}
-	for( int d=0 ; d<Dim ; d++ )
-	{
-		if( I[d]<0 || I[d]>cres ){ T[g] = 0 ; return; }
-		lo[d] = max( max( 2*I[d]-1 , K[d]-1 ) , 0 );
-		hi[d] = min( min( 2*I[d]+1 , K[d]+1 ) , fres );
-		if( lo[d]>hi[d] ){ T[g] = 0 ; return; }
-		M[d] = lo[d];
-	}
-	double t = 0;
-	bool any = false;
-	const double *col = Mf + k*W;
-	while( true )
-	{
-		double pmi = 1;
-		int wm = 0;
-		for( int d=0 ; d<Dim ; d++ ){ if( M[d]!=2*I[d] ) pmi *= 0.5 ; wm = wm*3 + ( M[d]-K[d]+1 ); }
-		const double mv = col[wm];
-		if( mv!=0 ){ const double pr = __dmul_rn( pmi , mv ) ; t = any ? __dadd_rn( t , pr ) : pr ; any = true; }
-		int d = Dim-1;
-		for( ; d>=0 ; d-- ){ if( ++M[d]<=hi[d] ) break ; M[d] = lo[d]; }
-		if( d<0 ) break;
-	}
-	T[g] = t;
-}
 
 // ---- T = P^T M_f by fine node, one launch per parity class of the fine node.
 // For a fine node K of parity class CLS (bit d = K_d & 1) every entry m = K + delta
@@ -1207,7 +1156,7 @@ __device__ __forceinline__ void GalerkinColumn( const double *col , const int *K
 // contiguous reads, scattered from LDS into register accumulators, and the
 // results leave through LDS with contiguous writes (node-major T[k*W + wo]).
 template< int Dim , int CLS >
-__global__ void __launch_bounds__( 64 ) KGalerkinTCls( const double *Mf , double *T , int cres , ColourSet< Dim > cs )
+__global__ void __launch_bounds__( 64 ) KGalerkinTCls( const double *Mf , double *T , int cres , ColourSet< Dim > cs , long long colBase )
 {
 	constexpr int W = DimC< Dim >::W;
 	__shared__ double buf[ 64*W ];
@@ -1224,7 +1173,9 @@ __global__ void __launch_bounds__( 64 ) KGalerkinTCls( const double *Mf , double
 	}
 	knode[lane] = k;
 	__syncthreads();
-	for( int e=lane ; e<64*W ; e+=64 ){ const long long kn = knode[ e/W ] ; buf[e] = kn>=0 ? Mf[ kn*W + e%W ] : 0.; }
+	// the block's columns are consecutive in the colour-grouped layout
+	const long long col0 = colBase + (long long)blockIdx.x*64;
+	for( int e=lane ; e<64*W ; e+=64 ){ const long long kn = knode[ e/W ] ; buf[e] = kn>=0 ? Mf[ col0*W + e ] : 0.; }
 	__syncthreads();
 	double acc[W];
 #pragma unroll
@@ -1238,7 +1189,7 @@ __global__ void __launch_bounds__( 64 ) KGalerkinTCls( const double *Mf , double
 }
 
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *Mc , int cres )
+__global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *Mc , int cres , ColLayout< Dim > lay )
 {
 	constexpr int W = DimC< Dim >::W;
 	const int fres = 2*cres;
@@ -1249,6 +1200,7 @@ __global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *M
 	const int wi = (int)( g % W );
 	int J[Dim] , I[Dim] , lo[Dim] , hi[Dim] , K[Dim];
 	Unflatten< Dim >( j , cres , J );
+	const long long out = lay.col( J )*W + wi;
 	{
 		int t = wi;
 		for( int d=Dim-1 ; d>=0 ; d-- ){ I[d] = J[d] + t%3 - 1 ; t /= 3; }
@@ -1260,10 +1212,10 @@ __global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *M
 	}
 	for( int d=0 ; d<Dim ; d++ )
 	{
-		if( I[d]<0 || I[d]>cres ){ Mc[g] = 0 ; return; }
+		if( I[d]<0 || I[d]>cres ){ Mc[out] = 0 ; return; }
 		lo[d] = max( max( 2*J[d]-1 , 2*I[d]-2 ) , 0 );
 		hi[d] = min( min( 2*J[d]+1 , 2*I[d]+2 ) , fres );
-		if( lo[d]>hi[d] ){ Mc[g] = 0 ; return; }
+		if( lo[d]>hi[d] ){ Mc[out] = 0 ; return; }
 		K[d] = lo[d];
 	}
 	double acc = 0;
@@ -1280,7 +1232,7 @@ __global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *M
 		for( ; d>=0 ; d-- ){ if( ++K[d]<=hi[d] ) break ; K[d] = lo[d]; }
 		if( d<0 ) break;
 	}
-	Mc[g] = acc;
+	Mc[out] = acc;
 }
 
 // All iterations of a small level in one workgroup with the coefficients (and,
@@ -1317,7 +1269,7 @@ __global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLds( LevelView<
 			cs.node( k , F );
 			const long long i = FlatIndex< Dim >( F , L.stride );
 			double q[5];
-			NodeGroup< Dim , MODE_GS >( Ls , F , i , lane , q );
+			NodeGroup< Dim , MODE_GS >( Ls , F , i , L.lay.cstart[c]+k , lane , q );
 			if( lane==0 )
 			{
 				double dx , dy;
@@ -1439,15 +1391,7 @@ __global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView<
 	double *cb = scr + wave*TinyLds< Dim >::PerWave , *wb = cb + 2*C;
 	for( int i=threadIdx.x ; i<N ; i+=blockDim.x ) sx[i] = L.x[i] , sy[i] = L.y[i];
 	for( int i=threadIdx.x ; i<J*J ; i+=blockDim.x ) sA[i] = WA< Dim >( i/J , i%J );
-	if constexpr( RB )
-		for( int k=threadIdx.x ; k<N*W ; k+=blockDim.x )
-		{
-			int t = k/W , cse = 0 , F[Dim];
-			for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = t % (res+1) ; t /= res+1; }
-			for( int d=0 ; d<Dim ; d++ ) cse = cse*3 + ( F[d]==0 ? 0 : ( F[d]==res ? 2 : 1 ) );
-			sRK[k] = L.R[k] + L.K[ cse*W + k%W ]*L.dW , sB[k] = L.B[k];
-		}
-	else for( int i=threadIdx.x ; i<W*W ; i+=blockDim.x ) sK[i] = L.K[i];
+	if constexpr( !RB ) for( int i=threadIdx.x ; i<W*W ; i+=blockDim.x ) sK[i] = L.K[i];
 	// node table, sorted by colour: (flat index, boundary case, first incident
 	// element, low/high boundary bits), bit Dim-1-d for coordinate d
 	{
@@ -1472,6 +1416,12 @@ __global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView<
 			int *o = nodeInfo + 4*( start( colour ) + k );
 			o[0] = n , o[1] = cse , o[2] = e0 , o[3] = low | ( high<<8 );
 		}
+	}
+	// R + dWeight K and B by table position (the columns of R and B are in the same colour order)
+	if constexpr( RB )
+	{
+		__syncthreads();
+		for( int k=threadIdx.x ; k<N*W ; k+=blockDim.x ){ const int cse = nodeInfo[ 4*(k/W)+1 ] ; sRK[k] = L.R[k] + L.K[ cse*W + k%W ]*L.dW , sB[k] = L.B[k]; }
 	}
 	if constexpr( !DIRECT ) for( int p=P+lane ; p<TinyLds< Dim >::WB ; p+=64 ) wb[p] = 0;   // padding pairs stay zero
 
@@ -1536,7 +1486,7 @@ __global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView<
 		{
 			const int w = rl + 16*m;
 			const bool ok = valid && w<W && !( wneg[m] & low ) && !( wpos[m] & high );
-			const int r = ok ? i + woff[m] : 0 , k = ok ? i*W + w : 0;
+			const int r = ok ? i + woff[m] : 0 , k = ok ? n*W + w : 0;
 			const double xr = sx[r] , yr = sy[r];
 			double Rv , Bv;
 			if constexpr( RB ) Rv = sRK[k] , Bv = sB[k];
