@@ -75,7 +75,7 @@ def load():
     up = ctypes.POINTER(ctypes.c_uint)
     L.epr_subgrid_triangulation.argtypes = [ci, sz, up, ci, ctypes.POINTER(vp)]
     L.epr_subgrid_triangulation.restype = ctypes.c_int
-    L.epr_level_set.argtypes = [ci, ci, sz, dp, sz, up, dp, dp, ctypes.POINTER(vp)]
+    L.epr_level_set.argtypes = [ci, ci, ci, sz, dp, sz, up, dp, dp, ctypes.POINTER(vp)]
     L.epr_level_set.restype = ctypes.c_int
     L.epr_mesh_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(ci), ctypes.POINTER(sz), ctypes.POINTER(ci)]
     L.epr_mesh_info.restype = ctypes.c_int

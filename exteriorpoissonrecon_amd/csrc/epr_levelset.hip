@@ -525,6 +525,176 @@ __global__ void __launch_bounds__( 256 ) KAssemble( const unsigned *recOffset , 
 	else count[s] = b==e ? 0u : (unsigned)AssembleSimplex< Dim , false >( vertexOf , local , b , e , nullptr );
 }
 
+// ---------------------------------------------------------------- level set (codim 1)
+// One value per vertex.  A triangle (the reference's CoDim==1, Dim==2 branch,
+// MarchingSimplices.h:497-541) with a vertex whose sign differs from both
+// others yields an oriented segment between the iso points of its two edges
+// at that vertex; the iso point of an edge is created by the first edge
+// occurrence that asks for it (key = sorted vertex pair), with the parameter
+// s = -c_ii / (c_o - c_ii) of that occurrence.  Higher simplices fan their
+// faces' simplices from the first vertex (:555-573).
+#pragma clang fp contract(off)
+template< int K >
+__device__ __forceinline__ bool Reaches1( const double *val , const int *idx , double iso )
+{
+	double c = 0;
+	for( int d=0 ; d<=K ; d++ ) c += val[ idx[d] ];
+	c *= 1. / (double)( K+1 );
+	double r2 = 0;
+	for( int d=0 ; d<=K ; d++ ){ const double t = -val[ idx[d] ] + c ; double n2 = 0 ; n2 += t*t ; r2 = fmax( r2 , n2 ); }
+	const double t = -iso + c;
+	double n2 = 0;
+	n2 += t*t;
+	return !( n2>r2 );
+}
+
+struct Ls1Records
+{
+	unsigned long long *key;     // sorted vertex pair of the edge
+	unsigned char *local;        // ( triangle index )*2 + edge (0, 1)
+	unsigned char *swap;         // the triangle's segment is reversed
+	double *data;                // vdim per record
+};
+
+template< int Dim , bool WRITE >
+__device__ int SimplexEdges( const LsMesh &M , long long s , Ls1Records R , unsigned at )
+{
+	unsigned gi[Dim+1];
+	double val[Dim+1];
+	for( int d=0 ; d<=Dim ; d++ ){ gi[d] = M.S[ s*( Dim+1 ) + d ] ; val[d] = M.values[ gi[d] ]; }
+	int top[Dim+1];
+	for( int d=0 ; d<=Dim ; d++ ) top[d] = d;
+	if( !Reaches1< Dim >( val , top , M.iso0 ) ) return 0;
+	int count = 0;
+	constexpr int NT = Dim==4 ? 5 : 1;
+	for( int g=0 ; g<NT ; g++ )
+	{
+		int tet[4];
+		if constexpr( Dim==4 ){ Face< 4 >( top , g , tet ) ; if( !Reaches1< 3 >( val , tet , M.iso0 ) ) continue; }
+		else for( int d=0 ; d<4 ; d++ ) tet[d] = top[d];
+		for( int f=0 ; f<4 ; f++ )
+		{
+			int tri[3];
+			Face< 3 >( tet , f , tri );
+			if( !Reaches1< 2 >( val , tri , M.iso0 ) ) continue;
+			double cv[3];
+			for( int i=0 ; i<3 ; i++ ){ cv[i] = val[ tri[i] ] - M.iso0 ; if( cv[i]==0 ) cv[i] = 1e-10; }
+			int ii = -1;
+			for( int d=0 ; d<3 ; d++ ) if( ii<0 && cv[d]*cv[(d+1)%3]<0 && cv[d]*cv[(d+2)%3]<0 ) ii = d;
+			if( ii<0 ) continue;
+			if constexpr( WRITE ) for( int i=0 ; i<2 ; i++ )
+			{
+				const unsigned r = at + count + i;
+				const int o = ( ii+1+i )%3;
+				const unsigned a = gi[ tri[ii] ] , b = gi[ tri[o] ];
+				R.key[r] = a<b ? ( (unsigned long long)a<<32 ) | b : ( (unsigned long long)b<<32 ) | a;
+				R.local[r] = (unsigned char)( ( g*4 + f )*2 + i );
+				R.swap[r] = cv[ii]<0 ? 1 : 0;
+				const double sp = - cv[ii] / ( cv[o] - cv[ii] );
+				for( int k=0 ; k<M.vdim ; k++ ) R.data[ (size_t)r*M.vdim+k ] = M.data[ (size_t)a*M.vdim+k ]*( 1.-sp ) + M.data[ (size_t)b*M.vdim+k ]*sp;
+			}
+			count += 2;
+		}
+	}
+	return count;
+}
+#pragma clang fp contract(on)
+
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KLs1Count( LsMesh M , unsigned *count )
+{
+	const long long s = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( s>=M.nS ) return;
+	Ls1Records none{};
+	count[s] = (unsigned)SimplexEdges< Dim , false >( M , s , none , 0 );
+}
+
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KLs1Records( LsMesh M , const unsigned *offset , Ls1Records R )
+{
+	const long long s = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( s>=M.nS ) return;
+	SimplexEdges< Dim , true >( M , s , R , offset[s] );
+}
+
+// the first record of every run of equal keys creates the vertex
+__global__ void __launch_bounds__( 256 ) KFirstOfKey( const unsigned long long *skey , const unsigned *order , unsigned n , unsigned *first )
+{
+	const unsigned j = blockIdx.x*blockDim.x + threadIdx.x;
+	if( j>=n ) return;
+	if( j>0 && skey[j]==skey[j-1] ) return;
+	const unsigned f = order[j];
+	for( unsigned t=j ; t<n && skey[t]==skey[j] ; t++ ) first[ order[t] ] = f;
+}
+
+__global__ void __launch_bounds__( 256 ) KLs1Vertices( const unsigned *first , const unsigned *vid , const double *recData , int vdim , unsigned n , unsigned *vertexOf , double *outV )
+{
+	const unsigned r = blockIdx.x*blockDim.x + threadIdx.x;
+	if( r>=n ) return;
+	if( first[r]==r ) for( int k=0 ; k<vdim ; k++ ) outV[ (size_t)vid[r]*vdim+k ] = recData[ (size_t)r*vdim+k ];
+	vertexOf[r] = vid[ first[r] ];
+}
+
+__global__ void __launch_bounds__( 256 ) KIsFirst( const unsigned *first , unsigned n , unsigned *flag )
+{
+	const unsigned r = blockIdx.x*blockDim.x + threadIdx.x;
+	if( r<n ) flag[r] = first[r]==r ? 1u : 0u;
+}
+
+// Output simplices of simplex s: per triangle its oriented segment, per
+// tetrahedron the fan of its segments into triangles; in 4D the fan of the
+// five tetrahedra's triangles into tetrahedra.
+template< int Dim , bool WRITE >
+__device__ int AssembleSimplex1( const unsigned *vertexOf , const unsigned char *local , const unsigned char *swp , unsigned b , unsigned e , unsigned *out )
+{
+	unsigned tris[15][3];
+	int nTri = 0;
+	unsigned r = b;
+	constexpr int NT = Dim==4 ? 5 : 1;
+	for( int g=0 ; g<NT ; g++ )
+	{
+		unsigned seg[4][2];
+		int nSeg = 0;
+		for( ; r<e && ( Dim==3 || ( local[r]>>1 )/4==g ) ; r+=2 )
+		{
+			unsigned v0 = vertexOf[r] , v1 = vertexOf[r+1];
+			if( swp[r] ){ const unsigned t = v0 ; v0 = v1 ; v1 = t; }
+			seg[nSeg][0] = v0 , seg[nSeg][1] = v1 , nSeg++;
+		}
+		if( !nSeg ) continue;
+		const unsigned v = seg[0][0];
+		for( int i=1 ; i<nSeg ; i++ ) if( seg[i][0]!=v && seg[i][1]!=v ){ tris[nTri][0] = v , tris[nTri][1] = seg[i][0] , tris[nTri][2] = seg[i][1] ; nTri++; }
+	}
+	if constexpr( Dim==3 )
+	{
+		if( WRITE ) for( int i=0 ; i<nTri ; i++ ) for( int k=0 ; k<3 ; k++ ) out[3*i+k] = tris[i][k];
+		return nTri;
+	}
+	else
+	{
+		if( !nTri ) return 0;
+		const unsigned v = tris[0][0];
+		int n = 0;
+		for( int i=1 ; i<nTri ; i++ ) if( tris[i][0]!=v && tris[i][1]!=v && tris[i][2]!=v )
+		{
+			if( WRITE ){ out[4*n] = v ; for( int k=0 ; k<3 ; k++ ) out[4*n+1+k] = tris[i][k]; }
+			n++;
+		}
+		return n;
+	}
+}
+
+template< int Dim , bool WRITE >
+__global__ void __launch_bounds__( 256 ) KAssemble1( const unsigned *recOffset , long long nS , const unsigned *vertexOf , const unsigned char *local , const unsigned char *swp ,
+                                                     unsigned *count , const unsigned *outOffset , unsigned *out )
+{
+	const long long s = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( s>=nS ) return;
+	const unsigned b = recOffset[s] , e = recOffset[s+1];
+	if( WRITE ) AssembleSimplex1< Dim , true >( vertexOf , local , swp , b , e , out + (size_t)outOffset[s]*Dim );
+	else count[s] = b==e ? 0u : (unsigned)AssembleSimplex1< Dim , false >( vertexOf , local , swp , b , e , nullptr );
+}
+
 static inline unsigned int Grid( long long n ){ return (unsigned int)std::max< long long >( 1 , ( n + 255 )/256 ); }
 
 // exclusive prefix sum of n values into out[0..n] (out[n] = total); returns the total
@@ -658,6 +828,60 @@ static epr_mesh *LevelSet2( int vdim , size_t nV , const double *vdata , size_t 
 	HIPCHECK( hipStreamSynchronize( st ) );
 	return m;
 }
+template< int Dim >
+static epr_mesh *LevelSet1( int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *S , const double *values , double iso )
+{
+	epr_mesh *m = new epr_mesh;
+	m->vdim = vdim , m->k = Dim;
+	if( !nS ) return m;
+	if( (unsigned long long)nS*( Dim==4 ? 40 : 8 )>=NONE ) throw std::string( "too many simplices for 32-bit occurrence indices" );
+	for( size_t i=0 ; i<nS*( Dim+1 ) ; i++ ) if( S[i]>=nV ) throw std::string( "simplex vertex index out of range" );
+	hipStream_t st = nullptr;
+	DevBuf< unsigned > dS , cnt , off;
+	DevBuf< double > dVal , dData;
+	DevBuf< unsigned char > tmp;
+	dS.upload( S , nS*( Dim+1 ) , st );
+	dVal.upload( values , nV , st );
+	dData.upload( vdata , nV*vdim , st );
+	LsMesh M{ dS.p , dVal.p , dData.p , vdim , (long long)nS , iso , 0. };
+	cnt.alloc( nS+1 ) , off.alloc( nS+1 );
+	KLs1Count< Dim ><<< Grid( nS ) , 256 , 0 , st >>>( M , cnt.p );
+	KFillU32<<< 1 , 256 , 0 , st >>>( cnt.p + nS , 1 , 0 );
+	const unsigned n = ExclusiveScan( cnt.p , off.p , (long long)nS , tmp , st );
+	if( !n ) return m;
+	DevBuf< unsigned long long > key , skey;
+	DevBuf< unsigned char > local , swp;
+	DevBuf< double > recData , outV;
+	DevBuf< unsigned > iota , order , first , flag , vid , vOf;
+	key.alloc( n ) , skey.alloc( n ) , local.alloc( n ) , swp.alloc( n ) , recData.alloc( (size_t)n*vdim );
+	Ls1Records R{ key.p , local.p , swp.p , recData.p };
+	KLs1Records< Dim ><<< Grid( nS ) , 256 , 0 , st >>>( M , off.p , R );
+	// the first occurrence of every edge (stable sort keeps occurrence order within a key)
+	iota.alloc( n ) , order.alloc( n ) , first.alloc( n ) , flag.alloc( n+1 ) , vid.alloc( n+1 ) , vOf.alloc( n );
+	KIota<<< Grid( n ) , 256 , 0 , st >>>( iota.p , n );
+	size_t bytes = 0;
+	HIPCHECK( hipcub::DeviceRadixSort::SortPairs( nullptr , bytes , key.p , skey.p , iota.p , order.p , (int)n , 0 , 64 , st ) );
+	tmp.ensure( bytes );
+	HIPCHECK( hipcub::DeviceRadixSort::SortPairs( tmp.p , bytes , key.p , skey.p , iota.p , order.p , (int)n , 0 , 64 , st ) );
+	KFirstOfKey<<< Grid( n ) , 256 , 0 , st >>>( skey.p , order.p , n , first.p );
+	KIsFirst<<< Grid( n ) , 256 , 0 , st >>>( first.p , n , flag.p );
+	KFillU32<<< 1 , 256 , 0 , st >>>( flag.p + n , 1 , 0 );
+	const unsigned nLV = ExclusiveScan( flag.p , vid.p , n , tmp , st );
+	outV.alloc( (size_t)nLV*vdim );
+	KLs1Vertices<<< Grid( n ) , 256 , 0 , st >>>( first.p , vid.p , recData.p , vdim , n , vOf.p , outV.p );
+	DevBuf< unsigned > oc , oo , out;
+	oc.alloc( nS+1 ) , oo.alloc( nS+1 );
+	KAssemble1< Dim , false ><<< Grid( nS ) , 256 , 0 , st >>>( off.p , (long long)nS , vOf.p , local.p , swp.p , oc.p , nullptr , nullptr );
+	KFillU32<<< 1 , 256 , 0 , st >>>( oc.p + nS , 1 , 0 );
+	const unsigned nLS = ExclusiveScan( oc.p , oo.p , (long long)nS , tmp , st );
+	out.alloc( (size_t)nLS*Dim );
+	if( nLS ) KAssemble1< Dim , true ><<< Grid( nS ) , 256 , 0 , st >>>( off.p , (long long)nS , vOf.p , local.p , swp.p , nullptr , oo.p , out.p );
+	m->v.resize( (size_t)nLV*vdim ) , m->s.resize( (size_t)nLS*Dim );
+	outV.download( m->v.data() , m->v.size() , st );
+	out.download( m->s.data() , m->s.size() , st );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	return m;
+}
 } // namespace ls
 } // namespace epr
 
@@ -676,16 +900,24 @@ int epr_subgrid_triangulation( int dim , size_t nCells , const unsigned *cells ,
 	return epr::ContextFree( [&](){ *out = epr::ls::Triangulate( dim , nCells , cells , orient!=0 ); } );
 }
 
-int epr_level_set( int dim , int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *simplices , const double *values , const double *iso , epr_mesh **out )
+int epr_level_set( int dim , int codim , int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *simplices , const double *values , const double *iso , epr_mesh **out )
 {
-	if( !out || ( dim!=3 && dim!=4 ) || vdim<1 || vdim>16 || ( nS && ( !simplices || !values || !vdata ) ) ){ epr::ThreadError() = "bad argument" ; return EPR_ERR_ARG; }
+	if( !out || ( dim!=3 && dim!=4 ) || ( codim!=1 && codim!=2 ) || vdim<1 || vdim>16 || ( nS && ( !simplices || !values || !vdata ) ) ){ epr::ThreadError() = "bad argument" ; return EPR_ERR_ARG; }
 	*out = nullptr;
 	const double zero[2] = { 0 , 0 };
 	if( !iso ) iso = zero;
 	return epr::ContextFree( [&]()
 	{
-		if( dim==3 ) *out = epr::ls::LevelSet2< 3 >( vdim , nV , vdata , nS , simplices , values , iso );
-		else         *out = epr::ls::LevelSet2< 4 >( vdim , nV , vdata , nS , simplices , values , iso );
+		if( codim==2 )
+		{
+			if( dim==3 ) *out = epr::ls::LevelSet2< 3 >( vdim , nV , vdata , nS , simplices , values , iso );
+			else         *out = epr::ls::LevelSet2< 4 >( vdim , nV , vdata , nS , simplices , values , iso );
+		}
+		else
+		{
+			if( dim==3 ) *out = epr::ls::LevelSet1< 3 >( vdim , nV , vdata , nS , simplices , values , iso[0] );
+			else         *out = epr::ls::LevelSet1< 4 >( vdim , nV , vdata , nS , simplices , values , iso[0] );
+		}
 	} );
 }
 

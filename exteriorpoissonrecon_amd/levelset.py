@@ -40,16 +40,18 @@ def subgrid_triangulation(dim, cells, orient=True):
 
 
 def level_set(vertices, simplices, values, iso=None):
-    """(vertices, simplices) of the codimension-2 zero set of values (nV x 2) minus iso over the mesh;
-    output vertices interpolate the input vertex rows."""
+    """(vertices, simplices) of the zero set of values - iso over the mesh: codimension 2 for
+    values nV x 2 (the joint zero set), codimension 1 for values of length nV; output vertices
+    interpolate the input vertex rows."""
     V = np.ascontiguousarray(vertices, dtype=np.float64)
     S = np.ascontiguousarray(simplices, dtype=np.uint32)
-    val = np.ascontiguousarray(values, dtype=np.float64).reshape(-1, 2)
-    if len(val) != len(V):
-        raise ValueError("one pair of values per vertex")
-    iso_ = None if iso is None else np.ascontiguousarray(iso, dtype=np.float64)
+    val = np.ascontiguousarray(values, dtype=np.float64)
+    codim = 1 if val.ndim == 1 else val.shape[1]
+    if len(val) != len(V) or codim not in (1, 2):
+        raise ValueError("one value (codimension 1) or one pair (codimension 2) per vertex")
+    iso_ = None if iso is None else np.ascontiguousarray(np.atleast_1d(iso), dtype=np.float64)
     h = ctypes.c_void_p()
-    _check(load().epr_level_set(S.shape[1] - 1, V.shape[1], len(V), _p(V), len(S),
+    _check(load().epr_level_set(S.shape[1] - 1, codim, V.shape[1], len(V), _p(V), len(S),
                                 S.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), _p(val),
                                 None if iso_ is None else _p(iso_), ctypes.byref(h)))
     return _mesh(h)

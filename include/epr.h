@@ -143,13 +143,15 @@ void epr_mesh_destroy( epr_mesh *m );
  * in order of first appearance, positions as doubles. */
 int  epr_subgrid_triangulation( int dim , size_t nCells , const unsigned *cells , int orient , epr_mesh **out );
 
-/* Replaces MarchingSimplices::LevelSet< double , 2 , dim >( mesh , PF , iso )
- * (Misha/MarchingSimplices.h:410-576): the (dim-2)-simplices of the joint
- * zero set of two values per vertex (values: nV x 2; iso NULL = (0,0)) over a
- * mesh of dim-simplices (simplices: nS x (dim+1)); output vertices carry the
- * barycentric interpolation of the input vertex data (nV x vdim), numbered
- * and ordered as the reference numbers and orders them. */
-int  epr_level_set( int dim , int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *simplices ,
+/* Replaces MarchingSimplices::LevelSet< double , codim , dim >( mesh , PF , iso )
+ * (Misha/MarchingSimplices.h:410-576), codim 1 or 2: the (dim-codim)-simplices
+ * of the zero set of codim values per vertex (values: nV x codim; iso NULL =
+ * 0) over a mesh of dim-simplices (simplices: nS x (dim+1)); output vertices
+ * carry the interpolation of the input vertex data (nV x vdim), numbered and
+ * ordered as the reference numbers and orders them.  Codim 2 is the joint zero
+ * set of Extract.cpp:205; codim 1 the per-function sets of its --levelSets
+ * output (:303-309). */
+int  epr_level_set( int dim , int codim , int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *simplices ,
                     const double *values , const double *iso , epr_mesh **out );
 
 /* ---- Multigrid V-cycle (the reference's Include/MG.h) ----

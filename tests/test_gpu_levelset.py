@@ -30,9 +30,14 @@ def test_gpu_matches_reference(path):
     V, S = subgrid_triangulation(dim, g["cells"])
     np.testing.assert_array_equal(V.ravel(), g["tri_vertices"])
     np.testing.assert_array_equal(S.ravel(), g["tri_simplices"].astype(np.uint32))
-    LV, LS = level_set(V, S, vertex_values(g["node_values"], res, V))
+    vals = vertex_values(g["node_values"], res, V)
+    LV, LS = level_set(V, S, vals)
     np.testing.assert_array_equal(LV.ravel(), g["ls_vertices"])
     np.testing.assert_array_equal(LS.ravel(), g["ls_simplices"].astype(np.uint32))
+    for c in range(2):                       # the codimension-1 sets of --levelSets
+        LV1, LS1 = level_set(V, S, vals[:, c])
+        np.testing.assert_array_equal(LV1.ravel(), g[f"ls1_{c}_vertices"])
+        np.testing.assert_array_equal(LS1.ravel(), g[f"ls1_{c}_simplices"].astype(np.uint32))
 
 
 @pytest.mark.parametrize("dim,res,kind", [(3, 40, "sphere"), (3, 12, "noise"), (4, 12, "sphere"), (4, 5, "noise")])
@@ -61,6 +66,10 @@ def test_gpu_matches_oracle_larger(dim, res, kind, oracle_lib):
     assert len(LV) > 0
     np.testing.assert_array_equal(LV, oLV)
     np.testing.assert_array_equal(LS, oLS)
+    LV1, LS1 = level_set(V, S, vals[:, 0])
+    oLV1, oLS1 = oracle_lib.level_set(V, S, vals[:, 0])
+    np.testing.assert_array_equal(LV1, oLV1)
+    np.testing.assert_array_equal(LS1, oLS1)
 
 
 def test_gpu_empty_inputs():
