@@ -131,7 +131,7 @@ def main():
     pos, skew = wl["gen"](samples)
     dim, depth = wl["dim"], wl["depth"]
     log(f"[bench] {args.config}: {pos.shape[0]} samples, dim {dim}, depth {depth}")
-    rec = Reconstructor(pos, skew, depth=depth)
+    rec = Reconstructor(pos, skew, depth=depth, aRadius=-1)
 
     E = None
     for i in range(args.warmup):

@@ -18,7 +18,7 @@ EXPORTS = [
     "epr_get_array", "epr_write_tree", "epr_write_grid", "epr_last_seconds", "epr_time_gs_sweeps",
     "epr_time_copy", "epr_subgrid_triangulation", "epr_level_set", "epr_mesh_info", "epr_mesh_copy",
     "epr_mesh_destroy", "epr_mg_create", "epr_mg_num_levels", "epr_mg_level_size", "epr_mg_set_colours",
-    "epr_mg_vcycle", "epr_mg_relax", "epr_mg_destroy",
+    "epr_mg_vcycle", "epr_mg_relax", "epr_mg_destroy", "epr_adapted_indexer_write",
 ]
 
 
@@ -30,7 +30,7 @@ class Params(ctypes.Structure):
                 ("singleLevel", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("nSigma", ctypes.c_double), ("scale", ctypes.c_double),
                 ("sWeight", ctypes.c_double), ("dWeight", ctypes.c_double),
-                ("iMult", ctypes.c_double)]
+                ("iMult", ctypes.c_double), ("aRadius", ctypes.c_int)]
 
 
 class EprError(RuntimeError):
@@ -98,6 +98,8 @@ def load():
     L.epr_mg_destroy.argtypes = [vp]
     L.epr_time_copy.argtypes = [sz, ci, dp]
     L.epr_time_copy.restype = ctypes.c_int
+    L.epr_adapted_indexer_write.argtypes = [ci, ci, ci, ctypes.c_double, sz, dp, ctypes.c_char_p]
+    L.epr_adapted_indexer_write.restype = ctypes.c_int
     L.epr_last_seconds.argtypes = [vp, ci]
     L.epr_last_seconds.restype = ctypes.c_double
     for name in ["epr_create", "epr_set_samples", "epr_build", "epr_init", "epr_gs_sweeps",

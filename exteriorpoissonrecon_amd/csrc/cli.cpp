@@ -1,4 +1,4 @@
-// ExteriorPoissonRecon command line over libepr_hip.so (dense --aRadius -1 path).
+// ExteriorPoissonRecon command line over libepr_hip.so (dense --aRadius -1 and adapted --aRadius 0..5 paths).
 //
 // Same flags, defaults and outputs as ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:52-279:
 // reads <in>.ply, writes <out>.tree (reference byte layout) and, with --grid,
@@ -20,11 +20,11 @@ static void Error( const std::string &msg )
 	exit( 0 );
 }
 
-// typeid( MishaK::Hat::HierarchicalRegularIndexer< Dim > ).hash_code() of a
-// libstdc++ build of the reference: the hash of the mangled type name.
-static unsigned long long IndexerHash( int dim )
+// typeid( MishaK::Hat::Hierarchical{Regular,Adapted}Indexer< Dim > ).hash_code()
+// of a libstdc++ build of the reference: the hash of the mangled type name.
+static unsigned long long IndexerHash( int dim , bool adapted )
 {
-	std::string name = "N6MishaK3Hat26HierarchicalRegularIndexerILj" + std::to_string( dim ) + "EEE";
+	std::string name = std::string( adapted ? "N6MishaK3Hat26HierarchicalAdaptedIndexerILj" : "N6MishaK3Hat26HierarchicalRegularIndexerILj" ) + std::to_string( dim ) + "EEE";
 	return (unsigned long long)std::_Hash_bytes( name.c_str() , name.size() , static_cast< size_t >( 0xc70f6907UL ) );
 }
 
@@ -40,7 +40,7 @@ static void Usage( const char *ex )
 	printf( "\t[--nSigma <noise-to-weight sigma>=0.250000]\n" );
 	printf( "\t[--dRadius <density kernel radius>=1]\n" );
 	printf( "\t[--tRadius <tensor kernel radius>=1]\n" );
-	printf( "\t[--aRadius <adaptive radius>=-1 (only the dense grid is supported)]\n" );
+	printf( "\t[--aRadius <adaptive radius>=3]\n" );
 	printf( "\t[--gsIters <gs iterations>=10]\n" );
 	printf( "\t[--seed <random seed>=0]\n" );
 	printf( "\t[--sWeight <screening weight>=100]\n" );
@@ -56,7 +56,7 @@ int main( int argc , char *argv[] )
 	epr_params p;
 	epr_default_params( &p );
 	std::string in , out , grid;
-	long long aRadius = -1;
+	long long aRadius = 3;
 	bool project = false , separate = false;
 	double jitter = 0;
 	for( int i=1 ; i<argc ; i++ )
@@ -89,7 +89,10 @@ int main( int argc , char *argv[] )
 		else fprintf( stderr , "[WARNING] Invalid parameter: --%s\n" , a.c_str() );
 	}
 	if( in.empty() ){ Usage( argv[0] ) ; return EXIT_SUCCESS; }
-	if( aRadius!=-1 && aRadius!=(long long)0xFFFFFFFFLL ) Error( "only the dense grid (--aRadius -1) is supported" );
+	// "-1" reads as UINT_MAX in the reference (an unsigned parameter) and selects the dense grid
+	if( aRadius==-1 || aRadius==(long long)0xFFFFFFFFLL ) p.aRadius = -1;
+	else if( aRadius<0 || aRadius>5 ) Error( "Only kernel radius in the range [0,5] supported: " + std::to_string( (unsigned int)aRadius ) );
+	else p.aRadius = (int)aRadius;
 	if( project ) Error( "--project is not supported" );
 	if( separate ) Error( "--separate is not supported" );
 	if( jitter>0 ) Error( "--jitter is not supported" );
@@ -108,7 +111,7 @@ int main( int argc , char *argv[] )
 	double E[2];
 	Check( epr_solve( ctx , E ) );
 	printf( "Error: %.17g -> %.17g = %.17g + %.17g\n" , 0. , E[0]+E[1] , E[0] , E[1] );
-	if( !out.empty() ) Check( epr_write_tree( ctx , ( out + ".tree" ).c_str() , IndexerHash( s.dim ) ) );
+	if( !out.empty() ) Check( epr_write_tree( ctx , ( out + ".tree" ).c_str() , IndexerHash( s.dim , p.aRadius!=-1 ) ) );
 	if( !grid.empty() ) Check( epr_write_grid( ctx , grid.c_str() ) );
 	if( p.verbose ) printf( "%32s: %.3f (s)\n" , "Executed" , std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count() );
 	epr_destroy( ctx );

@@ -19,12 +19,87 @@
 #include <vector>
 
 #include "../../include/epr.h"
+#include "epr_adapted.h"
 #include "epr_dev.h"
 #include "epr_hat.h"
 #include "epr_kernels.h"
 
 namespace epr
 {
+// The unit-cube map (ExteriorPoisson.h:100-123, with the matrix products of
+// Misha/Geometry.inl:97-107, 410-427) and the ordered sampler's cells
+// (OrderedSamples.inl:32-85: clamped floor binning, lexicographic std::sort,
+// grouping) of n world positions.
+template< int Dim >
+struct SampleCells
+{
+	std::vector< double > pos;                  // unit-cube positions
+	double unitToWorld[Dim+1][Dim+1];
+	std::vector< int > cellIdx , cellStart , cellSamples;
+	int ncell = 0;
+
+	void bin( size_t n , const double *wpos , double scale , int D )
+	{
+		if( !n ) throw std::string( "Expected some samples: 0" );
+		for( size_t i=0 ; i<n*Dim ; i++ ) if( !std::isfinite( wpos[i] ) ) throw std::string( "non-finite sample position" );
+		double bb0[Dim] , bb1[Dim];
+		for( int d=0 ; d<Dim ; d++ ) bb0[d] = bb1[d] = wpos[d];
+		for( size_t i=1 ; i<n ; i++ ) for( int d=0 ; d<Dim ; d++ ){ bb0[d] = std::min( bb0[d] , wpos[i*Dim+d] ) ; bb1[d] = std::max( bb1[d] , wpos[i*Dim+d] ); }
+		double sc = bb1[0]-bb0[0];
+		for( int d=1 ; d<Dim ; d++ ) sc = std::max( sc , bb1[d]-bb0[d] );
+		double t1[Dim+1][Dim+1] , s[Dim+1][Dim+1] , t2[Dim+1][Dim+1] , ts[Dim+1][Dim+1] , M[Dim+1][Dim+1];
+		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) t1[i][j] = s[i][j] = t2[i][j] = ( i==j ? 1. : 0. );
+		for( int d=0 ; d<Dim ; d++ ) t2[Dim][d] = -( bb0[d] + bb1[d] ) / 2.;
+		for( int d=0 ; d<Dim ; d++ ) s[d][d] = 1. / ( sc * scale );
+		for( int d=0 ; d<Dim ; d++ ) t1[Dim][d] = 0.5;
+		auto Mul = []( const double A[Dim+1][Dim+1] , const double B[Dim+1][Dim+1] , double O[Dim+1][Dim+1] )
+		{
+			for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ){ O[i][j] = 0 ; for( int k=0 ; k<=Dim ; k++ ) O[i][j] += B[i][k]*A[k][j]; }
+		};
+		Mul( t1 , s , ts );
+		Mul( ts , t2 , M );
+		pos.resize( n*Dim );
+		for( size_t i=0 ; i<n ; i++ )
+		{
+			double scl = 1;
+			for( int a=0 ; a<Dim ; a++ )
+			{
+				double o = 0;
+				for( int b=0 ; b<Dim ; b++ ) o += wpos[i*Dim+b]*M[b][a];
+				o += M[Dim][a];
+				scl += M[a][Dim];
+				pos[i*Dim+a] = o;
+			}
+			for( int a=0 ; a<Dim ; a++ ) pos[i*Dim+a] /= scl;
+		}
+		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) unitToWorld[i][j] = ( i==j ? 1. : 0. );
+		for( int d=0 ; d<Dim ; d++ ){ unitToWorld[d][d] = 1./M[d][d] ; unitToWorld[Dim][d] = -M[Dim][d]/M[d][d]; }
+		int res = 1<<D;
+		struct IS { size_t idx ; int I[Dim] ; bool operator<( const IS &o ) const { for( int d=0 ; d<Dim ; d++ ){ if( I[d]<o.I[d] ) return true ; if( I[d]>o.I[d] ) return false; } return false; } };
+		std::vector< IS > v( n );
+		for( size_t i=0 ; i<n ; i++ )
+		{
+			v[i].idx = i;
+			for( int d=0 ; d<Dim ; d++ ) v[i].I[d] = std::max< int >( 0 , std::min< int >( (int)floor( pos[i*Dim+d] * res ) , res-1 ) );
+		}
+		std::sort( v.begin() , v.end() );
+		cellIdx.clear() , cellStart.clear() , cellSamples.clear();
+		size_t start = 0;
+		while( start<v.size() )
+		{
+			size_t end = start;
+			auto same = [&]( size_t a , size_t b ){ for( int d=0 ; d<Dim ; d++ ) if( v[a].I[d]!=v[b].I[d] ) return false ; return true; };
+			while( end<v.size() && same( end , start ) ) end++;
+			cellStart.push_back( (int)cellSamples.size() );
+			for( int d=0 ; d<Dim ; d++ ) cellIdx.push_back( v[start].I[d] );
+			for( size_t i=start ; i<end ; i++ ) cellSamples.push_back( (int)v[i].idx );
+			start = end;
+		}
+		cellStart.push_back( (int)cellSamples.size() );
+		ncell = (int)cellStart.size()-1;
+	}
+};
+
 struct CtxBase
 {
 	virtual ~CtxBase(){}
@@ -80,18 +155,20 @@ struct Ctx : public CtxBase
 		long long stride[Dim];
 		ColLayout< Dim > lay;              // colour-grouped column order of R and B
 		DevBuf< double > x , y , R , B , K;
+		DevBuf< unsigned char > fm , em;   // adapted hierarchy: function (1 active, 2 padding) and element flags
 		hipGraphExec_t graph = nullptr;    // captured Gauss-Seidel launches of this level
 		int graphIters = 0;
 		const double *graphX = nullptr , *graphR = nullptr;
+		const unsigned char *graphF = nullptr;
 		Level( void ){}
-		Level( Level &&o ) noexcept : res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , graph( o.graph ) , graphIters( o.graphIters )
+		Level( Level &&o ) noexcept : res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , fm( std::move( o.fm ) ) , em( std::move( o.em ) ) , graph( o.graph ) , graphIters( o.graphIters )
 		{
 			for( int d=0 ; d<Dim ; d++ ) stride[d] = o.stride[d];
 			o.graph = nullptr;
 		}
 		~Level(){ if( graph ) (void)hipGraphExecDestroy( graph ); }
 		// drops the captured launches when the buffers they point at move
-		void checkGraph( void ){ if( graph && ( graphX!=x.p || graphR!=R.p ) ){ (void)hipGraphExecDestroy( graph ) ; graph = nullptr; } }
+		void checkGraph( void ){ if( graph && ( graphX!=x.p || graphR!=R.p || graphF!=fm.p ) ){ (void)hipGraphExecDestroy( graph ) ; graph = nullptr; } }
 	};
 	std::vector< Level > levels;
 
@@ -101,12 +178,16 @@ struct Ctx : public CtxBase
 	std::vector< std::unique_ptr< DevBuf< double > > > estData , skData;
 	std::vector< std::unique_ptr< DevBuf< unsigned char > > > skKids , estKids;
 	double c_ = 0;
+	// the adapted hierarchy (--aRadius >= 0), built with the other build-phase data
+	std::unique_ptr< AdaptedHierarchy< Dim > > ah;
+	bool adapted( void ) const { return prm.aRadius!=-1; }
 
 	explicit Ctx( const epr_params &p ) : prm( p ) , D( p.depth )
 	{
 		if( D<0 || D>12 ) throw std::string( "depth must be in [0,12]" );
 		if( prm.tRadius<0 || prm.tRadius>4 ) throw std::string( "Kernel radius must be in range [0,4]: " + std::to_string( prm.tRadius ) );
 		if( prm.dRadius<0 || prm.dRadius>4 ) throw std::string( "Only radii [0,4] supported: " + std::to_string( prm.dRadius ) );
+		if( prm.aRadius!=-1 && ( prm.aRadius<0 || prm.aRadius>5 ) ) throw std::string( "Only kernel radius in the range [0,5] supported: " + std::to_string( prm.aRadius ) );
 		HIPCHECK( hipStreamCreate( &stream ) );
 		levels.resize( D+1 );
 		for( int l=0 ; l<=D ; l++ )
@@ -135,75 +216,23 @@ struct Ctx : public CtxBase
 		v.sW = std::pow( (double)L.res , 4.-Dim ) , v.dW = prm.dWeight;
 		for( int d=0 ; d<Dim ; d++ ) v.stride[d] = L.stride[d];
 		v.lay = L.lay;
+		v.fmask = L.fm.p;
 		return v;
 	}
 
 	// ---------------- samples ----------------
 	void setSamples( size_t _n , const double *wpos , const double *wskew )
 	{
-		if( !_n ) throw std::string( "Expected some samples: 0" );
-		for( size_t i=0 ; i<_n*Dim ; i++ ) if( !std::isfinite( wpos[i] ) ) throw std::string( "non-finite sample position" );
+		SampleCells< Dim > sc;
+		sc.bin( _n , wpos , prm.scale , D );
 		n = _n;
 		skewPacked.resize( n*CD );
 		for( size_t s=0 ; s<n ; s++ ) for( int i=0 , idx=0 ; i<Dim ; i++ ) for( int j=0 ; j<i ; j++ , idx++ )
 			skewPacked[s*CD+idx] = ( wskew[ s*Dim*Dim + i*Dim + j ] - wskew[ s*Dim*Dim + j*Dim + i ] ) / 2.;
-		// ToUnitCube (ExteriorPoisson.h:100-123) with the matrix products of Misha/Geometry.inl:97-107, 410-427
-		double bb0[Dim] , bb1[Dim];
-		for( int d=0 ; d<Dim ; d++ ) bb0[d] = bb1[d] = wpos[d];
-		for( size_t i=1 ; i<n ; i++ ) for( int d=0 ; d<Dim ; d++ ){ bb0[d] = std::min( bb0[d] , wpos[i*Dim+d] ) ; bb1[d] = std::max( bb1[d] , wpos[i*Dim+d] ); }
-		double sc = bb1[0]-bb0[0];
-		for( int d=1 ; d<Dim ; d++ ) sc = std::max( sc , bb1[d]-bb0[d] );
-		double t1[Dim+1][Dim+1] , s[Dim+1][Dim+1] , t2[Dim+1][Dim+1] , ts[Dim+1][Dim+1] , M[Dim+1][Dim+1];
-		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) t1[i][j] = s[i][j] = t2[i][j] = ( i==j ? 1. : 0. );
-		for( int d=0 ; d<Dim ; d++ ) t2[Dim][d] = -( bb0[d] + bb1[d] ) / 2.;
-		for( int d=0 ; d<Dim ; d++ ) s[d][d] = 1. / ( sc * prm.scale );
-		for( int d=0 ; d<Dim ; d++ ) t1[Dim][d] = 0.5;
-		auto Mul = []( const double A[Dim+1][Dim+1] , const double B[Dim+1][Dim+1] , double O[Dim+1][Dim+1] )
-		{
-			for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ){ O[i][j] = 0 ; for( int k=0 ; k<=Dim ; k++ ) O[i][j] += B[i][k]*A[k][j]; }
-		};
-		Mul( t1 , s , ts );
-		Mul( ts , t2 , M );
-		pos.resize( n*Dim );
-		for( size_t i=0 ; i<n ; i++ )
-		{
-			double scl = 1;
-			for( int a=0 ; a<Dim ; a++ )
-			{
-				double o = 0;
-				for( int b=0 ; b<Dim ; b++ ) o += wpos[i*Dim+b]*M[b][a];
-				o += M[Dim][a];
-				scl += M[a][Dim];
-				pos[i*Dim+a] = o;
-			}
-			for( int a=0 ; a<Dim ; a++ ) pos[i*Dim+a] /= scl;
-		}
-		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) unitToWorld[i][j] = ( i==j ? 1. : 0. );
-		for( int d=0 ; d<Dim ; d++ ){ unitToWorld[d][d] = 1./M[d][d] ; unitToWorld[Dim][d] = -M[Dim][d]/M[d][d]; }
-		// OrderedSampler: clamped floor binning, lexicographic std::sort, grouping
-		int res = 1<<D;
-		struct IS { size_t idx ; int I[Dim] ; bool operator<( const IS &o ) const { for( int d=0 ; d<Dim ; d++ ){ if( I[d]<o.I[d] ) return true ; if( I[d]>o.I[d] ) return false; } return false; } };
-		std::vector< IS > v( n );
-		for( size_t i=0 ; i<n ; i++ )
-		{
-			v[i].idx = i;
-			for( int d=0 ; d<Dim ; d++ ) v[i].I[d] = std::max< int >( 0 , std::min< int >( (int)floor( pos[i*Dim+d] * res ) , res-1 ) );
-		}
-		std::sort( v.begin() , v.end() );
-		cellIdx.clear() , cellStart.clear() , cellSamples.clear();
-		size_t start = 0;
-		while( start<v.size() )
-		{
-			size_t end = start;
-			auto same = [&]( size_t a , size_t b ){ for( int d=0 ; d<Dim ; d++ ) if( v[a].I[d]!=v[b].I[d] ) return false ; return true; };
-			while( end<v.size() && same( end , start ) ) end++;
-			cellStart.push_back( (int)cellSamples.size() );
-			for( int d=0 ; d<Dim ; d++ ) cellIdx.push_back( v[start].I[d] );
-			for( size_t i=start ; i<end ; i++ ) cellSamples.push_back( (int)v[i].idx );
-			start = end;
-		}
-		cellStart.push_back( (int)cellSamples.size() );
-		ncell = (int)cellStart.size()-1;
+		pos.swap( sc.pos );
+		memcpy( unitToWorld , sc.unitToWorld , sizeof(unitToWorld) );
+		cellIdx.swap( sc.cellIdx ) , cellStart.swap( sc.cellStart ) , cellSamples.swap( sc.cellSamples );
+		ncell = sc.ncell;
 		built = solved = false;
 	}
 
@@ -269,6 +298,23 @@ struct Ctx : public CtxBase
 		dCellStart.upload( cellStart.data() , cellStart.size() , stream );
 		dCellSamples.upload( cellSamples.data() , cellSamples.size() , stream );
 
+		// ---- the adapted hierarchy (Hat.adaptive.inl:349-729) and its flags on every level
+		if( adapted() )
+		{
+			if( !ah ) ah.reset( new AdaptedHierarchy< Dim >() );
+			ah->build( cellIdx.data() , (size_t)ncell , D , prm.aRadius );
+			for( int l=0 ; l<=D ; l++ )
+			{
+				levels[l].fm.upload( ah->levels[l].fmask.data() , ah->levels[l].fmask.size() , stream );
+				levels[l].em.upload( ah->levels[l].emask.data() , ah->levels[l].emask.size() , stream );
+			}
+		}
+		else
+		{
+			ah.reset();
+			for( int l=0 ; l<=D ; l++ ) levels[l].fm.alloc( 0 ) , levels[l].em.alloc( 0 );
+		}
+
 		// ---- tree estimator (GridSamples.inl:740-771)
 		{
 			const int K = prm.dRadius;
@@ -326,6 +372,8 @@ struct Ctx : public CtxBase
 			long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= res;
 			dS.ensure( (size_t)ne*CD ) , dS2.ensure( (size_t)ne );
 			KField< Dim ><<< Blocks( ne , 256 ) , 256 , 0 , stream >>>( T , dS.p , dS2.p );
+			// adapted: c sums over the indexed elements only (SystemEnergy.h:455-461)
+			if( adapted() ) KMaskZero<<< Blocks( ne , 256 ) , 256 , 0 , stream >>>( dS2.p , levels[D].em.p , ne );
 			HIPCHECK( hipGetLastError() );
 			// c = sum |S(E)|^2 / #elements (SystemEnergy.h:455-461)
 			const int nb = 1024;
@@ -343,7 +391,7 @@ struct Ctx : public CtxBase
 			Level &F = levels[D];
 			F.R.ensure( (size_t)F.N*W ) , F.B.ensure( (size_t)F.N*W );
 			// B
-			KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay );
+			KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay , F.em.p , F.fm.p );
 			// R
 			if( prm.sWeight>0 )
 			{
@@ -370,9 +418,9 @@ struct Ctx : public CtxBase
 				Level &Cl = levels[l] , &Fl = levels[l+1];
 				Cl.R.ensure( (size_t)Cl.N*W ) , Cl.B.ensure( (size_t)Cl.N*W );
 				galerkinT( Fl.R.p , Fl );
-				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.R.p , Cl.res , Cl.lay );
+				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.R.p , Cl.res , Cl.lay , Cl.fm.p );
 				galerkinT( Fl.B.p , Fl );
-				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.B.p , Cl.res , Cl.lay );
+				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.B.p , Cl.res , Cl.lay , Cl.fm.p );
 			}
 			HIPCHECK( hipGetLastError() );
 		}
@@ -453,7 +501,7 @@ struct Ctx : public CtxBase
 				HIPCHECK( hipGraphInstantiate( &L.graph , g , nullptr , nullptr , 0 ) );
 				HIPCHECK( hipGraphDestroy( g ) );
 				L.graphIters = iters;
-				L.graphX = L.x.p , L.graphR = L.R.p;
+				L.graphX = L.x.p , L.graphR = L.R.p , L.graphF = L.fm.p;
 			}
 			HIPCHECK( hipGraphLaunch( L.graph , stream ) );
 		}
@@ -476,10 +524,14 @@ struct Ctx : public CtxBase
 		if( launches ) *launches = levels[l].N<=SmallLevel ? 1 : iters*C;
 	}
 
-	void prolong( int l , const double *in , double *out , int mode )
+	// P x (mode 0), out += P x (1) or out -= P x (2).  Adapted levels use the
+	// hierarchy's prolongation (rows of indexed fine functions only) unless
+	// regular is set (the regular prolongation of regularCoefficients,
+	// Hat.adaptive.inl:155-176).
+	void prolong( int l , const double *in , double *out , int mode , bool regular=false )
 	{
 		long long nf = levels[l+1].N;
-		KProlong< Dim ><<< Blocks( nf , 256 ) , 256 , 0 , stream >>>( in , out , levels[l].res , mode );
+		KProlong< Dim ><<< Blocks( nf , 256 ) , 256 , 0 , stream >>>( in , out , levels[l].res , mode , regular ? nullptr : levels[l+1].fm.p );
 	}
 
 	void prolongAdd( int l )
@@ -531,9 +583,10 @@ struct Ctx : public CtxBase
 		Level &L = levels[l];
 		if( x0 && y0 ){ L.x.upload( x0 , L.N , stream ) ; L.y.upload( y0 , L.N , stream ) ; HIPCHECK( hipStreamSynchronize( stream ) ) ; return; }
 		// ExteriorPoisson.h:333-341
-		std::vector< double > hx( L.N ) , hy( L.N );
+		std::vector< double > hx( L.N , 0. ) , hy( L.N , 0. );
 		srand( prm.seed );
-		for( long long i=0 ; i<L.N ; i++ ) hx[i] = Random() , hy[i] = Random();
+		if( ah ){ for( long long f : ah->levels[l].fOrder ) hx[f] = Random() , hy[f] = Random(); }   // the hierarchy's numbering
+		else for( long long i=0 ; i<L.N ; i++ ) hx[i] = Random() , hy[i] = Random();
 		L.x.upload( hx.data() , L.N , stream ) , L.y.upload( hy.data() , L.N , stream );
 		double t[4];
 		energyTerms( l , t );
@@ -647,6 +700,23 @@ struct Ctx : public CtxBase
 		else if( name=="sample_depth" ){ needBuilt() ; dl( dDepth ); }
 		else if( name=="S_elem" ){ needBuilt() ; dl( dS ); }
 		else if( name=="c" ){ needBuilt() ; v.push_back( c_ ); }
+		else if( name.rfind( "adapted_" , 0 )==0 )
+		{
+			// adapted_{fmask,emask,forder,eorder}_<depth> (dense indices, in the hierarchy's numbering for the orders)
+			needBuilt();
+			if( !ah ) throw std::string( "not an adapted reconstruction" );
+			const size_t u = name.rfind( '_' );
+			const int l = atoi( name.c_str()+u+1 );
+			checkLevel( l );
+			const std::string what = name.substr( 8 , u-8 );
+			const auto &A = ah->levels[l];
+			if( what=="fmask" ) v.assign( A.fmask.begin() , A.fmask.end() );
+			else if( what=="emask" ) v.assign( A.emask.begin() , A.emask.end() );
+			else if( what=="forder" ) v.assign( A.fOrder.begin() , A.fOrder.end() );
+			else if( what=="eorder" ) v.assign( A.eOrder.begin() , A.eOrder.end() );
+			else if( what=="nactive" ) v.push_back( (double)A.nActive );
+			else throw std::string( "unknown array: " + name );
+		}
 		else throw std::string( "unknown array: " + name );
 		if( out ) std::copy( v.begin() , v.begin()+std::min( cap , v.size() ) , out );
 		return v.size();
@@ -669,6 +739,11 @@ struct Ctx : public CtxBase
 		unsigned int dim = Dim;
 		fwrite( &dim , sizeof(dim) , 1 , fp );
 		fwrite( &hash , sizeof(hash) , 1 , fp );
+		// XForm< double , Dim+1 > written raw: the vtable pointer of its Algebra
+		// base (Misha/Algebra.h:168; meaningless in a file, written as 0), then the
+		// (Dim+1)^2 coefficients
+		const unsigned long long vptr = 0;
+		fwrite( &vptr , sizeof(vptr) , 1 , fp );
 		double v2w[Dim+1][Dim+1];
 		voxelToWorld( v2w );
 		fwrite( v2w , sizeof(double) , (Dim+1)*(Dim+1) , fp );
@@ -681,13 +756,27 @@ struct Ctx : public CtxBase
 				std::vector< double > h( levels[d].N );
 				( pass==0 ? levels[d].x : levels[d].y ).download( h.data() , h.size() , stream );
 				HIPCHECK( hipStreamSynchronize( stream ) );
-				long long sz = levels[d].N;
+				if( ah )   // the hierarchy's numbering (active functions, then padding)
+				{
+					std::vector< double > a( ah->numFunctions( d ) );
+					for( size_t i=0 ; i<a.size() ; i++ ) a[i] = h[ ah->levels[d].fOrder[i] ];
+					h.swap( a );
+				}
+				long long sz = (long long)h.size();
 				fwrite( &sz , sizeof(sz) , 1 , fp );
 				fwrite( h.data() , sizeof(double) , h.size() , fp );
 			}
 		}
-		unsigned int maxDepth = D;
-		fwrite( &maxDepth , sizeof(maxDepth) , 1 , fp );
+		try
+		{
+			if( ah ) ah->write( fp );   // HierarchicalAdaptedIndexer::write (Hat.adaptive.inl:408-414)
+			else
+			{
+				unsigned int maxDepth = D;   // HierarchicalRegularIndexer::write
+				fwrite( &maxDepth , sizeof(maxDepth) , 1 , fp );
+			}
+		}
+		catch( ... ){ fclose( fp ) ; throw; }
 		fclose( fp );
 	}
 
@@ -702,7 +791,7 @@ struct Ctx : public CtxBase
 		HIPCHECK( hipMemcpyAsync( ay.p , levels[0].y.p , levels[0].N*sizeof(double) , hipMemcpyDeviceToDevice , stream ) );
 		for( int d=1 ; d<=D ; d++ )
 		{
-			prolong( d-1 , ax.p , bx.p , 0 ) , prolong( d-1 , ay.p , by.p , 0 );
+			prolong( d-1 , ax.p , bx.p , 0 , true ) , prolong( d-1 , ay.p , by.p , 0 , true );
 			const long long N = levels[d].N;
 			KAddInto<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( levels[d].x.p , bx.p , ax.p , N );
 			KAddInto<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( levels[d].y.p , by.p , ay.p , N );
@@ -754,6 +843,29 @@ void epr_default_params( epr_params *p )
 	p->depth = 5 , p->seed = 0 , p->tRadius = 1 , p->dRadius = 1 , p->minSolveDepth = 0 , p->maxSolveDepth = -1 , p->gsIters = 10;
 	p->singleLevel = 0 , p->verbose = 0;
 	p->nSigma = 0.25 , p->scale = 1.1 , p->sWeight = 100. , p->dWeight = 0.00625 , p->iMult = 2.;
+	p->aRadius = 3;
+}
+
+int epr_adapted_indexer_write( int dim , int depth , int aRadius , double scale , size_t n , const double *pos , const char *path )
+{
+	return epr::ContextFree( [&]()
+	{
+		if( ( dim!=3 && dim!=4 ) || !pos || !path || depth<0 || depth>12 ) throw std::string( "bad arguments" );
+		auto run = [&]( auto tag )
+		{
+			constexpr int Dim = decltype( tag )::value;
+			epr::SampleCells< Dim > sc;
+			sc.bin( n , pos , scale , depth );
+			epr::AdaptedHierarchy< Dim > h;
+			h.build( sc.cellIdx.data() , (size_t)sc.ncell , depth , aRadius );
+			FILE *fp = fopen( path , "wb" );
+			if( !fp ) throw std::string( "cannot open " ) + path;
+			try { h.write( fp ); } catch( ... ){ fclose( fp ) ; throw; }
+			fclose( fp );
+		};
+		if( dim==3 ) run( std::integral_constant< int , 3 >() );
+		else         run( std::integral_constant< int , 4 >() );
+	} );
 }
 
 int epr_create( int dim , const epr_params *p , epr_ctx **out )

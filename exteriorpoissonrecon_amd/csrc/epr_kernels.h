@@ -71,6 +71,7 @@ struct LevelView
 	double dW;      // Dirichlet weight
 	long long stride[Dim];
 	ColLayout< Dim > lay;   // column position of a node in R and B
+	const unsigned char *fmask;   // adapted levels: 1 = active (updated by the sweeps); null: every node is active
 };
 
 enum { MODE_GS = 0 , MODE_QFIT = 1 , MODE_ENERGY = 2 };
@@ -315,6 +316,7 @@ __global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSe
 	int F[Dim];
 	cs.node( k , F );
 	const long long i = FlatIndex< Dim >( F , L.stride );
+	if( L.fmask && L.fmask[i]!=1 ) return;   // the whole group leaves together
 	double q[5];
 	NodeGroup< Dim , MODE_GS , Dim==3 >( L , F , i , colBase+k , lane , q );
 	if( lane==0 )
@@ -341,7 +343,7 @@ __global__ void __launch_bounds__( 512 ) KGsSmall( LevelView< Dim > L , int iter
 			const long long i = FlatIndex< Dim >( F , L.stride );
 			double q[5];
 			NodeGroup< Dim , MODE_GS >( L , F , i , L.lay.cstart[c]+k , lane , q );
-			if( lane==0 )
+			if( lane==0 && ( !L.fmask || L.fmask[i]==1 ) )
 			{
 				double dx , dy;
 				if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
@@ -430,12 +432,14 @@ __global__ void __launch_bounds__( 256 ) KPartialSum( const double *in , long lo
 // order (Eigen's column-major SpMV of SystemEnergy.h:162; weights of
 // Hat.scalars.inl:373-386).
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf , int cres , int mode )
+__global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf , int cres , int mode , const unsigned char *fmaskF )
 {
 	const int fres = 2*cres;
 	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= fres+1;
 	const long long f = (long long)blockIdx.x*blockDim.x + threadIdx.x;
 	if( f>=nf ) return;
+	// adapted levels: the prolongation has rows only for the fine functions that are indexed (Hat.scalars.inl:1694-1744)
+	if( fmaskF && !fmaskF[f] ){ if( mode==0 ) xf[f] = 0 ; return; }
 	int F[Dim];
 	Unflatten< Dim >( f , fres , F );
 	int nodd = 0 , oddDims[Dim];
@@ -830,6 +834,13 @@ __global__ void KScaleArray( double *a , long long n , double s )
 	if( i<n ) a[i] *= s;
 }
 
+// a[i] = 0 where m[i] is 0
+__global__ void KMaskZero( double *a , const unsigned char *m , long long n )
+{
+	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( i<n && !m[i] ) a[i] = 0;
+}
+
 // Target field at every finest element centre: sum over tree depths up to the
 // leaf containing the centre of the multilinear interpolation of the splatted
 // frames (ExteriorPoisson.h:294-307, GridSamples.inl:600-651).  Also writes
@@ -1065,7 +1076,7 @@ __device__ __forceinline__ void BOutputs( const double (&S)[1<<Dim][DimC< Dim >:
 }
 
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int res , double scale , double *B , ColLayout< Dim > lay )
+__global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int res , double scale , double *B , ColLayout< Dim > lay , const unsigned char *emask , const unsigned char *fmask )
 {
 	constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD;
 	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= res+1;
@@ -1080,10 +1091,22 @@ __global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int r
 		bool in = true;
 		long long e = 0;
 		for( int d=Dim-1 ; d>=0 ; d-- ){ const int E = F[d]-1+Bit< Dim >( b , d ) ; in = in && E>=0 && E<res ; e = e*res + E; }
+		if( emask ) in = in && emask[e];   // adapted levels: only indexed elements are integrated (Hat.scalars.inl:1640-1650)
 #pragma unroll
 		for( int c=0 ; c<CD ; c++ ) Sv[b][c] = in ? S[ e*CD + c ] : 0.;
 	}
-	BOutputs< Dim >( Sv , F , res , scale , B + lay.col( F )*W , std::make_integer_sequence< int , W >{} );
+	double *col = B + lay.col( F )*W;
+	BOutputs< Dim >( Sv , F , res , scale , col , std::make_integer_sequence< int , W >{} );
+	if( fmask )   // adapted levels: entries only between indexed functions
+	{
+		for( int w=0 ; w<W ; w++ )
+		{
+			bool in = fmask[f1]!=0;
+			long long g = 0;
+			for( int d=0 , t=w , p=W/3 ; d<Dim ; d++ , p/=3 ){ const int G = F[d] + ( t/p )%3 - 1 ; in = in && G>=0 && G<=res ; g = g*(res+1) + G; }
+			if( !in || !fmask[g] ) col[w] = 0;
+		}
+	}
 }
 
 
@@ -1189,7 +1212,7 @@ __global__ void __launch_bounds__( 64 ) KGalerkinTCls( const double *Mf , double
 }
 
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *Mc , int cres , ColLayout< Dim > lay )
+__global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *Mc , int cres , ColLayout< Dim > lay , const unsigned char *fmaskC )
 {
 	constexpr int W = DimC< Dim >::W;
 	const int fres = 2*cres;
@@ -1209,6 +1232,13 @@ __global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *M
 	{
 		long long s = 1;
 		for( int d=Dim-1 ; d>=0 ; d-- ){ fstride[d] = s ; s *= fres+1; }
+	}
+	if( fmaskC )   // adapted levels: only indexed coarse functions have rows and columns
+	{
+		bool in = fmaskC[j]!=0;
+		long long ii = 0;
+		for( int d=0 ; d<Dim ; d++ ){ in = in && I[d]>=0 && I[d]<=cres ; ii = ii*(cres+1) + I[d]; }
+		if( !in || !fmaskC[ii] ){ Mc[out] = 0 ; return; }
 	}
 	for( int d=0 ; d<Dim ; d++ )
 	{
@@ -1270,7 +1300,7 @@ __global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLds( LevelView<
 			const long long i = FlatIndex< Dim >( F , L.stride );
 			double q[5];
 			NodeGroup< Dim , MODE_GS >( Ls , F , i , L.lay.cstart[c]+k , lane , q );
-			if( lane==0 )
+			if( lane==0 && ( !L.fmask || L.fmask[i]==1 ) )
 			{
 				double dx , dy;
 				if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) sx[i] += dx , sy[i] += dy;
@@ -1572,7 +1602,7 @@ __global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView<
 		q20 = RowSum( q20 ) , q02 = RowSum( q02 ) , q11 = RowSum( q11 ) , q10 = RowSum( q10 ) , q01 = RowSum( q01 );
 		double dx , dy;
 		const bool step = NodeStepFast( q20 , q02 , q11 , q10 , q01 , dx , dy );
-		if( valid && rl==RC && step ) sx[i] = wx[MC] + dx , sy[i] = wy[MC] + dy;
+		if( valid && rl==RC && step && ( !L.fmask || L.fmask[i]==1 ) ) sx[i] = wx[MC] + dx , sy[i] = wy[MC] + dy;
 		__syncthreads();
 	}
 	for( int i=threadIdx.x ; i<N ; i+=blockDim.x ) L.x[i] = sx[i] , L.y[i] = sy[i];

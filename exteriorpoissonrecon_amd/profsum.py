@@ -1,7 +1,9 @@
 """Summaries of rocprofv3 CSV output for the profiles/ folder.
 
-    python -m exteriorpoissonrecon_amd.profsum stats <dir> [--top N]
-        per-kernel table from a `rocprofv3 --kernel-trace --stats --output-format csv` run
+    python -m exteriorpoissonrecon_amd.profsum stats <dir> [--top N] [--kernel NAME]
+        per-kernel table from a `rocprofv3 --kernel-trace --stats --output-format csv` run;
+        with --kernel, also the duration of the largest-grid dispatches of NAME
+        (the finest level) from the per-dispatch trace
     python -m exteriorpoissonrecon_amd.profsum traffic <fetch_dir> <write_dir> --kernel NAME --copy-bytes B
         HBM bytes per dispatch of the largest-grid dispatches of kernel NAME from
         two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs, scaled by what
@@ -44,6 +46,28 @@ def stats(d, top=25):
                    f"{100 * float(r['TotalDurationNs']) / total:.1f} |")
     out.append(f"\nTotal kernel time {total / 1e6:.3f} ms over {sum(int(r['Calls']) for r in rows)} dispatches.")
     return "\n".join(out)
+
+
+def finest(d, kernel):
+    """Durations of the largest-grid dispatches of `kernel` in the kernel trace."""
+    rows = []
+    for f in _files(d, "kernel_trace.csv"):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if kernel in r["Kernel_Name"]:
+                    g = int(r["Grid_Size_X"]) * int(r["Grid_Size_Y"]) * int(r["Grid_Size_Z"])
+                    rows.append((g, int(r["End_Timestamp"]) - int(r["Start_Timestamp"]), r))
+    if not rows:
+        raise SystemExit(f"no dispatches of {kernel} under {d}")
+    gmax = max(g for g, _, _ in rows)
+    big = [(g, t, r) for g, t, r in rows if g * 4 >= gmax]
+    ts = sorted(t for _, t, _ in big)
+    r0 = big[0][2]
+    return (f"{kernel}, largest-grid dispatches (grid sizes {sorted({g for g, _, _ in big})}): {len(ts)} dispatches, "
+            f"avg {sum(ts) / len(ts) / 1e3:.1f} us, median {ts[len(ts) // 2] / 1e3:.1f} us, "
+            f"min {ts[0] / 1e3:.1f} us, max {ts[-1] / 1e3:.1f} us; "
+            f"VGPR {r0['VGPR_Count']}, AGPR {r0['Accum_VGPR_Count']}, SGPR {r0['SGPR_Count']}, "
+            f"LDS {r0['LDS_Block_Size']} B, scratch {r0['Scratch_Size']} B, workgroup {r0['Workgroup_Size_X']}")
 
 
 def _counters(d, counter):
@@ -97,13 +121,19 @@ def main(argv=None):
     s = sub.add_parser("stats")
     s.add_argument("dir")
     s.add_argument("--top", type=int, default=25)
+    s.add_argument("--kernel", action="append", default=[])
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
     a = ap.parse_args(argv)
-    print(stats(a.dir, a.top) if a.cmd == "stats" else traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_bytes))
+    if a.cmd == "stats":
+        print(stats(a.dir, a.top))
+        for k in a.kernel:
+            print("\n" + finest(a.dir, k))
+    else:
+        print(traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_bytes))
 
 
 if __name__ == "__main__":

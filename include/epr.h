@@ -27,8 +27,7 @@ extern "C" {
 typedef struct epr_ctx epr_ctx;
 
 /* Command-line parameters of ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:52-78
- * (same names, same defaults; aRadius is implied: this library is the dense
- * --aRadius -1 path). */
+ * (same names, same defaults). */
 typedef struct
 {
 	int    depth;          /* --depth 5 */
@@ -45,6 +44,9 @@ typedef struct
 	double sWeight;        /* --sWeight 100 (screening, scales R) */
 	double dWeight;        /* --dWeight 0.00625 (Dirichlet, scales the stiffness) */
 	double iMult;          /* --iMult 2 */
+	int    aRadius;        /* --aRadius 3: -1 selects the dense grid (HierarchicalRegularIndexer),
+	                          0..5 the adapted hierarchy of that radius (HierarchicalAdaptedIndexer,
+	                          Include/Hat.adaptive.h:208-331) */
 } epr_params;
 
 /* Fills the reference defaults (ExteriorPoissonRecon.cpp:56-72). */
@@ -68,7 +70,11 @@ int  epr_set_samples( epr_ctx *ctx , size_t n , const double *pos , const double
  * (Include/GridSamples.inl:740-896), the skew splat and target field
  * (ExteriorPoisson.h:175-264, 290-309), R = deltaMass (Hat.scalars.inl:976-1063),
  * B = stiffness(S) (Hat.scalars.inl:1584-1690) and the Galerkin chain
- * (Include/SystemEnergy.h:464-477). */
+ * (Include/SystemEnergy.h:464-477).  With aRadius >= 0 it first builds the
+ * adapted hierarchy (HierarchicalAdaptedIndexer's constructor,
+ * Include/Hat.adaptive.inl:349-376, 587-729): levels keep their dense node
+ * layout, functions outside the hierarchy hold 0, and only the active
+ * functions are updated. */
 int  epr_build( epr_ctx *ctx );
 
 /* Sets the coefficients of one level; x0/y0 NULL selects the reference's
@@ -93,7 +99,8 @@ int  epr_energies( epr_ctx *ctx , int level , double out[2] );
  * (E_fit, E_reg) at maxSolveDepth. */
 int  epr_solve( epr_ctx *ctx , double out_E[2] );
 
-/* Number of hat functions (2^level+1)^dim of a level. */
+/* Number of hat functions (2^level+1)^dim of a level (the dense layout in
+ * which coefficients are exchanged, also for adapted reconstructions). */
 size_t epr_num_functions( const epr_ctx *ctx , int level );
 
 /* Copies the coefficients of one level to/from the host (Reconstructor::x/y,
@@ -110,15 +117,28 @@ int  epr_quadratic_fits( epr_ctx *ctx , int level , double *q );
 int  epr_apply( epr_ctx *ctx , int level , char which , const double *x , double *y );
 
 /* Named build products: unit_pos, cells, cell_counts, measure, noise,
- * sample_depth, S_elem, c, unit_to_world.  Writes min(count,cap) values when
- * out is not NULL and returns the count in *count. */
+ * sample_depth, S_elem, c, unit_to_world; for adapted reconstructions also
+ * adapted_fmask_<d> (0 not indexed, 1 active, 2 padding, per dense node),
+ * adapted_emask_<d>, adapted_forder_<d> and adapted_eorder_<d> (dense index of
+ * each function / element in the hierarchy's numbering) and adapted_nactive_<d>.
+ * Writes min(count,cap) values when out is not NULL and returns the count in
+ * *count. */
 int  epr_get_array( epr_ctx *ctx , const char *name , double *out , size_t cap , size_t *count );
 
 /* Writes <path> in the reference's .tree layout
  * (ExteriorPoissonRecon.cpp:259-270) from the hierarchical coefficients left
  * by epr_solve; indexer_hash is the reference build's
- * typeid(HierarchicalRegularIndexer<Dim>).hash_code(). */
+ * typeid(HierarchicalRegularIndexer<Dim>).hash_code() or, for adapted
+ * reconstructions, typeid(HierarchicalAdaptedIndexer<Dim>).hash_code().
+ * Adapted trees carry the coefficients in the hierarchy's numbering followed
+ * by HierarchicalAdaptedIndexer::write (Include/Hat.adaptive.inl:408-414). */
 int  epr_write_tree( epr_ctx *ctx , const char *path , unsigned long long indexer_hash );
+
+/* Context-free: builds the adapted hierarchy of n world positions (the
+ * HierarchicalAdaptedIndexer constructor, Include/Hat.adaptive.inl:349-376,
+ * 587-729, after the unit-cube map with the given bounding-box scale) and
+ * writes its serialisation (Hat.adaptive.inl:408-414) to path.  Host only. */
+int  epr_adapted_indexer_write( int dim , int depth , int aRadius , double scale , size_t n , const double *pos , const char *path );
 
 /* Writes the finest-level (phi1,phi2) values as a .grid file
  * (Misha/RegularGrid.inl:440-471 layout, 2 DOUBLE values per node). */

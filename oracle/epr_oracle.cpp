@@ -483,7 +483,20 @@ struct Recon : public Base
 		std::unique_ptr< ScalarFullStencil< Dim > > K;
 		std::unique_ptr< WedgeFullStencil< Dim > > W;
 		std::vector< std::vector< unsigned int > > colours;
+		// adapted hierarchy (Hat.adaptive.inl:349-729), empty for the dense grid:
+		// function flags (0 not indexed, 1 active, 2 padding), element flags, and
+		// the rank of every indexed function in the hierarchy's numbering
+		std::vector< unsigned char > fmask , emask;
+		std::vector< long long > rank;
+		std::vector< size_t > order;
+		bool indexed( size_t f ) const { return fmask.empty() || fmask[f]!=0; }
+		bool active( size_t f ) const { return fmask.empty() || fmask[f]==1; }
+		bool elementIndexed( size_t e ) const { return emask.empty() || emask[e]!=0; }
 	};
+	// set before build() for an adapted reconstruction
+	struct AdaptedInput { const unsigned char *fmask , *emask ; const long long *forder ; const size_t *nforder; };
+	const AdaptedInput *adaptedIn = nullptr;
+	bool adapted( void ) const { return !levels.empty() && !levels[0].fmask.empty(); }
 	std::vector< Level > levels;
 	double c_;
 
@@ -500,7 +513,7 @@ struct Recon : public Base
 		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ){ out[i][j] = 0 ; for( int k=0 ; k<=Dim ; k++ ) out[i][j] += B[i][k]*A[k][j]; }
 	}
 
-	Recon( const orc_params &p , size_t _n , const double *wpos , const double *wskew ) : prm(p) , D(p.depth) , n(_n)
+	Recon( const orc_params &p , size_t _n , const double *wpos , const double *wskew , const AdaptedInput *ad=nullptr ) : prm(p) , D(p.depth) , n(_n) , adaptedIn(ad)
 	{
 		if( !n ) throw std::string( "Expected some samples: 0" );
 		if( prm.tRadius<0 || prm.tRadius>4 ) throw std::string( "Kernel radius must be in range [0,4]" );
@@ -757,9 +770,19 @@ struct Recon : public Base
 			ElementTables< Dim > tab( L.res );
 			L.K.reset( new ScalarFullStencil< Dim >( tab.stiffness , L.res ) );
 			L.W.reset( new WedgeFullStencil< Dim >( tab.wedgeMass , L.res ) );
-			// colour lists (ExteriorPoisson.h:152-171)
+			if( adaptedIn )
+			{
+				size_t fo = 0 , eo = 0 , oo = 0;
+				for( int k=0 ; k<l ; k++ ){ fo += NumF( 1<<k ) ; eo += NumE( 1<<k ) ; oo += adaptedIn->nforder[k]; }
+				L.fmask.assign( adaptedIn->fmask+fo , adaptedIn->fmask+fo+L.N );
+				L.emask.assign( adaptedIn->emask+eo , adaptedIn->emask+eo+L.Ne );
+				L.rank.assign( L.N , -1 );
+				L.order.resize( adaptedIn->nforder[l] );
+				for( size_t i=0 ; i<adaptedIn->nforder[l] ; i++ ){ L.order[i] = (size_t)adaptedIn->forder[oo+i] ; L.rank[ L.order[i] ] = (long long)i; }
+			}
+			// colour lists over the active functions (ExteriorPoisson.h:152-171)
 			L.colours.resize( NC );
-			for( size_t f=0 ; f<L.N ; f++ )
+			for( size_t f=0 ; f<L.N ; f++ ) if( L.active( f ) )
 			{
 				Ix< Dim > F = FIndex< Dim >( f , L.res );
 				unsigned int mc = 0;
@@ -780,9 +803,9 @@ struct Recon : public Base
 			Se[e] = fieldAt( p );
 			for( int c=0 ; c<CD ; c++ ) S[e*CD+c] = Se[e].p[c];
 		}
-		// c = sum ||S||^2 / #elements (SystemEnergy.h:455-461)
+		// c = sum ||S||^2 over the indexed elements / #elements (SystemEnergy.h:455-461)
 		c_ = 0;
-		for( size_t e=0 ; e<F.Ne ; e++ ) c_ += Se[e].squareNorm();
+		for( size_t e=0 ; e<F.Ne ; e++ ) if( F.elementIndexed( e ) ) c_ += Se[e].squareNorm();
 		c_ /= F.Ne;
 		buildR( F );
 		buildB( F , Se );
@@ -875,18 +898,21 @@ struct Recon : public Base
 		ElementTables< Dim > tab( res );
 		for( size_t f1=0 ; f1<F.N ; f1++ )
 		{
+			if( !F.indexed( f1 ) ) continue;
 			Ix< Dim > F1 = FIndex< Dim >( f1 , res );
 			for( int w=0 ; w<NW ; w++ )
 			{
 				Ix< Dim > F2 = F1 + W3< Dim >( w );
 				bool in = true;
 				for( int d=0 ; d<Dim ; d++ ) if( F2[d]<0 || F2[d]>res ) in = false;
-				if( !in ) continue;
+				if( !in || !F.indexed( FIndex< Dim >( F2 , res ) ) ) continue;
 				double dot = 0;
 				Ix< Dim > elo , ehi;
 				for( int d=0 ; d<Dim ; d++ ) elo[d] = std::max( 0 , std::max( F1[d] , F2[d] )-1 ) , ehi[d] = std::min( res , std::min( F1[d] , F2[d] )+1 );
 				ForBox< Dim >( elo , ehi , [&]( const Ix< Dim > &E )
 				{
+					// adapted: elements outside the hierarchy contribute a zero frame (Hat.scalars.inl:1640-1650)
+					if( !F.elementIndexed( EIndex< Dim >( E , res ) ) ) return;
 					double m[Dim][Dim];
 					Se[ EIndex< Dim >( E , res ) ].full( m );
 					int i1 = Rel( E , F1 ) , i2 = Rel( E , F2 );
@@ -901,12 +927,16 @@ struct Recon : public Base
 	}
 
 	// ---- prolongation x_f = sum_c P(f,c) x_c (Hat.scalars.inl:373-404, 1694-1744): y[f] accumulated over coarse c in increasing order
+	//      adapted: P has rows for the indexed fine functions and columns for the
+	//      indexed coarse ones; the sparse product accumulates over the columns in
+	//      the hierarchy's numbering
 	void prolong( int l , const double *xc , double *xf ) const
 	{
 		const Level &C = levels[l] , &F = levels[l+1];
 		for( size_t f=0 ; f<F.N ; f++ ) xf[f] = 0;
-		for( size_t c=0 ; c<C.N ; c++ )
+		for( size_t ci=0 ; ci<( adapted() ? C.order.size() : C.N ) ; ci++ )
 		{
+			const size_t c = adapted() ? C.order[ci] : ci;
 			Ix< Dim > Cc = FIndex< Dim >( c , C.res );
 			for( int w=0 ; w<NW ; w++ )
 			{
@@ -914,7 +944,7 @@ struct Recon : public Base
 				double val = 1;
 				bool in = true;
 				for( int d=0 ; d<Dim ; d++ ){ G[d] = 2*Cc[d]+o[d] ; if( G[d]<0 || G[d]>F.res ) in = false ; if( o[d] ) val *= 0.5; }
-				if( !in ) continue;
+				if( !in || !F.indexed( FIndex< Dim >( G , F.res ) ) ) continue;
 				xf[ FIndex< Dim >( G , F.res ) ] += val * xc[c];
 			}
 		}
@@ -935,31 +965,55 @@ struct Recon : public Base
 		{
 			// T = P^T Mf: T(i,k) = sum over stored rows m of Mf(:,k) (increasing) of P(m,i)*Mf(m,k)
 			// Mc(i,j) = sum over k in P(:,j) (increasing) of T(i,k)*P(k,j)
+			// inner indices in increasing order: dense index, or the hierarchy's numbering
+			auto Ordered = [&]( const Level &L , const Ix< Dim > &base , int scale , int *ws )
+			{
+				int n = 0;
+				long long key[NW];
+				for( int w=0 ; w<NW ; w++ )
+				{
+					Ix< Dim > G; bool in = true;
+					for( int d=0 ; d<Dim ; d++ ){ G[d] = scale*base[d] + W3< Dim >( w )[d] ; if( G[d]<0 || G[d]>L.res ) in = false; }
+					if( !in ) continue;
+					size_t g = FIndex< Dim >( G , L.res );
+					if( !L.indexed( g ) ) continue;
+					key[n] = adapted() ? L.rank[g] : (long long)g;
+					ws[n++] = w;
+				}
+				for( int a=1 ; a<n ; a++ ) for( int b=a ; b>0 && key[b-1]>key[b] ; b-- ){ std::swap( key[b-1] , key[b] ) ; std::swap( ws[b-1] , ws[b] ); }
+				return n;
+			};
 			for( size_t j=0 ; j<C.N ; j++ )
 			{
+				if( !C.indexed( j ) ) continue;
 				Ix< Dim > J = FIndex< Dim >( j , C.res );
+				int wks[NW];
+				const int nk = Ordered( F , J , 2 , wks );
+				int wmss[NW][NW] , nms[NW];
+				for( int ik=0 ; ik<nk ; ik++ )
+				{
+					Ix< Dim > Kf; for( int d=0 ; d<Dim ; d++ ) Kf[d] = 2*J[d] + W3< Dim >( wks[ik] )[d];
+					nms[ik] = Ordered( F , Kf , 1 , wmss[ik] );
+				}
 				for( int wi=0 ; wi<NW ; wi++ )
 				{
 					Ix< Dim > I = J + W3< Dim >( wi );
 					bool in = true;
 					for( int d=0 ; d<Dim ; d++ ) if( I[d]<0 || I[d]>C.res ) in = false;
-					if( !in ) continue;
+					if( !in || !C.indexed( FIndex< Dim >( I , C.res ) ) ) continue;
 					double acc = 0; bool any = false;
-					for( int wk=0 ; wk<NW ; wk++ )   // k over fine children of j, increasing
+					for( int ik=0 ; ik<nk ; ik++ )   // k over fine children of j, increasing
 					{
+						const int wk = wks[ik];
 						Ix< Dim > Kf; for( int d=0 ; d<Dim ; d++ ) Kf[d] = 2*J[d] + W3< Dim >( wk )[d];
-						bool kin = true;
-						for( int d=0 ; d<Dim ; d++ ) if( Kf[d]<0 || Kf[d]>F.res ) kin = false;
-						if( !kin ) continue;
 						double pkj ; P( Kf , J , pkj );
 						size_t k = FIndex< Dim >( Kf , F.res );
 						double t = 0; bool tany = false;
-						for( int wm=0 ; wm<NW ; wm++ )
+						const int nm = nms[ik] , *wms = wmss[ik];
+						for( int im=0 ; im<nm ; im++ )
 						{
+							const int wm = wms[im];
 							Ix< Dim > Mi = Kf + W3< Dim >( wm );
-							bool min_ = true;
-							for( int d=0 ; d<Dim ; d++ ) if( Mi[d]<0 || Mi[d]>F.res ) min_ = false;
-							if( !min_ ) continue;
 							double pmi;
 							if( !P( Mi , I , pmi ) ) continue;
 							double mv = Mf[ k*NW+wm ];
@@ -1197,7 +1251,8 @@ struct Recon : public Base
 	{
 		int l = minD();
 		srand( prm.seed );
-		for( size_t i=0 ; i<levels[l].N ; i++ ) x[i] = Random() , y[i] = Random();
+		if( adapted() ){ for( size_t f : levels[l].order ) x[f] = Random() , y[f] = Random(); }   // the hierarchy's numbering
+		else for( size_t i=0 ; i<levels[l].N ; i++ ) x[i] = Random() , y[i] = Random();
 		double s = wedge( l , x , y );
 		s = sqrt( sqrt( c_ / s ) );
 		for( size_t i=0 ; i<levels[l].N ; i++ ) x[i] *= s , y[i] *= s;
@@ -1283,6 +1338,8 @@ struct Recon : public Base
 			for( int d=0 ; d<Dim ; d++ ) g2c[d][d] = 1./(1<<D);
 			MatMul( unitCubeToWorld , g2c , v2w );
 		}
+		const unsigned long long vptr = 0;   // XForm written raw: the vtable pointer of its Algebra base first (Misha/Algebra.h:168)
+		fwrite( &vptr , sizeof(vptr) , 1 , fp );
 		fwrite( v2w , sizeof(double) , (Dim+1)*(Dim+1) , fp );
 		for( int pass=0 ; pass<2 ; pass++ )
 		{
@@ -1316,10 +1373,17 @@ void orc_default_params( orc_params *p )
 
 void *orc_create( int dim , const orc_params *p , size_t n , const double *pos , const double *skew )
 {
+	return orc_create_adapted( dim , p , n , pos , skew , nullptr , nullptr , nullptr , nullptr );
+}
+
+void *orc_create_adapted( int dim , const orc_params *p , size_t n , const double *pos , const double *skew ,
+                          const unsigned char *fmask , const unsigned char *emask , const long long *forder , const size_t *nforder )
+{
 	try
 	{
-		if( dim==3 ) return static_cast< Base* >( new Recon< 3 >( *p , n , pos , skew ) );
-		if( dim==4 ) return static_cast< Base* >( new Recon< 4 >( *p , n , pos , skew ) );
+		if( fmask && ( !emask || !forder || !nforder ) ){ g_error = "incomplete hierarchy" ; return nullptr; }
+		if( dim==3 ){ Recon< 3 >::AdaptedInput a{ fmask , emask , forder , nforder } ; return static_cast< Base* >( new Recon< 3 >( *p , n , pos , skew , fmask ? &a : nullptr ) ); }
+		if( dim==4 ){ Recon< 4 >::AdaptedInput a{ fmask , emask , forder , nforder } ; return static_cast< Base* >( new Recon< 4 >( *p , n , pos , skew , fmask ? &a : nullptr ) ); }
 		g_error = "dimension must be 3 or 4";
 	}
 	catch( const std::string &s ){ g_error = s; }

@@ -1,5 +1,6 @@
 /*
- * CPU oracle for the ExteriorPoissonRecon solve path (dense grid, --aRadius -1).
+ * CPU oracle for the ExteriorPoissonRecon solve path (dense grid, --aRadius -1,
+ * and adapted hierarchies given by the reference's own tree).
  *
  * TEST INFRASTRUCTURE ONLY.  This is a from-scratch FP64 restatement of the
  * reference algorithm, written to reproduce the reference's arithmetic in the
@@ -46,6 +47,18 @@ void   orc_default_params( orc_params *p );
  * pos: n*dim world positions; skew: n*dim*dim full frame matrices m[c][r]
  * (PLY properties skew_c_r).  Returns NULL on error (message via orc_error). */
 void  *orc_create( int dim , const orc_params *p , size_t n , const double *pos , const double *skew );
+/* The same for an adapted hierarchy (ExteriorPoissonRecon --aRadius K,
+ * Include/Hat.adaptive.inl:349-729), given as it appears in a reference .tree:
+ * per level l = 0..depth concatenated, fmask holds (2^l+1)^dim function flags
+ * (0 not indexed, 1 active, 2 padding; coordinate 0 most significant), emask
+ * (2^l)^dim element flags (coordinate 0 least significant), and forder the
+ * nforder[l] dense indices of the indexed functions in the hierarchy's
+ * numbering.  Coefficients stay in the dense layout, zero outside the
+ * hierarchy; only active functions are relaxed; B integrates indexed elements
+ * only; c sums indexed elements only; P and the Galerkin products keep indexed
+ * rows and columns and accumulate in the hierarchy's numbering. */
+void  *orc_create_adapted( int dim , const orc_params *p , size_t n , const double *pos , const double *skew ,
+                           const unsigned char *fmask , const unsigned char *emask , const long long *forder , const size_t *nforder );
 void   orc_destroy( void *h );
 const char *orc_error( void );
 

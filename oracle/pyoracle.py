@@ -39,6 +39,10 @@ def lib():
         L.orc_default_params.argtypes = [ctypes.POINTER(Params)]
         L.orc_create.restype = vp
         L.orc_create.argtypes = [ctypes.c_int, ctypes.POINTER(Params), sz, dp, dp]
+        L.orc_create_adapted.restype = vp
+        L.orc_create_adapted.argtypes = [ctypes.c_int, ctypes.POINTER(Params), sz, dp, dp,
+                                         ctypes.POINTER(ctypes.c_ubyte), ctypes.POINTER(ctypes.c_ubyte),
+                                         ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(sz)]
         L.orc_destroy.argtypes = [vp]
         L.orc_error.restype = ctypes.c_char_p
         L.orc_num_functions.restype = sz
@@ -79,15 +83,61 @@ def default_params(**kw):
     return p
 
 
-class Oracle:
-    """One reconstruction built by the CPU oracle."""
+def hierarchy_from_tree(fields, dim, depth):
+    """Per-level (fmask, emask, forder) of an adapted hierarchy from the node
+    records of a reference .tree (exteriorpoissonrecon_amd.treefile fields or a
+    tests/golden/adapted_*.npz): function flags over the dense (2^l+1)^dim
+    layout (coordinate 0 most significant; 1 active, 2 padding), element flags
+    over (2^l)^dim (coordinate 0 least significant), and the dense index of
+    every function in the hierarchy's numbering."""
+    nd, off = np.asarray(fields["node_depth"]), np.asarray(fields["node_offset"]).astype(np.int64)
+    nf, ne = np.asarray(fields["node_function"]), np.asarray(fields["node_element"])
+    act = np.asarray(fields["active_sizes"])
+    out = []
+    for l in range(depth + 1):
+        res = 1 << l
+        sel = nd == l + 1
+        fsel = sel & (nf >= 0)
+        F = off[fsel]
+        fl = np.zeros(len(F), dtype=np.int64)
+        for k in range(dim):
+            fl = fl * (res + 1) + F[:, k]
+        order = np.zeros(len(F), dtype=np.int64)
+        order[nf[fsel]] = fl
+        fmask = np.zeros((res + 1) ** dim, dtype=np.uint8)
+        fmask[order[:act[l]]] = 1
+        fmask[order[act[l]:]] = 2
+        E = off[sel & (ne >= 0)]
+        el = np.zeros(len(E), dtype=np.int64)
+        for k in range(dim - 1, -1, -1):
+            el = el * res + E[:, k]
+        emask = np.zeros(res ** dim, dtype=np.uint8)
+        emask[el] = 1
+        out.append((fmask, emask, order))
+    return out
 
-    def __init__(self, pos, skew, **params):
+
+class Oracle:
+    """One reconstruction built by the CPU oracle (hierarchy: the result of
+    hierarchy_from_tree for an adapted reconstruction)."""
+
+    def __init__(self, pos, skew, hierarchy=None, **params):
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         skew = np.ascontiguousarray(skew, dtype=np.float64)
         self.dim = pos.shape[1]
         self.params = default_params(**params)
-        self.h = lib().orc_create(self.dim, ctypes.byref(self.params), pos.shape[0], _p(pos), _p(skew))
+        if hierarchy is None:
+            self.h = lib().orc_create(self.dim, ctypes.byref(self.params), pos.shape[0], _p(pos), _p(skew))
+        else:
+            fm = np.ascontiguousarray(np.concatenate([h[0] for h in hierarchy]), dtype=np.uint8)
+            em = np.ascontiguousarray(np.concatenate([h[1] for h in hierarchy]), dtype=np.uint8)
+            fo = np.ascontiguousarray(np.concatenate([h[2] for h in hierarchy]), dtype=np.int64)
+            no = np.ascontiguousarray([len(h[2]) for h in hierarchy], dtype=np.uint64)
+            ub = ctypes.POINTER(ctypes.c_ubyte)
+            self.h = lib().orc_create_adapted(self.dim, ctypes.byref(self.params), pos.shape[0], _p(pos), _p(skew),
+                                              fm.ctypes.data_as(ub), em.ctypes.data_as(ub),
+                                              fo.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                              no.ctypes.data_as(ctypes.POINTER(ctypes.c_size_t)))
         if not self.h:
             raise RuntimeError(lib().orc_error().decode())
         self.depth = self.params.depth

@@ -22,7 +22,27 @@ def load_golden(name):
 
 
 def case_args(g):
-    return {k[4:]: (g[k].item()) for k in g if k.startswith("arg_")}
+    """Reference command-line arguments of a fixture; fixtures without aRadius are dense-grid runs."""
+    args = {k[4:]: (g[k].item()) for k in g if k.startswith("arg_")}
+    args.setdefault("aRadius", -1)
+    return args
+
+
+def read_grid_values(path):
+    """Values of a RegularGrid file (Misha/RegularGrid.inl:440-471): "G<dim>",
+    "<n> DOUBLE", the resolutions, dim+1 transform rows, then the binary values
+    (an array of shape (count,) or (count, n))."""
+    b = open(path, "rb").read()
+    o = 0
+    lines = []
+    while len(lines) < 3:
+        e = b.index(b"\n", o); lines.append(b[o:e].decode()); o = e + 1
+    dim = int(lines[0][1:])
+    nv = int(lines[1].split()[0])
+    for _ in range(dim + 1):
+        o = b.index(b"\n", o) + 1
+    v = np.frombuffer(b, dtype="<f8", offset=o)
+    return v if nv == 1 else v.reshape(-1, nv)
 
 
 def rel_err(a, b):

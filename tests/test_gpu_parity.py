@@ -14,7 +14,8 @@ import struct
 import numpy as np
 import pytest
 
-from conftest import CASES, ROOT, case_args, load_golden, rel_err
+from conftest import CASES, ROOT, case_args, load_golden, read_grid_values, rel_err
+from exteriorpoissonrecon_amd.treefile import TreeFile
 
 pytestmark = pytest.mark.gpu
 
@@ -106,7 +107,7 @@ def test_solve_matches_oracle_larger(dim, depth, oracle_lib):
     print(f"[gpu] oracle vs gpu dim={dim} depth={depth}", flush=True)
     o = oracle_lib.Oracle(pos, skew, depth=depth)
     so = o.solve()
-    r = Reconstructor(pos, skew, depth=depth).build()
+    r = Reconstructor(pos, skew, depth=depth, aRadius=-1).build()
     E = r.solve()
     assert rel_err(E, so["final_E"]) < TOL_SOLVE
     for d in range(depth + 1):
@@ -121,7 +122,7 @@ def test_colour_phases_match_single_workgroup_path():
     kernel gives the same iterates (colour independence, ExteriorPoisson.h:161-169)."""
     from exteriorpoissonrecon_amd import Reconstructor, samples
     pos, skew = samples.torus_knot(1024)
-    r = Reconstructor(pos, skew, depth=5).build()
+    r = Reconstructor(pos, skew, depth=5, aRadius=-1).build()
     rng = np.random.default_rng(0)
     # level 4 (4913 nodes) runs in one workgroup, level 5 (35937) with per-colour launches;
     # compare level 5 against the oracle instead of itself: one iteration from random data
@@ -148,7 +149,7 @@ def test_sweeps_every_level_match_oracle(dim, depth, oracle_lib):
         pos, skew = samples.torus_knot(1024)
     else:
         pos, skew = samples.clifford_torus(side=24, seed=1)
-    r = Reconstructor(pos, skew, depth=depth).build()
+    r = Reconstructor(pos, skew, depth=depth, aRadius=-1).build()
     o = oracle_lib.Oracle(pos, skew, depth=depth)
     rng = np.random.default_rng(5)
     for d in range(depth + 1):
@@ -174,19 +175,17 @@ def test_cli_tree_output(tmp_path, oracle_lib):
     assert res.returncode == 0 and "[ERROR]" not in res.stderr, res.stderr
     b = open(out + ".tree", "rb").read()
     assert b[:12] == g["tree_head"].tobytes()          # dim + reference indexer hash
-    o = 12 + 8 * 16
-    hx = []
-    for pass_ in range(2):
-        nl = struct.unpack_from("<Q", b, o)[0]; o += 8
-        assert nl == 5
-        for d in range(nl):
-            n = struct.unpack_from("<q", b, o)[0]; o += 8
-            v = np.frombuffer(b, dtype="<f8", count=n, offset=o); o += 8 * n
-            if pass_ == 0:
-                hx.append(v)
-    assert struct.unpack_from("<I", b, o)[0] == 4 and o + 4 == len(b)
+    t = TreeFile(out + ".tree")
+    assert not t.adapted and t.max_depth == 4 and len(t.x) == 5
     scale = max(np.abs(g["final_x"]).max(), 1.0)
     for d in range(5):
-        assert np.abs(hx[d] - g[f"hx{d}"]).max() < TOL_SOLVE * scale
+        assert np.abs(t.x[d] - g[f"hx{d}"]).max() < TOL_SOLVE * scale
+    # the reference's own Extract reads the file (its --grid output holds the regular coefficients)
+    ext = os.path.join(ROOT, "oracle", "_ref", "Extract")
+    if os.path.exists(ext):
+        res = subprocess.run([ext, "--in", out + ".tree", "--grid", str(tmp_path / "rg")], capture_output=True, text=True, timeout=300)
+        assert res.returncode == 0 and "ERROR" not in res.stderr, res.stderr
+        gx = read_grid_values(str(tmp_path / "rg.0.grid"))
+        assert np.abs(gx - g["final_x"]).max() < TOL_SOLVE * scale
     grid = open(str(tmp_path / "o.grid"), "rb").read()
     assert grid.startswith(b"G3\n2 DOUBLE\n17 17 17\n")

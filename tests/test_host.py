@@ -43,6 +43,7 @@ def test_default_params_match_reference():
     # ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:56-72
     assert (p.depth, p.seed, p.tRadius, p.dRadius, p.minSolveDepth, p.gsIters) == (5, 0, 1, 1, 0, 10)
     assert (p.nSigma, p.scale, p.sWeight, p.dWeight, p.iMult) == (0.25, 1.1, 100.0, 0.00625, 2.0)
+    assert p.aRadius == 3
 
 
 def test_ply_roundtrip(tmp_path):
@@ -67,7 +68,7 @@ def test_cli_usage_without_input():
 
 @pytest.mark.parametrize("args,msg", [
     (["--in", "x.txt"], "Only .ply files supported"),
-    (["--in", "missing.ply", "--aRadius", "3"], "only the dense grid"),
+    (["--in", "missing.ply", "--aRadius", "7"], "Only kernel radius in the range [0,5] supported: 7"),
     (["--in", "missing.ply"], "Invalid sample file"),
 ])
 def test_cli_errors_exit_zero(args, msg):
