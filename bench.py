@@ -12,6 +12,10 @@ Workloads (synthetic samples, exteriorpoissonrecon_amd.samples):
   clifford4d_d6   4D Clifford torus, 65536 samples, 2 deg angular noise, depth 6
   knot3d_d5       3D (2,3) torus knot, 1024 samples, depth 5
 
+--aRadius K (default -1, the dense grid the headline configurations name)
+measures the adapted hierarchy of radius K instead (the reference's default
+command line uses 3): the same kernels over the active functions only.
+
 Prints one JSON line (stdout); progress goes to stderr.
 """
 import argparse
@@ -54,11 +58,11 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def reference_energy(name):
+def reference_energy(name, a_radius=-1):
     path = os.path.join(ROOT, "tests", "golden", "bench_reference.json")
     if not os.path.exists(path):
         return None
-    return json.load(open(path)).get(name)
+    return json.load(open(path)).get(name if a_radius < 0 else f"{name}_r{a_radius}")
 
 
 COPY_BYTES = 1 << 30      # calibration copy: 1 GiB read + 1 GiB written per dispatch
@@ -86,7 +90,7 @@ def copy_bandwidth():
     return 2 * COPY_BYTES / (ms.value * 1e-3) / 1e9
 
 
-def cpu_baseline(name, wl, pos, skew):
+def cpu_baseline(name, wl, pos, skew, a_radius=-1):
     """The reference build (oracle/_ref/ExteriorPoissonRecon, compiled from the
     reference sources) timed on this host's cores on the same samples."""
     exe = os.path.join(ROOT, "oracle", "_ref", "ExteriorPoissonRecon")
@@ -103,12 +107,12 @@ def cpu_baseline(name, wl, pos, skew):
         if wl["dim"] == 4 and depth > 5:
             depth = 5
         t0 = time.perf_counter()
-        r = subprocess.run([exe, "--in", ply, "--depth", str(depth), "--aRadius", "-1"],
+        r = subprocess.run([exe, "--in", ply, "--depth", str(depth), "--aRadius", str(a_radius)],
                            capture_output=True, text=True, timeout=1800)
         dt = time.perf_counter() - t0
         if r.returncode != 0:
             return None
-    sample = f"{name} samples, reference CLI --depth {depth} --aRadius -1 (full build+solve), {cores} threads"
+    sample = f"{name} samples, reference CLI --depth {depth} --aRadius {a_radius} (full build+solve), {cores} threads"
     return {"value": dt, "unit": "s", "cores": cores, "kind": "reference", "sample": sample}
 
 
@@ -120,6 +124,7 @@ def main():
     ap.add_argument("--config", default="borromean3d_d7", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
+    ap.add_argument("--aRadius", type=int, default=-1, help="-1: dense grid; 0..5: adapted hierarchy of that radius")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -131,7 +136,7 @@ def main():
     pos, skew = wl["gen"](samples)
     dim, depth = wl["dim"], wl["depth"]
     log(f"[bench] {args.config}: {pos.shape[0]} samples, dim {dim}, depth {depth}")
-    rec = Reconstructor(pos, skew, depth=depth, aRadius=-1)
+    rec = Reconstructor(pos, skew, depth=depth, aRadius=args.aRadius)
 
     E = None
     for i in range(args.warmup):
@@ -147,7 +152,7 @@ def main():
         log(f"[bench] step {i}: {dt:.4f}s (build {builds[-1]:.4f} solve {solves[-1]:.4f}) E={E.tolist()}")
     t = float(np.mean(times))
 
-    Eref = reference_energy(args.config)
+    Eref = reference_energy(args.config, args.aRadius)
     rel = None
     if Eref is not None:
         rel = abs(float(E.sum()) - float(sum(Eref))) / abs(float(sum(Eref)))
@@ -158,7 +163,9 @@ def main():
     N = rec.num_functions(depth)
     ms, launches = rec.time_gauss_seidel(depth, iters)
     per_launch_s = ms * 1e-3 / launches
-    nodes_per_launch = N / (1 << dim)
+    # nodes relaxed per launch: a colour of the level (the active ones of an adapted level)
+    updated = N if args.aRadius < 0 else int(rec.array(f"adapted_nactive_{depth}")[0])
+    nodes_per_launch = updated / (1 << dim)
     flops_per_launch = F_NODE[dim] * nodes_per_launch
     bytes_per_launch = B_NODE[dim] * nodes_per_launch
     achieved = flops_per_launch / per_launch_s / 1e12
@@ -176,12 +183,12 @@ def main():
     copy_gbs = copy_bandwidth()
     if copy_gbs:
         log(f"[bench] copy kernel: {copy_gbs:.1f} GB/s ({2 * COPY_BYTES} bytes per copy)")
-    traffic = measured_traffic(args.config)
+    traffic = measured_traffic(args.config if args.aRadius < 0 else f"{args.config}_r{args.aRadius}")
     if traffic:
         log(f"[bench] measured HBM traffic of the finest-sweep kernel: {traffic / 1e6:.2f} MB per launch "
             f"({traffic / per_launch_s / 1e9:.1f} GB/s at this launch time)")
 
-    cpu = None if args.no_cpu else cpu_baseline(args.config, wl, pos, skew)
+    cpu = None if args.no_cpu else cpu_baseline(args.config, wl, pos, skew, args.aRadius)
     if cpu:
         log(f"[bench] cpu baseline {cpu['value']:.3f}s ({cpu['sample']})")
 
@@ -199,11 +206,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": args.config, "dim": dim, "depth": depth, "samples": int(pos.shape[0]),
-                   "grid_nodes": int(N), "build_s": float(np.mean(builds)), "solve_s": float(np.mean(solves)),
+                   "aRadius": args.aRadius, "grid_nodes": int(N), "active_nodes": int(updated), "build_s": float(np.mean(builds)), "solve_s": float(np.mean(solves)),
                    "final_energy": float(E.sum()), "energy_rel_err_vs_reference": rel},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": f"KGsPhase<{dim}> (finest level, one colour)",
+                     "kernel": f"{'KGsPhase' if args.aRadius < 0 else 'KGsPhaseList'}<{dim}> (finest level, one colour)",
                      "hbm_gbs_algorithmic": bytes_per_launch / per_launch_s / 1e9,
                      "flops_per_node": F_NODE[dim], "reference_equivalent_tflops": achieved_ref,
                      "us_per_launch": per_launch_s * 1e6, "copy_kernel_gbs": copy_gbs,

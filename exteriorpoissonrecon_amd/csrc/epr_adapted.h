@@ -81,15 +81,14 @@ struct AdaptedHierarchy
 	}
 	static void SortUnique( std::vector< unsigned long long > &v ){ std::sort( v.begin() , v.end() ) ; v.erase( std::unique( v.begin() , v.end() ) , v.end() ); }
 
-	// cells: ncell x Dim cell indices at depth D (the ordered sampler's cells)
-	void build( const int *cells , size_t ncell , int depth , int radius )
+	static void CheckArgs( int depth , int radius )
 	{
 		if( radius<0 || radius>5 ) throw std::string( "Only kernel radius in the range [0,5] supported: " + std::to_string( radius ) );
 		if( depth>14 ) throw std::string( "adapted depth must be at most 14" );
-		D = depth , K = radius;
-		levels.assign( D+1 , Level() );
-		expanded.assign( D+1 , std::vector< unsigned char >() );
-		// unique ancestors of the sample cells at every depth
+	}
+	// unique ancestors of the sample cells at every depth (packed offsets, sorted)
+	static std::vector< std::vector< unsigned long long > > Ancestors( const int *cells , size_t ncell , int D )
+	{
 		std::vector< std::vector< unsigned long long > > anc( D+1 );
 		for( size_t c=0 ; c<ncell ; c++ ) anc[D].push_back( Pack( cells + c*Dim ) );
 		SortUnique( anc[D] );
@@ -99,6 +98,23 @@ struct AdaptedHierarchy
 			for( unsigned long long k : anc[d+1] ){ int A[Dim] ; Unpack( k , A ) ; for( int dd=0 ; dd<Dim ; dd++ ) A[dd] >>= 1 ; anc[d].push_back( Pack( A ) ); }
 			SortUnique( anc[d] );
 		}
+		return anc;
+	}
+
+	// cells: ncell x Dim cell indices at depth D (the ordered sampler's cells)
+	void build( const int *cells , size_t ncell , int depth , int radius )
+	{
+		buildLevels( cells , ncell , depth , radius );
+		buildTree( cells , ncell );
+	}
+
+	// the per-depth function and element sets and their numbering
+	void buildLevels( const int *cells , size_t ncell , int depth , int radius )
+	{
+		CheckArgs( depth , radius );
+		D = depth , K = radius;
+		levels.assign( D+1 , Level() );
+		std::vector< std::vector< unsigned long long > > anc = Ancestors( cells , ncell , D );
 		// active functions, their depth-first order, then padding and elements by discovery
 		for( int d=0 ; d<=D ; d++ )
 		{
@@ -140,6 +156,14 @@ struct AdaptedHierarchy
 				} );
 			}
 		}
+	}
+
+	// the tree's shape (nodes with children) from the levels (however they were
+	// built: D, K and levels[] set) and the sample cells
+	void buildTree( const int *cells , size_t ncell )
+	{
+		std::vector< std::vector< unsigned long long > > anc = Ancestors( cells , ncell , D );
+		expanded.assign( D+1 , std::vector< unsigned char >() );
 		// nodes with children, by internal depth e (offsets in [0, 2^e))
 		for( int e=0 ; e<=D ; e++ ){ long long n = 1 ; for( int dd=0 ; dd<Dim ; dd++ ) n *= 1LL<<e ; expanded[e].assign( n , 0 ); }
 		expanded[0][0] = 1;

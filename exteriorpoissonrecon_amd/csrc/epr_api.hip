@@ -20,6 +20,7 @@
 
 #include "../../include/epr.h"
 #include "epr_adapted.h"
+#include "epr_adapted_dev.h"
 #include "epr_dev.h"
 #include "epr_hat.h"
 #include "epr_kernels.h"
@@ -155,14 +156,20 @@ struct Ctx : public CtxBase
 		long long stride[Dim];
 		ColLayout< Dim > lay;              // colour-grouped column order of R and B
 		DevBuf< double > x , y , R , B , K;
-		DevBuf< unsigned char > fm , em;   // adapted hierarchy: function (1 active, 2 padding) and element flags
+		// adapted hierarchy (empty for the dense grid): function flags (1 active,
+		// 2 padding), element flags, the numbering, the active nodes by colour
+		DevBuf< unsigned char > fm , em;
+		DevBuf< unsigned int > forder , eorder , act;
+		size_t nF = 0 , nActive = 0 , nE = 0;
+		long long actStart[(1<<Dim)+1] = {};
 		hipGraphExec_t graph = nullptr;    // captured Gauss-Seidel launches of this level
 		int graphIters = 0;
 		const double *graphX = nullptr , *graphR = nullptr;
 		const unsigned char *graphF = nullptr;
 		Level( void ){}
-		Level( Level &&o ) noexcept : res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , fm( std::move( o.fm ) ) , em( std::move( o.em ) ) , graph( o.graph ) , graphIters( o.graphIters )
+		Level( Level &&o ) noexcept : res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , fm( std::move( o.fm ) ) , em( std::move( o.em ) ) , forder( std::move( o.forder ) ) , eorder( std::move( o.eorder ) ) , act( std::move( o.act ) ) , nF( o.nF ) , nActive( o.nActive ) , nE( o.nE ) , graph( o.graph ) , graphIters( o.graphIters )
 		{
+			for( int c=0 ; c<=(1<<Dim) ; c++ ) actStart[c] = o.actStart[c];
 			for( int d=0 ; d<Dim ; d++ ) stride[d] = o.stride[d];
 			o.graph = nullptr;
 		}
@@ -178,9 +185,40 @@ struct Ctx : public CtxBase
 	std::vector< std::unique_ptr< DevBuf< double > > > estData , skData;
 	std::vector< std::unique_ptr< DevBuf< unsigned char > > > skKids , estKids;
 	double c_ = 0;
-	// the adapted hierarchy (--aRadius >= 0), built with the other build-phase data
-	std::unique_ptr< AdaptedHierarchy< Dim > > ah;
+	// --aRadius >= 0: the adapted hierarchy, built on the device with the other
+	// build-phase data (levels[].fm/em/forder/eorder/act); the host copy with
+	// the tree's shape is made only to write a .tree
 	bool adapted( void ) const { return prm.aRadius!=-1; }
+	std::unique_ptr< AdaptedHierarchy< Dim > > ah;
+	template< typename T > std::vector< T > download( const DevBuf< T > &b , size_t n )
+	{
+		std::vector< T > h( n );
+		b.download( h.data() , n , stream );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+		return h;
+	}
+	const AdaptedHierarchy< Dim > &hostHierarchy( void )
+	{
+		if( !ah )
+		{
+			ah.reset( new AdaptedHierarchy< Dim >() );
+			ah->D = D , ah->K = prm.aRadius;
+			ah->levels.resize( D+1 );
+			for( int l=0 ; l<=D ; l++ )
+			{
+				const Level &L = levels[l];
+				auto &A = ah->levels[l];
+				A.res = L.res , A.nActive = L.nActive;
+				A.fmask = download( L.fm , (size_t)L.N );
+				long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= L.res;
+				A.emask = download( L.em , (size_t)ne );
+				std::vector< unsigned int > fo = download( L.forder , L.nF ) , eo = download( L.eorder , L.nE );
+				A.fOrder.assign( fo.begin() , fo.end() ) , A.eOrder.assign( eo.begin() , eo.end() );
+			}
+			ah->buildTree( cellIdx.data() , (size_t)ncell );
+		}
+		return *ah;
+	}
 
 	explicit Ctx( const epr_params &p ) : prm( p ) , D( p.depth )
 	{
@@ -298,21 +336,27 @@ struct Ctx : public CtxBase
 		dCellStart.upload( cellStart.data() , cellStart.size() , stream );
 		dCellSamples.upload( cellSamples.data() , cellSamples.size() , stream );
 
-		// ---- the adapted hierarchy (Hat.adaptive.inl:349-729) and its flags on every level
+		// ---- the adapted hierarchy (Hat.adaptive.inl:349-729) on every level
+		ah.reset();
 		if( adapted() )
 		{
-			if( !ah ) ah.reset( new AdaptedHierarchy< Dim >() );
-			ah->build( cellIdx.data() , (size_t)ncell , D , prm.aRadius );
+			AdaptedHierarchy< Dim >::CheckArgs( D , prm.aRadius );
+			std::vector< AdaptedLevelDev > ad;
+			BuildAdaptedDevice< Dim >( dCellIdx.p , ncell , D , prm.aRadius , stream , ad );
 			for( int l=0 ; l<=D ; l++ )
 			{
-				levels[l].fm.upload( ah->levels[l].fmask.data() , ah->levels[l].fmask.size() , stream );
-				levels[l].em.upload( ah->levels[l].emask.data() , ah->levels[l].emask.size() , stream );
+				Level &L = levels[l];
+				L.fm = std::move( ad[l].fm ) , L.em = std::move( ad[l].em );
+				L.forder = std::move( ad[l].forder ) , L.eorder = std::move( ad[l].eorder ) , L.act = std::move( ad[l].act );
+				L.nF = ad[l].nF , L.nActive = ad[l].nActive , L.nE = ad[l].nE;
+				for( int c=0 ; c<=C ; c++ ) L.actStart[c] = ad[l].actStart[c];
 			}
 		}
-		else
+		else for( int l=0 ; l<=D ; l++ )
 		{
-			ah.reset();
-			for( int l=0 ; l<=D ; l++ ) levels[l].fm.alloc( 0 ) , levels[l].em.alloc( 0 );
+			Level &L = levels[l];
+			L.fm.alloc( 0 ) , L.em.alloc( 0 ) , L.forder.alloc( 0 ) , L.eorder.alloc( 0 ) , L.act.alloc( 0 );
+			L.nF = L.nActive = L.nE = 0;
 		}
 
 		// ---- tree estimator (GridSamples.inl:740-771)
@@ -494,8 +538,16 @@ struct Ctx : public CtxBase
 				HIPCHECK( hipStreamBeginCapture( stream , hipStreamCaptureModeThreadLocal ) );
 				for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
 				{
-					ColourSet< Dim > cs ; cs.set( L.res , c );
-					KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , v.lay.cstart[c] );
+					if( L.fm.p )   // adapted: the active nodes of the colour only
+					{
+						const long long cnt = L.actStart[c+1] - L.actStart[c];
+						if( cnt ) KGsPhaseList< Dim ><<< Blocks( cnt*C , 256 ) , 256 , 0 , stream >>>( v , L.act.p + L.actStart[c] , cnt );
+					}
+					else
+					{
+						ColourSet< Dim > cs ; cs.set( L.res , c );
+						KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , v.lay.cstart[c] );
+					}
 				}
 				HIPCHECK( hipStreamEndCapture( stream , &g ) );
 				HIPCHECK( hipGraphInstantiate( &L.graph , g , nullptr , nullptr , 0 ) );
@@ -585,7 +637,7 @@ struct Ctx : public CtxBase
 		// ExteriorPoisson.h:333-341
 		std::vector< double > hx( L.N , 0. ) , hy( L.N , 0. );
 		srand( prm.seed );
-		if( ah ){ for( long long f : ah->levels[l].fOrder ) hx[f] = Random() , hy[f] = Random(); }   // the hierarchy's numbering
+		if( adapted() ){ for( unsigned int f : download( L.forder , L.nF ) ) hx[f] = Random() , hy[f] = Random(); }   // the hierarchy's numbering
 		else for( long long i=0 ; i<L.N ; i++ ) hx[i] = Random() , hy[i] = Random();
 		L.x.upload( hx.data() , L.N , stream ) , L.y.upload( hy.data() , L.N , stream );
 		double t[4];
@@ -704,17 +756,20 @@ struct Ctx : public CtxBase
 		{
 			// adapted_{fmask,emask,forder,eorder}_<depth> (dense indices, in the hierarchy's numbering for the orders)
 			needBuilt();
-			if( !ah ) throw std::string( "not an adapted reconstruction" );
+			if( !adapted() ) throw std::string( "not an adapted reconstruction" );
 			const size_t u = name.rfind( '_' );
 			const int l = atoi( name.c_str()+u+1 );
 			checkLevel( l );
 			const std::string what = name.substr( 8 , u-8 );
-			const auto &A = ah->levels[l];
-			if( what=="fmask" ) v.assign( A.fmask.begin() , A.fmask.end() );
-			else if( what=="emask" ) v.assign( A.emask.begin() , A.emask.end() );
-			else if( what=="forder" ) v.assign( A.fOrder.begin() , A.fOrder.end() );
-			else if( what=="eorder" ) v.assign( A.eOrder.begin() , A.eOrder.end() );
-			else if( what=="nactive" ) v.push_back( (double)A.nActive );
+			const Level &L = levels[l];
+			long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= L.res;
+			auto put = [&]( const auto &h ){ v.assign( h.begin() , h.end() ); };
+			if( what=="fmask" ) put( download( L.fm , (size_t)L.N ) );
+			else if( what=="emask" ) put( download( L.em , (size_t)ne ) );
+			else if( what=="forder" ) put( download( L.forder , L.nF ) );
+			else if( what=="eorder" ) put( download( L.eorder , L.nE ) );
+			else if( what=="act" ) put( download( L.act , L.nActive ) );
+			else if( what=="nactive" ) v.push_back( (double)L.nActive );
 			else throw std::string( "unknown array: " + name );
 		}
 		else throw std::string( "unknown array: " + name );
@@ -756,10 +811,11 @@ struct Ctx : public CtxBase
 				std::vector< double > h( levels[d].N );
 				( pass==0 ? levels[d].x : levels[d].y ).download( h.data() , h.size() , stream );
 				HIPCHECK( hipStreamSynchronize( stream ) );
-				if( ah )   // the hierarchy's numbering (active functions, then padding)
+				if( adapted() )   // the hierarchy's numbering (active functions, then padding)
 				{
-					std::vector< double > a( ah->numFunctions( d ) );
-					for( size_t i=0 ; i<a.size() ; i++ ) a[i] = h[ ah->levels[d].fOrder[i] ];
+					const std::vector< long long > &fo = hostHierarchy().levels[d].fOrder;
+					std::vector< double > a( fo.size() );
+					for( size_t i=0 ; i<a.size() ; i++ ) a[i] = h[ fo[i] ];
 					h.swap( a );
 				}
 				long long sz = (long long)h.size();
@@ -769,7 +825,7 @@ struct Ctx : public CtxBase
 		}
 		try
 		{
-			if( ah ) ah->write( fp );   // HierarchicalAdaptedIndexer::write (Hat.adaptive.inl:408-414)
+			if( adapted() ) hostHierarchy().write( fp );   // HierarchicalAdaptedIndexer::write (Hat.adaptive.inl:408-414)
 			else
 			{
 				unsigned int maxDepth = D;   // HierarchicalRegularIndexer::write

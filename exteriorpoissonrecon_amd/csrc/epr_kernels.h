@@ -326,6 +326,33 @@ __global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSe
 	}
 }
 
+// One colour phase over a list of nodes (the active nodes of one colour of an
+// adapted level, dense indices in increasing order).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGsPhaseList( LevelView< Dim > L , const unsigned int *list , long long count )
+{
+	constexpr int C = DimC< Dim >::C;
+	const long long g = (long long)XcdBlock( blockIdx.x , gridDim.x )*blockDim.x + threadIdx.x;
+	const long long k = g / C;
+	const int lane = (int)( g % C );
+	if( k>=count ) return;
+	const unsigned int i = list[k];
+	int F[Dim];
+	{
+		const unsigned int n1 = (unsigned int)L.res + 1;
+		unsigned int t = i;
+#pragma unroll
+		for( int d=Dim-1 ; d>=0 ; d-- ){ const unsigned int q = t / n1 ; F[d] = (int)( t - q*n1 ) ; t = q; }
+	}
+	double q[5];
+	NodeGroup< Dim , MODE_GS , Dim==3 >( L , F , i , L.lay.col( F ) , lane , q );
+	if( lane==0 )
+	{
+		double dx , dy;
+		if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
+	}
+}
+
 // All iterations of a small level in one workgroup (barrier between colours).
 template< int Dim >
 __global__ void __launch_bounds__( 512 ) KGsSmall( LevelView< Dim > L , int iters )

@@ -104,6 +104,17 @@ def test_gpu_adapted_matches_oracle(case, oracle_lib):
         gx, gy = r.coefficients(d)
         outside = h[d][0] == 0
         assert not np.any(gx[outside]) and not np.any(gy[outside])
+    # the hierarchy built on the device serialises exactly as the host construction's
+    import tempfile
+    from exteriorpoissonrecon_amd.treefile import TreeFile, read_adapted_indexer
+    from test_adapted_indexer import build_indexer
+    with tempfile.TemporaryDirectory() as td:
+        r.write_tree(os.path.join(td, "o.tree"), 0)
+        t = TreeFile(os.path.join(td, "o.tree"))
+    host = build_indexer(pos, depth, case["aRadius"])
+    for k in ("node_depth", "node_offset", "node_has_children", "node_function", "node_element",
+              "active_sizes", "num_functions", "num_elements"):
+        assert np.array_equal(getattr(t, k), host[k]), k
     r.close()
 
 
