@@ -19,6 +19,9 @@ EXPORTS = [
     "epr_time_copy", "epr_subgrid_triangulation", "epr_level_set", "epr_mesh_info", "epr_mesh_copy",
     "epr_mesh_destroy", "epr_mg_create", "epr_mg_num_levels", "epr_mg_level_size", "epr_mg_set_colours",
     "epr_mg_vcycle", "epr_mg_relax", "epr_mg_destroy", "epr_adapted_indexer_write",
+    "epr_iso_open", "epr_iso_last_error", "epr_iso_info", "epr_iso_values", "epr_iso_level_set_nodes",
+    "epr_iso_extract", "epr_iso_level_set", "epr_iso_write_grid", "epr_iso_seconds", "epr_iso_destroy",
+    "epr_indexer_hash",
 ]
 
 
@@ -78,6 +81,20 @@ def load():
     L.epr_level_set.argtypes = [ci, ci, ci, sz, dp, sz, up, dp, dp, ctypes.POINTER(vp)]
     L.epr_level_set.restype = ctypes.c_int
     L.epr_mesh_info.argtypes = [vp, ctypes.POINTER(sz), ctypes.POINTER(ci), ctypes.POINTER(sz), ctypes.POINTER(ci)]
+    L.epr_iso_open.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.epr_iso_last_error.argtypes = [vp]
+    L.epr_iso_last_error.restype = ctypes.c_char_p
+    L.epr_iso_info.argtypes = [vp, ctypes.POINTER(ci), ctypes.POINTER(ci), ctypes.POINTER(ci), dp]
+    L.epr_iso_values.argtypes = [vp, dp]
+    L.epr_iso_level_set_nodes.argtypes = [vp, ci, sz, up, ctypes.POINTER(sz)]
+    L.epr_iso_extract.argtypes = [vp, ci, ctypes.POINTER(vp)]
+    L.epr_iso_level_set.argtypes = [vp, ci, ctypes.POINTER(vp)]
+    L.epr_iso_write_grid.argtypes = [vp, ctypes.c_char_p]
+    L.epr_iso_seconds.argtypes = [vp, ci]
+    L.epr_iso_seconds.restype = ctypes.c_double
+    L.epr_iso_destroy.argtypes = [vp]
+    L.epr_indexer_hash.argtypes = [ci, ci]
+    L.epr_indexer_hash.restype = ctypes.c_ulonglong
     L.epr_mesh_info.restype = ctypes.c_int
     L.epr_mesh_copy.argtypes = [vp, dp, up]
     L.epr_mesh_copy.restype = ctypes.c_int

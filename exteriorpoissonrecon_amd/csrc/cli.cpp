@@ -4,7 +4,6 @@
 // reads <in>.ply, writes <out>.tree (reference byte layout) and, with --grid,
 // a .grid of the finest (phi1,phi2) values.  Errors print "[ERROR] ..." to
 // stderr and exit with status 0, as MK_ERROR_OUT does (Misha/Exceptions.h:101-108).
-#include <bits/hash_bytes.h>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -18,14 +17,6 @@ static void Error( const std::string &msg )
 {
 	fprintf( stderr , "[ERROR] %s\n" , msg.c_str() );
 	exit( 0 );
-}
-
-// typeid( MishaK::Hat::Hierarchical{Regular,Adapted}Indexer< Dim > ).hash_code()
-// of a libstdc++ build of the reference: the hash of the mangled type name.
-static unsigned long long IndexerHash( int dim , bool adapted )
-{
-	std::string name = std::string( adapted ? "N6MishaK3Hat26HierarchicalAdaptedIndexerILj" : "N6MishaK3Hat26HierarchicalRegularIndexerILj" ) + std::to_string( dim ) + "EEE";
-	return (unsigned long long)std::_Hash_bytes( name.c_str() , name.size() , static_cast< size_t >( 0xc70f6907UL ) );
 }
 
 static void Usage( const char *ex )
@@ -111,7 +102,7 @@ int main( int argc , char *argv[] )
 	double E[2];
 	Check( epr_solve( ctx , E ) );
 	printf( "Error: %.17g -> %.17g = %.17g + %.17g\n" , 0. , E[0]+E[1] , E[0] , E[1] );
-	if( !out.empty() ) Check( epr_write_tree( ctx , ( out + ".tree" ).c_str() , IndexerHash( s.dim , p.aRadius!=-1 ) ) );
+	if( !out.empty() ) Check( epr_write_tree( ctx , ( out + ".tree" ).c_str() , epr_indexer_hash( s.dim , p.aRadius!=-1 ) ) );
 	if( !grid.empty() ) Check( epr_write_grid( ctx , grid.c_str() ) );
 	if( p.verbose ) printf( "%32s: %.3f (s)\n" , "Executed" , std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count() );
 	epr_destroy( ctx );

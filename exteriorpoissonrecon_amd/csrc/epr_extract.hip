@@ -1,0 +1,914 @@
+// Extraction of the reconstructed surface from a solved hierarchy on the
+// device: the reference's Extract (Extract/Extract.cpp:96-330) over its
+// IsoTree (Include/IsoTree.inl).
+//
+// Pipeline (one device-resident chain; only the orientation pass runs on the
+// host, since it is a sequential traversal):
+//   1. read the .tree (dense or adapted hierarchy) and form the finest regular
+//      values V = x_D + P( x_{D-1} + P( ... ) ), parents summed in the
+//      IsoTree's window order (IsoTree.inl:314-338)                 KRegular
+//   2. zero-crossing flags of every finest element (IsoTree.inl:135-163)
+//                                                                   KZeroCross
+//   3. adapted trees: keep the face-connected zero-crossing components that
+//      contain an indexed finest element (refineZeroCrossing, :165-222)
+//                                             KUfLink / KUfRoot / KKeep
+//   4. the kept elements in tree (Morton) order (levelSetNodes, :225-234),
+//      their triangulation and vertex values (Extract.cpp:161-199)
+//   5. the joint zero set (Extract.cpp:205), oriented (:208)  (host Orient)
+//   6. per level-set vertex: the IsoTree gradient (IsoTree.inl:101-133),
+//      scaled as Extract.cpp:240-257, and the voxel-to-world map (:259)
+//                                                                   KFrames
+// Floating-point steps follow the reference's operation order without fused
+// multiply-adds, so results are bit-identical to a build of the reference
+// without FMA contraction.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+#include <bits/hash_bytes.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/epr.h"
+#include "epr_dev.h"
+#include "epr_levelset_dev.h"
+
+#pragma clang fp contract(off)
+
+namespace epr
+{
+namespace iso
+{
+constexpr unsigned long long NONE64 = 0xffffffffffffffffull;
+
+// ------------------------------------------------------------------ .tree reading
+struct TreeData
+{
+	int dim = 0 , D = 0;
+	bool adapted = false;
+	unsigned long long hash = 0;
+	std::vector< double > xform;                          // (dim+1)^2, coords[c][r] c-major
+	std::vector< std::vector< double > > x , y;           // hierarchical coefficients per depth (file order)
+	// adapted: per depth, the multi-index of every function in the hierarchy's numbering
+	std::vector< std::vector< unsigned > > fOff;
+	std::vector< unsigned > eFinest;                      // adapted: indexed elements of depth D (dim each)
+};
+
+struct Reader
+{
+	FILE *fp;
+	std::string path;
+	void read( void *p , size_t n ){ if( n && fread( p , 1 , n , fp )!=n ) throw std::string( "truncated .tree file: " ) + path; }
+	template< typename T > T get( void ){ T v ; read( &v , sizeof(T) ) ; return v; }
+	// the indexer is the last thing in the file: the read position must be the end
+	void end( void )
+	{
+		const long here = ftell( fp );
+		if( fseek( fp , 0 , SEEK_END ) || ftell( fp )!=here ) throw std::string( here>ftell( fp ) ? "truncated .tree file: " : "trailing bytes in .tree file: " ) + path;
+	}
+	std::vector< std::vector< double > > vectors( void )
+	{
+		const unsigned long long n = get< unsigned long long >();
+		if( n>64 ) throw std::string( "bad depth count in " ) + path;
+		std::vector< std::vector< double > > out( n );
+		for( auto &v : out )
+		{
+			const unsigned long long s = get< unsigned long long >();
+			if( s>( 1ull<<36 ) ) throw std::string( "bad vector size in " ) + path;
+			v.resize( s );
+			read( v.data() , s*sizeof(double) );
+		}
+		return out;
+	}
+};
+
+// typeid( MishaK::Hat::Hierarchical{Regular,Adapted}Indexer< dim > ).hash_code() of a
+// libstdc++ build: std::_Hash_bytes of the mangled name, seed 0xc70f6907 (the
+// value ExteriorPoissonRecon.cpp:263 writes and Extract.cpp:352-361 dispatches on).
+static unsigned long long IndexerHash( int dim , bool adapted )
+{
+	const std::string name = std::string( adapted ? "N6MishaK3Hat26HierarchicalAdaptedIndexerILj" : "N6MishaK3Hat26HierarchicalRegularIndexerILj" ) + std::to_string( dim ) + "EEE";
+	return (unsigned long long)std::_Hash_bytes( name.c_str() , name.size() , static_cast< size_t >( 0xc70f6907UL ) );
+}
+
+static TreeData ReadTree( const char *path , const unsigned long long hashes[2][2] /* [dim-3][adapted] */ )
+{
+	TreeData t;
+	Reader r{ fopen( path , "rb" ) , path };
+	if( !r.fp ) throw std::string( "cannot open " ) + path;
+	struct Closer { FILE *f ; ~Closer(){ fclose( f ); } } closer{ r.fp };
+	t.dim = (int)r.get< unsigned >();
+	if( t.dim!=3 && t.dim!=4 ) throw std::string( "Only dimensions 3 and 4 supported: " ) + std::to_string( t.dim );
+	t.hash = r.get< unsigned long long >();
+	if     ( t.hash==hashes[ t.dim-3 ][0] ) t.adapted = false;
+	else if( t.hash==hashes[ t.dim-3 ][1] ) t.adapted = true;
+	else throw std::string( "Unrecognized hierarchical indexer hash: " ) + std::to_string( t.hash );
+	const int dim = t.dim;
+	(void)r.get< unsigned long long >();                  // XForm's vtable slot
+	t.xform.resize( ( dim+1 )*( dim+1 ) );
+	r.read( t.xform.data() , t.xform.size()*sizeof(double) );
+	t.x = r.vectors() , t.y = r.vectors();
+	if( t.x.empty() || t.x.size()!=t.y.size() ) throw std::string( "mismatched coefficient depths in " ) + path;
+	t.D = (int)t.x.size() - 1;
+	if( t.D*dim>30 ) throw std::string( "depth too large for the extraction's 32-bit element indices" );
+	for( int d=0 ; d<=t.D ; d++ ) if( t.x[d].size()!=t.y[d].size() ) throw std::string( "mismatched coefficient sizes in " ) + path;
+	if( !t.adapted )
+	{
+		const unsigned maxDepth = r.get< unsigned >();
+		r.end();
+		if( (int)maxDepth!=t.D ) throw std::string( "indexer depth does not match the coefficients" );
+		for( int d=0 ; d<=t.D ; d++ )
+		{
+			unsigned long long n = 1 ; for( int k=0 ; k<dim ; k++ ) n *= ( 1ull<<d ) + 1;
+			if( t.x[d].size()!=n ) throw std::string( "dense coefficient count mismatch at depth " ) + std::to_string( d );
+		}
+		return t;
+	}
+	// HierarchicalAdaptedIndexer::write (Hat.adaptive.inl:408-414): the tree
+	// depth first (RegularTree.inl:616-662), then active sizes and the tables.
+	const size_t head = ( ( 2*( dim+1 ) + 7 )/8 )*8 , rec = head + 32;
+	std::vector< unsigned char > buf( rec );
+	t.fOff.assign( t.D+1 , {} );
+	for( int d=0 ; d<=t.D ; d++ ) t.fOff[d].assign( t.x[d].size()*dim , 0xffffffffu );
+	std::vector< unsigned > fin;
+	auto Record = [&]( void )
+	{
+		r.read( buf.data() , rec );
+		unsigned short h[5];
+		memcpy( h , buf.data() , sizeof(unsigned short)*( dim+1 ) );
+		unsigned long long w[4];
+		memcpy( w , buf.data() + head , 32 );
+		const int depth = (int)h[0] - 1;
+		if( w[2]!=NONE64 )
+		{
+			if( depth<0 || depth>t.D || w[2]>=t.x[depth].size() ) throw std::string( "bad function index in the adapted tree" );
+			for( int k=0 ; k<dim ; k++ ) t.fOff[depth][ w[2]*dim+k ] = h[1+k];
+		}
+		if( w[3]!=NONE64 && depth==t.D ) for( int k=0 ; k<dim ; k++ ) fin.push_back( h[1+k] );
+		return w[1]!=0;
+	};
+	// a node with children is followed by its 2^dim children, then each child's subtree
+	struct Walk
+	{
+		int dim;
+		decltype( Record ) &rec;
+		void children( bool has , int depth )
+		{
+			if( !has ) return;
+			if( depth>32 ) throw std::string( "adapted tree too deep" );
+			bool c[16];
+			for( int i=0 ; i<( 1<<dim ) ; i++ ) c[i] = rec();
+			for( int i=0 ; i<( 1<<dim ) ; i++ ) children( c[i] , depth+1 );
+		}
+	} walk{ dim , Record };
+	walk.children( Record() , 0 );
+	t.eFinest = fin;
+	for( int d=0 ; d<=t.D ; d++ ) for( size_t i=0 ; i<t.fOff[d].size() ; i++ ) if( t.fOff[d][i]==0xffffffffu ) throw std::string( "adapted tree misses a function node" );
+	// active sizes and the node tables (their 8-byte pointers carry nothing)
+	const unsigned long long na = r.get< unsigned long long >();
+	for( unsigned long long i=0 ; i<na ; i++ ) (void)r.get< unsigned long long >();
+	for( int tab=0 ; tab<2 ; tab++ )
+	{
+		const unsigned long long nd = r.get< unsigned long long >();
+		if( (int)nd!=t.D+1 ) throw std::string( "adapted table depth mismatch" );
+		for( unsigned long long d=0 ; d<nd ; d++ )
+		{
+			const unsigned long long s = r.get< unsigned long long >();
+			if( tab==0 && s!=t.x[d].size() ) throw std::string( "adapted function count mismatch" );
+			if( s>( 1ull<<36 ) || fseek( r.fp , (long)( s*8 ) , SEEK_CUR ) ) throw std::string( "truncated .tree file: " ) + path;
+		}
+	}
+	r.end();
+	return t;
+}
+
+// ------------------------------------------------------------------ device kernels
+template< int Dim >
+__device__ __forceinline__ long long FlatF( const int F[Dim] , int res1 )
+{
+	long long f = 0;
+	for( int d=0 ; d<Dim ; d++ ) f = f*res1 + F[d];
+	return f;
+}
+
+// bit b of coordinate d sits at bit b*Dim+d: tree order, child index bit d = coordinate d
+template< int Dim >
+__device__ __forceinline__ void MortonDecode( unsigned m , int D , int E[Dim] )
+{
+	for( int d=0 ; d<Dim ; d++ ) E[d] = 0;
+	for( int b=0 ; b<D ; b++ ) for( int d=0 ; d<Dim ; d++ ) E[d] |= (int)( ( m>>( b*Dim+d ) ) & 1u ) << b;
+}
+template< int Dim >
+__device__ __forceinline__ unsigned MortonEncode( const int E[Dim] , int D )
+{
+	unsigned m = 0;
+	for( int b=0 ; b<D ; b++ ) for( int d=0 ; d<Dim ; d++ ) m |= (unsigned)( ( E[d]>>b ) & 1 ) << ( b*Dim+d );
+	return m;
+}
+
+// V_f = x_f + sum over the parents of f, in window order (coordinate 0 most
+// significant, lower parent first), of V_parent * 0.5^(odd coordinates)
+// (IsoTree.inl:314-338); pairs interleaved (component 0, component 1).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KRegular( const double *Vc , const double *hx , const double *hy , double *Vf , int cres )
+{
+	const int fres = 2*cres;
+	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= fres+1;
+	const long long f = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( f>=nf ) return;
+	int F[Dim];
+	{
+		long long r = f;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = (int)( r % ( fres+1 ) ) ; r /= fres+1; }
+	}
+	int nodd = 0;
+	for( int d=0 ; d<Dim ; d++ ) nodd += F[d]&1;
+	double w = 1;
+	for( int k=0 ; k<nodd ; k++ ) w *= 0.5;
+	double s0 = 0 , s1 = 0;
+	for( int m=0 ; m<( 1<<nodd ) ; m++ )
+	{
+		long long c = 0;
+		int oi = 0;
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			int cd = F[d]>>1;
+			if( F[d]&1 ){ cd += ( m>>( nodd-1-oi ) ) & 1 ; oi++; }
+			c = c*( cres+1 ) + cd;
+		}
+		s0 = s0 + Vc[2*c]*w;
+		s1 = s1 + Vc[2*c+1]*w;
+	}
+	Vf[2*f] = hx[f] + s0;
+	Vf[2*f+1] = hy[f] + s1;
+}
+
+__global__ void __launch_bounds__( 256 ) KInterleave( const double *a , const double *b , double *out , long long n )
+{
+	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( i<n ) out[2*i] = a[i] , out[2*i+1] = b[i];
+}
+
+// IsoTree::isZeroCrossingNode (IsoTree.inl:135-163) of every finest element, by Morton index
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KZeroCross( const double *V , int D , unsigned nE , int all , unsigned char *zc )
+{
+	const unsigned m = blockIdx.x*blockDim.x + threadIdx.x;
+	if( m>=nE ) return;
+	int E[Dim];
+	MortonDecode< Dim >( m , D , E );
+	const int res1 = ( 1<<D ) + 1;
+	unsigned f0 = 0 , f1 = 0;
+	for( int i=0 ; i<( 1<<Dim ) ; i++ )
+	{
+		int F[Dim];
+		for( int d=0 ; d<Dim ; d++ ) F[d] = E[d] + ( ( i>>d ) & 1 );
+		const long long f = FlatF< Dim >( F , res1 );
+		const double v0 = V[2*f] , v1 = V[2*f+1];
+		if( v0<0 ) f0 |= 1 ; else if( v0>0 ) f0 |= 2 ; else f0 |= 3;
+		if( v1<0 ) f1 |= 1 ; else if( v1>0 ) f1 |= 2 ; else f1 |= 3;
+	}
+	zc[m] = all ? ( f0==3 && f1==3 ) : ( f0==3 || f1==3 );
+}
+
+// union-find over the zero-crossing elements, face adjacency (IsoTree.inl:205-212:
+// offsets differing in exactly one coordinate, by one, inside the grid).
+// Roots only change through the compare-and-swap; path halving rewrites
+// non-roots with one of their ancestors.
+__device__ __forceinline__ unsigned UfLoad( const unsigned *p , unsigned i ){ return __hip_atomic_load( p + i , __ATOMIC_RELAXED , __HIP_MEMORY_SCOPE_AGENT ); }
+__device__ __forceinline__ unsigned UfFind( unsigned *parent , unsigned x )
+{
+	unsigned p = UfLoad( parent , x );
+	while( p!=x )
+	{
+		const unsigned g = UfLoad( parent , p );
+		if( g!=p ) __hip_atomic_store( parent + x , g , __ATOMIC_RELAXED , __HIP_MEMORY_SCOPE_AGENT );
+		x = p , p = g;
+	}
+	return x;
+}
+
+__global__ void __launch_bounds__( 256 ) KUfInit( unsigned *parent , unsigned nE )
+{
+	const unsigned m = blockIdx.x*blockDim.x + threadIdx.x;
+	if( m<nE ) parent[m] = m;
+}
+
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KUfLink( const unsigned char *zc , int D , unsigned nE , unsigned *parent )
+{
+	const unsigned m = blockIdx.x*blockDim.x + threadIdx.x;
+	if( m>=nE || !zc[m] ) return;
+	int E[Dim];
+	MortonDecode< Dim >( m , D , E );
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		if( E[d]+1>=( 1<<D ) ) continue;
+		E[d]++;
+		const unsigned n = MortonEncode< Dim >( E , D );
+		E[d]--;
+		if( !zc[n] ) continue;
+		unsigned a = m , b = n;
+		while( true )
+		{
+			a = UfFind( parent , a ) , b = UfFind( parent , b );
+			if( a==b ) break;
+			if( a>b ){ const unsigned t = a ; a = b ; b = t; }
+			// link the larger root under the smaller one
+			unsigned expected = b;
+			if( __hip_atomic_compare_exchange_strong( parent + b , &expected , a , __ATOMIC_RELAXED , __ATOMIC_RELAXED , __HIP_MEMORY_SCOPE_AGENT ) ) break;
+			b = expected;
+		}
+	}
+}
+
+// roots of components that hold an indexed (seed) element
+__global__ void __launch_bounds__( 256 ) KUfSeed( const unsigned char *zc , const unsigned char *indexed , unsigned nE , unsigned *parent , unsigned char *seedRoot )
+{
+	const unsigned m = blockIdx.x*blockDim.x + threadIdx.x;
+	if( m>=nE || !zc[m] || !indexed[m] ) return;
+	seedRoot[ UfFind( parent , m ) ] = 1;
+}
+
+__global__ void __launch_bounds__( 256 ) KUfKeep( const unsigned char *zc , unsigned nE , unsigned *parent , const unsigned char *seedRoot , unsigned char *keep )
+{
+	const unsigned m = blockIdx.x*blockDim.x + threadIdx.x;
+	if( m>=nE ) return;
+	keep[m] = zc[m] && seedRoot[ UfFind( parent , m ) ];
+}
+
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KCells( const unsigned *morton , unsigned n , int D , unsigned *cells )
+{
+	const unsigned i = blockIdx.x*blockDim.x + threadIdx.x;
+	if( i>=n ) return;
+	int E[Dim];
+	MortonDecode< Dim >( morton[i] , D , E );
+	for( int d=0 ; d<Dim ; d++ ) cells[ (size_t)i*Dim+d ] = (unsigned)E[d];
+}
+
+// IsoTree::functionValue at the rounded vertex positions (Extract.cpp:191-197)
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KVertexValues( const double *X , size_t nV , const double *V , int D , double *vals )
+{
+	const size_t v = (size_t)blockIdx.x*blockDim.x + threadIdx.x;
+	if( v>=nV ) return;
+	int F[Dim];
+	for( int d=0 ; d<Dim ; d++ ) F[d] = (int)floor( X[ v*Dim+d ] + 0.5 );
+	const long long f = FlatF< Dim >( F , ( 1<<D ) + 1 );
+	vals[2*v] = V[2*f] , vals[2*v+1] = V[2*f+1];
+}
+
+__global__ void __launch_bounds__( 256 ) KColumn( const double *pairs , size_t n , int c , double *out )
+{
+	const size_t i = (size_t)blockIdx.x*blockDim.x + threadIdx.x;
+	if( i<n ) out[i] = pairs[2*i+c];
+}
+
+struct XF { double m[5][5]; };
+
+// voxelToWorld( p ) (Misha/Geometry.inl:413-427): out_i = sum_j p_j m[j][i] + m[Dim][i], over 1 + sum_i m[i][Dim]
+template< int Dim >
+__device__ __forceinline__ void Transform( const XF &xf , const double p[Dim] , double out[Dim] )
+{
+	double scale = 1;
+	for( int i=0 ; i<Dim ; i++ )
+	{
+		out[i] = 0;
+		for( int j=0 ; j<Dim ; j++ ) out[i] = out[i] + p[j]*xf.m[j][i];
+		out[i] = out[i] + xf.m[Dim][i];
+		scale = scale + xf.m[i][Dim];
+	}
+	for( int i=0 ; i<Dim ; i++ ) out[i] = out[i] / scale;
+}
+
+// Per level-set vertex (grid units): the IsoTree gradient of both functions
+// (IsoTree.inl:101-133) on the leaf that getLeafNode reaches (a coordinate on
+// a cell face descends to the lower cell: RegularTree.inl:248-262 with
+// ChildIndex's strict '>'), with local coordinates from the clamped floor; the
+// frame is grad * (1/res) * (1/scale) (Extract.cpp:253); the position is
+// mapped to world space.  valid(leaf) is the IsoTree's valid-element test:
+// an indexed finest element or a face neighbour of a kept one.
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KFrames( const double *X , size_t nV , const double *V , int D , double invScale , XF xf ,
+                                                   const unsigned char *keep , const unsigned char *indexed , double *out , unsigned *invalid )
+{
+	const size_t v = (size_t)blockIdx.x*blockDim.x + threadIdx.x;
+	if( v>=nV ) return;
+	const int res = 1<<D;
+	const double dres = (double)res , invRes = 1.0/dres;
+	double g[Dim] , pl[Dim];
+	int leaf[Dim] , E[Dim];
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		g[d] = X[ v*Dim+d ];
+		const double p = g[d]*invRes;                    // vertex / res
+		const double pr = p*dres;
+		const double c = ceil( pr );
+		leaf[d] = pr>0 ? (int)c - 1 : 0;
+		leaf[d] = max( 0 , min( res-1 , leaf[d] ) );
+		E[d] = max( 0 , min( (int)floor( pr ) , res-1 ) );
+		pl[d] = pr - (double)E[d];
+	}
+	// the reference stops with "Leaf node is not a valid element node" otherwise
+	{
+		const unsigned m = MortonEncode< Dim >( leaf , D );
+		bool ok = keep[m] || ( indexed ? indexed[m]!=0 : true );
+		for( int d=0 ; d<Dim && !ok ; d++ ) for( int s=-1 ; s<=1 && !ok ; s+=2 )
+		{
+			const int e = leaf[d] + s;
+			if( e<0 || e>=res ) continue;
+			leaf[d] = e;
+			ok = keep[ MortonEncode< Dim >( leaf , D ) ]!=0;
+			leaf[d] -= s;
+		}
+		if( !ok ) atomicAdd( invalid , 1u );
+	}
+	double grad[2][Dim];
+	for( int c=0 ; c<2 ; c++ ) for( int d=0 ; d<Dim ; d++ ) grad[c][d] = 0;
+	const int res1 = res + 1;
+	for( int i=0 ; i<( 1<<Dim ) ; i++ )                 // Range order: coordinate 0 slowest
+	{
+		int I[Dim] , F[Dim];
+		for( int d=0 ; d<Dim ; d++ ) I[d] = ( i>>( Dim-1-d ) ) & 1 , F[d] = leaf[d] + I[d];
+		const long long f = FlatF< Dim >( F , res1 );
+		const double val[2] = { V[2*f] , V[2*f+1] };
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			double weight = dres;
+			for( int dd=0 ; dd<Dim ; dd++ )
+			{
+				if( dd==d ) weight = weight * ( I[dd]==1 ? 1. : -1. );
+				else        weight = weight * ( I[dd]==1 ? pl[dd] : 1.-pl[dd] );
+			}
+			for( int c=0 ; c<2 ; c++ ) grad[c][d] = grad[c][d] + val[c]*weight;
+		}
+	}
+	double *o = out + v*3*Dim;
+	double w[Dim];
+	Transform< Dim >( xf , g , w );
+	for( int d=0 ; d<Dim ; d++ ) o[d] = w[d];
+	for( int c=0 ; c<2 ; c++ ) for( int d=0 ; d<Dim ; d++ ) o[ Dim*( 1+c )+d ] = ( grad[c][d]*invRes )*invScale;
+}
+
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KToWorld( const double *X , size_t nV , XF xf , double *out )
+{
+	const size_t v = (size_t)blockIdx.x*blockDim.x + threadIdx.x;
+	if( v>=nV ) return;
+	double p[Dim] , w[Dim];
+	for( int d=0 ; d<Dim ; d++ ) p[d] = X[ v*Dim+d ];
+	Transform< Dim >( xf , p , w );
+	for( int d=0 ; d<Dim ; d++ ) out[ v*Dim+d ] = w[d];
+}
+
+static inline unsigned Grid( long long n ){ return (unsigned)std::max< long long >( 1 , ( n + 255 )/256 ); }
+
+// ------------------------------------------------------------------ orientation (host)
+// MarchingSimplices::Orient (Misha/MarchingSimplices.h:179-246) for (K+1)-vertex
+// simplices.  Face keys are the sorted vertex sets; the two simplices incident
+// on a face are the first two in (simplex, face) order, a third is an error.
+// Seeds are taken in index order; a depth-first stack visits the faces of
+// each simplex in the order of its current (possibly flipped) vertices; a
+// flip swaps the first two vertices.
+template< int K >
+static void OrientFaceKey( const unsigned *si , int v , unsigned long long &key , bool &even )
+{
+	if( K==1 )
+	{
+		key = si[ v==0 ? 1 : 0 ] , even = v==0;
+		return;
+	}
+	// K==2: the face opposite vertex v, {si[v+1], si[v+2]} cyclically (no swap for even K)
+	const unsigned a = si[ ( v+1 )%3 ] , b = si[ ( v+2 )%3 ];
+	even = a<b;
+	key = a<b ? ( (unsigned long long)a<<32 | b ) : ( (unsigned long long)b<<32 | a );
+}
+
+struct FaceLinks
+{
+	std::vector< long long > other;      // per (simplex, face): the other incident simplex, or -1
+	std::vector< unsigned char > same;   // per (simplex, face): both incidences carry the same orientation
+};
+
+template< int K >
+static FaceLinks LinkFaces( const std::vector< unsigned > &S )
+{
+	const size_t nS = S.size()/( K+1 ) , nR = nS*( K+1 );
+	std::vector< std::pair< unsigned long long , unsigned > > rec( nR );
+	std::vector< unsigned char > even( nR );
+	for( size_t s=0 ; s<nS ; s++ ) for( int v=0 ; v<=K ; v++ )
+	{
+		unsigned long long key ; bool e;
+		OrientFaceKey< K >( &S[ s*( K+1 ) ] , v , key , e );
+		rec[ s*( K+1 )+v ] = { key , (unsigned)( s*( K+1 )+v ) } , even[ s*( K+1 )+v ] = e;
+	}
+	std::sort( rec.begin() , rec.end() );                // by key, then by (simplex, face)
+	FaceLinks L;
+	L.other.assign( nR , -1 ) , L.same.assign( nR , 0 );
+	for( size_t i=0 ; i<nR ; )
+	{
+		size_t j = i+1;
+		while( j<nR && rec[j].first==rec[i].first ) j++;
+		if( j-i>2 ) throw std::string( "Both faces occupied" );
+		if( j-i==2 )
+		{
+			const unsigned a = rec[i].second , b = rec[i+1].second;
+			L.other[a] = b/( K+1 ) , L.other[b] = a/( K+1 );
+			L.same[a] = L.same[b] = even[a]==even[b];
+		}
+		i = j;
+	}
+	return L;
+}
+
+template< int K >
+static void Orient( std::vector< unsigned > &S )
+{
+	const size_t nS = S.size()/( K+1 );
+	if( !nS ) return;
+	FaceLinks L = LinkFaces< K >( S );
+	std::vector< unsigned char > oriented( nS , 0 ) , flipped( nS , 0 );
+	std::vector< std::pair< size_t , bool > > stack;
+	for( size_t idx=0 ; idx<nS ; idx++ )
+	{
+		if( oriented[idx] ) continue;
+		oriented[idx] = 1;
+		stack.push_back( { idx , true } );
+		while( !stack.empty() )
+		{
+			const std::pair< size_t , bool > s = stack.back();
+			stack.pop_back();
+			for( int v=0 ; v<=K ; v++ )
+			{
+				// after a flip the faces opposite vertices 0 and 1 trade places
+				const int vo = flipped[ s.first ] && v<2 ? 1-v : v;
+				const size_t r = s.first*( K+1 )+vo;
+				const long long o = L.other[r];
+				if( o<0 || oriented[o] ) continue;
+				oriented[o] = 1;
+				if( s.second!=( L.same[r]!=0 ) ) stack.push_back( { (size_t)o , true } );
+				else
+				{
+					std::swap( S[ o*( K+1 ) ] , S[ o*( K+1 )+1 ] );
+					flipped[o] = 1;
+					stack.push_back( { (size_t)o , false } );
+				}
+			}
+		}
+	}
+}
+} // namespace iso
+} // namespace epr
+
+struct epr_iso
+{
+	int dim = 0 , depth = 0;
+	bool adapted = false;
+	unsigned long long hash = 0;
+	std::vector< double > xform;
+	hipStream_t st = nullptr;
+	epr::DevBuf< double > V;                            // finest regular values, pairs
+	epr::DevBuf< unsigned char > indexed;               // adapted: indexed finest elements by Morton index
+	std::string err;
+	double seconds[4] = { 0 , 0 , 0 , 0 };               // open, level-set nodes + triangulation, level set + orient, frames
+};
+
+namespace epr
+{
+namespace iso
+{
+using Clock = std::chrono::steady_clock;
+static double Since( Clock::time_point t ){ return std::chrono::duration< double >( Clock::now() - t ).count(); }
+
+template< int Dim >
+static void Upload( epr_iso &I , const TreeData &t )
+{
+	const int D = t.D;
+	hipStream_t st = I.st;
+	DevBuf< double > Vc , Vf , hx , hy;
+	std::vector< double > dx , dy;
+	for( int d=0 ; d<=D ; d++ )
+	{
+		const long long n1 = ( 1ll<<d ) + 1;
+		long long N = 1 ; for( int k=0 ; k<Dim ; k++ ) N *= n1;
+		const double *px = t.x[d].data() , *py = t.y[d].data();
+		if( t.adapted )
+		{
+			dx.assign( N , 0. ) , dy.assign( N , 0. );
+			const std::vector< unsigned > &F = t.fOff[d];
+			for( size_t i=0 ; i<t.x[d].size() ; i++ )
+			{
+				long long f = 0;
+				for( int k=0 ; k<Dim ; k++ )
+				{
+					if( F[ i*Dim+k ]>=(unsigned)n1 ) throw std::string( "adapted function offset outside the grid" );
+					f = f*n1 + F[ i*Dim+k ];
+				}
+				dx[f] = t.x[d][i] , dy[f] = t.y[d][i];
+			}
+			px = dx.data() , py = dy.data();
+		}
+		hx.upload( px , N , st ) , hy.upload( py , N , st );
+		Vf.alloc( 2*N );
+		if( d==0 ) KInterleave<<< Grid( N ) , 256 , 0 , st >>>( hx.p , hy.p , Vf.p , N );
+		else       KRegular< Dim ><<< Grid( N ) , 256 , 0 , st >>>( Vc.p , hx.p , hy.p , Vf.p , 1<<( d-1 ) );
+		HIPCHECK( hipGetLastError() );
+		HIPCHECK( hipStreamSynchronize( st ) );     // the host staging buffers are reused
+		std::swap( Vc , Vf );
+	}
+	I.V = std::move( Vc );
+	if( t.adapted )
+	{
+		const unsigned nE = 1u<<( D*Dim );
+		std::vector< unsigned char > ind( nE , 0 );
+		for( size_t i=0 ; i<t.eFinest.size()/Dim ; i++ )
+		{
+			unsigned m = 0;
+			for( int b=0 ; b<D ; b++ ) for( int k=0 ; k<Dim ; k++ ) m |= ( ( t.eFinest[ i*Dim+k ]>>b ) & 1u ) << ( b*Dim+k );
+			if( m>=nE ) throw std::string( "adapted element offset outside the grid" );
+			ind[m] = 1;
+		}
+		I.indexed.upload( ind.data() , nE , st );
+		HIPCHECK( hipStreamSynchronize( st ) );
+	}
+}
+
+// the kept finest elements (levelSetNodes): keep flags by Morton index and the cell list in Morton order
+template< int Dim >
+static void LevelSetCells( epr_iso &I , bool all , DevBuf< unsigned char > &keep , DevBuf< unsigned > &cells , size_t &nCells )
+{
+	const int D = I.depth;
+	const unsigned nE = 1u<<( D*Dim );
+	hipStream_t st = I.st;
+	DevBuf< unsigned char > zc , tmp;
+	zc.alloc( nE );
+	KZeroCross< Dim ><<< Grid( nE ) , 256 , 0 , st >>>( I.V.p , D , nE , all ? 1 : 0 , zc.p );
+	if( I.adapted )
+	{
+		DevBuf< unsigned > parent;
+		DevBuf< unsigned char > seedRoot;
+		parent.alloc( nE ) , seedRoot.alloc( nE ) , keep.alloc( nE );
+		seedRoot.zero( st );
+		KUfInit<<< Grid( nE ) , 256 , 0 , st >>>( parent.p , nE );
+		KUfLink< Dim ><<< Grid( nE ) , 256 , 0 , st >>>( zc.p , D , nE , parent.p );
+		KUfSeed<<< Grid( nE ) , 256 , 0 , st >>>( zc.p , I.indexed.p , nE , parent.p , seedRoot.p );
+		KUfKeep<<< Grid( nE ) , 256 , 0 , st >>>( zc.p , nE , parent.p , seedRoot.p , keep.p );
+		HIPCHECK( hipGetLastError() );
+		HIPCHECK( hipStreamSynchronize( st ) );
+	}
+	else keep = std::move( zc );
+	DevBuf< unsigned > morton , count;
+	morton.alloc( nE ) , count.alloc( 1 );
+	size_t bytes = 0;
+	hipcub::CountingInputIterator< unsigned > it( 0u );
+	HIPCHECK( hipcub::DeviceSelect::Flagged( nullptr , bytes , it , keep.p , morton.p , count.p , (int)nE , st ) );
+	tmp.ensure( bytes );
+	HIPCHECK( hipcub::DeviceSelect::Flagged( tmp.p , bytes , it , keep.p , morton.p , count.p , (int)nE , st ) );
+	unsigned n = 0;
+	count.download( &n , 1 , st );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	nCells = n;
+	cells.alloc( (size_t)n*Dim );
+	if( n ) KCells< Dim ><<< Grid( n ) , 256 , 0 , st >>>( morton.p , n , D , cells.p );
+	HIPCHECK( hipGetLastError() );
+	HIPCHECK( hipStreamSynchronize( st ) );
+}
+
+template< int Dim >
+static void SubGrid( epr_iso &I , bool all , DevBuf< unsigned char > &keep , ls::DevMesh &tri , DevBuf< double > &vals )
+{
+	DevBuf< unsigned > cells;
+	size_t nCells = 0;
+	LevelSetCells< Dim >( I , all , keep , cells , nCells );
+	const unsigned ext[4] = { ( 1u<<I.depth ) - 1 , ( 1u<<I.depth ) - 1 , ( 1u<<I.depth ) - 1 , ( 1u<<I.depth ) - 1 };
+	ls::TriangulateDev( Dim , cells.p , nCells , ext , true , I.st , tri );
+	vals.alloc( 2*tri.nV + 2 );
+	if( tri.nV ) KVertexValues< Dim ><<< Grid( tri.nV ) , 256 , 0 , I.st >>>( tri.V.p , tri.nV , I.V.p , I.depth , vals.p );
+	HIPCHECK( hipGetLastError() );
+}
+
+static XF MakeXF( const epr_iso &I )
+{
+	XF xf{};
+	for( int c=0 ; c<=I.dim ; c++ ) for( int r=0 ; r<=I.dim ; r++ ) xf.m[c][r] = I.xform[ c*( I.dim+1 )+r ];
+	return xf;
+}
+
+// Extract.cpp:149-151
+static double FrameScale( const epr_iso &I )
+{
+	double scale = 1.;
+	for( int d=0 ; d<I.dim ; d++ ) scale *= I.xform[ d*( I.dim+1 )+d ];
+	return pow( scale , 1./I.dim );
+}
+
+template< int Dim >
+static epr_mesh *Extract( epr_iso &I , bool all )
+{
+	hipStream_t st = I.st;
+	Clock::time_point t0 = Clock::now();
+	DevBuf< unsigned char > keep;
+	ls::DevMesh tri , lvl;
+	DevBuf< double > vals;
+	SubGrid< Dim >( I , all , keep , tri , vals );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	I.seconds[1] = Since( t0 ) , t0 = Clock::now();
+	const double zero[2] = { 0 , 0 };
+	ls::LevelSet2Dev< Dim >( Dim , tri.nV , tri.V.p , tri.nS , tri.S.p , vals.p , zero , st , lvl );
+	epr_mesh *m = new epr_mesh;
+	m->vdim = 3*Dim , m->k = Dim-1;
+	m->s.resize( lvl.nS*lvl.k );
+	lvl.S.download( m->s.data() , m->s.size() , st );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	try
+	{
+		if( Dim==3 ) Orient< 1 >( m->s );
+		else         Orient< 2 >( m->s );
+	}
+	catch( ... ){ delete m ; throw; }
+	I.seconds[2] = Since( t0 ) , t0 = Clock::now();
+	DevBuf< double > out;
+	DevBuf< unsigned > invalid;
+	out.alloc( lvl.nV*3*Dim + 1 ) , invalid.alloc( 1 );
+	invalid.zero( st );
+	const double scale = FrameScale( I );
+	if( lvl.nV ) KFrames< Dim ><<< Grid( lvl.nV ) , 256 , 0 , st >>>( lvl.V.p , lvl.nV , I.V.p , I.depth , 1./scale , MakeXF( I ) , keep.p ,
+	                                                                  I.adapted ? I.indexed.p : nullptr , out.p , invalid.p );
+	HIPCHECK( hipGetLastError() );
+	unsigned bad = 0;
+	invalid.download( &bad , 1 , st );
+	m->v.resize( lvl.nV*3*Dim );
+	out.download( m->v.data() , m->v.size() , st );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	I.seconds[3] = Since( t0 );
+	if( bad ){ delete m ; throw std::string( "Leaf node is not a valid element node" ); }
+	return m;
+}
+
+// Extract.cpp:296-326: the codimension-1 zero set of component c over the
+// same triangulation (all = false), vertices in world space
+template< int Dim >
+static epr_mesh *LevelSet1( epr_iso &I , int c )
+{
+	hipStream_t st = I.st;
+	DevBuf< unsigned char > keep;
+	ls::DevMesh tri , lvl;
+	DevBuf< double > vals , col , world;
+	SubGrid< Dim >( I , false , keep , tri , vals );
+	col.alloc( tri.nV + 1 );
+	if( tri.nV ) KColumn<<< Grid( tri.nV ) , 256 , 0 , st >>>( vals.p , tri.nV , c , col.p );
+	ls::LevelSet1Dev< Dim >( Dim , tri.nV , tri.V.p , tri.nS , tri.S.p , col.p , 0. , st , lvl );
+	world.alloc( lvl.nV*Dim + 1 );
+	if( lvl.nV ) KToWorld< Dim ><<< Grid( lvl.nV ) , 256 , 0 , st >>>( lvl.V.p , lvl.nV , MakeXF( I ) , world.p );
+	HIPCHECK( hipGetLastError() );
+	epr_mesh *m = new epr_mesh;
+	m->vdim = Dim , m->k = Dim;
+	m->v.resize( lvl.nV*Dim ) , m->s.resize( lvl.nS*Dim );
+	world.download( m->v.data() , m->v.size() , st );
+	lvl.S.download( m->s.data() , m->s.size() , st );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	return m;
+}
+
+template< int Dim >
+static void Cells( epr_iso &I , bool all , std::vector< unsigned > &out )
+{
+	DevBuf< unsigned char > keep;
+	DevBuf< unsigned > cells;
+	size_t n = 0;
+	LevelSetCells< Dim >( I , all , keep , cells , n );
+	out.resize( n*Dim );
+	cells.download( out.data() , out.size() , I.st );
+	HIPCHECK( hipStreamSynchronize( I.st ) );
+}
+
+// Extract.cpp:132-144: per component, RegularGrid<Dim,double>::write
+// (Misha/RegularGrid.inl:441-471) of the finest regular coefficients
+static void WriteGrid( epr_iso &I , const char *header )
+{
+	const int dim = I.dim;
+	const long long n1 = ( 1ll<<I.depth ) + 1;
+	long long N = 1 ; for( int k=0 ; k<dim ; k++ ) N *= n1;
+	std::vector< double > V( 2*N );
+	I.V.download( V.data() , V.size() , I.st );
+	HIPCHECK( hipStreamSynchronize( I.st ) );
+	std::vector< double > comp( N );
+	for( int c=0 ; c<2 ; c++ )
+	{
+		for( long long i=0 ; i<N ; i++ ) comp[i] = V[ 2*i+c ];
+		const std::string name = std::string( header ) + "." + std::to_string( c ) + ".grid";
+		FILE *fp = fopen( name.c_str() , "wb" );
+		if( !fp ) throw std::string( "Failed to open grid file for writing: " ) + name;
+		fprintf( fp , "G%d\n" , dim );
+		fprintf( fp , "1 DOUBLE\n" );
+		for( int d=0 ; d<dim ; d++ ) fprintf( fp , d==dim-1 ? "%d\n" : "%d " , (int)n1 );
+		for( int j=0 ; j<=dim ; j++ ) for( int i=0 ; i<=dim ; i++ )
+			fprintf( fp , i==dim ? "%f\n" : "%f " , (float)I.xform[ i*( dim+1 )+j ] );
+		fwrite( comp.data() , sizeof(double) , N , fp );
+		fclose( fp );
+	}
+}
+} // namespace iso
+} // namespace epr
+
+template< typename F >
+static int IsoGuard( epr_iso *I , F f )
+{
+	if( !I ) return EPR_ERR_ARG;
+	try { f() ; I->err.clear() ; return EPR_OK; }
+	catch( const epr::HipError &e ){ I->err = e.msg ; return EPR_ERR_HIP; }
+	catch( const std::string &s ){ I->err = s ; return EPR_ERR_STATE; }
+	catch( const std::exception &e ){ I->err = e.what() ; return EPR_ERR_STATE; }
+}
+
+extern "C" {
+
+int epr_iso_open( const char *path , epr_iso **out )
+{
+	if( !path || !out ){ epr::ThreadError() = "bad argument" ; return EPR_ERR_ARG; }
+	*out = nullptr;
+	return epr::ContextFree( [&]()
+	{
+		using namespace epr::iso;
+		Clock::time_point t0 = Clock::now();
+		const unsigned long long hashes[2][2] = { { IndexerHash( 3 , false ) , IndexerHash( 3 , true ) } , { IndexerHash( 4 , false ) , IndexerHash( 4 , true ) } };
+		TreeData t = ReadTree( path , hashes );
+		epr_iso *I = new epr_iso;
+		try
+		{
+			I->dim = t.dim , I->depth = t.D , I->adapted = t.adapted , I->hash = t.hash , I->xform = t.xform;
+			if( t.dim==3 ) Upload< 3 >( *I , t ) ; else Upload< 4 >( *I , t );
+		}
+		catch( ... ){ delete I ; throw; }
+		I->seconds[0] = Since( t0 );
+		*out = I;
+	} );
+}
+
+const char *epr_iso_last_error( const epr_iso *iso ){ return iso ? iso->err.c_str() : epr::ThreadError().c_str(); }
+
+int epr_iso_info( const epr_iso *iso , int *dim , int *depth , int *adapted , double xform[25] )
+{
+	if( !iso ) return EPR_ERR_ARG;
+	if( dim ) *dim = iso->dim;
+	if( depth ) *depth = iso->depth;
+	if( adapted ) *adapted = iso->adapted ? 1 : 0;
+	if( xform ) for( size_t i=0 ; i<iso->xform.size() ; i++ ) xform[i] = iso->xform[i];
+	return EPR_OK;
+}
+
+int epr_iso_values( epr_iso *iso , double *out )
+{
+	return IsoGuard( iso , [&]()
+	{
+		long long N = 1 ; for( int k=0 ; k<iso->dim ; k++ ) N *= ( 1ll<<iso->depth ) + 1;
+		if( !out ) throw std::string( "null output" );
+		iso->V.download( out , 2*N , iso->st );
+		HIPCHECK( hipStreamSynchronize( iso->st ) );
+	} );
+}
+
+int epr_iso_level_set_nodes( epr_iso *iso , int all , size_t cap , unsigned *cells , size_t *count )
+{
+	return IsoGuard( iso , [&]()
+	{
+		std::vector< unsigned > c;
+		if( iso->dim==3 ) epr::iso::Cells< 3 >( *iso , all!=0 , c ) ; else epr::iso::Cells< 4 >( *iso , all!=0 , c );
+		const size_t n = c.size()/iso->dim;
+		if( count ) *count = n;
+		if( cells && cap ) memcpy( cells , c.data() , std::min( cap , n )*iso->dim*sizeof(unsigned) );
+	} );
+}
+
+int epr_iso_extract( epr_iso *iso , int all , epr_mesh **out )
+{
+	if( !out ) return EPR_ERR_ARG;
+	*out = nullptr;
+	return IsoGuard( iso , [&](){ *out = iso->dim==3 ? epr::iso::Extract< 3 >( *iso , all!=0 ) : epr::iso::Extract< 4 >( *iso , all!=0 ); } );
+}
+
+int epr_iso_level_set( epr_iso *iso , int component , epr_mesh **out )
+{
+	if( !out || ( component!=0 && component!=1 ) ) return EPR_ERR_ARG;
+	*out = nullptr;
+	return IsoGuard( iso , [&](){ *out = iso->dim==3 ? epr::iso::LevelSet1< 3 >( *iso , component ) : epr::iso::LevelSet1< 4 >( *iso , component ); } );
+}
+
+int epr_iso_write_grid( epr_iso *iso , const char *header )
+{
+	if( !header ) return EPR_ERR_ARG;
+	return IsoGuard( iso , [&](){ epr::iso::WriteGrid( *iso , header ); } );
+}
+
+double epr_iso_seconds( const epr_iso *iso , int which ){ return iso && which>=0 && which<4 ? iso->seconds[which] : 0.; }
+
+unsigned long long epr_indexer_hash( int dim , int adapted ){ return epr::iso::IndexerHash( dim , adapted!=0 ); }
+
+void epr_iso_destroy( epr_iso *iso ){ delete iso; }
+
+}

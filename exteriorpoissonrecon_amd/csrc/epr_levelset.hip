@@ -37,13 +37,7 @@
 
 #include "../../include/epr.h"
 #include "epr_dev.h"
-
-struct epr_mesh
-{
-	int vdim = 0 , k = 0;            // doubles per vertex, vertices per simplex
-	std::vector< double > v;
-	std::vector< unsigned > s;
-};
+#include "epr_levelset_dev.h"
 
 namespace epr
 {
@@ -731,67 +725,73 @@ static void SortTriples( const unsigned *k0 , const unsigned *k1 , const unsigne
 	std::swap( order , t0 );
 }
 
-static epr_mesh *Triangulate( int dim , size_t nCells , const unsigned *cells , bool orient )
+void TriangulateDev( int dim , const unsigned *dCells , size_t nCells , const unsigned ext[4] , bool orient , hipStream_t st , DevMesh &out )
 {
 	CubeTab tab = MakeCubeTab( dim , orient );
 	const long long nOcc = (long long)nCells * tab.n * ( dim+1 );
 	if( nOcc>=(long long)NONE ) throw std::string( "too many cells for 32-bit occurrence indices" );
 	KeyGrid kg{};
 	unsigned long long keys = 1;
-	{
-		unsigned ext[4] = { 0 , 0 , 0 , 0 };
-		for( size_t e=0 ; e<nCells ; e++ ) for( int d=0 ; d<dim ; d++ ) ext[d] = std::max( ext[d] , cells[ e*dim+d ] );
-		for( int d=dim-1 ; d>=0 ; d-- ){ kg.stride[d] = (long long)keys ; keys *= (unsigned long long)ext[d] + 2; }
-		if( keys>( 1ull<<32 ) ) throw std::string( "cell coordinates span too large a box" );
-	}
-	epr_mesh *m = new epr_mesh;
-	m->vdim = dim , m->k = dim+1;
-	if( !nCells ) return m;
-	hipStream_t st = nullptr;
-	DevBuf< unsigned > dCells , first , flag , vid , S;
+	for( int d=dim-1 ; d>=0 ; d-- ){ kg.stride[d] = (long long)keys ; keys *= (unsigned long long)ext[d] + 2; }
+	if( keys>( 1ull<<32 ) ) throw std::string( "cell coordinates span too large a box" );
+	out.vdim = dim , out.k = dim+1 , out.nV = out.nS = 0;
+	if( !nCells ) return;
+	DevBuf< unsigned > first , flag , vid;
 	DevBuf< unsigned char > tmp;
-	DevBuf< double > V;
-	dCells.upload( cells , nCells*dim , st );
-	first.alloc( keys ) , flag.alloc( nOcc+1 ) , vid.alloc( nOcc+1 ) , S.alloc( nOcc );
+	first.alloc( keys ) , flag.alloc( nOcc+1 ) , vid.alloc( nOcc+1 ) , out.S.alloc( nOcc );
 	KFillU32<<< 4096 , 256 , 0 , st >>>( first.p , (long long)keys , NONE );
-	KTriFirst<<< Grid( nOcc ) , 256 , 0 , st >>>( dCells.p , nOcc , dim , tab , kg , first.p );
-	KTriFlag<<< Grid( nOcc ) , 256 , 0 , st >>>( dCells.p , nOcc , dim , tab , kg , first.p , flag.p );
+	KTriFirst<<< Grid( nOcc ) , 256 , 0 , st >>>( dCells , nOcc , dim , tab , kg , first.p );
+	KTriFlag<<< Grid( nOcc ) , 256 , 0 , st >>>( dCells , nOcc , dim , tab , kg , first.p , flag.p );
 	KFillU32<<< 1 , 256 , 0 , st >>>( flag.p + nOcc , 1 , 0 );
 	const unsigned nV = ExclusiveScan( flag.p , vid.p , nOcc , tmp , st );
-	V.alloc( (size_t)nV*dim );
-	KTriVertex<<< Grid( nOcc ) , 256 , 0 , st >>>( dCells.p , nOcc , dim , tab , kg , flag.p , vid.p , first.p , V.p );
-	KTriSimplex<<< Grid( nOcc ) , 256 , 0 , st >>>( dCells.p , nOcc , dim , tab , kg , first.p , S.p );
-	m->v.resize( (size_t)nV*dim ) , m->s.resize( nOcc );
-	V.download( m->v.data() , m->v.size() , st );
-	S.download( m->s.data() , m->s.size() , st );
+	out.V.alloc( (size_t)nV*dim );
+	KTriVertex<<< Grid( nOcc ) , 256 , 0 , st >>>( dCells , nOcc , dim , tab , kg , flag.p , vid.p , first.p , out.V.p );
+	KTriSimplex<<< Grid( nOcc ) , 256 , 0 , st >>>( dCells , nOcc , dim , tab , kg , first.p , out.S.p );
+	HIPCHECK( hipGetLastError() );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	out.nV = nV , out.nS = (size_t)nOcc / ( dim+1 );
+}
+
+static epr_mesh *Download( const DevMesh &d , hipStream_t st )
+{
+	epr_mesh *m = new epr_mesh;
+	m->vdim = d.vdim , m->k = d.k;
+	m->v.resize( d.nV*d.vdim ) , m->s.resize( d.nS*d.k );
+	d.V.download( m->v.data() , m->v.size() , st );
+	d.S.download( m->s.data() , m->s.size() , st );
 	HIPCHECK( hipStreamSynchronize( st ) );
 	return m;
 }
 
-template< int Dim >
-static epr_mesh *LevelSet2( int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *S , const double *values , const double *iso )
+static epr_mesh *Triangulate( int dim , size_t nCells , const unsigned *cells , bool orient )
 {
-	epr_mesh *m = new epr_mesh;
-	m->vdim = vdim , m->k = Dim-1;
-	if( !nS ) return m;
-	if( (unsigned long long)nS*( Dim==4 ? 20 : 4 )>=NONE ) throw std::string( "too many simplices for 32-bit occurrence indices" );
-	for( size_t i=0 ; i<nS*( Dim+1 ) ; i++ ) if( S[i]>=nV ) throw std::string( "simplex vertex index out of range" );
+	unsigned ext[4] = { 0 , 0 , 0 , 0 };
+	for( size_t e=0 ; e<nCells ; e++ ) for( int d=0 ; d<dim ; d++ ) ext[d] = std::max( ext[d] , cells[ e*dim+d ] );
 	hipStream_t st = nullptr;
-	DevBuf< unsigned > dS , cnt , off;
-	DevBuf< double > dVal , dData;
+	DevBuf< unsigned > dCells;
+	dCells.upload( cells , nCells*dim , st );
+	DevMesh d;
+	TriangulateDev( dim , dCells.p , nCells , ext , orient , st , d );
+	return Download( d , st );
+}
+
+template< int Dim >
+void LevelSet2Dev( int vdim , size_t nV , const double *dVdata , size_t nS , const unsigned *dS , const double *dValues , const double iso[2] , hipStream_t st , DevMesh &out )
+{
+	out.vdim = vdim , out.k = Dim-1 , out.nV = out.nS = 0;
+	if( !nS ) return;
+	if( (unsigned long long)nS*( Dim==4 ? 20 : 4 )>=NONE ) throw std::string( "too many simplices for 32-bit occurrence indices" );
+	DevBuf< unsigned > cnt , off;
 	DevBuf< unsigned char > tmp;
-	dS.upload( S , nS*( Dim+1 ) , st );
-	dVal.upload( values , nV*2 , st );
-	dData.upload( vdata , nV*vdim , st );
-	LsMesh M{ dS.p , dVal.p , dData.p , vdim , (long long)nS , iso[0] , iso[1] };
+	LsMesh M{ dS , dValues , dVdata , vdim , (long long)nS , iso[0] , iso[1] };
 	cnt.alloc( nS+1 ) , off.alloc( nS+1 );
 	KLsCount< Dim ><<< Grid( nS ) , 256 , 0 , st >>>( M , cnt.p );
 	KFillU32<<< 1 , 256 , 0 , st >>>( cnt.p + nS , 1 , 0 );
 	const unsigned n = ExclusiveScan( cnt.p , off.p , (long long)nS , tmp , st );
-	if( !n ) return m;
+	if( !n ) return;
 	DevBuf< unsigned > k[6] , flags , parent , cP , cR , isC , vidB , vOf , order , t0 , t1 , e0 , e1 , e2;
 	DevBuf< unsigned char > local;
-	DevBuf< double > recData , outV;
+	DevBuf< double > recData;
 	DevBuf< unsigned long long > h0 , h1;
 	for( int i=0 ; i<6 ; i++ ) k[i].alloc( n );
 	flags.alloc( n ) , parent.alloc( n ) , local.alloc( n ) , recData.alloc( (size_t)n*vdim );
@@ -812,46 +812,39 @@ static epr_mesh *LevelSet2( int vdim , size_t nV , const double *vdata , size_t 
 	KCreatorFlags<<< Grid( n ) , 256 , 0 , st >>>( flags.p , cP.p , cR.p , n , isC.p );
 	KFillU32<<< 1 , 256 , 0 , st >>>( isC.p + n , 1 , 0 );
 	const unsigned nLV = ExclusiveScan( isC.p , vidB.p , n , tmp , st );
-	outV.alloc( (size_t)nLV*vdim );
-	KResolve<<< Grid( n ) , 256 , 0 , st >>>( flags.p , cP.p , cR.p , isC.p , vidB.p , recData.p , vdim , n , vOf.p , outV.p );
+	out.V.alloc( (size_t)nLV*vdim );
+	KResolve<<< Grid( n ) , 256 , 0 , st >>>( flags.p , cP.p , cR.p , isC.p , vidB.p , recData.p , vdim , n , vOf.p , out.V.p );
 	// output simplices in simplex order
-	DevBuf< unsigned > oc , oo , out;
+	DevBuf< unsigned > oc , oo;
 	oc.alloc( nS+1 ) , oo.alloc( nS+1 );
 	KAssemble< Dim , false ><<< Grid( nS ) , 256 , 0 , st >>>( off.p , (long long)nS , vOf.p , local.p , oc.p , nullptr , nullptr );
 	KFillU32<<< 1 , 256 , 0 , st >>>( oc.p + nS , 1 , 0 );
 	const unsigned nLS = ExclusiveScan( oc.p , oo.p , (long long)nS , tmp , st );
-	out.alloc( (size_t)nLS*( Dim-1 ) );
-	if( nLS ) KAssemble< Dim , true ><<< Grid( nS ) , 256 , 0 , st >>>( off.p , (long long)nS , vOf.p , local.p , nullptr , oo.p , out.p );
-	m->v.resize( (size_t)nLV*vdim ) , m->s.resize( (size_t)nLS*( Dim-1 ) );
-	outV.download( m->v.data() , m->v.size() , st );
-	out.download( m->s.data() , m->s.size() , st );
+	out.S.alloc( (size_t)nLS*( Dim-1 ) );
+	if( nLS ) KAssemble< Dim , true ><<< Grid( nS ) , 256 , 0 , st >>>( off.p , (long long)nS , vOf.p , local.p , nullptr , oo.p , out.S.p );
+	HIPCHECK( hipGetLastError() );
+	// the temporaries above are freed when this returns: finish their work first
 	HIPCHECK( hipStreamSynchronize( st ) );
-	return m;
+	out.nV = nLV , out.nS = nLS;
 }
+
 template< int Dim >
-static epr_mesh *LevelSet1( int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *S , const double *values , double iso )
+void LevelSet1Dev( int vdim , size_t nV , const double *dVdata , size_t nS , const unsigned *dS , const double *dValues , double iso , hipStream_t st , DevMesh &out )
 {
-	epr_mesh *m = new epr_mesh;
-	m->vdim = vdim , m->k = Dim;
-	if( !nS ) return m;
+	out.vdim = vdim , out.k = Dim , out.nV = out.nS = 0;
+	if( !nS ) return;
 	if( (unsigned long long)nS*( Dim==4 ? 40 : 8 )>=NONE ) throw std::string( "too many simplices for 32-bit occurrence indices" );
-	for( size_t i=0 ; i<nS*( Dim+1 ) ; i++ ) if( S[i]>=nV ) throw std::string( "simplex vertex index out of range" );
-	hipStream_t st = nullptr;
-	DevBuf< unsigned > dS , cnt , off;
-	DevBuf< double > dVal , dData;
+	DevBuf< unsigned > cnt , off;
 	DevBuf< unsigned char > tmp;
-	dS.upload( S , nS*( Dim+1 ) , st );
-	dVal.upload( values , nV , st );
-	dData.upload( vdata , nV*vdim , st );
-	LsMesh M{ dS.p , dVal.p , dData.p , vdim , (long long)nS , iso , 0. };
+	LsMesh M{ dS , dValues , dVdata , vdim , (long long)nS , iso , 0. };
 	cnt.alloc( nS+1 ) , off.alloc( nS+1 );
 	KLs1Count< Dim ><<< Grid( nS ) , 256 , 0 , st >>>( M , cnt.p );
 	KFillU32<<< 1 , 256 , 0 , st >>>( cnt.p + nS , 1 , 0 );
 	const unsigned n = ExclusiveScan( cnt.p , off.p , (long long)nS , tmp , st );
-	if( !n ) return m;
+	if( !n ) return;
 	DevBuf< unsigned long long > key , skey;
 	DevBuf< unsigned char > local , swp;
-	DevBuf< double > recData , outV;
+	DevBuf< double > recData;
 	DevBuf< unsigned > iota , order , first , flag , vid , vOf;
 	key.alloc( n ) , skey.alloc( n ) , local.alloc( n ) , swp.alloc( n ) , recData.alloc( (size_t)n*vdim );
 	Ls1Records R{ key.p , local.p , swp.p , recData.p };
@@ -867,20 +860,42 @@ static epr_mesh *LevelSet1( int vdim , size_t nV , const double *vdata , size_t 
 	KIsFirst<<< Grid( n ) , 256 , 0 , st >>>( first.p , n , flag.p );
 	KFillU32<<< 1 , 256 , 0 , st >>>( flag.p + n , 1 , 0 );
 	const unsigned nLV = ExclusiveScan( flag.p , vid.p , n , tmp , st );
-	outV.alloc( (size_t)nLV*vdim );
-	KLs1Vertices<<< Grid( n ) , 256 , 0 , st >>>( first.p , vid.p , recData.p , vdim , n , vOf.p , outV.p );
-	DevBuf< unsigned > oc , oo , out;
+	out.V.alloc( (size_t)nLV*vdim );
+	KLs1Vertices<<< Grid( n ) , 256 , 0 , st >>>( first.p , vid.p , recData.p , vdim , n , vOf.p , out.V.p );
+	DevBuf< unsigned > oc , oo;
 	oc.alloc( nS+1 ) , oo.alloc( nS+1 );
 	KAssemble1< Dim , false ><<< Grid( nS ) , 256 , 0 , st >>>( off.p , (long long)nS , vOf.p , local.p , swp.p , oc.p , nullptr , nullptr );
 	KFillU32<<< 1 , 256 , 0 , st >>>( oc.p + nS , 1 , 0 );
 	const unsigned nLS = ExclusiveScan( oc.p , oo.p , (long long)nS , tmp , st );
-	out.alloc( (size_t)nLS*Dim );
-	if( nLS ) KAssemble1< Dim , true ><<< Grid( nS ) , 256 , 0 , st >>>( off.p , (long long)nS , vOf.p , local.p , swp.p , nullptr , oo.p , out.p );
-	m->v.resize( (size_t)nLV*vdim ) , m->s.resize( (size_t)nLS*Dim );
-	outV.download( m->v.data() , m->v.size() , st );
-	out.download( m->s.data() , m->s.size() , st );
+	out.S.alloc( (size_t)nLS*Dim );
+	if( nLS ) KAssemble1< Dim , true ><<< Grid( nS ) , 256 , 0 , st >>>( off.p , (long long)nS , vOf.p , local.p , swp.p , nullptr , oo.p , out.S.p );
+	HIPCHECK( hipGetLastError() );
 	HIPCHECK( hipStreamSynchronize( st ) );
-	return m;
+	out.nV = nLV , out.nS = nLS;
+}
+
+template void LevelSet2Dev< 3 >( int , size_t , const double * , size_t , const unsigned * , const double * , const double [2] , hipStream_t , DevMesh & );
+template void LevelSet2Dev< 4 >( int , size_t , const double * , size_t , const unsigned * , const double * , const double [2] , hipStream_t , DevMesh & );
+template void LevelSet1Dev< 3 >( int , size_t , const double * , size_t , const unsigned * , const double * , double , hipStream_t , DevMesh & );
+template void LevelSet1Dev< 4 >( int , size_t , const double * , size_t , const unsigned * , const double * , double , hipStream_t , DevMesh & );
+
+template< int Dim >
+static epr_mesh *LevelSet( int codim , int vdim , size_t nV , const double *vdata , size_t nS , const unsigned *S , const double *values , const double *iso )
+{
+	for( size_t i=0 ; i<nS*( Dim+1 ) ; i++ ) if( S[i]>=nV ) throw std::string( "simplex vertex index out of range" );
+	hipStream_t st = nullptr;
+	DevBuf< unsigned > dS;
+	DevBuf< double > dVal , dData;
+	DevMesh d;
+	if( nS )
+	{
+		dS.upload( S , nS*( Dim+1 ) , st );
+		dVal.upload( values , nV*codim , st );
+		dData.upload( vdata , nV*vdim , st );
+	}
+	if( codim==2 ) LevelSet2Dev< Dim >( vdim , nV , dData.p , nS , dS.p , dVal.p , iso , st , d );
+	else           LevelSet1Dev< Dim >( vdim , nV , dData.p , nS , dS.p , dVal.p , iso[0] , st , d );
+	return Download( d , st );
 }
 } // namespace ls
 } // namespace epr
@@ -908,16 +923,8 @@ int epr_level_set( int dim , int codim , int vdim , size_t nV , const double *vd
 	if( !iso ) iso = zero;
 	return epr::ContextFree( [&]()
 	{
-		if( codim==2 )
-		{
-			if( dim==3 ) *out = epr::ls::LevelSet2< 3 >( vdim , nV , vdata , nS , simplices , values , iso );
-			else         *out = epr::ls::LevelSet2< 4 >( vdim , nV , vdata , nS , simplices , values , iso );
-		}
-		else
-		{
-			if( dim==3 ) *out = epr::ls::LevelSet1< 3 >( vdim , nV , vdata , nS , simplices , values , iso[0] );
-			else         *out = epr::ls::LevelSet1< 4 >( vdim , nV , vdata , nS , simplices , values , iso[0] );
-		}
+		if( dim==3 ) *out = epr::ls::LevelSet< 3 >( codim , vdim , nV , vdata , nS , simplices , values , iso );
+		else         *out = epr::ls::LevelSet< 4 >( codim , vdim , nV , vdata , nS , simplices , values , iso );
 	} );
 }
 

@@ -211,6 +211,56 @@ int  epr_time_copy( size_t bytes , int reps , double *ms );
 /* Wall-clock seconds of the last epr_build / epr_solve (device-synchronised). */
 double epr_last_seconds( const epr_ctx *ctx , int which /* 0 build, 1 solve */ );
 
+/* ---- Surface extraction from a .tree (the reference's Extract/Extract.cpp:96-330) ----
+ * An iso handle holds a solved hierarchy read from a .tree file (dense or
+ * adapted): the finest regular values of both functions in device memory,
+ * recomposed as the reference's IsoTree does (Include/IsoTree.inl:45-73,
+ * 314-338), and the indexed finest elements of an adapted hierarchy. */
+typedef struct epr_iso epr_iso;
+
+/* Replaces Extract.cpp:109-129 (read dim, indexer hash, voxel-to-world, the
+ * hierarchical x and y, the indexer; IsoTree construction).  Unrecognised
+ * hashes and dimensions other than 3 and 4 are errors, as Extract.cpp:352-363
+ * makes them.  Errors: epr_iso_last_error( NULL ). */
+int  epr_iso_open( const char *treePath , epr_iso **out );
+const char *epr_iso_last_error( const epr_iso *iso );
+/* depth = the finest depth D; xform (optional) receives the (dim+1)^2 voxel-to-world coordinates as stored. */
+int  epr_iso_info( const epr_iso *iso , int *dim , int *depth , int *adapted , double xform[25] );
+/* The finest regular values, (2^D+1)^dim pairs (phi1, phi2), last coordinate fastest. */
+int  epr_iso_values( epr_iso *iso , double *out );
+
+/* Replaces IsoTree::refineZeroCrossing + levelSetNodes( all ) (IsoTree.inl:165-234):
+ * the finest elements of the level set, in tree order, as dim offsets each.
+ * cells may be NULL (count only); at most cap cells are copied. */
+int  epr_iso_level_set_nodes( epr_iso *iso , int all , size_t cap , unsigned *cells , size_t *count );
+
+/* Replaces Extract.cpp:161-259 up to the PLY write: triangulation of the
+ * level-set nodes, vertex values, the joint zero set, MarchingSimplices::Orient,
+ * per-vertex gradient frames and the voxel-to-world map.  The mesh has
+ * 3*dim doubles per vertex (world position, then the frame n_c_d for c = 0, 1,
+ * d = 0..dim-1) and dim-1 vertices per simplex.  all = 1 for the mesh output
+ * alone, 0 when the --levelSets output is also requested (Extract.cpp:129, 161). */
+int  epr_iso_extract( epr_iso *iso , int all , epr_mesh **out );
+
+/* Replaces Extract.cpp:296-310: the codimension-1 zero set of one component
+ * (0 or 1) over the all = 0 triangulation, dim world coordinates per vertex,
+ * dim vertices per simplex. */
+int  epr_iso_level_set( epr_iso *iso , int component , epr_mesh **out );
+
+/* Replaces Extract.cpp:132-144: header.0.grid and header.1.grid of the finest
+ * regular values (RegularGrid<dim,double>::write, Misha/RegularGrid.inl:441-471). */
+int  epr_iso_write_grid( epr_iso *iso , const char *header );
+
+/* Seconds of the last open (0), level-set nodes and triangulation (1), level
+ * set and orientation (2), frames (3); no reference counterpart. */
+double epr_iso_seconds( const epr_iso *iso , int which );
+void epr_iso_destroy( epr_iso *iso );
+
+/* typeid( MishaK::Hat::Hierarchical{Regular,Adapted}Indexer< dim > ).hash_code()
+ * of a libstdc++ build of the reference: the value a .tree carries after its
+ * dimension (ExteriorPoissonRecon.cpp:263). */
+unsigned long long epr_indexer_hash( int dim , int adapted );
+
 #ifdef __cplusplus
 }
 #endif

@@ -109,6 +109,19 @@ int    orc_level_set( int dim , int codim , int vdim , size_t nV , const double 
                       const double *values , const double *iso , size_t *nLV , double **LV , size_t *nLS , unsigned **LS );
 void   orc_free( void *p );
 
+/* ---- Extract pipeline (epr_extract.cpp) ---- */
+/* Extract.cpp:109-259 on a .tree file: the level-set nodes (nCells x dim),
+ * the finest cumulative values ((2^D+1)^dim pairs), and the oriented joint
+ * zero set with 3*dim doubles per vertex (world position, frames n_0, n_1)
+ * and dim-1 vertices per simplex.  all as in IsoTree::levelSetNodes.  Arrays
+ * are malloc'ed (orc_free).  Nonzero on error (orc_extract_error). */
+int    orc_extract( const char *treePath , int all , size_t *nCells , unsigned **cells , double **values ,
+                    size_t *nV , double **V , size_t *nS , unsigned **S );
+/* Extract.cpp:296-310: the codimension-1 zero set of one component, world positions */
+int    orc_extract_level_set( const char *treePath , int component , size_t *nV , double **V , size_t *nS , unsigned **S );
+const char *orc_extract_error( void );
+unsigned long long orc_indexer_hash( int dim , int adapted );
+
 #ifdef __cplusplus
 }
 #endif

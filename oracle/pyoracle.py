@@ -67,6 +67,14 @@ def lib():
         L.orc_level_set.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, sz, dp, sz, up, dp, dp,
                                     ctypes.POINTER(sz), ctypes.POINTER(dp), ctypes.POINTER(sz), ctypes.POINTER(up)]
         L.orc_free.argtypes = [ctypes.c_void_p]
+        L.orc_extract.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(sz), ctypes.POINTER(up),
+                                  ctypes.POINTER(dp), ctypes.POINTER(sz), ctypes.POINTER(dp), ctypes.POINTER(sz),
+                                  ctypes.POINTER(up)]
+        L.orc_extract_level_set.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(sz), ctypes.POINTER(dp),
+                                            ctypes.POINTER(sz), ctypes.POINTER(up)]
+        L.orc_extract_error.restype = ctypes.c_char_p
+        L.orc_indexer_hash.restype = ctypes.c_ulonglong
+        L.orc_indexer_hash.argtypes = [ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -273,3 +281,49 @@ def level_set(vertices, simplices, values, iso=None):
         raise ValueError("bad level-set input")
     k = dim - codim + 1
     return _take(LV, nLV.value * vdim, np.float64).reshape(-1, vdim), _take(LS, nLS.value * k, np.uint32).reshape(-1, k)
+
+
+# ---- the Extract pipeline ---------------------------------------------------
+def tree_header(path):
+    """(dim, depth) of a .tree: u32 dim, u64 hash, the XForm (8 + (dim+1)^2 * 8 bytes), u64 depth count."""
+    with open(path, "rb") as f:
+        b = f.read(4 + 8 + 8 + 25 * 8 + 8)
+    dim = struct.unpack_from("<I", b, 0)[0]
+    n = struct.unpack_from("<Q", b, 20 + 8 * (dim + 1) ** 2)[0]
+    return dim, n - 1
+
+
+def extract(tree, all=True):
+    """The oracle's Extract.cpp:109-259 on a .tree: dict with cells (level-set
+    nodes, tree order), values ((2^D+1)^dim x 2 finest values), vertices (nV x
+    3 dim: world position, n_0, n_1) and simplices (nS x (dim-1), oriented)."""
+    dim, D = tree_header(tree)
+    sz, up, dp = ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_double)
+    nC, nV, nS = sz(), sz(), sz()
+    C, S = up(), up()
+    val, V = dp(), dp()
+    rc = lib().orc_extract(tree.encode(), int(all), ctypes.byref(nC), ctypes.byref(C), ctypes.byref(val),
+                           ctypes.byref(nV), ctypes.byref(V), ctypes.byref(nS), ctypes.byref(S))
+    if rc:
+        raise ValueError(lib().orc_extract_error().decode())
+    nF = ((1 << D) + 1) ** dim
+    return {"cells": _take(C, nC.value * dim, np.uint32).reshape(-1, dim),
+            "values": _take(val, 2 * nF, np.float64).reshape(-1, 2),
+            "vertices": _take(V, nV.value * 3 * dim, np.float64).reshape(-1, 3 * dim),
+            "simplices": _take(S, nS.value * (dim - 1), np.uint32).reshape(-1, dim - 1)}
+
+
+def extract_level_set(tree, component):
+    """The oracle's Extract.cpp:296-310 for one component: (world vertices, simplices)."""
+    dim, _ = tree_header(tree)
+    nV, nS = ctypes.c_size_t(), ctypes.c_size_t()
+    V, S = ctypes.POINTER(ctypes.c_double)(), ctypes.POINTER(ctypes.c_uint)()
+    rc = lib().orc_extract_level_set(tree.encode(), component, ctypes.byref(nV), ctypes.byref(V),
+                                     ctypes.byref(nS), ctypes.byref(S))
+    if rc:
+        raise ValueError(lib().orc_extract_error().decode())
+    return _take(V, nV.value * dim, np.float64).reshape(-1, dim), _take(S, nS.value * dim, np.uint32).reshape(-1, dim)
+
+
+def indexer_hash(dim, adapted):
+    return int(lib().orc_indexer_hash(dim, int(adapted)))

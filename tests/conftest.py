@@ -57,3 +57,53 @@ def oracle_lib():
     from oracle import pyoracle
     pyoracle.build()
     return pyoracle
+
+
+# ---- the reference's Extract PLY outputs (Misha/Ply.inl:673-737, PlyFile.inl:314-356:
+# binary little endian, faces as "list int int vertex_indices") -------------------
+def ply_bytes(props, nrows, row_bytes, faces, k):
+    head = ["ply", "format binary_little_endian 1.0", f"element vertex {nrows}"]
+    head += [f"property {p}" for p in props]
+    head += [f"element face {len(faces)}", "property list int int vertex_indices", "end_header", ""]
+    f = np.empty((len(faces), k + 1), dtype="<i4")
+    f[:, 0] = k
+    f[:, 1:] = np.asarray(faces).reshape(-1, k)
+    return "\n".join(head).encode() + row_bytes + f.tobytes()
+
+
+def mesh_ply(dim, V, S):
+    """Extract.cpp --out: x.. and n_c_d doubles per vertex, (dim-1)-vertex faces."""
+    props = [f"double {a}" for a in "xyzw"[:dim]] + [f"double n_{c}_{d}" for c in range(2) for d in range(dim)]
+    return ply_bytes(props, len(V), np.ascontiguousarray(V, "<f8").tobytes(), S, dim - 1)
+
+
+def level_sets_ply(dim, sets):
+    """Extract.cpp --levelSets: both components' sets, colours (223,41,53) and (55,114,255) as uchar."""
+    props = [f"double {a}" for a in "xyzw"[:dim]] + ["uchar red", "uchar green", "uchar blue"]
+    colours = ((223, 41, 53), (55, 114, 255))
+    row = np.dtype([("p", "<f8", (dim,)), ("c", "u1", (3,))])
+    rows, faces, off = [], [], 0
+    for c, (V, S) in enumerate(sets):
+        r = np.zeros(len(V), dtype=row)
+        r["p"], r["c"] = np.asarray(V).reshape(-1, dim), colours[c]
+        rows.append(r.tobytes())
+        faces.append(np.asarray(S, np.int64).reshape(-1, dim) + off)
+        off += len(V)
+    return ply_bytes(props, off, b"".join(rows), np.concatenate(faces) if faces else np.zeros((0, dim)), dim)
+
+
+def sha256(b):
+    import hashlib
+    return hashlib.sha256(b).hexdigest()
+
+
+EXTRACT_FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.startswith("extract_") and f.endswith(".npz"))
+
+
+def fixture_tree(name, tmp_path):
+    """Writes the .tree bytes of an Extract fixture to tmp_path and returns (path, fixture)."""
+    g = load_golden(name)
+    p = str(tmp_path / (name + ".tree"))
+    with open(p, "wb") as f:
+        f.write(g["tree"].tobytes())
+    return p, g

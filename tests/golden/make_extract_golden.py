@@ -1,0 +1,91 @@
+"""Writes the Extract fixtures (tests/golden/extract_*.npz) with the reference build.
+
+For each case: oracle/_ref/ExteriorPoissonRecon (the reference command line
+compiled by `make -C oracle ref`) writes a .tree from synthetic samples, then
+oracle/_ref/Extract runs on it three ways:
+
+    --out m.ply                      the joint zero set with frames (all = true)
+    --out a.ply --levelSets l.ply    the same with all = false, and both codimension-1 sets
+    --grid g                         the finest regular values of both functions
+
+A fixture holds the .tree bytes (the input), the SHA-256 and the element counts
+of every PLY the reference wrote, and (small grids) the --grid values.  Run here only:
+
+    python tests/golden/make_extract_golden.py
+"""
+import hashlib
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.abspath(os.path.join(os.path.dirname(__file__), "..", ".."))
+sys.path.insert(0, ROOT)
+from exteriorpoissonrecon_amd import samples  # noqa: E402
+
+REF = os.path.join(ROOT, "oracle", "_ref")
+
+CASES = {
+    # 3D torus knot, dense (--aRadius -1) and adapted (the default --aRadius 3)
+    "extract_knot3_d4": dict(gen=lambda: samples.torus_knot(4096, seed=4), args=dict(depth=4, aRadius=-1)),
+    "extract_knot3_d4_adapted": dict(gen=lambda: samples.torus_knot(4096, seed=4), args=dict(depth=4)),
+    "extract_knot3_d5_adapted_r1": dict(gen=lambda: samples.torus_knot(1024, p=3, q=5, seed=3), args=dict(depth=5, aRadius=1)),
+    # 4D Clifford torus, dense and adapted
+    "extract_clifford4_d3": dict(gen=lambda: samples.clifford_torus(side=40, ang_deg=2.0, seed=4), args=dict(depth=3, aRadius=-1)),
+    "extract_clifford4_d3_adapted": dict(gen=lambda: samples.clifford_torus(side=40, ang_deg=2.0, seed=4), args=dict(depth=3)),
+}
+
+
+def ply_counts(path):
+    b = open(path, "rb").read()
+    head = b[:b.index(b"end_header\n")].decode().split("\n")
+    nv = int([ln for ln in head if ln.startswith("element vertex")][0].split()[-1])
+    nf = int([ln for ln in head if ln.startswith("element face")][0].split()[-1])
+    return hashlib.sha256(b).hexdigest(), nv, nf
+
+
+def read_grid(path):
+    b = open(path, "rb").read()
+    lines, o = 0, 0
+    dim = int(b[1:b.index(b"\n")])
+    while lines < 3 + dim + 1:
+        o = b.index(b"\n", o) + 1
+        lines += 1
+    return np.frombuffer(b, "<f8", offset=o).copy()
+
+
+def run(name, case):
+    pos, skew = case["gen"]()
+    with tempfile.TemporaryDirectory() as td:
+        ply = os.path.join(td, "in.ply")
+        samples.write_ply(ply, pos, skew)
+        cmd = [os.path.join(REF, "ExteriorPoissonRecon"), "--in", ply, "--out", os.path.join(td, "o")]
+        for k, v in case["args"].items():
+            cmd += ["--" + k, str(v)]
+        subprocess.run(cmd, check=True, capture_output=True)
+        tree = os.path.join(td, "o.tree")
+        ext = os.path.join(REF, "Extract")
+        j = lambda f: os.path.join(td, f)  # noqa: E731
+        subprocess.run([ext, "--in", tree, "--out", j("m.ply")], check=True, capture_output=True)
+        subprocess.run([ext, "--in", tree, "--out", j("a.ply"), "--levelSets", j("l.ply")], check=True, capture_output=True)
+        subprocess.run([ext, "--in", tree, "--grid", j("g")], check=True, capture_output=True)
+        out = {"tree": np.frombuffer(open(tree, "rb").read(), np.uint8)}
+        for key, f in (("mesh", "m.ply"), ("mesh_any", "a.ply"), ("level_sets", "l.ply")):
+            h, nv, nf = ply_counts(j(f))
+            out[key + "_sha256"] = np.array(h)
+            out[key + "_counts"] = np.array([nv, nf])
+        g0, g1 = read_grid(j("g.0.grid")), read_grid(j("g.1.grid"))
+        if len(g0) <= 20000:                                 # small grids only: the values do not compress
+            out["grid0"], out["grid1"] = g0, g1
+            out["grid_header"] = np.frombuffer(open(j("g.0.grid"), "rb").read()[:-8 * len(g0)], np.uint8)
+    path = os.path.join(os.path.dirname(__file__), name + ".npz")
+    np.savez_compressed(path, **out)
+    print(name, {k: v.tolist() for k, v in out.items() if k.endswith("_counts")}, os.path.getsize(path), "bytes")
+
+
+if __name__ == "__main__":
+    for name, case in CASES.items():
+        if len(sys.argv) == 1 or name in sys.argv[1:]:
+            run(name, case)

@@ -76,3 +76,60 @@ def test_cli_errors_exit_zero(args, msg):
     r = run_cli(*args)
     assert r.returncode == 0
     assert "[ERROR]" in r.stderr and msg in r.stderr
+
+
+# ---- the Extract boundary: argument and file errors, before any device work -----------
+EXTRACT = os.path.join(ROOT, "exteriorpoissonrecon_amd", "bin", "Extract")
+
+
+def test_extract_usage_without_input():
+    r = subprocess.run([EXTRACT], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "--in" in r.stdout and "--levelSets" in r.stdout
+
+
+def test_indexer_hash_matches_reference_trees():
+    """typeid(...).hash_code() as the reference writes it: the fixtures' trees carry these values."""
+    from conftest import EXTRACT_FIXTURES, load_golden
+    from exteriorpoissonrecon_amd import load
+    for name in EXTRACT_FIXTURES:
+        t = load_golden(name)["tree"].tobytes()
+        dim = int(np.frombuffer(t, "<u4", 1, 0)[0])
+        h = int(np.frombuffer(t, "<u8", 1, 4)[0])
+        assert h == load().epr_indexer_hash(dim, int("adapted" in name)), name
+
+
+def _tree_bytes(name):
+    from conftest import load_golden
+    return bytearray(load_golden(name)["tree"].tobytes())
+
+
+@pytest.mark.parametrize("edit,msg", [
+    (lambda b: b.__setitem__(slice(4, 12), (12345).to_bytes(8, "little")), "Unrecognized hierarchical indexer hash: 12345"),
+    (lambda b: b.__setitem__(slice(0, 4), (5).to_bytes(4, "little")), "Only dimensions 3 and 4 supported: 5"),
+    (lambda b: b.__delitem__(slice(len(b) - 100, len(b))), "truncated"),
+])
+def test_extract_file_errors(edit, msg, tmp_path):
+    """Extract.cpp:352-363 rejects unknown indexers and dimensions; a short file is an error too."""
+    from exteriorpoissonrecon_amd import EprError
+    from exteriorpoissonrecon_amd.extract import IsoTree
+    b = _tree_bytes("extract_knot3_d4_adapted")
+    edit(b)
+    p = str(tmp_path / "bad.tree")
+    open(p, "wb").write(bytes(b))
+    with pytest.raises(EprError, match=msg.replace("[", r"\[")):
+        IsoTree(p)
+    r = subprocess.run([EXTRACT, "--in", p, "--out", str(tmp_path / "o.ply")], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and "[ERROR]" in r.stderr and msg in r.stderr
+
+
+def test_extract_ply_serialisers_agree(tmp_path):
+    """The package's PLY writers produce the bytes the reference's writer does (the tests' restatement)."""
+    from conftest import level_sets_ply, mesh_ply
+    from exteriorpoissonrecon_amd.extract import level_sets_ply_bytes, mesh_ply_bytes
+    rng = np.random.default_rng(0)
+    for dim in (3, 4):
+        V = rng.normal(size=(7, 3 * dim))
+        S = rng.integers(0, 7, size=(5, dim - 1))
+        assert mesh_ply_bytes(dim, V, S) == mesh_ply(dim, V, S)
+        sets = [(rng.normal(size=(4, dim)), rng.integers(0, 4, size=(3, dim))) for _ in range(2)]
+        assert level_sets_ply_bytes(dim, sets) == level_sets_ply(dim, sets)
