@@ -100,7 +100,6 @@ int main( int argc , char *argv[] )
 		else fprintf( stderr , "[WARNING] Unrecognized argument: --%s\n" , a.c_str() );
 	}
 	if( in.empty() ){ Usage( argv[0] ) ; return EXIT_SUCCESS; }
-	if( !densityPath.empty() && trim>0 ) Error( "density trimming (--density with --trimDensity>0) is not supported by this build" );
 
 	epr_iso *iso = nullptr;
 	if( epr_iso_open( in.c_str() , &iso ) ) Error( epr_iso_last_error( nullptr ) );
@@ -108,13 +107,23 @@ int main( int argc , char *argv[] )
 	epr_iso_info( iso , &dim , &depth , &adapted , nullptr );
 	if( verbose ) printf( "%25s: %.3f (s)\n" , "Iso-tree" , epr_iso_seconds( iso , 0 ) );
 	if( !grid.empty() && epr_iso_write_grid( iso , grid.c_str() ) ) Error( epr_iso_last_error( iso ) );
+	if( !densityPath.empty() && epr_iso_read_density( iso , densityPath.c_str() ) ) Error( epr_iso_last_error( iso ) );
 
 	const bool all = levelSets.empty();
 	if( !out.empty() )
 	{
 		epr_mesh *m = nullptr;
-		if( epr_iso_extract( iso , all ? 1 : 0 , &m ) ) Error( epr_iso_last_error( iso ) );
+		if( epr_iso_extract_trimmed( iso , all ? 1 : 0 , densityPath.empty() ? 0. : trim , &m ) ) Error( epr_iso_last_error( iso ) );
 		Mesh M = Take( m );
+		if( verbose && !densityPath.empty() && trim>0 )
+		{
+			size_t n = 0;
+			epr_iso_component_densities( iso , 0 , nullptr , &n );
+			std::vector< double > mx( n );
+			epr_iso_component_densities( iso , n , mx.data() , &n );
+			// std::cout's default formatting (6 significant digits), as Extract.cpp:227 prints
+			for( size_t i=0 ; i<n ; i++ ) printf( "Max Density[ %zu ] %g\n" , i , mx[i] );
+		}
 		if( verbose )
 		{
 			printf( "%25s: %.3f (s)\n" , "Triangulation" , epr_iso_seconds( iso , 1 ) );

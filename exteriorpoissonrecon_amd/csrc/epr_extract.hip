@@ -27,6 +27,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <memory>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -468,99 +469,332 @@ __global__ void __launch_bounds__( 256 ) KToWorld( const double *X , size_t nV ,
 
 static inline unsigned Grid( long long n ){ return (unsigned)std::max< long long >( 1 , ( n + 255 )/256 ); }
 
-// ------------------------------------------------------------------ orientation (host)
+// ------------------------------------------------------------------ orientation
 // MarchingSimplices::Orient (Misha/MarchingSimplices.h:179-246) for (K+1)-vertex
 // simplices.  Face keys are the sorted vertex sets; the two simplices incident
 // on a face are the first two in (simplex, face) order, a third is an error.
-// Seeds are taken in index order; a depth-first stack visits the faces of
-// each simplex in the order of its current (possibly flipped) vertices; a
-// flip swaps the first two vertices.
-template< int K >
-static void OrientFaceKey( const unsigned *si , int v , unsigned long long &key , bool &even )
-{
-	if( K==1 )
-	{
-		key = si[ v==0 ? 1 : 0 ] , even = v==0;
-		return;
-	}
-	// K==2: the face opposite vertex v, {si[v+1], si[v+2]} cyclically (no swap for even K)
-	const unsigned a = si[ ( v+1 )%3 ] , b = si[ ( v+2 )%3 ];
-	even = a<b;
-	key = a<b ? ( (unsigned long long)a<<32 | b ) : ( (unsigned long long)b<<32 | a );
-}
+// The face links are found on the device (keys, a radix sort, neighbours in
+// the sorted order); the traversal runs on the host: seeds in index order, a
+// depth-first stack visiting the faces of each simplex in the order of its
+// current (possibly flipped) vertices, a flip swapping the first two vertices.
+constexpr unsigned NONE32 = 0xffffffffu;
 
-struct FaceLinks
-{
-	std::vector< long long > other;      // per (simplex, face): the other incident simplex, or -1
-	std::vector< unsigned char > same;   // per (simplex, face): both incidences carry the same orientation
-};
-
+// the face opposite vertex v: for K=1 the other vertex (orientation: v==0),
+// for K=2 the edge {si[v+1], si[v+2]} cyclically (orientation: increasing)
+// (OrientedFaceIndex, MarchingSimplices.h:121-146; no swap for even K)
 template< int K >
-static FaceLinks LinkFaces( const std::vector< unsigned > &S )
+__global__ void __launch_bounds__( 256 ) KFaceRecords( const unsigned *S , unsigned nR , int vb , unsigned long long *key , unsigned *id , unsigned char *even )
 {
-	const size_t nS = S.size()/( K+1 ) , nR = nS*( K+1 );
-	std::vector< std::pair< unsigned long long , unsigned > > rec( nR );
-	std::vector< unsigned char > even( nR );
-	for( size_t s=0 ; s<nS ; s++ ) for( int v=0 ; v<=K ; v++ )
+	const unsigned r = blockIdx.x*blockDim.x + threadIdx.x;
+	if( r>=nR ) return;
+	const unsigned *si = S + ( r/( K+1 ) )*( K+1 );
+	const int v = r%( K+1 );
+	if( K==1 ){ key[r] = si[ v==0 ? 1 : 0 ] , even[r] = v==0; }
+	else
 	{
-		unsigned long long key ; bool e;
-		OrientFaceKey< K >( &S[ s*( K+1 ) ] , v , key , e );
-		rec[ s*( K+1 )+v ] = { key , (unsigned)( s*( K+1 )+v ) } , even[ s*( K+1 )+v ] = e;
+		const unsigned a = si[ ( v+1 )%3 ] , b = si[ ( v+2 )%3 ];
+		even[r] = a<b;
+		key[r] = a<b ? ( (unsigned long long)a<<vb | b ) : ( (unsigned long long)b<<vb | a );
 	}
-	std::sort( rec.begin() , rec.end() );                // by key, then by (simplex, face)
-	FaceLinks L;
-	L.other.assign( nR , -1 ) , L.same.assign( nR , 0 );
-	for( size_t i=0 ; i<nR ; )
-	{
-		size_t j = i+1;
-		while( j<nR && rec[j].first==rec[i].first ) j++;
-		if( j-i>2 ) throw std::string( "Both faces occupied" );
-		if( j-i==2 )
-		{
-			const unsigned a = rec[i].second , b = rec[i+1].second;
-			L.other[a] = b/( K+1 ) , L.other[b] = a/( K+1 );
-			L.same[a] = L.same[b] = even[a]==even[b];
-		}
-		i = j;
-	}
-	return L;
+	id[r] = r;
 }
 
 template< int K >
-static void Orient( std::vector< unsigned > &S )
+__global__ void __launch_bounds__( 256 ) KFaceLink( const unsigned long long *sk , const unsigned *sid , unsigned nR , const unsigned char *even ,
+                                                     unsigned *other , unsigned char *same , unsigned *err )
+{
+	const unsigned i = blockIdx.x*blockDim.x + threadIdx.x;
+	if( i>=nR ) return;
+	const unsigned long long k = sk[i];
+	if( i+2<nR && sk[i+2]==k ) atomicOr( err , 1u );
+	unsigned p = NONE32;
+	if( i>0 && sk[i-1]==k ) p = sid[i-1];
+	else if( i+1<nR && sk[i+1]==k ) p = sid[i+1];
+	const unsigned r = sid[i];
+	other[r] = p==NONE32 ? NONE32 : p/( K+1 );
+	same[r] = p==NONE32 ? 0 : ( even[r]==even[p] );
+}
+
+template< int K >
+static void LinkFaces( const unsigned *dS , size_t nS , size_t nV , hipStream_t st , std::vector< unsigned > &other , std::vector< unsigned char > &same )
+{
+	const size_t nR = nS*( K+1 );
+	if( nR>=NONE32 ) throw std::string( "too many simplices for 32-bit face records" );
+	int vb = 1;
+	while( ( 1ull<<vb )<nV ) vb++;
+	const int bits = K==1 ? vb : 2*vb;
+	DevBuf< unsigned long long > key , skey;
+	DevBuf< unsigned > id , sid , dOther , err;
+	DevBuf< unsigned char > even , dSame , tmp;
+	key.alloc( nR ) , skey.alloc( nR ) , id.alloc( nR ) , sid.alloc( nR ) , even.alloc( nR ) , dOther.alloc( nR ) , dSame.alloc( nR ) , err.alloc( 1 );
+	err.zero( st );
+	KFaceRecords< K ><<< Grid( nR ) , 256 , 0 , st >>>( dS , (unsigned)nR , vb , key.p , id.p , even.p );
+	size_t bytes = 0;
+	HIPCHECK( hipcub::DeviceRadixSort::SortPairs( nullptr , bytes , key.p , skey.p , id.p , sid.p , (int)nR , 0 , bits , st ) );
+	tmp.ensure( bytes );
+	HIPCHECK( hipcub::DeviceRadixSort::SortPairs( tmp.p , bytes , key.p , skey.p , id.p , sid.p , (int)nR , 0 , bits , st ) );
+	KFaceLink< K ><<< Grid( nR ) , 256 , 0 , st >>>( skey.p , sid.p , (unsigned)nR , even.p , dOther.p , dSame.p , err.p );
+	HIPCHECK( hipGetLastError() );
+	unsigned e = 0;
+	other.resize( nR ) , same.resize( nR );
+	err.download( &e , 1 , st );
+	dOther.download( other.data() , nR , st );
+	dSame.download( same.data() , nR , st );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	if( e ) throw std::string( "Both faces occupied" );
+}
+
+template< int K >
+static void Orient( std::vector< unsigned > &S , const std::vector< unsigned > &other , const std::vector< unsigned char > &same , std::vector< unsigned char > &flipped )
 {
 	const size_t nS = S.size()/( K+1 );
-	if( !nS ) return;
-	FaceLinks L = LinkFaces< K >( S );
-	std::vector< unsigned char > oriented( nS , 0 ) , flipped( nS , 0 );
-	std::vector< std::pair< size_t , bool > > stack;
+	std::vector< unsigned char > oriented( nS , 0 );
+	flipped.assign( nS , 0 );
+	std::vector< std::pair< unsigned , bool > > stack;
 	for( size_t idx=0 ; idx<nS ; idx++ )
 	{
 		if( oriented[idx] ) continue;
 		oriented[idx] = 1;
-		stack.push_back( { idx , true } );
+		stack.push_back( { (unsigned)idx , true } );
 		while( !stack.empty() )
 		{
-			const std::pair< size_t , bool > s = stack.back();
+			const std::pair< unsigned , bool > s = stack.back();
 			stack.pop_back();
 			for( int v=0 ; v<=K ; v++ )
 			{
 				// after a flip the faces opposite vertices 0 and 1 trade places
 				const int vo = flipped[ s.first ] && v<2 ? 1-v : v;
-				const size_t r = s.first*( K+1 )+vo;
-				const long long o = L.other[r];
-				if( o<0 || oriented[o] ) continue;
+				const size_t r = (size_t)s.first*( K+1 )+vo;
+				const unsigned o = other[r];
+				if( o==NONE32 || oriented[o] ) continue;
 				oriented[o] = 1;
-				if( s.second!=( L.same[r]!=0 ) ) stack.push_back( { (size_t)o , true } );
+				if( s.second!=( same[r]!=0 ) ) stack.push_back( { o , true } );
 				else
 				{
-					std::swap( S[ o*( K+1 ) ] , S[ o*( K+1 )+1 ] );
+					std::swap( S[ (size_t)o*( K+1 ) ] , S[ (size_t)o*( K+1 )+1 ] );
 					flipped[o] = 1;
-					stack.push_back( { (size_t)o , false } );
+					stack.push_back( { o , false } );
 				}
 			}
 		}
 	}
+}
+
+// ------------------------------------------------------------------ density trimming
+// The reference's density file (ExteriorPoissonRecon.cpp:272-277): the
+// voxel-to-world XForm, the TreeEstimator's tree written depth first
+// (RegularTreeNode::write, Misha/RegularTree.inl:614-640: the root, then every
+// brood followed by its children's subtrees), each node a raw
+// RegularTreeNode< Dim , DensityAndNoiseInfo< Dim > , unsigned short > whose
+// data starts with weightSum, then _targetSamplesPerCell.
+struct DensityTree
+{
+	int dim = 0 , maxDepth = 0;           // maxDepth: the TreeSplatter's _maxDepth (tree height - 1)
+	double target = 0;
+	std::vector< int > child;             // first child of every node (broods are contiguous), -1 for leaves
+	std::vector< double > w;              // weightSum
+};
+
+static DensityTree ReadDensity( const char *path , int dim )
+{
+	DensityTree T;
+	T.dim = dim;
+	Reader r{ fopen( path , "rb" ) , path };
+	if( !r.fp ) throw std::string( "cannot open " ) + path;
+	struct Closer { FILE *f ; ~Closer(){ fclose( f ); } } closer{ r.fp };
+	std::vector< unsigned char > skip( 8 + ( dim+1 )*( dim+1 )*8 );
+	r.read( skip.data() , skip.size() );
+	const size_t head = ( ( 2*( dim+1 ) + 7 )/8 )*8 , rec = head + 16 + 8*( 1 + dim + 1 + dim*dim );
+	std::vector< unsigned char > buf( rec );
+	int height = 0;
+	auto Record = [&]( int depth )
+	{
+		r.read( buf.data() , rec );
+		unsigned long long ch;
+		double w;
+		memcpy( &ch , buf.data() + head + 8 , 8 );
+		memcpy( &w , buf.data() + head + 16 , 8 );
+		T.child.push_back( -1 ) , T.w.push_back( w );
+		height = std::max( height , depth );
+		return ch!=0;
+	};
+	struct Walk
+	{
+		int dim;
+		DensityTree &T;
+		decltype( Record ) &rec;
+		void children( int node , bool has , int depth )
+		{
+			if( !has ) return;
+			if( depth>30 ) throw std::string( "density tree too deep" );
+			const int first = (int)T.child.size();
+			T.child[node] = first;
+			bool c[16];
+			for( int i=0 ; i<( 1<<dim ) ; i++ ) c[i] = rec( depth+1 );
+			for( int i=0 ; i<( 1<<dim ) ; i++ ) children( first+i , c[i] , depth+1 );
+		}
+	} walk{ dim , T , Record };
+	walk.children( 0 , Record( 0 ) , 0 );
+	T.target = r.get< double >();
+	r.end();
+	T.maxDepth = height - 1;
+	if( T.child[0]<0 ) throw std::string( "density tree without a space root" );
+	return T;
+}
+
+struct DensityQuery { int kind , d ; double a , b; };   // kind 0: extremal, 1: interior, 2: interior with no samples at d2
+
+// TreeEstimator::_samplesPerCell at depth t (GridSamples.inl:783-815) over the
+// 2^Dim window of the ancestor at t (left radius 0, right radius 1), with the
+// linear B-spline weights of the clamped floor cell (the reference's weights
+// and window need not align on a cell face; that is kept).
+template< int Dim >
+__device__ double SamplesPerCellAt( const int *win , int t , const double p[Dim] , const double *w )
+{
+	const int res = 1<<( t-1 );
+	double val[Dim][2];
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		const double x = p[d]*res;
+		const int e = (int)floor( x );
+		const double f = x - e;
+		val[d][0] = 1.*( 1.-f )*1. , val[d][1] = 1.*( f*1. );
+	}
+	double sum = 0;
+	for( int i=0 ; i<( 1<<Dim ) ; i++ )                 // Range order: coordinate 0 slowest
+	{
+		const int n = win[i];
+		if( n<0 ) continue;
+		double scale = 1.;
+		for( int d=0 ; d<Dim ; d++ ) scale = scale * val[d][ ( i>>( Dim-1-d ) ) & 1 ];
+		sum = sum + w[n]*scale;
+	}
+	return sum / (double)( 1<<( ( t-1 )*( Dim-2 ) ) );
+}
+
+// per level-set vertex: getLeafNode from the space root, the neighbour windows
+// of its ancestors (children of the parent's window), and the branch of
+// TreeEstimator::_depth (GridSamples.inl:863-896) with the samples-per-cell
+// values it needs; the logarithms are taken on the host.
+template< int Dim >
+__global__ void __launch_bounds__( 128 ) KDensityQuery( const double *X , size_t nV , int D , const int *child , const double *w , double target , DensityQuery *out )
+{
+	const size_t v = (size_t)blockIdx.x*blockDim.x + threadIdx.x;
+	if( v>=nV ) return;
+	constexpr int MAXT = 18 , NW = 1<<Dim;
+	const double invRes = 1.0/(double)( 1<<D );
+	double p[Dim];
+	for( int d=0 ; d<Dim ; d++ ) p[d] = X[ v*Dim+d ]*invRes;
+	int win[MAXT][NW];
+	int off[Dim];
+	for( int d=0 ; d<Dim ; d++ ) off[d] = 0;
+	// depth 1: the space root (child 0 of the root) and its siblings
+	for( int i=0 ; i<NW ; i++ )
+	{
+		int c = 0;
+		for( int d=0 ; d<Dim ; d++ ) c |= ( ( i>>( Dim-1-d ) ) & 1 ) << d;
+		win[1][i] = child[0] + c;
+	}
+	int node = child[0] , t = 1;
+	double center[Dim] , width = 1.;
+	for( int d=0 ; d<Dim ; d++ ) center[d] = 0.5;
+	while( child[node]>=0 && t+1<MAXT )
+	{
+		int c = 0;
+		for( int d=0 ; d<Dim ; d++ ) if( p[d]>center[d] ) c |= 1<<d;
+		node = child[node] + c;
+		width /= 2;
+		for( int d=0 ; d<Dim ; d++ ) center[d] += ( c&( 1<<d ) ) ? width/2 : -width/2;
+		int noff[Dim];
+		for( int d=0 ; d<Dim ; d++ ) noff[d] = 2*off[d] + ( ( c>>d ) & 1 );
+		for( int i=0 ; i<NW ; i++ )
+		{
+			int J = 0 , ci = 0;
+			for( int d=0 ; d<Dim ; d++ )
+			{
+				const int o = noff[d] + ( ( i>>( Dim-1-d ) ) & 1 );
+				J = 2*J + ( ( o>>1 ) - off[d] );
+				ci |= ( o&1 ) << d;
+			}
+			const int par = win[t][J];
+			win[t+1][i] = par>=0 && child[par]>=0 ? child[par] + ci : -1;
+		}
+		for( int d=0 ; d<Dim ; d++ ) off[d] = noff[d];
+		t++;
+	}
+	const int maxD = t;
+	int d1 = maxD , d2 = maxD+1;
+	while( d2>1 && SamplesPerCellAt< Dim >( win[d1] , d1 , p , w )<target ) d1-- , d2--;
+	DensityQuery q;
+	if( d2==1 || d2==maxD+1 )
+	{
+		const int d = d2==1 ? 1 : maxD;
+		q = { 0 , d , SamplesPerCellAt< Dim >( win[d] , d , p , w ) , 0. };
+	}
+	else
+	{
+		const double s1 = SamplesPerCellAt< Dim >( win[d1] , d1 , p , w ) , s2 = SamplesPerCellAt< Dim >( win[d2] , d2 , p , w );
+		q = { s2!=0 ? 1 : 2 , d1 , s1 , s2 };
+	}
+	out[v] = q;
+}
+
+// TreeEstimator::samplesPerCell (GridSamples.inl:845-850) from a query
+static double SamplesPerCell( const DensityTree &T , const DensityQuery &q , int dim )
+{
+	double depth;
+	if( q.kind==0 ) depth = q.d + log( q.a/T.target ) / log( 2. );
+	else if( q.kind==2 ) depth = q.d;
+	else depth = q.d + log( T.target/q.a ) / log( q.b/q.a );
+	depth -= 1.;
+	return T.target * pow( 0.5 , (double)( dim-2 ) * ( T.maxDepth - depth ) );
+}
+
+// MarchingSimplices::FaceAdjacentConnectedComponents (MarchingSimplices.h:288-365)
+// over the oriented simplices: seeds from the back of the unprocessed list,
+// a stack, faces in the order of each simplex's current vertices; after a
+// component, processed entries leave the list by swap-with-last in
+// decreasing position (what the reference's backward scan does).
+template< int K >
+static std::vector< std::vector< unsigned > > Components( size_t nS , const std::vector< unsigned > &other , const std::vector< unsigned char > &flipped )
+{
+	std::vector< std::vector< unsigned > > comps;
+	std::vector< unsigned char > processed( nS , 0 );
+	std::vector< unsigned > un( nS ) , pos( nS );
+	for( size_t i=0 ; i<nS ; i++ ) un[i] = (unsigned)i , pos[i] = (unsigned)i;
+	std::vector< unsigned > Q;
+	while( !un.empty() )
+	{
+		std::vector< unsigned > comp;
+		Q.push_back( un.back() );
+		processed[ un.back() ] = 1;
+		pos[ un.back() ] = NONE32;
+		un.pop_back();
+		while( !Q.empty() )
+		{
+			const unsigned i = Q.back();
+			Q.pop_back();
+			comp.push_back( i );
+			for( int v=0 ; v<=K ; v++ )
+			{
+				const int vo = flipped[i] && v<2 ? 1-v : v;
+				const unsigned o = other[ (size_t)i*( K+1 )+vo ];
+				if( o!=NONE32 && !processed[o] ) Q.push_back( o ) , processed[o] = 1;
+			}
+		}
+		std::vector< unsigned > at;
+		for( unsigned i : comp ) if( pos[i]!=NONE32 ) at.push_back( pos[i] );
+		std::sort( at.begin() , at.end() , std::greater< unsigned >() );
+		for( unsigned p : at )
+		{
+			const unsigned last = un.back();
+			un[p] = last , pos[last] = p;
+			un.pop_back();
+		}
+		for( unsigned i : comp ) pos[i] = NONE32;
+		comps.push_back( std::move( comp ) );
+	}
+	return comps;
 }
 } // namespace iso
 } // namespace epr
@@ -574,6 +808,10 @@ struct epr_iso
 	hipStream_t st = nullptr;
 	epr::DevBuf< double > V;                            // finest regular values, pairs
 	epr::DevBuf< unsigned char > indexed;               // adapted: indexed finest elements by Morton index
+	std::unique_ptr< epr::iso::DensityTree > density;   // a density file, when read
+	epr::DevBuf< int > dChild;                          // its tree: first child per node
+	epr::DevBuf< double > dW;                           // its weightSum per node
+	std::vector< double > componentMax;                 // the last trimmed extraction's per-component maxima
 	std::string err;
 	double seconds[4] = { 0 , 0 , 0 , 0 };               // open, level-set nodes + triangulation, level set + orient, frames
 };
@@ -707,8 +945,41 @@ static double FrameScale( const epr_iso &I )
 	return pow( scale , 1./I.dim );
 }
 
+// Extract.cpp:217-238: components of the oriented level set, the largest
+// samples-per-cell estimate over each component's vertices, and only the
+// components above the threshold kept, in component order
 template< int Dim >
-static epr_mesh *Extract( epr_iso &I , bool all )
+static void Trim( epr_iso &I , const ls::DevMesh &lvl , double trim , const std::vector< unsigned > &other , const std::vector< unsigned char > &flipped ,
+                  std::vector< unsigned > &S )
+{
+	constexpr int K = Dim-2;
+	const size_t nS = S.size()/( K+1 );
+	std::vector< double > spc( lvl.nV );
+	if( lvl.nV )
+	{
+		DevBuf< DensityQuery > q;
+		q.alloc( lvl.nV );
+		KDensityQuery< Dim ><<< (unsigned)( ( lvl.nV + 127 )/128 ) , 128 , 0 , I.st >>>( lvl.V.p , lvl.nV , I.depth , I.dChild.p , I.dW.p , I.density->target , q.p );
+		HIPCHECK( hipGetLastError() );
+		std::vector< DensityQuery > hq( lvl.nV );
+		q.download( hq.data() , hq.size() , I.st );
+		HIPCHECK( hipStreamSynchronize( I.st ) );
+		for( size_t v=0 ; v<lvl.nV ; v++ ) spc[v] = SamplesPerCell( *I.density , hq[v] , Dim );
+	}
+	std::vector< std::vector< unsigned > > comps = Components< K >( nS , other , flipped );
+	std::vector< unsigned > kept;
+	for( const std::vector< unsigned > &c : comps )
+	{
+		double mx = 0;
+		for( unsigned s : c ) for( int v=0 ; v<=K ; v++ ) mx = std::max< double >( mx , spc[ S[ (size_t)s*( K+1 )+v ] ] );
+		I.componentMax.push_back( mx );
+		if( mx>trim ) for( unsigned s : c ) for( int v=0 ; v<=K ; v++ ) kept.push_back( S[ (size_t)s*( K+1 )+v ] );
+	}
+	S.swap( kept );
+}
+
+template< int Dim >
+static epr_mesh *Extract( epr_iso &I , bool all , double trim )
 {
 	hipStream_t st = I.st;
 	Clock::time_point t0 = Clock::now();
@@ -727,8 +998,12 @@ static epr_mesh *Extract( epr_iso &I , bool all )
 	HIPCHECK( hipStreamSynchronize( st ) );
 	try
 	{
-		if( Dim==3 ) Orient< 1 >( m->s );
-		else         Orient< 2 >( m->s );
+		std::vector< unsigned > other;
+		std::vector< unsigned char > same , flipped;
+		if( lvl.nS ) LinkFaces< Dim-2 >( lvl.S.p , lvl.nS , lvl.nV , st , other , same );
+		Orient< Dim-2 >( m->s , other , same , flipped );
+		I.componentMax.clear();
+		if( I.density && trim>0 ) Trim< Dim >( I , lvl , trim , other , flipped , m->s );
 	}
 	catch( ... ){ delete m ; throw; }
 	I.seconds[2] = Since( t0 ) , t0 = Clock::now();
@@ -885,11 +1160,34 @@ int epr_iso_level_set_nodes( epr_iso *iso , int all , size_t cap , unsigned *cel
 	} );
 }
 
-int epr_iso_extract( epr_iso *iso , int all , epr_mesh **out )
+int epr_iso_extract( epr_iso *iso , int all , epr_mesh **out ){ return epr_iso_extract_trimmed( iso , all , 0. , out ); }
+
+int epr_iso_extract_trimmed( epr_iso *iso , int all , double trimDensity , epr_mesh **out )
 {
 	if( !out ) return EPR_ERR_ARG;
 	*out = nullptr;
-	return IsoGuard( iso , [&](){ *out = iso->dim==3 ? epr::iso::Extract< 3 >( *iso , all!=0 ) : epr::iso::Extract< 4 >( *iso , all!=0 ); } );
+	return IsoGuard( iso , [&](){ *out = iso->dim==3 ? epr::iso::Extract< 3 >( *iso , all!=0 , trimDensity ) : epr::iso::Extract< 4 >( *iso , all!=0 , trimDensity ); } );
+}
+
+int epr_iso_read_density( epr_iso *iso , const char *densityPath )
+{
+	if( !densityPath ) return EPR_ERR_ARG;
+	return IsoGuard( iso , [&]()
+	{
+		std::unique_ptr< epr::iso::DensityTree > T( new epr::iso::DensityTree( epr::iso::ReadDensity( densityPath , iso->dim ) ) );
+		iso->dChild.upload( T->child.data() , T->child.size() , iso->st );
+		iso->dW.upload( T->w.data() , T->w.size() , iso->st );
+		HIPCHECK( hipStreamSynchronize( iso->st ) );
+		iso->density = std::move( T );
+	} );
+}
+
+int epr_iso_component_densities( const epr_iso *iso , size_t cap , double *out , size_t *count )
+{
+	if( !iso ) return EPR_ERR_ARG;
+	if( count ) *count = iso->componentMax.size();
+	if( out ) for( size_t i=0 ; i<std::min( cap , iso->componentMax.size() ) ; i++ ) out[i] = iso->componentMax[i];
+	return EPR_OK;
 }
 
 int epr_iso_level_set( epr_iso *iso , int component , epr_mesh **out )

@@ -51,11 +51,25 @@ class IsoTree:
                                                    out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint)), ctypes.byref(n)))
         return out
 
-    def extract(self, all=True):
-        """The oriented joint zero set: (vertices (nV, 3 dim): world position, n_0, n_1; simplices (nS, dim-1))."""
+    def read_density(self, path):
+        """TreeEstimator::read of a density file (ExteriorPoissonRecon's <out>.density.tree)."""
+        self._check(load().epr_iso_read_density(self._h, str(path).encode()))
+
+    def extract(self, all=True, trim_density=0.0):
+        """The oriented joint zero set: (vertices (nV, 3 dim): world position, n_0, n_1; simplices (nS, dim-1)).
+        With a density read and trim_density > 0, only the face-connected components whose
+        largest samples-per-cell estimate exceeds trim_density (Extract.cpp:217-238)."""
         h = ctypes.c_void_p()
-        self._check(load().epr_iso_extract(self._h, int(all), ctypes.byref(h)))
+        self._check(load().epr_iso_extract_trimmed(self._h, int(all), float(trim_density), ctypes.byref(h)))
         return _mesh(h)
+
+    def component_densities(self):
+        """Per component of the last trimmed extraction: the largest samples-per-cell estimate."""
+        n = ctypes.c_size_t()
+        load().epr_iso_component_densities(self._h, 0, None, ctypes.byref(n))
+        out = np.zeros(n.value)
+        load().epr_iso_component_densities(self._h, n.value, out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n))
+        return out
 
     def level_set(self, component):
         """The codimension-1 zero set of one component: (world vertices (nV, dim), simplices (nS, dim))."""

@@ -242,6 +242,20 @@ int  epr_iso_level_set_nodes( epr_iso *iso , int all , size_t cap , unsigned *ce
  * alone, 0 when the --levelSets output is also requested (Extract.cpp:129, 161). */
 int  epr_iso_extract( epr_iso *iso , int all , epr_mesh **out );
 
+/* Replaces Extract.cpp:153-159 (TreeEstimator::read): a density file as the
+ * reference's ExteriorPoissonRecon writes it (<out>.density.tree). */
+int  epr_iso_read_density( epr_iso *iso , const char *densityPath );
+
+/* epr_iso_extract followed by Extract.cpp:217-238 when a density file has been
+ * read and trimDensity > 0: the face-adjacent components of the oriented set
+ * (MarchingSimplices::FaceAdjacentConnectedComponents) whose largest
+ * samples-per-cell estimate over their vertices exceeds trimDensity are kept,
+ * component by component in the reference's traversal order. */
+int  epr_iso_extract_trimmed( epr_iso *iso , int all , double trimDensity , epr_mesh **out );
+/* The largest samples-per-cell estimate of every component of the last trimmed
+ * extraction, in component order (the values Extract --verbose prints). */
+int  epr_iso_component_densities( const epr_iso *iso , size_t cap , double *out , size_t *count );
+
 /* Replaces Extract.cpp:296-310: the codimension-1 zero set of one component
  * (0 or 1) over the all = 0 triangulation, dim world coordinates per vertex,
  * dim vertices per simplex. */

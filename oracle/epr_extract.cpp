@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <bits/hash_bytes.h>
+#include <functional>
 #include <map>
 #include <set>
 #include <string>
@@ -384,6 +385,187 @@ bool Gradient( const Iso &iso , const std::set< Idx > &valid , const double *pIn
 	return true;
 }
 
+// ---- density trimming (Extract.cpp:153-159, 217-238) ----
+// TreeEstimator read (GridSamples.inl:700-738), RegularTreeNode::read
+// (RegularTree.inl:643-662); nodes found by (depth, offset).
+struct Density
+{
+	int dim = 0 , maxDepth = 0;
+	double target = 0;
+	std::map< std::pair< int , Idx > , double > w;           // weightSum of every node
+	std::map< std::pair< int , Idx > , bool > hasChildren;
+};
+
+bool ReadDensity( const char *path , int dim , Density &T )
+{
+	FILE *fp = fopen( path , "rb" );
+	if( !fp ){ g_err = "cannot open density" ; return false; }
+	bool ok = true;
+	auto Read = [&]( void *p , size_t n ){ if( ok && fread( p , 1 , n , fp )!=n ) ok = false; };
+	std::vector< unsigned char > skip( 8 + 8*( dim+1 )*( dim+1 ) );
+	Read( skip.data() , skip.size() );
+	const size_t head = ( ( 2*( dim+1 ) + 7 )/8 )*8 , rec = head + 16 + 8*( 2 + dim + dim*dim );
+	std::vector< unsigned char > b( rec );
+	T.dim = dim;
+	int height = 0;
+	auto Node = [&]( void )
+	{
+		Read( b.data() , rec );
+		unsigned short h[5] = { 0 , 0 , 0 , 0 , 0 };
+		memcpy( h , b.data() , 2*( dim+1 ) );
+		unsigned long long ch;
+		double w;
+		memcpy( &ch , b.data()+head+8 , 8 );
+		memcpy( &w , b.data()+head+16 , 8 );
+		Idx o{};
+		for( int k=0 ; k<dim ; k++ ) o[k] = h[1+k];
+		T.w[ { (int)h[0] , o } ] = w;
+		T.hasChildren[ { (int)h[0] , o } ] = ch!=0;
+		height = std::max( height , (int)h[0] );
+		return ch!=0;
+	};
+	std::function< void ( bool ) > Read_ = [&]( bool has )
+	{
+		if( !has || !ok ) return;
+		std::vector< bool > c( 1u<<dim );
+		for( unsigned i=0 ; i<( 1u<<dim ) ; i++ ) c[i] = Node();
+		for( unsigned i=0 ; i<( 1u<<dim ) ; i++ ) Read_( c[i] );
+	};
+	Read_( Node() );
+	Read( &T.target , 8 );
+	fclose( fp );
+	T.maxDepth = height - 1;
+	if( !ok ) g_err = "truncated density";
+	return ok;
+}
+
+double SamplesPerCell( const Density &T , const double *pIn )
+{
+	const int dim = T.dim;
+	double p[4];
+	for( int d=0 ; d<dim ; d++ ) p[d] = pIn[d];
+	// getLeafNode from the space root (depth 1, offset 0)
+	std::vector< Idx > anc( 2 );                          // anc[t]: ancestor offset at depth t
+	anc[1] = Idx{};
+	{
+		double center[4] , width = 1.;
+		for( int d=0 ; d<dim ; d++ ) center[d] = 0.5;
+		int t = 1;
+		while( true )
+		{
+			auto it = T.hasChildren.find( { t , anc[t] } );
+			if( it==T.hasChildren.end() || !it->second ) break;
+			int c = 0;
+			for( int d=0 ; d<dim ; d++ ) if( p[d]>center[d] ) c |= 1<<d;
+			Idx o = anc[t];
+			for( int d=0 ; d<dim ; d++ ) o[d] = 2*o[d] + ( ( c>>d ) & 1 );
+			anc.push_back( o );
+			width /= 2;
+			for( int d=0 ; d<dim ; d++ ) if( c&( 1<<d ) ) center[d] += width/2 ; else center[d] -= width/2;
+			t++;
+		}
+	}
+	const int leafDepth = (int)anc.size() - 1;
+	auto SPC = [&]( int depth )
+	{
+		const unsigned res = 1u<<( depth-1 );
+		double values[4][2];
+		for( int d=0 ; d<dim ; d++ )
+		{
+			const int e = (int)floor( p[d]*res );
+			const double x = p[d]*res - e;
+			values[d][1] = 1.;
+			values[d][0] = values[d][1] * ( 1.-x ) * 1.;
+			values[d][1] *= x * 1.;
+		}
+		double ws = 0;
+		for( int i=0 ; i<( 1<<dim ) ; i++ )
+		{
+			Idx o = anc[depth];
+			int f[4];
+			for( int d=0 ; d<dim ; d++ ) f[d] = ( i>>( dim-1-d ) ) & 1 , o[d] += f[d];
+			auto it = T.w.find( { depth , o } );
+			if( it==T.w.end() ) continue;
+			double scale = 1.;
+			for( int d=0 ; d<dim ; d++ ) scale *= values[d][ f[d] ];
+			ws += it->second * scale;
+		}
+		return ws / ( 1<<( ( depth-1 )*( dim-2 ) ) );
+	};
+	// TreeEstimator::_depth with the leaf's depth
+	double dep;
+	{
+		const unsigned maxD = (unsigned)leafDepth;
+		unsigned d1 = maxD , d2 = maxD+1;
+		while( d2>1 && SPC( d1 )<T.target ) d1-- , d2--;
+		if( d2==1 || d2==maxD+1 )
+		{
+			unsigned d = d2==1 ? 1 : maxD;
+			dep = d + log( SPC( d )/T.target ) / log( 2. );
+		}
+		else
+		{
+			const double s1 = SPC( d1 ) , s2 = SPC( d2 );
+			dep = !s2 ? d1 : d1 + log( T.target/s1 ) / log( s2/s1 );
+		}
+	}
+	dep -= 1.;
+	return T.target * pow( 0.5 , (double)( dim-2 ) * ( T.maxDepth - dep ) );
+}
+
+// MarchingSimplices::FaceAdjacentConnectedComponents, literally
+std::vector< std::vector< size_t > > Components( int K , const std::vector< unsigned > &S )
+{
+	const size_t n = S.size()/( K+1 );
+	auto Face = [&]( size_t s , int v )
+	{
+		const unsigned *si = &S[ s*( K+1 ) ];
+		std::vector< unsigned > f;
+		if( K==1 ) f = { si[ v==0 ? 1 : 0 ] };
+		else for( int d=0 ; d<K ; d++ ) f.push_back( si[ ( v+d+1 )%( K+1 ) ] );
+		std::sort( f.begin() , f.end() );
+		return f;
+	};
+	std::map< std::vector< unsigned > , std::pair< long long , long long > > fData;
+	for( size_t i=0 ; i<n ; i++ ) for( int v=0 ; v<=K ; v++ )
+	{
+		auto it = fData.find( Face( i , v ) );
+		if( it==fData.end() ) fData[ Face( i , v ) ] = { (long long)i , -1 };
+		else if( it->second.second==-1 ) it->second.second = (long long)i;
+		else throw std::string( "Face occuppied" );
+	}
+	std::vector< std::vector< size_t > > components;
+	std::vector< bool > processed( n , false );
+	std::vector< size_t > un( n );
+	for( size_t i=0 ; i<n ; i++ ) un[i] = i;
+	while( un.size() )
+	{
+		std::vector< size_t > component , Q;
+		Q.push_back( un.back() );
+		processed[ un.back() ] = true;
+		un.pop_back();
+		while( Q.size() )
+		{
+			const size_t i = Q.back();
+			Q.pop_back();
+			component.push_back( i );
+			for( int v=0 ; v<=K ; v++ )
+			{
+				const auto &fd = fData[ Face( i , v ) ];
+				const long long opposite = fd.first==(long long)i ? fd.second : fd.first;
+				if( opposite!=-1 && !processed[opposite] ) Q.push_back( (size_t)opposite ) , processed[opposite] = true;
+			}
+		}
+		components.push_back( component );
+		for( size_t i=un.size() ; i>0 ; i-- ) if( processed[ un[i-1] ] )
+		{
+			un[i-1] = un.back();
+			un.pop_back();
+		}
+	}
+	return components;
+}
+
 template< typename T > T *Dup( const std::vector< T > &v )
 {
 	T *p = (T *)malloc( std::max< size_t >( 1 , v.size() )*sizeof(T) );
@@ -423,9 +605,26 @@ const char *orc_extract_error( void ){ return g_err.c_str(); }
 
 unsigned long long orc_indexer_hash( int dim , int adapted ){ return IndexerHash( dim , adapted!=0 ); }
 
+int orc_extract_trimmed( const char *treePath , int all , const char *densityPath , double trim ,
+                         size_t *nCells , unsigned **cells , double **values /* (2^D+1)^dim x 2 */ ,
+                         size_t *nV , double **V /* nV x 3 dim */ , size_t *nS , unsigned **S /* nS x (dim-1) */ ,
+                         size_t *nComp , double **compMax );
+
 int orc_extract( const char *treePath , int all ,
                  size_t *nCells , unsigned **cells , double **values /* (2^D+1)^dim x 2 */ ,
                  size_t *nV , double **V /* nV x 3 dim */ , size_t *nS , unsigned **S /* nS x (dim-1) */ )
+{
+	size_t nc = 0;
+	double *cm = nullptr;
+	const int rc = orc_extract_trimmed( treePath , all , nullptr , 0. , nCells , cells , values , nV , V , nS , S , &nc , &cm );
+	free( cm );
+	return rc;
+}
+
+int orc_extract_trimmed( const char *treePath , int all , const char *densityPath , double trim ,
+                         size_t *nCells , unsigned **cells , double **values ,
+                         size_t *nV , double **V , size_t *nS , unsigned **S ,
+                         size_t *nComp , double **compMax )
 {
 	try
 	{
@@ -455,6 +654,30 @@ int orc_extract( const char *treePath , int all ,
 		std::vector< unsigned > simp( LS , LS + nLS*( dim-1 ) );
 		free( LS );
 		Orient( dim-2 , simp );
+		std::vector< double > cmax;
+		if( densityPath && trim>0 )
+		{
+			Density T;
+			if( !ReadDensity( densityPath , dim , T ) ){ free( LV ) ; return -4; }
+			const int K = dim-2;
+			const double res = (double)( 1u<<t.D );
+			std::vector< std::vector< size_t > > comps = Components( K , simp );
+			std::vector< unsigned > temp;
+			for( const std::vector< size_t > &c : comps )
+			{
+				double maxDensity = 0;
+				for( size_t j : c ) for( int v=0 ; v<=K ; v++ )
+				{
+					double p[4];
+					for( int d=0 ; d<dim ; d++ ) p[d] = LV[ simp[ j*( K+1 )+v ]*dim+d ] / res;
+					maxDensity = std::max< double >( maxDensity , SamplesPerCell( T , p ) );
+				}
+				cmax.push_back( maxDensity );
+				if( maxDensity>trim ) for( size_t j : c ) for( int v=0 ; v<=K ; v++ ) temp.push_back( simp[ j*( K+1 )+v ] );
+			}
+			simp.swap( temp );
+		}
+		*nComp = cmax.size() , *compMax = Dup( cmax );
 		double scale = 1.;
 		for( int d=0 ; d<dim ; d++ ) scale *= t.xf[ d*( dim+1 )+d ];
 		scale = pow( scale , 1./dim );
@@ -470,7 +693,7 @@ int orc_extract( const char *treePath , int all ,
 		}
 		free( LV );
 		*nV = nLV , *V = Dup( out );
-		*nS = nLS , *S = Dup( simp );
+		*nS = simp.size()/( dim-1 ) , *S = Dup( simp );
 		return 0;
 	}
 	catch( const std::string &e ){ g_err = e ; return -3; }

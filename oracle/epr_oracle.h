@@ -117,6 +117,12 @@ void   orc_free( void *p );
  * are malloc'ed (orc_free).  Nonzero on error (orc_extract_error). */
 int    orc_extract( const char *treePath , int all , size_t *nCells , unsigned **cells , double **values ,
                     size_t *nV , double **V , size_t *nS , unsigned **S );
+/* orc_extract with Extract.cpp:153-159, 217-238: with a density file and trim > 0,
+ * the face-adjacent components whose largest samples-per-cell estimate exceeds
+ * trim, in component order; compMax receives every component's maximum. */
+int    orc_extract_trimmed( const char *treePath , int all , const char *densityPath , double trim ,
+                            size_t *nCells , unsigned **cells , double **values ,
+                            size_t *nV , double **V , size_t *nS , unsigned **S , size_t *nComp , double **compMax );
 /* Extract.cpp:296-310: the codimension-1 zero set of one component, world positions */
 int    orc_extract_level_set( const char *treePath , int component , size_t *nV , double **V , size_t *nS , unsigned **S );
 const char *orc_extract_error( void );

@@ -72,6 +72,10 @@ def lib():
                                   ctypes.POINTER(up)]
         L.orc_extract_level_set.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(sz), ctypes.POINTER(dp),
                                             ctypes.POINTER(sz), ctypes.POINTER(up)]
+        L.orc_extract_trimmed.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_char_p, ctypes.c_double,
+                                          ctypes.POINTER(sz), ctypes.POINTER(up), ctypes.POINTER(dp), ctypes.POINTER(sz),
+                                          ctypes.POINTER(dp), ctypes.POINTER(sz), ctypes.POINTER(up), ctypes.POINTER(sz),
+                                          ctypes.POINTER(dp)]
         L.orc_extract_error.restype = ctypes.c_char_p
         L.orc_indexer_hash.restype = ctypes.c_ulonglong
         L.orc_indexer_hash.argtypes = [ctypes.c_int, ctypes.c_int]
@@ -293,24 +297,29 @@ def tree_header(path):
     return dim, n - 1
 
 
-def extract(tree, all=True):
+def extract(tree, all=True, density=None, trim=0.0):
     """The oracle's Extract.cpp:109-259 on a .tree: dict with cells (level-set
     nodes, tree order), values ((2^D+1)^dim x 2 finest values), vertices (nV x
-    3 dim: world position, n_0, n_1) and simplices (nS x (dim-1), oriented)."""
+    3 dim: world position, n_0, n_1), simplices (nS x (dim-1), oriented) and,
+    with a density file and trim > 0, the trimmed simplices and every
+    component's largest samples-per-cell estimate (component_max)."""
     dim, D = tree_header(tree)
     sz, up, dp = ctypes.c_size_t, ctypes.POINTER(ctypes.c_uint), ctypes.POINTER(ctypes.c_double)
-    nC, nV, nS = sz(), sz(), sz()
+    nC, nV, nS, nK = sz(), sz(), sz(), sz()
     C, S = up(), up()
-    val, V = dp(), dp()
-    rc = lib().orc_extract(tree.encode(), int(all), ctypes.byref(nC), ctypes.byref(C), ctypes.byref(val),
-                           ctypes.byref(nV), ctypes.byref(V), ctypes.byref(nS), ctypes.byref(S))
+    val, V, K = dp(), dp(), dp()
+    rc = lib().orc_extract_trimmed(tree.encode(), int(all), density.encode() if density else None, float(trim),
+                                   ctypes.byref(nC), ctypes.byref(C), ctypes.byref(val),
+                                   ctypes.byref(nV), ctypes.byref(V), ctypes.byref(nS), ctypes.byref(S),
+                                   ctypes.byref(nK), ctypes.byref(K))
     if rc:
         raise ValueError(lib().orc_extract_error().decode())
     nF = ((1 << D) + 1) ** dim
     return {"cells": _take(C, nC.value * dim, np.uint32).reshape(-1, dim),
             "values": _take(val, 2 * nF, np.float64).reshape(-1, 2),
             "vertices": _take(V, nV.value * 3 * dim, np.float64).reshape(-1, 3 * dim),
-            "simplices": _take(S, nS.value * (dim - 1), np.uint32).reshape(-1, dim - 1)}
+            "simplices": _take(S, nS.value * (dim - 1), np.uint32).reshape(-1, dim - 1),
+            "component_max": _take(K, nK.value, np.float64)}
 
 
 def extract_level_set(tree, component):

@@ -101,9 +101,16 @@ EXTRACT_FIXTURES = sorted(f[:-4] for f in os.listdir(GOLDEN) if f.startswith("ex
 
 
 def fixture_tree(name, tmp_path):
-    """Writes the .tree bytes of an Extract fixture to tmp_path and returns (path, fixture)."""
+    """Writes the .tree bytes of an Extract fixture to tmp_path and returns (path, fixture);
+    a density file, when the fixture has one, goes to tmp_path/<name>.density.tree."""
     g = load_golden(name)
     p = str(tmp_path / (name + ".tree"))
     with open(p, "wb") as f:
         f.write(g["tree"].tobytes())
+    if "density" in g:
+        with open(str(tmp_path / (name + ".density.tree")), "wb") as f:
+            f.write(g["density"].tobytes())
     return p, g
+
+
+TRIM_FIXTURES = [n for n in EXTRACT_FIXTURES if "density" in np.load(os.path.join(GOLDEN, n + ".npz")).files]

@@ -9,7 +9,14 @@ oracle/_ref/Extract runs on it three ways:
     --grid g                         the finest regular values of both functions
 
 A fixture holds the .tree bytes (the input), the SHA-256 and the element counts
-of every PLY the reference wrote, and (small grids) the --grid values.  Run here only:
+of every PLY the reference wrote, and (small grids) the --grid values.  Cases
+with a trim value also hold the density file the reference wrote beside the
+tree, reduced to what Extract reads from it (tree structure and weightSum;
+pointers become 0 / 1, the other statistics 0), and the reference's
+
+    --out t.ply --density <reduced file> --trimDensity <trim> --verbose
+
+mesh and printed per-component maxima.  Run here only:
 
     python tests/golden/make_extract_golden.py
 """
@@ -30,11 +37,11 @@ REF = os.path.join(ROOT, "oracle", "_ref")
 CASES = {
     # 3D torus knot, dense (--aRadius -1) and adapted (the default --aRadius 3)
     "extract_knot3_d4": dict(gen=lambda: samples.torus_knot(4096, seed=4), args=dict(depth=4, aRadius=-1)),
-    "extract_knot3_d4_adapted": dict(gen=lambda: samples.torus_knot(4096, seed=4), args=dict(depth=4)),
-    "extract_knot3_d5_adapted_r1": dict(gen=lambda: samples.torus_knot(1024, p=3, q=5, seed=3), args=dict(depth=5, aRadius=1)),
+    "extract_knot3_d4_adapted": dict(gen=lambda: samples.torus_knot(4096, seed=4), args=dict(depth=4), trim=2.0),
+    "extract_knot3_d5_adapted_r1": dict(gen=lambda: samples.torus_knot(1024, p=3, q=5, seed=3), args=dict(depth=5, aRadius=1), trim=3.0),
     # 4D Clifford torus, dense and adapted
     "extract_clifford4_d3": dict(gen=lambda: samples.clifford_torus(side=40, ang_deg=2.0, seed=4), args=dict(depth=3, aRadius=-1)),
-    "extract_clifford4_d3_adapted": dict(gen=lambda: samples.clifford_torus(side=40, ang_deg=2.0, seed=4), args=dict(depth=3)),
+    "extract_clifford4_d3_adapted": dict(gen=lambda: samples.clifford_torus(side=40, ang_deg=2.0, seed=4), args=dict(depth=3), trim=2.0),
 }
 
 
@@ -54,6 +61,25 @@ def read_grid(path):
         o = b.index(b"\n", o) + 1
         lines += 1
     return np.frombuffer(b, "<f8", offset=o).copy()
+
+
+def reduced_density(src, dst, dim):
+    """The density file with only the tree structure and weightSum kept (RegularTreeNode
+    records: u16 depth and offsets padded to 8 bytes, parent, children, then the
+    DensityAndNoiseInfo whose first double is weightSum)."""
+    b = bytearray(open(src, "rb").read())
+    head = ((2 * (dim + 1) + 7) // 8) * 8
+    rec = head + 16 + 8 * (2 + dim + dim * dim)
+    start = 8 + 8 * (dim + 1) ** 2
+    b[:start] = bytes(8) + bytes(b[8:start])                 # the XForm's vtable slot
+    n = (len(b) - start - 8) // rec
+    assert start + n * rec + 8 == len(b)
+    for i in range(n):
+        o = start + i * rec
+        ch = int.from_bytes(b[o + head + 8:o + head + 16], "little")
+        b[o + head:o + head + 16] = bytes(8) + (1 if ch else 0).to_bytes(8, "little")
+        b[o + head + 24:o + rec] = bytes(rec - head - 24)
+    open(dst, "wb").write(bytes(b))
 
 
 def run(name, case):
@@ -76,6 +102,17 @@ def run(name, case):
             h, nv, nf = ply_counts(j(f))
             out[key + "_sha256"] = np.array(h)
             out[key + "_counts"] = np.array([nv, nf])
+        if "trim" in case:
+            dim = pos.shape[1]
+            reduced_density(os.path.join(td, "o.density.tree"), j("d.tree"), dim)
+            r = subprocess.run([ext, "--in", tree, "--out", j("t.ply"), "--density", j("d.tree"), "--trimDensity",
+                                str(case["trim"]), "--verbose"], check=True, capture_output=True, text=True)
+            mx = [float(ln.split()[-1]) for ln in r.stdout.splitlines() if ln.startswith("Max Density")]
+            h, nv, nf = ply_counts(j("t.ply"))
+            out["density"] = np.frombuffer(open(j("d.tree"), "rb").read(), np.uint8)
+            out["trim"] = np.array(case["trim"])
+            out["trimmed_sha256"], out["trimmed_counts"] = np.array(h), np.array([nv, nf])
+            out["printed_component_max"] = np.array(mx)
         g0, g1 = read_grid(j("g.0.grid")), read_grid(j("g.1.grid"))
         if len(g0) <= 20000:                                 # small grids only: the values do not compress
             out["grid0"], out["grid1"] = g0, g1

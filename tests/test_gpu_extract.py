@@ -4,15 +4,17 @@ runs (tests/golden/extract_*.npz) and against the oracle on larger trees.
 Bar: every PLY byte-identical to the reference's -- the oriented joint zero
 set with frames (all = true and all = false) and both codimension-1 sets --
 and the finest values, level-set nodes and meshes identical bit for bit to the
-oracle's.  The --grid values agree with the reference's to 1e-12 relative (its
-prolongation sums with parallel atomics) and their headers byte for byte."""
+oracle's; with a density file, the trimmed mesh byte-identical to the
+reference's and the per-component maxima identical to the oracle's.  The
+--grid values agree with the reference's to 1e-12 relative (its prolongation
+sums with parallel atomics) and their headers byte for byte."""
 import os
 import subprocess
 
 import numpy as np
 import pytest
 
-from conftest import EXTRACT_FIXTURES, ROOT, fixture_tree, level_sets_ply, mesh_ply, read_grid_values, sha256
+from conftest import EXTRACT_FIXTURES, ROOT, TRIM_FIXTURES, fixture_tree, level_sets_ply, mesh_ply, read_grid_values, sha256
 
 pytestmark = pytest.mark.gpu
 EXTRACT = os.path.join(ROOT, "exteriorpoissonrecon_amd", "bin", "Extract")
@@ -49,6 +51,31 @@ def test_gpu_extract_matches_reference(name, tmp_path, oracle_lib):
             head = open(str(tmp_path / f"g.{c}.grid"), "rb").read()[:-8 * len(mine)]
             assert head == g["grid_header"].tobytes()
     iso.close()
+
+
+@pytest.mark.parametrize("name", TRIM_FIXTURES)
+def test_gpu_density_trimming_matches_reference(name, tmp_path, oracle_lib):
+    from exteriorpoissonrecon_amd.extract import IsoTree, mesh_ply_bytes
+    print("case", name, flush=True)
+    tree, g = fixture_tree(name, tmp_path)
+    dens = str(tmp_path / (name + ".density.tree"))
+    iso = IsoTree(tree)
+    iso.read_density(dens)
+    V, S = iso.extract(True, float(g["trim"]))
+    assert [len(V), len(S)] == g["trimmed_counts"].tolist()
+    assert sha256(mesh_ply_bytes(iso.dim, V, S)) == str(g["trimmed_sha256"])
+    o = oracle_lib.extract(tree, True, dens, float(g["trim"]))
+    assert np.array_equal(iso.component_densities(), o["component_max"])
+    # no trimming without a threshold, even with a density read
+    V0, S0 = iso.extract(True, 0.0)
+    assert sha256(mesh_ply_bytes(iso.dim, V0, S0)) == str(g["mesh_sha256"])
+    iso.close()
+    res = subprocess.run([EXTRACT, "--in", tree, "--out", str(tmp_path / "t.ply"), "--density", dens,
+                          "--trimDensity", str(float(g["trim"])), "--verbose"], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0 and "[ERROR]" not in res.stderr, res.stderr
+    assert sha256(open(str(tmp_path / "t.ply"), "rb").read()) == str(g["trimmed_sha256"])
+    printed = [float(ln.split()[-1]) for ln in res.stdout.splitlines() if ln.startswith("Max Density")]
+    assert printed == g["printed_component_max"].tolist()
 
 
 @pytest.mark.parametrize("name", [n for n in EXTRACT_FIXTURES if "knot3_d4" in n or "clifford4_d3" == n[8:]])

@@ -6,14 +6,16 @@ all = true (--out alone) and all = false (--out with --levelSets), and the two
 codimension-1 sets of --levelSets -- must be reproduced byte for byte from the
 same .tree: cumulative values, zero-crossing refinement, tree order,
 triangulation, level set, orientation, gradient frames and the voxel-to-world
-map all take part.  The --grid values agree to 1e-12 relative (the reference
+map all take part; with a density file and --trimDensity, the samples-per-cell
+estimates, the face-adjacent components and their traversal order too.  The
+--grid values agree to 1e-12 relative (the reference
 sums the prolongation with parallel atomics, so its last bits vary).  The
 orientation is also checked for consistency on its own: every face shared by
 two simplices appears with opposite orientations."""
 import numpy as np
 import pytest
 
-from conftest import EXTRACT_FIXTURES, fixture_tree, level_sets_ply, mesh_ply, sha256
+from conftest import EXTRACT_FIXTURES, TRIM_FIXTURES, fixture_tree, level_sets_ply, mesh_ply, sha256
 
 
 @pytest.mark.parametrize("name", EXTRACT_FIXTURES)
@@ -65,3 +67,16 @@ def test_oracle_orientation_is_consistent(name, tmp_path, oracle_lib):
     S = r["simplices"].copy()
     S[len(S) // 2, [0, 1]] = S[len(S) // 2, [1, 0]]
     assert not oriented(S)
+
+
+@pytest.mark.parametrize("name", TRIM_FIXTURES)
+def test_oracle_density_trimming_matches_reference(name, tmp_path, oracle_lib):
+    tree, g = fixture_tree(name, tmp_path)
+    dim = oracle_lib.tree_header(tree)[0]
+    r = oracle_lib.extract(tree, True, str(tmp_path / (name + ".density.tree")), float(g["trim"]))
+    assert [len(r["vertices"]), len(r["simplices"])] == g["trimmed_counts"].tolist()
+    assert sha256(mesh_ply(dim, r["vertices"], r["simplices"])) == str(g["trimmed_sha256"])
+    # the maxima the reference prints (6 significant digits)
+    printed = g["printed_component_max"]
+    assert len(r["component_max"]) == len(printed)
+    assert np.all(np.abs(r["component_max"] - printed) <= 5e-6 * np.abs(printed))
