@@ -21,7 +21,7 @@ EXPORTS = [
     "epr_mg_vcycle", "epr_mg_relax", "epr_mg_destroy", "epr_adapted_indexer_write",
     "epr_iso_open", "epr_iso_last_error", "epr_iso_info", "epr_iso_values", "epr_iso_level_set_nodes",
     "epr_iso_extract", "epr_iso_level_set", "epr_iso_write_grid", "epr_iso_seconds", "epr_iso_destroy",
-    "epr_indexer_hash", "epr_iso_read_density", "epr_iso_extract_trimmed", "epr_iso_component_densities",
+    "epr_indexer_hash", "epr_write_density", "epr_iso_read_density", "epr_iso_extract_trimmed", "epr_iso_component_densities",
 ]
 
 
@@ -73,6 +73,7 @@ def load():
     L.epr_get_array.argtypes = [vp, ctypes.c_char_p, dp, sz, ctypes.POINTER(sz)]
     L.epr_write_tree.argtypes = [vp, ctypes.c_char_p, ctypes.c_ulonglong]
     L.epr_write_grid.argtypes = [vp, ctypes.c_char_p]
+    L.epr_write_density.argtypes = [vp, ctypes.c_char_p]
     L.epr_time_gs_sweeps.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
     L.epr_time_gs_sweeps.restype = ctypes.c_int
     up = ctypes.POINTER(ctypes.c_uint)

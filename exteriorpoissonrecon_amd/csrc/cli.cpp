@@ -1,7 +1,7 @@
 // ExteriorPoissonRecon command line over libepr_hip.so (dense --aRadius -1 and adapted --aRadius 0..5 paths).
 //
 // Same flags, defaults and outputs as ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:52-279:
-// reads <in>.ply, writes <out>.tree (reference byte layout) and, with --grid,
+// reads <in>.ply, writes <out>.tree and <out>.density.tree (reference byte layouts) and, with --grid,
 // a .grid of the finest (phi1,phi2) values.  Errors print "[ERROR] ..." to
 // stderr and exit with status 0, as MK_ERROR_OUT does (Misha/Exceptions.h:101-108).
 #include <chrono>
@@ -102,7 +102,11 @@ int main( int argc , char *argv[] )
 	double E[2];
 	Check( epr_solve( ctx , E ) );
 	printf( "Error: %.17g -> %.17g = %.17g + %.17g\n" , 0. , E[0]+E[1] , E[0] , E[1] );
-	if( !out.empty() ) Check( epr_write_tree( ctx , ( out + ".tree" ).c_str() , epr_indexer_hash( s.dim , p.aRadius!=-1 ) ) );
+	if( !out.empty() )
+	{
+		Check( epr_write_tree( ctx , ( out + ".tree" ).c_str() , epr_indexer_hash( s.dim , p.aRadius!=-1 ) ) );
+		Check( epr_write_density( ctx , ( out + ".density.tree" ).c_str() ) );
+	}
 	if( !grid.empty() ) Check( epr_write_grid( ctx , grid.c_str() ) );
 	if( p.verbose ) printf( "%32s: %.3f (s)\n" , "Executed" , std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count() );
 	epr_destroy( ctx );

@@ -120,6 +120,7 @@ struct CtxBase
 	virtual size_t getArray( const std::string &name , double *out , size_t cap ) = 0;
 	virtual void writeTree( const char *path , unsigned long long hash ) = 0;
 	virtual void writeGrid( const char *path ) = 0;
+	virtual void writeDensity( const char *path ) = 0;
 	double seconds[2] = { 0 , 0 };
 };
 
@@ -391,6 +392,17 @@ struct Ctx : public CtxBase
 			for( int t=1 ; t<=D+1 ; t++ ){ if( !estData[t] ) estData[t].reset( new DevBuf< double >() ) ; estData[t]->ensure( sz[t] ) ; estData[t]->zero( stream ) ; T.data[t] = estData[t]->p; }
 			switch( K ){ case 0: launchEstSplat<0>( T ) ; break ; case 1: launchEstSplat<1>( T ) ; break ; case 2: launchEstSplat<2>( T ) ; break ; case 3: launchEstSplat<3>( T ) ; break ; default: launchEstSplat<4>( T ); }
 			HIPCHECK( hipGetLastError() );
+			// the estimator tree's shape (which nodes have children), for its file
+			{
+				std::vector< long long > kz = treeSizes( 1 , true );
+				TreeGrids< Dim > TK;
+				memset( &TK , 0 , sizeof(TK) );
+				TK.maxDepth = D;
+				estKids.resize( D+1 );
+				for( int t=1 ; t<=D ; t++ ){ if( !estKids[t] ) estKids[t].reset( new DevBuf< unsigned char >() ) ; estKids[t]->ensure( kz[t] ) ; estKids[t]->zero( stream ) ; TK.kids[t] = estKids[t]->p; }
+				if( D>=1 ) KTouch< Dim ><<< Blocks( (long long)ncell*D , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , TK , K );
+				HIPCHECK( hipGetLastError() );
+			}
 			dMeasure.ensure( n ) , dNoise.ensure( n ) , dDepth.ensure( n ) , dRW.ensure( n ) , dSkewScaled.ensure( n*CD );
 			KEstQuery< Dim ><<< Blocks( (long long)n , 128 ) , 128 , 0 , stream >>>( dPos.p , dSkew.p , (int)n , T , prm.nSigma , dMeasure.p , dNoise.p , dDepth.p , dRW.p , dSkewScaled.p );
 			HIPCHECK( hipGetLastError() );
@@ -869,6 +881,113 @@ struct Ctx : public CtxBase
 		for( long long i=0 ; i<N ; i++ ){ fwrite( &hx[i] , sizeof(double) , 1 , fp ) ; fwrite( &hy[i] , sizeof(double) , 1 , fp ); }
 		fclose( fp );
 	}
+
+	// <out>.density.tree (ExteriorPoissonRecon.cpp:272-277): the voxel-to-world
+	// XForm, then TreeEstimator::write( stream , false ) (GridSamples.inl:734-738):
+	// the tree depth first (Misha/RegularTree.inl:614-640: the root, then every
+	// brood followed by its children's subtrees) as raw
+	// RegularTreeNode< Dim , DensityAndNoiseInfo< Dim > , unsigned short > records
+	// (u16 depth and offsets padded to 8 bytes, parent and children pointers --
+	// written 0, and 1 for "has children" -- then weightSum, positionSum, the
+	// covariance matrix's vtable slot (0) and its Dim^2 entries), then
+	// _targetSamplesPerCell = 2^(Dim-CoDim).
+	void writeDensity( const char *path )
+	{
+		if( !built ) throw std::string( "epr_build has not run" );
+		// the nodes in file order: a node exists when its parent has children
+		std::vector< std::vector< unsigned char > > kids( D+1 );
+		for( int t=1 ; t<=D ; t++ )
+		{
+			kids[t].resize( estKids[t]->n );
+			estKids[t]->download( kids[t].data() , kids[t].size() , stream );
+		}
+		HIPCHECK( hipStreamSynchronize( stream ) );
+		struct Rec { unsigned char t ; unsigned short o[Dim] ; bool has; };
+		std::vector< Rec > seq;
+		auto Has = [&]( int t , const unsigned short *o )
+		{
+			if( t==0 ) return true;
+			if( t>D ) return false;
+			long long i = 0;
+			for( int d=0 ; d<Dim ; d++ ) i = i*( 1LL<<t ) + o[d];
+			return kids[t][i]!=0;
+		};
+		Rec root{};
+		root.has = true;
+		seq.push_back( root );
+		std::function< void ( const Rec & ) > Children = [&]( const Rec &r )
+		{
+			if( !r.has ) return;
+			const size_t first = seq.size();
+			for( int c=0 ; c<( 1<<Dim ) ; c++ )
+			{
+				Rec k;
+				k.t = (unsigned char)( r.t+1 );
+				for( int d=0 ; d<Dim ; d++ ) k.o[d] = (unsigned short)( 2*r.o[d] + ( ( c>>d ) & 1 ) );
+				k.has = Has( k.t , k.o );
+				seq.push_back( k );
+			}
+			for( int c=0 ; c<( 1<<Dim ) ; c++ ){ const Rec k = seq[ first+c ] ; Children( k ); }
+		};
+		Children( root );
+		// gather the statistics of every node from the dense corner grids, depth by depth
+		std::vector< std::vector< long long > > idx( D+2 );
+		for( const Rec &r : seq )
+		{
+			if( r.t<1 ) continue;
+			const long long rr = 1LL<<( r.t-1 );
+			long long i = 0;
+			for( int d=0 ; d<Dim && i>=0 ; d++ ) i = r.o[d]<=rr ? i*( rr+1 ) + r.o[d] : -1;
+			idx[ r.t ].push_back( i );
+		}
+		std::vector< std::vector< double > > rows( D+2 );
+		DevBuf< long long > dIdx;
+		DevBuf< double > dRows;
+		for( int t=1 ; t<=D+1 ; t++ )
+		{
+			const size_t m = idx[t].size();
+			if( !m ) continue;
+			dIdx.upload( idx[t].data() , m , stream );
+			dRows.ensure( m*NV );
+			KGatherRows<<< Blocks( (long long)m , 256 ) , 256 , 0 , stream >>>( estData[t]->p , dIdx.p , (long long)m , NV , dRows.p );
+			HIPCHECK( hipGetLastError() );
+			rows[t].resize( m*NV );
+			dRows.download( rows[t].data() , m*NV , stream );
+			HIPCHECK( hipStreamSynchronize( stream ) );
+		}
+		FILE *fp = fopen( path , "wb" );
+		if( !fp ) throw std::string( "cannot open " ) + path;
+		const unsigned long long zero = 0;
+		fwrite( &zero , sizeof(zero) , 1 , fp );
+		double v2w[Dim+1][Dim+1];
+		voxelToWorld( v2w );
+		fwrite( v2w , sizeof(double) , (Dim+1)*(Dim+1) , fp );
+		constexpr size_t head = ( ( 2*( Dim+1 ) + 7 )/8 )*8 , rec = head + 16 + 8*( 2 + Dim + Dim*Dim );
+		std::vector< size_t > cur( D+2 , 0 );
+		std::vector< unsigned char > buf( rec );
+		for( const Rec &r : seq )
+		{
+			memset( buf.data() , 0 , rec );
+			unsigned short h[Dim+1];
+			h[0] = r.t;
+			for( int d=0 ; d<Dim ; d++ ) h[1+d] = r.o[d];
+			memcpy( buf.data() , h , sizeof(h) );
+			const unsigned long long ch = r.has ? 1 : 0;
+			memcpy( buf.data() + head + 8 , &ch , 8 );
+			if( r.t>=1 )
+			{
+				const double *v = &rows[ r.t ][ ( cur[ r.t ]++ )*NV ];
+				double *o = (double *)( buf.data() + head + 16 );
+				o[0] = v[0];
+				for( int d=0 ; d<Dim ; d++ ) o[1+d] = v[1+d];
+				for( int k=0 ; k<Dim*Dim ; k++ ) o[2+Dim+k] = v[1+Dim+k];
+			}
+			fwrite( buf.data() , 1 , rec , fp );
+		}
+		const double target = std::pow( 2. , Dim-CoDim );
+		fwrite( &target , sizeof(double) , 1 , fp );
+		fclose( fp );
+	}
 };
 } // namespace epr
 
@@ -968,6 +1087,7 @@ int epr_get_array( epr_ctx *ctx , const char *name , double *out , size_t cap , 
 }
 int epr_write_tree( epr_ctx *ctx , const char *path , unsigned long long hash ){ IMPL( ctx->impl->writeTree( path , hash ) ); }
 int epr_write_grid( epr_ctx *ctx , const char *path ){ IMPL( ctx->impl->writeGrid( path ) ); }
+int epr_write_density( epr_ctx *ctx , const char *path ){ IMPL( ctx->impl->writeDensity( path ) ); }
 int epr_time_copy( size_t bytes , int reps , double *ms )
 {
 	if( !ms || reps<=0 || bytes<16 ) return EPR_ERR_ARG;

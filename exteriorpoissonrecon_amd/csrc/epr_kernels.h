@@ -621,6 +621,15 @@ __global__ void KTouch( const int *cellIdx , int ncell , TreeGrids< Dim > T , in
 	}
 }
 
+// rows of a row-major array by index (negative: a row of zeros)
+__global__ void __launch_bounds__( 256 ) KGatherRows( const double *src , const long long *idx , long long n , int w , double *dst )
+{
+	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( i>=n ) return;
+	const long long s = idx[i];
+	for( int k=0 ; k<w ; k++ ) dst[ i*w+k ] = s>=0 ? src[ s*w+k ] : 0.;
+}
+
 // Estimator splat (GridSamples.inl:436-491, merged cells): one thread per
 // (cell, tree depth, tap); density statistics added with FP64 atomics.
 // Tap `tap` of cell c at depth t: the corner offset and the B-spline weight;

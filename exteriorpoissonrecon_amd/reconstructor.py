@@ -129,5 +129,9 @@ class Reconstructor:
     def write_grid(self, path):
         self._check(load().epr_write_grid(self._h, path.encode()))
 
+    def write_density(self, path):
+        """<out>.density.tree: the density estimator as ExteriorPoissonRecon.cpp:272-277 writes it."""
+        self._check(load().epr_write_density(self._h, path.encode()))
+
     def seconds(self, which):
         return load().epr_last_seconds(self._h, 0 if which == "build" else 1)

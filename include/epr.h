@@ -144,6 +144,14 @@ int  epr_adapted_indexer_write( int dim , int depth , int aRadius , double scale
  * (Misha/RegularGrid.inl:440-471 layout, 2 DOUBLE values per node). */
 int  epr_write_grid( epr_ctx *ctx , const char *path );
 
+/* Writes the density estimator as <out>.density.tree
+ * (ExteriorPoissonRecon.cpp:272-277: the voxel-to-world XForm, then
+ * TreeEstimator::write, Include/GridSamples.inl:734-738): the estimator tree
+ * with the reference's node set and depth-first order, each node's
+ * DensityAndNoiseInfo statistics, and the target samples per cell.  Extract
+ * reads it with --density (epr_iso_read_density). */
+int  epr_write_density( epr_ctx *ctx , const char *path );
+
 /* Measurement hook (no reference counterpart): runs iters Gauss-Seidel
  * iterations at one level bracketed by HIP events on the context's stream and
  * returns the elapsed device milliseconds and the number of kernel launches. */

@@ -114,3 +114,26 @@ def fixture_tree(name, tmp_path):
 
 
 TRIM_FIXTURES = [n for n in EXTRACT_FIXTURES if "density" in np.load(os.path.join(GOLDEN, n + ".npz")).files]
+
+
+def decode_density(path, dim):
+    """A density file (ExteriorPoissonRecon's <out>.density.tree): the voxel-to-world
+    matrix, then per node in file order depth, offsets, has-children flag and the
+    DensityAndNoiseInfo statistics (weightSum, positionSum, covarianceSum; the
+    matrix's vtable slot dropped), then the target samples per cell."""
+    b = open(path, "rb").read()
+    o = 8
+    v2w = np.frombuffer(b, "<f8", (dim + 1) ** 2, o).reshape(dim + 1, dim + 1).copy()
+    o += 8 * (dim + 1) ** 2
+    head = ((2 * (dim + 1) + 7) // 8) * 8
+    rec = head + 16 + 8 * (2 + dim + dim * dim)
+    n = (len(b) - o - 8) // rec
+    assert o + n * rec + 8 == len(b)
+    raw = np.frombuffer(b, np.uint8, n * rec, o).reshape(n, rec)
+    h = raw[:, :2 * (dim + 1)].copy().view("<u2").reshape(n, dim + 1)
+    ch = raw[:, head + 8:head + 16].copy().view("<u8").ravel()
+    data = raw[:, head + 16:].copy().view("<f8").reshape(n, 2 + dim + dim * dim)
+    stats = np.concatenate([data[:, :1 + dim], data[:, 2 + dim:]], axis=1)
+    return {"voxel_to_world": v2w, "node_depth": h[:, 0].astype(np.int16), "node_offset": h[:, 1:].astype(np.int16),
+            "node_has_children": (ch != 0).astype(np.int8), "stats": stats,
+            "target": np.frombuffer(b, "<f8", 1, len(b) - 8)[0]}

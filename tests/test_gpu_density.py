@@ -1,0 +1,76 @@
+"""The density file the GPU reconstructor writes (<out>.density.tree,
+epr_write_density) against the reference's (tests/golden/density_*.npz,
+written by the reference build from the same samples), and its use by
+Extract --density.
+
+Bar: the same nodes in the same order (depth, offsets, has-children flag: the
+tree the reference's neighbour keys grow, written depth first), the same
+voxel-to-world matrix and target, and statistics within 1e-12 of each
+column's largest magnitude (the device splat sums with atomics, the reference
+in sample order).  End to end, Extract --density --trimDensity on the
+product's own .tree and .density.tree writes the same bytes as the reference
+build's Extract on those files."""
+import glob
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import ROOT, decode_density, sha256
+
+pytestmark = pytest.mark.gpu
+GOLD = sorted(glob.glob(os.path.join(ROOT, "tests", "golden", "density_*.npz")))
+CLI = os.path.join(ROOT, "exteriorpoissonrecon_amd", "bin", "ExteriorPoissonRecon")
+EXTRACT = os.path.join(ROOT, "exteriorpoissonrecon_amd", "bin", "Extract")
+REF_EXTRACT = os.path.join(ROOT, "oracle", "_ref", "Extract")
+
+
+@pytest.mark.parametrize("path", GOLD, ids=[os.path.basename(p)[:-4] for p in GOLD])
+def test_gpu_density_file_matches_reference(path, tmp_path):
+    from exteriorpoissonrecon_amd import Reconstructor
+    g = np.load(path)
+    print("case", os.path.basename(path), flush=True)
+    args = {k[4:]: g[k].item() for k in g.files if k.startswith("arg_")}
+    args.setdefault("aRadius", 3)
+    r = Reconstructor(g["in_pos"], g["in_skew"], **args).build()
+    out = str(tmp_path / "o.density.tree")
+    r.write_density(out)
+    r.close()
+    d = decode_density(out, g["in_pos"].shape[1])
+    for k in ("node_depth", "node_offset", "node_has_children"):
+        assert np.array_equal(d[k], g[k]), k
+    assert np.abs(d["voxel_to_world"] - g["voxel_to_world"]).max() <= 1e-15 * np.abs(g["voxel_to_world"]).max()
+    assert d["target"] == g["target"]
+    scale = np.maximum(np.abs(g["stats"]).max(axis=0), 1e-300)
+    assert (np.abs(d["stats"] - g["stats"]) / scale).max() <= 1e-12
+
+
+@pytest.mark.parametrize("dim", [3, 4])
+def test_gpu_extract_trims_with_product_density(dim, tmp_path):
+    from exteriorpoissonrecon_amd import samples
+    if dim == 3:
+        pos, skew = samples.torus_knot(4096, seed=4)
+        args = ["--depth", "5"]
+    else:
+        pos, skew = samples.clifford_torus(side=40, ang_deg=2.0, seed=4)
+        args = ["--depth", "3"]
+    ply = str(tmp_path / "in.ply")
+    samples.write_ply(ply, pos, skew)
+    res = subprocess.run([CLI, "--in", ply, "--out", str(tmp_path / "o"), *args], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0 and "[ERROR]" not in res.stderr, res.stderr
+    tree, dens = str(tmp_path / "o.tree"), str(tmp_path / "o.density.tree")
+    mine = str(tmp_path / "m.ply")
+    cmd = ["--in", tree, "--out", mine, "--density", dens, "--trimDensity", "2", "--verbose"]
+    res = subprocess.run([EXTRACT, *cmd], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0 and "[ERROR]" not in res.stderr, res.stderr
+    assert "Max Density[ 0 ]" in res.stdout
+    if os.path.exists(REF_EXTRACT):
+        ref = str(tmp_path / "r.ply")
+        cmd[3] = ref
+        rr = subprocess.run([REF_EXTRACT, *cmd], capture_output=True, text=True, timeout=600)
+        assert rr.returncode == 0, rr.stderr
+        assert sha256(open(mine, "rb").read()) == sha256(open(ref, "rb").read())
+        mx = [ln for ln in res.stdout.splitlines() if ln.startswith("Max Density")]
+        rmx = [ln for ln in rr.stdout.splitlines() if ln.startswith("Max Density")]
+        assert mx == rmx
