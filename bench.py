@@ -116,6 +116,45 @@ def cpu_baseline(name, wl, pos, skew, a_radius=-1):
     return {"value": dt, "unit": "s", "cores": cores, "kind": "reference", "sample": sample}
 
 
+def extract_timing(rec, args, dim, reps=2):
+    """Extract (Extract.cpp:109-259) on the reconstruction's own .tree: the wall
+    time of opening the tree and producing the oriented joint zero set with
+    frames (the --out mesh, not its PLY write), best of reps, with the stage
+    split.  Returns (timings, the folder holding the files)."""
+    from exteriorpoissonrecon_amd import load
+    from exteriorpoissonrecon_amd.extract import IsoTree
+    td = tempfile.mkdtemp(prefix="epr_extract_")
+    tree, dens = os.path.join(td, "o.tree"), os.path.join(td, "o.density.tree")
+    rec.write_tree(tree, load().epr_indexer_hash(dim, int(args.aRadius >= 0)))
+    rec.write_density(dens)
+    best = None
+    for r in range(reps):
+        t0 = time.perf_counter()
+        iso = IsoTree(tree)
+        V, S = iso.extract(True)
+        dt = time.perf_counter() - t0
+        stages = {k: iso.seconds(i) * 1e3 for i, k in enumerate(("open", "nodes_triangulation", "level_set_orient", "frames"))}
+        iso.close()
+        log(f"[bench] extract {r}: {dt * 1e3:.1f} ms {stages} -> {len(V)} vertices, {len(S)} simplices")
+        if best is None or dt < best[0]:
+            best = (dt, stages, len(V), len(S))
+    out = {"ms": best[0] * 1e3, "stages_ms": best[1], "vertices": best[2], "simplices": best[3],
+           "tree_bytes": os.path.getsize(tree)}
+    return out, td
+
+
+def extract_cpu_baseline(td, dim):
+    """The reference build's Extract (oracle/_ref/Extract, compiled from the
+    reference sources) on the same .tree, all host cores, its PLY write
+    included; 3D only (its 4D depth-6 run takes tens of minutes)."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "Extract")
+    if dim != 3 or not os.path.exists(ref):
+        return None
+    t0 = time.perf_counter()
+    subprocess.run([ref, "--in", os.path.join(td, "o.tree"), "--out", os.path.join(td, "r.ply")], check=True, capture_output=True)
+    return time.perf_counter() - t0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,6 +164,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
     ap.add_argument("--aRadius", type=int, default=-1, help="-1: dense grid; 0..5: adapted hierarchy of that radius")
+    ap.add_argument("--extract", action="store_true",
+                    help="also time the Extract pipeline on the .tree the last step writes (an 'extract' object in the JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
@@ -188,7 +229,16 @@ def main():
         log(f"[bench] measured HBM traffic of the finest-sweep kernel: {traffic / 1e6:.2f} MB per launch "
             f"({traffic / per_launch_s / 1e9:.1f} GB/s at this launch time)")
 
+    extract, extract_dir = extract_timing(rec, args, dim) if args.extract else (None, None)
+
     cpu = None if args.no_cpu else cpu_baseline(args.config, wl, pos, skew, args.aRadius)
+    if extract_dir:
+        if not args.no_cpu:
+            extract["reference_cpu_s"] = extract_cpu_baseline(extract_dir, dim)
+            log(f"[bench] reference Extract: {extract['reference_cpu_s']}")
+        for f in os.listdir(extract_dir):
+            os.remove(os.path.join(extract_dir, f))
+        os.rmdir(extract_dir)
     if cpu:
         log(f"[bench] cpu baseline {cpu['value']:.3f}s ({cpu['sample']})")
 
@@ -217,6 +267,8 @@ def main():
                      "traffic_gbs": traffic / per_launch_s / 1e9 if traffic else None},
         "cpu_baseline": cpu,
     }
+    if extract:
+        out["extract"] = extract
     print(json.dumps(out))
 
 
