@@ -33,7 +33,8 @@ class Params(ctypes.Structure):
                 ("singleLevel", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("nSigma", ctypes.c_double), ("scale", ctypes.c_double),
                 ("sWeight", ctypes.c_double), ("dWeight", ctypes.c_double),
-                ("iMult", ctypes.c_double), ("aRadius", ctypes.c_int)]
+                ("iMult", ctypes.c_double), ("aRadius", ctypes.c_int), ("separate", ctypes.c_int),
+                ("jitter", ctypes.c_double)]
 
 
 class EprError(RuntimeError):

@@ -32,12 +32,14 @@ static void Usage( const char *ex )
 	printf( "\t[--dRadius <density kernel radius>=1]\n" );
 	printf( "\t[--tRadius <tensor kernel radius>=1]\n" );
 	printf( "\t[--aRadius <adaptive radius>=3]\n" );
+	printf( "\t[--jitter <jitter magnitude>=%f]\n" , 0. );
 	printf( "\t[--gsIters <gs iterations>=10]\n" );
 	printf( "\t[--seed <random seed>=0]\n" );
 	printf( "\t[--sWeight <screening weight>=100]\n" );
 	printf( "\t[--dWeight <dirichlet weight>=0.00625]\n" );
 	printf( "\t[--scale <bounding box scale>=1.100000]\n" );
 	printf( "\t[--iMult <iteration multiplier>=2.000000]\n" );
+	printf( "\t[--separate]\n" );
 	printf( "\t[--singleLevel]\n" );
 	printf( "\t[--verbose]\n" );
 }
@@ -48,8 +50,7 @@ int main( int argc , char *argv[] )
 	epr_default_params( &p );
 	std::string in , out , grid;
 	long long aRadius = 3;
-	bool project = false , separate = false;
-	double jitter = 0;
+	bool project = false;
 	for( int i=1 ; i<argc ; i++ )
 	{
 		std::string a = argv[i];
@@ -72,9 +73,9 @@ int main( int argc , char *argv[] )
 		else if( a=="sWeight" ) p.sWeight = atof( val().c_str() );
 		else if( a=="dWeight" ) p.dWeight = atof( val().c_str() );
 		else if( a=="iMult" ) p.iMult = atof( val().c_str() );
-		else if( a=="jitter" ) jitter = atof( val().c_str() );
+		else if( a=="jitter" ) p.jitter = atof( val().c_str() );
 		else if( a=="singleLevel" ) p.singleLevel = 1;
-		else if( a=="separate" ) separate = true;
+		else if( a=="separate" ) p.separate = 1;
 		else if( a=="project" ) project = true;
 		else if( a=="verbose" ) p.verbose = 1;
 		else fprintf( stderr , "[WARNING] Invalid parameter: --%s\n" , a.c_str() );
@@ -85,8 +86,6 @@ int main( int argc , char *argv[] )
 	else if( aRadius<0 || aRadius>5 ) Error( "Only kernel radius in the range [0,5] supported: " + std::to_string( (unsigned int)aRadius ) );
 	else p.aRadius = (int)aRadius;
 	if( project ) Error( "--project is not supported" );
-	if( separate ) Error( "--separate is not supported" );
-	if( jitter>0 ) Error( "--jitter is not supported" );
 	if( p.maxSolveDepth>p.depth ) p.maxSolveDepth = p.depth;
 	if( in.size()<4 || in.substr( in.size()-4 )!=".ply" ) Error( "Only .ply files supported" );
 	std::string err;

@@ -39,7 +39,17 @@ struct SampleCells
 	std::vector< int > cellIdx , cellStart , cellSamples;
 	int ncell = 0;
 
-	void bin( size_t n , const double *wpos , double scale , int D )
+	// glibc rand() as Random<double>() draws it (Misha/Geometry.inl:429-438)
+	static double Random( void )
+	{
+		static const unsigned long long ULL_RAND_MAX = (unsigned long long)( RAND_MAX ) + 1;
+		static const double RAND_MAX_SQUARED = double( ULL_RAND_MAX * ULL_RAND_MAX - 1 );
+		unsigned long long r1 = rand() , r2 = rand();
+		long long foo = r1 * ULL_RAND_MAX + r2;
+		return double( foo ) / RAND_MAX_SQUARED;
+	}
+
+	void bin( size_t n , const double *wpos , double scale , int D , double jitter=0 )
 	{
 		if( !n ) throw std::string( "Expected some samples: 0" );
 		for( size_t i=0 ; i<n*Dim ; i++ ) if( !std::isfinite( wpos[i] ) ) throw std::string( "non-finite sample position" );
@@ -75,6 +85,31 @@ struct SampleCells
 		}
 		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) unitToWorld[i][j] = ( i==j ? 1. : 0. );
 		for( int d=0 ; d<Dim ; d++ ){ unitToWorld[d][d] = 1./M[d][d] ; unitToWorld[Dim][d] = -M[Dim][d]/M[d][d]; }
+		// --jitter (ExteriorPoissonRecon.cpp:174-180): a random translation of the
+		// unit-cube positions drawn before the seed is set, i.e. from glibc's
+		// unseeded sequence (srand(1)); the unit-cube-to-world map absorbs its inverse
+		if( jitter>0 )
+		{
+			double T[Dim+1][Dim+1] , Ti[Dim+1][Dim+1] , U[Dim+1][Dim+1];
+			for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) T[i][j] = Ti[i][j] = ( i==j ? 1. : 0. );
+			srand( 1 );
+			for( int d=0 ; d<Dim ; d++ ) T[Dim][d] = ( Random() * 2. - 1. ) * jitter , Ti[Dim][d] = -T[Dim][d];
+			for( size_t i=0 ; i<n ; i++ )
+			{
+				double q[Dim] , scl = 1;
+				for( int a=0 ; a<Dim ; a++ )
+				{
+					double o = 0;
+					for( int b=0 ; b<Dim ; b++ ) o += pos[i*Dim+b]*T[b][a];
+					o += T[Dim][a];
+					scl += T[a][Dim];
+					q[a] = o;
+				}
+				for( int a=0 ; a<Dim ; a++ ) pos[i*Dim+a] = q[a] / scl;
+			}
+			Mul( unitToWorld , Ti , U );
+			memcpy( unitToWorld , U , sizeof(U) );
+		}
 		int res = 1<<D;
 		struct IS { size_t idx ; int I[Dim] ; bool operator<( const IS &o ) const { for( int d=0 ; d<Dim ; d++ ){ if( I[d]<o.I[d] ) return true ; if( I[d]>o.I[d] ) return false; } return false; } };
 		std::vector< IS > v( n );
@@ -256,6 +291,7 @@ struct Ctx : public CtxBase
 		for( int d=0 ; d<Dim ; d++ ) v.stride[d] = L.stride[d];
 		v.lay = L.lay;
 		v.fmask = L.fm.p;
+		v.separate = prm.separate;
 		return v;
 	}
 
@@ -263,7 +299,7 @@ struct Ctx : public CtxBase
 	void setSamples( size_t _n , const double *wpos , const double *wskew )
 	{
 		SampleCells< Dim > sc;
-		sc.bin( _n , wpos , prm.scale , D );
+		sc.bin( _n , wpos , prm.scale , D , prm.jitter );
 		n = _n;
 		skewPacked.resize( n*CD );
 		for( size_t s=0 ; s<n ; s++ ) for( int i=0 , idx=0 ; i<Dim ; i++ ) for( int j=0 ; j<i ; j++ , idx++ )
@@ -1016,7 +1052,7 @@ extern "C" {
 void epr_default_params( epr_params *p )
 {
 	p->depth = 5 , p->seed = 0 , p->tRadius = 1 , p->dRadius = 1 , p->minSolveDepth = 0 , p->maxSolveDepth = -1 , p->gsIters = 10;
-	p->singleLevel = 0 , p->verbose = 0;
+	p->singleLevel = 0 , p->verbose = 0 , p->separate = 0 , p->jitter = 0;
 	p->nSigma = 0.25 , p->scale = 1.1 , p->sWeight = 100. , p->dWeight = 0.00625 , p->iMult = 2.;
 	p->aRadius = 3;
 }

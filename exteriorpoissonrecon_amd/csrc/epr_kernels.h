@@ -72,6 +72,7 @@ struct LevelView
 	long long stride[Dim];
 	ColLayout< Dim > lay;   // column position of a node in R and B
 	const unsigned char *fmask;   // adapted levels: 1 = active (updated by the sweeps); null: every node is active
+	int separate;                 // --separate: x then y per node (SeparateGaussSeidelSolve) instead of the paired step
 };
 
 enum { MODE_GS = 0 , MODE_QFIT = 1 , MODE_ENERGY = 2 };
@@ -161,6 +162,27 @@ __device__ __forceinline__ bool NodeStep( double q20 , double q02 , double q11 ,
 	o0 = o0 + q01*m10 , o1 = o1 + q01*m11;
 	dx = o0*0.5 , dy = o1*0.5;
 	return true;
+}
+
+// The --separate step of ExteriorPoissonSolvers.h:198-238: x with y fixed,
+// then y with the new x.  The reference guards each with fabs( H(k,k)>1e-20 ),
+// the absolute value of a comparison, i.e. H(k,k)>1e-20; the update is always
+// added (zero when a guard fails).
+__device__ __forceinline__ void NodeStepSeparate( double q20 , double q02 , double q11 , double q10 , double q01 , double &dx , double &dy )
+{
+	const double H00 = 2*q20 , H11 = 2*q02 , H01 = q11 , H10 = q11;
+	double p0 = 0 , p1 = 0;
+	if( H00>1e-20 ) p0 = -( ( H10 + H01 )*p1 + q10 ) / ( 2*H00 );
+	if( H11>1e-20 ) p1 = -( ( H10 + H01 )*p0 + q01 ) / ( 2*H11 );
+	dx = p0 , dy = p1;
+}
+
+// The step a level view selects: paired (false: leave the node) or separate.
+template< int Dim >
+__device__ __forceinline__ bool LevelStep( const LevelView< Dim > &L , const double q[5] , double &dx , double &dy )
+{
+	if( L.separate ){ NodeStepSeparate( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ; return true; }
+	return NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy );
 }
 
 // Processes node i (multi-index F) with the C lanes of its group: lane = one
@@ -322,7 +344,7 @@ __global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSe
 	if( lane==0 )
 	{
 		double dx , dy;
-		if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
+		if( LevelStep< Dim >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
 	}
 }
 
@@ -349,7 +371,7 @@ __global__ void __launch_bounds__( 256 ) KGsPhaseList( LevelView< Dim > L , cons
 	if( lane==0 )
 	{
 		double dx , dy;
-		if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
+		if( LevelStep< Dim >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
 	}
 }
 
@@ -373,7 +395,7 @@ __global__ void __launch_bounds__( 512 ) KGsSmall( LevelView< Dim > L , int iter
 			if( lane==0 && ( !L.fmask || L.fmask[i]==1 ) )
 			{
 				double dx , dy;
-				if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
+				if( LevelStep< Dim >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
 			}
 		}
 		__syncthreads();
@@ -1339,7 +1361,7 @@ __global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLds( LevelView<
 			if( lane==0 && ( !L.fmask || L.fmask[i]==1 ) )
 			{
 				double dx , dy;
-				if( NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ) sx[i] += dx , sy[i] += dy;
+				if( LevelStep< Dim >( L , q , dx , dy ) ) sx[i] += dx , sy[i] += dy;
 			}
 		}
 		__syncthreads();
@@ -1637,7 +1659,9 @@ __global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView<
 		}
 		q20 = RowSum( q20 ) , q02 = RowSum( q02 ) , q11 = RowSum( q11 ) , q10 = RowSum( q10 ) , q01 = RowSum( q01 );
 		double dx , dy;
-		const bool step = NodeStepFast( q20 , q02 , q11 , q10 , q01 , dx , dy );
+		bool step;
+		if( L.separate ){ NodeStepSeparate( q20 , q02 , q11 , q10 , q01 , dx , dy ) ; step = true; }
+		else step = NodeStepFast( q20 , q02 , q11 , q10 , q01 , dx , dy );
 		if( valid && rl==RC && step && ( !L.fmask || L.fmask[i]==1 ) ) sx[i] = wx[MC] + dx , sy[i] = wy[MC] + dy;
 		__syncthreads();
 	}

@@ -47,6 +47,10 @@ typedef struct
 	int    aRadius;        /* --aRadius 3: -1 selects the dense grid (HierarchicalRegularIndexer),
 	                          0..5 the adapted hierarchy of that radius (HierarchicalAdaptedIndexer,
 	                          Include/Hat.adaptive.h:208-331) */
+	int    separate;       /* --separate: SeparateGaussSeidelSolve (ExteriorPoissonSolvers.h:198-250)
+	                          instead of PairedGaussSeidelSolve */
+	double jitter;         /* --jitter 0: a random translation of the unit-cube samples
+	                          (ExteriorPoissonRecon.cpp:174-180) */
 } epr_params;
 
 /* Fills the reference defaults (ExteriorPoissonRecon.cpp:56-72). */

@@ -553,6 +553,32 @@ struct Recon : public Base
 		// unitCubeToWorld = inverse( worldToUnitCube ) for a scale+translate transform
 		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) unitCubeToWorld[i][j] = ( i==j ? 1. : 0. );
 		for( int d=0 ; d<Dim ; d++ ){ unitCubeToWorld[d][d] = 1./M[d][d] ; unitCubeToWorld[Dim][d] = -M[Dim][d]/M[d][d]; }
+		// --jitter (ExteriorPoissonRecon.cpp:174-180): translation( Dim , d ) = ( Random()*2-1 )*jitter
+		// drawn from the unseeded rand() sequence (the seed is set only later, :220)
+		if( prm.jitter>0 )
+		{
+			double T[Dim+1][Dim+1] , Ti[Dim+1][Dim+1] , U[Dim+1][Dim+1];
+			for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) T[i][j] = ( i==j ? 1. : 0. );
+			srand( 1 );
+			for( int d=0 ; d<Dim ; d++ ) T[Dim][d] = ( Random() * 2. - 1. ) * prm.jitter;
+			for( size_t i=0 ; i<n ; i++ )
+			{
+				double q[Dim] , scl = 1;
+				for( int a=0 ; a<Dim ; a++ )
+				{
+					double o = 0;
+					for( int b=0 ; b<Dim ; b++ ) o += pos[i*Dim+b]*T[b][a];
+					o += T[Dim][a];
+					scl += T[a][Dim];
+					q[a] = o;
+				}
+				for( int a=0 ; a<Dim ; a++ ) pos[i*Dim+a] = q[a] / scl;
+			}
+			// translation.inverse(): the identity with the negated translation row
+			for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) Ti[i][j] = ( i==Dim && j<Dim ) ? -T[i][j] : T[i][j];
+			MatMul( unitCubeToWorld , Ti , U );
+			memcpy( unitCubeToWorld , U , sizeof(U) );
+		}
 
 		orderSamples();
 		build();
@@ -1131,6 +1157,18 @@ struct Recon : public Base
 		dx = o0 * ( 1./2 ) , dy = o1 * ( 1./2 );
 	}
 
+	// SeparateGaussSeidelSolve's node update (ExteriorPoissonSolvers.h:211-236):
+	// p = 0, then p[0] with p[1] fixed, then p[1] with the new p[0]; each guarded
+	// by fabs( H(k,k)>1e-20 ), which is true exactly when H(k,k)>1e-20
+	static void NodeStepSeparate( const double Q[5] , double &dx , double &dy )
+	{
+		const double H00 = 2*Q[0] , H11 = 2*Q[1] , H01 = Q[2] , H10 = Q[2] , g0 = Q[3] , g1 = Q[4];
+		double p[2] = { 0 , 0 };
+		if( fabs( (double)( H00>1e-20 ) ) ) p[0] = - ( ( H10 + H01 )*p[1] + g0 ) / ( 2 * H00 );
+		if( fabs( (double)( H11>1e-20 ) ) ) p[1] = - ( ( H10 + H01 )*p[0] + g1 ) / ( 2 * H11 );
+		dx = p[0] , dy = p[1];
+	}
+
 	void gs( int l , double *x , double *y , int iters ) const
 	{
 		const Level &L = levels[l];
@@ -1143,7 +1181,8 @@ struct Recon : public Base
 				unsigned int idx = col[j];
 				double Q[5] , dx , dy ; bool ok;
 				fitNode( L , idx , x , y , Q );
-				NodeStep( Q , dx , dy , ok );
+				if( prm.separate ){ NodeStepSeparate( Q , dx , dy ) ; ok = true; }
+				else NodeStep( Q , dx , dy , ok );
 				if( ok ) x[idx] += dx , y[idx] += dy;
 			}
 		}
@@ -1368,6 +1407,7 @@ extern "C" {
 void orc_default_params( orc_params *p )
 {
 	p->depth = 5 , p->seed = 0 , p->tRadius = 1 , p->dRadius = 1 , p->minSolveDepth = 0 , p->maxSolveDepth = -1 , p->gsIters = 10 , p->singleLevel = 0;
+	p->separate = 0 , p->jitter = 0;
 	p->nSigma = 0.25 , p->scale = 1.1 , p->sWeight = 100. , p->dWeight = 0.00625 , p->iMult = 2.;
 }
 

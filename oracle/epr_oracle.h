@@ -39,6 +39,8 @@ typedef struct
 	double sWeight;        /* --sWeight          (:59), scales R                */
 	double dWeight;        /* --dWeight          (:60), scales the stiffness    */
 	double iMult;          /* --iMult            (:61)                          */
+	int    separate;       /* --separate: SeparateGaussSeidelSolve (ExteriorPoissonSolvers.h:198-250) */
+	double jitter;         /* --jitter           (:62, applied at :174-180)     */
 } orc_params;
 
 void   orc_default_params( orc_params *p );

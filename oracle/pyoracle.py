@@ -19,7 +19,8 @@ class Params(ctypes.Structure):
                 ("maxSolveDepth", ctypes.c_int), ("gsIters", ctypes.c_int),
                 ("singleLevel", ctypes.c_int), ("nSigma", ctypes.c_double),
                 ("scale", ctypes.c_double), ("sWeight", ctypes.c_double),
-                ("dWeight", ctypes.c_double), ("iMult", ctypes.c_double)]
+                ("dWeight", ctypes.c_double), ("iMult", ctypes.c_double),
+                ("separate", ctypes.c_int), ("jitter", ctypes.c_double)]
 
 
 _lib = None
