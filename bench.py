@@ -116,7 +116,7 @@ def cpu_baseline(name, wl, pos, skew, a_radius=-1):
     return {"value": dt, "unit": "s", "cores": cores, "kind": "reference", "sample": sample}
 
 
-def extract_timing(rec, args, dim, reps=2):
+def extract_timing(rec, args, dim, reps=3):
     """Extract (Extract.cpp:109-259) on the reconstruction's own .tree: the wall
     time of opening the tree and producing the oriented joint zero set with
     frames (the --out mesh, not its PLY write), best of reps, with the stage
@@ -133,7 +133,7 @@ def extract_timing(rec, args, dim, reps=2):
         iso = IsoTree(tree)
         V, S = iso.extract(True)
         dt = time.perf_counter() - t0
-        stages = {k: iso.seconds(i) * 1e3 for i, k in enumerate(("open", "nodes_triangulation", "level_set_orient", "frames"))}
+        stages = {k: iso.seconds(i) * 1e3 for i, k in enumerate(("open", "nodes_triangulation", "level_set", "orient", "frames"))}
         iso.close()
         log(f"[bench] extract {r}: {dt * 1e3:.1f} ms {stages} -> {len(V)} vertices, {len(S)} simplices")
         if best is None or dt < best[0]:

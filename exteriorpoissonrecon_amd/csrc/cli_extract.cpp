@@ -127,8 +127,9 @@ int main( int argc , char *argv[] )
 		if( verbose )
 		{
 			printf( "%25s: %.3f (s)\n" , "Triangulation" , epr_iso_seconds( iso , 1 ) );
-			printf( "%25s: %.3f (s)\n" , "Level set + oriented" , epr_iso_seconds( iso , 2 ) );
-			printf( "%25s: %.3f (s)\n" , "Frames" , epr_iso_seconds( iso , 3 ) );
+			printf( "%25s: %.3f (s)\n" , "Level set" , epr_iso_seconds( iso , 2 ) );
+			printf( "%25s: %.3f (s)\n" , "Oriented" , epr_iso_seconds( iso , 3 ) );
+			printf( "%25s: %.3f (s)\n" , "Frames" , epr_iso_seconds( iso , 4 ) );
 		}
 		std::vector< std::string > props;
 		for( int d=0 ; d<dim ; d++ ) props.push_back( std::string( "double " ) + Axis[d] );

@@ -45,6 +45,8 @@ namespace epr
 namespace iso
 {
 constexpr unsigned long long NONE64 = 0xffffffffffffffffull;
+using Clock = std::chrono::steady_clock;
+static double Since( Clock::time_point t ){ return std::chrono::duration< double >( Clock::now() - t ).count(); }
 
 // ------------------------------------------------------------------ .tree reading
 struct TreeData
@@ -515,8 +517,15 @@ __global__ void __launch_bounds__( 256 ) KFaceLink( const unsigned long long *sk
 	same[r] = p==NONE32 ? 0 : ( even[r]==even[p] );
 }
 
+// The face links on the device: other[r] / same[r] per (simplex, face) record r
+struct Links
+{
+	DevBuf< unsigned > other;
+	DevBuf< unsigned char > same;
+};
+
 template< int K >
-static void LinkFaces( const unsigned *dS , size_t nS , size_t nV , hipStream_t st , std::vector< unsigned > &other , std::vector< unsigned char > &same )
+static void LinkFaces( const unsigned *dS , size_t nS , size_t nV , hipStream_t st , Links &L )
 {
 	const size_t nR = nS*( K+1 );
 	if( nR>=NONE32 ) throw std::string( "too many simplices for 32-bit face records" );
@@ -524,36 +533,126 @@ static void LinkFaces( const unsigned *dS , size_t nS , size_t nV , hipStream_t 
 	while( ( 1ull<<vb )<nV ) vb++;
 	const int bits = K==1 ? vb : 2*vb;
 	DevBuf< unsigned long long > key , skey;
-	DevBuf< unsigned > id , sid , dOther , err;
-	DevBuf< unsigned char > even , dSame , tmp;
-	key.alloc( nR ) , skey.alloc( nR ) , id.alloc( nR ) , sid.alloc( nR ) , even.alloc( nR ) , dOther.alloc( nR ) , dSame.alloc( nR ) , err.alloc( 1 );
+	DevBuf< unsigned > id , sid , err;
+	DevBuf< unsigned char > even , tmp;
+	key.alloc( nR ) , skey.alloc( nR ) , id.alloc( nR ) , sid.alloc( nR ) , even.alloc( nR ) , L.other.alloc( nR ) , L.same.alloc( nR ) , err.alloc( 1 );
 	err.zero( st );
 	KFaceRecords< K ><<< Grid( nR ) , 256 , 0 , st >>>( dS , (unsigned)nR , vb , key.p , id.p , even.p );
 	size_t bytes = 0;
 	HIPCHECK( hipcub::DeviceRadixSort::SortPairs( nullptr , bytes , key.p , skey.p , id.p , sid.p , (int)nR , 0 , bits , st ) );
 	tmp.ensure( bytes );
 	HIPCHECK( hipcub::DeviceRadixSort::SortPairs( tmp.p , bytes , key.p , skey.p , id.p , sid.p , (int)nR , 0 , bits , st ) );
-	KFaceLink< K ><<< Grid( nR ) , 256 , 0 , st >>>( skey.p , sid.p , (unsigned)nR , even.p , dOther.p , dSame.p , err.p );
+	KFaceLink< K ><<< Grid( nR ) , 256 , 0 , st >>>( skey.p , sid.p , (unsigned)nR , even.p , L.other.p , L.same.p , err.p );
 	HIPCHECK( hipGetLastError() );
 	unsigned e = 0;
-	other.resize( nR ) , same.resize( nR );
 	err.download( &e , 1 , st );
-	dOther.download( other.data() , nR , st );
-	dSame.download( same.data() , nR , st );
 	HIPCHECK( hipStreamSynchronize( st ) );
 	if( e ) throw std::string( "Both faces occupied" );
 }
 
+// ---- orientation of every orientable component on the device.  Crossing a
+// shared face, the walk flips the neighbour exactly when both incidences carry
+// the same orientation, so flip(o) = flip(s) xor same: a parity constraint per
+// link.  A union-find with the parity to the parent (entry = parent << 1 |
+// parity) solves it; the walk's seed is the component's smallest simplex and
+// keeps its orientation, so flip(s) = parity(s) xor parity(seed).  A component
+// whose links contradict each other is not orientable: there the walk's order
+// decides, and the host walks it as the reference does.
+__device__ __forceinline__ unsigned long long PLoad( const unsigned long long *p ){ return __hip_atomic_load( p , __ATOMIC_RELAXED , __HIP_MEMORY_SCOPE_AGENT ); }
+
+__device__ __forceinline__ void PFind( unsigned long long *node , unsigned x , unsigned &root , unsigned &par )
+{
+	unsigned acc = 0;
+	unsigned long long e = PLoad( node + x );
+	while( (unsigned)( e>>1 )!=x )
+	{
+		const unsigned p = (unsigned)( e>>1 ) , a = (unsigned)( e&1 );
+		const unsigned long long f = PLoad( node + p );
+		const unsigned g = (unsigned)( f>>1 ) , b = (unsigned)( f&1 );
+		if( g!=p )
+		{
+			// path halving: x's parent becomes its grandparent, with the combined parity
+			__hip_atomic_store( node + x , ( (unsigned long long)g<<1 ) | ( a^b ) , __ATOMIC_RELAXED , __HIP_MEMORY_SCOPE_AGENT );
+			acc ^= a^b , x = g , e = PLoad( node + g );
+		}
+		else acc ^= a , x = p , e = f;
+	}
+	root = x , par = acc;
+}
+
+__global__ void __launch_bounds__( 256 ) KParityInit( unsigned long long *node , unsigned nS , unsigned *seed )
+{
+	const unsigned s = blockIdx.x*blockDim.x + threadIdx.x;
+	if( s<nS ) node[s] = (unsigned long long)s<<1 , seed[s] = NONE32;
+}
+
 template< int K >
-static void Orient( std::vector< unsigned > &S , const std::vector< unsigned > &other , const std::vector< unsigned char > &same , std::vector< unsigned char > &flipped )
+__global__ void __launch_bounds__( 256 ) KParityLink( const unsigned *other , const unsigned char *same , unsigned nR , unsigned long long *node )
+{
+	const unsigned r = blockIdx.x*blockDim.x + threadIdx.x;
+	if( r>=nR ) return;
+	const unsigned a = r/( K+1 ) , b = other[r];
+	if( b==NONE32 || b<a ) return;                    // each link once
+	const unsigned w = same[r];
+	while( true )
+	{
+		unsigned ra , pa , rb , pb;
+		PFind( node , a , ra , pa ) , PFind( node , b , rb , pb );
+		if( ra==rb ) return;
+		const unsigned lo = ra<rb ? ra : rb , hi = ra<rb ? rb : ra;
+		unsigned long long expected = (unsigned long long)hi<<1;
+		if( __hip_atomic_compare_exchange_strong( node + hi , &expected , ( (unsigned long long)lo<<1 ) | ( pa^pb^w ) ,
+		                                          __ATOMIC_RELAXED , __ATOMIC_RELAXED , __HIP_MEMORY_SCOPE_AGENT ) ) return;
+	}
+}
+
+__global__ void __launch_bounds__( 256 ) KParityRoot( unsigned long long *node , unsigned nS , unsigned *rootOf , unsigned char *par , unsigned *seed )
+{
+	const unsigned s = blockIdx.x*blockDim.x + threadIdx.x;
+	if( s>=nS ) return;
+	unsigned r , p;
+	PFind( node , s , r , p );
+	rootOf[s] = r , par[s] = (unsigned char)p;
+	atomicMin( seed + r , s );
+}
+
+template< int K >
+__global__ void __launch_bounds__( 256 ) KParityCheck( const unsigned *other , const unsigned char *same , unsigned nR , const unsigned *rootOf ,
+                                                        const unsigned char *par , unsigned char *bad )
+{
+	const unsigned r = blockIdx.x*blockDim.x + threadIdx.x;
+	if( r>=nR ) return;
+	const unsigned a = r/( K+1 ) , b = other[r];
+	if( b!=NONE32 && ( par[a]^par[b] )!=same[r] ) bad[ rootOf[a] ] = 1;
+}
+
+// orientable components: swap the first two vertices of the flipped simplices
+template< int K >
+__global__ void __launch_bounds__( 256 ) KParityApply( unsigned *S , unsigned nS , const unsigned *rootOf , const unsigned char *par , const unsigned *seed ,
+                                                        const unsigned char *bad , unsigned char *flip )
+{
+	const unsigned s = blockIdx.x*blockDim.x + threadIdx.x;
+	if( s>=nS ) return;
+	const unsigned r = rootOf[s];
+	if( bad[r] ){ flip[s] = 2 ; return; }             // 2: left to the host walk
+	const unsigned char f = par[s] ^ par[ seed[r] ];
+	flip[s] = f;
+	if( f ){ const unsigned t = S[ (size_t)s*( K+1 ) ] ; S[ (size_t)s*( K+1 ) ] = S[ (size_t)s*( K+1 )+1 ] ; S[ (size_t)s*( K+1 )+1 ] = t; }
+}
+
+// The reference's walk (seeds in index order, a stack, faces in the order of
+// each simplex's current vertices, a flip swapping the first two vertices),
+// over the simplices marked 2 (non-orientable components) only.
+template< int K >
+static void OrientWalk( std::vector< unsigned > &S , const std::vector< unsigned > &other , const std::vector< unsigned char > &same , std::vector< unsigned char > &flipped )
 {
 	const size_t nS = S.size()/( K+1 );
 	std::vector< unsigned char > oriented( nS , 0 );
-	flipped.assign( nS , 0 );
 	std::vector< std::pair< unsigned , bool > > stack;
 	for( size_t idx=0 ; idx<nS ; idx++ )
 	{
-		if( oriented[idx] ) continue;
+		if( flipped[idx]!=2 || oriented[idx] ) continue;
+		flipped[idx] = 0;
 		oriented[idx] = 1;
 		stack.push_back( { (unsigned)idx , true } );
 		while( !stack.empty() )
@@ -563,12 +662,12 @@ static void Orient( std::vector< unsigned > &S , const std::vector< unsigned > &
 			for( int v=0 ; v<=K ; v++ )
 			{
 				// after a flip the faces opposite vertices 0 and 1 trade places
-				const int vo = flipped[ s.first ] && v<2 ? 1-v : v;
+				const int vo = flipped[ s.first ]==1 && v<2 ? 1-v : v;
 				const size_t r = (size_t)s.first*( K+1 )+vo;
 				const unsigned o = other[r];
 				if( o==NONE32 || oriented[o] ) continue;
 				oriented[o] = 1;
-				if( s.second!=( same[r]!=0 ) ) stack.push_back( { o , true } );
+				if( s.second!=( same[r]!=0 ) ){ flipped[o] = 0 ; stack.push_back( { o , true } ); }
 				else
 				{
 					std::swap( S[ (size_t)o*( K+1 ) ] , S[ (size_t)o*( K+1 )+1 ] );
@@ -577,6 +676,51 @@ static void Orient( std::vector< unsigned > &S , const std::vector< unsigned > &
 				}
 			}
 		}
+	}
+}
+
+// MarchingSimplices::Orient of the device simplices dS (nS x (K+1)): S receives
+// the oriented simplices, flipped the per-simplex flips; other (host) is filled
+// when wantLinks or a non-orientable component needs the walk
+template< int K >
+static void Orient( unsigned *dS , size_t nS , size_t nV , hipStream_t st , bool wantLinks , std::vector< unsigned > &S ,
+                    std::vector< unsigned char > &flipped , std::vector< unsigned > &other , double *walkSeconds=nullptr )
+{
+	S.resize( nS*( K+1 ) ) , flipped.assign( nS , 0 ) , other.clear();
+	if( !nS ) return;
+	Links L;
+	LinkFaces< K >( dS , nS , nV , st , L );
+	const size_t nR = nS*( K+1 );
+	DevBuf< unsigned long long > node;
+	DevBuf< unsigned > rootOf , seed;
+	DevBuf< unsigned char > par , bad , flip;
+	node.alloc( nS ) , rootOf.alloc( nS ) , seed.alloc( nS ) , par.alloc( nS ) , bad.alloc( nS ) , flip.alloc( nS );
+	bad.zero( st );
+	KParityInit<<< Grid( nS ) , 256 , 0 , st >>>( node.p , (unsigned)nS , seed.p );
+	KParityLink< K ><<< Grid( nR ) , 256 , 0 , st >>>( L.other.p , L.same.p , (unsigned)nR , node.p );
+	KParityRoot<<< Grid( nS ) , 256 , 0 , st >>>( node.p , (unsigned)nS , rootOf.p , par.p , seed.p );
+	KParityCheck< K ><<< Grid( nR ) , 256 , 0 , st >>>( L.other.p , L.same.p , (unsigned)nR , rootOf.p , par.p , bad.p );
+	KParityApply< K ><<< Grid( nS ) , 256 , 0 , st >>>( dS , (unsigned)nS , rootOf.p , par.p , seed.p , bad.p , flip.p );
+	HIPCHECK( hipGetLastError() );
+	flip.download( flipped.data() , nS , st );
+	HIPCHECK( hipMemcpyAsync( S.data() , dS , nR*sizeof(unsigned) , hipMemcpyDeviceToHost , st ) );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	bool anyBad = false;
+	for( size_t s=0 ; s<nS && !anyBad ; s++ ) anyBad = flipped[s]==2;
+	if( wantLinks || anyBad )
+	{
+		other.resize( nR );
+		L.other.download( other.data() , nR , st );
+		HIPCHECK( hipStreamSynchronize( st ) );
+	}
+	if( anyBad )
+	{
+		Clock::time_point t0 = Clock::now();
+		std::vector< unsigned char > same( nR );
+		L.same.download( same.data() , nR , st );
+		HIPCHECK( hipStreamSynchronize( st ) );
+		OrientWalk< K >( S , other , same , flipped );
+		if( walkSeconds ) *walkSeconds = Since( t0 );
 	}
 }
 
@@ -813,16 +957,13 @@ struct epr_iso
 	epr::DevBuf< double > dW;                           // its weightSum per node
 	std::vector< double > componentMax;                 // the last trimmed extraction's per-component maxima
 	std::string err;
-	double seconds[4] = { 0 , 0 , 0 , 0 };               // open, level-set nodes + triangulation, level set + orient, frames
+	double seconds[5] = { 0 , 0 , 0 , 0 , 0 };           // open, level-set nodes + triangulation, level set, orientation (+ trimming), frames
 };
 
 namespace epr
 {
 namespace iso
 {
-using Clock = std::chrono::steady_clock;
-static double Since( Clock::time_point t ){ return std::chrono::duration< double >( Clock::now() - t ).count(); }
-
 template< int Dim >
 static void Upload( epr_iso &I , const TreeData &t )
 {
@@ -991,22 +1132,21 @@ static epr_mesh *Extract( epr_iso &I , bool all , double trim )
 	I.seconds[1] = Since( t0 ) , t0 = Clock::now();
 	const double zero[2] = { 0 , 0 };
 	ls::LevelSet2Dev< Dim >( Dim , tri.nV , tri.V.p , tri.nS , tri.S.p , vals.p , zero , st , lvl );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	I.seconds[2] = Since( t0 ) , t0 = Clock::now();
 	epr_mesh *m = new epr_mesh;
 	m->vdim = 3*Dim , m->k = Dim-1;
-	m->s.resize( lvl.nS*lvl.k );
-	lvl.S.download( m->s.data() , m->s.size() , st );
-	HIPCHECK( hipStreamSynchronize( st ) );
 	try
 	{
+		const bool trimming = I.density && trim>0;
 		std::vector< unsigned > other;
-		std::vector< unsigned char > same , flipped;
-		if( lvl.nS ) LinkFaces< Dim-2 >( lvl.S.p , lvl.nS , lvl.nV , st , other , same );
-		Orient< Dim-2 >( m->s , other , same , flipped );
+		std::vector< unsigned char > flipped;
+		Orient< Dim-2 >( lvl.S.p , lvl.nS , lvl.nV , st , trimming , m->s , flipped , other );
 		I.componentMax.clear();
-		if( I.density && trim>0 ) Trim< Dim >( I , lvl , trim , other , flipped , m->s );
+		if( trimming ) Trim< Dim >( I , lvl , trim , other , flipped , m->s );
 	}
 	catch( ... ){ delete m ; throw; }
-	I.seconds[2] = Since( t0 ) , t0 = Clock::now();
+	I.seconds[3] = Since( t0 ) , t0 = Clock::now();
 	DevBuf< double > out;
 	DevBuf< unsigned > invalid;
 	out.alloc( lvl.nV*3*Dim + 1 ) , invalid.alloc( 1 );
@@ -1020,7 +1160,7 @@ static epr_mesh *Extract( epr_iso &I , bool all , double trim )
 	m->v.resize( lvl.nV*3*Dim );
 	out.download( m->v.data() , m->v.size() , st );
 	HIPCHECK( hipStreamSynchronize( st ) );
-	I.seconds[3] = Since( t0 );
+	I.seconds[4] = Since( t0 );
 	if( bad ){ delete m ; throw std::string( "Leaf node is not a valid element node" ); }
 	return m;
 }
@@ -1203,7 +1343,7 @@ int epr_iso_write_grid( epr_iso *iso , const char *header )
 	return IsoGuard( iso , [&](){ epr::iso::WriteGrid( *iso , header ); } );
 }
 
-double epr_iso_seconds( const epr_iso *iso , int which ){ return iso && which>=0 && which<4 ? iso->seconds[which] : 0.; }
+double epr_iso_seconds( const epr_iso *iso , int which ){ return iso && which>=0 && which<5 ? iso->seconds[which] : 0.; }
 
 unsigned long long epr_indexer_hash( int dim , int adapted ){ return epr::iso::IndexerHash( dim , adapted!=0 ); }
 

@@ -278,7 +278,7 @@ int  epr_iso_level_set( epr_iso *iso , int component , epr_mesh **out );
 int  epr_iso_write_grid( epr_iso *iso , const char *header );
 
 /* Seconds of the last open (0), level-set nodes and triangulation (1), level
- * set and orientation (2), frames (3); no reference counterpart. */
+ * set (2), orientation and trimming (3), frames (4); no reference counterpart. */
 double epr_iso_seconds( const epr_iso *iso , int which );
 void epr_iso_destroy( epr_iso *iso );
 
