@@ -21,7 +21,7 @@ EXPORTS = [
     "epr_mg_vcycle", "epr_mg_relax", "epr_mg_destroy", "epr_adapted_indexer_write",
     "epr_iso_open", "epr_iso_last_error", "epr_iso_info", "epr_iso_values", "epr_iso_level_set_nodes",
     "epr_iso_extract", "epr_iso_level_set", "epr_iso_write_grid", "epr_iso_seconds", "epr_iso_destroy",
-    "epr_indexer_hash", "epr_write_density", "epr_iso_read_density", "epr_iso_extract_trimmed", "epr_iso_component_densities",
+    "epr_indexer_hash", "epr_orient", "epr_write_density", "epr_iso_read_density", "epr_iso_extract_trimmed", "epr_iso_component_densities",
 ]
 
 
@@ -99,6 +99,7 @@ def load():
     L.epr_iso_extract_trimmed.argtypes = [vp, ci, ctypes.c_double, ctypes.POINTER(vp)]
     L.epr_iso_component_densities.argtypes = [vp, sz, dp, ctypes.POINTER(sz)]
     L.epr_indexer_hash.argtypes = [ci, ci]
+    L.epr_orient.argtypes = [ci, sz, up]
     L.epr_indexer_hash.restype = ctypes.c_ulonglong
     L.epr_mesh_info.restype = ctypes.c_int
     L.epr_mesh_copy.argtypes = [vp, dp, up]

@@ -1347,6 +1347,24 @@ double epr_iso_seconds( const epr_iso *iso , int which ){ return iso && which>=0
 
 unsigned long long epr_indexer_hash( int dim , int adapted ){ return epr::iso::IndexerHash( dim , adapted!=0 ); }
 
+int epr_orient( int k , size_t nS , unsigned *simplices )
+{
+	if( ( k!=2 && k!=3 ) || ( nS && !simplices ) ){ epr::ThreadError() = "bad argument" ; return EPR_ERR_ARG; }
+	return epr::ContextFree( [&]()
+	{
+		if( !nS ) return;
+		size_t nV = 0;
+		for( size_t i=0 ; i<nS*k ; i++ ) nV = std::max< size_t >( nV , (size_t)simplices[i]+1 );
+		epr::DevBuf< unsigned > dS;
+		dS.upload( simplices , nS*k , nullptr );
+		std::vector< unsigned > S , other;
+		std::vector< unsigned char > flipped;
+		if( k==2 ) epr::iso::Orient< 1 >( dS.p , nS , nV , nullptr , false , S , flipped , other );
+		else       epr::iso::Orient< 2 >( dS.p , nS , nV , nullptr , false , S , flipped , other );
+		memcpy( simplices , S.data() , S.size()*sizeof(unsigned) );
+	} );
+}
+
 void epr_iso_destroy( epr_iso *iso ){ delete iso; }
 
 }

@@ -95,6 +95,18 @@ class IsoTree:
             pass
 
 
+def orient(simplices):
+    """MarchingSimplices::Orient of segments (n x 2) or triangles (n x 3): a copy, oriented."""
+    from ._lib import EprError as _E
+    S = np.ascontiguousarray(simplices, dtype=np.uint32).copy()
+    if S.ndim != 2 or S.shape[1] not in (2, 3):
+        raise ValueError("simplices must be n x 2 or n x 3")
+    if load().epr_orient(S.shape[1], len(S), S.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))):
+        msg = load().epr_iso_last_error(None)
+        raise _E(msg.decode() if msg else "epr_orient failed")
+    return S
+
+
 def _ply(props, nrows, rows, faces):
     k = faces.shape[1] if faces.ndim == 2 else 0
     head = ["ply", "format binary_little_endian 1.0", f"element vertex {nrows}"]

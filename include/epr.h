@@ -282,6 +282,13 @@ int  epr_iso_write_grid( epr_iso *iso , const char *header );
 double epr_iso_seconds( const epr_iso *iso , int which );
 void epr_iso_destroy( epr_iso *iso );
 
+/* Replaces MarchingSimplices::Orient( simplexIndices ) (Misha/MarchingSimplices.h:179-246)
+ * for k-vertex simplices, k = 2 (segments) or 3 (triangles), in place: the
+ * faces' incidences are paired (a face shared by three simplices is an error)
+ * and each face-connected component is oriented like its first simplex, with
+ * the reference's walk order deciding on non-orientable components. */
+int  epr_orient( int k , size_t nS , unsigned *simplices );
+
 /* typeid( MishaK::Hat::Hierarchical{Regular,Adapted}Indexer< dim > ).hash_code()
  * of a libstdc++ build of the reference: the value a .tree carries after its
  * dimension (ExteriorPoissonRecon.cpp:263). */

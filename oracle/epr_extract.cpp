@@ -605,6 +605,18 @@ const char *orc_extract_error( void ){ return g_err.c_str(); }
 
 unsigned long long orc_indexer_hash( int dim , int adapted ){ return IndexerHash( dim , adapted!=0 ); }
 
+int orc_orient( int k , size_t nS , unsigned *simplices )
+{
+	try
+	{
+		std::vector< unsigned > S( simplices , simplices + nS*k );
+		Orient( k-1 , S );
+		memcpy( simplices , S.data() , S.size()*sizeof(unsigned) );
+		return 0;
+	}
+	catch( const std::string &e ){ g_err = e ; return -3; }
+}
+
 int orc_extract_trimmed( const char *treePath , int all , const char *densityPath , double trim ,
                          size_t *nCells , unsigned **cells , double **values /* (2^D+1)^dim x 2 */ ,
                          size_t *nV , double **V /* nV x 3 dim */ , size_t *nS , unsigned **S /* nS x (dim-1) */ ,

@@ -129,6 +129,8 @@ int    orc_extract_trimmed( const char *treePath , int all , const char *density
 int    orc_extract_level_set( const char *treePath , int component , size_t *nV , double **V , size_t *nS , unsigned **S );
 const char *orc_extract_error( void );
 unsigned long long orc_indexer_hash( int dim , int adapted );
+/* MarchingSimplices::Orient of k-vertex simplices (k = 2, 3) in place */
+int    orc_orient( int k , size_t nS , unsigned *simplices );
 
 #ifdef __cplusplus
 }

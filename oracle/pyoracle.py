@@ -79,6 +79,7 @@ def lib():
                                           ctypes.POINTER(dp)]
         L.orc_extract_error.restype = ctypes.c_char_p
         L.orc_indexer_hash.restype = ctypes.c_ulonglong
+        L.orc_orient.argtypes = [ctypes.c_int, sz, up]
         L.orc_indexer_hash.argtypes = [ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
@@ -337,3 +338,11 @@ def extract_level_set(tree, component):
 
 def indexer_hash(dim, adapted):
     return int(lib().orc_indexer_hash(dim, int(adapted)))
+
+
+def orient(simplices):
+    """The oracle's MarchingSimplices::Orient of n x 2 or n x 3 simplices (a copy)."""
+    S = np.ascontiguousarray(simplices, dtype=np.uint32).copy()
+    if lib().orc_orient(S.shape[1], len(S), S.ctypes.data_as(ctypes.POINTER(ctypes.c_uint))):
+        raise ValueError(lib().orc_extract_error().decode())
+    return S

@@ -154,3 +154,21 @@ def test_gpu_extract_empty_level_set(tmp_path, oracle_lib):
     assert len(o["vertices"]) == 0 and len(o["cells"]) == 0
     assert mesh_ply_bytes(3, V, S).endswith(b"end_header\n")
     iso.close()
+
+
+@pytest.mark.parametrize("case", ["torus", "mobius", "klein", "mixed", "curves"])
+def test_gpu_orient_matches_reference(case):
+    """epr_orient: the device parity solve on orientable components, the host
+    walk on the Moebius strip and the Klein bottle; the reference's own output
+    (tests/golden/orient_cases.npz) in every case."""
+    from conftest import GOLDEN
+    from exteriorpoissonrecon_amd.extract import orient
+    g = np.load(os.path.join(GOLDEN, "orient_cases.npz"))
+    assert np.array_equal(orient(g[case + "_in"]), g[case + "_out"])
+
+
+def test_gpu_orient_rejects_three_incidences():
+    from exteriorpoissonrecon_amd import EprError
+    from exteriorpoissonrecon_amd.extract import orient
+    with pytest.raises(EprError, match="Both faces occupied"):
+        orient(np.array([[0, 1, 2], [1, 0, 3], [0, 1, 4]], dtype=np.uint32))

@@ -80,3 +80,26 @@ def test_oracle_density_trimming_matches_reference(name, tmp_path, oracle_lib):
     printed = g["printed_component_max"]
     assert len(r["component_max"]) == len(printed)
     assert np.all(np.abs(r["component_max"] - printed) <= 5e-6 * np.abs(printed))
+
+
+ORIENT_CASES = ["torus", "mobius", "klein", "mixed", "curves"]
+
+
+@pytest.mark.parametrize("case", ORIENT_CASES)
+def test_oracle_orient_matches_reference(case, oracle_lib):
+    """MarchingSimplices::Orient on shuffled, partly flipped complexes of known
+    topology (tests/golden/orient_cases.npz, written by the reference's own
+    Orient): orientable ones come out consistent, and on the Moebius strip and
+    the Klein bottle the walk's order decides -- both reproduced."""
+    import os
+    from conftest import GOLDEN
+    g = np.load(os.path.join(GOLDEN, "orient_cases.npz"))
+    out = oracle_lib.orient(g[case + "_in"])
+    assert np.array_equal(out, g[case + "_out"])
+    assert oriented(out) == (case in ("torus", "curves"))
+
+
+def test_oracle_orient_rejects_three_incidences(oracle_lib):
+    S = np.array([[0, 1, 2], [1, 0, 3], [0, 1, 4]], dtype=np.uint32)    # edge 0-1 in three triangles
+    with pytest.raises(ValueError, match="Both faces occupied"):
+        oracle_lib.orient(S)
