@@ -6,8 +6,9 @@
  * host pointers; device memory stays inside the context.  Every call returns
  * EPR_OK (0) or a negative status, with a message in epr_last_error().
  *
- * The entry points replace the reference's C++ call boundaries for the dense
- * (--aRadius -1) path; each declaration names the reference interface it
+ * The entry points replace the reference's C++ call boundaries of the
+ * reconstruction (dense grid and adapted hierarchy), the multigrid solver and
+ * the Extract program; each declaration names the reference interface it
  * replaces (paths relative to the reference tree).
  */
 #ifndef EPR_H

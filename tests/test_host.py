@@ -70,6 +70,7 @@ def test_cli_usage_without_input():
     (["--in", "x.txt"], "Only .ply files supported"),
     (["--in", "missing.ply", "--aRadius", "7"], "Only kernel radius in the range [0,5] supported: 7"),
     (["--in", "missing.ply"], "Invalid sample file"),
+    (["--in", "missing.ply", "--project"], "--project is not supported"),
 ])
 def test_cli_errors_exit_zero(args, msg):
     # MK_ERROR_OUT prints to stderr and exits with status 0 (Misha/Exceptions.h:101-108)
