@@ -11,6 +11,9 @@ Workloads (synthetic samples, exteriorpoissonrecon_amd.samples):
   borromean3d_d7  3D Borromean ellipses, 16384 samples + 10% spot noise, depth 7 (default)
   clifford4d_d6   4D Clifford torus, 65536 samples, 2 deg angular noise, depth 6
   knot3d_d5       3D (2,3) torus knot, 1024 samples, depth 5
+  batch3d_256     256 independent 3D inputs (random coprime (p,q) torus knots
+                  or r-rotation spirals, 2048-16384 samples, depths 4/5/6):
+                  a step is epr_build_batch + epr_solve_batch of all 256
 
 --aRadius K (default -1, the dense grid the headline configurations name)
 measures the adapted hierarchy of radius K instead (the reference's default
@@ -51,6 +54,7 @@ WORKLOADS = {
     "borromean3d_d7": dict(dim=3, depth=7, gen=lambda s: s.borromean(16384)),
     "clifford4d_d6": dict(dim=4, depth=6, gen=lambda s: s.clifford_torus(256, ang_deg=2.0)),
     "knot3d_d5": dict(dim=3, depth=5, gen=lambda s: s.torus_knot(1024)),
+    "batch3d_256": dict(dim=3, depth=6, batch=256),
 }
 
 
@@ -116,6 +120,81 @@ def cpu_baseline(name, wl, pos, skew, a_radius=-1):
     return {"value": dt, "unit": "s", "cores": cores, "kind": "reference", "sample": sample}
 
 
+def batch_main(args, wl):
+    """The batched workload: every step builds (epr_build_batch) and solves
+    (epr_solve_batch) all inputs; value = seconds per batch."""
+    from exteriorpoissonrecon_amd import Reconstructor, build_batch, samples, solve_batch
+    inputs = samples.batch_workload(wl["batch"])
+    nsamp = sum(len(p) for p, _, _, _ in inputs)
+    log(f"[bench] {args.config}: {len(inputs)} inputs, {nsamp} samples, depths "
+        f"{sorted(set(d for _, _, d, _ in inputs))}")
+    recs = [Reconstructor(p, s, depth=d, aRadius=args.aRadius) for p, s, d, _ in inputs]
+    E = None
+    for i in range(args.warmup):
+        t0 = time.perf_counter(); build_batch(recs); t1 = time.perf_counter(); E = solve_batch(recs)
+        log(f"[bench] warmup {i}: build {t1 - t0:.3f}s solve {time.perf_counter() - t1:.3f}s")
+    times, builds, solves = [], [], []
+    for i in range(args.steps):
+        t0 = time.perf_counter()
+        build_batch(recs)
+        t1 = time.perf_counter()
+        E = solve_batch(recs)
+        t2 = time.perf_counter()
+        times.append(t2 - t0); builds.append(t1 - t0); solves.append(t2 - t1)
+        log(f"[bench] step {i}: {times[-1]:.4f}s (build {builds[-1]:.4f} solve {solves[-1]:.4f})")
+    t = float(np.mean(times))
+    # algorithmic work of the batched solve: node updates of every level of every input
+    updates = 0
+    for r, (_, _, d, _) in zip(recs, inputs):
+        for l in range(d + 1):
+            updates += r.num_functions(l) * int(np.ceil(2.0 ** (d - l) * 10))
+    achieved = updates * F_NODE[3] / float(np.mean(solves)) / 1e12
+    log(f"[bench] batched solve: {updates:.3e} node updates, {achieved:.2f} TFLOP/s algorithmic over the whole solve")
+    # the same inputs one after another (epr_build + epr_solve each), once
+    t0 = time.perf_counter()
+    Es = []
+    for r in recs:
+        r.build(); Es.append(r.solve())
+    unbatched = time.perf_counter() - t0
+    # separate builds splat with FP64 atomics: the energies agree to rounding, not bit for bit
+    Es = np.array(Es)
+    diff = float(np.max(np.abs(Es.sum(1) - E.sum(1)) / np.abs(Es.sum(1))))
+    log(f"[bench] unbatched: {unbatched:.3f}s for the same {len(recs)} inputs; max relative energy difference to the batch {diff:.2e}")
+    cpu = None
+    if not args.no_cpu:
+        exe = os.path.join(ROOT, "oracle", "_ref", "ExteriorPoissonRecon")
+        if os.path.exists(exe):
+            k = 8
+            with tempfile.TemporaryDirectory() as td:
+                t0 = time.perf_counter()
+                for p, s, d, _ in inputs[:k]:
+                    ply = os.path.join(td, "in.ply")
+                    samples.write_ply(ply, p, s)
+                    subprocess.run([exe, "--in", ply, "--depth", str(d), "--aRadius", str(args.aRadius)],
+                                   capture_output=True, text=True, timeout=1800, check=True)
+                dt = time.perf_counter() - t0
+            cpu = {"value": dt, "unit": "s", "cores": os.cpu_count(), "kind": "reference",
+                   "sample": f"the first {k} of the {len(inputs)} inputs (depths "
+                             f"{[d for _, _, d, _ in inputs[:k]]}), reference CLI one after another, "
+                             f"{os.cpu_count()} threads; x{len(inputs) / k:.0f} = {dt * len(inputs) / k:.1f} s for the batch"}
+            log(f"[bench] cpu baseline {dt:.3f}s ({cpu['sample']})")
+    out = {
+        "metric": METRIC, "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": args.config, "dim": 3, "inputs": len(inputs), "samples": nsamp, "aRadius": args.aRadius,
+                   "depths": {str(d): sum(1 for _, _, dd, _ in inputs if dd == d) for d in (4, 5, 6)},
+                   "build_s": float(np.mean(builds)), "solve_s": float(np.mean(solves)),
+                   "reconstructions_per_s": len(inputs) / t, "unbatched_s": unbatched,
+                   "max_rel_energy_diff_vs_unbatched": diff},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "the whole batched solve (all levels, all inputs; algorithmic node-update flops / solve time)"},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
 def extract_timing(rec, args, dim, reps=3):
     """Extract (Extract.cpp:109-259) on the reconstruction's own .tree: the wall
     time of opening the tree and producing the oriented joint zero set with
@@ -174,6 +253,8 @@ def main():
 
     from exteriorpoissonrecon_amd import Reconstructor, samples
     wl = WORKLOADS[args.config]
+    if "batch" in wl:
+        return batch_main(args, wl)
     pos, skew = wl["gen"](samples)
     dim, depth = wl["dim"], wl["depth"]
     log(f"[bench] {args.config}: {pos.shape[0]} samples, dim {dim}, depth {depth}")

@@ -13,14 +13,20 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <atomic>
+#include <exception>
 #include <functional>
 #include <memory>
+#include <mutex>
+#include <set>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/epr.h"
 #include "epr_adapted.h"
 #include "epr_adapted_dev.h"
+#include "epr_batch.h"
 #include "epr_dev.h"
 #include "epr_hat.h"
 #include "epr_kernels.h"
@@ -534,45 +540,58 @@ struct Ctx : public CtxBase
 	void needBuilt( void ) const { if( !built ) throw std::string( "epr_build has not run" ); }
 
 	// ---------------- solve pieces ----------------
+	// How a level's sweeps run: all iterations in one workgroup (KGsTiny: one
+	// wavefront per element, a 16-lane row per node, all nodes of a colour at
+	// once; KGsSmallLds / KGsSmall: a node group per 2^Dim lanes), or one launch
+	// per colour phase.
+	enum { GS_TINY , GS_SMALL_LDS , GS_SMALL , GS_PHASES };
+	struct GsPlan { int kind , threads , rb ; size_t lds; };
+	GsPlan gsPlan( int l ) const
+	{
+		const long long N = levels[l].N;
+		long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= levels[l].res;
+		long long maxColour = 0;
+		for( int c=0 ; c<C ; c++ ){ ColourSet< Dim > cs ; cs.set( levels[l].res , c ) ; maxColour = std::max( maxColour , cs.total ); }
+		const int maxWaves = TinyMaxThreads< Dim >()/64;
+		const int tinyWaves = (int)std::min< long long >( maxWaves , std::max( ne , ( maxColour+3 )/4 ) );
+		const bool tinyRB = TinyLds< Dim >::Bytes( N , ne , tinyWaves , true ) <= LdsLimit;
+		if( ne<=tinyElements && maxColour<=4*maxWaves && TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB )<=LdsLimit )
+			return { GS_TINY , 64*tinyWaves , tinyRB ? 1 : 0 , TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB ) };
+		if( N<=SmallLevel )
+		{
+			// one workgroup sized to the largest colour, coefficients in LDS (and R, B when they fit)
+			const int threads = (int)std::min< long long >( Dim==3 ? 1024 : 512 , ( maxColour*C + 63 )/64*64 );
+			size_t lds = ( 2*N + W*W )*sizeof(double);
+			const int rb = lds + 2*N*W*sizeof(double) <= LdsLimit ? 1 : 0;
+			if( rb ) lds += 2*N*W*sizeof(double);
+			if( lds<=LdsLimit ) return { GS_SMALL_LDS , threads , rb , lds };
+			return { GS_SMALL , 512 , 0 , 0 };
+		}
+		return { GS_PHASES , 256 , 0 , 0 };
+	}
 	void gs( int l , int iters )
 	{
 		needBuilt() ; checkLevel( l );
 		if( iters<=0 ) return;
 		LevelView< Dim > v = view( l );
-		const long long N = levels[l].N;
-		long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= levels[l].res;
-		long long maxColour = 0;
-		for( int c=0 ; c<C ; c++ ){ ColourSet< Dim > cs ; cs.set( levels[l].res , c ) ; maxColour = std::max( maxColour , cs.total ); }
-		// tiny levels: one wavefront per element, a 16-lane row per node, all nodes of a colour at once
-		const int maxWaves = TinyMaxThreads< Dim >()/64;
-		const int tinyWaves = (int)std::min< long long >( maxWaves , std::max( ne , ( maxColour+3 )/4 ) );
-		const bool tinyRB = TinyLds< Dim >::Bytes( N , ne , tinyWaves , true ) <= LdsLimit;
-		if( ne<=tinyElements && maxColour<=4*maxWaves && TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB )<=LdsLimit )
+		const GsPlan pl = gsPlan( l );
+		if( pl.kind==GS_TINY )
 		{
-			const size_t lds = TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB );
 			if( !tinyAttrSet )
 			{
 				HIPCHECK( hipFuncSetAttribute( (const void*)KGsTiny< Dim , true > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) );
 				HIPCHECK( hipFuncSetAttribute( (const void*)KGsTiny< Dim , false > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) );
 				tinyAttrSet = true;
 			}
-			if( tinyRB ) KGsTiny< Dim , true  ><<< 1 , 64*tinyWaves , lds , stream >>>( v , iters );
-			else         KGsTiny< Dim , false ><<< 1 , 64*tinyWaves , lds , stream >>>( v , iters );
+			if( pl.rb ) KGsTiny< Dim , true  ><<< 1 , pl.threads , pl.lds , stream >>>( v , iters );
+			else        KGsTiny< Dim , false ><<< 1 , pl.threads , pl.lds , stream >>>( v , iters );
 		}
-		else if( N<=SmallLevel )
+		else if( pl.kind==GS_SMALL_LDS )
 		{
-			// one workgroup sized to the largest colour, coefficients in LDS (and R, B when they fit)
-			int threads = (int)std::min< long long >( Dim==3 ? 1024 : 512 , ( maxColour*C + 63 )/64*64 );
-			size_t lds = ( 2*N + W*W )*sizeof(double);
-			int rb = lds + 2*N*W*sizeof(double) <= LdsLimit ? 1 : 0;
-			if( rb ) lds += 2*N*W*sizeof(double);
-			if( lds<=LdsLimit )
-			{
-				if( !ldsAttrSet ){ HIPCHECK( hipFuncSetAttribute( (const void*)KGsSmallLds< Dim > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) ) ; ldsAttrSet = true; }
-				KGsSmallLds< Dim ><<< 1 , threads , lds , stream >>>( v , iters , rb );
-			}
-			else KGsSmall< Dim ><<< 1 , 512 , 0 , stream >>>( v , iters );
+			if( !ldsAttrSet ){ HIPCHECK( hipFuncSetAttribute( (const void*)KGsSmallLds< Dim > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) ) ; ldsAttrSet = true; }
+			KGsSmallLds< Dim ><<< 1 , pl.threads , pl.lds , stream >>>( v , iters , pl.rb );
 		}
+		else if( pl.kind==GS_SMALL ) KGsSmall< Dim ><<< 1 , 512 , 0 , stream >>>( v , iters );
 		else
 		{
 			// the iters*2^Dim colour launches of a level are captured once into a
@@ -621,7 +640,7 @@ struct Ctx : public CtxBase
 		HIPCHECK( hipEventElapsedTime( &t , e0 , e1 ) );
 		HIPCHECK( hipEventDestroy( e0 ) ) ; HIPCHECK( hipEventDestroy( e1 ) );
 		if( ms ) *ms = t;
-		if( launches ) *launches = levels[l].N<=SmallLevel ? 1 : iters*C;
+		if( launches ) *launches = gsPlan( l ).kind==GS_PHASES ? iters*C : 1;
 	}
 
 	// P x (mode 0), out += P x (1) or out -= P x (2).  Adapted levels use the
@@ -1025,6 +1044,331 @@ struct Ctx : public CtxBase
 		fclose( fp );
 	}
 };
+
+// ---------------- many reconstructions ----------------
+// Builds every context (Ctx::build) from up to `threads` host threads, each
+// context's kernels on its own stream; the first failure is rethrown after all
+// threads end, prefixed with the failing context's position.
+inline void BuildBatch( const std::vector< CtxBase * > &cs , int threads )
+{
+	if( cs.empty() ) return;
+	int dev = 0;
+	HIPCHECK( hipGetDevice( &dev ) );
+	const unsigned hw = std::max( 1u , std::thread::hardware_concurrency() );
+	const size_t nt = std::min< size_t >( threads>0 ? (size_t)threads : std::min( 16u , hw ) , cs.size() );
+	std::atomic< size_t > next( 0 );
+	std::atomic< bool > failed( false );
+	std::mutex m;
+	std::exception_ptr first;
+	auto fail = [&]( std::exception_ptr e ){ std::lock_guard< std::mutex > lock( m ) ; if( !first ) first = e ; failed = true; };
+	auto work = [&]( void )
+	{
+		if( hipSetDevice( dev )!=hipSuccess ){ fail( std::make_exception_ptr( HipError{ "hipSetDevice failed in a build thread" } ) ) ; return; }
+		for( size_t i ; !failed && ( i = next++ )<cs.size() ; )
+		{
+			const std::string where = "context " + std::to_string( i ) + ": ";
+			try { cs[i]->build(); }
+			catch( const HipError &e ){ fail( std::make_exception_ptr( HipError{ where + e.msg } ) ); }
+			catch( const std::string &s ){ fail( std::make_exception_ptr( where + s ) ); }
+			catch( const std::exception &e ){ fail( std::make_exception_ptr( where + e.what() ) ); }
+		}
+	};
+	std::vector< std::thread > ts;
+	for( size_t t=1 ; t<nt ; t++ ) ts.emplace_back( work );
+	work();
+	for( auto &t : ts ) t.join();
+	if( first ) std::rethrow_exception( first );
+}
+
+// The cascadic schedule of ExteriorPoisson.h:316-409 for many built contexts
+// of one dimension.  Level by level, the sweeps of every context that solves
+// at that level share launches: on the tiny and small levels one workgroup
+// per context runs all of that context's iterations (the latency-bound coarse
+// levels of different contexts run side by side on different CUs); on the
+// others each colour phase is one grid over that colour's nodes in every
+// context still iterating.  Fills, the random start's copies,
+// prolongations and the hierarchical split are batched the same way.  Every
+// context goes through the kernels and per-node arithmetic of its own
+// Ctx::solve, in the same order, so its coefficients and energies are the
+// same bit for bit.  The --verbose per-level energies are not printed.
+template< int Dim >
+void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
+{
+	using Cx = Ctx< Dim >;
+	using Lv = typename Cx::Level;
+	constexpr int C = DimC< Dim >::C;
+	const size_t n = cs.size();
+	if( !n ) return;
+	if( n>65535 ) throw std::string( "at most 65535 contexts per batch" );
+	{
+		std::set< const Cx * > seen;
+		for( const Cx *c : cs ) if( !seen.insert( c ).second ) throw std::string( "a context appears twice in the batch" );
+	}
+	for( Cx *c : cs ) c->needBuilt();
+	auto t0 = std::chrono::steady_clock::now();
+	hipStream_t st = cs[0]->stream;
+	int maxD = 0;
+	std::vector< int > lo( n ) , hi( n );
+	for( size_t r=0 ; r<n ; r++ ) lo[r] = cs[r]->minD() , hi[r] = cs[r]->maxD() , maxD = std::max( maxD , cs[r]->D );
+	auto Iters = [&]( size_t r , int d ) -> int
+	{
+		const Cx &c = *cs[r];
+		if( d<lo[r] || d>hi[r] ) return 0;
+		if( c.prm.singleLevel ) return d==hi[r] ? c.prm.gsIters : 0;
+		return (int)std::ceil( std::pow( c.prm.iMult , c.D-d ) * c.prm.gsIters );
+	};
+	auto Lev = [&]( size_t r , int d ) -> Lv & { return cs[r]->levels[d]; };
+	long long Nd[ MAXT ];   // nodes of a level (the same in every context)
+	for( int d=0 ; d<=maxD ; d++ ){ Nd[d] = 1 ; for( int k=0 ; k<Dim ; k++ ) Nd[d] *= ( 1LL<<d ) + 1; }
+
+	// ---- job tables, staged on the host and uploaded at once
+	std::vector< char > host;
+	auto put = [&]( const void *p , size_t bytes ){ const size_t o = ( host.size() + 15 ) & ~(size_t)15 ; host.resize( o + bytes ) ; if( bytes ) memcpy( host.data()+o , p , bytes ) ; return o; };
+	struct Jobs { size_t off = 0 , count = 0; };
+	auto zeroJobs = [&]( int d , const std::vector< size_t > &rs , bool x , bool y )
+	{
+		std::vector< BatchZero > z;
+		for( size_t r : rs ){ Lv &L = Lev( r , d ) ; if( x ) z.push_back( { L.x.p , L.N } ) ; if( y ) z.push_back( { L.y.p , L.N } ); }
+		return Jobs{ put( z.data() , z.size()*sizeof(BatchZero) ) , z.size() };
+	};
+	auto prolongJobs = [&]( int d , const std::vector< size_t > &rs , int mode )
+	{
+		std::vector< BatchProlong > j;
+		for( size_t r : rs )
+		{
+			Lv &Lc = Lev( r , d ) , &Lf = Lev( r , d+1 );
+			j.push_back( { Lc.x.p , Lf.x.p , Lf.fm.p , mode } ) , j.push_back( { Lc.y.p , Lf.y.p , Lf.fm.p , mode } );
+		}
+		return Jobs{ put( j.data() , j.size()*sizeof(BatchProlong) ) , j.size() };
+	};
+	auto levelJobs = [&]( int d , const std::vector< size_t > &rs )
+	{
+		std::vector< BatchLevel > b;
+		for( size_t r : rs )
+		{
+			Lv &L = Lev( r , d );
+			BatchLevel e;
+			e.x = L.x.p , e.y = L.y.p , e.R = L.R.p , e.B = L.B.p , e.K = L.K.p , e.fmask = L.fm.p;
+			e.dW = cs[r]->prm.dWeight , e.separate = cs[r]->prm.separate , e.iters = Iters( r , d );
+			b.push_back( e );
+		}
+		return Jobs{ put( b.data() , b.size()*sizeof(BatchLevel) ) , b.size() };
+	};
+
+	// fills of every level, copies of the random start
+	std::vector< Jobs > zeroAll( maxD+1 );
+	for( int d=0 ; d<=maxD ; d++ )
+	{
+		std::vector< size_t > rs;
+		for( size_t r=0 ; r<n ; r++ ) if( d<=cs[r]->D ) rs.push_back( r );
+		zeroAll[d] = zeroJobs( d , rs , true , true );
+	}
+	// the random start of Ctx::init (ExteriorPoisson.h:333-341), drawn in context order
+	std::vector< double > start;
+	std::vector< size_t > startAt( n );
+	for( size_t r=0 ; r<n ; r++ )
+	{
+		Cx &c = *cs[r];
+		Lv &L = Lev( r , lo[r] );
+		std::vector< double > hx( L.N , 0. ) , hy( L.N , 0. );
+		srand( c.prm.seed );
+		if( c.adapted() ){ for( unsigned int f : c.download( L.forder , L.nF ) ) hx[f] = Cx::Random() , hy[f] = Cx::Random(); }
+		else for( long long i=0 ; i<L.N ; i++ ) hx[i] = Cx::Random() , hy[i] = Cx::Random();
+		startAt[r] = start.size();
+		start.insert( start.end() , hx.begin() , hx.end() ) , start.insert( start.end() , hy.begin() , hy.end() );
+	}
+	DevBuf< double > dStart , dScale , eT;
+	dStart.upload( start.data() , start.size() , st );
+	dScale.alloc( n ) , eT.alloc( 4*n );
+	Jobs startCopy;
+	long long maxStart = 1;
+	{
+		std::vector< BatchCopy > cp;
+		for( size_t r=0 ; r<n ; r++ )
+		{
+			Lv &L = Lev( r , lo[r] );
+			cp.push_back( { dStart.p + startAt[r] , L.x.p , L.N } ) , cp.push_back( { dStart.p + startAt[r] + L.N , L.y.p , L.N } );
+			maxStart = std::max( maxStart , L.N );
+		}
+		startCopy = Jobs{ put( cp.data() , cp.size()*sizeof(BatchCopy) ) , cp.size() };
+	}
+
+	// per level: the sweeps, then the prolongation to the next level
+	struct Stage
+	{
+		typename Cx::GsPlan plan;
+		Jobs all;                              // GS_TINY / GS_SMALL_LDS / GS_SMALL: every context sweeping here
+		Jobs dense , adapted , lists;          // GS_PHASES: by decreasing iteration count
+		std::vector< int > denseIters , adaptedIters;
+		std::vector< std::vector< long long > > listCount;   // [context of `adapted`][colour]
+		std::vector< size_t > smallRs;         // GS_SMALL: the contexts, launched one by one
+		Jobs prolongAdd , prolongSet , zeroAfter;
+	};
+	std::vector< Stage > stages( maxD+1 );
+	for( int d=0 ; d<=maxD ; d++ )
+	{
+		Stage &S = stages[d];
+		std::vector< size_t > rs;
+		for( size_t r=0 ; r<n ; r++ ) if( d<=cs[r]->D && Iters( r , d )>0 ) rs.push_back( r );
+		if( !rs.empty() )
+		{
+			S.plan = cs[ rs[0] ]->gsPlan( d );
+			if( S.plan.kind==Cx::GS_PHASES )
+			{
+				std::stable_sort( rs.begin() , rs.end() , [&]( size_t a , size_t b ){ return Iters( a , d )>Iters( b , d ); } );
+				std::vector< size_t > rd , ra;
+				for( size_t r : rs ) ( cs[r]->adapted() ? ra : rd ).push_back( r );
+				S.dense = levelJobs( d , rd ) , S.adapted = levelJobs( d , ra );
+				for( size_t r : rd ) S.denseIters.push_back( Iters( r , d ) );
+				for( size_t r : ra ) S.adaptedIters.push_back( Iters( r , d ) );
+				std::vector< BatchList > ls;
+				for( size_t r : ra )
+				{
+					Lv &L = Lev( r , d );
+					S.listCount.emplace_back( C );
+					for( int c=0 ; c<C ; c++ ){ const long long cnt = L.actStart[c+1]-L.actStart[c] ; ls.push_back( { L.act.p + L.actStart[c] , cnt } ) ; S.listCount.back()[c] = cnt; }
+				}
+				S.lists = Jobs{ put( ls.data() , ls.size()*sizeof(BatchList) ) , ls.size() };
+			}
+			else if( S.plan.kind==Cx::GS_SMALL ) S.smallRs = rs;
+			else S.all = levelJobs( d , rs );
+		}
+		if( d<maxD )
+		{
+			std::vector< size_t > add , set;
+			for( size_t r=0 ; r<n ; r++ ) if( d>=lo[r] && d<hi[r] ) ( cs[r]->prm.singleLevel ? set : add ).push_back( r );
+			S.prolongAdd = prolongJobs( d , add , 1 ) , S.prolongSet = prolongJobs( d , set , 0 );
+			S.zeroAfter = zeroJobs( d , set , true , true );   // --singleLevel: the coarser levels end at zero
+		}
+	}
+	// the hierarchical split after the solve: levels above the last solved one
+	// are prolongations (=), then every level keeps only its own detail (-=)
+	std::vector< Jobs > splitSet( maxD+1 ) , splitSub( maxD+1 );
+	for( int d=0 ; d<maxD ; d++ )
+	{
+		std::vector< size_t > rs;
+		for( size_t r=0 ; r<n ; r++ ) if( d>=hi[r] && d<cs[r]->D ) rs.push_back( r );
+		splitSet[d] = prolongJobs( d , rs , 0 );
+	}
+	for( int d=maxD ; d>0 ; d-- )
+	{
+		std::vector< size_t > rs;
+		for( size_t r=0 ; r<n ; r++ ) if( d<=cs[r]->D ) rs.push_back( r );
+		splitSub[d] = prolongJobs( d-1 , rs , 2 );
+	}
+	DevBuf< char > table;
+	table.alloc( std::max< size_t >( host.size() , 16 ) );
+	HIPCHECK( hipMemcpyAsync( table.p , host.data() , host.size() , hipMemcpyHostToDevice , st ) );
+	auto at = [&]( const Jobs &j , size_t k=0 ){ return (const void *)( table.p + j.off + k ); };
+
+	auto launchZero = [&]( const Jobs &j , int d )
+	{
+		if( j.count ) KZeroBatch<<< dim3( (unsigned)std::min< long long >( Blocks( Nd[d] , 256 ) , 1024 ) , (unsigned)j.count ) , 256 , 0 , st >>>( (const BatchZero *)at( j ) );
+	};
+	auto launchProlong = [&]( const Jobs &j , int d )
+	{
+		if( j.count ) KProlongBatch< Dim ><<< dim3( Blocks( Nd[d+1] , 256 ) , (unsigned)j.count ) , 256 , 0 , st >>>( (const BatchProlong *)at( j ) , 1<<d );
+	};
+	// (W, yBx, R-form, K-form) of context r at level d into eT[4r..4r+3] (Ctx::energyTerms)
+	std::vector< double > hT( 4*n );
+	auto terms = [&]( const std::vector< int > &level )
+	{
+		for( size_t r=0 ; r<n ; r++ )
+		{
+			Cx &c = *cs[r];
+			const unsigned int nb = Blocks( Lev( r , level[r] ).N*C , 256 );
+			c.dPartials.ensure( (size_t)nb*4 );
+			KEnergy< Dim ><<< nb , 256 , 0 , st >>>( c.view( level[r] ) , c.dPartials.p );
+			KSumRows<4><<< 1 , 256 , 0 , st >>>( c.dPartials.p , nb , eT.p + 4*r );
+		}
+		HIPCHECK( hipGetLastError() );
+		HIPCHECK( hipMemcpyAsync( hT.data() , eT.p , 4*n*sizeof(double) , hipMemcpyDeviceToHost , st ) );
+		HIPCHECK( hipStreamSynchronize( st ) );
+	};
+
+	// ---- zero, random start, rescale s = ( c / W(x,y) )^(1/4)
+	for( int d=0 ; d<=maxD ; d++ ) launchZero( zeroAll[d] , d );
+	KCopyBatch<<< dim3( (unsigned)std::min< long long >( Blocks( maxStart , 256 ) , 1024 ) , (unsigned)startCopy.count ) , 256 , 0 , st >>>( (const BatchCopy *)at( startCopy ) );
+	terms( lo );
+	{
+		std::vector< double > s( n );
+		for( size_t r=0 ; r<n ; r++ ) s[r] = std::sqrt( std::sqrt( cs[r]->c_ / hT[4*r] ) );
+		dScale.upload( s.data() , n , st );
+		for( size_t r=0 ; r<n ; r++ ){ Lv &L = Lev( r , lo[r] ) ; KScale2<<< Blocks( L.N , 256 ) , 256 , 0 , st >>>( L.x.p , L.y.p , L.N , dScale.p + r ); }
+	}
+	// ---- the levels
+	for( int d=0 ; d<=maxD ; d++ )
+	{
+		Stage &S = stages[d];
+		// any context with this level gives the shared constants (resolution, strides, layout, wedge scale)
+		size_t rAny = 0;
+		while( cs[rAny]->D<d ) rAny++;
+		const LevelView< Dim > L0 = cs[rAny]->view( d );
+		if( S.all.count )
+		{
+			if( S.plan.kind==Cx::GS_TINY )
+			{
+				if( S.plan.rb ){ HIPCHECK( hipFuncSetAttribute( (const void*)KGsTinyBatch< Dim , true > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)Cx::LdsLimit ) ) ; KGsTinyBatch< Dim , true ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) ); }
+				else { HIPCHECK( hipFuncSetAttribute( (const void*)KGsTinyBatch< Dim , false > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)Cx::LdsLimit ) ) ; KGsTinyBatch< Dim , false ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) ); }
+			}
+			else
+			{
+				HIPCHECK( hipFuncSetAttribute( (const void*)KGsSmallLdsBatch< Dim > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)Cx::LdsLimit ) );
+				KGsSmallLdsBatch< Dim ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) , S.plan.rb );
+			}
+		}
+		for( size_t r : S.smallRs ) KGsSmall< Dim ><<< 1 , 512 , 0 , st >>>( cs[r]->view( d ) , Iters( r , d ) );
+		if( S.dense.count )
+		{
+			size_t k = S.dense.count;
+			for( int it=0 ; it<S.denseIters[0] ; it++ )
+			{
+				while( k && S.denseIters[k-1]<=it ) k--;
+				for( int c=0 ; c<C ; c++ )
+				{
+					ColourSet< Dim > cset ; cset.set( 1<<d , c );
+					KGsPhaseBatch< Dim ><<< dim3( Blocks( cset.total*C , 256 ) , (unsigned)k ) , 256 , 0 , st >>>( L0 , (const BatchLevel *)at( S.dense ) , cset , L0.lay.cstart[c] );
+				}
+			}
+		}
+		if( S.adapted.count )
+		{
+			size_t k = S.adapted.count;
+			for( int it=0 ; it<S.adaptedIters[0] ; it++ )
+			{
+				while( k && S.adaptedIters[k-1]<=it ) k--;
+				for( int c=0 ; c<C ; c++ )
+				{
+					long long mx = 0;
+					for( size_t q=0 ; q<k ; q++ ) mx = std::max( mx , S.listCount[q][c] );
+					if( mx ) KGsPhaseListBatch< Dim ><<< dim3( Blocks( mx*C , 256 ) , (unsigned)k ) , 256 , 0 , st >>>( L0 , (const BatchLevel *)at( S.adapted ) , (const BatchList *)at( S.lists ) , c );
+				}
+			}
+		}
+		if( d<maxD )
+		{
+			launchProlong( S.prolongAdd , d );
+			launchProlong( S.prolongSet , d );
+			launchZero( S.zeroAfter , d );
+		}
+	}
+	HIPCHECK( hipGetLastError() );
+	// ---- final energies (Ctx::energies), then the hierarchical split
+	terms( hi );
+	for( size_t r=0 ; r<n ; r++ )
+	{
+		const double *t = &hT[4*r];
+		double e0 = t[0] - 2.*t[1] + cs[r]->c_ , e1 = t[2];
+		if( cs[r]->prm.dWeight ) e1 += t[3]*cs[r]->prm.dWeight;
+		if( energies ) energies[2*r] = e0 , energies[2*r+1] = e1;
+	}
+	for( int d=0 ; d<maxD ; d++ ) launchProlong( splitSet[d] , d );
+	for( int d=maxD ; d>0 ; d-- ) launchProlong( splitSub[d] , d-1 );
+	HIPCHECK( hipGetLastError() );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	const double sec = std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count();
+	for( Cx *c : cs ) c->solved = true , c->seconds[1] = sec;
+}
 } // namespace epr
 
 struct epr_ctx
@@ -1147,6 +1491,43 @@ int epr_time_copy( size_t bytes , int reps , double *ms )
 	if( a ) (void)hipFree( a );
 	if( b ) (void)hipFree( b );
 	return rc;
+}
+static std::vector< epr::CtxBase * > BatchImpls( size_t n , epr_ctx *const *ctxs , bool oneDim , int &dim )
+{
+	if( n && !ctxs ) throw std::string( "null context array" );
+	std::vector< epr::CtxBase * > v;
+	std::set< const epr_ctx * > seen;
+	dim = 0;
+	for( size_t i=0 ; i<n ; i++ )
+	{
+		if( !ctxs[i] || !ctxs[i]->impl ) throw std::string( "context " + std::to_string( i ) + " is null or not initialised" );
+		if( !seen.insert( ctxs[i] ).second ) throw std::string( "a context appears twice in the batch" );
+		if( oneDim && dim && ctxs[i]->dim!=dim ) throw std::string( "contexts of different dimensions in one batch" );
+		dim = ctxs[i]->dim;
+		v.push_back( ctxs[i]->impl.get() );
+	}
+	return v;
+}
+int epr_build_batch( size_t n , epr_ctx *const *ctxs , int threads )
+{
+	return epr::ContextFree( [&](){ int dim ; epr::BuildBatch( BatchImpls( n , ctxs , false , dim ) , threads ); } );
+}
+int epr_solve_batch( size_t n , epr_ctx *const *ctxs , double *energies )
+{
+	return epr::ContextFree( [&]()
+	{
+		int dim;
+		std::vector< epr::CtxBase * > v = BatchImpls( n , ctxs , true , dim );
+		auto run = [&]( auto tag )
+		{
+			constexpr int Dim = decltype( tag )::value;
+			std::vector< epr::Ctx< Dim > * > c;
+			for( epr::CtxBase *b : v ) c.push_back( static_cast< epr::Ctx< Dim > * >( b ) );
+			epr::SolveBatch< Dim >( c , energies );
+		};
+		if( dim==3 ) run( std::integral_constant< int , 3 >() );
+		else if( dim==4 ) run( std::integral_constant< int , 4 >() );
+	} );
 }
 double epr_last_seconds( const epr_ctx *ctx , int which ){ return ( ctx && ctx->impl && ( which==0 || which==1 ) ) ? ctx->impl->seconds[which] : -1.; }
 

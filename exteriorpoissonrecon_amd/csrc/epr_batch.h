@@ -1,0 +1,84 @@
+// Kernels of the batched solve: the same level of many independent
+// reconstructions in one launch.  All reconstructions of a launch share the
+// level's resolution (hence its LevelView constants); what differs per
+// reconstruction -- its buffers, Dirichlet weight, step kind and iteration
+// count -- comes from a BatchLevel table in device memory.
+#pragma once
+#include "epr_kernels.h"
+
+namespace epr
+{
+struct BatchLevel
+{
+	double *x , *y;
+	const double *R , *B , *K;
+	const unsigned char *fmask;
+	double dW;
+	int separate , iters;
+};
+struct BatchList { const unsigned int *list ; long long count; };   // the active nodes of one colour (adapted levels)
+struct BatchProlong { const double *xc ; double *xf ; const unsigned char *fmaskF ; int mode; };
+struct BatchZero { double *p ; long long n; };
+
+template< int Dim >
+__device__ __forceinline__ LevelView< Dim > BatchView( const LevelView< Dim > &L0 , const BatchLevel &b )
+{
+	LevelView< Dim > L = L0;
+	L.x = b.x , L.y = b.y , L.R = b.R , L.B = b.B , L.K = b.K , L.fmask = b.fmask , L.dW = b.dW , L.separate = b.separate;
+	return L;
+}
+
+// one workgroup per reconstruction, all of its iterations (KGsTiny)
+template< int Dim , bool RB >
+__global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTinyBatch( LevelView< Dim > L0 , const BatchLevel *P )
+{
+	const BatchLevel &b = P[ blockIdx.x ];
+	GsTiny< Dim , RB >( BatchView( L0 , b ) , b.iters );
+}
+// one workgroup per reconstruction, all of its iterations (KGsSmallLds)
+template< int Dim >
+__global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLdsBatch( LevelView< Dim > L0 , const BatchLevel *P , int rbInLds )
+{
+	const BatchLevel &b = P[ blockIdx.x ];
+	GsSmallLds< Dim >( BatchView( L0 , b ) , b.iters , rbInLds );
+}
+// one colour phase of the first gridDim.y reconstructions; the workgroups of
+// one XCD take a contiguous range of (reconstruction, block) pairs
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGsPhaseBatch( LevelView< Dim > L0 , const BatchLevel *P , ColourSet< Dim > cs , long long colBase )
+{
+	const unsigned lin = XcdBlock( blockIdx.y*gridDim.x + blockIdx.x , gridDim.x*gridDim.y );
+	const unsigned r = lin / gridDim.x , b = lin - r*gridDim.x;
+	GsPhase< Dim >( BatchView( L0 , P[r] ) , cs , colBase , b );
+}
+// the adapted counterpart: colour c's active-node list of reconstruction r at lists[ r*2^Dim + c ]
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGsPhaseListBatch( LevelView< Dim > L0 , const BatchLevel *P , const BatchList *lists , int c )
+{
+	constexpr int C = DimC< Dim >::C;
+	const unsigned lin = XcdBlock( blockIdx.y*gridDim.x + blockIdx.x , gridDim.x*gridDim.y );
+	const unsigned r = lin / gridDim.x , b = lin - r*gridDim.x;
+	const BatchList &l = lists[ (size_t)r*C + c ];
+	GsPhaseList< Dim >( BatchView( L0 , P[r] ) , l.list , l.count , b );
+}
+// prolongation job blockIdx.y (KProlong)
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KProlongBatch( const BatchProlong *J , int cres )
+{
+	const BatchProlong &j = J[ blockIdx.y ];
+	Prolong< Dim >( j.xc , j.xf , cres , j.mode , j.fmaskF , (long long)blockIdx.x*blockDim.x + threadIdx.x );
+}
+// copy of job blockIdx.y
+struct BatchCopy { const double *src ; double *dst ; long long n; };
+__global__ void __launch_bounds__( 256 ) KCopyBatch( const BatchCopy *J )
+{
+	const BatchCopy &j = J[ blockIdx.y ];
+	for( long long i=(long long)blockIdx.x*blockDim.x + threadIdx.x ; i<j.n ; i+=(long long)gridDim.x*blockDim.x ) j.dst[i] = j.src[i];
+}
+// zero-fill of job blockIdx.y
+__global__ void __launch_bounds__( 256 ) KZeroBatch( const BatchZero *J )
+{
+	const BatchZero &j = J[ blockIdx.y ];
+	for( long long i=(long long)blockIdx.x*blockDim.x + threadIdx.x ; i<j.n ; i+=(long long)gridDim.x*blockDim.x ) j.p[i] = 0;
+}
+} // namespace epr

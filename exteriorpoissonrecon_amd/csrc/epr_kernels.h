@@ -326,12 +326,13 @@ struct ColourSet
 template< int Dim > __device__ __forceinline__ long long FlatIndex( const int *F , const long long *stride ){ long long i = 0 ; for( int d=0 ; d<Dim ; d++ ) i += F[d]*stride[d] ; return i; }
 template< int Dim > __device__ __forceinline__ void Unflatten( long long i , int res , int *F ){ for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = (int)( i % (res+1) ) ; i /= res+1; } }
 
-// One colour phase of the paired Gauss-Seidel sweep (one launch per colour).
+// One colour phase of the paired Gauss-Seidel sweep (one launch per colour);
+// block b of the phase.
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSet< Dim > cs , long long colBase )
+__device__ __forceinline__ void GsPhase( const LevelView< Dim > &L , const ColourSet< Dim > &cs , long long colBase , unsigned b )
 {
 	constexpr int C = DimC< Dim >::C;
-	const long long g = (long long)XcdBlock( blockIdx.x , gridDim.x )*blockDim.x + threadIdx.x;
+	const long long g = (long long)b*blockDim.x + threadIdx.x;
 	const long long k = g / C;
 	const int lane = (int)( g % C );
 	if( k>=cs.total ) return;
@@ -347,14 +348,16 @@ __global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSe
 		if( LevelStep< Dim >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
 	}
 }
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSet< Dim > cs , long long colBase ){ GsPhase< Dim >( L , cs , colBase , XcdBlock( blockIdx.x , gridDim.x ) ); }
 
 // One colour phase over a list of nodes (the active nodes of one colour of an
-// adapted level, dense indices in increasing order).
+// adapted level, dense indices in increasing order); block b of the phase.
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KGsPhaseList( LevelView< Dim > L , const unsigned int *list , long long count )
+__device__ __forceinline__ void GsPhaseList( const LevelView< Dim > &L , const unsigned int *list , long long count , unsigned b )
 {
 	constexpr int C = DimC< Dim >::C;
-	const long long g = (long long)XcdBlock( blockIdx.x , gridDim.x )*blockDim.x + threadIdx.x;
+	const long long g = (long long)b*blockDim.x + threadIdx.x;
 	const long long k = g / C;
 	const int lane = (int)( g % C );
 	if( k>=count ) return;
@@ -374,6 +377,8 @@ __global__ void __launch_bounds__( 256 ) KGsPhaseList( LevelView< Dim > L , cons
 		if( LevelStep< Dim >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
 	}
 }
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGsPhaseList( LevelView< Dim > L , const unsigned int *list , long long count ){ GsPhaseList< Dim >( L , list , count , XcdBlock( blockIdx.x , gridDim.x ) ); }
 
 // All iterations of a small level in one workgroup (barrier between colours).
 template< int Dim >
@@ -481,11 +486,10 @@ __global__ void __launch_bounds__( 256 ) KPartialSum( const double *in , long lo
 // order (Eigen's column-major SpMV of SystemEnergy.h:162; weights of
 // Hat.scalars.inl:373-386).
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf , int cres , int mode , const unsigned char *fmaskF )
+__device__ __forceinline__ void Prolong( const double *xc , double *xf , int cres , int mode , const unsigned char *fmaskF , long long f )
 {
 	const int fres = 2*cres;
 	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= fres+1;
-	const long long f = (long long)blockIdx.x*blockDim.x + threadIdx.x;
 	if( f>=nf ) return;
 	// adapted levels: the prolongation has rows only for the fine functions that are indexed (Hat.scalars.inl:1694-1744)
 	if( fmaskF && !fmaskF[f] ){ if( mode==0 ) xf[f] = 0 ; return; }
@@ -511,6 +515,8 @@ __global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf
 	else if( mode==1 ) xf[f] = xf[f] + s;
 	else xf[f] = xf[f] - s;
 }
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf , int cres , int mode , const unsigned char *fmaskF ){ Prolong< Dim >( xc , xf , cres , mode , fmaskF , (long long)blockIdx.x*blockDim.x + threadIdx.x ); }
 
 // out = a + b
 // y = M x for a level operator stored by columns (column f holds M(f+o_w, f)):
@@ -1326,7 +1332,7 @@ __global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *M
 // All iterations of a small level in one workgroup with the coefficients (and,
 // when they fit, the R and B columns) resident in LDS; barrier between colours.
 template< int Dim >
-__global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLds( LevelView< Dim > L , int iters , int rbInLds )
+__device__ __forceinline__ void GsSmallLds( const LevelView< Dim > &L , int iters , int rbInLds )
 {
 	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W;
 	extern __shared__ double sm[];
@@ -1346,7 +1352,8 @@ __global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLds( LevelView<
 		Ls.R = sR , Ls.B = sB;
 	}
 	__syncthreads();
-	for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+	// colour c's first column, lay.cstart[c], kept as a running sum (no dynamic index into the view)
+	for( int it=0 ; it<iters ; it++ ) for( int c=0 , colStart=0 ; c<C ; c++ )
 	{
 		ColourSet< Dim > cs ; cs.set( L.res , c );
 		for( long long g=threadIdx.x ; g<cs.total*C ; g+=blockDim.x )
@@ -1357,17 +1364,20 @@ __global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLds( LevelView<
 			cs.node( k , F );
 			const long long i = FlatIndex< Dim >( F , L.stride );
 			double q[5];
-			NodeGroup< Dim , MODE_GS >( Ls , F , i , L.lay.cstart[c]+k , lane , q );
+			NodeGroup< Dim , MODE_GS >( Ls , F , i , colStart+k , lane , q );
 			if( lane==0 && ( !L.fmask || L.fmask[i]==1 ) )
 			{
 				double dx , dy;
 				if( LevelStep< Dim >( L , q , dx , dy ) ) sx[i] += dx , sy[i] += dy;
 			}
 		}
+		colStart += (int)cs.total;
 		__syncthreads();
 	}
 	for( long long i=threadIdx.x ; i<L.N ; i+=blockDim.x ) L.x[i] = sx[i] , L.y[i] = sy[i];
 }
+template< int Dim >
+__global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLds( LevelView< Dim > L , int iters , int rbInLds ){ GsSmallLds< Dim >( L , iters , rbInLds ); }
 
 // ---- Tiny levels (a few elements per colour phase), one workgroup for all
 // iterations.  Every element has exactly one corner of each colour, so a colour
@@ -1457,7 +1467,7 @@ template< int Dim > struct TinyLds
 
 // Requires: res a power of two, at most 4 * (blockDim/64) nodes per colour.
 template< int Dim , bool RB >
-__global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView< Dim > L , int iters )
+__device__ __forceinline__ void GsTiny( const LevelView< Dim > &L , int iters )
 {
 	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W , J = C-1 , P = ( C*(C-1) )/2;
 	constexpr int NCH = TinyShape< Dim >::NCH , QC = TinyShape< Dim >::QC;
@@ -1667,6 +1677,8 @@ __global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView<
 	}
 	for( int i=threadIdx.x ; i<N ; i+=blockDim.x ) L.x[i] = sx[i] , L.y[i] = sy[i];
 }
+template< int Dim , bool RB >
+__global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView< Dim > L , int iters ){ GsTiny< Dim , RB >( L , iters ); }
 
 // mean of the per-sample measures (deterministic): out[1] = sum / n, out[0] = sWeight
 __global__ void KMeanMeasure( const double *partials , int nb , long long n , double sWeight , double *out )

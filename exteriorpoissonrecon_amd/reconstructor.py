@@ -135,3 +135,35 @@ class Reconstructor:
 
     def seconds(self, which):
         return load().epr_last_seconds(self._h, 0 if which == "build" else 1)
+
+
+def _handles(recs):
+    hs = [r._h for r in recs]
+    if any(not h for h in hs):
+        raise EprError("a reconstruction in the batch is closed")
+    return (ctypes.c_void_p * len(hs))(*hs)
+
+
+def _batch_check(rc):
+    if rc != 0:
+        msg = load().epr_last_error(None)
+        raise EprError(f"epr error {rc}: {msg.decode() if msg else ''}")
+
+
+def build_batch(recs, threads=0):
+    """Reconstructor.build() of every reconstruction, from up to `threads` host
+    threads (0: one per hardware thread, at most 16), each on its own stream."""
+    recs = list(recs)
+    _batch_check(load().epr_build_batch(len(recs), _handles(recs), int(threads)))
+    return recs
+
+
+def solve_batch(recs):
+    """Reconstructor.solve() of every (built) reconstruction of one dimension,
+    with each level's sweeps of all of them in shared launches.  Every
+    reconstruction ends with the coefficients and energies its own solve()
+    gives.  Returns an n x 2 array of (E_fit, E_reg)."""
+    recs = list(recs)
+    E = np.zeros((len(recs), 2))
+    _batch_check(load().epr_solve_batch(len(recs), _handles(recs), _p(E)))
+    return E

@@ -46,6 +46,42 @@ def torus_knot(n, p=2, q=3, seed=0):
     return pos, dual3(_unit(tan))
 
 
+def spiral(n, r=3, seed=0):
+    """Spiral with r rotations (cos 2 pi r s, 2 s - 1, sin 2 pi r s), s ~ U[0,1)
+    (the curve of the reference's spiral:r sampler, Samples.h:321-331)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s = rng.uniform(0.0, 1.0, n)
+    X = 2.0 * np.pi * r
+    pos = np.stack([np.cos(X * s), -1.0 + 2.0 * s, np.sin(X * s)], axis=-1)
+    tan = np.stack([-X * np.sin(X * s), np.full(n, 2.0), X * np.cos(X * s)], axis=-1)
+    return pos, dual3(_unit(tan))
+
+
+def batch_workload(count=256, seed=0, depths=(4, 5, 6)):
+    """The batched serving workload: `count` independent 3D inputs, each a
+    torus knot with random coprime (p, q) in [2,7]^2 or a spiral with r in
+    [2,6] rotations, 2048-16384 samples, depth uniform over `depths`.
+    Returns [(pos, skew, depth, description)]."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for k in range(count):
+        n = int(rng.integers(2048, 16385))
+        depth = int(rng.choice(depths))
+        if rng.random() < 0.5:
+            while True:
+                p, q = (int(v) for v in rng.integers(2, 8, 2))
+                if np.gcd(p, q) == 1:
+                    break
+            pos, skew = torus_knot(n, p, q, seed=seed * 100003 + k)
+            desc = f"torus_knot:{p}:{q}"
+        else:
+            r = int(rng.integers(2, 7))
+            pos, skew = spiral(n, r, seed=seed * 100003 + k)
+            desc = f"spiral:{r}"
+        out.append((pos, skew, depth, desc))
+    return out
+
+
 def _rotation(rng, axis, angle):
     axis = _unit(axis)
     k = np.zeros(axis.shape[:-1] + (3, 3))

@@ -104,6 +104,25 @@ int  epr_energies( epr_ctx *ctx , int level , double out[2] );
  * (E_fit, E_reg) at maxSolveDepth. */
 int  epr_solve( epr_ctx *ctx , double out_E[2] );
 
+/* Many independent reconstructions.  No counterpart in the reference, which
+ * runs one reconstruction per process (ExteriorPoissonRecon.cpp:201-279);
+ * these are for callers with many small inputs, whose coarse levels alone
+ * would leave most of the GPU idle.  Context-free errors (epr_last_error(NULL)).
+ *
+ * epr_build_batch: epr_build on each of n contexts from up to `threads` host
+ * threads (0: one per hardware thread, at most 16), each context's kernels on
+ * its own stream.
+ *
+ * epr_solve_batch: epr_solve on each of n built contexts of one dimension,
+ * with every level's sweeps of all contexts in shared launches (one
+ * workgroup per context on the coarse levels, one grid over all contexts'
+ * nodes of a colour on the others).  Each context ends with exactly the
+ * coefficients and energies its own epr_solve gives; energies (may be NULL)
+ * receives n x 2 values (E_fit, E_reg).  --verbose prints nothing here.  The
+ * batch runs on the first context's stream; at most 65535 contexts. */
+int  epr_build_batch( size_t n , epr_ctx *const *ctxs , int threads );
+int  epr_solve_batch( size_t n , epr_ctx *const *ctxs , double *energies );
+
 /* Number of hat functions (2^level+1)^dim of a level (the dense layout in
  * which coefficients are exchanged, also for adapted reconstructions). */
 size_t epr_num_functions( const epr_ctx *ctx , int level );

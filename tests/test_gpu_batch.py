@@ -1,0 +1,135 @@
+"""Many reconstructions at once (epr_build_batch / epr_solve_batch).
+
+Bar: every reconstruction of a batch ends with the coefficients (every level,
+hierarchical) and energies its own solve() gives on the same build, bit for
+bit -- the batch runs the same kernels on the same data in the same per-node
+order.  Builds splat with FP64 atomics, so two builds of one input differ in
+the last bits: a batch build is held to a separate build of the same input
+within 1e-9 relative in the final energy.  One batch member also matches the
+oracle's solve within the solve tolerance of the single-reconstruction tests
+(1e-6 relative, the energy bar)."""
+import numpy as np
+import pytest
+
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL_SOLVE = 1e-6     # the energy and (phi1, phi2) bar of the single-reconstruction solve tests
+
+
+def _inputs3():
+    from exteriorpoissonrecon_amd import samples
+    return [
+        (samples.torus_knot(2048, 2, 3, seed=1), dict(depth=4, aRadius=-1)),
+        (samples.spiral(3000, 3, seed=2), dict(depth=5, aRadius=-1, seed=7)),
+        (samples.torus_knot(4096, 3, 5, seed=3), dict(depth=6, aRadius=-1, separate=1)),
+        (samples.spiral(2500, 5, seed=4), dict(depth=3, aRadius=-1)),
+        (samples.torus_knot(3000, 2, 5, seed=5), dict(depth=5, aRadius=-1, minSolveDepth=2, gsIters=6)),
+        (samples.torus_knot(2048, 3, 4, seed=6), dict(depth=5, aRadius=-1, maxSolveDepth=3)),
+        (samples.spiral(2048, 2, seed=7), dict(depth=4, aRadius=-1, singleLevel=1)),
+        (samples.torus_knot(4096, 2, 7, seed=8), dict(depth=5, aRadius=3)),
+        (samples.spiral(4096, 4, seed=9), dict(depth=6, aRadius=2)),
+        (samples.torus_knot(2048, 5, 3, seed=10), dict(depth=4, aRadius=3, iMult=1.5, dWeight=0.01)),
+    ]
+
+
+def _solve_each(recs):
+    """solve() of every (built) reconstruction, one after another."""
+    out = []
+    for r in recs:
+        E = r.solve()
+        out.append((E, [r.coefficients(d) for d in range(r.depth + 1)]))
+    return out
+
+
+def _separate_builds(inputs):
+    from exteriorpoissonrecon_amd import Reconstructor
+    out = []
+    for (pos, skew), prm in inputs:
+        r = Reconstructor(pos, skew, **prm).build()
+        out.append(r.solve())
+        r.close()
+    return out
+
+
+def _check_same(recs, E, ref):
+    for k, (r, (Ek, coeffs)) in enumerate(zip(recs, ref)):
+        assert np.array_equal(E[k], Ek), (k, E[k], Ek)
+        for d in range(r.depth + 1):
+            x, y = r.coefficients(d)
+            assert np.array_equal(x, coeffs[d][0]) and np.array_equal(y, coeffs[d][1]), (k, d)
+
+
+def test_gpu_batch_3d_matches_single_solves():
+    """Dense and adapted, mixed depths 3-6, --separate, --singleLevel, min/max
+    solve depths, iteration multipliers and Dirichlet weights in one batch."""
+    from exteriorpoissonrecon_amd import Reconstructor, build_batch, solve_batch
+    inputs = _inputs3()
+    print("case batch3d", len(inputs), flush=True)
+    recs = [Reconstructor(pos, skew, **prm) for (pos, skew), prm in inputs]
+    build_batch(recs, threads=4)
+    ref = _solve_each(recs)
+    E = solve_batch(recs)
+    _check_same(recs, E, ref)
+    # solving the same batch again starts from the same random guess
+    E2 = solve_batch(recs)
+    assert np.array_equal(E, E2)
+    for Ek, Es in zip(E, _separate_builds(inputs)):
+        assert rel_err(Ek, Es) < 1e-9, (Ek, Es)
+    for r in recs:
+        r.close()
+
+
+def test_gpu_batch_4d_matches_single_solves():
+    from exteriorpoissonrecon_amd import Reconstructor, build_batch, samples, solve_batch
+    inputs = [
+        (samples.clifford_torus(side=24, seed=1), dict(depth=3, aRadius=-1)),
+        (samples.clifford_torus(side=32, seed=2), dict(depth=4, aRadius=-1)),
+        (samples.clifford_torus(side=20, seed=3), dict(depth=2, aRadius=-1, seed=3)),
+        (samples.clifford_torus(side=32, seed=4), dict(depth=4, aRadius=2)),
+    ]
+    print("case batch4d", len(inputs), flush=True)
+    recs = build_batch([Reconstructor(pos, skew, **prm) for (pos, skew), prm in inputs])
+    ref = _solve_each(recs)
+    E = solve_batch(recs)
+    _check_same(recs, E, ref)
+    for Ek, Es in zip(E, _separate_builds(inputs)):
+        assert rel_err(Ek, Es) < 1e-9, (Ek, Es)
+    for r in recs:
+        r.close()
+
+
+def test_gpu_batch_member_matches_oracle(oracle_lib):
+    from exteriorpoissonrecon_amd import Reconstructor, build_batch, samples, solve_batch
+    pos, skew = samples.spiral(4096, 3, seed=11)
+    others = [samples.torus_knot(3000, 2, 3, seed=12), samples.spiral(2048, 6, seed=13)]
+    print("case batch-oracle", flush=True)
+    recs = [Reconstructor(pos, skew, depth=5, aRadius=-1)] + [Reconstructor(p, s, depth=d, aRadius=-1) for (p, s), d in zip(others, (4, 6))]
+    build_batch(recs)
+    E = solve_batch(recs)
+    so = oracle_lib.Oracle(pos, skew, depth=5).solve()
+    assert rel_err(E[0], so["final_E"]) < TOL_SOLVE
+    scale = np.abs(so["final_x"]).max() + np.abs(so["final_y"]).max()
+    for d in range(6):
+        hx, hy = recs[0].coefficients(d)
+        assert np.abs(hx - so[f"hx{d}"]).max() < TOL_SOLVE * scale
+        assert np.abs(hy - so[f"hy{d}"]).max() < TOL_SOLVE * scale
+    for r in recs:
+        r.close()
+
+
+def test_gpu_batch_errors():
+    from exteriorpoissonrecon_amd import EprError, Reconstructor, build_batch, samples, solve_batch
+    a = Reconstructor(*samples.torus_knot(1024, seed=1), depth=3, aRadius=-1)
+    b = Reconstructor(*samples.clifford_torus(side=16, seed=1), depth=2, aRadius=-1)
+    with pytest.raises(EprError, match="epr_build has not run"):
+        solve_batch([a])
+    with pytest.raises(EprError, match="appears twice"):
+        build_batch([a, a])
+    build_batch([a, b])                      # builds of different dimensions may share a batch
+    with pytest.raises(EprError, match="different dimensions"):
+        solve_batch([a, b])
+    with pytest.raises(EprError, match="appears twice"):
+        solve_batch([a, a])
+    assert solve_batch([]).shape == (0, 2)
+    a.close(), b.close()
