@@ -330,14 +330,12 @@ struct Ctx : public CtxBase
 		return s;
 	}
 
-	// T = P^T M of a fine level into dGalT, one launch per parity class
-	template< int CLS > void galerkinTClass( const double *Mf , const Level &Fl )
+	// T = P^T M of a fine level into dGalT, all parity classes in one launch
+	void galerkinT( const double *Mf , const Level &Fl )
 	{
-		ColourSet< Dim > cs ; cs.set( Fl.res , CLS );
-		KGalerkinTCls< Dim , CLS ><<< Blocks( cs.total , 64 ) , 64 , 0 , stream >>>( Mf , dGalT.p , Fl.res/2 , cs , Fl.lay.cstart[CLS] );
+		ColourSet< Dim > cs0 ; cs0.set( Fl.res , 0 );   // class 0 (all coordinates even) is the largest
+		KGalerkinT< Dim ><<< dim3( Blocks( cs0.total , 64 ) , C ) , 64 , 0 , stream >>>( Mf , dGalT.p , Fl.res/2 , Fl.lay );
 	}
-	template< int ... Cs > void galerkinTAll( const double *Mf , const Level &Fl , std::integer_sequence< int , Cs... > ){ ( galerkinTClass< Cs >( Mf , Fl ) , ... ); }
-	void galerkinT( const double *Mf , const Level &Fl ){ galerkinTAll( Mf , Fl , std::make_integer_sequence< int , C >{} ); }
 
 	template< int KR > void launchEstSplat( TreeGrids< Dim > T )
 	{
@@ -378,6 +376,18 @@ struct Ctx : public CtxBase
 		dCellIdx.upload( cellIdx.data() , cellIdx.size() , stream );
 		dCellStart.upload( cellStart.data() , cellStart.size() , stream );
 		dCellSamples.upload( cellSamples.data() , cellSamples.size() , stream );
+
+		// ---- the trees of the estimator and the skew splat, and the coefficients: zeroed in one launch
+		{
+			const std::vector< long long > ez = treeSizes( NV , false ) , sz = treeSizes( CD , false ) , kz = treeSizes( 1 , true );
+			estData.resize( D+2 ) , estKids.resize( D+1 ) , skData.resize( D+2 ) , skKids.resize( D+2 );
+			std::vector< std::pair< void * , size_t > > spans;
+			auto dbl = [&]( std::unique_ptr< DevBuf< double > > &b , long long m ){ if( !b ) b.reset( new DevBuf< double >() ) ; b->ensure( m ) ; spans.push_back( { (void *)b->p , (size_t)m*sizeof(double) } ); };
+			auto u8 = [&]( std::unique_ptr< DevBuf< unsigned char > > &b , long long m ){ if( !b ) b.reset( new DevBuf< unsigned char >() ) ; b->ensure( m ) ; spans.push_back( { (void *)b->p , (size_t)m } ); };
+			for( int t=1 ; t<=D+1 ; t++ ){ dbl( estData[t] , ez[t] ) , dbl( skData[t] , sz[t] ) ; if( t<=D ) u8( estKids[t] , kz[t] ) , u8( skKids[t] , kz[t] ); }
+			for( int l=0 ; l<=D ; l++ ){ Level &L = levels[l] ; L.x.ensure( L.N ) , L.y.ensure( L.N ) ; spans.push_back( { (void *)L.x.p , (size_t)L.N*sizeof(double) } ) , spans.push_back( { (void *)L.y.p , (size_t)L.N*sizeof(double) } ); }
+			zeroSpans( spans );
+		}
 
 		// ---- the adapted hierarchy (Hat.adaptive.inl:349-729) on every level
 		ah.reset();
@@ -430,8 +440,7 @@ struct Ctx : public CtxBase
 			TreeGrids< Dim > T;
 			memset( &T , 0 , sizeof(T) );
 			T.maxDepth = D;
-			estData.resize( D+2 );
-			for( int t=1 ; t<=D+1 ; t++ ){ if( !estData[t] ) estData[t].reset( new DevBuf< double >() ) ; estData[t]->ensure( sz[t] ) ; estData[t]->zero( stream ) ; T.data[t] = estData[t]->p; }
+			for( int t=1 ; t<=D+1 ; t++ ) T.data[t] = estData[t]->p;
 			switch( K ){ case 0: launchEstSplat<0>( T ) ; break ; case 1: launchEstSplat<1>( T ) ; break ; case 2: launchEstSplat<2>( T ) ; break ; case 3: launchEstSplat<3>( T ) ; break ; default: launchEstSplat<4>( T ); }
 			HIPCHECK( hipGetLastError() );
 			// the estimator tree's shape (which nodes have children), for its file
@@ -440,8 +449,7 @@ struct Ctx : public CtxBase
 				TreeGrids< Dim > TK;
 				memset( &TK , 0 , sizeof(TK) );
 				TK.maxDepth = D;
-				estKids.resize( D+1 );
-				for( int t=1 ; t<=D ; t++ ){ if( !estKids[t] ) estKids[t].reset( new DevBuf< unsigned char >() ) ; estKids[t]->ensure( kz[t] ) ; estKids[t]->zero( stream ) ; TK.kids[t] = estKids[t]->p; }
+				for( int t=1 ; t<=D ; t++ ) TK.kids[t] = estKids[t]->p;
 				if( D>=1 ) KTouch< Dim ><<< Blocks( (long long)ncell*D , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , TK , K );
 				HIPCHECK( hipGetLastError() );
 			}
@@ -453,20 +461,19 @@ struct Ctx : public CtxBase
 		{
 			const int K = prm.tRadius;
 			std::vector< long long > sz = treeSizes( CD , false ) , kz = treeSizes( 1 , true );
-			skData.resize( D+2 ) , skKids.resize( D+2 );
 			TreeGrids< Dim > T;
 			memset( &T , 0 , sizeof(T) );
 			T.maxDepth = D;
-			for( int t=1 ; t<=D+1 ; t++ )
-			{
-				if( !skData[t] ) skData[t].reset( new DevBuf< double >() );
-				skData[t]->ensure( sz[t] ) ; skData[t]->zero( stream ) ; T.data[t] = skData[t]->p;
-				if( t<=D ){ if( !skKids[t] ) skKids[t].reset( new DevBuf< unsigned char >() ) ; skKids[t]->ensure( kz[t] ) ; skKids[t]->zero( stream ) ; T.kids[t] = skKids[t]->p; }
-			}
+			for( int t=1 ; t<=D+1 ; t++ ){ T.data[t] = skData[t]->p ; if( t<=D ) T.kids[t] = skKids[t]->p; }
 			if( D>=1 ) KTouch< Dim ><<< Blocks( (long long)ncell*D , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , T , K );
 			switch( K ){ case 0: launchSkewSplat<0>( T ) ; break ; case 1: launchSkewSplat<1>( T ) ; break ; case 2: launchSkewSplat<2>( T ) ; break ; case 3: launchSkewSplat<3>( T ) ; break ; default: launchSkewSplat<4>( T ); }
 			const double scl = std::pow( std::pow( 0.5 , D ) , 1.5 * CoDim );
-			for( int t=1 ; t<=D+1 ; t++ ) KScaleArray<<< Blocks( sz[t] , 256 ) , 256 , 0 , stream >>>( T.data[t] , sz[t] , scl );
+			{
+				Spans sp;
+				unsigned long long mx = 1;
+				for( int t=1 ; t<=D+1 ; t++ ) sp.p[t-1] = T.data[t] , sp.n[t-1] = (unsigned long long)sz[t] , mx = std::max( mx , sp.n[t-1] );
+				KScaleSpans<<< dim3( (unsigned)std::min< unsigned long long >( ( mx+255 )/256 , 1024 ) , D+1 ) , 256 , 0 , stream >>>( sp , scl );
+			}
 			long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= res;
 			dS.ensure( (size_t)ne*CD ) , dS2.ensure( (size_t)ne );
 			KField< Dim ><<< Blocks( ne , 256 ) , 256 , 0 , stream >>>( T , dS.p , dS2.p );
@@ -494,18 +501,14 @@ struct Ctx : public CtxBase
 			if( prm.sWeight>0 )
 			{
 				long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= res;
-				std::vector< int > cmap( ne , -1 );
-				for( int c=0 ; c<ncell ; c++ )
-				{
-					long long e = 0;
-					for( int d=Dim-1 ; d>=0 ; d-- ) e = e*res + cellIdx[ (size_t)c*Dim+d ];
-					cmap[e] = c;
-				}
-				dCellMap.upload( cmap.data() , cmap.size() , stream );
+				// element -> occupied cell (-1: none), the element index with coordinate 0 fastest
+				dCellMap.ensure( (size_t)ne );
+				HIPCHECK( hipMemsetAsync( dCellMap.p , 0xFF , (size_t)ne*sizeof(int) , stream ) );
+				KCellMap< Dim ><<< Blocks( ncell , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , res , dCellMap.p );
 				const int nb = 1024;
 				dScale.ensure( 2 );
 				KPartialSum<<< nb , 256 , 0 , stream >>>( dMeasure.p , (long long)n , dPartials.p );
-				KMeanMeasure<<< 1 , 1 , 0 , stream >>>( dPartials.p , nb , (long long)n , prm.sWeight , dScale.p );
+				KMeanMeasure<<< 1 , 256 , 0 , stream >>>( dPartials.p , nb , (long long)n , prm.sWeight , dScale.p );
 				KAssembleR< Dim ><<< Blocks( F.N*W , 256 ) , 256 , 0 , stream >>>( dPos.p , dCellMap.p , dCellStart.p , dCellSamples.p , dRW.p , res , dScale.p , F.R.p , F.lay );
 			}
 			else F.R.zero( stream );
@@ -526,10 +529,7 @@ struct Ctx : public CtxBase
 		for( int l=0 ; l<=D ; l++ )
 		{
 			Level &L = levels[l];
-			std::vector< double > k = hat::StiffnessCaseRows< Dim >( L.res );
-			L.K.upload( k.data() , k.size() , stream );
-			L.x.ensure( L.N ) , L.y.ensure( L.N );
-			L.x.zero( stream ) , L.y.zero( stream );
+			if( !L.K.n ){ std::vector< double > k = hat::StiffnessCaseRows< Dim >( L.res ) ; L.K.upload( k.data() , k.size() , stream ); }   // depends on the resolution only
 			L.checkGraph();
 		}
 		HIPCHECK( hipStreamSynchronize( stream ) );
@@ -538,6 +538,20 @@ struct Ctx : public CtxBase
 	}
 
 	void needBuilt( void ) const { if( !built ) throw std::string( "epr_build has not run" ); }
+
+	// zero-fills of many device ranges, 64 per KZeroSpans launch
+	void zeroSpans( const std::vector< std::pair< void * , size_t > > &spans )
+	{
+		for( size_t i=0 ; i<spans.size() ; i+=64 )
+		{
+			Spans sp;
+			const unsigned cnt = (unsigned)std::min< size_t >( 64 , spans.size()-i );
+			unsigned long long mx = 16;
+			for( unsigned j=0 ; j<cnt ; j++ ) sp.p[j] = spans[i+j].first , sp.n[j] = spans[i+j].second , mx = std::max( mx , sp.n[j] );
+			KZeroSpans<<< dim3( (unsigned)std::min< unsigned long long >( ( mx/16+255 )/256 , 512 ) , cnt ) , 256 , 0 , stream >>>( sp );
+		}
+		HIPCHECK( hipGetLastError() );
+	}
 
 	// ---------------- solve pieces ----------------
 	// How a level's sweeps run: all iterations in one workgroup (KGsTiny: one
@@ -722,7 +736,11 @@ struct Ctx : public CtxBase
 		needBuilt();
 		auto t0 = std::chrono::steady_clock::now();
 		const int lo = minD() , hi = maxD();
-		for( int l=0 ; l<=D ; l++ ) levels[l].x.zero( stream ) , levels[l].y.zero( stream );
+		{
+			std::vector< std::pair< void * , size_t > > spans;
+			for( int l=0 ; l<=D ; l++ ) spans.push_back( { (void *)levels[l].x.p , (size_t)levels[l].N*sizeof(double) } ) , spans.push_back( { (void *)levels[l].y.p , (size_t)levels[l].N*sizeof(double) } );
+			zeroSpans( spans );
+		}
 		init( lo , nullptr , nullptr );
 		auto Iters = [&]( int d ){ return (int)std::ceil( std::pow( prm.iMult , D-d ) * prm.gsIters ); };
 		if( prm.singleLevel )

@@ -550,6 +550,24 @@ __global__ void __launch_bounds__( 256 ) KCopy( const double2 *__restrict__ a , 
 	for( long long k=(long long)blockIdx.x*blockDim.x + threadIdx.x ; k<n ; k+=(long long)gridDim.x*blockDim.x ) b[k] = a[k];
 }
 
+// Up to 64 device ranges for one launch (blockIdx.y selects the range).
+struct Spans { void *p[64] ; unsigned long long n[64]; };
+// zero-fills n[j] bytes at p[j] (16-byte aligned starts)
+__global__ void __launch_bounds__( 256 ) KZeroSpans( Spans s )
+{
+	unsigned char *p = (unsigned char *)s.p[ blockIdx.y ];
+	const unsigned long long nb = s.n[ blockIdx.y ] , nw = nb/16;
+	for( unsigned long long i=(unsigned long long)blockIdx.x*blockDim.x + threadIdx.x ; i<nw ; i+=(unsigned long long)gridDim.x*blockDim.x ) ( (uint4 *)p )[i] = make_uint4( 0 , 0 , 0 , 0 );
+	if( blockIdx.x==0 && threadIdx.x<nb-nw*16 ) p[ nw*16 + threadIdx.x ] = 0;
+}
+// scales n[j] doubles at p[j] by a
+__global__ void __launch_bounds__( 256 ) KScaleSpans( Spans s , double a )
+{
+	double *p = (double *)s.p[ blockIdx.y ];
+	const unsigned long long n = s.n[ blockIdx.y ];
+	for( unsigned long long i=(unsigned long long)blockIdx.x*blockDim.x + threadIdx.x ; i<n ; i+=(unsigned long long)gridDim.x*blockDim.x ) p[i] *= a;
+}
+
 __global__ void KAddInto( const double *a , const double *b , double *out , long long n )
 {
 	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
@@ -890,6 +908,17 @@ __global__ void KSkewSplat( const double *pos , const double *skewScaled , const
 			atomicAdd( node+c , dv*scl );
 		}
 	}
+}
+
+// cmap[e] = c for every occupied cell c (element index e with coordinate 0 fastest)
+template< int Dim >
+__global__ void KCellMap( const int *cellIdx , int ncell , int res , int *cmap )
+{
+	const int c = blockIdx.x*blockDim.x + threadIdx.x;
+	if( c>=ncell ) return;
+	long long e = 0;
+	for( int d=Dim-1 ; d>=0 ; d-- ) e = e*res + cellIdx[ (size_t)c*Dim+d ];
+	cmap[e] = c;
 }
 
 __global__ void KScaleArray( double *a , long long n , double s )
@@ -1239,17 +1268,16 @@ __device__ __forceinline__ void GalerkinColumn( const double *col , const int *K
 	}() , ... );
 }
 
-// 64 fine nodes per workgroup: their columns are loaded into LDS with
-// contiguous reads, scattered from LDS into register accumulators, and the
-// results leave through LDS with contiguous writes (node-major T[k*W + wo]).
+// 64 fine nodes of parity class CLS per workgroup (block b of the class):
+// their columns are loaded into LDS with contiguous reads, scattered from LDS
+// into register accumulators, and the results leave through LDS with
+// contiguous writes (node-major T[k*W + wo]).
 template< int Dim , int CLS >
-__global__ void __launch_bounds__( 64 ) KGalerkinTCls( const double *Mf , double *T , int cres , ColourSet< Dim > cs , long long colBase )
+__device__ __forceinline__ void GalerkinTBlock( const double *Mf , double *T , int cres , const ColourSet< Dim > &cs , long long colBase , unsigned b , double *buf , long long *knode )
 {
 	constexpr int W = DimC< Dim >::W;
-	__shared__ double buf[ 64*W ];
-	__shared__ long long knode[64];
 	const int lane = threadIdx.x , fres = 2*cres;
-	const long long g = (long long)blockIdx.x*64 + lane;
+	const long long g = (long long)b*64 + lane;
 	int K[Dim];
 	long long k = -1;
 	if( g<cs.total )
@@ -1261,7 +1289,7 @@ __global__ void __launch_bounds__( 64 ) KGalerkinTCls( const double *Mf , double
 	knode[lane] = k;
 	__syncthreads();
 	// the block's columns are consecutive in the colour-grouped layout
-	const long long col0 = colBase + (long long)blockIdx.x*64;
+	const long long col0 = colBase + (long long)b*64;
 	for( int e=lane ; e<64*W ; e+=64 ){ const long long kn = knode[ e/W ] ; buf[e] = kn>=0 ? Mf[ col0*W + e ] : 0.; }
 	__syncthreads();
 	double acc[W];
@@ -1273,6 +1301,24 @@ __global__ void __launch_bounds__( 64 ) KGalerkinTCls( const double *Mf , double
 	for( int w=0 ; w<W ; w++ ) buf[ lane*W + w ] = acc[w];
 	__syncthreads();
 	for( int e=lane ; e<64*W ; e+=64 ){ const long long kn = knode[ e/W ] ; if( kn>=0 ) T[ kn*W + e%W ] = buf[e]; }
+}
+template< int Dim , int ... Cs >
+__device__ __forceinline__ void GalerkinTDispatch( int cls , const double *Mf , double *T , int cres , long long colBase , unsigned b , double *buf , long long *knode , std::integer_sequence< int , Cs... > )
+{
+	( [&]{
+		if( cls!=Cs ) return;
+		ColourSet< Dim > cs ; cs.set( 2*cres , Cs );
+		if( (long long)b*64<cs.total ) GalerkinTBlock< Dim , Cs >( Mf , T , cres , cs , colBase , b , buf , knode );   // block-uniform
+	}() , ... );
+}
+// T = P^T M of a fine level: blockIdx.y is the parity class of the fine nodes
+template< int Dim >
+__global__ void __launch_bounds__( 64 ) KGalerkinT( const double *Mf , double *T , int cres , ColLayout< Dim > lay )
+{
+	__shared__ double buf[ 64*DimC< Dim >::W ];
+	__shared__ long long knode[64];
+	const int cls = blockIdx.y;
+	GalerkinTDispatch< Dim >( cls , Mf , T , cres , lay.cstart[cls] , blockIdx.x , buf , knode , std::make_integer_sequence< int , DimC< Dim >::C >{} );
 }
 
 template< int Dim >
@@ -1680,12 +1726,17 @@ __device__ __forceinline__ void GsTiny( const LevelView< Dim > &L , int iters )
 template< int Dim , bool RB >
 __global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTiny( LevelView< Dim > L , int iters ){ GsTiny< Dim , RB >( L , iters ); }
 
-// mean of the per-sample measures (deterministic): out[1] = sum / n, out[0] = sWeight
-__global__ void KMeanMeasure( const double *partials , int nb , long long n , double sWeight , double *out )
+// mean of the per-sample measures (deterministic): out[1] = sum / n, out[0] =
+// sWeight.  The partials are staged in LDS by the whole block and summed in
+// order by one thread.  nb <= 1024.
+__global__ void __launch_bounds__( 256 ) KMeanMeasure( const double *partials , int nb , long long n , double sWeight , double *out )
 {
-	if( threadIdx.x || blockIdx.x ) return;
+	__shared__ double sh[1024];
+	for( int b=threadIdx.x ; b<nb ; b+=blockDim.x ) sh[b] = partials[b];
+	__syncthreads();
+	if( threadIdx.x ) return;
 	double s = 0;
-	for( int b=0 ; b<nb ; b++ ) s += partials[b];
+	for( int b=0 ; b<nb ; b++ ) s += sh[b];
 	out[0] = sWeight , out[1] = s / (double)n;
 }
 } // namespace epr
