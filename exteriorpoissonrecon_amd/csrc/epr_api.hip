@@ -1110,19 +1110,13 @@ inline void BuildBatch( const std::vector< CtxBase * > &cs , int threads )
 // Ctx::solve, in the same order, so its coefficients and energies are the
 // same bit for bit.  The --verbose per-level energies are not printed.
 template< int Dim >
-void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
+void SolveBatchGroup( const std::vector< Ctx< Dim > * > &cs , double *energies , std::mutex &randLock )
 {
 	using Cx = Ctx< Dim >;
 	using Lv = typename Cx::Level;
 	constexpr int C = DimC< Dim >::C;
 	const size_t n = cs.size();
 	if( !n ) return;
-	if( n>65535 ) throw std::string( "at most 65535 contexts per batch" );
-	{
-		std::set< const Cx * > seen;
-		for( const Cx *c : cs ) if( !seen.insert( c ).second ) throw std::string( "a context appears twice in the batch" );
-	}
-	for( Cx *c : cs ) c->needBuilt();
 	auto t0 = std::chrono::steady_clock::now();
 	hipStream_t st = cs[0]->stream;
 	int maxD = 0;
@@ -1184,6 +1178,7 @@ void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
 	// the random start of Ctx::init (ExteriorPoisson.h:333-341), drawn in context order
 	std::vector< double > start;
 	std::vector< size_t > startAt( n );
+	std::unique_lock< std::mutex > randHeld( randLock );   // glibc's rand() state is shared by every thread
 	for( size_t r=0 ; r<n ; r++ )
 	{
 		Cx &c = *cs[r];
@@ -1195,6 +1190,7 @@ void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
 		startAt[r] = start.size();
 		start.insert( start.end() , hx.begin() , hx.end() ) , start.insert( start.end() , hy.begin() , hy.end() );
 	}
+	randHeld.unlock();
 	DevBuf< double > dStart , dScale , eT;
 	dStart.upload( start.data() , start.size() , st );
 	dScale.alloc( n ) , eT.alloc( 4*n );
@@ -1386,6 +1382,54 @@ void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
 	HIPCHECK( hipStreamSynchronize( st ) );
 	const double sec = std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count();
 	for( Cx *c : cs ) c->solved = true , c->seconds[1] = sec;
+}
+
+// The batched solve: contexts of equal depth form a group (their cascades have
+// the same number of stages), and the groups run side by side, each from its
+// own host thread on its own stream, so that the throughput-bound fine levels
+// of the shallower groups fill the GPU while the deepest group's latency-bound
+// coarse levels run.  Within a group the levels are batched (SolveBatchGroup).
+template< int Dim >
+void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
+{
+	using Cx = Ctx< Dim >;
+	const size_t n = cs.size();
+	if( !n ) return;
+	if( n>65535 ) throw std::string( "at most 65535 contexts per batch" );
+	{
+		std::set< const Cx * > seen;
+		for( const Cx *c : cs ) if( !seen.insert( c ).second ) throw std::string( "a context appears twice in the batch" );
+	}
+	for( Cx *c : cs ) c->needBuilt();
+	std::vector< std::vector< size_t > > groups( MAXT );
+	for( size_t r=0 ; r<n ; r++ ) groups[ cs[r]->D ].push_back( r );
+	std::vector< std::vector< Cx * > > gc;
+	std::vector< std::vector< double > > ge;
+	std::vector< const std::vector< size_t > * > gi;
+	for( const auto &g : groups ) if( !g.empty() )
+	{
+		gc.emplace_back() , ge.emplace_back( 2*g.size() ) , gi.push_back( &g );
+		for( size_t r : g ) gc.back().push_back( cs[r] );
+	}
+	std::mutex randLock , m;
+	std::exception_ptr first;
+	int dev = 0;
+	HIPCHECK( hipGetDevice( &dev ) );
+	auto work = [&]( size_t k )
+	{
+		try
+		{
+			HIPCHECK( hipSetDevice( dev ) );
+			SolveBatchGroup< Dim >( gc[k] , ge[k].data() , randLock );
+		}
+		catch( ... ){ std::lock_guard< std::mutex > lock( m ) ; if( !first ) first = std::current_exception(); }
+	};
+	std::vector< std::thread > ts;
+	for( size_t k=1 ; k<gc.size() ; k++ ) ts.emplace_back( work , k );
+	work( 0 );
+	for( auto &t : ts ) t.join();
+	if( first ) std::rethrow_exception( first );
+	if( energies ) for( size_t k=0 ; k<gc.size() ; k++ ) for( size_t j=0 ; j<gi[k]->size() ; j++ ) energies[ 2*(*gi[k])[j] ] = ge[k][2*j] , energies[ 2*(*gi[k])[j]+1 ] = ge[k][2*j+1];
 }
 } // namespace epr
 
