@@ -123,7 +123,10 @@ def cpu_baseline(name, wl, pos, skew, a_radius=-1):
 def batch_main(args, wl):
     """The batched workload: every step builds (epr_build_batch) and solves
     (epr_solve_batch) all inputs; value = seconds per batch."""
+    if args.serial:
+        os.environ["EPR_BATCH_SERIAL"] = "1"
     from exteriorpoissonrecon_amd import Reconstructor, build_batch, samples, solve_batch
+    threads = 1 if args.serial else 0
     inputs = samples.batch_workload(wl["batch"])
     nsamp = sum(len(p) for p, _, _, _ in inputs)
     log(f"[bench] {args.config}: {len(inputs)} inputs, {nsamp} samples, depths "
@@ -131,12 +134,12 @@ def batch_main(args, wl):
     recs = [Reconstructor(p, s, depth=d, aRadius=args.aRadius) for p, s, d, _ in inputs]
     E = None
     for i in range(args.warmup):
-        t0 = time.perf_counter(); build_batch(recs); t1 = time.perf_counter(); E = solve_batch(recs)
+        t0 = time.perf_counter(); build_batch(recs, threads); t1 = time.perf_counter(); E = solve_batch(recs)
         log(f"[bench] warmup {i}: build {t1 - t0:.3f}s solve {time.perf_counter() - t1:.3f}s")
     times, builds, solves = [], [], []
     for i in range(args.steps):
         t0 = time.perf_counter()
-        build_batch(recs)
+        build_batch(recs, threads)
         t1 = time.perf_counter()
         E = solve_batch(recs)
         t2 = time.perf_counter()
@@ -151,15 +154,17 @@ def batch_main(args, wl):
     achieved = updates * F_NODE[3] / float(np.mean(solves)) / 1e12
     log(f"[bench] batched solve: {updates:.3e} node updates, {achieved:.2f} TFLOP/s algorithmic over the whole solve")
     # the same inputs one after another (epr_build + epr_solve each), once
-    t0 = time.perf_counter()
-    Es = []
-    for r in recs:
-        r.build(); Es.append(r.solve())
-    unbatched = time.perf_counter() - t0
-    # separate builds splat with FP64 atomics: the energies agree to rounding, not bit for bit
-    Es = np.array(Es)
-    diff = float(np.max(np.abs(Es.sum(1) - E.sum(1)) / np.abs(Es.sum(1))))
-    log(f"[bench] unbatched: {unbatched:.3f}s for the same {len(recs)} inputs; max relative energy difference to the batch {diff:.2e}")
+    unbatched = diff = None
+    if not args.no_unbatched:
+        t0 = time.perf_counter()
+        Es = []
+        for r in recs:
+            r.build(); Es.append(r.solve())
+        unbatched = time.perf_counter() - t0
+        # separate builds splat with FP64 atomics: the energies agree to rounding, not bit for bit
+        Es = np.array(Es)
+        diff = float(np.max(np.abs(Es.sum(1) - E.sum(1)) / np.abs(Es.sum(1))))
+        log(f"[bench] unbatched: {unbatched:.3f}s for the same {len(recs)} inputs; max relative energy difference to the batch {diff:.2e}")
     cpu = None
     if not args.no_cpu:
         exe = os.path.join(ROOT, "oracle", "_ref", "ExteriorPoissonRecon")
@@ -243,6 +248,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
     ap.add_argument("--aRadius", type=int, default=-1, help="-1: dense grid; 0..5: adapted hierarchy of that radius")
+    ap.add_argument("--serial", action="store_true",
+                    help="batch workload: one build thread and the depth groups one after another (for profilers)")
+    ap.add_argument("--no-unbatched", action="store_true", help="batch workload: skip the one-at-a-time comparison")
     ap.add_argument("--extract", action="store_true",
                     help="also time the Extract pipeline on the .tree the last step writes (an 'extract' object in the JSON line)")
     args = ap.parse_args()

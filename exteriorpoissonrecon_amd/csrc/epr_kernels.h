@@ -1041,27 +1041,24 @@ __global__ void __launch_bounds__( 256 ) KAssembleB( const double *S , const dou
 // R = (E + E^T) * sWeight * mean(measure), E the sample-weighted delta mass of
 // Hat.scalars.inl:1006-1062: for the pair (f,g) summed over the occupied cells
 // adjacent to both in cell order, within a cell over its samples in order.
+// Entry M(G, F) of R for window offset w of node F (0 outside the grid or
+// without samples): the common elements E of the two supports in
+// lexicographic order, each the sum over its samples of the two hat values
+// times the sample weight.
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const int *cellMap , const int *cellStart , const int *cellSamples ,
-                                                      const double *rweight , int res , const double *scale , double *R , ColLayout< Dim > lay )
+__device__ __forceinline__ double REntry( const double *pos , const int *cellMap , const int *cellStart , const int *cellSamples , const double *rweight , int res , const int *F , int w , bool &any )
 {
 	constexpr int W = DimC< Dim >::W;
-	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= res+1;
-	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-	if( g>=nf*W ) return;
-	const long long f = g / W;
-	const int w = (int)( g % W );
-	int F[Dim] , G[Dim];
-	Unflatten< Dim >( f , res , F );
-	const long long out = lay.col( F )*W + w;
+	int G[Dim];
 	{
 		int t = w;
 		for( int d=Dim-1 ; d>=0 ; d-- ){ G[d] = F[d] + t%3 - 1 ; t /= 3; }
 	}
 	int lo[Dim] , hi[Dim] , E[Dim];
+	any = false;
 	for( int d=0 ; d<Dim ; d++ )
 	{
-		if( G[d]<0 || G[d]>res ){ R[out] = 0 ; return; }
+		if( G[d]<0 || G[d]>res ) return 0;
 		lo[d] = max( 0 , max( F[d] , G[d] )-1 );
 		hi[d] = min( res , min( F[d] , G[d] )+1 );
 		E[d] = lo[d];
@@ -1072,7 +1069,6 @@ __global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const i
 	const int *A = FleG ? F : G , *Bn = FleG ? G : F;
 	const bool diag = ( w==W/2 );
 	double acc = 0;
-	bool any = false;
 	while( true )
 	{
 		long long e = 0;
@@ -1101,9 +1097,36 @@ __global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const i
 		for( ; d>=0 ; d-- ){ if( ++E[d]<hi[d] ) break ; E[d] = lo[d]; }
 		if( d<0 ) break;
 	}
-	if( !any ){ R[out] = 0 ; return; }
-	const double v = diag ? acc+acc : acc;
-	R[out] = v * scale[0] * scale[1];
+	return any ? ( diag ? acc+acc : acc ) : 0;
+}
+// The nodes with an occupied incident element, once each (list order is
+// arbitrary): thread t marks corner t mod 2^Dim of occupied cell t / 2^Dim.
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KOccupiedNodes( const int *cellIdx , int ncell , int res , int *flag , unsigned int *list , unsigned int *count )
+{
+	const long long t = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( t>=( (long long)ncell<<Dim ) ) return;
+	const int c = (int)( t>>Dim ) , k = (int)( t & ( (1<<Dim)-1 ) );
+	long long f = 0;
+	for( int d=0 ; d<Dim ; d++ ) f = f*( res+1 ) + cellIdx[ (size_t)c*Dim+d ] + ( ( k>>d )&1 );
+	if( atomicExch( flag+f , 1 )==0 ) list[ atomicAdd( count , 1u ) ] = (unsigned int)f;
+}
+// One thread per window entry of a listed node (R is zero elsewhere: every
+// entry of a node without an occupied incident element vanishes).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const int *cellMap , const int *cellStart , const int *cellSamples ,
+                                                      const double *rweight , int res , const double *scale , double *R , ColLayout< Dim > lay ,
+                                                      const unsigned int *list , const unsigned int *count )
+{
+	constexpr int W = DimC< Dim >::W;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=(long long)count[0]*W ) return;
+	const int w = (int)( g % W );
+	int F[Dim];
+	Unflatten< Dim >( list[ g/W ] , res , F );
+	bool any;
+	const double v = REntry< Dim >( pos , cellMap , cellStart , cellSamples , rweight , res , F , w , any );
+	R[ lay.col( F )*W + w ] = any ? v * scale[0] * scale[1] : 0;
 }
 
 // ---- B assembly, one thread per finest node f1 and all 3^Dim outputs of its

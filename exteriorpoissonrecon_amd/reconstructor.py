@@ -152,7 +152,7 @@ def _batch_check(rc):
 
 def build_batch(recs, threads=0):
     """Reconstructor.build() of every reconstruction, from up to `threads` host
-    threads (0: one per hardware thread, at most 16), each on its own stream."""
+    threads (0: four), each on its own stream."""
     recs = list(recs)
     _batch_check(load().epr_build_batch(len(recs), _handles(recs), int(threads)))
     return recs

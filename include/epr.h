@@ -110,8 +110,8 @@ int  epr_solve( epr_ctx *ctx , double out_E[2] );
  * would leave most of the GPU idle.  Context-free errors (epr_last_error(NULL)).
  *
  * epr_build_batch: epr_build on each of n contexts from up to `threads` host
- * threads (0: one per hardware thread, at most 16), each context's kernels on
- * its own stream.
+ * threads (0: 4, one per hardware queue of the HIP runtime's default), each
+ * context's kernels on its own stream.
  *
  * epr_solve_batch: epr_solve on each of n built contexts of one dimension,
  * with every level's sweeps of all contexts in shared launches (one
