@@ -222,7 +222,7 @@ struct Ctx : public CtxBase
 	std::vector< Level > levels;
 
 	// build-phase device data
-	DevBuf< double > dGalT , dPos , dSkew , dCellP , dCellData , dMeasure , dNoise , dDepth , dRW , dSkewScaled , dS , dS2 , dFS , dScale , dPartials , dTmp , dTmp2 , dDepthScale;
+	DevBuf< double > dGalT , dPosS , dRWS , dPos , dSkew , dCellP , dCellData , dMeasure , dNoise , dDepth , dRW , dSkewScaled , dS , dS2 , dFS , dScale , dPartials , dTmp , dTmp2 , dDepthScale;
 	DevBuf< int > dCellIdx , dCellStart , dCellSamples , dCellMap , dNodeFlag;
 	DevBuf< unsigned int > dNodeList , dNodeCount;
 	std::vector< std::unique_ptr< DevBuf< double > > > estData , skData;
@@ -514,7 +514,9 @@ struct Ctx : public CtxBase
 				dNodeFlag.ensure( (size_t)F.N ) , dNodeList.ensure( std::min< size_t >( (size_t)F.N , (size_t)ncell<<Dim ) ) , dNodeCount.ensure( 1 );
 				zeroSpans( { { (void *)F.R.p , (size_t)F.N*W*sizeof(double) } , { (void *)dNodeFlag.p , (size_t)F.N*sizeof(int) } , { (void *)dNodeCount.p , sizeof(unsigned int) } } );
 				KOccupiedNodes< Dim ><<< Blocks( (long long)ncell<<Dim , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , res , dNodeFlag.p , dNodeList.p , dNodeCount.p );
-				KAssembleR< Dim ><<< Blocks( std::min< long long >( F.N , (long long)ncell<<Dim )*W , 256 ) , 256 , 0 , stream >>>( dPos.p , dCellMap.p , dCellStart.p , dCellSamples.p , dRW.p , res , dScale.p , F.R.p , F.lay , dNodeList.p , dNodeCount.p );
+				dPosS.ensure( n*Dim ) , dRWS.ensure( n );
+				KCellOrder< Dim ><<< Blocks( (long long)n , 256 ) , 256 , 0 , stream >>>( dPos.p , dRW.p , dCellSamples.p , (int)n , dPosS.p , dRWS.p );
+				KAssembleR< Dim ><<< Blocks( std::min< long long >( F.N , (long long)ncell<<Dim )*W , 256 ) , 256 , 0 , stream >>>( dPosS.p , dCellMap.p , dCellStart.p , dRWS.p , res , dScale.p , F.R.p , F.lay , dNodeList.p , dNodeCount.p );
 			}
 			else F.R.zero( stream );
 			HIPCHECK( hipGetLastError() );

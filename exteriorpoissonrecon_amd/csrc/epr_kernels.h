@@ -1041,12 +1041,24 @@ __global__ void __launch_bounds__( 256 ) KAssembleB( const double *S , const dou
 // R = (E + E^T) * sWeight * mean(measure), E the sample-weighted delta mass of
 // Hat.scalars.inl:1006-1062: for the pair (f,g) summed over the occupied cells
 // adjacent to both in cell order, within a cell over its samples in order.
+// Samples in cell order (k-th entry of cellSamples): positions and weights
+// stored contiguously, so that the per-cell sums of REntry read sequentially.
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KCellOrder( const double *pos , const double *rweight , const int *cellSamples , int n , double *posS , double *rwS )
+{
+	const int k = blockIdx.x*blockDim.x + threadIdx.x;
+	if( k>=n ) return;
+	const int s = cellSamples[k];
+	for( int d=0 ; d<Dim ; d++ ) posS[ (long long)k*Dim+d ] = pos[ (long long)s*Dim+d ];
+	rwS[k] = rweight[s];
+}
 // Entry M(G, F) of R for window offset w of node F (0 outside the grid or
 // without samples): the common elements E of the two supports in
-// lexicographic order, each the sum over its samples of the two hat values
-// times the sample weight.
+// lexicographic order, each the sum over its samples (in the cell's order;
+// posS, rwS as KCellOrder writes them) of the two hat values times the sample
+// weight.
 template< int Dim >
-__device__ __forceinline__ double REntry( const double *pos , const int *cellMap , const int *cellStart , const int *cellSamples , const double *rweight , int res , const int *F , int w , bool &any )
+__device__ __forceinline__ double REntry( const double *posS , const int *cellMap , const int *cellStart , const double *rwS , int res , const int *F , int w , bool &any )
 {
 	constexpr int W = DimC< Dim >::W;
 	int G[Dim];
@@ -1079,16 +1091,15 @@ __device__ __forceinline__ double REntry( const double *pos , const int *cellMap
 			double value = 0;
 			for( int k=cellStart[c] ; k<cellStart[c+1] ; k++ )
 			{
-				const int s = cellSamples[k];
 				double va = 1. , vb = 1.;
 				for( int d=0 ; d<Dim ; d++ )
 				{
-					double lp = pos[ (long long)s*Dim+d ] * res;
+					double lp = posS[ (long long)k*Dim+d ] * res;
 					lp -= E[d];
 					va *= ( A[d]-E[d] ) ? lp : (1.-lp);
 					vb *= ( Bn[d]-E[d] ) ? lp : (1.-lp);
 				}
-				value += va * vb * rweight[s];
+				value += va * vb * rwS[k];
 			}
 			if( diag ) value /= 2.;
 			if( any ) acc = acc + value ; else acc = value , any = true;
@@ -1114,8 +1125,8 @@ __global__ void __launch_bounds__( 256 ) KOccupiedNodes( const int *cellIdx , in
 // One thread per window entry of a listed node (R is zero elsewhere: every
 // entry of a node without an occupied incident element vanishes).
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const int *cellMap , const int *cellStart , const int *cellSamples ,
-                                                      const double *rweight , int res , const double *scale , double *R , ColLayout< Dim > lay ,
+__global__ void __launch_bounds__( 256 ) KAssembleR( const double *posS , const int *cellMap , const int *cellStart , const double *rwS ,
+                                                      int res , const double *scale , double *R , ColLayout< Dim > lay ,
                                                       const unsigned int *list , const unsigned int *count )
 {
 	constexpr int W = DimC< Dim >::W;
@@ -1125,7 +1136,7 @@ __global__ void __launch_bounds__( 256 ) KAssembleR( const double *pos , const i
 	int F[Dim];
 	Unflatten< Dim >( list[ g/W ] , res , F );
 	bool any;
-	const double v = REntry< Dim >( pos , cellMap , cellStart , cellSamples , rweight , res , F , w , any );
+	const double v = REntry< Dim >( posS , cellMap , cellStart , rwS , res , F , w , any );
 	R[ lay.col( F )*W + w ] = any ? v * scale[0] * scale[1] : 0;
 }
 
