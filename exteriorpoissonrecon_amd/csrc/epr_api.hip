@@ -1404,7 +1404,8 @@ void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
 	using Cx = Ctx< Dim >;
 	const size_t n = cs.size();
 	if( !n ) return;
-	if( n>65535 ) throw std::string( "at most 65535 contexts per batch" );
+	// grid.y of the fill, copy and prolongation launches is twice the contexts of a level (x and y), at most 65535
+	if( n>32767 ) throw std::string( "at most 32767 contexts per batch" );
 	{
 		std::set< const Cx * > seen;
 		for( const Cx *c : cs ) if( !seen.insert( c ).second ) throw std::string( "a context appears twice in the batch" );

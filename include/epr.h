@@ -120,7 +120,7 @@ int  epr_solve( epr_ctx *ctx , double out_E[2] );
  * coefficients and energies its own epr_solve gives; energies (may be NULL)
  * receives n x 2 values (E_fit, E_reg).  --verbose prints nothing here.
  * Contexts of equal depth are solved together, each such group from its own
- * host thread on its first context's stream; at most 65535 contexts. */
+ * host thread on its first context's stream; at most 32767 contexts. */
 int  epr_build_batch( size_t n , epr_ctx *const *ctxs , int threads );
 int  epr_solve_batch( size_t n , epr_ctx *const *ctxs , double *energies );
 

@@ -134,3 +134,50 @@ def test_extract_ply_serialisers_agree(tmp_path):
         assert mesh_ply_bytes(dim, V, S) == mesh_ply(dim, V, S)
         sets = [(rng.normal(size=(4, dim)), rng.integers(0, 4, size=(3, dim))) for _ in range(2)]
         assert level_sets_ply_bytes(dim, sets) == level_sets_ply(dim, sets)
+
+
+def test_batch_argument_errors():
+    """epr_build_batch / epr_solve_batch reject bad context arrays before any
+    device call (context-free errors through epr_last_error(NULL))."""
+    import ctypes
+    from exteriorpoissonrecon_amd._lib import load
+    L = load()
+    assert L.epr_build_batch(2, None, 0) != 0
+    assert b"null context array" in L.epr_last_error(None)
+    arr = (ctypes.c_void_p * 2)(None, None)
+    assert L.epr_solve_batch(2, arr, None) != 0
+    assert b"context 0 is null" in L.epr_last_error(None)
+    assert L.epr_solve_batch(0, None, None) == 0 and L.epr_build_batch(0, None, 0) == 0
+
+
+def test_batch_workload_shape():
+    """The batched serving workload: 256 3D inputs, depths 4-6, 2048-16384
+    samples, coprime torus knots or spirals; deterministic for a seed."""
+    w = samples.batch_workload(256)
+    assert len(w) == 256
+    for pos, skew, depth, desc in w:
+        assert depth in (4, 5, 6) and 2048 <= len(pos) <= 16384
+        assert pos.shape == (len(pos), 3) and skew.shape == (len(pos), 3, 3)
+        assert np.allclose(skew, -np.transpose(skew, (0, 2, 1)))
+        kind, *args = desc.split(":")
+        if kind == "torus_knot":
+            p, q = map(int, args)
+            assert 2 <= p <= 7 and 2 <= q <= 7 and np.gcd(p, q) == 1
+        else:
+            assert kind == "spiral" and 2 <= int(args[0]) <= 6
+    w2 = samples.batch_workload(4)
+    assert all(np.array_equal(a[0], b[0]) for a, b in zip(w[:4], w2))
+
+
+def test_spiral_frames_are_tangent_duals():
+    """spiral(): points on (cos 2 pi r s, 2 s - 1, sin 2 pi r s) and frames
+    Dual(unit tangent), the reference's spiral:r curve (Samples.h:321-331)."""
+    pos, skew = samples.spiral(512, 4, seed=3)
+    s = (pos[:, 1] + 1) / 2
+    X = 2 * np.pi * 4
+    assert np.allclose(pos[:, 0], np.cos(X * s)) and np.allclose(pos[:, 2], np.sin(X * s))
+    tan = np.stack([-X * np.sin(X * s), np.full(len(s), 2.0), X * np.cos(X * s)], -1)
+    tan /= np.linalg.norm(tan, axis=1, keepdims=True)
+    # Dual(t): n((d+1)%3,(d+2)%3) = t[d]
+    for d in range(3):
+        assert np.allclose(skew[:, (d + 1) % 3, (d + 2) % 3], tan[:, d])
