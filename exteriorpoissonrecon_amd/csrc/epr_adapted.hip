@@ -156,119 +156,113 @@ __global__ void KColourStarts( const unsigned int *keys , unsigned int n , long 
 
 int KeyBits( unsigned long long maxKey ){ int b = 1 ; while( b<64 && ( maxKey>>b ) ) b++ ; return b; }
 
+// the construction's scratch with its stream
 struct Scratch
 {
-	DevBuf< unsigned char > tmp , ga , gb;
-	DevBuf< unsigned int > sel , vals , ckeys , ckeys2;
-	DevBuf< unsigned long long > keys , keys2 , pkey;
-	DevBuf< unsigned int > dn;
-	DevBuf< long long > dstarts;
+	AdaptedScratch &b;
 	hipStream_t st;
-
 	template< typename F > void cub( F f )
 	{
 		size_t bytes = 0;
 		HIPCHECK( f( nullptr , bytes ) );
-		tmp.ensure( bytes );
-		HIPCHECK( f( tmp.p , bytes ) );
+		b.tmp.ensure( bytes );
+		HIPCHECK( f( b.tmp.p , bytes ) );
 	}
 	unsigned int count( void )
 	{
 		unsigned int h = 0;
-		HIPCHECK( hipMemcpyAsync( &h , dn.p , sizeof(h) , hipMemcpyDeviceToHost , st ) );
+		HIPCHECK( hipMemcpyAsync( &h , b.dn.p , sizeof(h) , hipMemcpyDeviceToHost , st ) );
 		HIPCHECK( hipStreamSynchronize( st ) );
 		return h;
 	}
 	// increasing indices i < n with pred(i), into sel; returns the count
 	template< typename Pred > unsigned int select( long long n , Pred pred )
 	{
-		sel.ensure( (size_t)n );
+		b.sel.ensure( (size_t)n );
 		hipcub::CountingInputIterator< unsigned int > it( 0 );
-		cub( [&]( void *t , size_t &b ){ return hipcub::DeviceSelect::If( t , b , it , sel.p , dn.p , (int)n , pred , st ); } );
+		cub( [&]( void *t , size_t &bytes ){ return hipcub::DeviceSelect::If( t , bytes , it , b.sel.p , b.dn.p , (int)n , pred , st ); } );
 		return count();
 	}
 	// sorts (keys, sel[0..n)) by key into out
 	void sortByKey( unsigned int n , int bits , unsigned int *out )
 	{
-		keys2.ensure( n );
-		cub( [&]( void *t , size_t &b ){ return hipcub::DeviceRadixSort::SortPairs( t , b , keys.p , keys2.p , sel.p , out , (int)n , 0 , bits , st ); } );
+		b.keys2.ensure( n );
+		cub( [&]( void *t , size_t &bytes ){ return hipcub::DeviceRadixSort::SortPairs( t , bytes , b.keys.p , b.keys2.p , b.sel.p , out , (int)n , 0 , bits , st ); } );
 	}
 };
 } // namespace
 
 template< int Dim >
-void BuildAdaptedDevice( const int *dCells , int ncell , int D , int K , hipStream_t st , std::vector< AdaptedLevelDev > &out )
+void BuildAdaptedDevice( const int *dCells , int ncell , int D , int K , hipStream_t st , const std::vector< AdaptedLevelDev * > &out , AdaptedScratch &scratch )
 {
 	constexpr int C = 1<<Dim , W = Pow3< Dim >();
-	out.clear();
-	out.resize( D+1 );
-	Scratch s;
-	s.st = st;
-	s.dn.alloc( 1 );
-	s.dstarts.alloc( C );
+	if( (int)out.size()<D+1 ) throw std::string( "BuildAdaptedDevice: too few output levels" );
+	Scratch s{ scratch , st };
+	scratch.dn.ensure( 1 );
+	scratch.dstarts.ensure( C );
 	for( int d=0 ; d<=D ; d++ )
 	{
-		AdaptedLevelDev &L = out[d];
+		AdaptedLevelDev &L = *out[d];
 		const int res = 1<<d;
 		long long N = 1 , Ne = 1;
 		for( int k=0 ; k<Dim ; k++ ) N *= res+1 , Ne *= res;
 		// ---- active flags: ancestors, then the separable box filter
-		s.ga.ensure( N ) , s.gb.ensure( N );
-		HIPCHECK( hipMemsetAsync( s.ga.p , 0 , N , st ) );
-		KMarkAnc< Dim ><<< Blocks( ncell , 256 ) , 256 , 0 , st >>>( dCells , ncell , D-d , res , s.ga.p );
-		L.fm.alloc( N );
+		s.b.ga.ensure( N ) , s.b.gb.ensure( N );
+		HIPCHECK( hipMemsetAsync( s.b.ga.p , 0 , N , st ) );
+		KMarkAnc< Dim ><<< Blocks( ncell , 256 ) , 256 , 0 , st >>>( dCells , ncell , D-d , res , s.b.ga.p );
+		L.fm.ensure( N );
 		for( int a=0 ; a<Dim ; a++ )
 		{
-			const unsigned char *in = ( a&1 ) ? s.gb.p : s.ga.p;
-			unsigned char *o = a==Dim-1 ? L.fm.p : ( ( a&1 ) ? s.ga.p : s.gb.p );
+			const unsigned char *in = ( a&1 ) ? s.b.gb.p : s.b.ga.p;
+			unsigned char *o = a==Dim-1 ? L.fm.p : ( ( a&1 ) ? s.b.ga.p : s.b.gb.p );
 			KDilate< Dim ><<< Blocks( N , 256 ) , 256 , 0 , st >>>( in , o , res , a , -(K+1) , K , N );
 		}
 		HIPCHECK( hipGetLastError() );
 		// ---- active functions in depth-first (Morton) order
 		const unsigned int nA = s.select( N , FlagIs{ L.fm.p , 1 } );
-		s.keys.ensure( nA );
-		KMortonKeys< Dim ><<< Blocks( nA , 256 ) , 256 , 0 , st >>>( s.sel.p , nA , res , d+1 , s.keys.p );
-		s.vals.ensure( nA );
-		s.sortByKey( nA , ( d+1 )*Dim , s.vals.p );     // vals: the active functions in the hierarchy's order
+		s.b.keys.ensure( nA );
+		KMortonKeys< Dim ><<< Blocks( nA , 256 ) , 256 , 0 , st >>>( s.b.sel.p , nA , res , d+1 , s.b.keys.p );
+		s.b.vals.ensure( nA );
+		s.sortByKey( nA , ( d+1 )*Dim , s.b.vals.p );     // vals: the active functions in the hierarchy's order
 		// ---- colour lists (from sel: increasing dense index; the radix sort is stable)
-		L.act.alloc( nA );
-		s.ckeys.ensure( nA ) , s.ckeys2.ensure( nA );
-		KColourKeys< Dim ><<< Blocks( nA , 256 ) , 256 , 0 , st >>>( s.sel.p , nA , res , s.ckeys.p );
-		s.cub( [&]( void *t , size_t &b ){ return hipcub::DeviceRadixSort::SortPairs( t , b , s.ckeys.p , s.ckeys2.p , s.sel.p , L.act.p , (int)nA , 0 , Dim , st ); } );
+		L.act.ensure( nA );
+		s.b.ckeys.ensure( nA ) , s.b.ckeys2.ensure( nA );
+		KColourKeys< Dim ><<< Blocks( nA , 256 ) , 256 , 0 , st >>>( s.b.sel.p , nA , res , s.b.ckeys.p );
+		s.cub( [&]( void *t , size_t &b ){ return hipcub::DeviceRadixSort::SortPairs( t , b , s.b.ckeys.p , s.b.ckeys2.p , s.b.sel.p , L.act.p , (int)nA , 0 , Dim , st ); } );
 		{
 			std::vector< long long > starts( C , -1 );
-			HIPCHECK( hipMemcpyAsync( s.dstarts.p , starts.data() , C*sizeof(long long) , hipMemcpyHostToDevice , st ) );
-			KColourStarts<<< Blocks( nA , 256 ) , 256 , 0 , st >>>( s.ckeys2.p , nA , s.dstarts.p );
-			HIPCHECK( hipMemcpyAsync( starts.data() , s.dstarts.p , C*sizeof(long long) , hipMemcpyDeviceToHost , st ) );
+			HIPCHECK( hipMemcpyAsync( s.b.dstarts.p , starts.data() , C*sizeof(long long) , hipMemcpyHostToDevice , st ) );
+			KColourStarts<<< Blocks( nA , 256 ) , 256 , 0 , st >>>( s.b.ckeys2.p , nA , s.b.dstarts.p );
+			HIPCHECK( hipMemcpyAsync( starts.data() , s.b.dstarts.p , C*sizeof(long long) , hipMemcpyDeviceToHost , st ) );
 			HIPCHECK( hipStreamSynchronize( st ) );
 			L.actStart[C] = nA;
 			for( int c=C-1 ; c>=0 ; c-- ) L.actStart[c] = starts[c]>=0 ? starts[c] : L.actStart[c+1];
 		}
 		// ---- padding functions by discovery
-		s.pkey.ensure( std::max( N , Ne ) );
-		HIPCHECK( hipMemsetAsync( s.pkey.p , 0xFF , N*sizeof(unsigned long long) , st ) );
-		KPadKeys< Dim ><<< Blocks( nA , 128 ) , 128 , 0 , st >>>( s.vals.p , nA , res , L.fm.p , s.pkey.p );
-		const unsigned int nP = s.select( N , KeySet{ s.pkey.p } );
+		s.b.pkey.ensure( std::max( N , Ne ) );
+		HIPCHECK( hipMemsetAsync( s.b.pkey.p , 0xFF , N*sizeof(unsigned long long) , st ) );
+		KPadKeys< Dim ><<< Blocks( nA , 128 ) , 128 , 0 , st >>>( s.b.vals.p , nA , res , L.fm.p , s.b.pkey.p );
+		const unsigned int nP = s.select( N , KeySet{ s.b.pkey.p } );
 		L.nActive = nA , L.nF = (size_t)nA + nP;
-		L.forder.alloc( L.nF );
-		HIPCHECK( hipMemcpyAsync( L.forder.p , s.vals.p , (size_t)nA*sizeof(unsigned int) , hipMemcpyDeviceToDevice , st ) );
+		L.forder.ensure( L.nF );
+		HIPCHECK( hipMemcpyAsync( L.forder.p , s.b.vals.p , (size_t)nA*sizeof(unsigned int) , hipMemcpyDeviceToDevice , st ) );
 		if( nP )
 		{
-			s.keys.ensure( nP );
-			KGatherKeys<<< Blocks( nP , 256 ) , 256 , 0 , st >>>( s.pkey.p , s.sel.p , nP , s.keys.p );
+			s.b.keys.ensure( nP );
+			KGatherKeys<<< Blocks( nP , 256 ) , 256 , 0 , st >>>( s.b.pkey.p , s.b.sel.p , nP , s.b.keys.p );
 			s.sortByKey( nP , KeyBits( (unsigned long long)nA*W ) , L.forder.p + nA );
 			KSetFlags<<< Blocks( nP , 256 ) , 256 , 0 , st >>>( L.forder.p + nA , nP , L.fm.p , 2 );
 		}
 		// ---- elements by discovery
-		HIPCHECK( hipMemsetAsync( s.pkey.p , 0xFF , Ne*sizeof(unsigned long long) , st ) );
-		KElemKeys< Dim ><<< Blocks( nA , 128 ) , 128 , 0 , st >>>( s.vals.p , nA , res , s.pkey.p );
-		const unsigned int nE = s.select( Ne , KeySet{ s.pkey.p } );
+		HIPCHECK( hipMemsetAsync( s.b.pkey.p , 0xFF , Ne*sizeof(unsigned long long) , st ) );
+		KElemKeys< Dim ><<< Blocks( nA , 128 ) , 128 , 0 , st >>>( s.b.vals.p , nA , res , s.b.pkey.p );
+		const unsigned int nE = s.select( Ne , KeySet{ s.b.pkey.p } );
 		L.nE = nE;
-		L.eorder.alloc( nE );
-		L.em.alloc( Ne );
+		L.eorder.ensure( nE );
+		L.em.ensure( Ne );
 		HIPCHECK( hipMemsetAsync( L.em.p , 0 , Ne , st ) );
-		s.keys.ensure( nE );
-		KGatherKeys<<< Blocks( nE , 256 ) , 256 , 0 , st >>>( s.pkey.p , s.sel.p , nE , s.keys.p );
+		s.b.keys.ensure( nE );
+		KGatherKeys<<< Blocks( nE , 256 ) , 256 , 0 , st >>>( s.b.pkey.p , s.b.sel.p , nE , s.b.keys.p );
 		s.sortByKey( nE , KeyBits( (unsigned long long)nA*W ) , L.eorder.p );
 		KSetFlags<<< Blocks( nE , 256 ) , 256 , 0 , st >>>( L.eorder.p , nE , L.em.p , 1 );
 		HIPCHECK( hipGetLastError() );
@@ -276,6 +270,6 @@ void BuildAdaptedDevice( const int *dCells , int ncell , int D , int K , hipStre
 	HIPCHECK( hipStreamSynchronize( st ) );
 }
 
-template void BuildAdaptedDevice< 3 >( const int * , int , int , int , hipStream_t , std::vector< AdaptedLevelDev > & );
-template void BuildAdaptedDevice< 4 >( const int * , int , int , int , hipStream_t , std::vector< AdaptedLevelDev > & );
+template void BuildAdaptedDevice< 3 >( const int * , int , int , int , hipStream_t , const std::vector< AdaptedLevelDev * > & , AdaptedScratch & );
+template void BuildAdaptedDevice< 4 >( const int * , int , int , int , hipStream_t , const std::vector< AdaptedLevelDev * > & , AdaptedScratch & );
 } // namespace epr

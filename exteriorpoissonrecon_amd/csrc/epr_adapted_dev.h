@@ -17,7 +17,19 @@ struct AdaptedLevelDev
 	long long actStart[17] = {};              // colour c occupies act[ actStart[c] , actStart[c+1] )
 };
 
-// dCells: ncell x Dim cell indices at depth D (device); out receives depths 0..D
+// Device temporaries of the construction, kept between builds (a build of
+// the same size allocates nothing).
+struct AdaptedScratch
+{
+	DevBuf< unsigned char > tmp , ga , gb;
+	DevBuf< unsigned int > sel , vals , ckeys , ckeys2;
+	DevBuf< unsigned long long > keys , keys2 , pkey;
+	DevBuf< unsigned int > dn;
+	DevBuf< long long > dstarts;
+};
+
+// dCells: ncell x Dim cell indices at depth D (device); out[d] (d = 0..D)
+// receives depth d, its buffers grown as needed and reused otherwise
 template< int Dim >
-void BuildAdaptedDevice( const int *dCells , int ncell , int D , int K , hipStream_t st , std::vector< AdaptedLevelDev > &out );
+void BuildAdaptedDevice( const int *dCells , int ncell , int D , int K , hipStream_t st , const std::vector< AdaptedLevelDev * > &out , AdaptedScratch &scratch );
 } // namespace epr

@@ -191,33 +191,38 @@ struct Ctx : public CtxBase
 	int ncell = 0;
 	bool built = false , solved = false;
 
-	struct Level
+	// One depth: the dense storage and, for the adapted hierarchy (empty for
+	// the dense grid), the AdaptedLevelDev sets -- function flags (1 active,
+	// 2 padding), element flags, the numbering, the active nodes by colour --
+	// which the device builder refills in place on every build.
+	struct Level : public AdaptedLevelDev
 	{
 		int res;
 		long long N;
 		long long stride[Dim];
 		ColLayout< Dim > lay;              // colour-grouped column order of R and B
 		DevBuf< double > x , y , R , B , K;
-		// adapted hierarchy (empty for the dense grid): function flags (1 active,
-		// 2 padding), element flags, the numbering, the active nodes by colour
-		DevBuf< unsigned char > fm , em;
-		DevBuf< unsigned int > forder , eorder , act;
-		size_t nF = 0 , nActive = 0 , nE = 0;
-		long long actStart[(1<<Dim)+1] = {};
 		hipGraphExec_t graph = nullptr;    // captured Gauss-Seidel launches of this level
 		int graphIters = 0;
 		const double *graphX = nullptr , *graphR = nullptr;
 		const unsigned char *graphF = nullptr;
+		const unsigned int *graphAct = nullptr;
+		long long graphActStart[(1<<Dim)+1] = {};
 		Level( void ){}
-		Level( Level &&o ) noexcept : res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , fm( std::move( o.fm ) ) , em( std::move( o.em ) ) , forder( std::move( o.forder ) ) , eorder( std::move( o.eorder ) ) , act( std::move( o.act ) ) , nF( o.nF ) , nActive( o.nActive ) , nE( o.nE ) , graph( o.graph ) , graphIters( o.graphIters )
+		Level( Level &&o ) noexcept : AdaptedLevelDev( std::move( o ) ) , res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , graph( o.graph ) , graphIters( o.graphIters )
 		{
-			for( int c=0 ; c<=(1<<Dim) ; c++ ) actStart[c] = o.actStart[c];
 			for( int d=0 ; d<Dim ; d++ ) stride[d] = o.stride[d];
 			o.graph = nullptr;
 		}
 		~Level(){ if( graph ) (void)hipGraphExecDestroy( graph ); }
-		// drops the captured launches when the buffers they point at move
-		void checkGraph( void ){ if( graph && ( graphX!=x.p || graphR!=R.p || graphF!=fm.p ) ){ (void)hipGraphExecDestroy( graph ) ; graph = nullptr; } }
+		// drops the captured launches when the buffers they point at move or the
+		// active lists they cover change
+		void checkGraph( void )
+		{
+			bool same = graphX==x.p && graphR==R.p && graphF==fm.p && graphAct==act.p;
+			for( int c=0 ; c<=(1<<Dim) ; c++ ) same = same && graphActStart[c]==actStart[c];
+			if( graph && !same ){ (void)hipGraphExecDestroy( graph ) ; graph = nullptr; }
+		}
 	};
 	std::vector< Level > levels;
 
@@ -233,6 +238,7 @@ struct Ctx : public CtxBase
 	// the tree's shape is made only to write a .tree
 	bool adapted( void ) const { return prm.aRadius!=-1; }
 	std::unique_ptr< AdaptedHierarchy< Dim > > ah;
+	AdaptedScratch adScratch;   // the device builder's temporaries, kept between builds
 	template< typename T > std::vector< T > download( const DevBuf< T > &b , size_t n )
 	{
 		std::vector< T > h( n );
@@ -395,16 +401,9 @@ struct Ctx : public CtxBase
 		if( adapted() )
 		{
 			AdaptedHierarchy< Dim >::CheckArgs( D , prm.aRadius );
-			std::vector< AdaptedLevelDev > ad;
-			BuildAdaptedDevice< Dim >( dCellIdx.p , ncell , D , prm.aRadius , stream , ad );
-			for( int l=0 ; l<=D ; l++ )
-			{
-				Level &L = levels[l];
-				L.fm = std::move( ad[l].fm ) , L.em = std::move( ad[l].em );
-				L.forder = std::move( ad[l].forder ) , L.eorder = std::move( ad[l].eorder ) , L.act = std::move( ad[l].act );
-				L.nF = ad[l].nF , L.nActive = ad[l].nActive , L.nE = ad[l].nE;
-				for( int c=0 ; c<=C ; c++ ) L.actStart[c] = ad[l].actStart[c];
-			}
+			std::vector< AdaptedLevelDev * > ad;
+			for( int l=0 ; l<=D ; l++ ) ad.push_back( &levels[l] );
+			BuildAdaptedDevice< Dim >( dCellIdx.p , ncell , D , prm.aRadius , stream , ad , adScratch );
 		}
 		else for( int l=0 ; l<=D ; l++ )
 		{
@@ -641,7 +640,8 @@ struct Ctx : public CtxBase
 				HIPCHECK( hipGraphInstantiate( &L.graph , g , nullptr , nullptr , 0 ) );
 				HIPCHECK( hipGraphDestroy( g ) );
 				L.graphIters = iters;
-				L.graphX = L.x.p , L.graphR = L.R.p , L.graphF = L.fm.p;
+				L.graphX = L.x.p , L.graphR = L.R.p , L.graphF = L.fm.p , L.graphAct = L.act.p;
+				for( int c=0 ; c<=C ; c++ ) L.graphActStart[c] = L.actStart[c];
 			}
 			HIPCHECK( hipGraphLaunch( L.graph , stream ) );
 		}
