@@ -22,7 +22,7 @@ EXPORTS = [
     "epr_iso_open", "epr_iso_last_error", "epr_iso_info", "epr_iso_values", "epr_iso_level_set_nodes",
     "epr_iso_extract", "epr_iso_level_set", "epr_iso_write_grid", "epr_iso_seconds", "epr_iso_destroy",
     "epr_indexer_hash", "epr_orient", "epr_write_density", "epr_iso_read_density", "epr_iso_extract_trimmed", "epr_iso_component_densities",
-    "epr_build_batch", "epr_solve_batch",
+    "epr_build_batch", "epr_solve_batch", "epr_density_grid",
 ]
 
 
@@ -78,6 +78,7 @@ def load():
     L.epr_write_density.argtypes = [vp, ctypes.c_char_p]
     L.epr_build_batch.argtypes = [sz, ctypes.POINTER(vp), ci]
     L.epr_solve_batch.argtypes = [sz, ctypes.POINTER(vp), dp]
+    L.epr_density_grid.argtypes = [ctypes.c_char_p, ci, ci, ctypes.c_char_p]
     L.epr_time_gs_sweeps.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
     L.epr_time_gs_sweeps.restype = ctypes.c_int
     up = ctypes.POINTER(ctypes.c_uint)

@@ -2,7 +2,8 @@
 //
 // Same flags, defaults and outputs as ExteriorPoissonRecon/ExteriorPoissonRecon.cpp:52-279:
 // reads <in>.ply, writes <out>.tree and <out>.density.tree (reference byte layouts) and, with --grid,
-// a .grid of the finest (phi1,phi2) values.  Errors print "[ERROR] ..." to
+// a .grid of the finest (phi1,phi2) values, with --densityGrid a .grid of the
+// estimated samples per cell at the finest nodes.  Errors print "[ERROR] ..." to
 // stderr and exit with status 0, as MK_ERROR_OUT does (Misha/Exceptions.h:101-108).
 #include <chrono>
 #include <cstdio>
@@ -25,6 +26,7 @@ static void Usage( const char *ex )
 	printf( "\t --in <input points and frames>\n" );
 	printf( "\t[--out <output voxel grid header>]\n" );
 	printf( "\t[--grid <output .grid of the finest (phi1,phi2)>]\n" );
+	printf( "\t[--densityGrid <output .grid of the samples per cell at the finest nodes; needs --out>]\n" );
 	printf( "\t[--depth <grid depth>=5]\n" );
 	printf( "\t[--minSolveDepth <minimum hierarchy solver depth>=0]\n" );
 	printf( "\t[--maxSolveDepth <maximum hierarchy solver depth>]\n" );
@@ -48,7 +50,7 @@ int main( int argc , char *argv[] )
 {
 	epr_params p;
 	epr_default_params( &p );
-	std::string in , out , grid;
+	std::string in , out , grid , densityGrid;
 	long long aRadius = 3;
 	bool project = false;
 	for( int i=1 ; i<argc ; i++ )
@@ -60,6 +62,7 @@ int main( int argc , char *argv[] )
 		if     ( a=="in" ) in = val();
 		else if( a=="out" ) out = val();
 		else if( a=="grid" ) grid = val();
+		else if( a=="densityGrid" ) densityGrid = val();
 		else if( a=="depth" ) p.depth = atoi( val().c_str() );
 		else if( a=="seed" ) p.seed = atoi( val().c_str() );
 		else if( a=="tRadius" ) p.tRadius = atoi( val().c_str() );
@@ -86,6 +89,7 @@ int main( int argc , char *argv[] )
 	else if( aRadius<0 || aRadius>5 ) Error( "Only kernel radius in the range [0,5] supported: " + std::to_string( (unsigned int)aRadius ) );
 	else p.aRadius = (int)aRadius;
 	if( project ) Error( "--project is not supported" );
+	if( !densityGrid.empty() && out.empty() ) Error( "--densityGrid needs --out (it is computed from <out>.density.tree)" );
 	if( p.maxSolveDepth>p.depth ) p.maxSolveDepth = p.depth;
 	if( in.size()<4 || in.substr( in.size()-4 )!=".ply" ) Error( "Only .ply files supported" );
 	std::string err;
@@ -107,6 +111,7 @@ int main( int argc , char *argv[] )
 		Check( epr_write_density( ctx , ( out + ".density.tree" ).c_str() ) );
 	}
 	if( !grid.empty() ) Check( epr_write_grid( ctx , grid.c_str() ) );
+	if( !densityGrid.empty() && epr_density_grid( ( out + ".density.tree" ).c_str() , s.dim , p.depth , densityGrid.c_str() )!=EPR_OK ) Error( epr_last_error( nullptr ) );
 	if( p.verbose ) printf( "%32s: %.3f (s)\n" , "Executed" , std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count() );
 	epr_destroy( ctx );
 	return EXIT_SUCCESS;

@@ -739,7 +739,8 @@ struct DensityTree
 	std::vector< double > w;              // weightSum
 };
 
-static DensityTree ReadDensity( const char *path , int dim )
+// xform (may be null) receives the file's voxel-to-world matrix, (dim+1)^2 doubles
+static DensityTree ReadDensity( const char *path , int dim , double *xform=nullptr )
 {
 	DensityTree T;
 	T.dim = dim;
@@ -748,6 +749,7 @@ static DensityTree ReadDensity( const char *path , int dim )
 	struct Closer { FILE *f ; ~Closer(){ fclose( f ); } } closer{ r.fp };
 	std::vector< unsigned char > skip( 8 + ( dim+1 )*( dim+1 )*8 );
 	r.read( skip.data() , skip.size() );
+	if( xform ) memcpy( xform , skip.data()+8 , ( dim+1 )*( dim+1 )*sizeof(double) );
 	const size_t head = ( ( 2*( dim+1 ) + 7 )/8 )*8 , rec = head + 16 + 8*( 1 + dim + 1 + dim*dim );
 	std::vector< unsigned char > buf( rec );
 	int height = 0;
@@ -891,7 +893,61 @@ static double SamplesPerCell( const DensityTree &T , const DensityQuery &q , int
 	else if( q.kind==2 ) depth = q.d;
 	else depth = q.d + log( T.target/q.a ) / log( q.b/q.a );
 	depth -= 1.;
-	return T.target * pow( 0.5 , (double)( dim-2 ) * ( T.maxDepth - depth ) );
+	// pow( 0.5 , y ) in the reference, which its clang -O3 build lowers to
+	// exp2( -y ); written out so the result does not depend on that rewrite
+	return T.target * exp2( -( (double)( dim-2 ) * ( T.maxDepth - depth ) ) );
+}
+
+// The voxel coordinates of every node of the (2^D+1)^Dim grid, in the
+// function index order (coordinate 0 most significant).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGridNodes( long long N , int n1 , double *X )
+{
+	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( i>=N ) return;
+	long long t = i;
+	for( int d=Dim-1 ; d>=0 ; d-- ){ X[ i*Dim+d ] = (double)( t % n1 ) ; t /= n1; }
+}
+
+// The density estimator's samples per cell (TreeEstimator::samplesPerCell,
+// GridSamples.inl:845-850, the quantity Extract --trimDensity thresholds) at
+// every node of the depth-`depth` grid, written as a .grid with one DOUBLE per
+// node (Misha/RegularGrid.inl:440-471 layout, the file's voxel-to-world
+// transform in the header, as Extract --grid writes the coefficients).
+template< int Dim >
+static void DensityGrid( const char *densityPath , int depth , const char *gridPath )
+{
+	if( depth<0 || depth>14 ) throw std::string( "depth must be in [0,14]" );
+	double xform[ ( Dim+1 )*( Dim+1 ) ];
+	const DensityTree T = ReadDensity( densityPath , Dim , xform );
+	const int n1 = ( 1<<depth ) + 1;
+	long long N = 1 ; for( int d=0 ; d<Dim ; d++ ) N *= n1;
+	hipStream_t st;
+	HIPCHECK( hipStreamCreate( &st ) );
+	struct StreamCloser { hipStream_t s ; ~StreamCloser(){ (void)hipStreamDestroy( s ); } } closer{ st };
+	DevBuf< double > X , dW;
+	DevBuf< int > dChild;
+	DevBuf< DensityQuery > q;
+	X.alloc( (size_t)N*Dim ) , q.alloc( (size_t)N );
+	dChild.upload( T.child.data() , T.child.size() , st );
+	dW.upload( T.w.data() , T.w.size() , st );
+	KGridNodes< Dim ><<< Blocks( N , 256 ) , 256 , 0 , st >>>( N , n1 , X.p );
+	KDensityQuery< Dim ><<< Blocks( N , 128 ) , 128 , 0 , st >>>( X.p , (size_t)N , depth , dChild.p , dW.p , T.target , q.p );
+	HIPCHECK( hipGetLastError() );
+	std::vector< DensityQuery > hq( (size_t)N );
+	q.download( hq.data() , hq.size() , st );
+	HIPCHECK( hipStreamSynchronize( st ) );
+	std::vector< double > v( (size_t)N );
+	for( long long i=0 ; i<N ; i++ ) v[i] = SamplesPerCell( T , hq[i] , Dim );
+	FILE *fp = fopen( gridPath , "wb" );
+	if( !fp ) throw std::string( "Failed to open grid file for writing: " ) + gridPath;
+	fprintf( fp , "G%d\n" , Dim );
+	fprintf( fp , "1 DOUBLE\n" );
+	for( int d=0 ; d<Dim ; d++ ) fprintf( fp , d==Dim-1 ? "%d\n" : "%d " , n1 );
+	for( int j=0 ; j<=Dim ; j++ ) for( int i=0 ; i<=Dim ; i++ ) fprintf( fp , i==Dim ? "%f\n" : "%f " , (float)xform[ i*( Dim+1 )+j ] );
+	const size_t wrote = fwrite( v.data() , sizeof(double) , (size_t)N , fp );
+	fclose( fp );
+	if( wrote!=(size_t)N ) throw std::string( "Failed to write " ) + gridPath;
 }
 
 // MarchingSimplices::FaceAdjacentConnectedComponents (MarchingSimplices.h:288-365)
@@ -1320,6 +1376,12 @@ int epr_iso_read_density( epr_iso *iso , const char *densityPath )
 		HIPCHECK( hipStreamSynchronize( iso->st ) );
 		iso->density = std::move( T );
 	} );
+}
+
+int epr_density_grid( const char *densityPath , int dim , int depth , const char *gridPath )
+{
+	if( !densityPath || !gridPath || ( dim!=3 && dim!=4 ) ){ epr::ThreadError() = "bad argument" ; return EPR_ERR_ARG; }
+	return epr::ContextFree( [&](){ if( dim==3 ) epr::iso::DensityGrid< 3 >( densityPath , depth , gridPath ) ; else epr::iso::DensityGrid< 4 >( densityPath , depth , gridPath ); } );
 }
 
 int epr_iso_component_densities( const epr_iso *iso , size_t cap , double *out , size_t *count )

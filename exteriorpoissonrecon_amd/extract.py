@@ -107,6 +107,15 @@ def orient(simplices):
     return S
 
 
+def density_grid(density_path, dim, depth, grid_path):
+    """Writes grid_path: the density estimator's samples per cell at every node
+    of the (2^depth+1)^dim grid (a .grid with one DOUBLE per node), from a
+    <out>.density.tree."""
+    if load().epr_density_grid(density_path.encode(), int(dim), int(depth), grid_path.encode()):
+        msg = load().epr_last_error(None)
+        raise EprError(msg.decode() if msg else "epr_density_grid failed")
+
+
 def _ply(props, nrows, rows, faces):
     k = faces.shape[1] if faces.ndim == 2 else 0
     head = ["ply", "format binary_little_endian 1.0", f"element vertex {nrows}"]

@@ -289,6 +289,16 @@ int  epr_iso_extract_trimmed( epr_iso *iso , int all , double trimDensity , epr_
  * extraction, in component order (the values Extract --verbose prints). */
 int  epr_iso_component_densities( const epr_iso *iso , size_t cap , double *out , size_t *count );
 
+/* Context-free: the <out>.density.grid output (the reference's README lists
+ * .grid / .density.grid outputs of an older version; its current code writes
+ * only the .density.tree this reads).  The density estimator's samples per
+ * cell (TreeEstimator::samplesPerCell, Include/GridSamples.inl:845-850, the
+ * quantity Extract --trimDensity thresholds) at every node of the
+ * (2^depth+1)^dim grid, in the function index order, written as a .grid with
+ * one DOUBLE per node (Misha/RegularGrid.inl:440-471 layout, the density
+ * file's voxel-to-world transform in the header). */
+int  epr_density_grid( const char *densityPath , int dim , int depth , const char *gridPath );
+
 /* Replaces Extract.cpp:296-310: the codimension-1 zero set of one component
  * (0 or 1) over the all = 0 triangulation, dim world coordinates per vertex,
  * dim vertices per simplex. */

@@ -510,7 +510,11 @@ double SamplesPerCell( const Density &T , const double *pIn )
 		}
 	}
 	dep -= 1.;
-	return T.target * pow( 0.5 , (double)( dim-2 ) * ( T.maxDepth - dep ) );
+	// pow( 0.5 , y ) in GridSamples.inl:849; the reference's build (clang -O3)
+	// lowers it to exp2( -y ) (LLVM's power-of-two-base pow simplification), and
+	// glibc's exp2 and pow differ in the last bit for some y, so the oracle
+	// follows the reference binary
+	return T.target * exp2( -( (double)( dim-2 ) * ( T.maxDepth - dep ) ) );
 }
 
 // MarchingSimplices::FaceAdjacentConnectedComponents, literally
@@ -604,6 +608,15 @@ extern "C" {
 const char *orc_extract_error( void ){ return g_err.c_str(); }
 
 unsigned long long orc_indexer_hash( int dim , int adapted ){ return IndexerHash( dim , adapted!=0 ); }
+
+// samples per cell (SamplesPerCell above) of a density file at n unit-cube points
+int orc_density_at( const char *densityPath , int dim , size_t n , const double *points , double *out )
+{
+	Density T;
+	if( !ReadDensity( densityPath , dim , T ) ) return -4;
+	for( size_t i=0 ; i<n ; i++ ) out[i] = SamplesPerCell( T , points + i*dim );
+	return 0;
+}
 
 int orc_orient( int k , size_t nS , unsigned *simplices )
 {

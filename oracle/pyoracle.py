@@ -81,6 +81,7 @@ def lib():
         L.orc_indexer_hash.restype = ctypes.c_ulonglong
         L.orc_orient.argtypes = [ctypes.c_int, sz, up]
         L.orc_indexer_hash.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_density_at.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_size_t, dp, dp]
         _lib = L
     return _lib
 
@@ -338,6 +339,16 @@ def extract_level_set(tree, component):
 
 def indexer_hash(dim, adapted):
     return int(lib().orc_indexer_hash(dim, int(adapted)))
+
+
+def density_at(density_path, dim, points):
+    """Samples per cell of a .density.tree at unit-cube points (n x dim)."""
+    pts = np.ascontiguousarray(points, dtype=np.float64)
+    out = np.zeros(len(pts))
+    rc = lib().orc_density_at(density_path.encode(), int(dim), len(pts), _p(pts), _p(out))
+    if rc != 0:
+        raise RuntimeError(lib().orc_extract_error().decode())
+    return out
 
 
 def orient(simplices):

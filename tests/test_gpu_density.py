@@ -74,3 +74,56 @@ def test_gpu_extract_trims_with_product_density(dim, tmp_path):
         mx = [ln for ln in res.stdout.splitlines() if ln.startswith("Max Density")]
         rmx = [ln for ln in rr.stdout.splitlines() if ln.startswith("Max Density")]
         assert mx == rmx
+
+
+def _node_positions(dim, depth):
+    n1 = (1 << depth) + 1
+    idx = np.indices((n1,) * dim).reshape(dim, -1).T      # coordinate 0 most significant
+    return idx / float(1 << depth)
+
+
+@pytest.mark.parametrize("case", [dict(dim=3, depth=4, aRadius=3), dict(dim=3, depth=5, aRadius=-1), dict(dim=4, depth=3, aRadius=2)],
+                         ids=["knot3_d4_r3", "knot3_d5_dense", "clifford4_d3_r2"])
+def test_gpu_density_grid_matches_oracle(case, tmp_path, oracle_lib):
+    """<out>.density.grid (epr_density_grid, ExteriorPoissonRecon --densityGrid):
+    the samples per cell at every finest node equal the oracle's evaluation of
+    the same density file bit for bit; the header is the one Extract --grid
+    writes for the same reconstruction, with one DOUBLE per node."""
+    from exteriorpoissonrecon_amd import Reconstructor, samples
+    from exteriorpoissonrecon_amd.extract import IsoTree, density_grid
+    from conftest import read_grid_values
+    case = dict(case)
+    print("case", case, flush=True)
+    dim, depth = case.pop("dim"), case["depth"]
+    pos, skew = samples.torus_knot(3000, seed=9) if dim == 3 else samples.clifford_torus(side=32, seed=9)
+    r = Reconstructor(pos, skew, **case).build()
+    r.solve()
+    dens, tree, grid = (str(tmp_path / f) for f in ("o.density.tree", "o.tree", "o.density.grid"))
+    r.write_density(dens)
+    r.write_tree(tree, oracle_lib.indexer_hash(dim, case["aRadius"] != -1))
+    r.close()
+    density_grid(dens, dim, depth, grid)
+    v = read_grid_values(grid)
+    ref = oracle_lib.density_at(dens, dim, _node_positions(dim, depth))
+    assert v.shape == ref.shape and np.array_equal(v, ref)
+    assert np.all(v > 0)
+    iso = IsoTree(tree)
+    iso.write_grid(str(tmp_path / "g"))
+    iso.close()
+    head = open(grid, "rb").read()[:-8 * len(v)]
+    gh = open(str(tmp_path / "g.0.grid"), "rb").read()[:-8 * len(v)]
+    assert head == gh and head.startswith(f"G{dim}\n1 DOUBLE\n".encode())
+
+
+def test_gpu_density_grid_command_line(tmp_path, oracle_lib):
+    """ExteriorPoissonRecon --densityGrid writes the same file as epr_density_grid on <out>.density.tree."""
+    from exteriorpoissonrecon_amd import samples
+    from exteriorpoissonrecon_amd.extract import density_grid
+    pos, skew = samples.torus_knot(2048, seed=3)
+    ply = str(tmp_path / "in.ply")
+    samples.write_ply(ply, pos, skew)
+    out, grid = str(tmp_path / "o"), str(tmp_path / "cli.density.grid")
+    res = subprocess.run([CLI, "--in", ply, "--out", out, "--depth", "4", "--densityGrid", grid], capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0 and "[ERROR]" not in res.stderr, res.stderr
+    density_grid(out + ".density.tree", 3, 4, str(tmp_path / "api.density.grid"))
+    assert open(grid, "rb").read() == open(str(tmp_path / "api.density.grid"), "rb").read()

@@ -181,3 +181,19 @@ def test_spiral_frames_are_tangent_duals():
     # Dual(t): n((d+1)%3,(d+2)%3) = t[d]
     for d in range(3):
         assert np.allclose(skew[:, (d + 1) % 3, (d + 2) % 3], tan[:, d])
+
+
+def test_cli_density_grid_needs_out(tmp_path):
+    ply = str(tmp_path / "in.ply")
+    pos, skew = samples.torus_knot(64, seed=1)
+    samples.write_ply(ply, pos, skew)
+    cli = os.path.join(ROOT, "exteriorpoissonrecon_amd", "bin", "ExteriorPoissonRecon")
+    res = subprocess.run([cli, "--in", ply, "--densityGrid", str(tmp_path / "d.grid")], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 0 and "--densityGrid needs --out" in res.stderr
+
+
+def test_density_grid_argument_errors():
+    from exteriorpoissonrecon_amd._lib import load
+    L = load()
+    assert L.epr_density_grid(None, 3, 4, b"x.grid") != 0
+    assert L.epr_density_grid(b"missing.density.tree", 5, 4, b"x.grid") != 0
