@@ -1320,6 +1320,19 @@ __device__ __forceinline__ void GalerkinTBlock( const double *Mf , double *T , i
 		k = 0;
 		for( int d=0 ; d<Dim ; d++ ) k = k*( fres+1 ) + K[d];
 	}
+	if constexpr( Dim==4 )
+	{
+		// 4D: 81 accumulators per thread already hold occupancy at two waves
+		// per SIMD; staging 64 columns in LDS (41 KB) would cut it below one
+		if( k<0 ) return;
+		double acc[W];
+#pragma unroll
+		for( int w=0 ; w<W ; w++ ) acc[w] = 0;
+		GalerkinColumn< Dim , CLS >( Mf + ( colBase + g )*W , K , fres , cres , acc , std::make_integer_sequence< int , W >{} );
+#pragma unroll
+		for( int w=0 ; w<W ; w++ ) T[ k*W + w ] = acc[w];
+		return;
+	}
 	knode[lane] = k;
 	__syncthreads();
 	// the block's columns are consecutive in the colour-grouped layout
@@ -1349,7 +1362,7 @@ __device__ __forceinline__ void GalerkinTDispatch( int cls , const double *Mf , 
 template< int Dim >
 __global__ void __launch_bounds__( 64 ) KGalerkinT( const double *Mf , double *T , int cres , ColLayout< Dim > lay )
 {
-	__shared__ double buf[ 64*DimC< Dim >::W ];
+	__shared__ double buf[ Dim==4 ? 1 : 64*DimC< Dim >::W ];
 	__shared__ long long knode[64];
 	const int cls = blockIdx.y;
 	GalerkinTDispatch< Dim >( cls , Mf , T , cres , lay.cstart[cls] , blockIdx.x , buf , knode , std::make_integer_sequence< int , DimC< Dim >::C >{} );
