@@ -9,7 +9,9 @@
  * The entry points replace the reference's C++ call boundaries of the
  * reconstruction (dense grid and adapted hierarchy), the multigrid solver and
  * the Extract program; each declaration names the reference interface it
- * replaces (paths relative to the reference tree).
+ * replaces (paths relative to the reference tree).  The few without a
+ * reference counterpart (many reconstructions at once, the density grid,
+ * the timing hooks) say so.
  */
 #ifndef EPR_H
 #define EPR_H
