@@ -143,14 +143,17 @@ def clifford_torus(side=256, ang_deg=2.0, seed=0):
 
 
 def write_ply(path, pos, skew):
-    """Binary little-endian PLY with x,y,z[,w] and skew_c_r (double)."""
+    """Binary little-endian PLY with x,y,z[,w] and skew_c_r (double): the
+    bytes the reference's Sample writes for the same values
+    (PLY::WritePoints with a position and a "skew" matrix factory,
+    Sample/Sample.cpp:225-229)."""
     n, dim = pos.shape
     names = ["x", "y", "z", "w"][:dim]
     props = names + [f"skew_{c}_{r}" for c in range(dim) for r in range(dim)]
     header = "ply\nformat binary_little_endian 1.0\n"
     header += f"element vertex {n}\n"
     header += "".join(f"property double {p}\n" for p in props)
-    header += "element face 0\nproperty list uchar int vertex_indices\nend_header\n"
+    header += "end_header\n"
     data = np.concatenate([pos, skew.reshape(n, dim * dim)], axis=1).astype("<f8")
     with open(path, "wb") as f:
         f.write(header.encode("ascii"))
