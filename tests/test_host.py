@@ -4,6 +4,7 @@ reproduces the reference's argument and error behaviour without touching a GPU."
 import os
 import re
 import subprocess
+import sys
 
 import numpy as np
 import pytest
@@ -220,3 +221,9 @@ def test_ply_writer_matches_reference_sample():
             q = os.path.join(td, "out.ply")
             samples.write_ply(q, pos, skew)
             assert open(q, "rb").read() == ref, name
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with a message, before touching a device."""
+    res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True, timeout=120)
+    assert res.returncode == 2 and "one GPU" in res.stderr and res.stdout == ""
