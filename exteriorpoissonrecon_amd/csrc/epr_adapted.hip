@@ -170,10 +170,9 @@ struct Scratch
 	}
 	unsigned int count( void )
 	{
-		unsigned int h = 0;
-		HIPCHECK( hipMemcpyAsync( &h , b.dn.p , sizeof(h) , hipMemcpyDeviceToHost , st ) );
+		HIPCHECK( hipMemcpyAsync( b.hCount , b.dn.p , sizeof(unsigned int) , hipMemcpyDeviceToHost , st ) );
 		HIPCHECK( hipStreamSynchronize( st ) );
-		return h;
+		return *b.hCount;
 	}
 	// increasing indices i < n with pred(i), into sel; returns the count
 	template< typename Pred > unsigned int select( long long n , Pred pred )
@@ -200,6 +199,8 @@ void BuildAdaptedDevice( const int *dCells , int ncell , int D , int K , hipStre
 	Scratch s{ scratch , st };
 	scratch.dn.ensure( 1 );
 	scratch.dstarts.ensure( C );
+	if( !scratch.hCount ) HIPCHECK( hipHostMalloc( (void **)&scratch.hCount , sizeof(unsigned int) ) );
+	if( !scratch.hStarts ) HIPCHECK( hipHostMalloc( (void **)&scratch.hStarts , 17*sizeof(long long) ) );
 	for( int d=0 ; d<=D ; d++ )
 	{
 		AdaptedLevelDev &L = *out[d];
@@ -230,10 +231,10 @@ void BuildAdaptedDevice( const int *dCells , int ncell , int D , int K , hipStre
 		KColourKeys< Dim ><<< Blocks( nA , 256 ) , 256 , 0 , st >>>( s.b.sel.p , nA , res , s.b.ckeys.p );
 		s.cub( [&]( void *t , size_t &b ){ return hipcub::DeviceRadixSort::SortPairs( t , b , s.b.ckeys.p , s.b.ckeys2.p , s.b.sel.p , L.act.p , (int)nA , 0 , Dim , st ); } );
 		{
-			std::vector< long long > starts( C , -1 );
-			HIPCHECK( hipMemcpyAsync( s.b.dstarts.p , starts.data() , C*sizeof(long long) , hipMemcpyHostToDevice , st ) );
+			const long long *starts = scratch.hStarts;
+			HIPCHECK( hipMemsetAsync( s.b.dstarts.p , 0xFF , C*sizeof(long long) , st ) );   // -1: colour absent
 			KColourStarts<<< Blocks( nA , 256 ) , 256 , 0 , st >>>( s.b.ckeys2.p , nA , s.b.dstarts.p );
-			HIPCHECK( hipMemcpyAsync( starts.data() , s.b.dstarts.p , C*sizeof(long long) , hipMemcpyDeviceToHost , st ) );
+			HIPCHECK( hipMemcpyAsync( scratch.hStarts , s.b.dstarts.p , C*sizeof(long long) , hipMemcpyDeviceToHost , st ) );
 			HIPCHECK( hipStreamSynchronize( st ) );
 			L.actStart[C] = nA;
 			for( int c=C-1 ; c>=0 ; c-- ) L.actStart[c] = starts[c]>=0 ? starts[c] : L.actStart[c+1];

@@ -26,6 +26,13 @@ struct AdaptedScratch
 	DevBuf< unsigned long long > keys , keys2 , pkey;
 	DevBuf< unsigned int > dn;
 	DevBuf< long long > dstarts;
+	// pinned host words for the counts and colour starts read back at every depth
+	unsigned int *hCount = nullptr;
+	long long *hStarts = nullptr;
+	AdaptedScratch( void ){}
+	AdaptedScratch( const AdaptedScratch & ) = delete;
+	AdaptedScratch &operator=( const AdaptedScratch & ) = delete;
+	~AdaptedScratch(){ if( hCount ) (void)hipHostFree( hCount ) ; if( hStarts ) (void)hipHostFree( hStarts ); }
 };
 
 // dCells: ncell x Dim cell indices at depth D (device); out[d] (d = 0..D)
