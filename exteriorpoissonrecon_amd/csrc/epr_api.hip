@@ -233,6 +233,7 @@ struct Ctx : public CtxBase
 	std::vector< std::unique_ptr< DevBuf< double > > > estData , skData;
 	std::vector< std::unique_ptr< DevBuf< unsigned char > > > skKids , estKids;
 	double c_ = 0;
+	long long cElements = 1;   // elements the target-field mean of c_ runs over
 	// --aRadius >= 0: the adapted hierarchy, built on the device with the other
 	// build-phase data (levels[].fm/em/forder/eorder/act); the host copy with
 	// the tree's shape is made only to write a .tree
@@ -486,10 +487,8 @@ struct Ctx : public CtxBase
 			dTmp.ensure( 8 );
 			KPartialSum<<< nb , 256 , 0 , stream >>>( dS2.p , ne , dPartials.p );
 			KSumRows<1><<< 1 , 256 , 0 , stream >>>( dPartials.p , nb , dTmp.p );
-			double csum;
-			dTmp.download( &csum , 1 , stream );
-			HIPCHECK( hipStreamSynchronize( stream ) );
-			c_ = csum / (double)ne;
+			// read back with the build's final synchronisation (nothing later in the build writes dTmp)
+			cElements = ne;
 		}
 		// ---- finest R and B, then the Galerkin chain
 		{
@@ -538,7 +537,12 @@ struct Ctx : public CtxBase
 			if( !L.K.n ){ std::vector< double > k = hat::StiffnessCaseRows< Dim >( L.res ) ; L.K.upload( k.data() , k.size() , stream ); }   // depends on the resolution only
 			L.checkGraph();
 		}
-		HIPCHECK( hipStreamSynchronize( stream ) );
+		{
+			double csum;
+			dTmp.download( &csum , 1 , stream );
+			HIPCHECK( hipStreamSynchronize( stream ) );
+			c_ = csum / (double)cElements;
+		}
 		built = true , solved = false;
 		seconds[0] = std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count();
 	}
