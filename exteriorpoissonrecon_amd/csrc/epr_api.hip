@@ -1064,7 +1064,7 @@ struct Ctx : public CtxBase
 				double *o = (double *)( buf.data() + head + 16 );
 				o[0] = v[0];
 				for( int d=0 ; d<Dim ; d++ ) o[1+d] = v[1+d];
-				for( int k=0 ; k<Dim*Dim ; k++ ) o[2+Dim+k] = v[1+Dim+k];
+				for( int i=0 ; i<Dim ; i++ ) for( int j=0 ; j<Dim ; j++ ) o[ 2+Dim+i*Dim+j ] = v[ 1+Dim+( i>j ? j*Dim+i : i*Dim+j ) ];   // lower covariance mirrored (EstLower)
 			}
 			fwrite( buf.data() , 1 , rec , fp );
 		}

@@ -702,6 +702,17 @@ __device__ __forceinline__ bool EstTap( const double *p , int t , int tap , int 
 
 // The coarse depths (few corners, every cell hits them): per-workgroup sums in
 // LDS, one global atomic per corner value and workgroup.
+// Estimator values per node: weight, position sum (Dim), covariance sum
+// (Dim x Dim, symmetric).  Only the diagonal and upper covariance entries are
+// splatted; readers mirror the lower ones (EstMirror), which keeps the sums
+// exactly symmetric, as the reference's ordered sums are, with 29 % (4D) /
+// 23 % (3D) fewer atomics.
+template< int Dim > __host__ __device__ constexpr bool EstLower( int k ){ return k>Dim && ( k-1-Dim )/Dim > ( k-1-Dim )%Dim; }
+template< int Dim > __host__ __device__ inline void EstMirror( double *v )
+{
+	for( int i=1 ; i<Dim ; i++ ) for( int j=0 ; j<i ; j++ ) v[ 1+Dim+i*Dim+j ] = v[ 1+Dim+j*Dim+i ];
+}
+
 template< int Dim , int KR >
 __global__ void __launch_bounds__( 256 ) KEstSplatLds( const double *cellP , const double *cellData , int ncell , TreeGrids< Dim > T , const double *depthScale , int t )
 {
@@ -720,7 +731,7 @@ __global__ void __launch_bounds__( 256 ) KEstSplatLds( const double *cellP , con
 		double scl;
 		if( !EstTap< Dim , KR >( cellP + (long long)c*Dim , t , tap , o , scl ) ) continue;
 		double *node = acc + CornerIndex< Dim >( t , o )*NV;
-		for( int k=0 ; k<NV ; k++ ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
+		for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
 	}
 	__syncthreads();
 	for( int i=threadIdx.x ; i<corners*NV ; i+=blockDim.x ) if( acc[i]!=0 ) atomicAdd( T.data[t]+i , acc[i] );
@@ -743,7 +754,7 @@ __global__ void KEstSplat( const double *cellP , const double *cellData , int nc
 	if( !EstTap< Dim , KR >( cellP + (long long)c*Dim , t , tap , o , scl ) ) return;
 	double *node = T.data[t] + CornerIndex< Dim >( t , o )*NV;
 	const double dw = depthScale[t-1];
-	for( int k=0 ; k<NV ; k++ ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
+	for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
 }
 
 // Linear interpolation of a tree depth at p (GridSamples.inl:774-809).
@@ -821,15 +832,15 @@ __global__ void KEstQuery( const double *pos , const double *skewPacked , int n 
 		}
 	}
 	double info[NV];
-	if( !( dep<maxT ) ) TreeEval< Dim , NV >( T , maxT , p , info , NV );      // also catches NaN
-	else if( !( dep>=1 ) ) TreeEval< Dim , NV >( T , 1 , p , info , NV );
+	if( !( dep<maxT ) ) TreeEval< Dim , NV >( T , maxT , p , info , NV ) , EstMirror< Dim >( info );      // also catches NaN
+	else if( !( dep>=1 ) ) TreeEval< Dim , NV >( T , 1 , p , info , NV ) , EstMirror< Dim >( info );
 	else
 	{
 		const int d1 = min( max( (int)floor( dep ) , 1 ) , maxT-1 ) , d2 = d1+1;
 		const double dd = dep-d1;
 		double a[NV] , b[NV];
-		TreeEval< Dim , NV >( T , d1 , p , a , NV );
-		TreeEval< Dim , NV >( T , d2 , p , b , NV );
+		TreeEval< Dim , NV >( T , d1 , p , a , NV ) , EstMirror< Dim >( a );
+		TreeEval< Dim , NV >( T , d2 , p , b , NV ) , EstMirror< Dim >( b );
 		for( int k=0 ; k<NV ; k++ ) info[k] = a[k]*(1.-dd) + b[k]*dd;
 	}
 	const double w = info[0];
