@@ -37,7 +37,7 @@ sys.path.insert(0, ROOT)
 METRIC = "time-to-energy (1e-6 rel vs FP64) at depth 7 3D / 6 4D; GS sweep HBM GB/s vs peak"
 FP64_PEAK_TFLOPS = 78.6
 HBM_PEAK_GBS = 8000.0
-# Algorithmic FP64 flops of one node update in this build's element-local
+# Algorithmic FP64 flops of one node update in the element-local
 # formulation: per incident element 3 per corner pair (wedge products), 2 per
 # nonzero of the cubic table (130 / 1481), 4 per nonzero of the quadratic table
 # (37 / 225), 20 per table row for the dot products; 12 per window term; 20 for
@@ -255,8 +255,8 @@ def main():
                     help="also time the Extract pipeline on the .tree the last step writes (an 'extract' object in the JSON line)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build measures one GPU only (--gpus {args.gpus} requested); "
-              "the solve path is single-device, see DESIGN.md for how it would shard", file=sys.stderr)
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
+              "the solve path runs on a single device", file=sys.stderr)
         sys.exit(2)
 
     from exteriorpoissonrecon_amd import Reconstructor, samples

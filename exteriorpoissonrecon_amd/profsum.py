@@ -11,7 +11,7 @@
         B bytes written per dispatch, launched by bench.py through epr_time_copy)
 
 Counter values are KiB per dispatch.  The scale factor of a counter is
-(known KiB of the copy) / (counter value of the copy); DESIGN.md records it.
+(known KiB of the copy) / (counter value of the copy); the summary prints it.
 """
 import argparse
 import csv

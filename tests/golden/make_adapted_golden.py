@@ -26,7 +26,7 @@ from exteriorpoissonrecon_amd.treefile import TreeFile  # noqa: E402
 REF = os.path.join(ROOT, "oracle", "_ref", "ExteriorPoissonRecon")
 
 CASES = {
-    # the reference's default radius on BASELINE config 1 at depths 4 and 5
+    # the reference's default radius on the 1024-sample torus knot at depths 4 and 5
     "adapted_knot3_d4": dict(gen=lambda: samples.torus_knot(1024), args=dict(depth=4, aRadius=3)),
     "adapted_knot3_d5": dict(gen=lambda: samples.torus_knot(1024), args=dict(depth=5, aRadius=3)),
     # small radii: padding and sparse coarse levels

@@ -22,9 +22,9 @@ from oracle.pyoracle import load_dump  # noqa: E402
 REF_DUMP = os.path.join(ROOT, "oracle", "_ref", "ref_dump")
 
 CASES = {
-    # BASELINE config 1 at reduced depth (full dump)
+    # the 1024-sample (2,3) torus knot at reduced depth (full dump)
     "knot3_d4": dict(gen=lambda: samples.torus_knot(1024), args=dict(depth=4)),
-    # BASELINE config 1 itself: depth 5 (finest arrays subsampled)
+    # the same knot at depth 5 (finest arrays subsampled)
     "knot3_d5": dict(gen=lambda: samples.torus_knot(1024), args=dict(depth=5), stride=7,
                      keep=("unit_pos", "cells", "cell_counts", "measure", "noise", "sample_depth", "S_elem", "c",
                            "init_W", "init_x", "init_y", "final_E", "final_x", "final_y", "E5", "W5", "Q5",

@@ -5,7 +5,7 @@ larger cases with the CPU oracle on the same inputs.  Tolerances: the GPU
 sums in a different order (element-wise wedge terms, FP64 atomics in the
 splats, device libm), so values agree to rounding, amplified along the
 non-convex Gauss-Seidel trajectory.  The acceptance bar for the solve is the
-north-star's 1e-6 relative on (E, phi1, phi2); observed errors are far below.
+1e-6 relative on (E, phi1, phi2); observed errors are far below.
 """
 import os
 import subprocess
