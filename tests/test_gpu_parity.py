@@ -189,3 +189,19 @@ def test_cli_tree_output(tmp_path, oracle_lib):
         assert np.abs(gx - g["final_x"]).max() < TOL_SOLVE * scale
     grid = open(str(tmp_path / "o.grid"), "rb").read()
     assert grid.startswith(b"G3\n2 DOUBLE\n17 17 17\n")
+
+
+def test_sample_input_errors():
+    """No samples and non-finite positions are refused before any kernel runs."""
+    from exteriorpoissonrecon_amd import EprError, Reconstructor, samples
+    pos, skew = samples.torus_knot(256, seed=1)
+    print("case input errors", flush=True)
+    with pytest.raises(EprError, match="Expected some samples"):
+        Reconstructor(pos[:0], skew[:0], depth=3).build()
+    bad = pos.copy()
+    bad[3, 1] = np.nan
+    with pytest.raises(EprError, match="non-finite"):
+        Reconstructor(bad, skew, depth=3).build()
+    bad[3, 1] = np.inf
+    with pytest.raises(EprError, match="non-finite"):
+        Reconstructor(bad, skew, depth=3).build()

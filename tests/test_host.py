@@ -227,3 +227,17 @@ def test_bench_refuses_more_than_one_gpu():
     """bench.py measures one GPU: --gpus above 1 exits with a message, before touching a device."""
     res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True, timeout=120)
     assert res.returncode == 2 and "one GPU" in res.stderr and res.stdout == ""
+
+
+@pytest.mark.parametrize("kw,msg", [
+    (dict(depth=13), "depth must be in"),
+    (dict(depth=3, aRadius=6), "Only kernel radius in the range"),
+    (dict(depth=3, tRadius=5), "Kernel radius must be in range"),
+    (dict(depth=3, dRadius=5), "Only radii"),
+])
+def test_parameter_range_errors(kw, msg):
+    """Parameter ranges are checked before any device call (the reference's own messages)."""
+    from exteriorpoissonrecon_amd import EprError, Reconstructor, samples
+    pos, skew = samples.torus_knot(256, seed=1)
+    with pytest.raises(EprError, match=msg):
+        Reconstructor(pos, skew, **kw).build()
