@@ -778,24 +778,33 @@ __device__ __forceinline__ void TreeEval( const TreeGrids< Dim > &T , int t , co
 	}
 }
 
-// Symmetric eigenvalues by cyclic Jacobi, ascending.
+// Symmetric eigenvalues by cyclic Jacobi, ascending.  Up to 64 sweeps, ending
+// early when the off-diagonal sum of squares is zero or when a sweep leaves
+// every entry bit for bit unchanged: the sweep is then a fixed point, so the
+// remaining sweeps would change nothing and the result equals the full 64
+// sweeps'.  Tiny off-diagonals whose theta^2 overflows give t = 0 (identity
+// rotations) while their squares stay nonzero, so without this test most
+// estimator covariances ran all 64 sweeps instead of 6-20.
 template< int Dim >
 __device__ void SymEig( double A[Dim][Dim] , double *ev )
 {
+	auto same = []( double a , double b ){ return __double_as_longlong( a )==__double_as_longlong( b ); };
 	for( int sweep=0 ; sweep<64 ; sweep++ )
 	{
 		double off = 0;
 		for( int i=0 ; i<Dim ; i++ ) for( int j=i+1 ; j<Dim ; j++ ) off += A[i][j]*A[i][j];
 		if( off==0 ) break;
+		bool changed = false;
 		for( int p=0 ; p<Dim ; p++ ) for( int q=p+1 ; q<Dim ; q++ )
 		{
 			if( A[p][q]==0 ) continue;
 			const double theta = ( A[q][q]-A[p][p] ) / ( 2*A[p][q] );
 			const double t = ( theta>=0 ? 1. : -1. ) / ( fabs( theta ) + sqrt( theta*theta+1 ) );
 			const double c = 1./sqrt( t*t+1 ) , s = t*c;
-			for( int k=0 ; k<Dim ; k++ ){ const double akp = A[k][p] , akq = A[k][q] ; A[k][p] = c*akp - s*akq ; A[k][q] = s*akp + c*akq; }
-			for( int k=0 ; k<Dim ; k++ ){ const double apk = A[p][k] , aqk = A[q][k] ; A[p][k] = c*apk - s*aqk ; A[q][k] = s*apk + c*aqk; }
+			for( int k=0 ; k<Dim ; k++ ){ const double akp = A[k][p] , akq = A[k][q] ; A[k][p] = c*akp - s*akq ; A[k][q] = s*akp + c*akq ; changed |= !same( A[k][p] , akp ) || !same( A[k][q] , akq ); }
+			for( int k=0 ; k<Dim ; k++ ){ const double apk = A[p][k] , aqk = A[q][k] ; A[p][k] = c*apk - s*aqk ; A[q][k] = s*apk + c*aqk ; changed |= !same( A[p][k] , apk ) || !same( A[q][k] , aqk ); }
 		}
+		if( !changed ) break;
 	}
 	for( int i=0 ; i<Dim ; i++ ) ev[i] = A[i][i];
 	for( int i=1 ; i<Dim ; i++ ) for( int j=i ; j>0 && ev[j-1]>ev[j] ; j-- ){ const double t = ev[j] ; ev[j] = ev[j-1] ; ev[j-1] = t; }
@@ -818,18 +827,20 @@ __global__ void KEstQuery( const double *pos , const double *skewPacked , int n 
 	auto spc = [&]( int t ){ double w ; TreeEval< Dim , NV >( T , t , p , &w , 1 ) ; return w / (double)( 1<<( (t-1)*(Dim-CoDim) ) ); };
 	double dep;
 	{
+		// walk down from the finest depth while the count stays below the
+		// target; the last two counts evaluated are the bracketing ones
+		// (cur at d1, prev at d2 = d1+1), so none is evaluated twice
 		int d1 = maxT , d2 = maxT+1;
-		while( d2>1 && spc( d1 )<target ) d1-- , d2--;
-		if( d2==1 || d2==maxT+1 )
+		double cur = 0 , prev = 0;
+		while( d2>1 )
 		{
-			const int d = d2==1 ? 1 : maxT;
-			dep = d + log( spc( d )/target ) / log( 2. );
+			const double v = spc( d1 );
+			if( !( v<target ) ){ cur = v ; break; }   // also stops on NaN
+			prev = v , d1-- , d2--;
 		}
-		else
-		{
-			const double s1 = spc( d1 ) , s2 = spc( d2 );
-			dep = !s2 ? (double)d1 : d1 + log( target/s1 ) / log( s2/s1 );
-		}
+		if( d2==1 ) dep = 1 + log( prev/target ) / log( 2. );
+		else if( d2==maxT+1 ) dep = maxT + log( cur/target ) / log( 2. );
+		else dep = !prev ? (double)d1 : d1 + log( target/cur ) / log( prev/cur );
 	}
 	double info[NV];
 	if( !( dep<maxT ) ) TreeEval< Dim , NV >( T , maxT , p , info , NV ) , EstMirror< Dim >( info );      // also catches NaN
