@@ -62,6 +62,16 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# where each pinned energy of tests/golden/bench_reference.json comes from
+# (tests/golden/make_bench_reference.py writes them)
+REFERENCE_SOURCE = {
+    "borromean3d_d7": "reference build (oracle/_ref/ref_dump), same samples and schedule",
+    "knot3d_d5": "reference build (oracle/_ref/ref_dump), same samples and schedule",
+    "clifford4d_d5": "reference build (oracle/_ref/ref_dump), same samples and schedule",
+    "clifford4d_d6": "CPU oracle (oracle/epr_oracle.cpp, the reference's summation order), same samples and schedule",
+}
+
+
 def reference_energy(name, a_radius=-1):
     path = os.path.join(ROOT, "tests", "golden", "bench_reference.json")
     if not os.path.exists(path):
@@ -117,7 +127,10 @@ def cpu_baseline(name, wl, pos, skew, a_radius=-1):
         if r.returncode != 0:
             return None
     sample = f"{name} samples, reference CLI --depth {depth} --aRadius {a_radius} (full build+solve), {cores} threads"
-    return {"value": dt, "unit": "s", "cores": cores, "kind": "reference", "sample": sample}
+    if depth != wl["depth"]:
+        sample += (f"; a depth-{depth} leg, one level coarser than the GPU's depth {wl['depth']} "
+                   f"(about 1/16 of the nodes), so it is not a same-size comparison")
+    return {"value": dt, "unit": "s", "cores": cores, "kind": "reference", "sample": sample, "depth": depth}
 
 
 def batch_main(args, wl):
@@ -239,57 +252,42 @@ def extract_cpu_baseline(td, dim):
     return time.perf_counter() - t0
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="borromean3d_d7", choices=sorted(WORKLOADS))
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
-    ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
-    ap.add_argument("--aRadius", type=int, default=-1, help="-1: dense grid; 0..5: adapted hierarchy of that radius")
-    ap.add_argument("--serial", action="store_true",
-                    help="batch workload: one build thread and the depth groups one after another (for profilers)")
-    ap.add_argument("--no-unbatched", action="store_true", help="batch workload: skip the one-at-a-time comparison")
-    ap.add_argument("--extract", action="store_true",
-                    help="also time the Extract pipeline on the .tree the last step writes (an 'extract' object in the JSON line)")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
-              "the solve path runs on a single device", file=sys.stderr)
-        sys.exit(2)
-
+def measure_workload(name, args, steps, warmup, with_cpu, extract=False):
+    """Times `steps` whole reconstructions of one workload (after `warmup`
+    untimed ones) and its finest-sweep kernel; returns (result dict, last
+    Reconstructor).  A step is set_samples (unit-cube map and the ordered
+    sampler's host sort, OrderedSamples.inl:32-85) + build + solve."""
     from exteriorpoissonrecon_amd import Reconstructor, samples
-    wl = WORKLOADS[args.config]
-    if "batch" in wl:
-        return batch_main(args, wl)
+    wl = WORKLOADS[name]
     pos, skew = wl["gen"](samples)
     dim, depth = wl["dim"], wl["depth"]
-    log(f"[bench] {args.config}: {pos.shape[0]} samples, dim {dim}, depth {depth}")
+    log(f"[bench] {name}: {pos.shape[0]} samples, dim {dim}, depth {depth}")
     rec = Reconstructor(pos, skew, depth=depth, aRadius=args.aRadius)
 
     E = None
-    for i in range(args.warmup):
-        rec.build(); E = rec.solve()
+    for i in range(warmup):
+        rec.set_samples(pos, skew); rec.build(); E = rec.solve()
         log(f"[bench] warmup {i}: build {rec.seconds('build'):.3f}s solve {rec.seconds('solve'):.3f}s E={E.tolist()}")
-    times, builds, solves = [], [], []
-    for i in range(args.steps):
+    times, sets, builds, solves = [], [], [], []
+    for i in range(steps):
         t0 = time.perf_counter()
+        rec.set_samples(pos, skew)
+        t1 = time.perf_counter()
         rec.build()
         E = rec.solve()
         dt = time.perf_counter() - t0
-        times.append(dt); builds.append(rec.seconds("build")); solves.append(rec.seconds("solve"))
-        log(f"[bench] step {i}: {dt:.4f}s (build {builds[-1]:.4f} solve {solves[-1]:.4f}) E={E.tolist()}")
+        times.append(dt); sets.append(t1 - t0); builds.append(rec.seconds("build")); solves.append(rec.seconds("solve"))
+        log(f"[bench] step {i}: {dt:.4f}s (set_samples {sets[-1]:.4f} build {builds[-1]:.4f} solve {solves[-1]:.4f}) E={E.tolist()}")
     t = float(np.mean(times))
 
-    Eref = reference_energy(args.config, args.aRadius)
+    Eref = reference_energy(name, args.aRadius)
     rel = None
     if Eref is not None:
         rel = abs(float(E.sum()) - float(sum(Eref))) / abs(float(sum(Eref)))
         log(f"[bench] E={E.sum():.17g} reference {sum(Eref):.17g} rel {rel:.3e}")
 
     # dominant kernel: one colour phase of the finest-level sweep
-    iters = int(np.ceil(10))
+    iters = 10
     N = rec.num_functions(depth)
     ms, launches = rec.time_gauss_seidel(depth, iters)
     per_launch_s = ms * 1e-3 / launches
@@ -304,50 +302,47 @@ def main():
         f"{per_launch_s * 1e6:.1f} us/launch, {achieved:.2f} TFLOP/s algorithmic "
         f"({achieved_ref:.2f} reference-equivalent), "
         f"{bytes_per_launch / per_launch_s / 1e9:.1f} GB/s algorithmic")
+    levels = None
     if args.detail:
+        levels = []
         for d in range(depth + 1):
             it = int(np.ceil(2.0 ** (depth - d) * 10))
             lms, ln = rec.time_gauss_seidel(d, it)
+            levels.append({"level": d, "nodes": int(rec.num_functions(d)), "iters": it, "ms": lms, "launches": ln})
             log(f"[bench]   level {d}: N={rec.num_functions(d)} iters={it} {lms:.3f} ms ({ln} launches)")
 
     copy_gbs = copy_bandwidth()
     if copy_gbs:
         log(f"[bench] copy kernel: {copy_gbs:.1f} GB/s ({2 * COPY_BYTES} bytes per copy)")
-    traffic = measured_traffic(args.config if args.aRadius < 0 else f"{args.config}_r{args.aRadius}")
+    traffic = measured_traffic(name if args.aRadius < 0 else f"{name}_r{args.aRadius}")
     if traffic:
         log(f"[bench] measured HBM traffic of the finest-sweep kernel: {traffic / 1e6:.2f} MB per launch "
             f"({traffic / per_launch_s / 1e9:.1f} GB/s at this launch time)")
 
-    extract, extract_dir = extract_timing(rec, args, dim) if args.extract else (None, None)
-
-    cpu = None if args.no_cpu else cpu_baseline(args.config, wl, pos, skew, args.aRadius)
+    ext, extract_dir = extract_timing(rec, args, dim) if extract else (None, None)
+    cpu = cpu_baseline(name, wl, pos, skew, args.aRadius) if with_cpu else None
     if extract_dir:
-        if not args.no_cpu:
-            extract["reference_cpu_s"] = extract_cpu_baseline(extract_dir, dim)
-            log(f"[bench] reference Extract: {extract['reference_cpu_s']}")
+        if with_cpu:
+            ext["reference_cpu_s"] = extract_cpu_baseline(extract_dir, dim)
+            log(f"[bench] reference Extract: {ext['reference_cpu_s']}")
         for f in os.listdir(extract_dir):
             os.remove(os.path.join(extract_dir, f))
         os.rmdir(extract_dir)
     if cpu:
         log(f"[bench] cpu baseline {cpu['value']:.3f}s ({cpu['sample']})")
 
-    out = {
-        "metric": METRIC,
+    res = {
         "value": t,
-        "unit": "s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": t * 1e3,
-        "higher_is_better": False,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {"workload": args.config, "dim": dim, "depth": depth, "samples": int(pos.shape[0]),
-                   "aRadius": args.aRadius, "grid_nodes": int(N), "active_nodes": int(updated), "build_s": float(np.mean(builds)), "solve_s": float(np.mean(solves)),
-                   "final_energy": float(E.sum()), "energy_rel_err_vs_reference": rel},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "config": {"workload": name, "dim": dim, "depth": depth, "samples": int(pos.shape[0]),
+                   "aRadius": args.aRadius, "grid_nodes": int(N), "active_nodes": int(updated),
+                   "set_samples_s": float(np.mean(sets)), "build_s": float(np.mean(builds)),
+                   "solve_s": float(np.mean(solves)), "final_energy": float(E.sum()),
+                   "energy_rel_err_vs_reference": rel,
+                   "reference_energy_source": REFERENCE_SOURCE.get(name) if Eref is not None else None},
+        # the sweep runs on the FP64 vector ALU (v_fma_f64, no matrix
+        # instructions); MI355X's FP64 vector peak equals its FP64 matrix peak
+        "roofline": {"bound": "mfma", "peak_kind": "fp64 vector ALU (78.6 TFLOP/s, = the FP64 matrix peak)",
+                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
                      "kernel": f"{'KGsPhase' if args.aRadius < 0 else 'KGsPhaseList'}<{dim}> (finest level, one colour)",
                      "hbm_gbs_algorithmic": bytes_per_launch / per_launch_s / 1e9,
@@ -356,8 +351,70 @@ def main():
                      "traffic_gbs": traffic / per_launch_s / 1e9 if traffic else None},
         "cpu_baseline": cpu,
     }
-    if extract:
-        out["extract"] = extract
+    if levels:
+        res["levels"] = levels
+    if ext:
+        res["extract"] = ext
+    return res, rec
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="borromean3d_d7", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--no-4d", action="store_true",
+                    help="default workload: skip the nested 4D depth-6 (clifford4d_d6) measurement")
+    ap.add_argument("--steps-4d", type=int, default=5, help="timed steps of the nested 4D measurement")
+    ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
+    ap.add_argument("--aRadius", type=int, default=-1, help="-1: dense grid; 0..5: adapted hierarchy of that radius")
+    ap.add_argument("--serial", action="store_true",
+                    help="batch workload: one build thread and the depth groups one after another (for profilers)")
+    ap.add_argument("--no-unbatched", action="store_true", help="batch workload: skip the one-at-a-time comparison")
+    ap.add_argument("--extract", action="store_true",
+                    help="also time the Extract pipeline on the .tree the last step writes (an 'extract' object in the JSON line)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (--gpus {args.gpus} requested); "
+              "the solve path runs on a single device", file=sys.stderr)
+        sys.exit(2)
+
+    wl = WORKLOADS[args.config]
+    if "batch" in wl:
+        return batch_main(args, wl)
+    res, rec = measure_workload(args.config, args, args.steps, args.warmup, not args.no_cpu, args.extract)
+    out = {
+        "metric": METRIC,
+        "value": res["value"],
+        "unit": "s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["value"] * 1e3,
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": res["config"],
+        "roofline": res["roofline"],
+        "cpu_baseline": res["cpu_baseline"],
+    }
+    for k in ("levels", "extract"):
+        if k in res:
+            out[k] = res[k]
+    # the metric's second half: the 4D depth-6 reconstruction, in the same run
+    if args.config == "borromean3d_d7" and not args.no_4d:
+        rec.close()
+        del rec
+        r4, rec4 = measure_workload("clifford4d_d6", args, args.steps_4d, 1, not args.no_cpu)
+        rec4.close()
+        out["clifford4d_d6"] = {"ms_per_reconstruction": r4["value"] * 1e3, "steps": args.steps_4d, "warmup": 1,
+                                **r4["config"], "roofline": r4["roofline"], "cpu_baseline": r4["cpu_baseline"]}
+        if "levels" in r4:
+            out["clifford4d_d6"]["levels"] = r4["levels"]
     print(json.dumps(out))
 
 

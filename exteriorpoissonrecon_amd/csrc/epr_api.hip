@@ -30,6 +30,7 @@
 #include "epr_dev.h"
 #include "epr_hat.h"
 #include "epr_kernels.h"
+#include "epr_rand.h"
 
 namespace epr
 {
@@ -44,16 +45,6 @@ struct SampleCells
 	double unitToWorld[Dim+1][Dim+1];
 	std::vector< int > cellIdx , cellStart , cellSamples;
 	int ncell = 0;
-
-	// glibc rand() as Random<double>() draws it (Misha/Geometry.inl:429-438)
-	static double Random( void )
-	{
-		static const unsigned long long ULL_RAND_MAX = (unsigned long long)( RAND_MAX ) + 1;
-		static const double RAND_MAX_SQUARED = double( ULL_RAND_MAX * ULL_RAND_MAX - 1 );
-		unsigned long long r1 = rand() , r2 = rand();
-		long long foo = r1 * ULL_RAND_MAX + r2;
-		return double( foo ) / RAND_MAX_SQUARED;
-	}
 
 	void bin( size_t n , const double *wpos , double scale , int D , double jitter=0 )
 	{
@@ -98,8 +89,8 @@ struct SampleCells
 		{
 			double T[Dim+1][Dim+1] , Ti[Dim+1][Dim+1] , U[Dim+1][Dim+1];
 			for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) T[i][j] = Ti[i][j] = ( i==j ? 1. : 0. );
-			srand( 1 );
-			for( int d=0 ; d<Dim ; d++ ) T[Dim][d] = ( Random() * 2. - 1. ) * jitter , Ti[Dim][d] = -T[Dim][d];
+			GlibcRand g( 1 );
+			for( int d=0 ; d<Dim ; d++ ) T[Dim][d] = ( g.random01() * 2. - 1. ) * jitter , Ti[Dim][d] = -T[Dim][d];
 			for( size_t i=0 ; i<n ; i++ )
 			{
 				double q[Dim] , scl = 1;
@@ -142,6 +133,30 @@ struct SampleCells
 	}
 };
 
+// The kernels launched with more than 64 KB of dynamic LDS get their limit
+// raised once per device, before any context of the process launches them:
+// hipFuncSetAttribute then never runs concurrently with launches of the same
+// kernel from the batch API's build and solve threads.
+template< int Dim > void SetLdsAttributesDim( int bytes )
+{
+	auto set = [&]( const void *f ){ HIPCHECK( hipFuncSetAttribute( f , hipFuncAttributeMaxDynamicSharedMemorySize , bytes ) ); };
+	set( (const void *)KEstSplatLds< Dim , 0 > ) , set( (const void *)KEstSplatLds< Dim , 1 > ) , set( (const void *)KEstSplatLds< Dim , 2 > );
+	set( (const void *)KEstSplatLds< Dim , 3 > ) , set( (const void *)KEstSplatLds< Dim , 4 > );
+	set( (const void *)KGsTiny< Dim , true > ) , set( (const void *)KGsTiny< Dim , false > ) , set( (const void *)KGsSmallLds< Dim > );
+	set( (const void *)KGsTinyBatch< Dim , true > ) , set( (const void *)KGsTinyBatch< Dim , false > ) , set( (const void *)KGsSmallLdsBatch< Dim > );
+}
+inline void SetLdsAttributes( void )
+{
+	constexpr int MaxDevices = 64;
+	static std::once_flag once[MaxDevices];
+	static std::exception_ptr failed[MaxDevices];
+	int dev = 0;
+	HIPCHECK( hipGetDevice( &dev ) );
+	if( dev<0 || dev>=MaxDevices ) throw std::string( "device index out of range" );
+	std::call_once( once[dev] , [&]{ try { SetLdsAttributesDim< 3 >( 160*1024 ) ; SetLdsAttributesDim< 4 >( 160*1024 ); } catch( ... ){ failed[dev] = std::current_exception(); } } );
+	if( failed[dev] ) std::rethrow_exception( failed[dev] );
+}
+
 struct CtxBase
 {
 	virtual ~CtxBase(){}
@@ -175,7 +190,6 @@ struct Ctx : public CtxBase
 	// for 125 nodes (3D level 2), colour launches for 729 (3D) and 625 (4D).
 	long long SmallLevel = getenv( "EPR_SMALL_LEVEL" ) ? atoll( getenv( "EPR_SMALL_LEVEL" ) ) : ( Dim==3 ? 200 : 100 );
 	static constexpr size_t LdsLimit = 160*1024;   // gfx950 LDS per workgroup
-	bool ldsAttrSet = false , tinyAttrSet = false;
 	// levels with at most this many elements use KGsTiny (EPR_TINY_ELEMENTS overrides, for tuning)
 	long long tinyElements = getenv( "EPR_TINY_ELEMENTS" ) ? atoll( getenv( "EPR_TINY_ELEMENTS" ) ) : ( Dim==3 ? 8 : 16 );
 
@@ -276,6 +290,7 @@ struct Ctx : public CtxBase
 		if( prm.tRadius<0 || prm.tRadius>4 ) throw std::string( "Kernel radius must be in range [0,4]: " + std::to_string( prm.tRadius ) );
 		if( prm.dRadius<0 || prm.dRadius>4 ) throw std::string( "Only radii [0,4] supported: " + std::to_string( prm.dRadius ) );
 		if( prm.aRadius!=-1 && ( prm.aRadius<0 || prm.aRadius>5 ) ) throw std::string( "Only kernel radius in the range [0,5] supported: " + std::to_string( prm.aRadius ) );
+		SetLdsAttributes();
 		HIPCHECK( hipStreamCreate( &stream ) );
 		levels.resize( D+1 );
 		for( int l=0 ; l<=D ; l++ )
@@ -355,7 +370,6 @@ struct Ctx : public CtxBase
 			long long corners = 1 ; for( int d=0 ; d<Dim ; d++ ) corners *= ( 1LL<<(t-1) ) + 1;
 			const size_t lds = (size_t)corners*NV*sizeof(double);
 			if( lds>LdsLimit ) break;
-			if( lds>64*1024 ) HIPCHECK( hipFuncSetAttribute( (const void*)KEstSplatLds< Dim , KR > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) );
 			const long long items = (long long)ncell*taps;
 			const unsigned blocks = (unsigned)std::max< long long >( 1 , std::min< long long >( 256 , items/4096 ) );
 			KEstSplatLds< Dim , KR ><<< blocks , 256 , lds , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p , t );
@@ -601,18 +615,11 @@ struct Ctx : public CtxBase
 		const GsPlan pl = gsPlan( l );
 		if( pl.kind==GS_TINY )
 		{
-			if( !tinyAttrSet )
-			{
-				HIPCHECK( hipFuncSetAttribute( (const void*)KGsTiny< Dim , true > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) );
-				HIPCHECK( hipFuncSetAttribute( (const void*)KGsTiny< Dim , false > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) );
-				tinyAttrSet = true;
-			}
 			if( pl.rb ) KGsTiny< Dim , true  ><<< 1 , pl.threads , pl.lds , stream >>>( v , iters );
 			else        KGsTiny< Dim , false ><<< 1 , pl.threads , pl.lds , stream >>>( v , iters );
 		}
 		else if( pl.kind==GS_SMALL_LDS )
 		{
-			if( !ldsAttrSet ){ HIPCHECK( hipFuncSetAttribute( (const void*)KGsSmallLds< Dim > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)LdsLimit ) ) ; ldsAttrSet = true; }
 			KGsSmallLds< Dim ><<< 1 , pl.threads , pl.lds , stream >>>( v , iters , pl.rb );
 		}
 		else if( pl.kind==GS_SMALL ) KGsSmall< Dim ><<< 1 , 512 , 0 , stream >>>( v , iters );
@@ -712,15 +719,6 @@ struct Ctx : public CtxBase
 		if( prm.dWeight ) out[1] += t[3]*prm.dWeight;
 	}
 
-	static double Random( void )   // Misha/Geometry.inl:429-438
-	{
-		static const unsigned long long ULL_RAND_MAX = (unsigned long long)( RAND_MAX ) + 1;
-		static const double RAND_MAX_SQUARED = double( ULL_RAND_MAX * ULL_RAND_MAX - 1 );
-		unsigned long long r1 = rand() , r2 = rand();
-		long long foo = r1 * ULL_RAND_MAX + r2;
-		return double( foo ) / RAND_MAX_SQUARED;
-	}
-
 	void init( int l , const double *x0 , const double *y0 )
 	{
 		needBuilt() ; checkLevel( l );
@@ -728,9 +726,9 @@ struct Ctx : public CtxBase
 		if( x0 && y0 ){ L.x.upload( x0 , L.N , stream ) ; L.y.upload( y0 , L.N , stream ) ; HIPCHECK( hipStreamSynchronize( stream ) ) ; return; }
 		// ExteriorPoisson.h:333-341
 		std::vector< double > hx( L.N , 0. ) , hy( L.N , 0. );
-		srand( prm.seed );
-		if( adapted() ){ for( unsigned int f : download( L.forder , L.nF ) ) hx[f] = Random() , hy[f] = Random(); }   // the hierarchy's numbering
-		else for( long long i=0 ; i<L.N ; i++ ) hx[i] = Random() , hy[i] = Random();
+		GlibcRand g( (unsigned int)prm.seed );
+		if( adapted() ){ for( unsigned int f : download( L.forder , L.nF ) ) hx[f] = g.random01() , hy[f] = g.random01(); }   // the hierarchy's numbering
+		else for( long long i=0 ; i<L.N ; i++ ) hx[i] = g.random01() , hy[i] = g.random01();
 		L.x.upload( hx.data() , L.N , stream ) , L.y.upload( hy.data() , L.N , stream );
 		double t[4];
 		energyTerms( l , t );
@@ -1123,7 +1121,7 @@ inline void BuildBatch( const std::vector< CtxBase * > &cs , int threads )
 // Ctx::solve, in the same order, so its coefficients and energies are the
 // same bit for bit.  The --verbose per-level energies are not printed.
 template< int Dim >
-void SolveBatchGroup( const std::vector< Ctx< Dim > * > &cs , double *energies , std::mutex &randLock )
+void SolveBatchGroup( const std::vector< Ctx< Dim > * > &cs , double *energies )
 {
 	using Cx = Ctx< Dim >;
 	using Lv = typename Cx::Level;
@@ -1191,19 +1189,17 @@ void SolveBatchGroup( const std::vector< Ctx< Dim > * > &cs , double *energies ,
 	// the random start of Ctx::init (ExteriorPoisson.h:333-341), drawn in context order
 	std::vector< double > start;
 	std::vector< size_t > startAt( n );
-	std::unique_lock< std::mutex > randHeld( randLock );   // glibc's rand() state is shared by every thread
 	for( size_t r=0 ; r<n ; r++ )
 	{
 		Cx &c = *cs[r];
 		Lv &L = Lev( r , lo[r] );
 		std::vector< double > hx( L.N , 0. ) , hy( L.N , 0. );
-		srand( c.prm.seed );
-		if( c.adapted() ){ for( unsigned int f : c.download( L.forder , L.nF ) ) hx[f] = Cx::Random() , hy[f] = Cx::Random(); }
-		else for( long long i=0 ; i<L.N ; i++ ) hx[i] = Cx::Random() , hy[i] = Cx::Random();
+		GlibcRand g( (unsigned int)c.prm.seed );
+		if( c.adapted() ){ for( unsigned int f : c.download( L.forder , L.nF ) ) hx[f] = g.random01() , hy[f] = g.random01(); }
+		else for( long long i=0 ; i<L.N ; i++ ) hx[i] = g.random01() , hy[i] = g.random01();
 		startAt[r] = start.size();
 		start.insert( start.end() , hx.begin() , hx.end() ) , start.insert( start.end() , hy.begin() , hy.end() );
 	}
-	randHeld.unlock();
 	DevBuf< double > dStart , dScale , eT;
 	dStart.upload( start.data() , start.size() , st );
 	dScale.alloc( n ) , eT.alloc( 4*n );
@@ -1335,12 +1331,11 @@ void SolveBatchGroup( const std::vector< Ctx< Dim > * > &cs , double *energies ,
 		{
 			if( S.plan.kind==Cx::GS_TINY )
 			{
-				if( S.plan.rb ){ HIPCHECK( hipFuncSetAttribute( (const void*)KGsTinyBatch< Dim , true > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)Cx::LdsLimit ) ) ; KGsTinyBatch< Dim , true ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) ); }
-				else { HIPCHECK( hipFuncSetAttribute( (const void*)KGsTinyBatch< Dim , false > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)Cx::LdsLimit ) ) ; KGsTinyBatch< Dim , false ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) ); }
+				if( S.plan.rb ) KGsTinyBatch< Dim , true ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) );
+				else KGsTinyBatch< Dim , false ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) );
 			}
 			else
 			{
-				HIPCHECK( hipFuncSetAttribute( (const void*)KGsSmallLdsBatch< Dim > , hipFuncAttributeMaxDynamicSharedMemorySize , (int)Cx::LdsLimit ) );
 				KGsSmallLdsBatch< Dim ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) , S.plan.rb );
 			}
 		}
@@ -1425,7 +1420,7 @@ void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
 		gc.emplace_back() , ge.emplace_back( 2*g.size() ) , gi.push_back( &g );
 		for( size_t r : g ) gc.back().push_back( cs[r] );
 	}
-	std::mutex randLock , m;
+	std::mutex m;
 	std::exception_ptr first;
 	int dev = 0;
 	HIPCHECK( hipGetDevice( &dev ) );
@@ -1434,7 +1429,7 @@ void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
 		try
 		{
 			HIPCHECK( hipSetDevice( dev ) );
-			SolveBatchGroup< Dim >( gc[k] , ge[k].data() , randLock );
+			SolveBatchGroup< Dim >( gc[k] , ge[k].data() );
 		}
 		catch( ... ){ std::lock_guard< std::mutex > lock( m ) ; if( !first ) first = std::current_exception(); }
 	};

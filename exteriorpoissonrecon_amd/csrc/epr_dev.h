@@ -32,17 +32,23 @@ struct BlockCache
 		if( bytes<MinBytes ) return nullptr;
 		int dev = 0;
 		if( hipGetDevice( &dev )!=hipSuccess ) return nullptr;
-		std::lock_guard< std::mutex > lock( m );
-		size_t best = blocks.size();
-		for( size_t i=0 ; i<blocks.size() ; i++ )
-			if( blocks[i].dev==dev && blocks[i].bytes>=bytes && blocks[i].bytes<=bytes+bytes/4 && ( best==blocks.size() || blocks[i].bytes<blocks[best].bytes ) ) best = i;
-		if( best==blocks.size() ) return nullptr;
-		if( hipDeviceSynchronize()!=hipSuccess ) return nullptr;
-		void *p = blocks[best].p;
-		got = blocks[best].bytes;
-		held -= blocks[best].bytes;
-		blocks[best] = blocks.back() , blocks.pop_back();
-		return p;
+		Block b;
+		{
+			std::lock_guard< std::mutex > lock( m );
+			size_t best = blocks.size();
+			for( size_t i=0 ; i<blocks.size() ; i++ )
+				if( blocks[i].dev==dev && blocks[i].bytes>=bytes && blocks[i].bytes<=bytes+bytes/4 && ( best==blocks.size() || blocks[i].bytes<blocks[best].bytes ) ) best = i;
+			if( best==blocks.size() ) return nullptr;
+			b = blocks[best];
+			held -= b.bytes;
+			blocks[best] = blocks.back() , blocks.pop_back();
+		}
+		// the block is this caller's now; the wait for the work that may still
+		// use it runs outside the lock, so other threads' takes and gives do not
+		// queue behind a device synchronisation
+		if( hipDeviceSynchronize()!=hipSuccess ){ (void)hipGetLastError() ; (void)hipFree( b.p ) ; return nullptr; }
+		got = b.bytes;
+		return b.p;
 	}
 	// keeps a block (true) or leaves it to the caller to free
 	bool give( void *p , size_t bytes )

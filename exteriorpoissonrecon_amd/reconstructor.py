@@ -20,22 +20,32 @@ class Reconstructor:
     maxSolveDepth, singleLevel, verbose)."""
 
     def __init__(self, positions, skews, **params):
-        pos = np.ascontiguousarray(positions, dtype=np.float64)
-        sk = np.ascontiguousarray(skews, dtype=np.float64)
+        pos = np.asarray(positions)
         if pos.ndim != 2 or pos.shape[1] not in (3, 4):
             raise ValueError("positions must be n x 3 or n x 4")
-        n, dim = pos.shape
-        if sk.shape != (n, dim, dim):
-            raise ValueError("skews must be n x dim x dim")
-        self.dim = dim
+        self.dim = pos.shape[1]
         self.params = default_params(**params)
         self.depth = self.params.depth
         L = load()
         h = ctypes.c_void_p()
-        rc = L.epr_create(dim, ctypes.byref(self.params), ctypes.byref(h))
+        rc = L.epr_create(self.dim, ctypes.byref(self.params), ctypes.byref(h))
         self._h = h
         self._check(rc)
-        self._check(L.epr_set_samples(self._h, n, _p(pos), _p(sk)))
+        self.set_samples(positions, skews)
+
+    def set_samples(self, positions, skews):
+        """Hands over (new) samples: the unit-cube map and the ordered sampler's
+        cells (ExteriorPoisson.h:100-123, OrderedSamples.inl:32-85); build()
+        must run again afterwards."""
+        pos = np.ascontiguousarray(positions, dtype=np.float64)
+        sk = np.ascontiguousarray(skews, dtype=np.float64)
+        if pos.ndim != 2 or pos.shape[1] != self.dim:
+            raise ValueError(f"positions must be n x {self.dim}")
+        n = pos.shape[0]
+        if sk.shape != (n, self.dim, self.dim):
+            raise ValueError("skews must be n x dim x dim")
+        self._check(load().epr_set_samples(self._h, n, _p(pos), _p(sk)))
+        return self
 
     def _check(self, rc):
         if rc != 0:
