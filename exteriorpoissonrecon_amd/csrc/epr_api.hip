@@ -164,6 +164,7 @@ struct CtxBase
 	virtual void build( void ) = 0;
 	virtual void init( int level , const double *x0 , const double *y0 ) = 0;
 	virtual void gs( int level , int iters ) = 0;
+	virtual void gsColour( int level , int colour ) = 0;
 	virtual void timeGs( int level , int iters , double *ms , int *launches ) = 0;
 	virtual void prolongAdd( int level ) = 0;
 	virtual void energies( int level , double out[2] ) = 0;
@@ -657,6 +658,30 @@ struct Ctx : public CtxBase
 			HIPCHECK( hipGraphLaunch( L.graph , stream ) );
 		}
 		HIPCHECK( hipGetLastError() );
+	}
+
+	// one colour of one Gauss-Seidel iteration (the body of the colour loop of
+	// PairedGaussSeidelSolve, ExteriorPoissonSolvers.h:163-185), on levels
+	// swept by colour launches
+	void gsColour( int l , int c )
+	{
+		needBuilt() ; checkLevel( l );
+		if( c<0 || c>=C ) throw std::string( "colour out of range" );
+		if( gsPlan( l ).kind!=GS_PHASES ) throw std::string( "single colour phases run on the levels swept by colour launches only" );
+		LevelView< Dim > v = view( l );
+		Level &L = levels[l];
+		if( L.fm.p )
+		{
+			const long long cnt = L.actStart[c+1] - L.actStart[c];
+			if( cnt ) KGsPhaseList< Dim ><<< Blocks( cnt*C , 256 ) , 256 , 0 , stream >>>( v , L.act.p + L.actStart[c] , cnt );
+		}
+		else
+		{
+			ColourSet< Dim > cs ; cs.set( L.res , c );
+			KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , v.lay.cstart[c] );
+		}
+		HIPCHECK( hipGetLastError() );
+		HIPCHECK( hipStreamSynchronize( stream ) );
 	}
 
 	void timeGs( int l , int iters , double *ms , int *launches )
@@ -1526,6 +1551,7 @@ int epr_set_samples( epr_ctx *ctx , size_t n , const double *pos , const double 
 int epr_build( epr_ctx *ctx ){ IMPL( ctx->impl->build() ); }
 int epr_init( epr_ctx *ctx , int level , const double *x0 , const double *y0 ){ IMPL( ctx->impl->init( level , x0 , y0 ) ); }
 int epr_gs_sweeps( epr_ctx *ctx , int level , int iters ){ IMPL( ctx->impl->gs( level , iters ) ; HIPCHECK( hipDeviceSynchronize() ) ); }
+int epr_gs_colour( epr_ctx *ctx , int level , int colour ){ IMPL( ctx->impl->gsColour( level , colour ) ); }
 int epr_time_gs_sweeps( epr_ctx *ctx , int level , int iters , double *ms , int *launches ){ IMPL( ctx->impl->timeGs( level , iters , ms , launches ) ); }
 int epr_prolong_add( epr_ctx *ctx , int level ){ IMPL( ctx->impl->prolongAdd( level ) ); }
 int epr_energies( epr_ctx *ctx , int level , double out[2] ){ IMPL( ctx->impl->energies( level , out ) ); }

@@ -80,6 +80,10 @@ class Reconstructor:
     def gauss_seidel(self, level, iters):
         self._check(load().epr_gs_sweeps(self._h, level, iters))
 
+    def gauss_seidel_colour(self, level, colour):
+        """One colour phase of a Gauss-Seidel iteration (levels swept by colour launches)."""
+        self._check(load().epr_gs_colour(self._h, level, colour))
+
     def time_gauss_seidel(self, level, iters):
         """(device milliseconds, kernel launches) of iters iterations at a level (HIP events)."""
         ms = ctypes.c_double(); n = ctypes.c_int()

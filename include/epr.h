@@ -85,13 +85,21 @@ int  epr_set_samples( epr_ctx *ctx , size_t n , const double *pos , const double
 int  epr_build( epr_ctx *ctx );
 
 /* Sets the coefficients of one level; x0/y0 NULL selects the reference's
- * random initial guess at minSolveDepth (ExteriorPoisson.h:333-341, glibc
- * srand/rand with the seed). */
+ * random initial guess at minSolveDepth (ExteriorPoisson.h:333-341: glibc's
+ * rand() sequence after srand(seed), drawn from a private copy of that
+ * generator, so the process-wide rand() state is neither read nor changed). */
 int  epr_init( epr_ctx *ctx , int level , const double *x0 , const double *y0 );
 
 /* iters paired multicolour Gauss-Seidel iterations at one level
  * (ExteriorPoisson/ExteriorPoissonSolvers.h:156-196). */
 int  epr_gs_sweeps( epr_ctx *ctx , int level , int iters );
+
+/* One colour of one of those iterations: every active node of colour
+ * `colour` (0 .. 2^dim-1, the parity colour of ExteriorPoisson.h:161-169)
+ * takes its step -- the body of the colour loop of PairedGaussSeidelSolve
+ * (ExteriorPoissonSolvers.h:163-185).  Levels swept by one-workgroup kernels
+ * (the few coarsest) are an error. */
+int  epr_gs_colour( epr_ctx *ctx , int level , int colour );
 
 /* x[level+1] += P x[level], y likewise (ExteriorPoisson.h:386). */
 int  epr_prolong_add( epr_ctx *ctx , int level );

@@ -241,3 +241,34 @@ def test_parameter_range_errors(kw, msg):
     pos, skew = samples.torus_knot(256, seed=1)
     with pytest.raises(EprError, match=msg):
         Reconstructor(pos, skew, **kw).build()
+
+
+def test_private_rand_matches_glibc(tmp_path):
+    """The library's private copy of glibc's rand() (csrc/epr_rand.h) gives
+    glibc's own sequence for every seed, including 0 (glibc treats it as 1)
+    and seeds above INT_MAX, without touching the process-wide state."""
+    import subprocess
+    src = tmp_path / "t.cpp"
+    src.write_text(r'''
+#include "epr_rand.h"
+#include <cstdio>
+#include <cstdlib>
+int main(){
+  unsigned seeds[] = { 0u , 1u , 2u , 7u , 12345u , 2147483647u , 2147483648u , 4294967295u };
+  for( unsigned s : seeds ){
+    srand( s ); epr::GlibcRand g( s );
+    for( int i=0 ; i<20000 ; i++ ){ int a = rand() , b = g.rand(); if( a!=b ){ printf( "seed %u draw %d: %d %d\n" , s , i , a , b ); return 1; } }
+  }
+  srand( 5 ); int a = rand(); epr::GlibcRand g( 9 ); (void)g.random01(); srand( 5 ); if( rand()!=a ) return 2;
+  epr::GlibcRand h( 3 ); srand( 3 );
+  for( int i=0 ; i<1000 ; i++ ){
+    unsigned long long r1 = rand() , r2 = rand();
+    double ref = double( (long long)( r1*( (unsigned long long)RAND_MAX+1 ) + r2 ) ) / double( ( (unsigned long long)RAND_MAX+1 )*( (unsigned long long)RAND_MAX+1 ) - 1 );
+    if( h.random01()!=ref ) return 3;
+  }
+  puts( "ok" ); return 0;
+}''')
+    exe = tmp_path / "t"
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "exteriorpoissonrecon_amd", "csrc"),
+                           str(src), "-o", str(exe)])
+    assert subprocess.run([str(exe)], capture_output=True, text=True).stdout.strip() == "ok"
