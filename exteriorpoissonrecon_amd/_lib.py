@@ -22,7 +22,7 @@ EXPORTS = [
     "epr_iso_open", "epr_iso_last_error", "epr_iso_info", "epr_iso_values", "epr_iso_level_set_nodes",
     "epr_iso_extract", "epr_iso_level_set", "epr_iso_write_grid", "epr_iso_seconds", "epr_iso_destroy",
     "epr_indexer_hash", "epr_orient", "epr_write_density", "epr_iso_read_density", "epr_iso_extract_trimmed", "epr_iso_component_densities",
-    "epr_build_batch", "epr_solve_batch", "epr_density_grid", "epr_gs_colour",
+    "epr_build_batch", "epr_solve_batch", "epr_density_grid", "epr_gs_colour", "epr_energies_noalloc",
 ]
 
 
@@ -64,6 +64,7 @@ def load():
     L.epr_init.argtypes = [vp, ci, dp, dp]
     L.epr_gs_sweeps.argtypes = [vp, ci, ci]
     L.epr_gs_colour.argtypes = [vp, ci, ci]
+    L.epr_energies_noalloc.argtypes = [vp, ci, dp]
     L.epr_prolong_add.argtypes = [vp, ci]
     L.epr_energies.argtypes = [vp, ci, dp]
     L.epr_solve.argtypes = [vp, dp]
@@ -133,7 +134,7 @@ def load():
     for name in ["epr_create", "epr_set_samples", "epr_build", "epr_init", "epr_gs_sweeps",
                  "epr_prolong_add", "epr_energies", "epr_solve", "epr_get_coeffs", "epr_set_coeffs",
                  "epr_quadratic_fits", "epr_apply", "epr_get_array", "epr_write_tree", "epr_write_grid",
-                 "epr_gs_colour"]:
+                 "epr_gs_colour", "epr_energies_noalloc"]:
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

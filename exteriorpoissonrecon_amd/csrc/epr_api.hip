@@ -133,6 +133,34 @@ struct SampleCells
 	}
 };
 
+// Device scratch of the allocation-free energies (epr_energies_noalloc): the
+// per-workgroup partial sums of KEnergy and the four summed terms.  One copy
+// per device, held until the process ends; every call on a device shares it,
+// so calls that overlap on one device race.  Host code never passes the
+// array's name (a host-side stand-in) to a kernel or a copy: it uses the
+// device address that hipGetSymbolAddress returns, looked up once per device.
+constexpr size_t NoAllocScratchDoubles = size_t( 1 ) << 19;   // 4 MiB
+constexpr long long NoAllocMaxBlocks = (long long)( NoAllocScratchDoubles - 4 ) / 4;
+static __device__ __align__( 256 ) double scratch[ NoAllocScratchDoubles ];
+// A device-side reference to the array.  Without one the device compiler
+// drops the (internal-linkage, otherwise device-unused) array from the code
+// object, and the symbol the host side registers for hipGetSymbolAddress does
+// not exist on the device.  Never launched.
+__global__ void KKeepNoAllocScratch( void ){ scratch[0] = 0.; }
+inline double *NoAllocScratch( void )
+{
+	constexpr int MaxDevices = 64;
+	static std::once_flag once[MaxDevices];
+	static double *ptr[MaxDevices] = {};
+	static hipError_t status[MaxDevices] = {};
+	int dev = 0;
+	HIPCHECK( hipGetDevice( &dev ) );
+	if( dev<0 || dev>=MaxDevices ) throw std::string( "device index out of range" );
+	std::call_once( once[dev] , [&]{ status[dev] = hipGetSymbolAddress( (void **)&ptr[dev] , HIP_SYMBOL( scratch ) ); } );
+	if( status[dev]!=hipSuccess || !ptr[dev] ) throw HipError{ std::string( "hipGetSymbolAddress( scratch ): " ) + hipGetErrorString( status[dev] ) };
+	return ptr[dev];
+}
+
 // The kernels launched with more than 64 KB of dynamic LDS get their limit
 // raised once per device, before any context of the process launches them:
 // hipFuncSetAttribute then never runs concurrently with launches of the same
@@ -168,6 +196,7 @@ struct CtxBase
 	virtual void timeGs( int level , int iters , double *ms , int *launches ) = 0;
 	virtual void prolongAdd( int level ) = 0;
 	virtual void energies( int level , double out[2] ) = 0;
+	virtual void energiesNoAlloc( int level , double out[2] ) = 0;
 	virtual void solve( double out[2] ) = 0;
 	virtual size_t numFunctions( int level ) const = 0;
 	virtual void getCoeffs( int level , double *x , double *y ) = 0;
@@ -739,6 +768,29 @@ struct Ctx : public CtxBase
 		needBuilt() ; checkLevel( l );
 		double t[4];
 		energyTerms( l , t );
+		out[0] = t[0] - 2.*t[1] + c_;
+		out[1] = t[2];
+		if( prm.dWeight ) out[1] += t[3]*prm.dWeight;
+	}
+
+	// energies() with the partial sums in the device's static scratch instead
+	// of the context's buffers: no allocation, the same kernels and summation
+	// order (so the same bits)
+	void energiesNoAlloc( int l , double out[2] )
+	{
+		needBuilt() ; checkLevel( l );
+		LevelView< Dim > v = view( l );
+		const long long threads = levels[l].N*C;
+		const long long nb = ( threads + 255 )/256;
+		if( nb>NoAllocMaxBlocks )
+			throw std::string( "level too large for epr_energies_noalloc: " + std::to_string( nb ) + " partial sums, the scratch holds " + std::to_string( NoAllocMaxBlocks ) + " (at most " + std::to_string( NoAllocMaxBlocks*256/C ) + " functions)" );
+		double *sp = NoAllocScratch();
+		KEnergy< Dim ><<< (unsigned)nb , 256 , 0 , stream >>>( v , sp + 4 );
+		KSumRows<4><<< 1 , 256 , 0 , stream >>>( sp + 4 , nb , sp );
+		HIPCHECK( hipGetLastError() );
+		double t[4];
+		HIPCHECK( hipMemcpyAsync( t , sp , 4*sizeof(double) , hipMemcpyDeviceToHost , stream ) );
+		HIPCHECK( hipStreamSynchronize( stream ) );
 		out[0] = t[0] - 2.*t[1] + c_;
 		out[1] = t[2];
 		if( prm.dWeight ) out[1] += t[3]*prm.dWeight;
@@ -1555,6 +1607,7 @@ int epr_gs_colour( epr_ctx *ctx , int level , int colour ){ IMPL( ctx->impl->gsC
 int epr_time_gs_sweeps( epr_ctx *ctx , int level , int iters , double *ms , int *launches ){ IMPL( ctx->impl->timeGs( level , iters , ms , launches ) ); }
 int epr_prolong_add( epr_ctx *ctx , int level ){ IMPL( ctx->impl->prolongAdd( level ) ); }
 int epr_energies( epr_ctx *ctx , int level , double out[2] ){ IMPL( ctx->impl->energies( level , out ) ); }
+int epr_energies_noalloc( epr_ctx *ctx , int level , double out[2] ){ if( !out ) return EPR_ERR_ARG ; IMPL( ctx->impl->energiesNoAlloc( level , out ) ); }
 int epr_solve( epr_ctx *ctx , double out_E[2] ){ IMPL( ctx->impl->solve( out_E ) ); }
 size_t epr_num_functions( const epr_ctx *ctx , int level ){ return ( ctx && ctx->impl && level>=0 && level<=ctx->prm.depth ) ? ctx->impl->numFunctions( level ) : 0; }
 int epr_get_coeffs( epr_ctx *ctx , int level , double *x , double *y ){ IMPL( ctx->impl->getCoeffs( level , x , y ) ); }

@@ -108,6 +108,18 @@ int  epr_prolong_add( epr_ctx *ctx , int level );
  * (Include/SystemEnergy.h:486-497). */
 int  epr_energies( epr_ctx *ctx , int level , double out[2] );
 
+/* epr_energies without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * partial sums live in a static 4 MiB device array, one per device, shared by
+ * every call on that device and kept until the process ends: calls that
+ * overlap on one device race.  Same kernels and summation order as
+ * epr_energies (same result bits).  Levels whose partial sums do not fit --
+ * more than 131071 * 256 / 2^dim functions (3D up to depth 7, 4D up to
+ * depth 5 on the dense grid) -- are an error (EPR_ERR_STATE, with the limit
+ * in epr_last_error). */
+int  epr_energies_noalloc( epr_ctx *ctx , int level , double out[2] );
+
 /* The whole cascadic schedule with the random initial guess
  * (ExteriorPoisson.h:316-409); afterwards the coefficients are hierarchical
  * as in the reference's .tree output. out_E (may be NULL) receives the final
