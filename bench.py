@@ -358,6 +358,55 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False):
     return res, rec
 
 
+def measure_learned(args):
+    """The learned initializer (4D Hopf torus, depth 6): time-to-energy of the
+    plain solver and of the network's start, in the same harness
+    (learned.time_to_energy), best of three each; and the network's forward
+    pass on the device (MFMA convolutions) as algorithmic BF16 TFLOP/s."""
+    from exteriorpoissonrecon_amd import Reconstructor, learned, samples
+    if not os.path.exists(learned.WEIGHTS_4D):
+        return None
+    net = learned.UNet.load(learned.WEIGHTS_4D)
+    pos, skew = samples.hopf_torus(256, lobes=3, amp=0.3, latitude="sin")
+    log(f"[bench] hopf4d_d6_learned: {len(pos)} samples, dim 4, depth 6")
+    rec = Reconstructor(pos, skew, depth=6, aRadius=-1).build()
+    E_ref = float(rec.solve().sum())
+    solves = []
+    for _ in range(3):
+        rec.solve()
+        solves.append(rec.seconds("solve"))
+    solve_s = min(solves)
+    target = E_ref * (1 + 1e-6)
+    best = lambda runs: min(runs, key=lambda r: r[0] if r[0] is not None else 1e9)
+    plain = best([learned.time_to_energy(rec, target, None) for _ in range(3)])
+    lrn = best([learned.time_to_energy(rec, target, net, max_finest=40) for _ in range(3)])
+    X = learned.features(rec, 4)
+    net.forward(X)
+    t0 = time.perf_counter()
+    for _ in range(5):
+        net.forward(X)
+    fwd_s = (time.perf_counter() - t0) / 5
+    t0 = time.perf_counter()
+    for _ in range(5):
+        learned.init_learned(rec, net, 4)
+    init_s = (time.perf_counter() - t0) / 5
+    flops = learned.forward_flops(4, 4, 6)
+    E_l = float(learned.solve_learned(rec, net).sum())
+    log(f"[bench] learned: plain {plain[0]:.3f} s ({plain[1]} finest iterations), learned {lrn[0]} s "
+        f"({lrn[1]}), E_ref {E_ref:.12g}, forward {fwd_s * 1e3:.2f} ms ({flops / fwd_s / 1e12:.2f} TFLOP/s bf16), "
+        f"learned start {init_s * 1e3:.2f} ms")
+    rec.close(); net.close()
+    return {"workload": "hopf4d_d6 (Hopf torus of theta = pi/2 + 0.3 sin 3s, 65536 samples, depth 6, dense)",
+            "reference_energy": E_ref, "target": target, "plain_solve_s": solve_s,
+            "plain_time_to_energy_s": plain[0], "plain_finest_iterations": plain[1],
+            "learned_time_to_energy_s": lrn[0], "learned_finest_iterations": lrn[1],
+            "learned_final_energy": lrn[2], "speedup": (plain[0] / lrn[0]) if lrn[0] else None,
+            "solve_learned_energy": E_l, "learned_start_ms": init_s * 1e3,
+            "unet_forward_ms_host_io": fwd_s * 1e3, "unet_forward_flops": flops,
+            "unet_forward_tflops_bf16": flops / fwd_s / 1e12, "bf16_dense_peak_tflops": 2500.0,
+            "harness": "reference schedule below the finest level, then the finest iterations; the count that reaches the target is found with an energy check after each iteration, then that schedule alone is timed"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -368,6 +417,8 @@ def main():
     ap.add_argument("--no-4d", action="store_true",
                     help="default workload: skip the nested 4D depth-6 (clifford4d_d6) measurement")
     ap.add_argument("--steps-4d", type=int, default=5, help="timed steps of the nested 4D measurement")
+    ap.add_argument("--no-learned", action="store_true",
+                    help="default workload: skip the nested learned-initializer measurement (4D Hopf torus)")
     ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
     ap.add_argument("--aRadius", type=int, default=-1, help="-1: dense grid; 0..5: adapted hierarchy of that radius")
     ap.add_argument("--serial", action="store_true",
@@ -415,6 +466,8 @@ def main():
                                 **r4["config"], "roofline": r4["roofline"], "cpu_baseline": r4["cpu_baseline"]}
         if "levels" in r4:
             out["clifford4d_d6"]["levels"] = r4["levels"]
+    if args.config == "borromean3d_d7" and not args.no_learned:
+        out["hopf4d_d6_learned"] = measure_learned(args)
     print(json.dumps(out))
 
 

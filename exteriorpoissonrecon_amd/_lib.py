@@ -23,6 +23,9 @@ EXPORTS = [
     "epr_iso_extract", "epr_iso_level_set", "epr_iso_write_grid", "epr_iso_seconds", "epr_iso_destroy",
     "epr_indexer_hash", "epr_orient", "epr_write_density", "epr_iso_read_density", "epr_iso_extract_trimmed", "epr_iso_component_densities",
     "epr_build_batch", "epr_solve_batch", "epr_density_grid", "epr_gs_colour", "epr_energies_noalloc",
+    "epr_unet_create", "epr_unet_destroy", "epr_unet_last_error", "epr_unet_num_params", "epr_unet_num_nodes",
+    "epr_unet_set_params", "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients",
+    "epr_unet_set_dataset", "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned",
 ]
 
 
@@ -66,6 +69,24 @@ def load():
     L.epr_gs_colour.argtypes = [vp, ci, ci]
     L.epr_energies_noalloc.argtypes = [vp, ci, dp]
     L.epr_prolong_add.argtypes = [vp, ci]
+    fp = ctypes.POINTER(ctypes.c_float)
+    L.epr_unet_create.argtypes = [ci, ci, ci, ctypes.POINTER(vp)]
+    L.epr_unet_destroy.argtypes = [vp]
+    L.epr_unet_last_error.argtypes = [vp]
+    L.epr_unet_last_error.restype = ctypes.c_char_p
+    L.epr_unet_num_params.argtypes = [vp]
+    L.epr_unet_num_params.restype = sz
+    L.epr_unet_num_nodes.argtypes = [vp]
+    L.epr_unet_num_nodes.restype = sz
+    L.epr_unet_set_params.argtypes = [vp, fp]
+    L.epr_unet_get_params.argtypes = [vp, fp]
+    L.epr_unet_forward.argtypes = [vp, ci, fp, fp]
+    L.epr_unet_loss_gradients.argtypes = [vp, ci, fp, fp, dp, fp]
+    L.epr_unet_set_dataset.argtypes = [vp, sz, fp, fp]
+    L.epr_unet_train.argtypes = [vp, ci, ci, ctypes.c_double, ctypes.c_uint, dp]
+    L.epr_learned_features.argtypes = [vp, ci, fp]
+    L.epr_init_learned.argtypes = [vp, vp, ci, dp]
+    L.epr_solve_learned.argtypes = [vp, vp, dp]
     L.epr_energies.argtypes = [vp, ci, dp]
     L.epr_solve.argtypes = [vp, dp]
     L.epr_num_functions.argtypes = [vp, ci]
@@ -134,7 +155,9 @@ def load():
     for name in ["epr_create", "epr_set_samples", "epr_build", "epr_init", "epr_gs_sweeps",
                  "epr_prolong_add", "epr_energies", "epr_solve", "epr_get_coeffs", "epr_set_coeffs",
                  "epr_quadratic_fits", "epr_apply", "epr_get_array", "epr_write_tree", "epr_write_grid",
-                 "epr_gs_colour", "epr_energies_noalloc"]:
+                 "epr_gs_colour", "epr_energies_noalloc", "epr_unet_create", "epr_unet_set_params",
+                 "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients", "epr_unet_set_dataset",
+                 "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned"]:
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -31,6 +31,7 @@
 #include "epr_hat.h"
 #include "epr_kernels.h"
 #include "epr_rand.h"
+#include "epr_unet.h"
 
 namespace epr
 {
@@ -133,6 +134,91 @@ struct SampleCells
 	}
 };
 
+// ---- the learned initializer's input: the target field on a coarse level
+// mean of the finest per-element field S over each element of level L
+// (element index with coordinate 0 fastest, as S)
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KCoarseFieldMean( const double *S , int D , int L , double *out )
+{
+	constexpr int CD = DimC< Dim >::CD;
+	const int rl = 1<<L , rf = 1<<D , B = 1<<( D-L );
+	long long ne = 1 , cnt = 1;
+	for( int d=0 ; d<Dim ; d++ ) ne *= rl , cnt *= B;
+	const long long e = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( e>=ne ) return;
+	int E[Dim];
+	{ long long t = e ; for( int d=0 ; d<Dim ; d++ ){ E[d] = (int)( t % rl ) ; t /= rl; } }
+	double s[CD];
+	for( int c=0 ; c<CD ; c++ ) s[c] = 0;
+	for( long long k=0 ; k<cnt ; k++ )
+	{
+		long long u = k , fi = 0 , mul = 1;
+		for( int d=0 ; d<Dim ; d++ ){ const int o = (int)( u % B ) ; u /= B ; fi += ( (long long)E[d]*B + o )*mul ; mul *= rf; }
+		for( int c=0 ; c<CD ; c++ ) s[c] += S[ fi*CD + c ];
+	}
+	for( int c=0 ; c<CD ; c++ ) out[ e*CD + c ] = s[c] / (double)cnt;
+}
+// per node of level L (function index, last coordinate fastest): the mean of
+// its incident level-L elements' means
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KNodeFieldMean( const double *Ec , int L , double *out )
+{
+	constexpr int CD = DimC< Dim >::CD;
+	const int rl = 1<<L , side = rl+1;
+	long long N = 1;
+	for( int d=0 ; d<Dim ; d++ ) N *= side;
+	const long long n = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( n>=N ) return;
+	int F[Dim];
+	{ long long t = n ; for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = (int)( t % side ) ; t /= side; } }
+	double s[CD];
+	for( int c=0 ; c<CD ; c++ ) s[c] = 0;
+	int cnt = 0;
+	for( int m=0 ; m<(1<<Dim) ; m++ )
+	{
+		long long e = 0 , mul = 1;
+		bool ok = true;
+		for( int d=0 ; d<Dim ; d++ ){ const int q = F[d] - 1 + ( ( m>>d ) & 1 ) ; ok = ok && q>=0 && q<rl ; e += (long long)q*mul ; mul *= rl; }
+		if( !ok ) continue;
+		cnt++;
+		for( int c=0 ; c<CD ; c++ ) s[c] += Ec[ e*CD + c ];
+	}
+	for( int c=0 ; c<CD ; c++ ) out[ n*CD + c ] = cnt ? s[c]/cnt : 0.;
+}
+// out[0] = sum of squares of v (one workgroup, fixed order)
+__global__ void __launch_bounds__( 256 ) KSumSquares( const double *v , long long n , double *out )
+{
+	__shared__ double sh[256];
+	double s = 0;
+	for( long long i=threadIdx.x ; i<n ; i+=256 ) s += v[i]*v[i];
+	sh[threadIdx.x] = s;
+	__syncthreads();
+	for( int h=128 ; h>0 ; h>>=1 ){ if( (int)threadIdx.x<h ) sh[threadIdx.x] += sh[threadIdx.x+h] ; __syncthreads(); }
+	if( threadIdx.x==0 ) out[0] = sh[0];
+}
+// the network input: v / rms(v), bf16 rows padded to 16 channels, and as floats
+__global__ void __launch_bounds__( 256 ) KFeatureRows( const double *v , long long N , int CD , const double *sumsq , unet::bf16 *rows , float *flt )
+{
+	const long long k = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( k>=N*unet::CT ) return;
+	const long long n = k / unet::CT;
+	const int c = (int)( k % unet::CT );
+	const double rms = sqrt( sumsq[0] / ( (double)N*CD ) );
+	const double f = c<CD ? ( rms>0 ? v[ n*CD + c ]/rms : 0. ) : 0.;
+	rows[k] = unet::ToBf16( (float)f );
+	if( flt && c<CD ) flt[ n*CD + c ] = (float)f;
+}
+// x, y of a level from the network output (channels 0, 1); functions outside
+// an adapted hierarchy stay zero
+__global__ void __launch_bounds__( 256 ) KCoeffsFromNet( const float *yNet , long long N , const unsigned char *fmask , double *x , double *y )
+{
+	const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( i>=N ) return;
+	const bool on = !fmask || fmask[i]!=0;
+	x[i] = on ? (double)yNet[ i*unet::CT ] : 0.;
+	y[i] = on ? (double)yNet[ i*unet::CT + 1 ] : 0.;
+}
+
 // Device scratch of the allocation-free energies (epr_energies_noalloc): the
 // per-workgroup partial sums of KEnergy and the four summed terms.  One copy
 // per device, held until the process ends; every call on a device shares it,
@@ -198,6 +284,9 @@ struct CtxBase
 	virtual void energies( int level , double out[2] ) = 0;
 	virtual void energiesNoAlloc( int level , double out[2] ) = 0;
 	virtual void solve( double out[2] ) = 0;
+	virtual void learnedFeatures( int level , float *out ) = 0;
+	virtual void learnedInit( int level , unet::Net *net , double info[3] ) = 0;
+	virtual void solveLearned( unet::Net *net , double out[2] ) = 0;
 	virtual size_t numFunctions( int level ) const = 0;
 	virtual void getCoeffs( int level , double *x , double *y ) = 0;
 	virtual void setCoeffs( int level , const double *x , const double *y ) = 0;
@@ -801,7 +890,12 @@ struct Ctx : public CtxBase
 		needBuilt() ; checkLevel( l );
 		Level &L = levels[l];
 		if( x0 && y0 ){ L.x.upload( x0 , L.N , stream ) ; L.y.upload( y0 , L.N , stream ) ; HIPCHECK( hipStreamSynchronize( stream ) ) ; return; }
-		// ExteriorPoisson.h:333-341
+		// ExteriorPoisson.h:333-341; the levels above start at zero (the cascade adds into them)
+		{
+			std::vector< std::pair< void * , size_t > > spans;
+			for( int k=l+1 ; k<=D ; k++ ) spans.push_back( { (void *)levels[k].x.p , (size_t)levels[k].N*sizeof(double) } ) , spans.push_back( { (void *)levels[k].y.p , (size_t)levels[k].N*sizeof(double) } );
+			if( !spans.empty() ) zeroSpans( spans );
+		}
 		std::vector< double > hx( L.N , 0. ) , hy( L.N , 0. );
 		GlibcRand g( (unsigned int)prm.seed );
 		if( adapted() ){ for( unsigned int f : download( L.forder , L.nF ) ) hx[f] = g.random01() , hy[f] = g.random01(); }   // the hierarchy's numbering
@@ -817,17 +911,26 @@ struct Ctx : public CtxBase
 	}
 
 	// the cascadic schedule (ExteriorPoisson.h:316-409)
-	void solve( double out[2] )
+	void solve( double out[2] ){ solveFrom( out , nullptr ); }
+	void solveFrom( double out[2] , unet::Net *net )
 	{
 		needBuilt();
 		auto t0 = std::chrono::steady_clock::now();
-		const int lo = minD() , hi = maxD();
+		int lo = minD();
+		const int hi = maxD();
+		if( net )
+		{
+			lo = unet::Level( net );
+			if( lo<minD() || lo>hi ) throw std::string( "the network's level is outside [minSolveDepth, maxSolveDepth]" );
+			if( prm.singleLevel ) throw std::string( "--singleLevel has no coarse start to replace" );
+		}
 		{
 			std::vector< std::pair< void * , size_t > > spans;
 			for( int l=0 ; l<=D ; l++ ) spans.push_back( { (void *)levels[l].x.p , (size_t)levels[l].N*sizeof(double) } ) , spans.push_back( { (void *)levels[l].y.p , (size_t)levels[l].N*sizeof(double) } );
 			zeroSpans( spans );
 		}
-		init( lo , nullptr , nullptr );
+		if( net ) learnedInit( lo , net , nullptr );
+		else init( lo , nullptr , nullptr );
 		auto Iters = [&]( int d ){ return (int)std::ceil( std::pow( prm.iMult , D-d ) * prm.gsIters ); };
 		if( prm.singleLevel )
 		{
@@ -861,6 +964,71 @@ struct Ctx : public CtxBase
 		solved = true;
 		seconds[1] = std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count();
 	}
+
+	// ---------------- learned initializer ----------------
+	DevBuf< double > dFeatE , dFeatN , dFeatS;
+	DevBuf< unet::bf16 > dFeatRows;
+	DevBuf< float > dFeatF , dNetOut;
+
+	// the network input of level L: the finest target field averaged over each
+	// level-L element, then over each node's incident elements, divided by its
+	// root mean square (bf16 rows in dFeatRows; floats in out when given)
+	void features( int L , float *out )
+	{
+		needBuilt() ; checkLevel( L );
+		long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= 1LL<<L;
+		const long long N = levels[L].N;
+		dFeatE.ensure( (size_t)ne*CD ) , dFeatN.ensure( (size_t)N*CD ) , dFeatS.ensure( 1 );
+		dFeatRows.ensure( (size_t)N*unet::CT );
+		if( out ) dFeatF.ensure( (size_t)N*CD );
+		KCoarseFieldMean< Dim ><<< Blocks( ne , 256 ) , 256 , 0 , stream >>>( dS.p , D , L , dFeatE.p );
+		KNodeFieldMean< Dim ><<< Blocks( N , 256 ) , 256 , 0 , stream >>>( dFeatE.p , L , dFeatN.p );
+		KSumSquares<<< 1 , 256 , 0 , stream >>>( dFeatN.p , N*CD , dFeatS.p );
+		KFeatureRows<<< Blocks( N*unet::CT , 256 ) , 256 , 0 , stream >>>( dFeatN.p , N , CD , dFeatS.p , dFeatRows.p , out ? dFeatF.p : nullptr );
+		HIPCHECK( hipGetLastError() );
+		if( out ){ dFeatF.download( out , (size_t)N*CD , stream ) ; HIPCHECK( hipStreamSynchronize( stream ) ); }
+	}
+	void learnedFeatures( int L , float *out ){ features( L , out ); }
+
+	// The start of level L from the network: its (x, y), reflected (y -> -y)
+	// when that raises y^T B x (the energy's only odd term), scaled by the
+	// factor a that minimises the fitting energy E_fit(a x, a y) =
+	// a^4 W - 2 a^2 y^T B x + c, a^2 = y^T B x / W.  (The screening and
+	// Dirichlet terms are left out of the scale: a prediction that is close to
+	// the solution in L2 but not zero exactly at the samples pays a screening
+	// energy far above the solution's, which would shrink the start towards
+	// zero; the cascade's first sweeps remove that residual.)
+	// Levels above L are zeroed (the cascade adds into them).  info: a, the
+	// reflection (0/1), the energy after scaling.
+	void learnedInit( int L , unet::Net *net , double info[3] )
+	{
+		needBuilt() ; checkLevel( L );
+		if( unet::Dim( net )!=Dim || unet::Level( net )!=L || unet::InChannels( net )!=CD ) throw std::string( "the network's dimension, level or input channels do not match this reconstruction" );
+		features( L , nullptr );
+		Level &Lv = levels[L];
+		dNetOut.ensure( (size_t)Lv.N*unet::CT );
+		unet::ForwardDevice( net , dFeatRows.p , dNetOut.p , stream );
+		KCoeffsFromNet<<< Blocks( Lv.N , 256 ) , 256 , 0 , stream >>>( dNetOut.p , Lv.N , Lv.fm.p , Lv.x.p , Lv.y.p );
+		for( int l=L+1 ; l<=D ; l++ ) levels[l].x.zero( stream ) , levels[l].y.zero( stream );
+		double t[4];
+		energyTerms( L , t );
+		bool reflect = false;
+		if( t[1]<0 ){ reflect = true ; t[1] = -t[1]; }
+		const double a2 = t[1] / t[0];
+		if( !( a2>0 ) || !std::isfinite( a2 ) ) throw std::string( "the network's prediction has no fitting-energy-lowering scale; use the random start" );
+		const double a = std::sqrt( a2 );
+		dTmp2.ensure( 1 );
+		dTmp2.upload( &a , 1 , stream );
+		KScale2<<< Blocks( Lv.N , 256 ) , 256 , 0 , stream >>>( Lv.x.p , Lv.y.p , Lv.N , dTmp2.p );
+		if( reflect ) KScaleArray<<< Blocks( Lv.N , 256 ) , 256 , 0 , stream >>>( Lv.y.p , Lv.N , -1. );
+		HIPCHECK( hipGetLastError() );
+		double e[2];
+		energies( L , e );
+		if( info ) info[0] = a , info[1] = reflect ? 1. : 0. , info[2] = e[0] + e[1];
+	}
+
+	// the cascade of solve() started at the network's level from its prediction
+	void solveLearned( unet::Net *net , double out[2] ){ solveFrom( out , net ); }
 
 	void getCoeffs( int l , double *x , double *y )
 	{
@@ -1530,7 +1698,8 @@ struct epr_ctx
 	std::string err;
 };
 
-const char *EprMeshLastError( void );   // epr_levelset.hip: the calling thread's last context-free error
+const char *EprMeshLastError( void );
+epr::unet::Net *EprUnetNet( epr_unet *u );   // epr_unet.hip   // epr_levelset.hip: the calling thread's last context-free error
 
 template< typename F >
 static int Guard( epr_ctx *ctx , F f )
@@ -1609,6 +1778,9 @@ int epr_prolong_add( epr_ctx *ctx , int level ){ IMPL( ctx->impl->prolongAdd( le
 int epr_energies( epr_ctx *ctx , int level , double out[2] ){ IMPL( ctx->impl->energies( level , out ) ); }
 int epr_energies_noalloc( epr_ctx *ctx , int level , double out[2] ){ if( !out ) return EPR_ERR_ARG ; IMPL( ctx->impl->energiesNoAlloc( level , out ) ); }
 int epr_solve( epr_ctx *ctx , double out_E[2] ){ IMPL( ctx->impl->solve( out_E ) ); }
+int epr_learned_features( epr_ctx *ctx , int level , float *out ){ if( !out ) return EPR_ERR_ARG ; IMPL( ctx->impl->learnedFeatures( level , out ) ); }
+int epr_init_learned( epr_ctx *ctx , epr_unet *net , int level , double info[3] ){ if( !EprUnetNet( net ) ) return EPR_ERR_ARG ; IMPL( ctx->impl->learnedInit( level , EprUnetNet( net ) , info ) ); }
+int epr_solve_learned( epr_ctx *ctx , epr_unet *net , double out_E[2] ){ if( !EprUnetNet( net ) ) return EPR_ERR_ARG ; IMPL( ctx->impl->solveLearned( EprUnetNet( net ) , out_E ) ); }
 size_t epr_num_functions( const epr_ctx *ctx , int level ){ return ( ctx && ctx->impl && level>=0 && level<=ctx->prm.depth ) ? ctx->impl->numFunctions( level ) : 0; }
 int epr_get_coeffs( epr_ctx *ctx , int level , double *x , double *y ){ IMPL( ctx->impl->getCoeffs( level , x , y ) ); }
 int epr_set_coeffs( epr_ctx *ctx , int level , const double *x , const double *y ){ IMPL( ctx->impl->setCoeffs( level , x , y ) ); }

@@ -142,6 +142,91 @@ def clifford_torus(side=256, ang_deg=2.0, seed=0):
     return pos, blade(tu, tv)
 
 
+def dual4(t):
+    """Hodge dual of a 4x4 skew matrix with the reference's index map
+    (Surface::Dual, Include/Samples.h:798-808): n(1,0) = t(2,3), n(2,0) = t(3,1),
+    n(3,0) = t(1,2), n(0,1) = t(3,2), n(2,1) = t(0,3), n(3,1) = t(2,0),
+    n(0,2) = t(1,3), n(1,2) = t(3,0), n(3,2) = t(0,1), n(0,3) = t(2,1),
+    n(1,3) = t(0,2), n(2,3) = t(1,0)."""
+    n = np.zeros(t.shape)
+    pairs = {(1, 0): (2, 3), (2, 0): (3, 1), (3, 0): (1, 2), (0, 1): (3, 2), (2, 1): (0, 3), (3, 1): (2, 0),
+             (0, 2): (1, 3), (1, 2): (3, 0), (3, 2): (0, 1), (0, 3): (2, 1), (1, 3): (0, 2), (2, 3): (1, 0)}
+    for (a, b), (c, d) in pairs.items():
+        n[..., a, b] = t[..., c, d]
+    return n
+
+
+def hopf_map_torus(s, t, lat, dlat):
+    """Points and frames of a Hopf torus at parameters (s, t) in [0,1)^2: the
+    spherical curve (cos g cos f, cos g sin f, sin g) with longitude f = 2 pi s
+    and latitude g = lat(s) (derivative dlat) is lifted through the Hopf
+    fibration as the reference does (Samples.h:1072-1107): with (a, b, c) the
+    curve point and T = 2 pi t, the point is ((1+c) cos T, a sin T - b cos T,
+    a cos T + b sin T, (1+c) sin T) / sqrt(2 (1+c)); the frame is the Hodge dual
+    of the tangent blade (d/ds, d/dt), which is the reference's frame with its
+    sign (its derivative along the curve comes from the curve sampler's dual
+    frame)."""
+    f = 2.0 * np.pi * s
+    g, dg = lat, dlat
+    a, b, c = np.cos(g) * np.cos(f), np.cos(g) * np.sin(f), np.sin(g)
+    da = -np.sin(g) * dg * np.cos(f) - np.cos(g) * np.sin(f) * 2.0 * np.pi
+    db = -np.sin(g) * dg * np.sin(f) + np.cos(g) * np.cos(f) * 2.0 * np.pi
+    dc = np.cos(g) * dg
+    T = 2.0 * np.pi * t
+    ct, st = np.cos(T), np.sin(T)
+    k = 1.0 / np.sqrt(2.0 * (1.0 + c))
+    dk = -dc / (2.0 * (1.0 + c)) ** 1.5
+    pos = np.stack([(1 + c) * ct * k, (a * st - b * ct) * k, (a * ct + b * st) * k, (1 + c) * st * k], -1)
+    d_t = np.stack([-(1 + c) * st * k, (a * ct + b * st) * k, (-a * st + b * ct) * k, (1 + c) * ct * k], -1)
+    d_s = np.stack([dc * ct * k + (1 + c) * ct * dk,
+                    (da * st - db * ct) * k + (a * st - b * ct) * dk,
+                    (da * ct + db * st) * k + (a * ct + b * st) * dk,
+                    dc * st * k + (1 + c) * st * dk], -1)
+    return pos, dual4(blade(d_s, d_t))
+
+
+def hopf_torus(side=256, lobes=3, amp=0.3, latitude="sin", phase=0.0, ang_deg=0.0, seed=0, rotation=None):
+    """Hopf torus: the preimage under the Hopf map of a spherical curve of
+    `lobes` lobes, side^2 samples with (s, t) ~ U[0,1)^2.  latitude="sin": the
+    polar angle pi/2 + amp sin(2 pi lobes s + phase) (the benchmark's curve
+    theta = pi/2 + 0.3 sin(3 s), phi = s); latitude="cos": latitude
+    amp cos(2 pi lobes s + phase), the reference's hopf_torus:lobes:amp curve
+    (Samples.h:596-608).  Optional angular noise of the frame's two tangents
+    and a 4x4 rotation of positions and frames.  With latitude="cos" and the
+    reference Sample's parameter draws the points and frames are the reference's
+    (tests/test_host.py)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = side * side
+    s = rng.uniform(0.0, 1.0, n)
+    t = rng.uniform(0.0, 1.0, n)
+    w = 2.0 * np.pi * lobes
+    if latitude == "sin":
+        lat, dlat = -amp * np.sin(w * s + phase), -amp * w * np.cos(w * s + phase)
+    else:
+        lat, dlat = amp * np.cos(w * s + phase), -amp * w * np.sin(w * s + phase)
+    pos, skew = hopf_map_torus(s, t, lat, dlat)
+    if ang_deg > 0:
+        # perturb the normal plane: rotate the frame by a small random rotation
+        eps = np.deg2rad(ang_deg)
+        A = rng.normal(size=(n, 4, 4)) * eps / 2.0
+        A = A - np.swapaxes(A, -1, -2)
+        Q = np.eye(4) + A + 0.5 * A @ A
+        skew = Q @ skew @ np.swapaxes(Q, -1, -2)
+    if rotation is not None:
+        pos = pos @ rotation.T
+        skew = rotation @ skew @ rotation.T
+    return pos, skew
+
+
+def random_rotation(dim, rng):
+    """A uniformly random rotation (Haar measure, QR of a Gaussian matrix)."""
+    q, r = np.linalg.qr(rng.normal(size=(dim, dim)))
+    q = q * np.sign(np.diag(r))
+    if np.linalg.det(q) < 0:
+        q[:, 0] = -q[:, 0]
+    return q
+
+
 def write_ply(path, pos, skew):
     """Binary little-endian PLY with x,y,z[,w] and skew_c_r (double): the
     bytes the reference's Sample writes for the same values

@@ -87,7 +87,8 @@ int  epr_build( epr_ctx *ctx );
 /* Sets the coefficients of one level; x0/y0 NULL selects the reference's
  * random initial guess at minSolveDepth (ExteriorPoisson.h:333-341: glibc's
  * rand() sequence after srand(seed), drawn from a private copy of that
- * generator, so the process-wide rand() state is neither read nor changed). */
+ * generator, so the process-wide rand() state is neither read nor changed);
+ * the random start also zeroes the levels above `level`. */
 int  epr_init( epr_ctx *ctx , int level , const double *x0 , const double *y0 );
 
 /* iters paired multicolour Gauss-Seidel iterations at one level
@@ -341,6 +342,57 @@ void epr_iso_destroy( epr_iso *iso );
  * and each face-connected component is oriented like its first simplex, with
  * the reference's walk order deciding on non-orientable components. */
 int  epr_orient( int k , size_t nS , unsigned *simplices );
+
+/* ---- Learned initializer ----
+ * No counterpart in the reference.  A convolutional U-Net on the hat-function
+ * node grid of one coarse level L, (2^L+1)^dim nodes (four grids of 2^L+1,
+ * 2^(L-1)+1, 2^(L-2)+1, 2^(L-3)+1 nodes per side, 16/32/64/128 channels,
+ * 3^dim kernels, BF16 storage and FP32 accumulation on the device's matrix
+ * cores), maps the target field averaged onto that level to the level's
+ * coefficients (x, y); they replace the random initial guess
+ * (ExteriorPoisson.h:333-341) and the reference's cascade refines them from
+ * level L on.  Errors: epr_unet_last_error( net ) (NULL: the calling
+ * thread's last error of epr_unet_create). */
+typedef struct epr_unet epr_unet;
+int  epr_unet_create( int dim , int level , int inChannels , epr_unet **out );   /* inChannels: dim*(dim-1)/2 for epr_learned_features */
+void epr_unet_destroy( epr_unet *net );
+const char *epr_unet_last_error( const epr_unet *net );
+size_t epr_unet_num_params( const epr_unet *net );
+size_t epr_unet_num_nodes( const epr_unet *net );   /* (2^level+1)^dim */
+/* Parameters in the canonical order: per convolution (encoder 2 per grid,
+ * finest first; decoder 2 per grid, coarsest first; the output) the weights
+ * W[tap][out][in] then the biases[out]; tap = base-3 offset index, first
+ * coordinate most significant.  set resets the optimizer. */
+int  epr_unet_set_params( epr_unet *net , const float *params );
+int  epr_unet_get_params( epr_unet *net , float *params );
+/* in: batch x nodes x inChannels, out: batch x nodes x 2 (host). */
+int  epr_unet_forward( epr_unet *net , int batch , const float *in , float *out );
+/* The training loss -- per sample the mean squared distance to the target
+ * after the best rotation of the (x, y) plane, averaged over the batch -- and
+ * its gradient with respect to every parameter (canonical order; grad may be
+ * NULL).  target: batch x nodes x 2. */
+int  epr_unet_loss_gradients( epr_unet *net , int batch , const float *in , const float *target , double *loss , float *grad );
+/* Training on the device: the set is uploaded once; each step takes `batch`
+ * samples of a seeded shuffle, runs the forward and backward passes and an
+ * Adam update (learning rate lr); losses (may be NULL) receives every step's
+ * loss. */
+int  epr_unet_set_dataset( epr_unet *net , size_t count , const float *in , const float *target );
+int  epr_unet_train( epr_unet *net , int steps , int batch , double lr , unsigned seed , double *losses );
+
+/* The network input of a built reconstruction at a level: the finest target
+ * field averaged over each element of the level, then over each node's
+ * incident elements, divided by its root mean square; out: nodes x C(dim,2). */
+int  epr_learned_features( epr_ctx *ctx , int level , float *out );
+/* Sets the coefficients of a level from the network (the reference's random
+ * start replaced): its (x, y), y negated when that lowers the energy, scaled by
+ * the factor that minimises the fitting energy E_fit = |grad x ^ grad y - S|^2;
+ * finer levels are zeroed.  info (may be NULL): the factor, the negation
+ * (0/1), the level's energy afterwards.  A prediction with no
+ * fitting-energy-lowering scale is an error (use epr_init's random start). */
+int  epr_init_learned( epr_ctx *ctx , epr_unet *net , int level , double info[3] );
+/* epr_solve with the cascade started at the network's level from its
+ * prediction instead of the random start at minSolveDepth. */
+int  epr_solve_learned( epr_ctx *ctx , epr_unet *net , double out_E[2] );
 
 /* typeid( MishaK::Hat::Hierarchical{Regular,Adapted}Indexer< dim > ).hash_code()
  * of a libstdc++ build of the reference: the value a .tree carries after its

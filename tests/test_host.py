@@ -272,3 +272,36 @@ int main(){
     subprocess.check_call(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "exteriorpoissonrecon_amd", "csrc"),
                            str(src), "-o", str(exe)])
     assert subprocess.run([str(exe)], capture_output=True, text=True).stdout.strip() == "ok"
+
+
+def test_hopf_sampler_matches_reference_sample(tmp_path):
+    """samples.hopf_map_torus with the reference's curve (hopf_torus:3:0.3, the
+    latitude 0.3 cos(6 pi s)) at the parameters the reference Sample drew
+    (std::default_random_engine = minstd_rand0 seeded 1, each
+    uniform_real_distribution<double> value from two draws,
+    generate_canonical<double,53>; s then t) gives the reference file's points
+    and frames (tests/golden/sample_plys.npz, written by the reference Sample)."""
+    z = np.load(os.path.join(ROOT, "tests", "golden", "sample_plys.npz"), allow_pickle=False)
+    assert str(z["hopf4_args"]).split()[1] == "hopf_torus:3:0.3"
+    f = tmp_path / "h.ply"
+    f.write_bytes(z["hopf4"].tobytes())
+    P, K = samples.read_ply(str(f))
+    x = 1
+    R = 2147483646.0
+
+    def draw():
+        nonlocal x
+        x = (16807 * x) % 2147483647
+        return x - 1
+
+    st = []
+    for _ in range(len(P)):
+        s = (draw() + draw() * R) / (R * R)
+        t = (draw() + draw() * R) / (R * R)
+        st.append((s, t))
+    s, t = np.array(st).T
+    w = 6.0 * np.pi
+    pos, skew = samples.hopf_map_torus(s, t, 0.3 * np.cos(w * s), -0.3 * w * np.sin(w * s))
+    assert np.abs(pos - P).max() < 1e-12
+    unit = lambda k: k / np.linalg.norm(k.reshape(len(k), -1), axis=1)[:, None, None]
+    assert np.abs(unit(skew) - unit(K)).max() < 1e-12
