@@ -36,6 +36,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "time-to-energy (1e-6 rel vs FP64) at depth 7 3D / 6 4D; GS sweep HBM GB/s vs peak"
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 # Algorithmic FP64 flops of one node update in the element-local
 # formulation: per incident element 3 per corner pair (wedge products), 2 per
@@ -252,7 +253,7 @@ def extract_cpu_baseline(td, dim):
     return time.perf_counter() - t0
 
 
-def measure_workload(name, args, steps, warmup, with_cpu, extract=False):
+def measure_workload(name, args, steps, warmup, with_cpu, extract=False, fp32=0):
     """Times `steps` whole reconstructions of one workload (after `warmup`
     untimed ones) and its finest-sweep kernel; returns (result dict, last
     Reconstructor).  A step is set_samples (unit-cube map and the ordered
@@ -261,8 +262,8 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False):
     wl = WORKLOADS[name]
     pos, skew = wl["gen"](samples)
     dim, depth = wl["dim"], wl["depth"]
-    log(f"[bench] {name}: {pos.shape[0]} samples, dim {dim}, depth {depth}")
-    rec = Reconstructor(pos, skew, depth=depth, aRadius=args.aRadius)
+    log(f"[bench] {name}: {pos.shape[0]} samples, dim {dim}, depth {depth}{' --fp32' if fp32 else ''}")
+    rec = Reconstructor(pos, skew, depth=depth, aRadius=args.aRadius, fp32Sweeps=fp32)
 
     E = None
     for i in range(warmup):
@@ -286,16 +287,16 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False):
         rel = abs(float(E.sum()) - float(sum(Eref))) / abs(float(sum(Eref)))
         log(f"[bench] E={E.sum():.17g} reference {sum(Eref):.17g} rel {rel:.3e}")
 
-    # dominant kernel: one colour phase of the finest-level sweep
+    # dominant kernel: one colour phase of the finest-level sweep (FP32 form with fp32)
     iters = 10
     N = rec.num_functions(depth)
-    ms, launches = rec.time_gauss_seidel(depth, iters)
+    ms, launches = (rec.time_gauss_seidel_fp32 if fp32 else rec.time_gauss_seidel)(depth, iters)
     per_launch_s = ms * 1e-3 / launches
     # nodes relaxed per launch: a colour of the level (the active ones of an adapted level)
     updated = N if args.aRadius < 0 else int(rec.array(f"adapted_nactive_{depth}")[0])
     nodes_per_launch = updated / (1 << dim)
     flops_per_launch = F_NODE[dim] * nodes_per_launch
-    bytes_per_launch = B_NODE[dim] * nodes_per_launch
+    bytes_per_launch = (B_NODE[dim] // 2 if fp32 else B_NODE[dim]) * nodes_per_launch
     achieved = flops_per_launch / per_launch_s / 1e12
     achieved_ref = F_NODE_REF[dim] * nodes_per_launch / per_launch_s / 1e12
     log(f"[bench] finest sweep: {ms / iters:.3f} ms/iteration, {launches} launches, "
@@ -314,7 +315,7 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False):
     copy_gbs = copy_bandwidth()
     if copy_gbs:
         log(f"[bench] copy kernel: {copy_gbs:.1f} GB/s ({2 * COPY_BYTES} bytes per copy)")
-    traffic = measured_traffic(name if args.aRadius < 0 else f"{name}_r{args.aRadius}")
+    traffic = measured_traffic((name if args.aRadius < 0 else f"{name}_r{args.aRadius}") + ("_fp32" if fp32 else ""))
     if traffic:
         log(f"[bench] measured HBM traffic of the finest-sweep kernel: {traffic / 1e6:.2f} MB per launch "
             f"({traffic / per_launch_s / 1e9:.1f} GB/s at this launch time)")
@@ -341,10 +342,12 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False):
                    "reference_energy_source": REFERENCE_SOURCE.get(name) if Eref is not None else None},
         # the sweep runs on the FP64 vector ALU (v_fma_f64, no matrix
         # instructions); MI355X's FP64 vector peak equals its FP64 matrix peak
-        "roofline": {"bound": "mfma", "peak_kind": "fp64 vector ALU (78.6 TFLOP/s, = the FP64 matrix peak)",
-                     "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": traffic,
-                     "kernel": f"{'KGsPhase' if args.aRadius < 0 else 'KGsPhaseList'}<{dim}> (finest level, one colour)",
+        "roofline": {"bound": "mfma",
+                     "peak_kind": "fp32 vector ALU (157.3 TFLOP/s published)" if fp32
+                     else "fp64 vector ALU (78.6 TFLOP/s, = the FP64 matrix peak)",
+                     "achieved": achieved, "peak": FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved / (FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS), "traffic": traffic,
+                     "kernel": f"{'KGsPhase' if args.aRadius < 0 else 'KGsPhaseList'}<{dim}{', float' if fp32 else ''}> (finest level, one colour)",
                      "hbm_gbs_algorithmic": bytes_per_launch / per_launch_s / 1e9,
                      "flops_per_node": F_NODE[dim], "reference_equivalent_tflops": achieved_ref,
                      "us_per_launch": per_launch_s * 1e6, "copy_kernel_gbs": copy_gbs,
@@ -417,6 +420,7 @@ def main():
     ap.add_argument("--no-4d", action="store_true",
                     help="default workload: skip the nested 4D depth-6 (clifford4d_d6) measurement")
     ap.add_argument("--steps-4d", type=int, default=5, help="timed steps of the nested 4D measurement")
+    ap.add_argument("--no-fp32", action="store_true", help="skip the --fp32 (FP32 sweeps, FP64 refinement) measurement")
     ap.add_argument("--no-learned", action="store_true",
                     help="default workload: skip the nested learned-initializer measurement (4D Hopf torus)")
     ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
@@ -435,35 +439,60 @@ def main():
     wl = WORKLOADS[args.config]
     if "batch" in wl:
         return batch_main(args, wl)
-    res, rec = measure_workload(args.config, args, args.steps, args.warmup, not args.no_cpu, args.extract)
+    def both(name, steps, warmup, with_cpu, extract=False):
+        """FP64 and --fp32 measurements of one workload; the --fp32 one is the
+        headline when its final energy is within 1e-6 of the pinned reference
+        energy (the metric's bar), else the FP64 one."""
+        r64, rec = measure_workload(name, args, steps, warmup, with_cpu, extract)
+        rec.close()
+        r32 = None
+        if not args.no_fp32 and args.aRadius < 0:
+            r32, rec = measure_workload(name, args, steps, warmup, False, False, fp32=1)
+            rec.close()
+            if r32["config"]["energy_rel_err_vs_reference"] is None:
+                # no pinned energy: the metric's FP64 reference is this run's FP64 path
+                e64, e32 = r64["config"]["final_energy"], r32["config"]["final_energy"]
+                r32["config"]["energy_rel_err_vs_reference"] = abs(e32 - e64) / abs(e64)
+                r32["config"]["reference_energy_source"] = "the FP64 path of this run (same samples and schedule)"
+        ok = r32 is not None and r32["config"]["energy_rel_err_vs_reference"] is not None \
+            and r32["config"]["energy_rel_err_vs_reference"] <= 1e-6
+        top = r32 if ok else r64
+        cfg = dict(top["config"])
+        cfg["precision"] = "fp32 sweeps, each level's last iteration and the energies fp64" if ok else "fp64"
+        cfg["fp64"] = {"ms_per_step": r64["value"] * 1e3, **{k: r64["config"][k] for k in (
+            "set_samples_s", "build_s", "solve_s", "final_energy", "energy_rel_err_vs_reference")},
+            "roofline": r64["roofline"]}
+        if r32 is not None and not ok:
+            cfg["fp32_rejected"] = {"ms_per_step": r32["value"] * 1e3, "energy_rel_err_vs_reference": r32["config"]["energy_rel_err_vs_reference"]}
+        return top, r64, cfg, ok
+
+    top, r64, cfg, ok = both(args.config, args.steps, args.warmup, not args.no_cpu, args.extract)
     out = {
         "metric": METRIC,
-        "value": res["value"],
+        "value": top["value"],
         "unit": "s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": res["value"] * 1e3,
+        "ms_per_step": top["value"] * 1e3,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64" if ok else "f64",
         "data": "synthetic",
-        "config": res["config"],
-        "roofline": res["roofline"],
-        "cpu_baseline": res["cpu_baseline"],
+        "config": cfg,
+        "roofline": top["roofline"],
+        "cpu_baseline": r64["cpu_baseline"],
     }
     for k in ("levels", "extract"):
-        if k in res:
-            out[k] = res[k]
+        if k in r64:
+            out[k] = r64[k]
     # the metric's second half: the 4D depth-6 reconstruction, in the same run
     if args.config == "borromean3d_d7" and not args.no_4d:
-        rec.close()
-        del rec
-        r4, rec4 = measure_workload("clifford4d_d6", args, args.steps_4d, 1, not args.no_cpu)
-        rec4.close()
-        out["clifford4d_d6"] = {"ms_per_reconstruction": r4["value"] * 1e3, "steps": args.steps_4d, "warmup": 1,
-                                **r4["config"], "roofline": r4["roofline"], "cpu_baseline": r4["cpu_baseline"]}
+        t4, r4, cfg4, ok4 = both("clifford4d_d6", args.steps_4d, 1, not args.no_cpu)
+        out["clifford4d_d6"] = {"ms_per_reconstruction": t4["value"] * 1e3, "steps": args.steps_4d, "warmup": 1,
+                                "dtype": "f32+f64" if ok4 else "f64", **cfg4, "roofline": t4["roofline"],
+                                "cpu_baseline": r4["cpu_baseline"]}
         if "levels" in r4:
             out["clifford4d_d6"]["levels"] = r4["levels"]
     if args.config == "borromean3d_d7" and not args.no_learned:

@@ -25,7 +25,7 @@ EXPORTS = [
     "epr_build_batch", "epr_solve_batch", "epr_density_grid", "epr_gs_colour", "epr_energies_noalloc",
     "epr_unet_create", "epr_unet_destroy", "epr_unet_last_error", "epr_unet_num_params", "epr_unet_num_nodes",
     "epr_unet_set_params", "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients",
-    "epr_unet_set_dataset", "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned",
+    "epr_unet_set_dataset", "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned", "epr_time_gs_sweeps_fp32",
 ]
 
 
@@ -38,7 +38,7 @@ class Params(ctypes.Structure):
                 ("nSigma", ctypes.c_double), ("scale", ctypes.c_double),
                 ("sWeight", ctypes.c_double), ("dWeight", ctypes.c_double),
                 ("iMult", ctypes.c_double), ("aRadius", ctypes.c_int), ("separate", ctypes.c_int),
-                ("jitter", ctypes.c_double)]
+                ("jitter", ctypes.c_double), ("fp32Sweeps", ctypes.c_int)]
 
 
 class EprError(RuntimeError):
@@ -104,6 +104,8 @@ def load():
     L.epr_density_grid.argtypes = [ctypes.c_char_p, ci, ci, ctypes.c_char_p]
     L.epr_time_gs_sweeps.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
     L.epr_time_gs_sweeps.restype = ctypes.c_int
+    L.epr_time_gs_sweeps_fp32.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
+    L.epr_time_gs_sweeps_fp32.restype = ctypes.c_int
     up = ctypes.POINTER(ctypes.c_uint)
     L.epr_subgrid_triangulation.argtypes = [ci, sz, up, ci, ctypes.POINTER(vp)]
     L.epr_subgrid_triangulation.restype = ctypes.c_int

@@ -35,6 +35,7 @@ static void Usage( const char *ex )
 	printf( "\t[--tRadius <tensor kernel radius>=1]\n" );
 	printf( "\t[--aRadius <adaptive radius>=3]\n" );
 	printf( "\t[--jitter <jitter magnitude>=%f]\n" , 0. );
+	printf( "\t[--fp32] (Gauss-Seidel iterations in FP32 but each level's last, on the larger levels)\n" );
 	printf( "\t[--gsIters <gs iterations>=10]\n" );
 	printf( "\t[--seed <random seed>=0]\n" );
 	printf( "\t[--sWeight <screening weight>=100]\n" );
@@ -79,6 +80,7 @@ int main( int argc , char *argv[] )
 		else if( a=="jitter" ) p.jitter = atof( val().c_str() );
 		else if( a=="singleLevel" ) p.singleLevel = 1;
 		else if( a=="separate" ) p.separate = 1;
+		else if( a=="fp32" ) p.fp32Sweeps = 1;
 		else if( a=="project" ) project = true;
 		else if( a=="verbose" ) p.verbose = 1;
 		else fprintf( stderr , "[WARNING] Invalid parameter: --%s\n" , a.c_str() );

@@ -219,6 +219,9 @@ __global__ void __launch_bounds__( 256 ) KCoeffsFromNet( const float *yNet , lon
 	y[i] = on ? (double)yNet[ i*unet::CT + 1 ] : 0.;
 }
 
+__global__ void __launch_bounds__( 256 ) KNarrow( const double *a , float *b , long long n ){ const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x ; if( i<n ) b[i] = (float)a[i]; }
+__global__ void __launch_bounds__( 256 ) KWiden( const float *a , double *b , long long n ){ const long long i = (long long)blockIdx.x*blockDim.x + threadIdx.x ; if( i<n ) b[i] = (double)a[i]; }
+
 // Device scratch of the allocation-free energies (epr_energies_noalloc): the
 // per-workgroup partial sums of KEnergy and the four summed terms.  One copy
 // per device, held until the process ends; every call on a device shares it,
@@ -280,6 +283,7 @@ struct CtxBase
 	virtual void gs( int level , int iters ) = 0;
 	virtual void gsColour( int level , int colour ) = 0;
 	virtual void timeGs( int level , int iters , double *ms , int *launches ) = 0;
+	virtual void timeGsF32( int level , int iters , double *ms , int *launches ) = 0;
 	virtual void prolongAdd( int level ) = 0;
 	virtual void energies( int level , double out[2] ) = 0;
 	virtual void energiesNoAlloc( int level , double out[2] ) = 0;
@@ -337,17 +341,24 @@ struct Ctx : public CtxBase
 		DevBuf< double > x , y , R , B , K;
 		hipGraphExec_t graph = nullptr;    // captured Gauss-Seidel launches of this level
 		int graphIters = 0;
+		// --fp32: float copies of x, y, R, B, K (R, B, K narrowed once per build)
+		// and the captured float launches
+		DevBuf< float > fx , fy , fR , fB , fK;
+		bool narrowed = false;
+		hipGraphExec_t graph32 = nullptr;
+		int graph32Iters = 0;
 		const double *graphX = nullptr , *graphR = nullptr;
 		const unsigned char *graphF = nullptr;
 		const unsigned int *graphAct = nullptr;
 		long long graphActStart[(1<<Dim)+1] = {};
 		Level( void ){}
-		Level( Level &&o ) noexcept : AdaptedLevelDev( std::move( o ) ) , res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , graph( o.graph ) , graphIters( o.graphIters )
+		Level( Level &&o ) noexcept : AdaptedLevelDev( std::move( o ) ) , res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , graph( o.graph ) , graphIters( o.graphIters ) ,
+		                              fx( std::move( o.fx ) ) , fy( std::move( o.fy ) ) , fR( std::move( o.fR ) ) , fB( std::move( o.fB ) ) , fK( std::move( o.fK ) ) , narrowed( o.narrowed ) , graph32( o.graph32 ) , graph32Iters( o.graph32Iters )
 		{
 			for( int d=0 ; d<Dim ; d++ ) stride[d] = o.stride[d];
-			o.graph = nullptr;
+			o.graph = nullptr , o.graph32 = nullptr;
 		}
-		~Level(){ if( graph ) (void)hipGraphExecDestroy( graph ); }
+		~Level(){ if( graph ) (void)hipGraphExecDestroy( graph ) ; if( graph32 ) (void)hipGraphExecDestroy( graph32 ); }
 		// drops the captured launches when the buffers they point at move or the
 		// active lists they cover change
 		void checkGraph( void )
@@ -510,6 +521,7 @@ struct Ctx : public CtxBase
 	void build( void )
 	{
 		if( !n ) throw std::string( "no samples" );
+		for( Level &L : levels ) L.narrowed = false;   // the float copies of R, B follow the new operators
 		auto t0 = std::chrono::steady_clock::now();
 		const int res = 1<<D;
 		dPos.upload( pos.data() , pos.size() , stream );
@@ -744,10 +756,17 @@ struct Ctx : public CtxBase
 		else if( pl.kind==GS_SMALL ) KGsSmall< Dim ><<< 1 , 512 , 0 , stream >>>( v , iters );
 		else
 		{
+			Level &L = levels[l];
+			if( prm.fp32Sweeps && !L.fm.p && iters>1 )
+			{
+				// --fp32: all iterations but the last in FP32, the last in FP64 from
+				// the widened iterate (the refinement), energies FP64
+				gsFloat( l , iters-1 );
+				iters = 1;
+			}
 			// the iters*2^Dim colour launches of a level are captured once into a
 			// HIP graph and replayed (the level's buffers are stable between builds
 			// of the same size)
-			Level &L = levels[l];
 			if( !L.graph || L.graphIters!=iters )
 			{
 				if( L.graph ){ (void)hipGraphExecDestroy( L.graph ) ; L.graph = nullptr; }
@@ -800,6 +819,93 @@ struct Ctx : public CtxBase
 		}
 		HIPCHECK( hipGetLastError() );
 		HIPCHECK( hipStreamSynchronize( stream ) );
+	}
+
+	// iters FP32 iterations of a dense level swept by colour launches: x, y
+	// (and, once per build, R, B, K) rounded to float, the float colour
+	// launches (captured into a graph and replayed), x, y widened back
+	void gsFloat( int l , int iters )
+	{
+		Level &L = levels[l];
+		const long long N = L.N;
+		if( !L.narrowed )
+		{
+			L.fx.ensure( N ) , L.fy.ensure( N ) , L.fR.ensure( (size_t)N*W ) , L.fB.ensure( (size_t)N*W ) , L.fK.ensure( (size_t)W*W );
+			KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.R.p , L.fR.p , N*W );
+			KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.B.p , L.fB.p , N*W );
+			KNarrow<<< Blocks( (long long)W*W , 256 ) , 256 , 0 , stream >>>( L.K.p , L.fK.p , (long long)W*W );
+			L.narrowed = true;
+			if( L.graph32 ){ (void)hipGraphExecDestroy( L.graph32 ) ; L.graph32 = nullptr; }
+		}
+		KNarrow<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.x.p , L.fx.p , N );
+		KNarrow<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.y.p , L.fy.p , N );
+		if( !L.graph32 || L.graph32Iters!=iters )
+		{
+			if( L.graph32 ){ (void)hipGraphExecDestroy( L.graph32 ) ; L.graph32 = nullptr; }
+			const LevelView< Dim > v = view( l );
+			LevelView< Dim , float > f;
+			f.res = v.res , f.N = v.N , f.x = L.fx.p , f.y = L.fy.p , f.R = L.fR.p , f.B = L.fB.p , f.K = L.fK.p;
+			f.sW = (float)v.sW , f.dW = (float)v.dW , f.lay = v.lay , f.fmask = nullptr , f.separate = v.separate;
+			for( int d=0 ; d<Dim ; d++ ) f.stride[d] = v.stride[d];
+			hipGraph_t g;
+			HIPCHECK( hipStreamBeginCapture( stream , hipStreamCaptureModeThreadLocal ) );
+			for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+			{
+				ColourSet< Dim > cs ; cs.set( L.res , c );
+				KGsPhase< Dim , float ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( f , cs , f.lay.cstart[c] );
+			}
+			HIPCHECK( hipStreamEndCapture( stream , &g ) );
+			HIPCHECK( hipGraphInstantiate( &L.graph32 , g , nullptr , nullptr , 0 ) );
+			HIPCHECK( hipGraphDestroy( g ) );
+			L.graph32Iters = iters;
+		}
+		HIPCHECK( hipGraphLaunch( L.graph32 , stream ) );
+		KWiden<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.fx.p , L.x.p , N );
+		KWiden<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.fy.p , L.y.p , N );
+		HIPCHECK( hipGetLastError() );
+	}
+
+	// The FP32 form of the colour launches (measurement): the level's x, y, R,
+	// B and K rounded to float, iters iterations of KGsPhase< Dim , float >
+	// timed with HIP events, the float x, y written back to the level.
+	DevBuf< float > fX , fY , fR , fB , fK;
+	void timeGsF32( int l , int iters , double *ms , int *launches )
+	{
+		needBuilt() ; checkLevel( l );
+		Level &L = levels[l];
+		if( L.fm.p ) throw std::string( "the FP32 sweep measurement runs on dense levels" );
+		if( gsPlan( l ).kind!=GS_PHASES ) throw std::string( "the FP32 sweep measurement runs on levels swept by colour launches" );
+		const long long N = L.N;
+		fX.ensure( N ) , fY.ensure( N ) , fR.ensure( (size_t)N*W ) , fB.ensure( (size_t)N*W ) , fK.ensure( (size_t)W*W );
+		KNarrow<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.x.p , fX.p , N );
+		KNarrow<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.y.p , fY.p , N );
+		KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.R.p , fR.p , N*W );
+		KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.B.p , fB.p , N*W );
+		KNarrow<<< Blocks( (long long)W*W , 256 ) , 256 , 0 , stream >>>( L.K.p , fK.p , (long long)W*W );
+		const LevelView< Dim > v = view( l );
+		LevelView< Dim , float > f;
+		f.res = v.res , f.N = v.N , f.x = fX.p , f.y = fY.p , f.R = fR.p , f.B = fB.p , f.K = fK.p;
+		f.sW = (float)v.sW , f.dW = (float)v.dW , f.lay = v.lay , f.fmask = nullptr , f.separate = v.separate;
+		for( int d=0 ; d<Dim ; d++ ) f.stride[d] = v.stride[d];
+		hipEvent_t e0 , e1;
+		HIPCHECK( hipEventCreate( &e0 ) ) ; HIPCHECK( hipEventCreate( &e1 ) );
+		HIPCHECK( hipEventRecord( e0 , stream ) );
+		for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+		{
+			ColourSet< Dim > cs ; cs.set( L.res , c );
+			KGsPhase< Dim , float ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( f , cs , f.lay.cstart[c] );
+		}
+		HIPCHECK( hipEventRecord( e1 , stream ) );
+		HIPCHECK( hipEventSynchronize( e1 ) );
+		float t = 0;
+		HIPCHECK( hipEventElapsedTime( &t , e0 , e1 ) );
+		HIPCHECK( hipEventDestroy( e0 ) ) ; HIPCHECK( hipEventDestroy( e1 ) );
+		KWiden<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( fX.p , L.x.p , N );
+		KWiden<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( fY.p , L.y.p , N );
+		HIPCHECK( hipGetLastError() );
+		HIPCHECK( hipStreamSynchronize( stream ) );
+		if( ms ) *ms = t;
+		if( launches ) *launches = iters*C;
 	}
 
 	void timeGs( int l , int iters , double *ms , int *launches )
@@ -1716,7 +1822,7 @@ extern "C" {
 void epr_default_params( epr_params *p )
 {
 	p->depth = 5 , p->seed = 0 , p->tRadius = 1 , p->dRadius = 1 , p->minSolveDepth = 0 , p->maxSolveDepth = -1 , p->gsIters = 10;
-	p->singleLevel = 0 , p->verbose = 0 , p->separate = 0 , p->jitter = 0;
+	p->singleLevel = 0 , p->verbose = 0 , p->separate = 0 , p->jitter = 0 , p->fp32Sweeps = 0;
 	p->nSigma = 0.25 , p->scale = 1.1 , p->sWeight = 100. , p->dWeight = 0.00625 , p->iMult = 2.;
 	p->aRadius = 3;
 }
@@ -1773,6 +1879,7 @@ int epr_build( epr_ctx *ctx ){ IMPL( ctx->impl->build() ); }
 int epr_init( epr_ctx *ctx , int level , const double *x0 , const double *y0 ){ IMPL( ctx->impl->init( level , x0 , y0 ) ); }
 int epr_gs_sweeps( epr_ctx *ctx , int level , int iters ){ IMPL( ctx->impl->gs( level , iters ) ; HIPCHECK( hipDeviceSynchronize() ) ); }
 int epr_gs_colour( epr_ctx *ctx , int level , int colour ){ IMPL( ctx->impl->gsColour( level , colour ) ); }
+int epr_time_gs_sweeps_fp32( epr_ctx *ctx , int level , int iters , double *ms , int *launches ){ IMPL( ctx->impl->timeGsF32( level , iters , ms , launches ) ); }
 int epr_time_gs_sweeps( epr_ctx *ctx , int level , int iters , double *ms , int *launches ){ IMPL( ctx->impl->timeGs( level , iters , ms , launches ) ); }
 int epr_prolong_add( epr_ctx *ctx , int level ){ IMPL( ctx->impl->prolongAdd( level ) ); }
 int epr_energies( epr_ctx *ctx , int level , double out[2] ){ IMPL( ctx->impl->energies( level , out ) ); }

@@ -60,15 +60,15 @@ struct ColLayout
 };
 
 // One level as the node kernels see it.
-template< int Dim >
+template< int Dim , typename Real = double >
 struct LevelView
 {
 	int res;
 	long long N;
-	double *x , *y;
-	const double *R , *B , *K;
-	double sW;      // wedge scale res^(4-Dim)
-	double dW;      // Dirichlet weight
+	Real *x , *y;
+	const Real *R , *B , *K;
+	Real sW;      // wedge scale res^(4-Dim)
+	Real dW;      // Dirichlet weight
 	long long stride[Dim];
 	ColLayout< Dim > lay;   // column position of a node in R and B
 	const unsigned char *fmask;   // adapted levels: 1 = active (updated by the sweeps); null: every node is active
@@ -97,31 +97,31 @@ template<> struct WTab< 4 >
 };
 
 // z[j] += T(j,p) * w for the non-zero entries of column p (expanded at compile time)
-template< int Dim , int p , int ... js >
-__device__ __forceinline__ void WedgeColumn( double *z , double w , std::integer_sequence< int , js... > )
+template< int Dim , int p , typename Real , int ... js >
+__device__ __forceinline__ void WedgeColumn( Real *z , Real w , std::integer_sequence< int , js... > )
 {
-	( [&]{ if constexpr( WTab< Dim >::T( js , p )!=0.0 ) z[js] = fma( WTab< Dim >::T( js , p ) , w , z[js] ); }() , ... );
+	( [&]{ if constexpr( WTab< Dim >::T( js , p )!=0.0 ) z[js] = fma( (Real)WTab< Dim >::T( js , p ) , w , z[js] ); }() , ... );
 }
-template< int Dim , int ... ps >
-__device__ __forceinline__ void WedgeContract( const double *xp , const double *yp , double *z , std::integer_sequence< int , ps... > )
+template< int Dim , typename Real , int ... ps >
+__device__ __forceinline__ void WedgeContract( const Real *xp , const Real *yp , Real *z , std::integer_sequence< int , ps... > )
 {
 	( [&]{
 		constexpr int k = WTab< Dim >::K( ps ) , l = WTab< Dim >::L( ps );
-		const double w = xp[k]*yp[l] - xp[l]*yp[k];
+		const Real w = xp[k]*yp[l] - xp[l]*yp[k];
 		WedgeColumn< Dim , ps >( z , w , std::make_integer_sequence< int , WTab< Dim >::C-1 >{} );
 	}() , ... );
 }
 // u[j] = sum_k A(j,k) yp[k+1], v[j] = sum_k A(j,k) xp[k+1]
-template< int Dim , int j , int ... ks >
-__device__ __forceinline__ void QuadRow( const double *xp , const double *yp , double &u , double &v , std::integer_sequence< int , ks... > )
+template< int Dim , int j , typename Real , int ... ks >
+__device__ __forceinline__ void QuadRow( const Real *xp , const Real *yp , Real &u , Real &v , std::integer_sequence< int , ks... > )
 {
-	( [&]{ if constexpr( WTab< Dim >::A( j , ks )!=0.0 ){ u = fma( WTab< Dim >::A( j , ks ) , yp[ks+1] , u ) ; v = fma( WTab< Dim >::A( j , ks ) , xp[ks+1] , v ); } }() , ... );
+	( [&]{ if constexpr( WTab< Dim >::A( j , ks )!=0.0 ){ u = fma( (Real)WTab< Dim >::A( j , ks ) , yp[ks+1] , u ) ; v = fma( (Real)WTab< Dim >::A( j , ks ) , xp[ks+1] , v ); } }() , ... );
 }
-template< int Dim , int ... js >
-__device__ __forceinline__ void QuadForms( const double *xp , const double *yp , const double *z , double &a20 , double &a02 , double &a11 , double &b10 , double &b01 , std::integer_sequence< int , js... > )
+template< int Dim , typename Real , int ... js >
+__device__ __forceinline__ void QuadForms( const Real *xp , const Real *yp , const Real *z , Real &a20 , Real &a02 , Real &a11 , Real &b10 , Real &b01 , std::integer_sequence< int , js... > )
 {
 	( [&]{
-		double u = 0 , v = 0;
+		Real u = 0 , v = 0;
 		QuadRow< Dim , js >( xp , yp , u , v , std::make_integer_sequence< int , WTab< Dim >::C-1 >{} );
 		a20 = fma( yp[js+1] , u , a20 );
 		a02 = fma( xp[js+1] , v , a02 );
@@ -136,31 +136,32 @@ __device__ __forceinline__ void QuadForms( const double *xp , const double *yp ,
 // reflections).  xp/yp hold the relabelled corner coefficients.  Returns the
 // unscaled coefficients of E(x+s e_i, y+t e_i) restricted to the element.
 // The quadratic terms follow SystemEnergy.h:623-639, the cubic terms :616-621.
-template< int Dim >
-__device__ __forceinline__ void ElementWedge( const double *xp , const double *yp , double &q20 , double &q02 , double &q11 , double &q10 , double &q01 , double &zy )
+template< int Dim , typename Real >
+__device__ __forceinline__ void ElementWedge( const Real *xp , const Real *yp , Real &q20 , Real &q02 , Real &q11 , Real &q10 , Real &q01 , Real &zy )
 {
 	constexpr int C = 1<<Dim , P = ( C*(C-1) )/2;
-	double z[C-1];
+	Real z[C-1];
 #pragma unroll
 	for( int j=0 ; j<C-1 ; j++ ) z[j] = 0;
-	WedgeContract< Dim >( xp , yp , z , std::make_integer_sequence< int , P >{} );
-	double a20 = 0 , a02 = 0 , a11 = 0 , b10 = 0 , b01 = 0;
-	QuadForms< Dim >( xp , yp , z , a20 , a02 , a11 , b10 , b01 , std::make_integer_sequence< int , C-1 >{} );
-	q20 = a20 , q02 = a02 , q11 = -2.*a11 , q10 = 2.*b10 , q01 = -2.*b01 , zy = b10;
+	WedgeContract< Dim , Real >( xp , yp , z , std::make_integer_sequence< int , P >{} );
+	Real a20 = 0 , a02 = 0 , a11 = 0 , b10 = 0 , b01 = 0;
+	QuadForms< Dim , Real >( xp , yp , z , a20 , a02 , a11 , b10 , b01 , std::make_integer_sequence< int , C-1 >{} );
+	q20 = a20 , q02 = a02 , q11 = Real( -2 )*a11 , q10 = Real( 2 )*b10 , q01 = Real( -2 )*b01 , zy = b10;
 }
 
 // The omega=1/2 block-Newton step of ExteriorPoissonSolvers.h:173-181 with the
 // 2x2 inverse of Misha/Geometry.inl:319-331.
-__device__ __forceinline__ bool NodeStep( double q20 , double q02 , double q11 , double q10 , double q01 , double &dx , double &dy )
+template< typename Real >
+__device__ __forceinline__ bool NodeStep( Real q20 , Real q02 , Real q11 , Real q10 , Real q01 , Real &dx , Real &dy )
 {
-	const double H00 = 2*q20 , H11 = 2*q02 , H01 = q11 , H10 = q11;
-	const double det = H00*H11 - H01*H10;
-	if( !( fabs( det )>1e-20 ) ) return false;
-	const double d = 1. / det;
-	const double m00 = -( H11*d ) , m11 = -( H00*d ) , m01 = -( (-H01)*d ) , m10 = -( (-H10)*d );
-	double o0 = q10*m00 , o1 = q10*m01;
+	const Real H00 = 2*q20 , H11 = 2*q02 , H01 = q11 , H10 = q11;
+	const Real det = H00*H11 - H01*H10;
+	if( !( fabs( det )>Real( 1e-20 ) ) ) return false;
+	const Real d = Real( 1 ) / det;
+	const Real m00 = -( H11*d ) , m11 = -( H00*d ) , m01 = -( (-H01)*d ) , m10 = -( (-H10)*d );
+	Real o0 = q10*m00 , o1 = q10*m01;
 	o0 = o0 + q01*m10 , o1 = o1 + q01*m11;
-	dx = o0*0.5 , dy = o1*0.5;
+	dx = o0*Real( 0.5 ) , dy = o1*Real( 0.5 );
 	return true;
 }
 
@@ -168,18 +169,19 @@ __device__ __forceinline__ bool NodeStep( double q20 , double q02 , double q11 ,
 // then y with the new x.  The reference guards each with fabs( H(k,k)>1e-20 ),
 // the absolute value of a comparison, i.e. H(k,k)>1e-20; the update is always
 // added (zero when a guard fails).
-__device__ __forceinline__ void NodeStepSeparate( double q20 , double q02 , double q11 , double q10 , double q01 , double &dx , double &dy )
+template< typename Real >
+__device__ __forceinline__ void NodeStepSeparate( Real q20 , Real q02 , Real q11 , Real q10 , Real q01 , Real &dx , Real &dy )
 {
-	const double H00 = 2*q20 , H11 = 2*q02 , H01 = q11 , H10 = q11;
-	double p0 = 0 , p1 = 0;
-	if( H00>1e-20 ) p0 = -( ( H10 + H01 )*p1 + q10 ) / ( 2*H00 );
-	if( H11>1e-20 ) p1 = -( ( H10 + H01 )*p0 + q01 ) / ( 2*H11 );
+	const Real H00 = 2*q20 , H11 = 2*q02 , H01 = q11 , H10 = q11;
+	Real p0 = 0 , p1 = 0;
+	if( H00>Real( 1e-20 ) ) p0 = -( ( H10 + H01 )*p1 + q10 ) / ( 2*H00 );
+	if( H11>Real( 1e-20 ) ) p1 = -( ( H10 + H01 )*p0 + q01 ) / ( 2*H11 );
 	dx = p0 , dy = p1;
 }
 
 // The step a level view selects: paired (false: leave the node) or separate.
-template< int Dim >
-__device__ __forceinline__ bool LevelStep( const LevelView< Dim > &L , const double q[5] , double &dx , double &dy )
+template< int Dim , typename Real >
+__device__ __forceinline__ bool LevelStep( const LevelView< Dim , Real > &L , const Real q[5] , Real &dx , Real &dy )
 {
 	if( L.separate ){ NodeStepSeparate( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ; return true; }
 	return NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy );
@@ -188,18 +190,18 @@ __device__ __forceinline__ bool LevelStep( const LevelView< Dim > &L , const dou
 // Processes node i (multi-index F) with the C lanes of its group: lane = one
 // incident element (wedge terms) + a share of the 3^Dim window (R, K, B terms).
 // All lanes return the group totals.
-template< int Dim , int MODE , bool PREFETCH = true >
-__device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int *F , long long i , long long col , int lane , double acc[5] )
+template< int Dim , int MODE , bool PREFETCH = true , typename Real = double >
+__device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , const int *F , long long i , long long col , int lane , Real acc[5] )
 {
 	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W;
-	double q20 = 0 , q02 = 0 , q11 = 0 , q10 = 0 , q01 = 0;
-	double eW = 0 , eB = 0 , eR = 0 , eK = 0;
+	Real q20 = 0 , q02 = 0 , q11 = 0 , q10 = 0 , q01 = 0;
+	Real eW = 0 , eB = 0 , eR = 0 , eK = 0;
 	// ---- the lane's share of the 3^Dim window: issue its loads first
 	constexpr int NWL = ( W + C - 1 ) / C;
 	int cse = 0;
 #pragma unroll
 	for( int d=0 ; d<Dim ; d++ ) cse = cse*3 + ( F[d]<=0 ? 0 : ( F[d]>=L.res ? 2 : 1 ) );
-	double wx[NWL] , wy[NWL] , wR[NWL] , wB[NWL] , wK[NWL];
+	Real wx[NWL] , wy[NWL] , wR[NWL] , wB[NWL] , wK[NWL];
 #define EPR_LOAD_WINDOW( m ) \
 	{ \
 		const int w_ = lane + (m)*C; \
@@ -222,7 +224,7 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int
 #pragma unroll
 		for( int m=0 ; m<NWL ; m++ ) EPR_LOAD_WINDOW( m )
 	}
-	const double xi = L.x[i] , yi = L.y[i];
+	const Real xi = L.x[i] , yi = L.y[i];
 	// ---- the incident element of this lane
 	{
 		bool valid = true;
@@ -237,7 +239,7 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int
 		}
 		if( valid )
 		{
-			double xp[C] , yp[C];
+			Real xp[C] , yp[C];
 #pragma unroll
 			for( int c=0 ; c<C ; c++ )
 			{
@@ -246,8 +248,8 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int
 				for( int d=0 ; d<Dim ; d++ ) if( Bit< Dim >( c , d ) ) o += sig[d];
 				xp[c] = L.x[o] , yp[c] = L.y[o];
 			}
-			double a20 , a02 , a11 , a10 , a01 , zy;
-			ElementWedge< Dim >( xp , yp , a20 , a02 , a11 , a10 , a01 , zy );
+			Real a20 , a02 , a11 , a10 , a01 , zy;
+			ElementWedge< Dim , Real >( xp , yp , a20 , a02 , a11 , a10 , a01 , zy );
 			q20 = a20*L.sW , q02 = a02*L.sW , q11 = a11*L.sW , q10 = a10*L.sW , q01 = a01*L.sW;
 			if constexpr( MODE==MODE_ENERGY ) eW = xp[0]*zy*L.sW;
 		}
@@ -257,7 +259,7 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int
 	{
 		if constexpr( !PREFETCH ) EPR_LOAD_WINDOW( m )
 		const int w = lane + m*C;
-		const double xr = wx[m] , yr = wy[m] , Rv = wR[m] , Bv = wB[m] , Kv = wK[m];
+		const Real xr = wx[m] , yr = wy[m] , Rv = wR[m] , Bv = wB[m] , Kv = wK[m];
 		if constexpr( MODE==MODE_ENERGY )
 		{
 			eR += xi*Rv*xr + yi*Rv*yr;
@@ -266,11 +268,11 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim > &L , const int
 		}
 		else
 		{
-			const double Dv = Kv*L.dW;
+			const Real Dv = Kv*L.dW;
 			if( w==W/2 ){ q20 += Rv + Dv ; q02 += Rv + Dv; }
-			q10 += 2.*Rv*xr + 2.*Dv*xr;
-			q01 += 2.*Rv*yr + 2.*Dv*yr;
-			if( w!=W/2 ){ q10 -= 2.*Bv*yr ; q01 += 2.*Bv*xr; }
+			q10 += Real( 2 )*Rv*xr + Real( 2 )*Dv*xr;
+			q01 += Real( 2 )*Rv*yr + Real( 2 )*Dv*yr;
+			if( w!=W/2 ){ q10 -= Real( 2 )*Bv*yr ; q01 += Real( 2 )*Bv*xr; }
 		}
 	}
 #undef EPR_LOAD_WINDOW
@@ -328,8 +330,8 @@ template< int Dim > __device__ __forceinline__ void Unflatten( long long i , int
 
 // One colour phase of the paired Gauss-Seidel sweep (one launch per colour);
 // block b of the phase.
-template< int Dim >
-__device__ __forceinline__ void GsPhase( const LevelView< Dim > &L , const ColourSet< Dim > &cs , long long colBase , unsigned b )
+template< int Dim , typename Real = double >
+__device__ __forceinline__ void GsPhase( const LevelView< Dim , Real > &L , const ColourSet< Dim > &cs , long long colBase , unsigned b )
 {
 	constexpr int C = DimC< Dim >::C;
 	const long long g = (long long)b*blockDim.x + threadIdx.x;
@@ -340,16 +342,16 @@ __device__ __forceinline__ void GsPhase( const LevelView< Dim > &L , const Colou
 	cs.node( k , F );
 	const long long i = FlatIndex< Dim >( F , L.stride );
 	if( L.fmask && L.fmask[i]!=1 ) return;   // the whole group leaves together
-	double q[5];
-	NodeGroup< Dim , MODE_GS , Dim==3 >( L , F , i , colBase+k , lane , q );
+	Real q[5];
+	NodeGroup< Dim , MODE_GS , Dim==3 , Real >( L , F , i , colBase+k , lane , q );
 	if( lane==0 )
 	{
-		double dx , dy;
-		if( LevelStep< Dim >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
+		Real dx , dy;
+		if( LevelStep< Dim , Real >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
 	}
 }
-template< int Dim >
-__global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim > L , ColourSet< Dim > cs , long long colBase ){ GsPhase< Dim >( L , cs , colBase , XcdBlock( blockIdx.x , gridDim.x ) ); }
+template< int Dim , typename Real = double >
+__global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim , Real > L , ColourSet< Dim > cs , long long colBase ){ GsPhase< Dim , Real >( L , cs , colBase , XcdBlock( blockIdx.x , gridDim.x ) ); }
 
 // One colour phase over a list of nodes (the active nodes of one colour of an
 // adapted level, dense indices in increasing order); block b of the phase.

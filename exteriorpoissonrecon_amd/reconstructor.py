@@ -90,6 +90,13 @@ class Reconstructor:
         self._check(load().epr_time_gs_sweeps(self._h, level, iters, ctypes.byref(ms), ctypes.byref(n)))
         return ms.value, n.value
 
+    def time_gauss_seidel_fp32(self, level, iters):
+        """The same iterations in FP32 (measurement): (device ms, launches); the
+        level's coefficients are left at the FP32 iterate."""
+        ms = ctypes.c_double(); n = ctypes.c_int()
+        self._check(load().epr_time_gs_sweeps_fp32(self._h, level, iters, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
+
     def prolong_add(self, level):
         self._check(load().epr_prolong_add(self._h, level))
 

@@ -54,6 +54,10 @@ typedef struct
 	                          instead of PairedGaussSeidelSolve */
 	double jitter;         /* --jitter 0: a random translation of the unit-cube samples
 	                          (ExteriorPoissonRecon.cpp:174-180) */
+	int    fp32Sweeps;     /* --fp32 (no reference counterpart, default 0): on dense levels swept by
+	                          colour launches, every iteration of a level but its last runs in
+	                          FP32 (coefficients and R, B, K rounded to float); the last runs in
+	                          FP64 from the widened iterate, and energies stay FP64 */
 } epr_params;
 
 /* Fills the reference defaults (ExteriorPoissonRecon.cpp:56-72). */
@@ -204,6 +208,14 @@ int  epr_write_density( epr_ctx *ctx , const char *path );
  * iterations at one level bracketed by HIP events on the context's stream and
  * returns the elapsed device milliseconds and the number of kernel launches. */
 int  epr_time_gs_sweeps( epr_ctx *ctx , int level , int iters , double *ms , int *launches );
+
+/* Measurement hook (no reference counterpart): the same iterations computed
+ * in FP32 -- the level's coefficients and operators rounded to float, the
+ * colour launches instantiated for float, timed like epr_time_gs_sweeps --
+ * with the float coefficients written back to the level (so epr_energies
+ * gives the FP32 iterate's FP64 energy).  Dense levels swept by colour
+ * launches only. */
+int  epr_time_gs_sweeps_fp32( epr_ctx *ctx , int level , int iters , double *ms , int *launches );
 
 /* ---- Simplicial extraction (the reference's Extract, Extract/Extract.cpp:161-208) ----
  * A mesh handle holds vertices (nVertices x vertexDim doubles) and simplices
