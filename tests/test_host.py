@@ -18,7 +18,7 @@ CLI = os.path.join(ROOT, "exteriorpoissonrecon_amd", "bin", "ExteriorPoissonReco
 
 def declared():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(epr_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(epr_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_expected_surface():
