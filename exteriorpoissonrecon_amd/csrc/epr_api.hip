@@ -713,11 +713,14 @@ struct Ctx : public CtxBase
 	// wavefront per element, a 16-lane row per node, all nodes of a colour at
 	// once; KGsSmallLds / KGsSmall: a node group per 2^Dim lanes), or one launch
 	// per colour phase.
-	enum { GS_TINY , GS_SMALL_LDS , GS_SMALL , GS_PHASES };
+	enum { GS_TINY , GS_SMALL_LDS , GS_SMALL , GS_PHASES , GS_CELL };
 	struct GsPlan { int kind , threads , rb ; size_t lds; };
-	GsPlan gsPlan( int l ) const
+	GsPlan gsPlan( int l , bool cell=true ) const
 	{
 		const long long N = levels[l].N;
+		// 3D only: in 4D each row lane's 120 cubic-table coefficients do not fit
+		// in registers (512 VGPRs and 1.7 K spills measured)
+		if( Dim==3 && cell && levels[l].res==1 && !getenv( "EPR_NO_CELL" ) ) return { GS_CELL , 64 , 0 , 0 };
 		long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= levels[l].res;
 		long long maxColour = 0;
 		for( int c=0 ; c<C ; c++ ){ ColourSet< Dim > cs ; cs.set( levels[l].res , c ) ; maxColour = std::max( maxColour , cs.total ); }
@@ -744,7 +747,8 @@ struct Ctx : public CtxBase
 		if( iters<=0 ) return;
 		LevelView< Dim > v = view( l );
 		const GsPlan pl = gsPlan( l );
-		if( pl.kind==GS_TINY )
+		if( pl.kind==GS_CELL ){ if constexpr( Dim==3 ) KGsCell< Dim ><<< 1 , 64 , 0 , stream >>>( v , iters ); }
+		else if( pl.kind==GS_TINY )
 		{
 			if( pl.rb ) KGsTiny< Dim , true  ><<< 1 , pl.threads , pl.lds , stream >>>( v , iters );
 			else        KGsTiny< Dim , false ><<< 1 , pl.threads , pl.lds , stream >>>( v , iters );
@@ -1680,7 +1684,8 @@ void SolveBatchGroup( const std::vector< Ctx< Dim > * > &cs , double *energies )
 		const LevelView< Dim > L0 = cs[rAny]->view( d );
 		if( S.all.count )
 		{
-			if( S.plan.kind==Cx::GS_TINY )
+			if( S.plan.kind==Cx::GS_CELL ){ if constexpr( Dim==3 ) KGsCellBatch< Dim ><<< (unsigned)S.all.count , 64 , 0 , st >>>( L0 , (const BatchLevel *)at( S.all ) ); }
+			else if( S.plan.kind==Cx::GS_TINY )
 			{
 				if( S.plan.rb ) KGsTinyBatch< Dim , true ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) );
 				else KGsTinyBatch< Dim , false ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) );

@@ -35,6 +35,13 @@ __global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTinyBatch( Level
 	const BatchLevel &b = P[ blockIdx.x ];
 	GsTiny< Dim , RB >( BatchView( L0 , b ) , b.iters );
 }
+// one wavefront per reconstruction, all of its iterations (KGsCell: resolution-1 levels)
+template< int Dim >
+__global__ void __launch_bounds__( 64 ) KGsCellBatch( LevelView< Dim > L0 , const BatchLevel *P )
+{
+	const BatchLevel &b = P[ blockIdx.x ];
+	GsCell< Dim >( BatchView( L0 , b ) , b.iters );
+}
 // one workgroup per reconstruction, all of its iterations (KGsSmallLds)
 template< int Dim >
 __global__ void __launch_bounds__( Dim==3 ? 1024 : 512 ) KGsSmallLdsBatch( LevelView< Dim > L0 , const BatchLevel *P , int rbInLds )

@@ -1811,3 +1811,128 @@ __global__ void __launch_bounds__( 256 ) KMeanMeasure( const double *partials , 
 	out[0] = sWeight , out[1] = s / (double)n;
 }
 } // namespace epr
+
+namespace epr
+{
+// ---- One-element levels (resolution 1: 2^Dim nodes, each the corner of the
+// single element with corner index = node index, so colour c holds one node).
+// One wavefront, no LDS and no barriers.  Every lane keeps all 2^Dim (x, y)
+// in registers; lane k < 2^Dim also keeps node k's own pair.  A colour phase,
+// for its node (corner b, a compile-time constant of the unrolled colour loop):
+//   * every lane forms the corner-pair wedges of the element relabelled
+//     c -> c ^ b (register renaming, no data movement);
+//   * row lane j < 2^Dim - 1 contracts row j of the cubic table (its row in
+//     registers) and the quadratic table against them and fetches its corner
+//     value (j+1)^b from node lane (j+1)^b with ds_bpermute;
+//   * node lane k adds its R + dW K and B window entries of node b;
+//   * one DPP row reduction of the five coefficients, read to every lane,
+//     the step, and every lane updates its copies.
+template< int Dim > __host__ __device__ constexpr int CellCorner( int colour ){ int b = 0 ; for( int d=0 ; d<Dim ; d++ ) if( ( colour>>d ) & 1 ) b |= 1<<( Dim-1-d ) ; return b; }
+
+__device__ __forceinline__ double BPermute( int addr , double v )
+{
+	const unsigned long long u = (unsigned long long)__double_as_longlong( v );
+	const int lo = __builtin_amdgcn_ds_bpermute( addr , (int)(unsigned)u ) , hi = __builtin_amdgcn_ds_bpermute( addr , (int)(unsigned)( u>>32 ) );
+	return __longlong_as_double( (long long)( ( (unsigned long long)(unsigned)hi<<32 ) | (unsigned)lo ) );
+}
+__device__ __forceinline__ double FirstLane( double v )
+{
+	const unsigned long long u = (unsigned long long)__double_as_longlong( v );
+	const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane( (int)(unsigned)u ) , hi = (unsigned)__builtin_amdgcn_readfirstlane( (int)(unsigned)( u>>32 ) );
+	return __longlong_as_double( (long long)( ( (unsigned long long)hi<<32 ) | lo ) );
+}
+
+template< int Dim , int B , int ... ps >
+__device__ __forceinline__ double CellContract( const double *x , const double *y , const double *Tl , std::integer_sequence< int , ps... > )
+{
+	double z0 = 0 , z1 = 0;
+	( [&]{
+		constexpr int k = WTab< Dim >::K( ps ) ^ B , l = WTab< Dim >::L( ps ) ^ B;
+		const double w = x[k]*y[l] - x[l]*y[k];
+		if constexpr( ps & 1 ) z1 = fma( Tl[ps] , w , z1 ); else z0 = fma( Tl[ps] , w , z0 );
+	}() , ... );
+	return z0 + z1;
+}
+template< int Dim , int B , int ... ks >
+__device__ __forceinline__ void CellQuad( const double *x , const double *y , const double *Al , double &u , double &v , std::integer_sequence< int , ks... > )
+{
+	( [&]{ u = fma( Al[ks] , y[ (ks+1)^B ] , u ) ; v = fma( Al[ks] , x[ (ks+1)^B ] , v ); }() , ... );
+}
+
+template< int Dim , int COL >
+__device__ __forceinline__ void CellPhase( const LevelView< Dim > &L , double *x , double *y , double &xl , double &yl , const double *Tl , const double *Al , const double *RKl , const double *Bl , int lane , bool active )
+{
+	constexpr int C = 1<<Dim , J = C-1 , P = ( C*(C-1) )/2;
+	constexpr int Bc = CellCorner< Dim >( COL );
+	// row lane j's corner (j+1)^b from its node lane; other lanes fetch their own
+	const int src = lane<J ? ( ( lane+1 ) ^ Bc ) : lane;
+	const double xq = BPermute( src*4 , xl ) , yq = BPermute( src*4 , yl );
+	const double xj = lane<J ? xq : 0. , yj = lane<J ? yq : 0.;
+	const double z = CellContract< Dim , Bc >( x , y , Tl , std::make_integer_sequence< int , P >{} );
+	double u = 0 , v = 0;
+	CellQuad< Dim , Bc >( x , y , Al , u , v , std::make_integer_sequence< int , J >{} );
+	double q20 = yj*u , q02 = xj*v , q11 = -2.*( xj*u ) , q10 = 2.*( yj*z ) , q01 = -2.*( xj*z );
+	q20 *= L.sW , q02 *= L.sW , q11 *= L.sW , q10 *= L.sW , q01 *= L.sW;
+	const double rk = RKl[Bc] , bb = Bl[Bc];
+	q10 += 2.*rk*xl , q01 += 2.*rk*yl;
+	q10 -= 2.*bb*yl , q01 += 2.*bb*xl;
+	if( lane==Bc ){ q20 += rk ; q02 += rk; }
+	q20 = FirstLane( RowSum( q20 ) ) , q02 = FirstLane( RowSum( q02 ) ) , q11 = FirstLane( RowSum( q11 ) ) , q10 = FirstLane( RowSum( q10 ) ) , q01 = FirstLane( RowSum( q01 ) );
+	double dx , dy;
+	bool step;
+	if( L.separate ){ NodeStepSeparate( q20 , q02 , q11 , q10 , q01 , dx , dy ) ; step = true; }
+	else step = NodeStepFast( q20 , q02 , q11 , q10 , q01 , dx , dy );
+	if( step && active )
+	{
+		x[Bc] += dx , y[Bc] += dy;
+		if( lane==Bc ) xl += dx , yl += dy;
+	}
+}
+template< int Dim , int ... cs >
+__device__ __forceinline__ void CellIteration( const LevelView< Dim > &L , double *x , double *y , double &xl , double &yl , const double *Tl , const double *Al , const double *RKl , const double *Bl , int lane , const bool *active , std::integer_sequence< int , cs... > )
+{
+	( CellPhase< Dim , cs >( L , x , y , xl , yl , Tl , Al , RKl , Bl , lane , active[ CellCorner< Dim >( cs ) ] ) , ... );
+}
+
+// Requires res = 1; one workgroup of one wavefront.
+template< int Dim >
+__device__ __forceinline__ void GsCell( const LevelView< Dim > &L , int iters )
+{
+	constexpr int C = 1<<Dim , J = C-1 , P = ( C*(C-1) )/2 , W = DimC< Dim >::W;
+	const int lane = threadIdx.x;
+	double x[C] , y[C];
+#pragma unroll
+	for( int c=0 ; c<C ; c++ ) x[c] = L.x[c] , y[c] = L.y[c];
+	const int nl = lane<C ? lane : 0 , rl = lane<J ? lane : 0;
+	double xl = lane<C ? L.x[nl] : 0. , yl = lane<C ? L.y[nl] : 0.;
+	double Tl[P] , Al[J];
+#pragma unroll
+	for( int p=0 ; p<P ; p++ ) Tl[p] = lane<J ? WT< Dim >( rl , p ) : 0.;
+#pragma unroll
+	for( int k=0 ; k<J ; k++ ) Al[k] = lane<J ? WA< Dim >( rl , k ) : 0.;
+	// node lane k: the window entries of every node b at neighbour k
+	double RKl[C] , Bl[C];
+	bool active[C];
+#pragma unroll
+	for( int b=0 ; b<C ; b++ )
+	{
+		int F[Dim];
+		int cse = 0 , w = 0;
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			F[d] = Bit< Dim >( b , d );
+			cse = cse*3 + ( F[d]==0 ? 0 : 2 );            // a resolution-1 node is at the low (0) or high (2) boundary of every axis
+			w = w*3 + ( Bit< Dim >( nl , d ) - F[d] + 1 );
+		}
+		const long long col = L.lay.col( F );
+		RKl[b] = lane<C ? L.R[ col*W + w ] + L.K[ cse*W + w ]*L.dW : 0.;
+		Bl[b] = ( lane<C && lane!=b ) ? L.B[ col*W + w ] : 0.;
+		active[b] = !L.fmask || L.fmask[b]==1;
+	}
+	for( int it=0 ; it<iters ; it++ ) CellIteration< Dim >( L , x , y , xl , yl , Tl , Al , RKl , Bl , lane , active , std::make_integer_sequence< int , C >{} );
+	if( lane<C ) L.x[lane] = xl , L.y[lane] = yl;
+}
+template< int Dim >
+__global__ void __launch_bounds__( 64 ) KGsCell( LevelView< Dim > L , int iters ){ GsCell< Dim >( L , iters ); }
+} // namespace epr
