@@ -741,6 +741,14 @@ struct Ctx : public CtxBase
 		}
 		return { GS_PHASES , 256 , 0 , 0 };
 	}
+	// one colour launch of the sweep; 32-bit indices when every index of the
+	// level (node, column entry) is below 2^31
+	template< typename Real >
+	void launchPhase( const LevelView< Dim , Real > &v , const ColourSet< Dim > &cs , long long colBase )
+	{
+		if( v.N*W < ( 1LL<<31 ) ) KGsPhase< Dim , Real , int ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , colBase );
+		else KGsPhase< Dim , Real , long long ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , colBase );
+	}
 	void gs( int l , int iters )
 	{
 		needBuilt() ; checkLevel( l );
@@ -786,7 +794,7 @@ struct Ctx : public CtxBase
 					else
 					{
 						ColourSet< Dim > cs ; cs.set( L.res , c );
-						KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , v.lay.cstart[c] );
+						launchPhase( v , cs , v.lay.cstart[c] );
 					}
 				}
 				HIPCHECK( hipStreamEndCapture( stream , &g ) );
@@ -819,7 +827,7 @@ struct Ctx : public CtxBase
 		else
 		{
 			ColourSet< Dim > cs ; cs.set( L.res , c );
-			KGsPhase< Dim ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , v.lay.cstart[c] );
+			launchPhase( v , cs , v.lay.cstart[c] );
 		}
 		HIPCHECK( hipGetLastError() );
 		HIPCHECK( hipStreamSynchronize( stream ) );
@@ -856,7 +864,7 @@ struct Ctx : public CtxBase
 			for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
 			{
 				ColourSet< Dim > cs ; cs.set( L.res , c );
-				KGsPhase< Dim , float ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( f , cs , f.lay.cstart[c] );
+				launchPhase( f , cs , f.lay.cstart[c] );
 			}
 			HIPCHECK( hipStreamEndCapture( stream , &g ) );
 			HIPCHECK( hipGraphInstantiate( &L.graph32 , g , nullptr , nullptr , 0 ) );
@@ -897,7 +905,7 @@ struct Ctx : public CtxBase
 		for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
 		{
 			ColourSet< Dim > cs ; cs.set( L.res , c );
-			KGsPhase< Dim , float ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( f , cs , f.lay.cstart[c] );
+			launchPhase( f , cs , f.lay.cstart[c] );
 		}
 		HIPCHECK( hipEventRecord( e1 , stream ) );
 		HIPCHECK( hipEventSynchronize( e1 ) );

@@ -187,11 +187,35 @@ __device__ __forceinline__ bool LevelStep( const LevelView< Dim , Real > &L , co
 	return NodeStep( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy );
 }
 
+// v + v[lane'] for the DPP lane pattern CTRL within a row of 16 lanes; a source
+// lane outside the row or inactive contributes 0
+template< int CTRL , typename Real >
+__device__ __forceinline__ Real DppAddT( Real v )
+{
+	if constexpr( sizeof( Real )==8 ) return v + __builtin_amdgcn_update_dpp( 0.0 , v , CTRL , 0xF , 0xF , false );
+	else return v + __int_as_float( __builtin_amdgcn_update_dpp( 0 , __float_as_int( v ) , CTRL , 0xF , 0xF , false ) );
+}
+// sum over an aligned group of G = 2, 4, 8 or 16 lanes: xor 1, xor 2, then the
+// half-row and row mirrors (after the quad stage every lane of a quad holds
+// the quad's sum, so pairing lane i with 7-i equals pairing it with i^4, and
+// likewise 15-i with i^8: the same sums in the same order as an xor butterfly)
+template< int G , typename Real >
+__device__ __forceinline__ Real GroupSum( Real v )
+{
+	static_assert( G==2 || G==4 || G==8 || G==16 , "group width" );
+	v = DppAddT< 0xB1 >( v );
+	if constexpr( G>=4 ) v = DppAddT< 0x4E >( v );
+	if constexpr( G>=8 ) v = DppAddT< 0x141 >( v );
+	if constexpr( G>=16 ) v = DppAddT< 0x140 >( v );
+	return v;
+}
+
 // Processes node i (multi-index F) with the C lanes of its group: lane = one
 // incident element (wedge terms) + a share of the 3^Dim window (R, K, B terms).
 // All lanes return the group totals.
-template< int Dim , int MODE , bool PREFETCH = true , typename Real = double >
-__device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , const int *F , long long i , long long col , int lane , Real acc[5] )
+template< typename T > struct NoDeduce { using type = T; };
+template< int Dim , int MODE , bool PREFETCH = true , typename Real = double , typename Idx = long long >
+__device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , const int *F , typename NoDeduce< Idx >::type i , typename NoDeduce< Idx >::type col , int lane , Real acc[5] )
 {
 	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W;
 	Real q20 = 0 , q02 = 0 , q11 = 0 , q10 = 0 , q01 = 0;
@@ -206,7 +230,7 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , co
 	{ \
 		const int w_ = lane + (m)*C; \
 		int t_ = w_; \
-		long long r_ = i; \
+		Idx r_ = i; \
 		bool in_ = w_<W; \
 		_Pragma( "unroll" ) \
 		for( int d=Dim-1 ; d>=0 ; d-- ) \
@@ -214,7 +238,7 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , co
 			const int o_ = t_%3 - 1 ; t_ /= 3; \
 			const int g_ = F[d]+o_; \
 			in_ = in_ && g_>=0 && g_<=L.res; \
-			r_ += o_*L.stride[d]; \
+			r_ += o_*(Idx)L.stride[d]; \
 		} \
 		if( in_ ) wx[m] = L.x[r_] , wy[m] = L.y[r_] , wR[m] = L.R[ col*W+w_ ] , wB[m] = L.B[ col*W+w_ ] , wK[m] = L.K[ cse*W + w_ ]; \
 		else      wx[m] = wy[m] = wR[m] = wB[m] = wK[m] = 0; \
@@ -228,14 +252,14 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , co
 	// ---- the incident element of this lane
 	{
 		bool valid = true;
-		long long sig[Dim];
+		Idx sig[Dim];
 #pragma unroll
 		for( int d=0 ; d<Dim ; d++ )
 		{
 			const int b = Bit< Dim >( lane , d );
 			const int e = F[d] - 1 + b;
 			valid = valid && e>=0 && e<L.res;
-			sig[d] = b ? L.stride[d] : -L.stride[d];
+			sig[d] = b ? (Idx)L.stride[d] : -(Idx)L.stride[d];
 		}
 		if( valid )
 		{
@@ -243,7 +267,7 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , co
 #pragma unroll
 			for( int c=0 ; c<C ; c++ )
 			{
-				long long o = i;
+				Idx o = i;
 #pragma unroll
 				for( int d=0 ; d<Dim ; d++ ) if( Bit< Dim >( c , d ) ) o += sig[d];
 				xp[c] = L.x[o] , yp[c] = L.y[o];
@@ -277,16 +301,8 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , co
 	}
 #undef EPR_LOAD_WINDOW
 	if constexpr( MODE==MODE_ENERGY ){ q20 = eW , q02 = eB , q11 = eR , q10 = eK , q01 = 0; }
-	// ---- group reduction (xor butterfly inside the C-lane group)
-#pragma unroll
-	for( int m=1 ; m<C ; m<<=1 )
-	{
-		q20 += __shfl_xor( q20 , m , C );
-		q02 += __shfl_xor( q02 , m , C );
-		q11 += __shfl_xor( q11 , m , C );
-		q10 += __shfl_xor( q10 , m , C );
-		q01 += __shfl_xor( q01 , m , C );
-	}
+	// ---- group reduction (DPP inside the C-lane group)
+	q20 = GroupSum< C >( q20 ) , q02 = GroupSum< C >( q02 ) , q11 = GroupSum< C >( q11 ) , q10 = GroupSum< C >( q10 ) , q01 = GroupSum< C >( q01 );
 	acc[0] = q20 , acc[1] = q02 , acc[2] = q11 , acc[3] = q10 , acc[4] = q01;
 }
 
@@ -330,7 +346,7 @@ template< int Dim > __device__ __forceinline__ void Unflatten( long long i , int
 
 // One colour phase of the paired Gauss-Seidel sweep (one launch per colour);
 // block b of the phase.
-template< int Dim , typename Real = double >
+template< int Dim , typename Real = double , typename Idx = long long >
 __device__ __forceinline__ void GsPhase( const LevelView< Dim , Real > &L , const ColourSet< Dim > &cs , long long colBase , unsigned b )
 {
 	constexpr int C = DimC< Dim >::C;
@@ -340,18 +356,19 @@ __device__ __forceinline__ void GsPhase( const LevelView< Dim , Real > &L , cons
 	if( k>=cs.total ) return;
 	int F[Dim];
 	cs.node( k , F );
-	const long long i = FlatIndex< Dim >( F , L.stride );
+	const Idx i = (Idx)FlatIndex< Dim >( F , L.stride );
 	if( L.fmask && L.fmask[i]!=1 ) return;   // the whole group leaves together
 	Real q[5];
-	NodeGroup< Dim , MODE_GS , Dim==3 , Real >( L , F , i , colBase+k , lane , q );
+	NodeGroup< Dim , MODE_GS , Dim==3 , Real , Idx >( L , F , i , (Idx)( colBase+k ) , lane , q );
 	if( lane==0 )
 	{
 		Real dx , dy;
 		if( LevelStep< Dim , Real >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
 	}
 }
-template< int Dim , typename Real = double >
-__global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim , Real > L , ColourSet< Dim > cs , long long colBase ){ GsPhase< Dim , Real >( L , cs , colBase , XcdBlock( blockIdx.x , gridDim.x ) ); }
+// Idx = int when N 3^Dim < 2^31 (every index of the level fits), else long long
+template< int Dim , typename Real = double , typename Idx = long long >
+__global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim , Real > L , ColourSet< Dim > cs , long long colBase ){ GsPhase< Dim , Real , Idx >( L , cs , colBase , XcdBlock( blockIdx.x , gridDim.x ) ); }
 
 // One colour phase over a list of nodes (the active nodes of one colour of an
 // adapted level, dense indices in increasing order); block b of the phase.
