@@ -140,7 +140,7 @@ def batch_main(args, wl):
     if args.serial:
         os.environ["EPR_BATCH_SERIAL"] = "1"
     from exteriorpoissonrecon_amd import Reconstructor, build_batch, samples, solve_batch
-    threads = 1 if args.serial else 0
+    threads = 1 if args.serial else args.build_threads
     inputs = samples.batch_workload(wl["batch"])
     nsamp = sum(len(p) for p, _, _, _ in inputs)
     log(f"[bench] {args.config}: {len(inputs)} inputs, {nsamp} samples, depths "
@@ -428,6 +428,7 @@ def main():
     ap.add_argument("--serial", action="store_true",
                     help="batch workload: one build thread and the depth groups one after another (for profilers)")
     ap.add_argument("--no-unbatched", action="store_true", help="batch workload: skip the one-at-a-time comparison")
+    ap.add_argument("--build-threads", type=int, default=0, help="batch workload: host threads of epr_build_batch (0: four)")
     ap.add_argument("--extract", action="store_true",
                     help="also time the Extract pipeline on the .tree the last step writes (an 'extract' object in the JSON line)")
     args = ap.parse_args()

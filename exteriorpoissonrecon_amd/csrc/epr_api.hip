@@ -347,6 +347,7 @@ struct Ctx : public CtxBase
 		bool narrowed = false;
 		hipGraphExec_t graph32 = nullptr;
 		int graph32Iters = 0;
+		const float *graph32P[5] = { nullptr , nullptr , nullptr , nullptr , nullptr };
 		const double *graphX = nullptr , *graphR = nullptr;
 		const unsigned char *graphF = nullptr;
 		const unsigned int *graphAct = nullptr;
@@ -356,6 +357,9 @@ struct Ctx : public CtxBase
 		                              fx( std::move( o.fx ) ) , fy( std::move( o.fy ) ) , fR( std::move( o.fR ) ) , fB( std::move( o.fB ) ) , fK( std::move( o.fK ) ) , narrowed( o.narrowed ) , graph32( o.graph32 ) , graph32Iters( o.graph32Iters )
 		{
 			for( int d=0 ; d<Dim ; d++ ) stride[d] = o.stride[d];
+			for( int k=0 ; k<5 ; k++ ) graph32P[k] = o.graph32P[k];
+			graphX = o.graphX , graphR = o.graphR , graphF = o.graphF , graphAct = o.graphAct;
+			for( int c=0 ; c<=(1<<Dim) ; c++ ) graphActStart[c] = o.graphActStart[c];
 			o.graph = nullptr , o.graph32 = nullptr;
 		}
 		~Level(){ if( graph ) (void)hipGraphExecDestroy( graph ) ; if( graph32 ) (void)hipGraphExecDestroy( graph32 ); }
@@ -847,11 +851,14 @@ struct Ctx : public CtxBase
 			KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.B.p , L.fB.p , N*W );
 			KNarrow<<< Blocks( (long long)W*W , 256 ) , 256 , 0 , stream >>>( L.K.p , L.fK.p , (long long)W*W );
 			L.narrowed = true;
-			if( L.graph32 ){ (void)hipGraphExecDestroy( L.graph32 ) ; L.graph32 = nullptr; }
 		}
 		KNarrow<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.x.p , L.fx.p , N );
 		KNarrow<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.y.p , L.fy.p , N );
-		if( !L.graph32 || L.graph32Iters!=iters )
+		// the captured launches stay valid across builds while the float buffers stay put
+		const float *bufs[5] = { L.fx.p , L.fy.p , L.fR.p , L.fB.p , L.fK.p };
+		bool moved = false;
+		for( int k=0 ; k<5 ; k++ ) moved |= bufs[k]!=L.graph32P[k];
+		if( !L.graph32 || L.graph32Iters!=iters || moved )
 		{
 			if( L.graph32 ){ (void)hipGraphExecDestroy( L.graph32 ) ; L.graph32 = nullptr; }
 			const LevelView< Dim > v = view( l );
@@ -870,6 +877,7 @@ struct Ctx : public CtxBase
 			HIPCHECK( hipGraphInstantiate( &L.graph32 , g , nullptr , nullptr , 0 ) );
 			HIPCHECK( hipGraphDestroy( g ) );
 			L.graph32Iters = iters;
+			for( int k=0 ; k<5 ; k++ ) L.graph32P[k] = bufs[k];
 		}
 		HIPCHECK( hipGraphLaunch( L.graph32 , stream ) );
 		KWiden<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.fx.p , L.x.p , N );
