@@ -9,6 +9,12 @@
         two `rocprofv3 --pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs, scaled by what
         the same counters read for the KCopy calibration kernel (B bytes read and
         B bytes written per dispatch, launched by bench.py through epr_time_copy)
+    python -m exteriorpoissonrecon_amd.profsum counters <dir> --kernel NAME [--nodes N]
+        mean per-dispatch value of every counter of a `rocprofv3 --pmc` run over the
+        largest-grid dispatches of NAME; with --nodes, also per node (N nodes a launch):
+        an SQ_INSTS_* counter counts wavefront instructions, so its per-node figure is
+        scaled by the 64 lanes of a wavefront (lane operations per node; SQ_WAVES
+        becomes lanes per node)
 
 Counter values are KiB per dispatch.  The scale factor of a counter is
 (known KiB of the copy) / (counter value of the copy); the summary prints it.
@@ -115,6 +121,22 @@ def traffic(fetch_dir, write_dir, kernel, copy_bytes):
     return "\n".join(out)
 
 
+def counters(d, kernel, nodes=0):
+    names = set()
+    for f in _files(d, "counter_collection.csv"):
+        with open(f) as fh:
+            names |= {r["Counter_Name"] for r in csv.DictReader(fh)}
+    if not names:
+        raise SystemExit(f"no counter rows under {d}")
+    out = [f"{kernel}: mean per dispatch over the largest-grid dispatches" + (f" ({nodes} nodes each)" if nodes else ""),
+           "", "| counter | dispatches | grid sizes | per dispatch |" + (" x 64 lanes / node |" if nodes else ""),
+           "|---|---|---|---|" + ("---|" if nodes else "")]
+    for c in sorted(names):
+        v, n, g = _select(_counters(d, c), kernel)
+        out.append(f"| {c} | {n} | {g} | {v:.4g} |" + (f" {v * 64 / nodes:.4g} |" if nodes else ""))
+    return "\n".join(out)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -127,11 +149,17 @@ def main(argv=None):
     t.add_argument("write_dir")
     t.add_argument("--kernel", required=True)
     t.add_argument("--copy-bytes", type=int, required=True)
+    c = sub.add_parser("counters")
+    c.add_argument("dir")
+    c.add_argument("--kernel", required=True)
+    c.add_argument("--nodes", type=int, default=0)
     a = ap.parse_args(argv)
     if a.cmd == "stats":
         print(stats(a.dir, a.top))
         for k in a.kernel:
             print("\n" + finest(a.dir, k))
+    elif a.cmd == "counters":
+        print(counters(a.dir, a.kernel, a.nodes))
     else:
         print(traffic(a.fetch_dir, a.write_dir, a.kernel, a.copy_bytes))
 
