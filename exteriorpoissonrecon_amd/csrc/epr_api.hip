@@ -261,6 +261,7 @@ template< int Dim > void SetLdsAttributesDim( int bytes )
 	set( (const void *)KEstSplatLds< Dim , 3 > ) , set( (const void *)KEstSplatLds< Dim , 4 > );
 	set( (const void *)KGsTiny< Dim , true > ) , set( (const void *)KGsTiny< Dim , false > ) , set( (const void *)KGsSmallLds< Dim > );
 	set( (const void *)KGsTinyBatch< Dim , true > ) , set( (const void *)KGsTinyBatch< Dim , false > ) , set( (const void *)KGsSmallLdsBatch< Dim > );
+	if constexpr( Dim==3 ) set( (const void *)KGsBlock ) , set( (const void *)KGsBlockBatch );
 }
 inline void SetLdsAttributes( void )
 {
@@ -717,14 +718,20 @@ struct Ctx : public CtxBase
 	// wavefront per element, a 16-lane row per node, all nodes of a colour at
 	// once; KGsSmallLds / KGsSmall: a node group per 2^Dim lanes), or one launch
 	// per colour phase.
-	enum { GS_TINY , GS_SMALL_LDS , GS_SMALL , GS_PHASES , GS_CELL };
+	enum { GS_TINY , GS_SMALL_LDS , GS_SMALL , GS_PHASES , GS_WAVE , GS_BLOCK };
 	struct GsPlan { int kind , threads , rb ; size_t lds; };
 	GsPlan gsPlan( int l , bool cell=true ) const
 	{
 		const long long N = levels[l].N;
-		// 3D only: in 4D each row lane's 120 cubic-table coefficients do not fit
-		// in registers (512 VGPRs and 1.7 K spills measured)
-		if( Dim==3 && cell && levels[l].res==1 && !getenv( "EPR_NO_CELL" ) ) return { GS_CELL , 64 , 0 , 0 };
+		// one wavefront, registers: 3D levels of resolution 1 and 2 (in 4D each
+		// row lane's 120 cubic-table coefficients do not fit in registers: 512
+		// VGPRs and 1.7 K spills measured for the resolution-1 form)
+		{
+			const int r = levels[l].res;
+			if( Dim==3 && cell && r<=2 && N==(long long)( r+1 )*( r+1 )*( r+1 ) && !getenv( "EPR_NO_WAVE" ) ) return { GS_WAVE , 64 , 0 , 0 };
+			// eight wavefronts, one LDS exchange per colour: the 3D resolution-4 level
+			if( Dim==3 && cell && r==4 && N==125 && !getenv( "EPR_NO_WAVE" ) ) return { GS_BLOCK , 512 , 0 , (size_t)BlockLdsBytes };
+		}
 		long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= levels[l].res;
 		long long maxColour = 0;
 		for( int c=0 ; c<C ; c++ ){ ColourSet< Dim > cs ; cs.set( levels[l].res , c ) ; maxColour = std::max( maxColour , cs.total ); }
@@ -759,7 +766,15 @@ struct Ctx : public CtxBase
 		if( iters<=0 ) return;
 		LevelView< Dim > v = view( l );
 		const GsPlan pl = gsPlan( l );
-		if( pl.kind==GS_CELL ){ if constexpr( Dim==3 ) KGsCell< Dim ><<< 1 , 64 , 0 , stream >>>( v , iters ); }
+		if( pl.kind==GS_WAVE )
+		{
+			if constexpr( Dim==3 )
+			{
+				if( levels[l].res==1 ) KGsWave< 1 ><<< 1 , 64 , 0 , stream >>>( v , iters );
+				else KGsWave< 2 ><<< 1 , 64 , 0 , stream >>>( v , iters );
+			}
+		}
+		else if( pl.kind==GS_BLOCK ){ if constexpr( Dim==3 ) KGsBlock<<< 1 , 512 , pl.lds , stream >>>( v , iters ); }
 		else if( pl.kind==GS_TINY )
 		{
 			if( pl.rb ) KGsTiny< Dim , true  ><<< 1 , pl.threads , pl.lds , stream >>>( v , iters );
@@ -1700,7 +1715,15 @@ void SolveBatchGroup( const std::vector< Ctx< Dim > * > &cs , double *energies )
 		const LevelView< Dim > L0 = cs[rAny]->view( d );
 		if( S.all.count )
 		{
-			if( S.plan.kind==Cx::GS_CELL ){ if constexpr( Dim==3 ) KGsCellBatch< Dim ><<< (unsigned)S.all.count , 64 , 0 , st >>>( L0 , (const BatchLevel *)at( S.all ) ); }
+			if( S.plan.kind==Cx::GS_WAVE )
+			{
+				if constexpr( Dim==3 )
+				{
+					if( L0.res==1 ) KGsWaveBatch< 1 ><<< (unsigned)S.all.count , 64 , 0 , st >>>( L0 , (const BatchLevel *)at( S.all ) );
+					else KGsWaveBatch< 2 ><<< (unsigned)S.all.count , 64 , 0 , st >>>( L0 , (const BatchLevel *)at( S.all ) );
+				}
+			}
+			else if( S.plan.kind==Cx::GS_BLOCK ){ if constexpr( Dim==3 ) KGsBlockBatch<<< (unsigned)S.all.count , 512 , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) ); }
 			else if( S.plan.kind==Cx::GS_TINY )
 			{
 				if( S.plan.rb ) KGsTinyBatch< Dim , true ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) );

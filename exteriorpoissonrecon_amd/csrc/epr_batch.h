@@ -35,12 +35,18 @@ __global__ void __launch_bounds__( TinyMaxThreads< Dim >() ) KGsTinyBatch( Level
 	const BatchLevel &b = P[ blockIdx.x ];
 	GsTiny< Dim , RB >( BatchView( L0 , b ) , b.iters );
 }
-// one wavefront per reconstruction, all of its iterations (KGsCell: resolution-1 levels)
-template< int Dim >
-__global__ void __launch_bounds__( 64 ) KGsCellBatch( LevelView< Dim > L0 , const BatchLevel *P )
+// one wavefront per reconstruction, all of its iterations (KGsWave: 3D levels of resolution 1 or 2)
+template< int RES >
+__global__ void __launch_bounds__( 64 ) KGsWaveBatch( LevelView< 3 > L0 , const BatchLevel *P )
 {
 	const BatchLevel &b = P[ blockIdx.x ];
-	GsCell< Dim >( BatchView( L0 , b ) , b.iters );
+	GsWave< RES >( BatchView( L0 , b ) , b.iters );
+}
+// one workgroup of 512 threads per reconstruction (KGsBlock: the 3D resolution-4 level)
+__global__ void __launch_bounds__( 512 ) KGsBlockBatch( LevelView< 3 > L0 , const BatchLevel *P )
+{
+	const BatchLevel &b = P[ blockIdx.x ];
+	GsBlock( BatchView( L0 , b ) , b.iters );
 }
 // one workgroup per reconstruction, all of its iterations (KGsSmallLds)
 template< int Dim >

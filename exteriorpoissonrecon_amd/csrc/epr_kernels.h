@@ -188,13 +188,10 @@ __device__ __forceinline__ bool LevelStep( const LevelView< Dim , Real > &L , co
 }
 
 // v + v[lane'] for the DPP lane pattern CTRL within a row of 16 lanes; a source
-// lane outside the row or inactive contributes 0
+// lane outside the row or inactive contributes 0 (bound_ctrl: no old value to
+// initialise, and the FP32 add takes the DPP operand directly)
 template< int CTRL , typename Real >
-__device__ __forceinline__ Real DppAddT( Real v )
-{
-	if constexpr( sizeof( Real )==8 ) return v + __builtin_amdgcn_update_dpp( 0.0 , v , CTRL , 0xF , 0xF , false );
-	else return v + __int_as_float( __builtin_amdgcn_update_dpp( 0 , __float_as_int( v ) , CTRL , 0xF , 0xF , false ) );
-}
+__device__ __forceinline__ Real DppAddT( Real v ){ return v + __builtin_amdgcn_mov_dpp( v , CTRL , 0xF , 0xF , true ); }
 // sum over an aligned group of G = 2, 4, 8 or 16 lanes: xor 1, xor 2, then the
 // half-row and row mirrors (after the quad stage every lane of a quad holds
 // the quad's sum, so pairing lane i with 7-i equals pairing it with i^4, and
@@ -1533,7 +1530,7 @@ template<> struct TinyShape< 4 >{ static constexpr int NCH = 4 , QC = 30; };   /
 template< int Dim > constexpr int TinyMaxThreads( void ){ return Dim==3 ? 1024 : 512; }   // 4D: 256 VGPRs for the 60 of the table slice
 
 // v + v[lane'] for the DPP lane pattern CTRL (the whole wavefront must be active)
-template< int CTRL > __device__ __forceinline__ double DppAdd( double v ){ return v + __builtin_amdgcn_update_dpp( 0.0 , v , CTRL , 0xF , 0xF , false ); }
+template< int CTRL > __device__ __forceinline__ double DppAdd( double v ){ return v + __builtin_amdgcn_mov_dpp( v , CTRL , 0xF , 0xF , true ); }
 enum { DPP_XOR1 = 0xB1 , DPP_XOR2 = 0x4E , DPP_HALF_MIRROR = 0x141 , DPP_MIRROR = 0x140 , DPP_ROR4 = 0x124 , DPP_ROR8 = 0x128 };
 // sum over the 16 lanes of each row
 __device__ __forceinline__ double RowSum( double v ){ return DppAdd< DPP_MIRROR >( DppAdd< DPP_HALF_MIRROR >( DppAdd< DPP_XOR2 >( DppAdd< DPP_XOR1 >( v ) ) ) ); }
@@ -1831,19 +1828,32 @@ __global__ void __launch_bounds__( 256 ) KMeanMeasure( const double *partials , 
 
 namespace epr
 {
-// ---- One-element levels (resolution 1: 2^Dim nodes, each the corner of the
-// single element with corner index = node index, so colour c holds one node).
-// One wavefront, no LDS and no barriers.  Every lane keeps all 2^Dim (x, y)
-// in registers; lane k < 2^Dim also keeps node k's own pair.  A colour phase,
-// for its node (corner b, a compile-time constant of the unrolled colour loop):
-//   * every lane forms the corner-pair wedges of the element relabelled
-//     c -> c ^ b (register renaming, no data movement);
-//   * row lane j < 2^Dim - 1 contracts row j of the cubic table (its row in
-//     registers) and the quadratic table against them and fetches its corner
-//     value (j+1)^b from node lane (j+1)^b with ds_bpermute;
-//   * node lane k adds its R + dW K and B window entries of node b;
-//   * one DPP row reduction of the five coefficients, read to every lane,
-//     the step, and every lane updates its copies.
+// ---- Single-wavefront levels (3D, resolution 1 or 2: one or eight elements,
+// 8 or 27 nodes).  One wavefront, no barriers.  Lane (g, s) = (lane/8,
+// lane%8): group g is element g (at resolution 2 its coordinates are the bits
+// of g, bit 2-d for coordinate d, which is also its parity pattern e).  Every
+// lane of group g keeps in registers the element's 8 corner pairs by parity
+// pattern -- slot k holds the corner whose node has parity pattern k, corner
+// k ^ e -- and the 28 slot wedges x_a y_b - x_b y_a (a < b); lane s also keeps
+// slot s's pair.  Colour c's node in every element is the corner of parity
+// pattern B = CellCorner(c), so with that node relabelled to corner 0, corner
+// k of the relabelled element is slot k ^ B: the element stage of each colour
+// addresses registers at compile-time positions.  A colour phase:
+//   * row lane s < 7 contracts row s of the cubic table against the relabelled
+//     wedges and the quadratic table against the relabelled corners, with its
+//     own corner (s+1) ^ B fetched from lane (s+1) ^ B of the group;
+//   * lane s adds window terms s + 8m of the node (neighbour pairs and its
+//     R + dW K and B entries, from LDS) in the group that owns the node: the
+//     groups sharing colour c's node differ in the pattern bits of B, the
+//     owner has them 0, the others carry zero weights;
+//   * the five coefficients are summed over the 8 lanes (DPP), then over the
+//     groups sharing the node (lane xor 8, 16, 32 for pattern bits 0, 1, 2);
+//     every butterfly step adds two equal partial sums, so all lanes of those
+//     groups hold the same bits and take the same step;
+//   * slot B and its 7 wedges are refreshed, and the owner's lane s = B
+//     writes the node's pair to LDS for the window loads that follow.
+// The cubic and quadratic table rows and the window weights are held
+// pre-multiplied by sW and 2, powers of two for these levels.
 template< int Dim > __host__ __device__ constexpr int CellCorner( int colour ){ int b = 0 ; for( int d=0 ; d<Dim ; d++ ) if( ( colour>>d ) & 1 ) b |= 1<<( Dim-1-d ) ; return b; }
 
 __device__ __forceinline__ double BPermute( int addr , double v )
@@ -1852,104 +1862,348 @@ __device__ __forceinline__ double BPermute( int addr , double v )
 	const int lo = __builtin_amdgcn_ds_bpermute( addr , (int)(unsigned)u ) , hi = __builtin_amdgcn_ds_bpermute( addr , (int)(unsigned)( u>>32 ) );
 	return __longlong_as_double( (long long)( ( (unsigned long long)(unsigned)hi<<32 ) | (unsigned)lo ) );
 }
-__device__ __forceinline__ double FirstLane( double v )
+// sum over the 8 lanes of each half row (the whole wavefront must be active)
+__device__ __forceinline__ double OctSum( double v ){ return DppAdd< DPP_HALF_MIRROR >( DppAdd< DPP_XOR2 >( DppAdd< DPP_XOR1 >( v ) ) ); }
+// v + v[lane ^ 8 M'] over the bits M' of M (8, 16 and 32 for bits 0, 1, 2)
+template< int M > __device__ __forceinline__ double GroupSum8( double v )
 {
-	const unsigned long long u = (unsigned long long)__double_as_longlong( v );
-	const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane( (int)(unsigned)u ) , hi = (unsigned)__builtin_amdgcn_readfirstlane( (int)(unsigned)( u>>32 ) );
-	return __longlong_as_double( (long long)( ( (unsigned long long)hi<<32 ) | lo ) );
+	if constexpr( ( M & 1 )!=0 ) v = DppAdd< DPP_ROR8 >( v );
+	if constexpr( ( M & 2 )!=0 ) v = SwapAdd< 16 >( v );
+	if constexpr( ( M & 4 )!=0 ) v = SwapAdd< 32 >( v );
+	return v;
 }
+// position of slot pair (a, b), a < b, in the 28 stored wedges
+__host__ __device__ constexpr int WedgeSlot( int a , int b ){ return a*8 + b - ( ( a+1 )*( a+2 ) )/2; }
 
-template< int Dim , int B , int ... ps >
-__device__ __forceinline__ double CellContract( const double *x , const double *y , const double *Tl , std::integer_sequence< int , ps... > )
+template< int RES >
+struct WaveState
 {
-	double z0 = 0 , z1 = 0;
+	static constexpr int C = 8 , J = 7 , P = 28 , W = 27 , WM = 4 , G = RES*RES*RES , NN = ( RES+1 )*( RES+1 )*( RES+1 );
+	double X[C] , Y[C];    // the group's element, by parity pattern
+	double Wd[P];          // slot wedges
+	double T[P] , A[J];    // row s of the cubic and quadratic tables (times sW)
+	double xl , yl;        // slot s's pair
+	int nbr[C][WM];        // single wavefront: window neighbours of colour c's node (LDS index; NN: the zero pad)
+	unsigned nodeP[2] , csrcP[2];   // block, 8 bits per colour: colour c's node, and the element holding it as corner s
+	unsigned offP;                  // block, 8 bits per window slot m: flat offset of window term s + 8m, plus 64
+	int lane , s , g , own , act , myNode;
+};
+
+// z0, z1 += row terms of the relabelled wedges (even / odd pairs)
+template< int B , int ... ps >
+__device__ __forceinline__ void WaveContract( const double *Wd , const double *T , double &z0 , double &z1 , std::integer_sequence< int , ps... > )
+{
 	( [&]{
-		constexpr int k = WTab< Dim >::K( ps ) ^ B , l = WTab< Dim >::L( ps ) ^ B;
-		const double w = x[k]*y[l] - x[l]*y[k];
-		if constexpr( ps & 1 ) z1 = fma( Tl[ps] , w , z1 ); else z0 = fma( Tl[ps] , w , z0 );
+		constexpr int k = WTab< 3 >::K( ps ) ^ B , l = WTab< 3 >::L( ps ) ^ B;
+		const double w = k<l ? Wd[ WedgeSlot( k , l ) ] : -Wd[ WedgeSlot( l , k ) ];
+		if constexpr( ( ps & 1 )!=0 ) z1 = fma( T[ps] , w , z1 ); else z0 = fma( T[ps] , w , z0 );
 	}() , ... );
-	return z0 + z1;
 }
-template< int Dim , int B , int ... ks >
-__device__ __forceinline__ void CellQuad( const double *x , const double *y , const double *Al , double &u , double &v , std::integer_sequence< int , ks... > )
+// the 7 wedges of slot B
+template< int B , int ... ks >
+__device__ __forceinline__ void WaveRefresh( const double *X , const double *Y , double *Wd , std::integer_sequence< int , ks... > )
 {
-	( [&]{ u = fma( Al[ks] , y[ (ks+1)^B ] , u ) ; v = fma( Al[ks] , x[ (ks+1)^B ] , v ); }() , ... );
+	( [&]{
+		if constexpr( ks!=B )
+		{
+			constexpr int a = ks<B ? ks : B , b = ks<B ? B : ks;
+			Wd[ WedgeSlot( a , b ) ] = X[a]*Y[b] - X[b]*Y[a];
+		}
+	}() , ... );
 }
 
-template< int Dim , int COL >
-__device__ __forceinline__ void CellPhase( const LevelView< Dim > &L , double *x , double *y , double &xl , double &yl , const double *Tl , const double *Al , const double *RKl , const double *Bl , int lane , bool active )
+// The element stage of the colour of pattern B: row lane s's terms of the five
+// coefficients (q20, q02, q11, q10, q01) of the element's colour node.
+template< int B , typename St >
+__device__ __forceinline__ void WaveElement( const St &S , double *q )
 {
-	constexpr int C = 1<<Dim , J = C-1 , P = ( C*(C-1) )/2;
-	constexpr int Bc = CellCorner< Dim >( COL );
-	// row lane j's corner (j+1)^b from its node lane; other lanes fetch their own
-	const int src = lane<J ? ( ( lane+1 ) ^ Bc ) : lane;
-	const double xq = BPermute( src*4 , xl ) , yq = BPermute( src*4 , yl );
-	const double xj = lane<J ? xq : 0. , yj = lane<J ? yq : 0.;
-	const double z = CellContract< Dim , Bc >( x , y , Tl , std::make_integer_sequence< int , P >{} );
+	constexpr int J = 7 , P = 28;
+	// row lane s: its corner (s+1) ^ B of the relabelled element, from lane (s+1) ^ B of the group
+	const int src = ( S.lane & ~7 ) | ( S.s<J ? ( ( S.s+1 ) ^ B ) : S.s );
+	const double xq = BPermute( src*4 , S.xl ) , yq = BPermute( src*4 , S.yl );
+	double z0 = 0 , z1 = 0;
+	WaveContract< B >( S.Wd , S.T , z0 , z1 , std::make_integer_sequence< int , P >{} );
+	const double z = z0 + z1;
 	double u = 0 , v = 0;
-	CellQuad< Dim , Bc >( x , y , Al , u , v , std::make_integer_sequence< int , J >{} );
-	double q20 = yj*u , q02 = xj*v , q11 = -2.*( xj*u ) , q10 = 2.*( yj*z ) , q01 = -2.*( xj*z );
-	q20 *= L.sW , q02 *= L.sW , q11 *= L.sW , q10 *= L.sW , q01 *= L.sW;
-	const double rk = RKl[Bc] , bb = Bl[Bc];
-	q10 += 2.*rk*xl , q01 += 2.*rk*yl;
-	q10 -= 2.*bb*yl , q01 += 2.*bb*xl;
-	if( lane==Bc ){ q20 += rk ; q02 += rk; }
-	q20 = FirstLane( RowSum( q20 ) ) , q02 = FirstLane( RowSum( q02 ) ) , q11 = FirstLane( RowSum( q11 ) ) , q10 = FirstLane( RowSum( q10 ) ) , q01 = FirstLane( RowSum( q01 ) );
+#pragma unroll
+	for( int k=0 ; k<J ; k++ ){ u = fma( S.A[k] , S.Y[ ( k+1 )^B ] , u ) ; v = fma( S.A[k] , S.X[ ( k+1 )^B ] , v ); }
+	const double xj = S.s<J ? xq : 0. , yj = S.s<J ? yq : 0.;
+	q[0] = yj*u , q[1] = xj*v , q[2] = -2.*( xj*u ) , q[3] = 2.*( yj*z ) , q[4] = -2.*( xj*z );
+}
+// slot B moved by (dx, dy): the copies and the slot's 7 wedges
+template< int B , typename St >
+__device__ __forceinline__ void WaveUpdate( St &S , double dx , double dy )
+{
+	S.X[B] += dx , S.Y[B] += dy;
+	if( S.s==B ) S.xl += dx , S.yl += dy;
+	WaveRefresh< B >( S.X , S.Y , S.Wd , std::make_integer_sequence< int , 8 >{} );
+}
+
+template< int RES , int COL >
+__device__ __forceinline__ void WavePhase( const LevelView< 3 > &L , WaveState< RES > &S , double2 *sxy , const double2 *swin )
+{
+	using St = WaveState< RES >;
+	constexpr int J = St::J , P = St::P , WM = St::WM;
+	constexpr int B = CellCorner< 3 >( COL );
+	// window terms: weights and neighbour pairs (written by earlier phases)
+	double2 wv[WM] , nb[WM];
+#pragma unroll
+	for( int m=0 ; m<WM ; m++ ) wv[m] = swin[ ( COL*WM + m )*64 + S.lane ] , nb[m] = sxy[ S.nbr[COL][m] ];
+	double qe[5];
+	WaveElement< B >( S , qe );
+	double q20 = qe[0] , q02 = qe[1] , q11 = qe[2] , q10 = qe[3] , q01 = qe[4];
+#pragma unroll
+	for( int m=0 ; m<WM ; m++ )
+	{
+		q10 = fma( wv[m].x , nb[m].x , q10 ) , q10 = fma( -wv[m].y , nb[m].y , q10 );
+		q01 = fma( wv[m].x , nb[m].y , q01 ) , q01 = fma( wv[m].y , nb[m].x , q01 );
+	}
+	if( S.s==5 ){ q20 += 0.5*wv[1].x ; q02 += 0.5*wv[1].x; }   // the centre term, window slot 13 = 5 + 8
+	q20 = OctSum( q20 ) , q02 = OctSum( q02 ) , q11 = OctSum( q11 ) , q10 = OctSum( q10 ) , q01 = OctSum( q01 );
+	if constexpr( RES==2 )
+		q20 = GroupSum8< B >( q20 ) , q02 = GroupSum8< B >( q02 ) , q11 = GroupSum8< B >( q11 ) , q10 = GroupSum8< B >( q10 ) , q01 = GroupSum8< B >( q01 );
 	double dx , dy;
 	bool step;
 	if( L.separate ){ NodeStepSeparate( q20 , q02 , q11 , q10 , q01 , dx , dy ) ; step = true; }
 	else step = NodeStepFast( q20 , q02 , q11 , q10 , q01 , dx , dy );
-	if( step && active )
+	if( step && ( ( S.act>>COL ) & 1 ) )
 	{
-		x[Bc] += dx , y[Bc] += dy;
-		if( lane==Bc ) xl += dx , yl += dy;
+		WaveUpdate< B >( S , dx , dy );
+		if( S.s==B && ( ( S.own>>COL ) & 1 ) ) sxy[ S.myNode ] = make_double2( S.xl , S.yl );
 	}
+	WaveSync();
 }
-template< int Dim , int ... cs >
-__device__ __forceinline__ void CellIteration( const LevelView< Dim > &L , double *x , double *y , double &xl , double &yl , const double *Tl , const double *Al , const double *RKl , const double *Bl , int lane , const bool *active , std::integer_sequence< int , cs... > )
+template< int RES , int ... cs >
+__device__ __forceinline__ void WaveIteration( const LevelView< 3 > &L , WaveState< RES > &S , double2 *sxy , const double2 *swin , std::integer_sequence< int , cs... > )
 {
-	( CellPhase< Dim , cs >( L , x , y , xl , yl , Tl , Al , RKl , Bl , lane , active[ CellCorner< Dim >( cs ) ] ) , ... );
+	( WavePhase< RES , cs >( L , S , sxy , swin ) , ... );
 }
 
-// Requires res = 1; one workgroup of one wavefront.
-template< int Dim >
-__device__ __forceinline__ void GsCell( const LevelView< Dim > &L , int iters )
+// Requires a dense 3D level of resolution RES (1 or 2); one workgroup of one wavefront.
+template< int RES >
+__device__ __forceinline__ void GsWave( const LevelView< 3 > &L , int iters )
 {
-	constexpr int C = 1<<Dim , J = C-1 , P = ( C*(C-1) )/2 , W = DimC< Dim >::W;
-	const int lane = threadIdx.x;
-	double x[C] , y[C];
+	using St = WaveState< RES >;
+	constexpr int Dim = 3 , C = St::C , J = St::J , P = St::P , W = St::W , WM = St::WM , G = St::G , NN = St::NN;
+	__shared__ double2 sxy[ NN+1 ];
+	__shared__ double2 swin[ C*WM*64 ];
+	St S;
+	S.lane = threadIdx.x , S.s = S.lane & 7 , S.g = S.lane>>3;
+	const int g = S.g , e = g<G ? g : 0;
+	const bool live = g<G;
+	int E[Dim];
 #pragma unroll
-	for( int c=0 ; c<C ; c++ ) x[c] = L.x[c] , y[c] = L.y[c];
-	const int nl = lane<C ? lane : 0 , rl = lane<J ? lane : 0;
-	double xl = lane<C ? L.x[nl] : 0. , yl = lane<C ? L.y[nl] : 0.;
-	double Tl[P] , Al[J];
+	for( int d=0 ; d<Dim ; d++ ) E[d] = Bit< Dim >( e , d );
+	auto flat = [&]( const int *F ){ int i = 0 ; for( int d=0 ; d<Dim ; d++ ) i += F[d]*(int)L.stride[d] ; return i; };
+	auto slotNode = [&]( int k , int *F ){ for( int d=0 ; d<Dim ; d++ ) F[d] = E[d] + Bit< Dim >( k ^ e , d ) ; return flat( F ); };
+	for( int i=S.lane ; i<=NN ; i+=64 ) sxy[i] = i<NN ? make_double2( L.x[i] , L.y[i] ) : make_double2( 0. , 0. );
 #pragma unroll
-	for( int p=0 ; p<P ; p++ ) Tl[p] = lane<J ? WT< Dim >( rl , p ) : 0.;
-#pragma unroll
-	for( int k=0 ; k<J ; k++ ) Al[k] = lane<J ? WA< Dim >( rl , k ) : 0.;
-	// node lane k: the window entries of every node b at neighbour k
-	double RKl[C] , Bl[C];
-	bool active[C];
-#pragma unroll
-	for( int b=0 ; b<C ; b++ )
+	for( int k=0 ; k<C ; k++ ){ int F[Dim] ; const int i = slotNode( k , F ) ; S.X[k] = live ? L.x[i] : 0. , S.Y[k] = live ? L.y[i] : 0.; }
 	{
 		int F[Dim];
-		int cse = 0 , w = 0;
+		S.myNode = slotNode( S.s , F );
+		S.xl = live ? L.x[ S.myNode ] : 0. , S.yl = live ? L.y[ S.myNode ] : 0.;
+	}
 #pragma unroll
+	for( int a=0 ; a<C ; a++ )
+#pragma unroll
+		for( int b=a+1 ; b<C ; b++ ) S.Wd[ WedgeSlot( a , b ) ] = S.X[a]*S.Y[b] - S.X[b]*S.Y[a];
+	const int j = S.s<J ? S.s : 0;
+	const double sW = L.sW;
+#pragma unroll
+	for( int p=0 ; p<P ; p++ ) S.T[p] = ( live && S.s<J ) ? WT< Dim >( j , p )*sW : 0.;
+#pragma unroll
+	for( int k=0 ; k<J ; k++ ) S.A[k] = ( live && S.s<J ) ? WA< Dim >( j , k )*sW : 0.;
+	// per colour: ownership, activity, window weights and neighbours of the group's node
+	S.own = 0 , S.act = 0;
+#pragma unroll
+	for( int c=0 ; c<C ; c++ )
+	{
+		const int Bc = CellCorner< Dim >( c );
+		int F[Dim];
+		const int n = slotNode( Bc , F );
+		const bool owner = live && ( e & Bc )==0;
+		if( owner ) S.own |= 1<<c;
+		if( live && ( !L.fmask || L.fmask[n]==1 ) ) S.act |= 1<<c;
+		int cse = 0;
+		for( int d=0 ; d<Dim ; d++ ) cse = cse*3 + ( F[d]==0 ? 0 : ( F[d]==RES ? 2 : 1 ) );
+		const long long col = L.lay.col( F );
+#pragma unroll
+		for( int m=0 ; m<WM ; m++ )
+		{
+			const int w = S.s + 8*m;
+			int Fn[Dim] , t = w;
+			bool ok = owner && w<W;
+			for( int d=Dim-1 ; d>=0 ; d-- ){ Fn[d] = F[d] + t%3 - 1 ; t /= 3 ; ok = ok && Fn[d]>=0 && Fn[d]<=RES; }
+			S.nbr[c][m] = ok ? flat( Fn ) : NN;
+			double r = 0 , b = 0;
+			if( ok )
+			{
+				r = 2.*( L.R[ col*W + w ] + L.K[ cse*W + w ]*L.dW );
+				b = w==W/2 ? 0. : 2.*L.B[ col*W + w ];
+			}
+			swin[ ( c*WM + m )*64 + S.lane ] = make_double2( r , b );
+		}
+	}
+	__syncthreads();
+	for( int it=0 ; it<iters ; it++ ) WaveIteration< RES >( L , S , sxy , swin , std::make_integer_sequence< int , C >{} );
+	__syncthreads();
+	for( int i=S.lane ; i<NN ; i+=64 ){ const double2 q = sxy[i] ; L.x[i] = q.x , L.y[i] = q.y; }
+}
+template< int RES >
+__global__ void __launch_bounds__( 64 ) KGsWave( LevelView< 3 > L , int iters ){ GsWave< RES >( L , iters ); }
+// ---- The 3D resolution-4 level (64 elements, 125 nodes) in one workgroup of
+// eight wavefronts: the element stage of the single-wavefront form (group g =
+// element g, coordinates the base-4 digits of g), then one exchange through
+// LDS per colour phase.  Lane 0 of each group writes the element's five
+// coefficients (double-buffered by phase parity); after the barrier every
+// group sharing colour c's node gathers them -- lane s the element holding
+// the node as corner s -- adds the node's window terms (weights per node and
+// window slot in LDS, neighbour pairs from LDS, the node's own pair from its
+// registers), sums over its 8 lanes and takes the step itself, so all of them
+// hold the same bits without a second exchange.  Only colour c's nodes change
+// in phase c and no window of a colour-c node holds another colour-c node, so
+// the owner's LDS write of the new pair races with no read.
+template< int COL >
+__device__ __forceinline__ void BlockPhase( const LevelView< 3 > &L , WaveState< 4 > &S , double2 *sxy , const double2 *swt , double *contrib )
+{
+	constexpr int B = CellCorner< 3 >( COL ) , WM = 4 , NC = 64 , NN = 125;
+	constexpr int par = COL & 1;
+	double q[5];
+	WaveElement< B >( S , q );
+#pragma unroll
+	for( int k=0 ; k<5 ; k++ ) q[k] = OctSum( q[k] );
+	if( S.s==0 )
+	{
+		double *cw = contrib + ( par*( NC+1 ) + S.g )*6;
+		*(double2 *)cw = make_double2( q[0] , q[1] ) , *(double2 *)( cw+2 ) = make_double2( q[2] , q[3] ) , cw[4] = q[4];
+	}
+	__syncthreads();
+	const int n = ( S.nodeP[COL>>2]>>( 8*( COL&3 ) ) ) & 0xFF , src = ( S.csrcP[COL>>2]>>( 8*( COL&3 ) ) ) & 0xFF;
+	const double *cr = contrib + ( par*( NC+1 ) + src )*6;
+	const double2 c01 = *(const double2 *)cr , c23 = *(const double2 *)( cr+2 );
+	q[0] = c01.x , q[1] = c01.y , q[2] = c23.x , q[3] = c23.y , q[4] = cr[4];
+	double2 wv[WM] , nb[WM];
+#pragma unroll
+	for( int m=0 ; m<WM ; m++ ) wv[m] = swt[ n*32 + S.s + 8*m ] , nb[m] = sxy[ min( max( n + (int)( ( S.offP>>( 8*m ) ) & 0xFF ) - 64 , 0 ) , NN-1 ) ];
+	if( S.s==5 ) nb[1] = make_double2( S.X[B] , S.Y[B] );   // the node itself: window slot 13
+#pragma unroll
+	for( int m=0 ; m<WM ; m++ )
+	{
+		q[3] = fma( wv[m].x , nb[m].x , q[3] ) , q[3] = fma( -wv[m].y , nb[m].y , q[3] );
+		q[4] = fma( wv[m].x , nb[m].y , q[4] ) , q[4] = fma( wv[m].y , nb[m].x , q[4] );
+	}
+	if( S.s==5 ){ q[0] += 0.5*wv[1].x ; q[1] += 0.5*wv[1].x; }
+#pragma unroll
+	for( int k=0 ; k<5 ; k++ ) q[k] = OctSum( q[k] );
+	double dx , dy;
+	bool step;
+	if( L.separate ){ NodeStepSeparate( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy ) ; step = true; }
+	else step = NodeStepFast( q[0] , q[1] , q[2] , q[3] , q[4] , dx , dy );
+	if( step && ( ( S.act>>COL ) & 1 ) )
+	{
+		WaveUpdate< B >( S , dx , dy );
+		if( S.s==B && ( ( S.own>>COL ) & 1 ) ) sxy[n] = make_double2( S.xl , S.yl );
+	}
+}
+template< int ... cs >
+__device__ __forceinline__ void BlockIteration( const LevelView< 3 > &L , WaveState< 4 > &S , double2 *sxy , const double2 *swt , double *contrib , std::integer_sequence< int , cs... > )
+{
+	( BlockPhase< cs >( L , S , sxy , swt , contrib ) , ... );
+}
+// dynamic LDS of GsBlock: the pairs (+ pad), window weights (32 slots per
+// node), contributions (2 parities x 65 rows x 6 doubles)
+constexpr int BlockLdsBytes = ( 126 + 125*32 )*16 + 2*65*6*8;
+
+// Requires a dense 3D level of resolution 4; one workgroup of 512 threads and BlockLdsBytes of dynamic LDS.
+__device__ __forceinline__ void GsBlock( const LevelView< 3 > &L , int iters )
+{
+	using St = WaveState< 4 >;
+	constexpr int Dim = 3 , RES = 4 , C = 8 , J = 7 , P = 28 , W = 27 , WM = 4 , NN = 125 , NC = 64;
+	extern __shared__ double2 smb[];
+	double2 *sxy = smb , *swt = smb + 126;
+	double *contrib = (double *)( swt + NN*32 );
+	St S;
+	S.lane = threadIdx.x & 63 , S.s = threadIdx.x & 7 , S.g = threadIdx.x>>3;
+	const int g = S.g;
+	int E[Dim];
+	E[0] = g>>4 , E[1] = ( g>>2 ) & 3 , E[2] = g & 3;
+	int e = 0;
+	for( int d=0 ; d<Dim ; d++ ) e |= ( E[d] & 1 )<<( Dim-1-d );
+	auto flat = [&]( const int *F ){ int i = 0 ; for( int d=0 ; d<Dim ; d++ ) i += F[d]*(int)L.stride[d] ; return i; };
+	auto slotNode = [&]( int k , int *F ){ for( int d=0 ; d<Dim ; d++ ) F[d] = E[d] + Bit< Dim >( k ^ e , d ) ; return flat( F ); };
+	// LDS: pairs, zeroed contributions, window weights by node
+	for( int i=threadIdx.x ; i<=NN ; i+=blockDim.x ) sxy[i] = i<NN ? make_double2( L.x[i] , L.y[i] ) : make_double2( 0. , 0. );
+	for( int i=threadIdx.x ; i<2*( NC+1 )*6 ; i+=blockDim.x ) contrib[i] = 0.;
+	for( int i=threadIdx.x ; i<NN*32 ; i+=blockDim.x )
+	{
+		const int n = i>>5 , w = i & 31;
+		int F[Dim] , Fn[Dim] , t = n , u = w;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = t % ( RES+1 ) ; t /= RES+1; }
+		bool ok = w<W;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ Fn[d] = F[d] + u%3 - 1 ; u /= 3 ; ok = ok && Fn[d]>=0 && Fn[d]<=RES; }
+		double r = 0 , b = 0;
+		if( ok )
+		{
+			int cse = 0;
+			for( int d=0 ; d<Dim ; d++ ) cse = cse*3 + ( F[d]==0 ? 0 : ( F[d]==RES ? 2 : 1 ) );
+			const long long col = L.lay.col( F );
+			r = 2.*( L.R[ col*W + w ] + L.K[ cse*W + w ]*L.dW );
+			b = w==W/2 ? 0. : 2.*L.B[ col*W + w ];
+		}
+		swt[i] = make_double2( r , b );
+	}
+	// registers: the element's corners by parity pattern, wedges, table rows
+#pragma unroll
+	for( int k=0 ; k<C ; k++ ){ int F[Dim] ; const int i = slotNode( k , F ) ; S.X[k] = L.x[i] , S.Y[k] = L.y[i]; }
+	{
+		int F[Dim];
+		S.myNode = slotNode( S.s , F );
+		S.xl = L.x[ S.myNode ] , S.yl = L.y[ S.myNode ];
+	}
+#pragma unroll
+	for( int a=0 ; a<C ; a++ )
+#pragma unroll
+		for( int b=a+1 ; b<C ; b++ ) S.Wd[ WedgeSlot( a , b ) ] = S.X[a]*S.Y[b] - S.X[b]*S.Y[a];
+	const int j = S.s<J ? S.s : 0;
+	const double sW = L.sW;
+#pragma unroll
+	for( int p=0 ; p<P ; p++ ) S.T[p] = S.s<J ? WT< Dim >( j , p )*sW : 0.;
+#pragma unroll
+	for( int k=0 ; k<J ; k++ ) S.A[k] = S.s<J ? WA< Dim >( j , k )*sW : 0.;
+	S.offP = 0;
+#pragma unroll
+	for( int m=0 ; m<WM ; m++ )
+	{
+		int t = S.s + 8*m , o = 0;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ o += ( t%3 - 1 )*(int)L.stride[d] ; t /= 3; }
+		S.offP |= (unsigned)( o + 64 )<<( 8*m );   // |o| <= 31 for resolution 4
+	}
+	// per colour: the group's node, ownership, activity, and the element holding the node as corner s
+	S.own = 0 , S.act = 0 , S.nodeP[0] = S.nodeP[1] = S.csrcP[0] = S.csrcP[1] = 0;
+#pragma unroll
+	for( int c=0 ; c<C ; c++ )
+	{
+		const int Bc = CellCorner< Dim >( c );
+		int F[Dim];
+		const int n = slotNode( Bc , F );
+		S.nodeP[c>>2] |= (unsigned)n<<( 8*( c&3 ) );
+		bool owner = true , in = true;
+		int Es[Dim];
 		for( int d=0 ; d<Dim ; d++ )
 		{
-			F[d] = Bit< Dim >( b , d );
-			cse = cse*3 + ( F[d]==0 ? 0 : 2 );            // a resolution-1 node is at the low (0) or high (2) boundary of every axis
-			w = w*3 + ( Bit< Dim >( nl , d ) - F[d] + 1 );
+			owner = owner && E[d]==min( F[d] , RES-1 );
+			Es[d] = F[d] - Bit< Dim >( S.s , d );
+			in = in && Es[d]>=0 && Es[d]<RES;
 		}
-		const long long col = L.lay.col( F );
-		RKl[b] = lane<C ? L.R[ col*W + w ] + L.K[ cse*W + w ]*L.dW : 0.;
-		Bl[b] = ( lane<C && lane!=b ) ? L.B[ col*W + w ] : 0.;
-		active[b] = !L.fmask || L.fmask[b]==1;
+		S.csrcP[c>>2] |= (unsigned)( in ? Es[0]*16 + Es[1]*4 + Es[2] : NC )<<( 8*( c&3 ) );
+		if( owner ) S.own |= 1<<c;
+		if( !L.fmask || L.fmask[n]==1 ) S.act |= 1<<c;
 	}
-	for( int it=0 ; it<iters ; it++ ) CellIteration< Dim >( L , x , y , xl , yl , Tl , Al , RKl , Bl , lane , active , std::make_integer_sequence< int , C >{} );
-	if( lane<C ) L.x[lane] = xl , L.y[lane] = yl;
+	__syncthreads();
+	for( int it=0 ; it<iters ; it++ ) BlockIteration( L , S , sxy , swt , contrib , std::make_integer_sequence< int , C >{} );
+	__syncthreads();
+	for( int i=threadIdx.x ; i<NN ; i+=blockDim.x ){ const double2 q = sxy[i] ; L.x[i] = q.x , L.y[i] = q.y; }
 }
-template< int Dim >
-__global__ void __launch_bounds__( 64 ) KGsCell( LevelView< Dim > L , int iters ){ GsCell< Dim >( L , iters ); }
+__global__ void __launch_bounds__( 512 ) KGsBlock( LevelView< 3 > L , int iters ){ GsBlock( L , iters ); }
 } // namespace epr
