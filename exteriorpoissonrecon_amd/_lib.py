@@ -26,6 +26,7 @@ EXPORTS = [
     "epr_unet_create", "epr_unet_destroy", "epr_unet_last_error", "epr_unet_num_params", "epr_unet_num_nodes",
     "epr_unet_set_params", "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients",
     "epr_unet_set_dataset", "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned", "epr_time_gs_sweeps_fp32",
+    "epr_autotune", "epr_autotune_batch", "epr_tune_get", "epr_tune_set",
 ]
 
 
@@ -102,6 +103,10 @@ def load():
     L.epr_build_batch.argtypes = [sz, ctypes.POINTER(vp), ci]
     L.epr_solve_batch.argtypes = [sz, ctypes.POINTER(vp), dp]
     L.epr_density_grid.argtypes = [ctypes.c_char_p, ci, ci, ctypes.c_char_p]
+    L.epr_autotune.argtypes = [vp, ci, ctypes.c_char_p, sz]
+    L.epr_autotune_batch.argtypes = [sz, ctypes.POINTER(vp), ci, ctypes.c_char_p, sz]
+    L.epr_tune_get.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(sz)]
+    L.epr_tune_set.argtypes = [ctypes.c_char_p]
     L.epr_time_gs_sweeps.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
     L.epr_time_gs_sweeps.restype = ctypes.c_int
     L.epr_time_gs_sweeps_fp32.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
@@ -159,7 +164,8 @@ def load():
                  "epr_quadratic_fits", "epr_apply", "epr_get_array", "epr_write_tree", "epr_write_grid",
                  "epr_gs_colour", "epr_energies_noalloc", "epr_unet_create", "epr_unet_set_params",
                  "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients", "epr_unet_set_dataset",
-                 "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned"]:
+                 "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned",
+                 "epr_autotune", "epr_autotune_batch", "epr_tune_get", "epr_tune_set"]:
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

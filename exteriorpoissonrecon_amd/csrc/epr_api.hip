@@ -16,11 +16,13 @@
 #include <atomic>
 #include <exception>
 #include <functional>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <set>
 #include <string>
 #include <thread>
+#include <tuple>
 #include <vector>
 
 #include "../../include/epr.h"
@@ -275,6 +277,71 @@ inline void SetLdsAttributes( void )
 	if( failed[dev] ) std::rethrow_exception( failed[dev] );
 }
 
+// ---- tuned sweep plans (epr_autotune, epr_autotune_batch)
+// Per device, dimension, level resolution, single/batched solve and precision:
+// the sweep kernel of the level (kind, one of Ctx::GS_*; -1 keeps the built-in
+// rule) and the workgroup size of its colour launches (threads: 64, 128 or
+// 256, i.e. 64/2^Dim .. 256/2^Dim nodes per workgroup; 0 keeps 256).  Entries
+// are written by the autotuners from timings on the device, or restored from
+// their text form; contexts consult the table whenever they plan a level.
+struct TunePick { int kind = -1 , threads = 0; double us = 0; };
+struct TuneTable
+{
+	using Key = std::tuple< int , int , int , int , int >;   // device, dim, res, batched, fp32
+	std::mutex m;
+	std::map< Key , TunePick > picks;
+	static TuneTable &Get( void ){ static TuneTable *t = new TuneTable ; return *t; }
+	static int Device( void ){ int d = 0 ; HIPCHECK( hipGetDevice( &d ) ) ; return d; }
+	TunePick find( int dim , int res , bool batched , bool fp32 )
+	{
+		const Key k{ Device() , dim , res , batched ? 1 : 0 , fp32 ? 1 : 0 };
+		std::lock_guard< std::mutex > lock( m );
+		auto it = picks.find( k );
+		return it==picks.end() ? TunePick() : it->second;
+	}
+	void set( int dim , int res , bool batched , bool fp32 , TunePick p )
+	{
+		const Key k{ Device() , dim , res , batched ? 1 : 0 , fp32 ? 1 : 0 };
+		std::lock_guard< std::mutex > lock( m );
+		picks[k] = p;
+	}
+	// one line per entry of the current device: "dim res batched fp32 kind threads us"
+	std::string text( void )
+	{
+		const int dev = Device();
+		std::lock_guard< std::mutex > lock( m );
+		std::string s;
+		char line[160];
+		for( const auto &e : picks ) if( std::get< 0 >( e.first )==dev )
+		{
+			snprintf( line , sizeof(line) , "%d %d %d %d %d %d %.3f\n" , std::get< 1 >( e.first ) , std::get< 2 >( e.first ) , std::get< 3 >( e.first ) , std::get< 4 >( e.first ) , e.second.kind , e.second.threads , e.second.us );
+			s += line;
+		}
+		return s;
+	}
+	// replaces the current device's entries by those of a text of text()'s form (empty: none)
+	void parse( const char *txt )
+	{
+		const int dev = Device();
+		std::map< Key , TunePick > add;
+		if( txt ) for( const char *p=txt ; *p ; )
+		{
+			const char *e = strchr( p , '\n' );
+			std::string line( p , e ? (size_t)( e-p ) : strlen( p ) );
+			p = e ? e+1 : p+line.size();
+			if( line.find_first_not_of( " \t\r" )==std::string::npos ) continue;
+			int dim , res , b , f , kind , th;
+			double us = 0;
+			if( sscanf( line.c_str() , "%d %d %d %d %d %d %lf" , &dim , &res , &b , &f , &kind , &th , &us )<6 ) throw std::string( "bad tuning line: " + line );
+			if( ( dim!=3 && dim!=4 ) || res<1 || ( b!=0 && b!=1 ) || ( f!=0 && f!=1 ) || kind<-1 || kind>5 || ( th!=0 && th!=64 && th!=128 && th!=256 ) ) throw std::string( "bad tuning line: " + line );
+			add[ Key{ dev , dim , res , b , f } ] = TunePick{ kind , th , us };
+		}
+		std::lock_guard< std::mutex > lock( m );
+		for( auto it=picks.begin() ; it!=picks.end() ; ) if( std::get< 0 >( it->first )==dev ) it = picks.erase( it ) ; else ++it;
+		for( const auto &e : add ) picks[ e.first ] = e.second;
+	}
+};
+
 struct CtxBase
 {
 	virtual ~CtxBase(){}
@@ -301,8 +368,38 @@ struct CtxBase
 	virtual void writeTree( const char *path , unsigned long long hash ) = 0;
 	virtual void writeGrid( const char *path ) = 0;
 	virtual void writeDensity( const char *path ) = 0;
+	virtual std::string autotune( int flags ) = 0;
 	double seconds[2] = { 0 , 0 };
 };
+
+// one-workgroup sweep kernels are tried on levels of at most this many nodes
+constexpr long long TuneOneGroupMax = 5000;
+// the workgroup sizes tried for colour launches
+constexpr int TuneThreads[3] = { 64 , 128 , 256 };
+// times f() on stream st in milliseconds
+template< typename F > double TimeOnStream( hipStream_t st , F f )
+{
+	hipEvent_t e0 = nullptr , e1 = nullptr;
+	HIPCHECK( hipEventCreate( &e0 ) );
+	float t = 0;
+	try
+	{
+		HIPCHECK( hipEventCreate( &e1 ) );
+		HIPCHECK( hipEventRecord( e0 , st ) );
+		f();
+		HIPCHECK( hipEventRecord( e1 , st ) );
+		HIPCHECK( hipEventSynchronize( e1 ) );
+		HIPCHECK( hipEventElapsedTime( &t , e0 , e1 ) );
+	}
+	catch( ... ){ (void)hipEventDestroy( e0 ) ; if( e1 ) (void)hipEventDestroy( e1 ) ; throw; }
+	(void)hipEventDestroy( e0 ) , (void)hipEventDestroy( e1 );
+	return t;
+}
+static const char *GsKindName( int k )
+{
+	static const char *names[] = { "tiny" , "small_lds" , "small" , "phases" , "wave" , "block" };
+	return k>=0 && k<6 ? names[k] : "default";
+}
 
 template< int Dim >
 struct Ctx : public CtxBase
@@ -341,21 +438,21 @@ struct Ctx : public CtxBase
 		ColLayout< Dim > lay;              // colour-grouped column order of R and B
 		DevBuf< double > x , y , R , B , K;
 		hipGraphExec_t graph = nullptr;    // captured Gauss-Seidel launches of this level
-		int graphIters = 0;
+		int graphIters = 0 , graphThreads = 0;
 		// --fp32: float copies of x, y, R, B, K (R, B, K narrowed once per build)
 		// and the captured float launches
 		DevBuf< float > fx , fy , fR , fB , fK;
 		bool narrowed = false;
 		hipGraphExec_t graph32 = nullptr;
-		int graph32Iters = 0;
+		int graph32Iters = 0 , graph32Threads = 0;
 		const float *graph32P[5] = { nullptr , nullptr , nullptr , nullptr , nullptr };
 		const double *graphX = nullptr , *graphR = nullptr;
 		const unsigned char *graphF = nullptr;
 		const unsigned int *graphAct = nullptr;
 		long long graphActStart[(1<<Dim)+1] = {};
 		Level( void ){}
-		Level( Level &&o ) noexcept : AdaptedLevelDev( std::move( o ) ) , res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , graph( o.graph ) , graphIters( o.graphIters ) ,
-		                              fx( std::move( o.fx ) ) , fy( std::move( o.fy ) ) , fR( std::move( o.fR ) ) , fB( std::move( o.fB ) ) , fK( std::move( o.fK ) ) , narrowed( o.narrowed ) , graph32( o.graph32 ) , graph32Iters( o.graph32Iters )
+		Level( Level &&o ) noexcept : AdaptedLevelDev( std::move( o ) ) , res( o.res ) , N( o.N ) , lay( o.lay ) , x( std::move( o.x ) ) , y( std::move( o.y ) ) , R( std::move( o.R ) ) , B( std::move( o.B ) ) , K( std::move( o.K ) ) , graph( o.graph ) , graphIters( o.graphIters ) , graphThreads( o.graphThreads ) ,
+		                              fx( std::move( o.fx ) ) , fy( std::move( o.fy ) ) , fR( std::move( o.fR ) ) , fB( std::move( o.fB ) ) , fK( std::move( o.fK ) ) , narrowed( o.narrowed ) , graph32( o.graph32 ) , graph32Iters( o.graph32Iters ) , graph32Threads( o.graph32Threads )
 		{
 			for( int d=0 ; d<Dim ; d++ ) stride[d] = o.stride[d];
 			for( int k=0 ; k<5 ; k++ ) graph32P[k] = o.graph32P[k];
@@ -687,6 +784,8 @@ struct Ctx : public CtxBase
 			if( !L.K.n ){ std::vector< double > k = hat::StiffnessCaseRows< Dim >( L.res ) ; L.K.upload( k.data() , k.size() , stream ); }   // depends on the resolution only
 			L.checkGraph();
 		}
+		// --fp32: the float copies of the operators of the levels swept by colour launches
+		if( prm.fp32Sweeps && !adapted() ) for( int l=0 ; l<=D ; l++ ) if( gsPlan( l ).kind==GS_PHASES || gsPlan( l , true ).kind==GS_PHASES ) narrowOperators( l );
 		{
 			double csum;
 			dTmp.download( &csum , 1 , stream );
@@ -720,27 +819,33 @@ struct Ctx : public CtxBase
 	// per colour phase.
 	enum { GS_TINY , GS_SMALL_LDS , GS_SMALL , GS_PHASES , GS_WAVE , GS_BLOCK };
 	struct GsPlan { int kind , threads , rb ; size_t lds; };
-	GsPlan gsPlan( int l , bool cell=true ) const
+	// the plan of level l with a given kind of sweep kernel; kind -1 when that
+	// kind cannot run the level
+	GsPlan planKind( int l , int kind ) const
 	{
 		const long long N = levels[l].N;
+		const int r = levels[l].res;
+		long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= r;
+		long long maxColour = 0;
+		for( int c=0 ; c<C ; c++ ){ ColourSet< Dim > cs ; cs.set( r , c ) ; maxColour = std::max( maxColour , cs.total ); }
+		switch( kind )
+		{
 		// one wavefront, registers: 3D levels of resolution 1 and 2 (in 4D each
 		// row lane's 120 cubic-table coefficients do not fit in registers: 512
 		// VGPRs and 1.7 K spills measured for the resolution-1 form)
+		case GS_WAVE: if( Dim==3 && r<=2 && N==(long long)( r+1 )*( r+1 )*( r+1 ) ) return { GS_WAVE , 64 , 0 , 0 }; break;
+		// eight wavefronts, one LDS exchange per colour: the 3D resolution-4 level
+		case GS_BLOCK: if( Dim==3 && r==4 && N==125 ) return { GS_BLOCK , 512 , 0 , (size_t)BlockLdsBytes }; break;
+		case GS_TINY:
 		{
-			const int r = levels[l].res;
-			if( Dim==3 && cell && r<=2 && N==(long long)( r+1 )*( r+1 )*( r+1 ) && !getenv( "EPR_NO_WAVE" ) ) return { GS_WAVE , 64 , 0 , 0 };
-			// eight wavefronts, one LDS exchange per colour: the 3D resolution-4 level
-			if( Dim==3 && cell && r==4 && N==125 && !getenv( "EPR_NO_WAVE" ) ) return { GS_BLOCK , 512 , 0 , (size_t)BlockLdsBytes };
+			const int maxWaves = TinyMaxThreads< Dim >()/64;
+			const int tinyWaves = (int)std::min< long long >( maxWaves , std::max( ne , ( maxColour+3 )/4 ) );
+			const bool tinyRB = TinyLds< Dim >::Bytes( N , ne , tinyWaves , true ) <= LdsLimit;
+			if( maxColour<=4*maxWaves && TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB )<=LdsLimit )
+				return { GS_TINY , 64*tinyWaves , tinyRB ? 1 : 0 , TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB ) };
+			break;
 		}
-		long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= levels[l].res;
-		long long maxColour = 0;
-		for( int c=0 ; c<C ; c++ ){ ColourSet< Dim > cs ; cs.set( levels[l].res , c ) ; maxColour = std::max( maxColour , cs.total ); }
-		const int maxWaves = TinyMaxThreads< Dim >()/64;
-		const int tinyWaves = (int)std::min< long long >( maxWaves , std::max( ne , ( maxColour+3 )/4 ) );
-		const bool tinyRB = TinyLds< Dim >::Bytes( N , ne , tinyWaves , true ) <= LdsLimit;
-		if( ne<=tinyElements && maxColour<=4*maxWaves && TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB )<=LdsLimit )
-			return { GS_TINY , 64*tinyWaves , tinyRB ? 1 : 0 , TinyLds< Dim >::Bytes( N , ne , tinyWaves , tinyRB ) };
-		if( N<=SmallLevel )
+		case GS_SMALL_LDS:
 		{
 			// one workgroup sized to the largest colour, coefficients in LDS (and R, B when they fit)
 			const int threads = (int)std::min< long long >( Dim==3 ? 1024 : 512 , ( maxColour*C + 63 )/64*64 );
@@ -748,17 +853,49 @@ struct Ctx : public CtxBase
 			const int rb = lds + 2*N*W*sizeof(double) <= LdsLimit ? 1 : 0;
 			if( rb ) lds += 2*N*W*sizeof(double);
 			if( lds<=LdsLimit ) return { GS_SMALL_LDS , threads , rb , lds };
-			return { GS_SMALL , 512 , 0 , 0 };
+			break;
 		}
-		return { GS_PHASES , 256 , 0 , 0 };
+		case GS_SMALL: return { GS_SMALL , 512 , 0 , 0 };
+		case GS_PHASES: return { GS_PHASES , 256 , 0 , 0 };
+		}
+		return { -1 , 0 , 0 , 0 };
 	}
-	// one colour launch of the sweep; 32-bit indices when every index of the
-	// level (node, column entry) is below 2^31
-	template< typename Real >
-	void launchPhase( const LevelView< Dim , Real > &v , const ColourSet< Dim > &cs , long long colBase )
+	// The plan of level l: the tuned kind and colour-launch workgroup (single
+	// or batched solve) where the tuning table has one, else the built-in rule
+	// (measured on MI355X): one wavefront for 3D resolutions 1 and 2, eight for
+	// 3D resolution 4, KGsTiny up to tinyElements elements, one workgroup up to
+	// SmallLevel nodes, colour launches of 256 threads above.
+	GsPlan gsPlan( int l , bool batched=false ) const
 	{
-		if( v.N*W < ( 1LL<<31 ) ) KGsPhase< Dim , Real , int ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , colBase );
-		else KGsPhase< Dim , Real , long long ><<< Blocks( cs.total*C , 256 ) , 256 , 0 , stream >>>( v , cs , colBase );
+		const TunePick t = TuneTable::Get().find( Dim , levels[l].res , batched , false );
+		if( t.kind>=0 )
+		{
+			GsPlan p = planKind( l , t.kind );
+			if( p.kind>=0 ){ if( p.kind==GS_PHASES && t.threads ) p.threads = t.threads ; return p; }
+		}
+		GsPlan p = { -1 , 0 , 0 , 0 };
+		if( !getenv( "EPR_NO_WAVE" ) ) p = planKind( l , GS_WAVE );
+		if( p.kind<0 && !getenv( "EPR_NO_WAVE" ) ) p = planKind( l , GS_BLOCK );
+		long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= levels[l].res;
+		if( p.kind<0 && ne<=tinyElements ) p = planKind( l , GS_TINY );
+		if( p.kind<0 && levels[l].N<=SmallLevel ){ p = planKind( l , GS_SMALL_LDS ) ; if( p.kind<0 ) p = planKind( l , GS_SMALL ); }
+		if( p.kind<0 ) p = planKind( l , GS_PHASES );
+		if( p.kind==GS_PHASES && t.threads ) p.threads = t.threads;
+		return p;
+	}
+	// workgroup size of level l's FP32 colour launches
+	int floatThreads( int l , bool batched=false ) const
+	{
+		const TunePick t = TuneTable::Get().find( Dim , levels[l].res , batched , true );
+		return t.threads ? t.threads : 256;
+	}
+	// one colour launch of the sweep with `threads` per workgroup; 32-bit
+	// indices when every index of the level (node, column entry) is below 2^31
+	template< typename Real >
+	void launchPhase( const LevelView< Dim , Real > &v , const ColourSet< Dim > &cs , long long colBase , int threads=256 )
+	{
+		if( v.N*W < ( 1LL<<31 ) ) KGsPhase< Dim , Real , int ><<< Blocks( cs.total*C , threads ) , threads , 0 , stream >>>( v , cs , colBase );
+		else KGsPhase< Dim , Real , long long ><<< Blocks( cs.total*C , threads ) , threads , 0 , stream >>>( v , cs , colBase );
 	}
 	void gs( int l , int iters )
 	{
@@ -798,7 +935,7 @@ struct Ctx : public CtxBase
 			// the iters*2^Dim colour launches of a level are captured once into a
 			// HIP graph and replayed (the level's buffers are stable between builds
 			// of the same size)
-			if( !L.graph || L.graphIters!=iters )
+			if( !L.graph || L.graphIters!=iters || L.graphThreads!=pl.threads )
 			{
 				if( L.graph ){ (void)hipGraphExecDestroy( L.graph ) ; L.graph = nullptr; }
 				hipGraph_t g;
@@ -808,18 +945,18 @@ struct Ctx : public CtxBase
 					if( L.fm.p )   // adapted: the active nodes of the colour only
 					{
 						const long long cnt = L.actStart[c+1] - L.actStart[c];
-						if( cnt ) KGsPhaseList< Dim ><<< Blocks( cnt*C , 256 ) , 256 , 0 , stream >>>( v , L.act.p + L.actStart[c] , cnt );
+						if( cnt ) KGsPhaseList< Dim ><<< Blocks( cnt*C , pl.threads ) , pl.threads , 0 , stream >>>( v , L.act.p + L.actStart[c] , cnt );
 					}
 					else
 					{
 						ColourSet< Dim > cs ; cs.set( L.res , c );
-						launchPhase( v , cs , v.lay.cstart[c] );
+						launchPhase( v , cs , v.lay.cstart[c] , pl.threads );
 					}
 				}
 				HIPCHECK( hipStreamEndCapture( stream , &g ) );
 				HIPCHECK( hipGraphInstantiate( &L.graph , g , nullptr , nullptr , 0 ) );
 				HIPCHECK( hipGraphDestroy( g ) );
-				L.graphIters = iters;
+				L.graphIters = iters , L.graphThreads = pl.threads;
 				L.graphX = L.x.p , L.graphR = L.R.p , L.graphF = L.fm.p , L.graphAct = L.act.p;
 				for( int c=0 ; c<=C ; c++ ) L.graphActStart[c] = L.actStart[c];
 			}
@@ -835,23 +972,37 @@ struct Ctx : public CtxBase
 	{
 		needBuilt() ; checkLevel( l );
 		if( c<0 || c>=C ) throw std::string( "colour out of range" );
-		if( gsPlan( l ).kind!=GS_PHASES ) throw std::string( "single colour phases run on the levels swept by colour launches only" );
+		const GsPlan pl = gsPlan( l );
+		if( pl.kind!=GS_PHASES ) throw std::string( "single colour phases run on the levels swept by colour launches only" );
 		LevelView< Dim > v = view( l );
 		Level &L = levels[l];
 		if( L.fm.p )
 		{
 			const long long cnt = L.actStart[c+1] - L.actStart[c];
-			if( cnt ) KGsPhaseList< Dim ><<< Blocks( cnt*C , 256 ) , 256 , 0 , stream >>>( v , L.act.p + L.actStart[c] , cnt );
+			if( cnt ) KGsPhaseList< Dim ><<< Blocks( cnt*C , pl.threads ) , pl.threads , 0 , stream >>>( v , L.act.p + L.actStart[c] , cnt );
 		}
 		else
 		{
 			ColourSet< Dim > cs ; cs.set( L.res , c );
-			launchPhase( v , cs , v.lay.cstart[c] );
+			launchPhase( v , cs , v.lay.cstart[c] , pl.threads );
 		}
 		HIPCHECK( hipGetLastError() );
 		HIPCHECK( hipStreamSynchronize( stream ) );
 	}
 
+	// the float copies of level l's R, B and K (once per build), and room for x, y
+	void narrowOperators( int l )
+	{
+		Level &L = levels[l];
+		const long long N = L.N;
+		if( L.narrowed ) return;
+		L.fx.ensure( N ) , L.fy.ensure( N ) , L.fR.ensure( (size_t)N*W ) , L.fB.ensure( (size_t)N*W ) , L.fK.ensure( (size_t)W*W );
+		KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.R.p , L.fR.p , N*W );
+		KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.B.p , L.fB.p , N*W );
+		KNarrow<<< Blocks( (long long)W*W , 256 ) , 256 , 0 , stream >>>( L.K.p , L.fK.p , (long long)W*W );
+		HIPCHECK( hipGetLastError() );
+		L.narrowed = true;
+	}
 	// iters FP32 iterations of a dense level swept by colour launches: x, y
 	// (and, once per build, R, B, K) rounded to float, the float colour
 	// launches (captured into a graph and replayed), x, y widened back
@@ -859,21 +1010,15 @@ struct Ctx : public CtxBase
 	{
 		Level &L = levels[l];
 		const long long N = L.N;
-		if( !L.narrowed )
-		{
-			L.fx.ensure( N ) , L.fy.ensure( N ) , L.fR.ensure( (size_t)N*W ) , L.fB.ensure( (size_t)N*W ) , L.fK.ensure( (size_t)W*W );
-			KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.R.p , L.fR.p , N*W );
-			KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.B.p , L.fB.p , N*W );
-			KNarrow<<< Blocks( (long long)W*W , 256 ) , 256 , 0 , stream >>>( L.K.p , L.fK.p , (long long)W*W );
-			L.narrowed = true;
-		}
+		narrowOperators( l );
 		KNarrow<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.x.p , L.fx.p , N );
 		KNarrow<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( L.y.p , L.fy.p , N );
 		// the captured launches stay valid across builds while the float buffers stay put
 		const float *bufs[5] = { L.fx.p , L.fy.p , L.fR.p , L.fB.p , L.fK.p };
 		bool moved = false;
 		for( int k=0 ; k<5 ; k++ ) moved |= bufs[k]!=L.graph32P[k];
-		if( !L.graph32 || L.graph32Iters!=iters || moved )
+		const int threads = floatThreads( l );
+		if( !L.graph32 || L.graph32Iters!=iters || moved || L.graph32Threads!=threads )
 		{
 			if( L.graph32 ){ (void)hipGraphExecDestroy( L.graph32 ) ; L.graph32 = nullptr; }
 			const LevelView< Dim > v = view( l );
@@ -886,12 +1031,12 @@ struct Ctx : public CtxBase
 			for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
 			{
 				ColourSet< Dim > cs ; cs.set( L.res , c );
-				launchPhase( f , cs , f.lay.cstart[c] );
+				launchPhase( f , cs , f.lay.cstart[c] , threads );
 			}
 			HIPCHECK( hipStreamEndCapture( stream , &g ) );
 			HIPCHECK( hipGraphInstantiate( &L.graph32 , g , nullptr , nullptr , 0 ) );
 			HIPCHECK( hipGraphDestroy( g ) );
-			L.graph32Iters = iters;
+			L.graph32Iters = iters , L.graph32Threads = threads;
 			for( int k=0 ; k<5 ; k++ ) L.graph32P[k] = bufs[k];
 		}
 		HIPCHECK( hipGraphLaunch( L.graph32 , stream ) );
@@ -922,13 +1067,14 @@ struct Ctx : public CtxBase
 		f.res = v.res , f.N = v.N , f.x = fX.p , f.y = fY.p , f.R = fR.p , f.B = fB.p , f.K = fK.p;
 		f.sW = (float)v.sW , f.dW = (float)v.dW , f.lay = v.lay , f.fmask = nullptr , f.separate = v.separate;
 		for( int d=0 ; d<Dim ; d++ ) f.stride[d] = v.stride[d];
+		const int threads = floatThreads( l );
 		hipEvent_t e0 , e1;
 		HIPCHECK( hipEventCreate( &e0 ) ) ; HIPCHECK( hipEventCreate( &e1 ) );
 		HIPCHECK( hipEventRecord( e0 , stream ) );
 		for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
 		{
 			ColourSet< Dim > cs ; cs.set( L.res , c );
-			launchPhase( f , cs , f.lay.cstart[c] );
+			launchPhase( f , cs , f.lay.cstart[c] , threads );
 		}
 		HIPCHECK( hipEventRecord( e1 , stream ) );
 		HIPCHECK( hipEventSynchronize( e1 ) );
@@ -1049,6 +1195,83 @@ struct Ctx : public CtxBase
 		dTmp2.upload( &s , 1 , stream );
 		KScale2<<< Blocks( L.N , 256 ) , 256 , 0 , stream >>>( L.x.p , L.y.p , L.N , dTmp2.p );
 		HIPCHECK( hipStreamSynchronize( stream ) );
+	}
+
+	// ---------------- autotuning ----------------
+	int scheduleIters( int l ) const { return (int)std::ceil( std::pow( prm.iMult , D-l ) * prm.gsIters ); }
+	// Times every sweep kernel that can run each level of the solve on this
+	// context's own data (x and y restored after every trial) and keeps the
+	// fastest in the tuning table (single-context plans): the one-workgroup
+	// kernels on levels of at most TuneOneGroupMax nodes, colour launches of
+	// 64, 128 and 256 threads; flags bit 0 also times the FP32 colour launches'
+	// workgroup sizes.  A trial runs the level's schedule iterations, at most
+	// 128 and at most 2^24 / N of them.  Returns one line per trial: "level res
+	// nodes precision kind threads us_per_iteration", the kept ones marked '*'.
+	std::string autotune( int flags )
+	{
+		needBuilt();
+		std::string rep;
+		char line[200];
+		const int fp32 = prm.fp32Sweeps;
+		prm.fp32Sweeps = 0;   // the FP64 form of a level alone
+		DevBuf< double > sx , sy;
+		try
+		{
+			for( int l=minD() ; l<=maxD() ; l++ )
+			{
+				Level &L = levels[l];
+				const int its = (int)std::max< long long >( 1 , std::min< long long >( { (long long)scheduleIters( l ) , 128LL , ( 1LL<<24 )/L.N } ) );
+				sx.ensure( L.N ) , sy.ensure( L.N );
+				HIPCHECK( hipMemcpyAsync( sx.p , L.x.p , L.N*sizeof(double) , hipMemcpyDeviceToDevice , stream ) );
+				HIPCHECK( hipMemcpyAsync( sy.p , L.y.p , L.N*sizeof(double) , hipMemcpyDeviceToDevice , stream ) );
+				auto restore = [&]( void )
+				{
+					HIPCHECK( hipMemcpyAsync( L.x.p , sx.p , L.N*sizeof(double) , hipMemcpyDeviceToDevice , stream ) );
+					HIPCHECK( hipMemcpyAsync( L.y.p , sy.p , L.N*sizeof(double) , hipMemcpyDeviceToDevice , stream ) );
+				};
+				std::vector< std::pair< int , int > > cand;
+				if( L.N<=TuneOneGroupMax ) for( int k : { (int)GS_WAVE , (int)GS_BLOCK , (int)GS_TINY , (int)GS_SMALL_LDS } ) if( planKind( l , k ).kind>=0 ) cand.push_back( { k , 0 } );
+				for( int th : TuneThreads ) cand.push_back( { (int)GS_PHASES , th } );
+				double best = 1e300;
+				std::pair< int , int > bc = cand.back();
+				std::vector< std::string > lines;
+				for( const auto &c : cand )
+				{
+					TuneTable::Get().set( Dim , L.res , false , false , TunePick{ c.first , c.second , 0 } );
+					gs( l , its ) , restore();   // warm-up (and the graph's capture)
+					const double us = 1000.*TimeOnStream( stream , [&]{ gs( l , its ); } )/its;
+					restore();
+					snprintf( line , sizeof(line) , "%d %d %lld fp64 %s %d %.3f" , l , L.res , L.N , GsKindName( c.first ) , c.first==GS_PHASES ? c.second : planKind( l , c.first ).threads , us );
+					lines.push_back( line );
+					if( us<best ) best = us , bc = c;
+				}
+				TuneTable::Get().set( Dim , L.res , false , false , TunePick{ bc.first , bc.second , best } );
+				for( size_t k=0 ; k<cand.size() ; k++ ) rep += lines[k] + ( cand[k]==bc ? " *\n" : "\n" );
+				if( ( flags & 1 ) && bc.first==GS_PHASES && !L.fm.p )
+				{
+					narrowOperators( l );
+					double bestF = 1e300;
+					int bt = 256;
+					std::vector< std::pair< std::string , int > > fl;
+					for( int th : TuneThreads )
+					{
+						TuneTable::Get().set( Dim , L.res , false , true , TunePick{ -1 , th , 0 } );
+						gsFloat( l , its ) , restore();
+						const double us = 1000.*TimeOnStream( stream , [&]{ gsFloat( l , its ); } )/its;
+						restore();
+						snprintf( line , sizeof(line) , "%d %d %lld fp32 phases %d %.3f" , l , L.res , L.N , th , us );
+						fl.push_back( { line , th } );
+						if( us<bestF ) bestF = us , bt = th;
+					}
+					TuneTable::Get().set( Dim , L.res , false , true , TunePick{ -1 , bt , bestF } );
+					for( const auto &f : fl ) rep += f.first + ( f.second==bt ? " *\n" : "\n" );
+				}
+				HIPCHECK( hipStreamSynchronize( stream ) );
+			}
+		}
+		catch( ... ){ prm.fp32Sweeps = fp32 ; throw; }
+		prm.fp32Sweeps = fp32;
+		return rep;
 	}
 
 	// the cascadic schedule (ExteriorPoisson.h:316-409)
@@ -1495,303 +1718,545 @@ inline void BuildBatch( const std::vector< CtxBase * > &cs , int threads )
 	if( first ) std::rethrow_exception( first );
 }
 
+// ---------------- the batched solve ----------------
 // The cascadic schedule of ExteriorPoisson.h:316-409 for many built contexts
-// of one dimension.  Level by level, the sweeps of every context that solves
-// at that level share launches: on the tiny and small levels one workgroup
-// per context runs all of that context's iterations (the latency-bound coarse
-// levels of different contexts run side by side on different CUs); on the
-// others each colour phase is one grid over that colour's nodes in every
-// context still iterating.  Fills, the random start's copies,
-// prolongations and the hierarchical split are batched the same way.  Every
-// context goes through the kernels and per-node arithmetic of its own
-// Ctx::solve, in the same order, so its coefficients and energies are the
-// same bit for bit.  The --verbose per-level energies are not printed.
+// of one dimension.  Contexts of equal depth form a group (their cascades
+// have the same number of stages); level by level, the sweeps of every
+// context of a group that solves at that level share launches: on the levels
+// swept by one-workgroup kernels one workgroup per context runs all of that
+// context's iterations (the latency-bound coarse levels of different contexts
+// run side by side on different CUs); on the others each colour phase is one
+// grid over that colour's nodes in every context still iterating.  Fills,
+// the random start (its draws made on the host, scattered and rescaled on the
+// device), prolongations and the hierarchical split are batched the same way.
+// With --fp32 a context's colour-launch levels run all iterations but the
+// last on the float copies, as its own solve does.  Every context goes
+// through the kernels and per-node arithmetic of its own Ctx::solve, in the
+// same order, so its coefficients and energies are the same bit for bit.  The
+// --verbose per-level energies are not printed.
+//
+// The device work of all groups -- each group on its own stream, the groups
+// side by side, so that the throughput-bound fine levels of the shallower
+// groups fill the GPU while the deepest group's latency-bound coarse levels
+// run -- is enqueued without a host synchronisation and captured once into a
+// HIP graph; later calls on the same contexts replay it with one
+// hipGraphLaunch, as long as nothing it depends on (the buffers, counts,
+// iteration numbers, parameters and plans of every level of every context,
+// summarised in a signature) has changed.  Only the target-field constants c
+// are uploaded per call.
+
+// host-side job tables of a batched solve, referenced by offset
+struct BatchHost
+{
+	std::vector< char > table;
+	std::vector< double > draws;
+	size_t partials = 0;
+	size_t put( const void *p , size_t bytes ){ const size_t o = ( table.size() + 15 ) & ~(size_t)15 ; table.resize( o + bytes ) ; if( bytes ) memcpy( table.data()+o , p , bytes ) ; return o; }
+};
+// the device copies and the per-call arrays
+struct BatchDevPtrs
+{
+	const char *table = nullptr;
+	const double *draws = nullptr , *c = nullptr;
+	double *partials = nullptr , *terms = nullptr;
+};
+
 template< int Dim >
-void SolveBatchGroup( const std::vector< Ctx< Dim > * > &cs , double *energies )
+struct BatchGroup
 {
 	using Cx = Ctx< Dim >;
 	using Lv = typename Cx::Level;
-	constexpr int C = DimC< Dim >::C;
-	const size_t n = cs.size();
-	if( !n ) return;
-	auto t0 = std::chrono::steady_clock::now();
-	hipStream_t st = cs[0]->stream;
+	static constexpr int C = DimC< Dim >::C;
+	struct Jobs { size_t off = 0 , count = 0; };
+	struct Stage
+	{
+		typename Cx::GsPlan plan = { -1 , 256 , 0 , 0 };
+		int floatThreads = 256;
+		Jobs all;                              // one-workgroup kernels: every context sweeping here
+		Jobs dense , dense32 , adapted , lists;  // colour launches: by decreasing iteration count
+		Jobs narrow;                           // --fp32: x, y of the contexts with float iterations
+		std::vector< int > denseIters , dense32Iters , adaptedIters;
+		std::vector< std::vector< long long > > listCount;   // [context of `adapted`][colour]
+		std::vector< size_t > smallRs;         // GS_SMALL: the contexts, launched one by one
+		Jobs prolongAdd , prolongSet , zeroAfter;
+	};
+	std::vector< Cx * > cs;
+	std::vector< int > ids;                    // positions in the whole batch (c, terms)
+	std::vector< int > lo , hi;
 	int maxD = 0;
-	std::vector< int > lo( n ) , hi( n );
-	for( size_t r=0 ; r<n ; r++ ) lo[r] = cs[r]->minD() , hi[r] = cs[r]->maxD() , maxD = std::max( maxD , cs[r]->D );
-	auto Iters = [&]( size_t r , int d ) -> int
+	long long Nd[ MAXT ];
+	std::vector< Jobs > zeroAll , splitSet , splitSub;
+	Jobs start , scale;
+	long long maxStart = 1;
+	std::vector< size_t > partOff;             // each context's partial sums in the shared buffer
+	std::vector< Stage > stages;
+
+	int iters( size_t r , int d ) const
 	{
 		const Cx &c = *cs[r];
 		if( d<lo[r] || d>hi[r] ) return 0;
 		if( c.prm.singleLevel ) return d==hi[r] ? c.prm.gsIters : 0;
 		return (int)std::ceil( std::pow( c.prm.iMult , c.D-d ) * c.prm.gsIters );
-	};
-	auto Lev = [&]( size_t r , int d ) -> Lv & { return cs[r]->levels[d]; };
-	long long Nd[ MAXT ];   // nodes of a level (the same in every context)
-	for( int d=0 ; d<=maxD ; d++ ){ Nd[d] = 1 ; for( int k=0 ; k<Dim ; k++ ) Nd[d] *= ( 1LL<<d ) + 1; }
-
-	// ---- job tables, staged on the host and uploaded at once
-	std::vector< char > host;
-	auto put = [&]( const void *p , size_t bytes ){ const size_t o = ( host.size() + 15 ) & ~(size_t)15 ; host.resize( o + bytes ) ; if( bytes ) memcpy( host.data()+o , p , bytes ) ; return o; };
-	struct Jobs { size_t off = 0 , count = 0; };
-	auto zeroJobs = [&]( int d , const std::vector< size_t > &rs , bool x , bool y )
-	{
-		std::vector< BatchZero > z;
-		for( size_t r : rs ){ Lv &L = Lev( r , d ) ; if( x ) z.push_back( { L.x.p , L.N } ) ; if( y ) z.push_back( { L.y.p , L.N } ); }
-		return Jobs{ put( z.data() , z.size()*sizeof(BatchZero) ) , z.size() };
-	};
-	auto prolongJobs = [&]( int d , const std::vector< size_t > &rs , int mode )
-	{
-		std::vector< BatchProlong > j;
-		for( size_t r : rs )
-		{
-			Lv &Lc = Lev( r , d ) , &Lf = Lev( r , d+1 );
-			j.push_back( { Lc.x.p , Lf.x.p , Lf.fm.p , mode } ) , j.push_back( { Lc.y.p , Lf.y.p , Lf.fm.p , mode } );
-		}
-		return Jobs{ put( j.data() , j.size()*sizeof(BatchProlong) ) , j.size() };
-	};
-	auto levelJobs = [&]( int d , const std::vector< size_t > &rs )
-	{
-		std::vector< BatchLevel > b;
-		for( size_t r : rs )
-		{
-			Lv &L = Lev( r , d );
-			BatchLevel e;
-			e.x = L.x.p , e.y = L.y.p , e.R = L.R.p , e.B = L.B.p , e.K = L.K.p , e.fmask = L.fm.p;
-			e.dW = cs[r]->prm.dWeight , e.separate = cs[r]->prm.separate , e.iters = Iters( r , d );
-			b.push_back( e );
-		}
-		return Jobs{ put( b.data() , b.size()*sizeof(BatchLevel) ) , b.size() };
-	};
-
-	// fills of every level, copies of the random start
-	std::vector< Jobs > zeroAll( maxD+1 );
-	for( int d=0 ; d<=maxD ; d++ )
-	{
-		std::vector< size_t > rs;
-		for( size_t r=0 ; r<n ; r++ ) if( d<=cs[r]->D ) rs.push_back( r );
-		zeroAll[d] = zeroJobs( d , rs , true , true );
 	}
-	// the random start of Ctx::init (ExteriorPoisson.h:333-341), drawn in context order
-	std::vector< double > start;
-	std::vector< size_t > startAt( n );
-	for( size_t r=0 ; r<n ; r++ )
+	// the context's float iterations at a colour-launch level (all but the last, as Ctx::gs)
+	int floatIters( size_t r , int d ) const
 	{
-		Cx &c = *cs[r];
-		Lv &L = Lev( r , lo[r] );
-		std::vector< double > hx( L.N , 0. ) , hy( L.N , 0. );
-		GlibcRand g( (unsigned int)c.prm.seed );
-		if( c.adapted() ){ for( unsigned int f : c.download( L.forder , L.nF ) ) hx[f] = g.random01() , hy[f] = g.random01(); }
-		else for( long long i=0 ; i<L.N ; i++ ) hx[i] = g.random01() , hy[i] = g.random01();
-		startAt[r] = start.size();
-		start.insert( start.end() , hx.begin() , hx.end() ) , start.insert( start.end() , hy.begin() , hy.end() );
+		const int it = iters( r , d );
+		return ( cs[r]->prm.fp32Sweeps && !cs[r]->adapted() && it>1 ) ? it-1 : 0;
 	}
-	DevBuf< double > dStart , dScale , eT;
-	dStart.upload( start.data() , start.size() , st );
-	dScale.alloc( n ) , eT.alloc( 4*n );
-	Jobs startCopy;
-	long long maxStart = 1;
-	{
-		std::vector< BatchCopy > cp;
-		for( size_t r=0 ; r<n ; r++ )
-		{
-			Lv &L = Lev( r , lo[r] );
-			cp.push_back( { dStart.p + startAt[r] , L.x.p , L.N } ) , cp.push_back( { dStart.p + startAt[r] + L.N , L.y.p , L.N } );
-			maxStart = std::max( maxStart , L.N );
-		}
-		startCopy = Jobs{ put( cp.data() , cp.size()*sizeof(BatchCopy) ) , cp.size() };
-	}
+	Lv &lev( size_t r , int d ) const { return cs[r]->levels[d]; }
 
-	// per level: the sweeps, then the prolongation to the next level
-	struct Stage
+	void plan( const std::vector< Cx * > &_cs , const std::vector< int > &_ids , BatchHost &H )
 	{
-		typename Cx::GsPlan plan;
-		Jobs all;                              // GS_TINY / GS_SMALL_LDS / GS_SMALL: every context sweeping here
-		Jobs dense , adapted , lists;          // GS_PHASES: by decreasing iteration count
-		std::vector< int > denseIters , adaptedIters;
-		std::vector< std::vector< long long > > listCount;   // [context of `adapted`][colour]
-		std::vector< size_t > smallRs;         // GS_SMALL: the contexts, launched one by one
-		Jobs prolongAdd , prolongSet , zeroAfter;
-	};
-	std::vector< Stage > stages( maxD+1 );
-	for( int d=0 ; d<=maxD ; d++ )
-	{
-		Stage &S = stages[d];
-		std::vector< size_t > rs;
-		for( size_t r=0 ; r<n ; r++ ) if( d<=cs[r]->D && Iters( r , d )>0 ) rs.push_back( r );
-		if( !rs.empty() )
+		cs = _cs , ids = _ids;
+		const size_t n = cs.size();
+		lo.resize( n ) , hi.resize( n );
+		for( size_t r=0 ; r<n ; r++ ) lo[r] = cs[r]->minD() , hi[r] = cs[r]->maxD() , maxD = std::max( maxD , cs[r]->D );
+		for( int d=0 ; d<=maxD ; d++ ){ Nd[d] = 1 ; for( int k=0 ; k<Dim ; k++ ) Nd[d] *= ( 1LL<<d ) + 1; }
+		auto zeroJobs = [&]( int d , const std::vector< size_t > &rs )
 		{
-			S.plan = cs[ rs[0] ]->gsPlan( d );
-			if( S.plan.kind==Cx::GS_PHASES )
+			std::vector< BatchZero > z;
+			for( size_t r : rs ){ Lv &L = lev( r , d ) ; z.push_back( { L.x.p , L.N } ) , z.push_back( { L.y.p , L.N } ); }
+			return Jobs{ H.put( z.data() , z.size()*sizeof(BatchZero) ) , z.size() };
+		};
+		auto prolongJobs = [&]( int d , const std::vector< size_t > &rs , int mode )
+		{
+			std::vector< BatchProlong > j;
+			for( size_t r : rs )
 			{
-				std::stable_sort( rs.begin() , rs.end() , [&]( size_t a , size_t b ){ return Iters( a , d )>Iters( b , d ); } );
-				std::vector< size_t > rd , ra;
-				for( size_t r : rs ) ( cs[r]->adapted() ? ra : rd ).push_back( r );
-				S.dense = levelJobs( d , rd ) , S.adapted = levelJobs( d , ra );
-				for( size_t r : rd ) S.denseIters.push_back( Iters( r , d ) );
-				for( size_t r : ra ) S.adaptedIters.push_back( Iters( r , d ) );
-				std::vector< BatchList > ls;
-				for( size_t r : ra )
-				{
-					Lv &L = Lev( r , d );
-					S.listCount.emplace_back( C );
-					for( int c=0 ; c<C ; c++ ){ const long long cnt = L.actStart[c+1]-L.actStart[c] ; ls.push_back( { L.act.p + L.actStart[c] , cnt } ) ; S.listCount.back()[c] = cnt; }
-				}
-				S.lists = Jobs{ put( ls.data() , ls.size()*sizeof(BatchList) ) , ls.size() };
+				Lv &Lc = lev( r , d ) , &Lf = lev( r , d+1 );
+				j.push_back( { Lc.x.p , Lf.x.p , Lf.fm.p , mode } ) , j.push_back( { Lc.y.p , Lf.y.p , Lf.fm.p , mode } );
 			}
-			else if( S.plan.kind==Cx::GS_SMALL ) S.smallRs = rs;
-			else S.all = levelJobs( d , rs );
-		}
-		if( d<maxD )
+			return Jobs{ H.put( j.data() , j.size()*sizeof(BatchProlong) ) , j.size() };
+		};
+		auto levelJobs = [&]( int d , const std::vector< size_t > &rs , const std::vector< int > &its )
 		{
-			std::vector< size_t > add , set;
-			for( size_t r=0 ; r<n ; r++ ) if( d>=lo[r] && d<hi[r] ) ( cs[r]->prm.singleLevel ? set : add ).push_back( r );
-			S.prolongAdd = prolongJobs( d , add , 1 ) , S.prolongSet = prolongJobs( d , set , 0 );
-			S.zeroAfter = zeroJobs( d , set , true , true );   // --singleLevel: the coarser levels end at zero
-		}
-	}
-	// the hierarchical split after the solve: levels above the last solved one
-	// are prolongations (=), then every level keeps only its own detail (-=)
-	std::vector< Jobs > splitSet( maxD+1 ) , splitSub( maxD+1 );
-	for( int d=0 ; d<maxD ; d++ )
-	{
-		std::vector< size_t > rs;
-		for( size_t r=0 ; r<n ; r++ ) if( d>=hi[r] && d<cs[r]->D ) rs.push_back( r );
-		splitSet[d] = prolongJobs( d , rs , 0 );
-	}
-	for( int d=maxD ; d>0 ; d-- )
-	{
-		std::vector< size_t > rs;
-		for( size_t r=0 ; r<n ; r++ ) if( d<=cs[r]->D ) rs.push_back( r );
-		splitSub[d] = prolongJobs( d-1 , rs , 2 );
-	}
-	DevBuf< char > table;
-	table.alloc( std::max< size_t >( host.size() , 16 ) );
-	HIPCHECK( hipMemcpyAsync( table.p , host.data() , host.size() , hipMemcpyHostToDevice , st ) );
-	auto at = [&]( const Jobs &j , size_t k=0 ){ return (const void *)( table.p + j.off + k ); };
+			std::vector< BatchLevel > b;
+			for( size_t q=0 ; q<rs.size() ; q++ )
+			{
+				const size_t r = rs[q];
+				Lv &L = lev( r , d );
+				BatchLevel e;
+				e.x = L.x.p , e.y = L.y.p , e.R = L.R.p , e.B = L.B.p , e.K = L.K.p , e.fmask = L.fm.p;
+				e.dW = cs[r]->prm.dWeight , e.separate = cs[r]->prm.separate , e.iters = its[q];
+				b.push_back( e );
+			}
+			return Jobs{ H.put( b.data() , b.size()*sizeof(BatchLevel) ) , b.size() };
+		};
+		auto floatJobs = [&]( int d , const std::vector< size_t > &rs , const std::vector< int > &its )
+		{
+			std::vector< BatchLevelT< float > > b;
+			for( size_t q=0 ; q<rs.size() ; q++ )
+			{
+				const size_t r = rs[q];
+				Lv &L = lev( r , d );
+				BatchLevelT< float > e;
+				e.x = L.fx.p , e.y = L.fy.p , e.R = L.fR.p , e.B = L.fB.p , e.K = L.fK.p , e.fmask = nullptr;
+				e.dW = (float)cs[r]->prm.dWeight , e.separate = cs[r]->prm.separate , e.iters = its[q];
+				b.push_back( e );
+			}
+			return Jobs{ H.put( b.data() , b.size()*sizeof(BatchLevelT< float >) ) , b.size() };
+		};
 
-	auto launchZero = [&]( const Jobs &j , int d )
-	{
-		if( j.count ) KZeroBatch<<< dim3( (unsigned)std::min< long long >( Blocks( Nd[d] , 256 ) , 1024 ) , (unsigned)j.count ) , 256 , 0 , st >>>( (const BatchZero *)at( j ) );
-	};
-	auto launchProlong = [&]( const Jobs &j , int d )
-	{
-		if( j.count ) KProlongBatch< Dim ><<< dim3( Blocks( Nd[d+1] , 256 ) , (unsigned)j.count ) , 256 , 0 , st >>>( (const BatchProlong *)at( j ) , 1<<d );
-	};
-	// (W, yBx, R-form, K-form) of context r at level d into eT[4r..4r+3] (Ctx::energyTerms)
-	std::vector< double > hT( 4*n );
-	auto terms = [&]( const std::vector< int > &level )
-	{
+		// fills of every level
+		zeroAll.resize( maxD+1 );
+		for( int d=0 ; d<=maxD ; d++ )
+		{
+			std::vector< size_t > rs;
+			for( size_t r=0 ; r<n ; r++ ) if( d<=cs[r]->D ) rs.push_back( r );
+			zeroAll[d] = zeroJobs( d , rs );
+		}
+		// the random start of Ctx::init (ExteriorPoisson.h:333-341): the draws of
+		// each context's private rand() sequence, in the order they are assigned
+		// (the hierarchy's numbering of an adapted level, function order on a dense one)
+		{
+			std::vector< BatchStart > st;
+			std::vector< BatchScale > sc;
+			for( size_t r=0 ; r<n ; r++ )
+			{
+				Cx &c = *cs[r];
+				Lv &L = lev( r , lo[r] );
+				const long long cnt = c.adapted() ? (long long)L.nF : L.N;
+				GlibcRand g( (unsigned int)c.prm.seed );
+				const long long off = (long long)H.draws.size();
+				for( long long k=0 ; k<2*cnt ; k++ ) H.draws.push_back( g.random01() );
+				st.push_back( { off , L.x.p , L.y.p , c.adapted() ? L.forder.p : nullptr , cnt } );
+				sc.push_back( { L.x.p , L.y.p , L.N , ids[r] } );
+				maxStart = std::max( maxStart , L.N );
+			}
+			start = Jobs{ H.put( st.data() , st.size()*sizeof(BatchStart) ) , st.size() };
+			scale = Jobs{ H.put( sc.data() , sc.size()*sizeof(BatchScale) ) , sc.size() };
+		}
+		// partial sums of the energy terms (the start's and the final level's)
+		partOff.resize( n );
 		for( size_t r=0 ; r<n ; r++ )
 		{
-			Cx &c = *cs[r];
-			const unsigned int nb = Blocks( Lev( r , level[r] ).N*C , 256 );
-			c.dPartials.ensure( (size_t)nb*4 );
-			KEnergy< Dim ><<< nb , 256 , 0 , st >>>( c.view( level[r] ) , c.dPartials.p );
-			KSumRows<4><<< 1 , 256 , 0 , st >>>( c.dPartials.p , nb , eT.p + 4*r );
+			const long long nb = std::max( Blocks( lev( r , lo[r] ).N*C , 256 ) , Blocks( lev( r , hi[r] ).N*C , 256 ) );
+			partOff[r] = H.partials , H.partials += (size_t)nb*4;
+		}
+
+		// per level: the sweeps, then the prolongation to the next level
+		stages.resize( maxD+1 );
+		for( int d=0 ; d<=maxD ; d++ )
+		{
+			Stage &S = stages[d];
+			std::vector< size_t > rs;
+			for( size_t r=0 ; r<n ; r++ ) if( d<=cs[r]->D && iters( r , d )>0 ) rs.push_back( r );
+			if( !rs.empty() )
+			{
+				S.plan = cs[ rs[0] ]->gsPlan( d , true );
+				S.floatThreads = cs[ rs[0] ]->floatThreads( d , true );
+				if( S.plan.kind==Cx::GS_PHASES )
+				{
+					// FP64 iterations: all of a context's, or the last one after its float iterations
+					auto dIters = [&]( size_t r ){ return floatIters( r , d ) ? 1 : iters( r , d ); };
+					std::vector< size_t > rd , ra , rf;
+					for( size_t r : rs ) ( cs[r]->adapted() ? ra : rd ).push_back( r );
+					std::stable_sort( rd.begin() , rd.end() , [&]( size_t a , size_t b ){ return dIters( a )>dIters( b ); } );
+					std::stable_sort( ra.begin() , ra.end() , [&]( size_t a , size_t b ){ return iters( a , d )>iters( b , d ); } );
+					for( size_t r : rd ) S.denseIters.push_back( dIters( r ) );
+					for( size_t r : ra ) S.adaptedIters.push_back( iters( r , d ) );
+					S.dense = levelJobs( d , rd , S.denseIters ) , S.adapted = levelJobs( d , ra , S.adaptedIters );
+					for( size_t r : rd ) if( floatIters( r , d ) ) rf.push_back( r );
+					std::stable_sort( rf.begin() , rf.end() , [&]( size_t a , size_t b ){ return floatIters( a , d )>floatIters( b , d ); } );
+					for( size_t r : rf ) S.dense32Iters.push_back( floatIters( r , d ) );
+					S.dense32 = floatJobs( d , rf , S.dense32Iters );
+					{
+						std::vector< BatchCast > nc;
+						for( size_t r : rf ){ Lv &L = lev( r , d ) ; nc.push_back( { L.x.p , L.fx.p , L.N } ) , nc.push_back( { L.y.p , L.fy.p , L.N } ); }
+						S.narrow = Jobs{ H.put( nc.data() , nc.size()*sizeof(BatchCast) ) , nc.size() };
+					}
+					std::vector< BatchList > ls;
+					for( size_t r : ra )
+					{
+						Lv &L = lev( r , d );
+						S.listCount.emplace_back( C );
+						for( int c=0 ; c<C ; c++ ){ const long long cnt = L.actStart[c+1]-L.actStart[c] ; ls.push_back( { L.act.p + L.actStart[c] , cnt } ) ; S.listCount.back()[c] = cnt; }
+					}
+					S.lists = Jobs{ H.put( ls.data() , ls.size()*sizeof(BatchList) ) , ls.size() };
+				}
+				else if( S.plan.kind==Cx::GS_SMALL ) S.smallRs = rs;
+				else
+				{
+					std::vector< int > its;
+					for( size_t r : rs ) its.push_back( iters( r , d ) );
+					S.all = levelJobs( d , rs , its );
+				}
+			}
+			if( d<maxD )
+			{
+				std::vector< size_t > add , set;
+				for( size_t r=0 ; r<n ; r++ ) if( d>=lo[r] && d<hi[r] ) ( cs[r]->prm.singleLevel ? set : add ).push_back( r );
+				S.prolongAdd = prolongJobs( d , add , 1 ) , S.prolongSet = prolongJobs( d , set , 0 );
+				S.zeroAfter = zeroJobs( d , set );   // --singleLevel: the coarser levels end at zero
+			}
+		}
+		// the hierarchical split after the solve: levels above the last solved one
+		// are prolongations (=), then every level keeps only its own detail (-=)
+		splitSet.resize( maxD+1 ) , splitSub.resize( maxD+1 );
+		for( int d=0 ; d<maxD ; d++ )
+		{
+			std::vector< size_t > rs;
+			for( size_t r=0 ; r<n ; r++ ) if( d>=hi[r] && d<cs[r]->D ) rs.push_back( r );
+			splitSet[d] = prolongJobs( d , rs , 0 );
+		}
+		for( int d=maxD ; d>0 ; d-- )
+		{
+			std::vector< size_t > rs;
+			for( size_t r=0 ; r<n ; r++ ) if( d<=cs[r]->D ) rs.push_back( r );
+			splitSub[d] = prolongJobs( d-1 , rs , 2 );
+		}
+	}
+
+	// the sweeps of level d of every context that solves there
+	void enqueueSweeps( hipStream_t st , const BatchDevPtrs &P , int d ) const
+	{
+		auto at = [&]( const Jobs &j ){ return (const void *)( P.table + j.off ); };
+		{
+			const Stage &S = stages[d];
+			// any context with this level gives the shared constants (resolution, strides, layout, wedge scale)
+			size_t rAny = 0;
+			while( cs[rAny]->D<d ) rAny++;
+			const LevelView< Dim > L0 = cs[rAny]->view( d );
+			if( S.all.count )
+			{
+				const unsigned g = (unsigned)S.all.count;
+				const BatchLevel *J = (const BatchLevel *)at( S.all );
+				if( S.plan.kind==Cx::GS_WAVE )
+				{
+					if constexpr( Dim==3 ){ if( L0.res==1 ) KGsWaveBatch< 1 ><<< g , 64 , 0 , st >>>( L0 , J ) ; else KGsWaveBatch< 2 ><<< g , 64 , 0 , st >>>( L0 , J ); }
+				}
+				else if( S.plan.kind==Cx::GS_BLOCK ){ if constexpr( Dim==3 ) KGsBlockBatch<<< g , 512 , S.plan.lds , st >>>( L0 , J ); }
+				else if( S.plan.kind==Cx::GS_TINY )
+				{
+					if( S.plan.rb ) KGsTinyBatch< Dim , true ><<< g , S.plan.threads , S.plan.lds , st >>>( L0 , J );
+					else KGsTinyBatch< Dim , false ><<< g , S.plan.threads , S.plan.lds , st >>>( L0 , J );
+				}
+				else KGsSmallLdsBatch< Dim ><<< g , S.plan.threads , S.plan.lds , st >>>( L0 , J , S.plan.rb );
+			}
+			for( size_t r : S.smallRs ) KGsSmall< Dim ><<< 1 , 512 , 0 , st >>>( cs[r]->view( d ) , iters( r , d ) );
+			const int th = S.plan.threads;
+			const bool idx32 = L0.N*DimC< Dim >::W < ( 1LL<<31 );
+			if( S.dense32.count )
+			{
+				// --fp32: x, y rounded, the float iterations, x, y widened (Ctx::gsFloat)
+				const unsigned g = (unsigned)std::min< long long >( Blocks( Nd[d] , 256 ) , 1024 );
+				KNarrowBatch<<< dim3( g , (unsigned)S.narrow.count ) , 256 , 0 , st >>>( (const BatchCast *)at( S.narrow ) );
+				LevelView< Dim , float > F0;
+				F0.res = L0.res , F0.N = L0.N , F0.x = F0.y = nullptr , F0.R = F0.B = F0.K = nullptr;
+				F0.sW = (float)L0.sW , F0.dW = 0 , F0.lay = L0.lay , F0.fmask = nullptr , F0.separate = 0;
+				for( int k=0 ; k<Dim ; k++ ) F0.stride[k] = L0.stride[k];
+				const BatchLevelT< float > *J = (const BatchLevelT< float > *)at( S.dense32 );
+				size_t k = S.dense32.count;
+				for( int it=0 ; it<S.dense32Iters[0] ; it++ )
+				{
+					while( k && S.dense32Iters[k-1]<=it ) k--;
+					for( int c=0 ; c<C ; c++ )
+					{
+						ColourSet< Dim > cset ; cset.set( 1<<d , c );
+						const dim3 grid( Blocks( cset.total*C , S.floatThreads ) , (unsigned)k );
+						if( idx32 ) KGsPhaseBatch< Dim , float , int ><<< grid , S.floatThreads , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
+						else KGsPhaseBatch< Dim , float , long long ><<< grid , S.floatThreads , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
+					}
+				}
+				KWidenBatch<<< dim3( g , (unsigned)S.narrow.count ) , 256 , 0 , st >>>( (const BatchCast *)at( S.narrow ) );
+			}
+			if( S.dense.count )
+			{
+				const BatchLevel *J = (const BatchLevel *)at( S.dense );
+				size_t k = S.dense.count;
+				for( int it=0 ; it<S.denseIters[0] ; it++ )
+				{
+					while( k && S.denseIters[k-1]<=it ) k--;
+					for( int c=0 ; c<C ; c++ )
+					{
+						ColourSet< Dim > cset ; cset.set( 1<<d , c );
+						const dim3 grid( Blocks( cset.total*C , th ) , (unsigned)k );
+						if( idx32 ) KGsPhaseBatch< Dim , double , int ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
+						else KGsPhaseBatch< Dim , double , long long ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
+					}
+				}
+			}
+			if( S.adapted.count )
+			{
+				size_t k = S.adapted.count;
+				for( int it=0 ; it<S.adaptedIters[0] ; it++ )
+				{
+					while( k && S.adaptedIters[k-1]<=it ) k--;
+					for( int c=0 ; c<C ; c++ )
+					{
+						long long mx = 0;
+						for( size_t q=0 ; q<k ; q++ ) mx = std::max( mx , S.listCount[q][c] );
+						if( mx ) KGsPhaseListBatch< Dim ><<< dim3( Blocks( mx*C , th ) , (unsigned)k ) , th , 0 , st >>>( L0 , (const BatchLevel *)at( S.adapted ) , (const BatchList *)at( S.lists ) , c );
+					}
+				}
+			}
 		}
 		HIPCHECK( hipGetLastError() );
-		HIPCHECK( hipMemcpyAsync( hT.data() , eT.p , 4*n*sizeof(double) , hipMemcpyDeviceToHost , st ) );
-		HIPCHECK( hipStreamSynchronize( st ) );
-	};
+	}
 
-	// ---- zero, random start, rescale s = ( c / W(x,y) )^(1/4)
-	for( int d=0 ; d<=maxD ; d++ ) launchZero( zeroAll[d] , d );
-	KCopyBatch<<< dim3( (unsigned)std::min< long long >( Blocks( maxStart , 256 ) , 1024 ) , (unsigned)startCopy.count ) , 256 , 0 , st >>>( (const BatchCopy *)at( startCopy ) );
-	terms( lo );
+	// Every device step of the group's solve, on stream st, with no host
+	// synchronisation (so that it can be captured into a graph).
+	void enqueue( hipStream_t st , const BatchDevPtrs &P ) const
 	{
-		std::vector< double > s( n );
-		for( size_t r=0 ; r<n ; r++ ) s[r] = std::sqrt( std::sqrt( cs[r]->c_ / hT[4*r] ) );
-		dScale.upload( s.data() , n , st );
-		for( size_t r=0 ; r<n ; r++ ){ Lv &L = Lev( r , lo[r] ) ; KScale2<<< Blocks( L.N , 256 ) , 256 , 0 , st >>>( L.x.p , L.y.p , L.N , dScale.p + r ); }
+		const size_t n = cs.size();
+		auto at = [&]( const Jobs &j ){ return (const void *)( P.table + j.off ); };
+		auto launchZero = [&]( const Jobs &j , int d )
+		{
+			if( j.count ) KZeroBatch<<< dim3( (unsigned)std::min< long long >( Blocks( Nd[d] , 256 ) , 1024 ) , (unsigned)j.count ) , 256 , 0 , st >>>( (const BatchZero *)at( j ) );
+		};
+		auto launchProlong = [&]( const Jobs &j , int d )
+		{
+			if( j.count ) KProlongBatch< Dim ><<< dim3( Blocks( Nd[d+1] , 256 ) , (unsigned)j.count ) , 256 , 0 , st >>>( (const BatchProlong *)at( j ) , 1<<d );
+		};
+		// (W, yBx, R-form, K-form) of every context at its level into terms[4 id] (Ctx::energyTerms)
+		auto terms = [&]( const std::vector< int > &level )
+		{
+			for( size_t r=0 ; r<n ; r++ )
+			{
+				const unsigned int nb = Blocks( lev( r , level[r] ).N*C , 256 );
+				double *part = P.partials + partOff[r];
+				KEnergy< Dim ><<< nb , 256 , 0 , st >>>( cs[r]->view( level[r] ) , part );
+				KSumRows<4><<< 1 , 256 , 0 , st >>>( part , nb , P.terms + 4*ids[r] );
+			}
+		};
+		// ---- zero, random start, rescale s = ( c / W(x,y) )^(1/4)
+		for( int d=0 ; d<=maxD ; d++ ) launchZero( zeroAll[d] , d );
+		KStartBatch<<< dim3( (unsigned)std::min< long long >( Blocks( maxStart , 256 ) , 1024 ) , (unsigned)start.count ) , 256 , 0 , st >>>( (const BatchStart *)at( start ) , P.draws );
+		terms( lo );
+		KRescaleBatch<<< dim3( (unsigned)std::min< long long >( Blocks( maxStart , 256 ) , 1024 ) , (unsigned)scale.count ) , 256 , 0 , st >>>( (const BatchScale *)at( scale ) , P.c , P.terms );
+		// ---- the levels
+		for( int d=0 ; d<=maxD ; d++ )
+		{
+			const Stage &S = stages[d];
+			enqueueSweeps( st , P , d );
+			if( d<maxD )
+			{
+				launchProlong( S.prolongAdd , d );
+				launchProlong( S.prolongSet , d );
+				launchZero( S.zeroAfter , d );
+			}
+		}
+		// ---- final energy terms (Ctx::energies), then the hierarchical split
+		terms( hi );
+		for( int d=0 ; d<maxD ; d++ ) launchProlong( splitSet[d] , d );
+		for( int d=maxD ; d>0 ; d-- ) launchProlong( splitSub[d] , d-1 );
+		HIPCHECK( hipGetLastError() );
 	}
-	// ---- the levels
-	for( int d=0 ; d<=maxD ; d++ )
+};
+
+// Everything a batched solve's recorded device work depends on: the contexts,
+// their parameters, and for every level the buffers, counts and plans.
+template< int Dim >
+std::vector< long long > BatchSignature( const std::vector< Ctx< Dim > * > &cs )
+{
+	constexpr int C = DimC< Dim >::C;
+	std::vector< long long > s;
+	auto bits = []( double v ){ long long b ; memcpy( &b , &v , sizeof(b) ) ; return b; };
+	auto ptr = []( const void *p ){ return (long long)(uintptr_t)p; };
+	s.push_back( (long long)cs.size() );
+	for( const Ctx< Dim > *c : cs )
 	{
-		Stage &S = stages[d];
-		// any context with this level gives the shared constants (resolution, strides, layout, wedge scale)
-		size_t rAny = 0;
-		while( cs[rAny]->D<d ) rAny++;
-		const LevelView< Dim > L0 = cs[rAny]->view( d );
-		if( S.all.count )
+		const epr_params &p = c->prm;
+		for( long long v : { ptr( c ) , (long long)c->D , (long long)c->minD() , (long long)c->maxD() , (long long)p.seed , (long long)p.gsIters , bits( p.iMult ) , (long long)p.singleLevel ,
+		                     (long long)p.separate , bits( p.dWeight ) , (long long)p.fp32Sweeps , (long long)p.aRadius } ) s.push_back( v );
+		for( int d=0 ; d<=c->D ; d++ )
 		{
-			if( S.plan.kind==Cx::GS_WAVE )
-			{
-				if constexpr( Dim==3 )
-				{
-					if( L0.res==1 ) KGsWaveBatch< 1 ><<< (unsigned)S.all.count , 64 , 0 , st >>>( L0 , (const BatchLevel *)at( S.all ) );
-					else KGsWaveBatch< 2 ><<< (unsigned)S.all.count , 64 , 0 , st >>>( L0 , (const BatchLevel *)at( S.all ) );
-				}
-			}
-			else if( S.plan.kind==Cx::GS_BLOCK ){ if constexpr( Dim==3 ) KGsBlockBatch<<< (unsigned)S.all.count , 512 , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) ); }
-			else if( S.plan.kind==Cx::GS_TINY )
-			{
-				if( S.plan.rb ) KGsTinyBatch< Dim , true ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) );
-				else KGsTinyBatch< Dim , false ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) );
-			}
-			else
-			{
-				KGsSmallLdsBatch< Dim ><<< (unsigned)S.all.count , S.plan.threads , S.plan.lds , st >>>( L0 , (const BatchLevel *)at( S.all ) , S.plan.rb );
-			}
-		}
-		for( size_t r : S.smallRs ) KGsSmall< Dim ><<< 1 , 512 , 0 , st >>>( cs[r]->view( d ) , Iters( r , d ) );
-		if( S.dense.count )
-		{
-			size_t k = S.dense.count;
-			for( int it=0 ; it<S.denseIters[0] ; it++ )
-			{
-				while( k && S.denseIters[k-1]<=it ) k--;
-				for( int c=0 ; c<C ; c++ )
-				{
-					ColourSet< Dim > cset ; cset.set( 1<<d , c );
-					KGsPhaseBatch< Dim ><<< dim3( Blocks( cset.total*C , 256 ) , (unsigned)k ) , 256 , 0 , st >>>( L0 , (const BatchLevel *)at( S.dense ) , cset , L0.lay.cstart[c] );
-				}
-			}
-		}
-		if( S.adapted.count )
-		{
-			size_t k = S.adapted.count;
-			for( int it=0 ; it<S.adaptedIters[0] ; it++ )
-			{
-				while( k && S.adaptedIters[k-1]<=it ) k--;
-				for( int c=0 ; c<C ; c++ )
-				{
-					long long mx = 0;
-					for( size_t q=0 ; q<k ; q++ ) mx = std::max( mx , S.listCount[q][c] );
-					if( mx ) KGsPhaseListBatch< Dim ><<< dim3( Blocks( mx*C , 256 ) , (unsigned)k ) , 256 , 0 , st >>>( L0 , (const BatchLevel *)at( S.adapted ) , (const BatchList *)at( S.lists ) , c );
-				}
-			}
-		}
-		if( d<maxD )
-		{
-			launchProlong( S.prolongAdd , d );
-			launchProlong( S.prolongSet , d );
-			launchZero( S.zeroAfter , d );
+			const auto &L = c->levels[d];
+			for( const void *q : { (const void *)L.x.p , (const void *)L.y.p , (const void *)L.R.p , (const void *)L.B.p , (const void *)L.K.p , (const void *)L.fm.p , (const void *)L.act.p , (const void *)L.forder.p ,
+			                       (const void *)L.fx.p , (const void *)L.fy.p , (const void *)L.fR.p , (const void *)L.fB.p , (const void *)L.fK.p } ) s.push_back( ptr( q ) );
+			s.push_back( L.N ) , s.push_back( (long long)L.nF );
+			for( int k=0 ; k<=C ; k++ ) s.push_back( L.actStart[k] );
+			const auto pl = c->gsPlan( d , true );
+			s.push_back( pl.kind ) , s.push_back( pl.threads ) , s.push_back( pl.rb ) , s.push_back( (long long)pl.lds ) , s.push_back( c->floatThreads( d , true ) );
 		}
 	}
-	HIPCHECK( hipGetLastError() );
-	// ---- final energies (Ctx::energies), then the hierarchical split
-	terms( hi );
-	for( size_t r=0 ; r<n ; r++ )
-	{
-		const double *t = &hT[4*r];
-		double e0 = t[0] - 2.*t[1] + cs[r]->c_ , e1 = t[2];
-		if( cs[r]->prm.dWeight ) e1 += t[3]*cs[r]->prm.dWeight;
-		if( energies ) energies[2*r] = e0 , energies[2*r+1] = e1;
-	}
-	for( int d=0 ; d<maxD ; d++ ) launchProlong( splitSet[d] , d );
-	for( int d=maxD ; d>0 ; d-- ) launchProlong( splitSub[d] , d-1 );
-	HIPCHECK( hipGetLastError() );
-	HIPCHECK( hipStreamSynchronize( st ) );
-	const double sec = std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count();
-	for( Cx *c : cs ) c->solved = true , c->seconds[1] = sec;
+	return s;
 }
 
-// The batched solve: contexts of equal depth form a group (their cascades have
-// the same number of stages), and the groups run side by side, each from its
-// own host thread on its own stream, so that the throughput-bound fine levels
-// of the shallower groups fill the GPU while the deepest group's latency-bound
-// coarse levels run.  Within a group the levels are batched (SolveBatchGroup).
+// A recorded batched solve: the plans, their device tables, the streams and the graph.
+template< int Dim >
+struct BatchExec
+{
+	std::vector< long long > sig;
+	std::vector< BatchGroup< Dim > > groups;
+	DevBuf< char > table;
+	DevBuf< double > draws , partials , terms , c;
+	double *hC = nullptr , *hT = nullptr;      // pinned: the per-call c values and the energy terms
+	size_t n = 0;
+	hipStream_t origin = nullptr;
+	std::vector< hipStream_t > streams;
+	std::vector< hipEvent_t > events;          // [0]: fork, [1..]: one join per group
+	hipGraphExec_t graph = nullptr;
+	unsigned long long uses = 0;
+	std::mutex busy;
+	~BatchExec()
+	{
+		if( graph ) (void)hipGraphExecDestroy( graph );
+		for( hipEvent_t e : events ) (void)hipEventDestroy( e );
+		for( hipStream_t s : streams ) (void)hipStreamDestroy( s );
+		if( origin ) (void)hipStreamDestroy( origin );
+		if( hC ) (void)hipHostFree( hC );
+		if( hT ) (void)hipHostFree( hT );
+	}
+	// all groups: forked from the origin stream, each on its own, joined back
+	void enqueueAll( void )
+	{
+		BatchDevPtrs P;
+		P.table = table.p , P.draws = draws.p , P.c = c.p , P.partials = partials.p , P.terms = terms.p;
+		HIPCHECK( hipEventRecord( events[0] , origin ) );
+		for( size_t k=0 ; k<groups.size() ; k++ )
+		{
+			HIPCHECK( hipStreamWaitEvent( streams[k] , events[0] , 0 ) );
+			groups[k].enqueue( streams[k] , P );
+			HIPCHECK( hipEventRecord( events[k+1] , streams[k] ) );
+			HIPCHECK( hipStreamWaitEvent( origin , events[k+1] , 0 ) );
+		}
+	}
+	void setup( const std::vector< Ctx< Dim > * > &cs , std::vector< long long > &&s , bool useGraph )
+	{
+		sig = std::move( s );
+		n = cs.size();
+		std::vector< std::vector< Ctx< Dim > * > > gc( MAXT );
+		std::vector< std::vector< int > > gi( MAXT );
+		for( size_t r=0 ; r<n ; r++ ) gc[ cs[r]->D ].push_back( cs[r] ) , gi[ cs[r]->D ].push_back( (int)r );
+		BatchHost H;
+		for( int d=0 ; d<MAXT ; d++ ) if( !gc[d].empty() ){ groups.emplace_back() ; groups.back().plan( gc[d] , gi[d] , H ); }
+		HIPCHECK( hipStreamCreate( &origin ) );
+		table.alloc( std::max< size_t >( H.table.size() , 16 ) ) , draws.alloc( std::max< size_t >( H.draws.size() , 1 ) );
+		partials.alloc( std::max< size_t >( H.partials , 1 ) ) , terms.alloc( 4*n ) , c.alloc( n );
+		HIPCHECK( hipMemcpyAsync( table.p , H.table.data() , H.table.size() , hipMemcpyHostToDevice , origin ) );
+		if( !H.draws.empty() ) HIPCHECK( hipMemcpyAsync( draws.p , H.draws.data() , H.draws.size()*sizeof(double) , hipMemcpyHostToDevice , origin ) );
+		HIPCHECK( hipStreamSynchronize( origin ) );
+		HIPCHECK( hipHostMalloc( (void **)&hC , n*sizeof(double) , hipHostMallocDefault ) );
+		HIPCHECK( hipHostMalloc( (void **)&hT , 4*n*sizeof(double) , hipHostMallocDefault ) );
+		streams.resize( groups.size() , nullptr ) , events.resize( groups.size()+1 , nullptr );
+		for( auto &s : streams ) HIPCHECK( hipStreamCreate( &s ) );
+		for( auto &e : events ) HIPCHECK( hipEventCreateWithFlags( &e , hipEventDisableTiming ) );
+		if( useGraph )
+		{
+			hipGraph_t g;
+			HIPCHECK( hipStreamBeginCapture( origin , hipStreamCaptureModeThreadLocal ) );
+			try { enqueueAll(); }
+			catch( ... ){ hipGraph_t dropped ; (void)hipStreamEndCapture( origin , &dropped ) ; if( dropped ) (void)hipGraphDestroy( dropped ) ; throw; }
+			HIPCHECK( hipStreamEndCapture( origin , &g ) );
+			const hipError_t e = hipGraphInstantiate( &graph , g , nullptr , nullptr , 0 );
+			(void)hipGraphDestroy( g );
+			HIPCHECK( e );
+		}
+	}
+	// one solve: the contexts' c, the recorded work, the energy terms back
+	void run( const std::vector< Ctx< Dim > * > &cs , double *energies )
+	{
+		for( size_t r=0 ; r<n ; r++ ) hC[r] = cs[r]->c_;
+		HIPCHECK( hipMemcpyAsync( c.p , hC , n*sizeof(double) , hipMemcpyHostToDevice , origin ) );
+		if( graph ) HIPCHECK( hipGraphLaunch( graph , origin ) );
+		else enqueueAll();
+		HIPCHECK( hipMemcpyAsync( hT , terms.p , 4*n*sizeof(double) , hipMemcpyDeviceToHost , origin ) );
+		HIPCHECK( hipStreamSynchronize( origin ) );
+		for( size_t r=0 ; r<n ; r++ )
+		{
+			const double *t = hT + 4*r;
+			double e0 = t[0] - 2.*t[1] + cs[r]->c_ , e1 = t[2];
+			if( cs[r]->prm.dWeight ) e1 += t[3]*cs[r]->prm.dWeight;
+			if( energies ) energies[2*r] = e0 , energies[2*r+1] = e1;
+		}
+		uses++;
+	}
+};
+
+// The recorded solves of the process (per dimension), the most recently used
+// first; a few are kept (each holds its tables, streams and graph).
+template< int Dim >
+struct BatchExecCache
+{
+	static constexpr size_t Keep = 4;
+	std::mutex m;
+	std::vector< std::shared_ptr< BatchExec< Dim > > > list;
+	static BatchExecCache &Get( void ){ static BatchExecCache *c = new BatchExecCache ; return *c; }
+	std::shared_ptr< BatchExec< Dim > > find( const std::vector< long long > &sig , bool useGraph )
+	{
+		std::lock_guard< std::mutex > lock( m );
+		for( size_t i=0 ; i<list.size() ; i++ ) if( list[i]->sig==sig && ( list[i]->graph!=nullptr )==useGraph )
+		{
+			std::shared_ptr< BatchExec< Dim > > e = list[i];
+			list.erase( list.begin()+i ) , list.insert( list.begin() , e );
+			return e;
+		}
+		return nullptr;
+	}
+	void add( const std::shared_ptr< BatchExec< Dim > > &e )
+	{
+		std::lock_guard< std::mutex > lock( m );
+		list.insert( list.begin() , e );
+		while( list.size()>Keep ) list.pop_back();
+	}
+	void clear( void ){ std::lock_guard< std::mutex > lock( m ) ; list.clear(); }
+};
+
+// The batched solve (see above).  EPR_BATCH_GRAPH=0 enqueues the same work
+// directly on every call instead of recording and replaying it (for tools
+// that do not follow graph launches).
 template< int Dim >
 void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
 {
@@ -1805,38 +2270,146 @@ void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
 		for( const Cx *c : cs ) if( !seen.insert( c ).second ) throw std::string( "a context appears twice in the batch" );
 	}
 	for( Cx *c : cs ) c->needBuilt();
-	std::vector< std::vector< size_t > > groups( MAXT );
-	for( size_t r=0 ; r<n ; r++ ) groups[ cs[r]->D ].push_back( r );
-	std::vector< std::vector< Cx * > > gc;
-	std::vector< std::vector< double > > ge;
-	std::vector< const std::vector< size_t > * > gi;
-	for( const auto &g : groups ) if( !g.empty() )
+	auto t0 = std::chrono::steady_clock::now();
+	// the float copies of the operators of every level that sweeps in FP32 (made by the build; again after a re-plan)
+	for( Cx *c : cs ) if( c->prm.fp32Sweeps && !c->adapted() )
 	{
-		gc.emplace_back() , ge.emplace_back( 2*g.size() ) , gi.push_back( &g );
-		for( size_t r : g ) gc.back().push_back( cs[r] );
+		bool any = false;
+		for( int d=0 ; d<=c->D ; d++ ) if( c->gsPlan( d , true ).kind==Cx::GS_PHASES && !c->levels[d].narrowed ) c->narrowOperators( d ) , any = true;
+		if( any ) HIPCHECK( hipStreamSynchronize( c->stream ) );
 	}
-	std::mutex m;
-	std::exception_ptr first;
-	int dev = 0;
-	HIPCHECK( hipGetDevice( &dev ) );
-	auto work = [&]( size_t k )
+	static const bool useGraph = !( getenv( "EPR_BATCH_GRAPH" ) && atoi( getenv( "EPR_BATCH_GRAPH" ) )==0 );
+	std::vector< long long > sig = BatchSignature< Dim >( cs );
+	std::shared_ptr< BatchExec< Dim > > e = BatchExecCache< Dim >::Get().find( sig , useGraph );
+	if( !e )
 	{
-		try
+		e = std::make_shared< BatchExec< Dim > >();
+		e->setup( cs , std::move( sig ) , useGraph );
+		BatchExecCache< Dim >::Get().add( e );
+	}
+	{
+		std::lock_guard< std::mutex > lock( e->busy );
+		e->run( cs , energies );
+	}
+	const double sec = std::chrono::duration< double >( std::chrono::steady_clock::now()-t0 ).count();
+	for( Cx *c : cs ) c->solved = true , c->seconds[1] = sec;
+}
+
+// Tunes the batched solve's plans (the batched entries of the tuning table)
+// on these contexts: for every level, each sweep kernel that can run it --
+// the one-workgroup-per-context kernels on levels of at most TuneOneGroupMax
+// nodes, colour grids of 64, 128 and 256 threads -- times the level's sweeps
+// of every context with that level (their schedule's iterations, the depth
+// groups one after another on one stream, x and y restored after every
+// trial); the fastest is kept.  flags bit 0 also times the workgroup size of
+// the FP32 colour grids (contexts with --fp32).  Returns one line per trial:
+// "level res nodes contexts precision kind threads us", the kept ones marked '*'.
+template< int Dim >
+std::string AutotuneBatch( const std::vector< Ctx< Dim > * > &cs , int flags )
+{
+	using Cx = Ctx< Dim >;
+	const size_t n = cs.size();
+	if( !n ) return "";
+	{
+		std::set< const Cx * > seen;
+		for( const Cx *c : cs ) if( !seen.insert( c ).second ) throw std::string( "a context appears twice in the batch" );
+	}
+	for( Cx *c : cs ) c->needBuilt();
+	std::vector< std::vector< Cx * > > gc( MAXT );
+	std::vector< std::vector< int > > gi( MAXT );
+	int maxD = 0;
+	for( size_t r=0 ; r<n ; r++ ) gc[ cs[r]->D ].push_back( cs[r] ) , gi[ cs[r]->D ].push_back( (int)r ) , maxD = std::max( maxD , cs[r]->D );
+	hipStream_t st = nullptr;
+	HIPCHECK( hipStreamCreate( &st ) );
+	std::string rep;
+	char line[200];
+	DevBuf< char > table;
+	DevBuf< double > save;
+	try
+	{
+		for( int d=0 ; d<=maxD ; d++ )
 		{
-			HIPCHECK( hipSetDevice( dev ) );
-			SolveBatchGroup< Dim >( gc[k] , ge[k].data() );
+			// the contexts with this level, and room for their x, y
+			std::vector< Cx * > with;
+			for( Cx *c : cs ) if( c->D>=d && c->minD()<=d && c->maxD()>=d ) with.push_back( c );
+			if( with.empty() ) continue;
+			const long long N = with[0]->levels[d].N;
+			const int res = with[0]->levels[d].res;
+			save.ensure( (size_t)2*N*with.size() );
+			for( size_t q=0 ; q<with.size() ; q++ )
+			{
+				HIPCHECK( hipMemcpyAsync( save.p + 2*q*N , with[q]->levels[d].x.p , N*sizeof(double) , hipMemcpyDeviceToDevice , st ) );
+				HIPCHECK( hipMemcpyAsync( save.p + ( 2*q+1 )*N , with[q]->levels[d].y.p , N*sizeof(double) , hipMemcpyDeviceToDevice , st ) );
+			}
+			auto restore = [&]( void )
+			{
+				for( size_t q=0 ; q<with.size() ; q++ )
+				{
+					HIPCHECK( hipMemcpyAsync( with[q]->levels[d].x.p , save.p + 2*q*N , N*sizeof(double) , hipMemcpyDeviceToDevice , st ) );
+					HIPCHECK( hipMemcpyAsync( with[q]->levels[d].y.p , save.p + ( 2*q+1 )*N , N*sizeof(double) , hipMemcpyDeviceToDevice , st ) );
+				}
+			};
+			// the level's sweeps of every group under the current table
+			auto trial = [&]( void )
+			{
+				for( Cx *c : with ) if( c->prm.fp32Sweeps && !c->adapted() && c->gsPlan( d , true ).kind==Cx::GS_PHASES && !c->levels[d].narrowed )
+				{
+					c->narrowOperators( d );
+					HIPCHECK( hipStreamSynchronize( c->stream ) );
+				}
+				BatchHost H;
+				std::vector< BatchGroup< Dim > > groups;
+				for( int g=0 ; g<MAXT ; g++ ) if( !gc[g].empty() && g>=d ){ groups.emplace_back() ; groups.back().plan( gc[g] , gi[g] , H ); }
+				table.ensure( std::max< size_t >( H.table.size() , 16 ) );
+				HIPCHECK( hipMemcpyAsync( table.p , H.table.data() , H.table.size() , hipMemcpyHostToDevice , st ) );
+				BatchDevPtrs P;
+				P.table = table.p;
+				auto sweeps = [&]{ for( const auto &g : groups ) g.enqueueSweeps( st , P , d ); };
+				sweeps() , restore();   // warm-up
+				const double us = 1000.*TimeOnStream( st , sweeps );
+				restore();
+				HIPCHECK( hipStreamSynchronize( st ) );   // the host table outlives no launch
+				return us;
+			};
+			std::vector< std::pair< int , int > > cand;
+			if( N<=TuneOneGroupMax ) for( int k : { (int)Cx::GS_WAVE , (int)Cx::GS_BLOCK , (int)Cx::GS_TINY , (int)Cx::GS_SMALL_LDS } ) if( with[0]->planKind( d , k ).kind>=0 ) cand.push_back( { k , 0 } );
+			for( int th : TuneThreads ) cand.push_back( { (int)Cx::GS_PHASES , th } );
+			double best = 1e300;
+			std::pair< int , int > bc = cand.back();
+			std::vector< std::string > lines;
+			for( const auto &c : cand )
+			{
+				TuneTable::Get().set( Dim , res , true , false , TunePick{ c.first , c.second , 0 } );
+				const double us = trial();
+				snprintf( line , sizeof(line) , "%d %d %lld %zu fp64 %s %d %.3f" , d , res , N , with.size() , GsKindName( c.first ) , c.first==Cx::GS_PHASES ? c.second : with[0]->planKind( d , c.first ).threads , us );
+				lines.push_back( line );
+				if( us<best ) best = us , bc = c;
+			}
+			TuneTable::Get().set( Dim , res , true , false , TunePick{ bc.first , bc.second , best } );
+			for( size_t k=0 ; k<cand.size() ; k++ ) rep += lines[k] + ( cand[k]==bc ? " *\n" : "\n" );
+			bool anyF = false;
+			for( Cx *c : with ) anyF |= c->prm.fp32Sweeps && !c->adapted();
+			if( ( flags & 1 ) && bc.first==Cx::GS_PHASES && anyF )
+			{
+				double bestF = 1e300;
+				int bt = 256;
+				std::vector< std::pair< std::string , int > > fl;
+				for( int th : TuneThreads )
+				{
+					TuneTable::Get().set( Dim , res , true , true , TunePick{ -1 , th , 0 } );
+					const double us = trial();
+					snprintf( line , sizeof(line) , "%d %d %lld %zu fp32 phases %d %.3f" , d , res , N , with.size() , th , us );
+					fl.push_back( { line , th } );
+					if( us<bestF ) bestF = us , bt = th;
+				}
+				TuneTable::Get().set( Dim , res , true , true , TunePick{ -1 , bt , bestF } );
+				for( const auto &f : fl ) rep += f.first + ( f.second==bt ? " *\n" : "\n" );
+			}
 		}
-		catch( ... ){ std::lock_guard< std::mutex > lock( m ) ; if( !first ) first = std::current_exception(); }
-	};
-	// EPR_BATCH_SERIAL=1: the groups one after another on the calling thread
-	// (for tools that do not follow kernel launches from several threads)
-	static const bool serial = getenv( "EPR_BATCH_SERIAL" ) && atoi( getenv( "EPR_BATCH_SERIAL" ) )!=0;
-	std::vector< std::thread > ts;
-	for( size_t k=1 ; k<gc.size() ; k++ ){ if( serial ) work( k ) ; else ts.emplace_back( work , k ); }
-	work( 0 );
-	for( auto &t : ts ) t.join();
-	if( first ) std::rethrow_exception( first );
-	if( energies ) for( size_t k=0 ; k<gc.size() ; k++ ) for( size_t j=0 ; j<gi[k]->size() ; j++ ) energies[ 2*(*gi[k])[j] ] = ge[k][2*j] , energies[ 2*(*gi[k])[j]+1 ] = ge[k][2*j+1];
+	}
+	catch( ... ){ (void)hipStreamSynchronize( st ) ; (void)hipStreamDestroy( st ) ; throw; }
+	(void)hipStreamDestroy( st );
+	return rep;
 }
 } // namespace epr
 
@@ -2005,6 +2578,40 @@ int epr_solve_batch( size_t n , epr_ctx *const *ctxs , double *energies )
 		else if( dim==4 ) run( std::integral_constant< int , 4 >() );
 	} );
 }
+static void CopyReport( const std::string &r , char *out , size_t cap )
+{
+	if( !out || !cap ) return;
+	const size_t k = std::min( r.size() , cap-1 );
+	memcpy( out , r.data() , k ) , out[k] = 0;
+}
+int epr_autotune( epr_ctx *ctx , int flags , char *report , size_t cap ){ IMPL( CopyReport( ctx->impl->autotune( flags ) , report , cap ) ); }
+int epr_autotune_batch( size_t n , epr_ctx *const *ctxs , int flags , char *report , size_t cap )
+{
+	return epr::ContextFree( [&]()
+	{
+		int dim;
+		std::vector< epr::CtxBase * > v = BatchImpls( n , ctxs , true , dim );
+		auto run = [&]( auto tag )
+		{
+			constexpr int Dim = decltype( tag )::value;
+			std::vector< epr::Ctx< Dim > * > c;
+			for( epr::CtxBase *b : v ) c.push_back( static_cast< epr::Ctx< Dim > * >( b ) );
+			CopyReport( epr::AutotuneBatch< Dim >( c , flags ) , report , cap );
+		};
+		if( dim==3 ) run( std::integral_constant< int , 3 >() );
+		else if( dim==4 ) run( std::integral_constant< int , 4 >() );
+	} );
+}
+int epr_tune_get( char *buf , size_t cap , size_t *len )
+{
+	return epr::ContextFree( [&]()
+	{
+		const std::string t = epr::TuneTable::Get().text();
+		if( len ) *len = t.size();
+		CopyReport( t , buf , cap );
+	} );
+}
+int epr_tune_set( const char *text ){ return epr::ContextFree( [&](){ epr::TuneTable::Get().parse( text ); } ); }
 double epr_last_seconds( const epr_ctx *ctx , int which ){ return ( ctx && ctx->impl && ( which==0 || which==1 ) ) ? ctx->impl->seconds[which] : -1.; }
 
 }

@@ -157,6 +157,15 @@ class Reconstructor:
     def seconds(self, which):
         return load().epr_last_seconds(self._h, 0 if which == "build" else 1)
 
+    def autotune(self, fp32=False):
+        """Times every sweep plan of each level of this (built) reconstruction's
+        solve and keeps the fastest for single solves (epr_autotune).  Returns
+        the trials as (level, res, nodes, precision, kind, threads,
+        us_per_iteration, kept) tuples."""
+        buf = ctypes.create_string_buffer(1 << 16)
+        self._check(load().epr_autotune(self._h, 1 if fp32 else 0, buf, len(buf)))
+        return _trials(buf.value.decode(), 7)
+
 
 def _handles(recs):
     hs = [r._h for r in recs]
@@ -188,3 +197,49 @@ def solve_batch(recs):
     E = np.zeros((len(recs), 2))
     _batch_check(load().epr_solve_batch(len(recs), _handles(recs), _p(E)))
     return E
+
+
+def _trials(text, nfields):
+    """Report lines of the autotuners as tuples: integer fields, the kind and
+    precision names, the time (the last field) and whether the trial was kept."""
+    out = []
+    for line in text.splitlines():
+        f = line.split()
+        kept = len(f) > nfields and f[nfields] == "*"
+        row = []
+        for k, v in enumerate(f[:nfields]):
+            if k == nfields - 1:
+                row.append(float(v))
+            else:
+                try:
+                    row.append(int(v))
+                except ValueError:
+                    row.append(v)
+        out.append(tuple(row) + (kept,))
+    return out
+
+
+def autotune_batch(recs, fp32=False):
+    """Times every sweep plan of each level of solve_batch(recs) and keeps the
+    fastest for batched solves (epr_autotune_batch).  Returns the trials as
+    (level, res, nodes, contexts, precision, kind, threads, us, kept) tuples."""
+    recs = list(recs)
+    buf = ctypes.create_string_buffer(1 << 16)
+    _batch_check(load().epr_autotune_batch(len(recs), _handles(recs), 1 if fp32 else 0, buf, len(buf)))
+    return _trials(buf.value.decode(), 8)
+
+
+def tune_table():
+    """The current device's tuning table as text (epr_tune_get): one line per
+    entry, "dim res batched fp32 kind threads us"."""
+    n = ctypes.c_size_t(0)
+    _batch_check(load().epr_tune_get(None, 0, ctypes.byref(n)))
+    buf = ctypes.create_string_buffer(n.value + 1)
+    _batch_check(load().epr_tune_get(buf, len(buf), None))
+    return buf.value.decode()
+
+
+def set_tune_table(text):
+    """Replaces the current device's tuning table (epr_tune_set); None or ""
+    restores the built-in rules."""
+    _batch_check(load().epr_tune_set(text.encode() if text else None))

@@ -143,13 +143,46 @@ int  epr_solve( epr_ctx *ctx , double out_E[2] );
  * epr_solve_batch: epr_solve on each of n built contexts of one dimension,
  * with every level's sweeps of all contexts in shared launches (one
  * workgroup per context on the coarse levels, one grid over all contexts'
- * nodes of a colour on the others).  Each context ends with exactly the
- * coefficients and energies its own epr_solve gives; energies (may be NULL)
- * receives n x 2 values (E_fit, E_reg).  --verbose prints nothing here.
- * Contexts of equal depth are solved together, each such group from its own
- * host thread on its first context's stream; at most 32767 contexts. */
+ * nodes of a colour on the others; contexts with fp32Sweeps take their float
+ * iterations in shared float grids).  Each context ends with exactly the
+ * coefficients and energies its own epr_solve gives (with the same tuned
+ * plans); energies (may be NULL) receives n x 2 values (E_fit, E_reg).
+ * --verbose prints nothing here.  Contexts of equal depth are solved
+ * together, the groups side by side on their own streams; the whole device
+ * work is recorded into one HIP graph on the first call and replayed by later
+ * calls on the same contexts while their buffers, counts, parameters and
+ * plans are unchanged (only the target-field constants are uploaded per
+ * call); at most 32767 contexts. */
 int  epr_build_batch( size_t n , epr_ctx *const *ctxs , int threads );
 int  epr_solve_batch( size_t n , epr_ctx *const *ctxs , double *energies );
+
+/* Autotuning of the sweep plans (no reference counterpart: the reference has
+ * one CPU loop per level, ExteriorPoissonSolvers.h:156-196).  A plan is, per
+ * dimension and level resolution, the sweep kernel of the level (one of the
+ * one-workgroup kernels, or one launch per colour) and the workgroup size of
+ * its colour launches (64, 128 or 256 threads); one table per device, kept
+ * until the process ends, consulted whenever a context plans a level.
+ *
+ * epr_autotune: times every candidate plan on each level of this built
+ * context's solve (on its own data; x and y are restored) and keeps the
+ * fastest for single-context solves; flags bit 0 also tunes the FP32 colour
+ * launches.  report (may be NULL) receives one line per trial, "level res
+ * nodes precision kind threads us_per_iteration", kept ones ending in " *",
+ * truncated to cap bytes including the terminating NUL.
+ *
+ * epr_autotune_batch: the same for epr_solve_batch on these contexts: per
+ * level, the sweeps of every context with that level under each candidate
+ * ("level res nodes contexts precision kind threads us" lines).
+ *
+ * epr_tune_get: the current device's table as text, one line per entry
+ * "dim res batched fp32 kind threads us" (kind -1: the built-in rule); *len
+ * (may be NULL) receives the full length, buf (may be NULL) the text
+ * truncated to cap bytes including the NUL.  epr_tune_set: replaces the
+ * current device's table with such a text (NULL or "": the built-in rules). */
+int  epr_autotune( epr_ctx *ctx , int flags , char *report , size_t cap );
+int  epr_autotune_batch( size_t n , epr_ctx *const *ctxs , int flags , char *report , size_t cap );
+int  epr_tune_get( char *buf , size_t cap , size_t *len );
+int  epr_tune_set( const char *text );
 
 /* Number of hat functions (2^level+1)^dim of a level (the dense layout in
  * which coefficients are exchanged, also for adapted reconstructions). */

@@ -133,3 +133,101 @@ def test_gpu_batch_errors():
         solve_batch([a, a])
     assert solve_batch([]).shape == (0, 2)
     a.close(), b.close()
+
+
+def test_gpu_batch_fp32_matches_single_solves():
+    """Contexts with --fp32 (their float iterations in shared float grids, the
+    last iteration of each colour-launch level in FP64) next to FP64 and
+    adapted ones: each ends bit for bit where its own solve() ends, and the
+    replayed graph of a second call gives the same bits."""
+    from exteriorpoissonrecon_amd import Reconstructor, build_batch, samples, solve_batch
+    inputs = [
+        (samples.torus_knot(4096, 2, 3, seed=21), dict(depth=6, aRadius=-1, fp32Sweeps=1)),
+        (samples.spiral(3000, 4, seed=22), dict(depth=5, aRadius=-1, fp32Sweeps=1, seed=5)),
+        (samples.torus_knot(4096, 3, 5, seed=23), dict(depth=6, aRadius=-1)),
+        (samples.spiral(2048, 3, seed=24), dict(depth=4, aRadius=-1, fp32Sweeps=1, separate=1)),
+        (samples.torus_knot(3000, 2, 5, seed=25), dict(depth=5, aRadius=3, fp32Sweeps=1)),
+        (samples.torus_knot(2048, 3, 4, seed=26), dict(depth=6, aRadius=-1, fp32Sweeps=1, iMult=1.5)),
+    ]
+    print("case batch3d-fp32", len(inputs), flush=True)
+    recs = build_batch([Reconstructor(pos, skew, **prm) for (pos, skew), prm in inputs])
+    ref = _solve_each(recs)
+    E = solve_batch(recs)
+    _check_same(recs, E, ref)
+    E2 = solve_batch(recs)
+    assert np.array_equal(E, E2)
+    _check_same(recs, E2, ref)
+    for r in recs:
+        r.close()
+
+
+def test_gpu_batch_follows_rebuilds():
+    """New samples of the same depths rebuilt into the same contexts keep their
+    buffers, so the recorded graph is replayed; it must follow the new
+    operators, target constants and random starts: equal to single solves
+    after the rebuild."""
+    from exteriorpoissonrecon_amd import Reconstructor, build_batch, samples, solve_batch
+    first = [samples.torus_knot(2048, 2, 3, seed=31), samples.spiral(3000, 3, seed=32), samples.torus_knot(3000, 3, 4, seed=33)]
+    second = [samples.spiral(2500, 5, seed=41), samples.torus_knot(4096, 2, 5, seed=42), samples.spiral(2048, 2, seed=43)]
+    prms = [dict(depth=4, aRadius=-1), dict(depth=5, aRadius=-1, fp32Sweeps=1), dict(depth=5, aRadius=3)]
+    print("case batch-rebuild", flush=True)
+    recs = build_batch([Reconstructor(p, s, **prm) for (p, s), prm in zip(first, prms)])
+    E1 = solve_batch(recs)
+    for r, (p, s) in zip(recs, second):
+        r.set_samples(p, s)
+    build_batch(recs)
+    ref = _solve_each(recs)
+    E = solve_batch(recs)
+    _check_same(recs, E, ref)
+    assert not np.array_equal(E, E1)
+    for r in recs:
+        r.close()
+
+
+def test_gpu_autotune_plans():
+    """The autotuners time every candidate plan of every level and keep one per
+    level; solves under the tuned plans stay within the solve bar of the
+    built-in plans' (the kernels differ only in their summation order), the
+    table's text form round-trips, and clearing it restores the built-in
+    plans bit for bit (the batch records a new graph when a plan changes)."""
+    from exteriorpoissonrecon_amd import (Reconstructor, autotune_batch, build_batch, samples, set_tune_table, solve_batch,
+                                          tune_table)
+    set_tune_table(None)
+    assert tune_table() == ""
+    print("case autotune", flush=True)
+    recs = build_batch([Reconstructor(*samples.torus_knot(4096, 2, 3, seed=51), depth=6, aRadius=-1, fp32Sweeps=1),
+                        Reconstructor(*samples.spiral(3000, 3, seed=52), depth=5, aRadius=-1),
+                        Reconstructor(*samples.torus_knot(3000, 3, 5, seed=53), depth=4, aRadius=3)])
+    E0 = solve_batch(recs)
+    S0 = recs[0].solve()
+    try:
+        trials = recs[0].autotune(fp32=True)
+        levels = sorted({t[0] for t in trials})
+        assert levels == list(range(7))
+        for l in levels:
+            assert sum(1 for t in trials if t[0] == l and t[3] == "fp64" and t[-1]) == 1
+            assert all(t[6] > 0 for t in trials if t[0] == l)
+        assert any(t[3] == "fp32" and t[-1] for t in trials)
+        S1 = recs[0].solve()
+        assert rel_err(S1, S0) < TOL_SOLVE, (S1, S0)
+        bt = autotune_batch(recs, fp32=True)
+        assert sorted({t[0] for t in bt}) == list(range(7))
+        for l in range(7):
+            assert sum(1 for t in bt if t[0] == l and t[4] == "fp64" and t[-1]) == 1
+        E1 = solve_batch(recs)
+        assert rel_err(E1, E0) < TOL_SOLVE, (E1, E0)
+        text = tune_table()
+        lines = text.splitlines()
+        assert len(lines) >= 14 and all(len(l.split()) == 7 for l in lines)
+        set_tune_table(text)
+        assert tune_table() == text
+        assert np.array_equal(solve_batch(recs), E1)
+        with pytest.raises(Exception, match="bad tuning line"):
+            set_tune_table("3 4 0 0 3 96 1.0\n")
+        assert tune_table() == text
+    finally:
+        set_tune_table(None)
+    assert np.array_equal(solve_batch(recs), E0)
+    assert np.array_equal(recs[0].solve(), S0)
+    for r in recs:
+        r.close()
