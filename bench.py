@@ -14,6 +14,7 @@ Workloads (synthetic samples, exteriorpoissonrecon_amd.samples):
   batch3d_256     256 independent 3D inputs (random coprime (p,q) torus knots
                   or r-rotation spirals, 2048-16384 samples, depths 4/5/6):
                   a step is epr_build_batch + epr_solve_batch of all 256
+                  (also a nested object of the default run)
 
 --aRadius K (default -1, the dense grid the headline configurations name)
 measures the adapted hierarchy of radius K instead (the reference's default
@@ -134,53 +135,98 @@ def cpu_baseline(name, wl, pos, skew, a_radius=-1):
     return {"value": dt, "unit": "s", "cores": cores, "kind": "reference", "sample": sample, "depth": depth}
 
 
-def batch_main(args, wl):
-    """The batched workload: every step builds (epr_build_batch) and solves
-    (epr_solve_batch) all inputs; value = seconds per batch."""
+def measure_batch(args, steps, warmup, with_cpu, unbatched=True, tune=True):
+    """The batched workload (256 independent 3D inputs, depths 4/5/6): every
+    step builds (epr_build_batch) and solves (epr_solve_batch, one replayed HIP
+    graph) all inputs.  The FP32 form (float sweeps, each colour-launch level's
+    last iteration and the energies FP64) is the value when every input's final
+    energy is within 1e-6 of its FP64 batched solve's; the batched plans are
+    autotuned on the box first (outside the timed steps).  Returns a dict."""
+    from exteriorpoissonrecon_amd import (Reconstructor, autotune_batch, build_batch, samples, solve_batch,
+                                          time_batch_sweeps)
     if args.serial:
-        os.environ["EPR_BATCH_SERIAL"] = "1"
-    from exteriorpoissonrecon_amd import Reconstructor, build_batch, samples, solve_batch
+        os.environ["EPR_BATCH_GRAPH"] = "0"
     threads = 1 if args.serial else args.build_threads
-    inputs = samples.batch_workload(wl["batch"])
+    inputs = samples.batch_workload(256)
     nsamp = sum(len(p) for p, _, _, _ in inputs)
-    log(f"[bench] {args.config}: {len(inputs)} inputs, {nsamp} samples, depths "
+    log(f"[bench] batch3d_256: {len(inputs)} inputs, {nsamp} samples, depths "
         f"{sorted(set(d for _, _, d, _ in inputs))}")
-    recs = [Reconstructor(p, s, depth=d, aRadius=args.aRadius) for p, s, d, _ in inputs]
-    E = None
-    for i in range(args.warmup):
-        t0 = time.perf_counter(); build_batch(recs, threads); t1 = time.perf_counter(); E = solve_batch(recs)
-        log(f"[bench] warmup {i}: build {t1 - t0:.3f}s solve {time.perf_counter() - t1:.3f}s")
-    times, builds, solves = [], [], []
-    for i in range(args.steps):
-        t0 = time.perf_counter()
+
+    def run(fp32):
+        recs = [Reconstructor(p, s, depth=d, aRadius=args.aRadius, fp32Sweeps=fp32) for p, s, d, _ in inputs]
         build_batch(recs, threads)
-        t1 = time.perf_counter()
-        E = solve_batch(recs)
-        t2 = time.perf_counter()
-        times.append(t2 - t0); builds.append(t1 - t0); solves.append(t2 - t1)
-        log(f"[bench] step {i}: {times[-1]:.4f}s (build {builds[-1]:.4f} solve {solves[-1]:.4f})")
-    t = float(np.mean(times))
+        tuned = None
+        if tune:
+            t0 = time.perf_counter()
+            trials = autotune_batch(recs, fp32=bool(fp32))
+            tuned = {"seconds": time.perf_counter() - t0,
+                     "picks": [f"level {t[0]}: {t[4]} {t[5]} {t[6]} threads, {t[7]:.1f} us" for t in trials if t[-1]]}
+            log(f"[bench] batch autotune ({'fp32' if fp32 else 'fp64'}): {tuned['seconds']:.2f} s; " + "; ".join(tuned["picks"]))
+        E = None
+        for i in range(warmup):
+            t0 = time.perf_counter(); build_batch(recs, threads); t1 = time.perf_counter(); E = solve_batch(recs)
+            log(f"[bench] batch warmup {i}: build {t1 - t0:.3f}s solve {time.perf_counter() - t1:.3f}s")
+        times, builds, solves = [], [], []
+        for i in range(steps):
+            t0 = time.perf_counter()
+            build_batch(recs, threads)
+            t1 = time.perf_counter()
+            E = solve_batch(recs)
+            t2 = time.perf_counter()
+            times.append(t2 - t0); builds.append(t1 - t0); solves.append(t2 - t1)
+            log(f"[bench] batch step {i} ({'fp32' if fp32 else 'fp64'}): {times[-1]:.4f}s (build {builds[-1]:.4f} solve {solves[-1]:.4f})")
+        return recs, E, tuned, float(np.mean(times)), float(np.mean(builds)), float(np.mean(solves))
+
+    recs64, E64, tuned64, t64, b64, s64 = run(0)
+    recs, E, tuned, t, bt, st = run(1)
+    rel = float(np.max(np.abs(E.sum(1) - E64.sum(1)) / np.abs(E64.sum(1))))
+    ok = rel <= 1e-6
+    log(f"[bench] batch fp32: {t:.4f}s, fp64: {t64:.4f}s, max relative energy difference {rel:.2e}")
     # algorithmic work of the batched solve: node updates of every level of every input
     updates = 0
     for r, (_, _, d, _) in zip(recs, inputs):
         for l in range(d + 1):
             updates += r.num_functions(l) * int(np.ceil(2.0 ** (d - l) * 10))
-    achieved = updates * F_NODE[3] / float(np.mean(solves)) / 1e12
-    log(f"[bench] batched solve: {updates:.3e} node updates, {achieved:.2f} TFLOP/s algorithmic over the whole solve")
-    # the same inputs one after another (epr_build + epr_solve each), once
-    unbatched = diff = None
-    if not args.no_unbatched:
+    whole = updates * F_NODE[3] / (st if ok else s64) / 1e12
+    # dominant kernel: one colour grid of the finest level of the depth-6 inputs
+    top = max(d for _, _, d, _ in inputs)
+    deep = [r for r, (_, _, d, _) in zip(recs if ok else recs64, inputs) if d == top]
+    ms, launches, nodes = time_batch_sweeps(deep, top, fp32=ok, iters=5)
+    per_launch = ms * 1e-3 / launches
+    per_node_bytes = B_NODE[3] // 2 if ok else B_NODE[3]
+    achieved = F_NODE[3] * nodes / 8 / per_launch / 1e12
+    kname = f"KGsPhaseBatch<3, {'float' if ok else 'double'}, int>"
+    traffic = measured_traffic("batch3d_256" + ("_fp32" if ok else ""))
+    peak = FP32_PEAK_TFLOPS if ok else FP64_PEAK_TFLOPS
+    log(f"[bench] {kname}: {per_launch * 1e6:.1f} us per launch over {len(deep)} inputs ({nodes / 8:.0f} nodes), "
+        f"{achieved:.2f} TFLOP/s, {per_node_bytes * nodes / 8 / per_launch / 1e9:.1f} GB/s algorithmic")
+    log(f"[bench] batched solve: {updates:.3e} node updates, {whole:.2f} TFLOP/s algorithmic over the whole solve")
+    # the same inputs one after another (epr_build + epr_solve each, the
+    # value's precision), once; their energies against the FP64 batch (the
+    # single solves run the single-context plans, the batch its own tuned
+    # ones, so FP32 iterations may differ in which levels they cover)
+    unb = diff = diff64 = None
+    if unbatched:
         t0 = time.perf_counter()
         Es = []
-        for r in recs:
+        for r in (recs if ok else recs64):
             r.build(); Es.append(r.solve())
-        unbatched = time.perf_counter() - t0
-        # separate builds splat with FP64 atomics: the energies agree to rounding, not bit for bit
+        unb = time.perf_counter() - t0
         Es = np.array(Es)
-        diff = float(np.max(np.abs(Es.sum(1) - E.sum(1)) / np.abs(Es.sum(1))))
-        log(f"[bench] unbatched: {unbatched:.3f}s for the same {len(recs)} inputs; max relative energy difference to the batch {diff:.2e}")
+        diff = float(np.max(np.abs(Es.sum(1) - E64.sum(1)) / np.abs(E64.sum(1))))
+        log(f"[bench] unbatched: {unb:.3f}s for the same {len(recs)} inputs; max relative energy difference to the FP64 batch {diff:.2e}")
+        if ok:
+            # FP64 one after another against the FP64 batch (separate builds splat with FP64 atomics)
+            Es64 = []
+            for r in recs64:
+                r.build(); Es64.append(r.solve())
+            Es64 = np.array(Es64)
+            diff64 = float(np.max(np.abs(Es64.sum(1) - E64.sum(1)) / np.abs(E64.sum(1))))
+            log(f"[bench] unbatched FP64 against the FP64 batch: max relative energy difference {diff64:.2e}")
+    for r in recs + recs64:
+        r.close()
     cpu = None
-    if not args.no_cpu:
+    if with_cpu:
         exe = os.path.join(ROOT, "oracle", "_ref", "ExteriorPoissonRecon")
         if os.path.exists(exe):
             k = 8
@@ -196,22 +242,57 @@ def batch_main(args, wl):
                    "sample": f"the first {k} of the {len(inputs)} inputs (depths "
                              f"{[d for _, _, d, _ in inputs[:k]]}), reference CLI one after another, "
                              f"{os.cpu_count()} threads; x{len(inputs) / k:.0f} = {dt * len(inputs) / k:.1f} s for the batch"}
-            log(f"[bench] cpu baseline {dt:.3f}s ({cpu['sample']})")
-    out = {
-        "metric": METRIC, "value": t, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": t * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64", "data": "synthetic",
-        "config": {"workload": args.config, "dim": 3, "inputs": len(inputs), "samples": nsamp, "aRadius": args.aRadius,
+            log(f"[bench] batch cpu baseline {dt:.3f}s ({cpu['sample']})")
+    return {
+        "value": t if ok else t64, "ms_per_batch": (t if ok else t64) * 1e3, "steps": steps, "warmup": warmup,
+        "dtype": "f32+f64" if ok else "f64",
+        "config": {"workload": "batch3d_256", "dim": 3, "inputs": len(inputs), "samples": nsamp, "aRadius": args.aRadius,
                    "depths": {str(d): sum(1 for _, _, dd, _ in inputs if dd == d) for d in (4, 5, 6)},
-                   "build_s": float(np.mean(builds)), "solve_s": float(np.mean(solves)),
-                   "reconstructions_per_s": len(inputs) / t, "unbatched_s": unbatched,
-                   "max_rel_energy_diff_vs_unbatched": diff},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "the whole batched solve (all levels, all inputs; algorithmic node-update flops / solve time)"},
+                   "build_s": bt if ok else b64, "solve_s": st if ok else s64,
+                   "reconstructions_per_s": len(inputs) / (t if ok else t64),
+                   "precision": "fp32 sweeps, each colour-launch level's last iteration and the energies fp64" if ok else "fp64",
+                   "fp64": {"ms_per_batch": t64 * 1e3, "build_s": b64, "solve_s": s64},
+                   "max_rel_energy_diff_fp32_vs_fp64": rel,
+                   "solve": "one replayed HIP graph per batch (all depth groups side by side on their own streams)"
+                            if not args.serial else "direct launches (EPR_BATCH_GRAPH=0)",
+                   "autotune": tuned if ok else tuned64,
+                   "unbatched_s": unb, "max_rel_energy_diff_unbatched_vs_fp64_batch": diff,
+                   "max_rel_energy_diff_unbatched_fp64_vs_fp64_batch": diff64,
+                   "whole_solve_tflops_algorithmic": whole},
+        "roofline": {"bound": "mfma", "peak_kind": "fp32 vector ALU (157.3 TFLOP/s published)" if ok
+                     else "fp64 vector ALU (78.6 TFLOP/s, = the FP64 matrix peak)",
+                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
+                     "kernel": f"{kname} (finest level of the {len(deep)} depth-{top} inputs, one colour grid over all of them)",
+                     "us_per_launch": per_launch * 1e6, "nodes_per_launch": nodes / 8,
+                     "hbm_gbs_algorithmic": per_node_bytes * nodes / 8 / per_launch / 1e9,
+                     "traffic_gbs": traffic / per_launch / 1e9 if traffic else None},
         "cpu_baseline": cpu,
     }
+
+
+def batch_main(args, wl):
+    """--config batch3d_256: the batched workload as the JSON line's value."""
+    b = measure_batch(args, args.steps, args.warmup, not args.no_cpu, unbatched=not args.no_unbatched)
+    out = {
+        "metric": METRIC, "value": b["value"], "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": b["value"] * 1e3, "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": b["dtype"], "data": "synthetic", "config": b["config"], "roofline": b["roofline"],
+        "cpu_baseline": b["cpu_baseline"],
+    }
     print(json.dumps(out))
+
+
+def hbm_kernels(rec, depth):
+    """The HBM-bound kernels of the build and the cascade at the finest level
+    (epr_time_op): ms per launch, algorithmic GB/s and the fraction of the
+    8 TB/s HBM peak."""
+    out = {}
+    for op, level in (("prolong", depth - 1), ("galerkin_t", depth), ("galerkin_c", depth), ("energy", depth)):
+        ms, b = rec.time_op(op, level, reps=5)
+        gbs = b / (ms * 1e-3) / 1e9
+        out[op] = {"level": level, "us_per_launch": ms * 1e3, "algorithmic_bytes": b, "gbs": gbs, "frac_hbm_peak": gbs / HBM_PEAK_GBS}
+        log(f"[bench]   {op} (level {level}): {ms * 1e3:.1f} us, {gbs:.0f} GB/s algorithmic ({gbs / HBM_PEAK_GBS:.2f} of HBM peak)")
+    return out
 
 
 def extract_timing(rec, args, dim, reps=3):
@@ -320,6 +401,8 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False, fp32=0)
         log(f"[bench] measured HBM traffic of the finest-sweep kernel: {traffic / 1e6:.2f} MB per launch "
             f"({traffic / per_launch_s / 1e9:.1f} GB/s at this launch time)")
 
+    hbm = hbm_kernels(rec, depth) if not fp32 else None
+
     ext, extract_dir = extract_timing(rec, args, dim) if extract else (None, None)
     cpu = cpu_baseline(name, wl, pos, skew, args.aRadius) if with_cpu else None
     if extract_dir:
@@ -356,6 +439,8 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False, fp32=0)
     }
     if levels:
         res["levels"] = levels
+    if hbm:
+        res["hbm_kernels"] = hbm
     if ext:
         res["extract"] = ext
     return res, rec
@@ -425,8 +510,11 @@ def main():
                     help="default workload: skip the nested learned-initializer measurement (4D Hopf torus)")
     ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
     ap.add_argument("--aRadius", type=int, default=-1, help="-1: dense grid; 0..5: adapted hierarchy of that radius")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="default workload: skip the nested batched-workload (batch3d_256) measurement")
+    ap.add_argument("--steps-batch", type=int, default=5, help="timed steps of the nested batched measurement")
     ap.add_argument("--serial", action="store_true",
-                    help="batch workload: one build thread and the depth groups one after another (for profilers)")
+                    help="batch workload: one build thread and direct launches instead of the recorded graph (for profilers)")
     ap.add_argument("--no-unbatched", action="store_true", help="batch workload: skip the one-at-a-time comparison")
     ap.add_argument("--build-threads", type=int, default=0, help="batch workload: host threads of epr_build_batch (0: four)")
     ap.add_argument("--extract", action="store_true",
@@ -463,6 +551,8 @@ def main():
         cfg["fp64"] = {"ms_per_step": r64["value"] * 1e3, **{k: r64["config"][k] for k in (
             "set_samples_s", "build_s", "solve_s", "final_energy", "energy_rel_err_vs_reference")},
             "roofline": r64["roofline"]}
+        if "hbm_kernels" in r64:
+            cfg["hbm_kernels"] = r64["hbm_kernels"]
         if r32 is not None and not ok:
             cfg["fp32_rejected"] = {"ms_per_step": r32["value"] * 1e3, "energy_rel_err_vs_reference": r32["config"]["energy_rel_err_vs_reference"]}
         return top, r64, cfg, ok
@@ -498,6 +588,9 @@ def main():
             out["clifford4d_d6"]["levels"] = r4["levels"]
     if args.config == "borromean3d_d7" and not args.no_learned:
         out["hopf4d_d6_learned"] = measure_learned(args)
+    # the batched serving workload, in the same run
+    if args.config == "borromean3d_d7" and not args.no_batch and args.aRadius < 0:
+        out["batch3d_256"] = measure_batch(args, args.steps_batch, 1, not args.no_cpu)
     print(json.dumps(out))
 
 

@@ -26,7 +26,7 @@ EXPORTS = [
     "epr_unet_create", "epr_unet_destroy", "epr_unet_last_error", "epr_unet_num_params", "epr_unet_num_nodes",
     "epr_unet_set_params", "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients",
     "epr_unet_set_dataset", "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned", "epr_time_gs_sweeps_fp32",
-    "epr_autotune", "epr_autotune_batch", "epr_tune_get", "epr_tune_set",
+    "epr_autotune", "epr_autotune_batch", "epr_tune_get", "epr_tune_set", "epr_time_op", "epr_time_batch_sweeps",
 ]
 
 
@@ -107,6 +107,8 @@ def load():
     L.epr_autotune_batch.argtypes = [sz, ctypes.POINTER(vp), ci, ctypes.c_char_p, sz]
     L.epr_tune_get.argtypes = [ctypes.c_char_p, sz, ctypes.POINTER(sz)]
     L.epr_tune_set.argtypes = [ctypes.c_char_p]
+    L.epr_time_op.argtypes = [vp, ctypes.c_char_p, ci, ci, dp, dp]
+    L.epr_time_batch_sweeps.argtypes = [sz, ctypes.POINTER(vp), ci, ci, ci, dp, ctypes.POINTER(ci), ctypes.POINTER(ctypes.c_longlong)]
     L.epr_time_gs_sweeps.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
     L.epr_time_gs_sweeps.restype = ctypes.c_int
     L.epr_time_gs_sweeps_fp32.argtypes = [vp, ci, ci, dp, ctypes.POINTER(ci)]
@@ -165,7 +167,8 @@ def load():
                  "epr_gs_colour", "epr_energies_noalloc", "epr_unet_create", "epr_unet_set_params",
                  "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients", "epr_unet_set_dataset",
                  "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned",
-                 "epr_autotune", "epr_autotune_batch", "epr_tune_get", "epr_tune_set"]:
+                 "epr_autotune", "epr_autotune_batch", "epr_tune_get", "epr_tune_set", "epr_time_op",
+                 "epr_time_batch_sweeps"]:
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

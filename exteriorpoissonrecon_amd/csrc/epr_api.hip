@@ -369,6 +369,7 @@ struct CtxBase
 	virtual void writeGrid( const char *path ) = 0;
 	virtual void writeDensity( const char *path ) = 0;
 	virtual std::string autotune( int flags ) = 0;
+	virtual void timeOp( const std::string &op , int level , int reps , double *ms , double *bytes ) = 0;
 	double seconds[2] = { 0 , 0 };
 };
 
@@ -1123,6 +1124,64 @@ struct Ctx : public CtxBase
 		prolong( l , levels[l].y.p , levels[l+1].y.p , 1 );
 		HIPCHECK( hipGetLastError() );
 		HIPCHECK( hipStreamSynchronize( stream ) );
+	}
+
+	// Measurement of the HBM-bound build and transfer kernels: reps launches of
+	// one of them at level l after a warm-up, on scratch outputs (the level's
+	// data is left as it is); ms per launch and the launch's algorithmic HBM
+	// bytes (each input and output value once):
+	//   "prolong"     KProlong, level l -> l+1, out += P x: N_l + 2 N_{l+1} doubles
+	//   "galerkin_t"  KGalerkinT, T = P^T R of level l (> 0): 2 N_l 3^Dim doubles
+	//   "galerkin_c"  KGalerkinC, R_{l-1} = T P from level l's T: (N_l + N_{l-1}) 3^Dim doubles
+	//   "energy"      KEnergy + KSumRows at level l: x, y, R, B (2 N_l + 2 N_l 3^Dim doubles)
+	void timeOp( const std::string &op , int l , int reps , double *ms , double *bytes )
+	{
+		needBuilt() ; checkLevel( l );
+		if( reps<=0 ) throw std::string( "reps must be positive" );
+		Level &L = levels[l];
+		std::function< void( void ) > f;
+		double b = 0;
+		if( op=="prolong" )
+		{
+			if( l>=D ) throw std::string( "prolong: level must be below the depth" );
+			Level &Fl = levels[l+1];
+			dTmp.ensure( (size_t)Fl.N );
+			HIPCHECK( hipMemsetAsync( dTmp.p , 0 , (size_t)Fl.N*sizeof(double) , stream ) );
+			f = [&]{ prolong( l , L.x.p , dTmp.p , 1 ); };
+			b = 8.*( L.N + 2*Fl.N );
+		}
+		else if( op=="galerkin_t" )
+		{
+			if( l<1 ) throw std::string( "galerkin_t: level must be above 0" );
+			dGalT.ensure( (size_t)L.N*W );
+			f = [&]{ galerkinT( L.R.p , L ); };
+			b = 16.*L.N*W;
+		}
+		else if( op=="galerkin_c" )
+		{
+			if( l<1 ) throw std::string( "galerkin_c: level must be above 0" );
+			Level &Cl = levels[l-1];
+			dGalT.ensure( (size_t)L.N*W );
+			galerkinT( L.R.p , L );
+			dTmp2.ensure( (size_t)Cl.N*W );
+			f = [&]{ KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , dTmp2.p , Cl.res , Cl.lay , Cl.fm.p ); };
+			b = 8.*( L.N + Cl.N )*W;
+		}
+		else if( op=="energy" )
+		{
+			const unsigned int nb = Blocks( L.N*C , 256 );
+			dPartials.ensure( (size_t)nb*4 ) , dTmp.ensure( 8 );
+			const LevelView< Dim > v = view( l );
+			f = [&]{ KEnergy< Dim ><<< nb , 256 , 0 , stream >>>( v , dPartials.p ) ; KSumRows<4><<< 1 , 256 , 0 , stream >>>( dPartials.p , nb , dTmp.p ); };
+			b = 8.*( 2*L.N + 2*L.N*W );
+		}
+		else throw std::string( "unknown operation: " + op );
+		f();
+		HIPCHECK( hipGetLastError() );
+		const double t = TimeOnStream( stream , [&]{ for( int r=0 ; r<reps ; r++ ) f(); } );
+		HIPCHECK( hipGetLastError() );
+		if( ms ) *ms = t/reps;
+		if( bytes ) *bytes = b;
 	}
 
 	// (W, yBx, R-form, K-form) at a level
@@ -2411,6 +2470,105 @@ std::string AutotuneBatch( const std::vector< Ctx< Dim > * > &cs , int flags )
 	(void)hipStreamDestroy( st );
 	return rep;
 }
+
+// Measurement of the batched solve's colour grids: iters iterations of level
+// d's colour launches (KGsPhaseBatch, the batched plan's workgroup size) over
+// every dense context of the batch with that level, timed with HIP events on
+// one stream.  fp32: on the float copies (x and y rounded first; the FP64
+// iterate is left as it is); FP64: x and y are restored afterwards.  Returns
+// ms for all launches, the launch count and the node updates per iteration
+// (summed over the contexts).
+template< int Dim >
+void TimeBatchPhases( const std::vector< Ctx< Dim > * > &all , int d , bool fp32 , int iters , double *ms , int *launches , long long *nodes )
+{
+	using Cx = Ctx< Dim >;
+	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W;
+	if( iters<=0 ) throw std::string( "iters must be positive" );
+	std::vector< Cx * > cs;
+	for( Cx *c : all ){ c->needBuilt() ; if( c->D>=d && !c->adapted() && d>=0 ) cs.push_back( c ); }
+	if( cs.empty() ) throw std::string( "no dense context of the batch has this level" );
+	const long long N = cs[0]->levels[d].N;
+	hipStream_t st = nullptr;
+	HIPCHECK( hipStreamCreate( &st ) );
+	DevBuf< char > table;
+	DevBuf< double > save;
+	try
+	{
+		std::vector< char > host;
+		if( fp32 )
+		{
+			std::vector< BatchLevelT< float > > b;
+			for( Cx *c : cs )
+			{
+				auto &L = c->levels[d];
+				c->narrowOperators( d );
+				KNarrow<<< Blocks( N , 256 ) , 256 , 0 , c->stream >>>( L.x.p , L.fx.p , N );
+				KNarrow<<< Blocks( N , 256 ) , 256 , 0 , c->stream >>>( L.y.p , L.fy.p , N );
+				HIPCHECK( hipStreamSynchronize( c->stream ) );
+				b.push_back( { L.fx.p , L.fy.p , L.fR.p , L.fB.p , L.fK.p , nullptr , (float)c->prm.dWeight , c->prm.separate , iters } );
+			}
+			host.assign( (const char *)b.data() , (const char *)( b.data()+b.size() ) );
+		}
+		else
+		{
+			std::vector< BatchLevel > b;
+			save.ensure( (size_t)2*N*cs.size() );
+			for( size_t q=0 ; q<cs.size() ; q++ )
+			{
+				auto &L = cs[q]->levels[d];
+				HIPCHECK( hipMemcpyAsync( save.p + 2*q*N , L.x.p , N*sizeof(double) , hipMemcpyDeviceToDevice , st ) );
+				HIPCHECK( hipMemcpyAsync( save.p + ( 2*q+1 )*N , L.y.p , N*sizeof(double) , hipMemcpyDeviceToDevice , st ) );
+				b.push_back( { L.x.p , L.y.p , L.R.p , L.B.p , L.K.p , nullptr , cs[q]->prm.dWeight , cs[q]->prm.separate , iters } );
+			}
+			host.assign( (const char *)b.data() , (const char *)( b.data()+b.size() ) );
+		}
+		table.alloc( host.size() );
+		HIPCHECK( hipMemcpyAsync( table.p , host.data() , host.size() , hipMemcpyHostToDevice , st ) );
+		HIPCHECK( hipStreamSynchronize( st ) );
+		const LevelView< Dim > L0 = cs[0]->view( d );
+		LevelView< Dim , float > F0;
+		F0.res = L0.res , F0.N = L0.N , F0.x = F0.y = nullptr , F0.R = F0.B = F0.K = nullptr;
+		F0.sW = (float)L0.sW , F0.dW = 0 , F0.lay = L0.lay , F0.fmask = nullptr , F0.separate = 0;
+		for( int k=0 ; k<Dim ; k++ ) F0.stride[k] = L0.stride[k];
+		const int th = fp32 ? cs[0]->floatThreads( d , true ) : cs[0]->gsPlan( d , true ).threads;
+		const bool idx32 = N*W < ( 1LL<<31 );
+		const unsigned k = (unsigned)cs.size();
+		auto run = [&]
+		{
+			for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
+			{
+				ColourSet< Dim > cset ; cset.set( 1<<d , c );
+				const dim3 grid( Blocks( cset.total*C , th ) , k );
+				if( fp32 )
+				{
+					const BatchLevelT< float > *J = (const BatchLevelT< float > *)table.p;
+					if( idx32 ) KGsPhaseBatch< Dim , float , int ><<< grid , th , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
+					else KGsPhaseBatch< Dim , float , long long ><<< grid , th , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
+				}
+				else
+				{
+					const BatchLevel *J = (const BatchLevel *)table.p;
+					if( idx32 ) KGsPhaseBatch< Dim , double , int ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
+					else KGsPhaseBatch< Dim , double , long long ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
+				}
+			}
+		};
+		const double t = TimeOnStream( st , run );
+		HIPCHECK( hipGetLastError() );
+		if( !fp32 ) for( size_t q=0 ; q<cs.size() ; q++ )
+		{
+			auto &L = cs[q]->levels[d];
+			HIPCHECK( hipMemcpyAsync( L.x.p , save.p + 2*q*N , N*sizeof(double) , hipMemcpyDeviceToDevice , st ) );
+			HIPCHECK( hipMemcpyAsync( L.y.p , save.p + ( 2*q+1 )*N , N*sizeof(double) , hipMemcpyDeviceToDevice , st ) );
+		}
+		HIPCHECK( hipStreamSynchronize( st ) );
+		if( ms ) *ms = t;
+		if( launches ) *launches = iters*C;
+		if( nodes ) *nodes = N*(long long)cs.size();
+	}
+	catch( ... ){ (void)hipStreamSynchronize( st ) ; (void)hipStreamDestroy( st ) ; throw; }
+	(void)hipStreamDestroy( st );
+}
 } // namespace epr
 
 struct epr_ctx
@@ -2609,6 +2767,25 @@ int epr_tune_get( char *buf , size_t cap , size_t *len )
 		const std::string t = epr::TuneTable::Get().text();
 		if( len ) *len = t.size();
 		CopyReport( t , buf , cap );
+	} );
+}
+int epr_time_op( epr_ctx *ctx , const char *op , int level , int reps , double *ms , double *bytes ){ IMPL( ctx->impl->timeOp( op ? op : "" , level , reps , ms , bytes ) ); }
+int epr_time_batch_sweeps( size_t n , epr_ctx *const *ctxs , int level , int fp32 , int iters , double *ms , int *launches , long long *nodes )
+{
+	return epr::ContextFree( [&]()
+	{
+		int dim;
+		std::vector< epr::CtxBase * > v = BatchImpls( n , ctxs , true , dim );
+		auto run = [&]( auto tag )
+		{
+			constexpr int Dim = decltype( tag )::value;
+			std::vector< epr::Ctx< Dim > * > c;
+			for( epr::CtxBase *b : v ) c.push_back( static_cast< epr::Ctx< Dim > * >( b ) );
+			epr::TimeBatchPhases< Dim >( c , level , fp32!=0 , iters , ms , launches , nodes );
+		};
+		if( dim==3 ) run( std::integral_constant< int , 3 >() );
+		else if( dim==4 ) run( std::integral_constant< int , 4 >() );
+		else throw std::string( "empty batch" );
 	} );
 }
 int epr_tune_set( const char *text ){ return epr::ContextFree( [&](){ epr::TuneTable::Get().parse( text ); } ); }

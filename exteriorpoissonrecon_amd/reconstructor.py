@@ -157,6 +157,13 @@ class Reconstructor:
     def seconds(self, which):
         return load().epr_last_seconds(self._h, 0 if which == "build" else 1)
 
+    def time_op(self, op, level, reps=10):
+        """(ms per launch, algorithmic HBM bytes) of one kernel at a level
+        (epr_time_op: "prolong", "galerkin_t", "galerkin_c", "energy")."""
+        ms, b = ctypes.c_double(), ctypes.c_double()
+        self._check(load().epr_time_op(self._h, op.encode(), int(level), int(reps), ctypes.byref(ms), ctypes.byref(b)))
+        return ms.value, b.value
+
     def autotune(self, fp32=False):
         """Times every sweep plan of each level of this (built) reconstruction's
         solve and keeps the fastest for single solves (epr_autotune).  Returns
@@ -227,6 +234,16 @@ def autotune_batch(recs, fp32=False):
     buf = ctypes.create_string_buffer(1 << 16)
     _batch_check(load().epr_autotune_batch(len(recs), _handles(recs), 1 if fp32 else 0, buf, len(buf)))
     return _trials(buf.value.decode(), 8)
+
+
+def time_batch_sweeps(recs, level, fp32=False, iters=1):
+    """(ms, launches, node updates per iteration) of `iters` iterations of a
+    level's colour grids as solve_batch runs them (epr_time_batch_sweeps)."""
+    recs = list(recs)
+    ms, nl, nn = ctypes.c_double(), ctypes.c_int(), ctypes.c_longlong()
+    _batch_check(load().epr_time_batch_sweeps(len(recs), _handles(recs), int(level), 1 if fp32 else 0, int(iters),
+                                              ctypes.byref(ms), ctypes.byref(nl), ctypes.byref(nn)))
+    return ms.value, nl.value, nn.value
 
 
 def tune_table():

@@ -309,6 +309,26 @@ void epr_mg_destroy( epr_mg *mg );
  * of known traffic and gives the achievable copy bandwidth. */
 int  epr_time_copy( size_t bytes , int reps , double *ms );
 
+/* Measurement hooks (no reference counterpart), for the HBM roofline of the
+ * transfer and energy kernels and the batched sweep.
+ *
+ * epr_time_op: reps launches (after one warm-up) of one kernel of a built
+ * context at `level`, on scratch outputs (the context's data is unchanged):
+ * "prolong" (KProlong, level -> level+1, out += P x), "galerkin_t"
+ * (KGalerkinT, T = P^T R of the level), "galerkin_c" (KGalerkinC, the coarser
+ * level's R from T), "energy" (KEnergy + its fixed-order row sums).  *ms
+ * receives milliseconds per launch, *bytes the launch's algorithmic HBM
+ * bytes (every input and output value once).
+ *
+ * epr_time_batch_sweeps: iters Gauss-Seidel iterations of `level` as
+ * epr_solve_batch runs them on colour grids (one grid per colour over every
+ * dense context of the batch with that level), FP32 on the float copies when
+ * fp32 is set (the FP64 iterate is left as it is), else FP64 with x and y
+ * restored afterwards; *ms all launches, *launches their number, *nodes the
+ * node updates of one iteration summed over the contexts. */
+int  epr_time_op( epr_ctx *ctx , const char *op , int level , int reps , double *ms , double *bytes );
+int  epr_time_batch_sweeps( size_t n , epr_ctx *const *ctxs , int level , int fp32 , int iters , double *ms , int *launches , long long *nodes );
+
 /* Wall-clock seconds of the last epr_build / epr_solve (device-synchronised). */
 double epr_last_seconds( const epr_ctx *ctx , int which /* 0 build, 1 solve */ );
 

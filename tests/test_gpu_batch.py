@@ -231,3 +231,37 @@ def test_gpu_autotune_plans():
     assert np.array_equal(recs[0].solve(), S0)
     for r in recs:
         r.close()
+
+
+def test_gpu_measurement_hooks_leave_data():
+    """epr_time_op and epr_time_batch_sweeps time kernels on scratch outputs
+    or restore what they sweep: the solved coefficients are unchanged, the
+    byte counts are the documented algorithmic ones, and bad requests are
+    errors."""
+    from exteriorpoissonrecon_amd import EprError, Reconstructor, build_batch, samples, solve_batch, time_batch_sweeps
+    print("case hooks", flush=True)
+    recs = build_batch([Reconstructor(*samples.torus_knot(3000, 2, 3, seed=61), depth=5, aRadius=-1, fp32Sweeps=1),
+                        Reconstructor(*samples.spiral(3000, 4, seed=62), depth=5, aRadius=-1)])
+    solve_batch(recs)
+    before = [[r.coefficients(d) for d in range(6)] for r in recs]
+    r = recs[0]
+    N = [r.num_functions(d) for d in range(6)]
+    for op, level, expect in (("prolong", 4, 8 * (N[4] + 2 * N[5])), ("galerkin_t", 5, 16 * N[5] * 27),
+                              ("galerkin_c", 5, 8 * (N[5] + N[4]) * 27), ("energy", 5, 8 * (2 * N[5] + 2 * N[5] * 27))):
+        ms, b = r.time_op(op, level, reps=2)
+        assert ms > 0 and b == expect, (op, ms, b, expect)
+    for fp32 in (False, True):
+        ms, launches, nodes = time_batch_sweeps(recs, 5, fp32=fp32, iters=2)
+        assert ms > 0 and launches == 16 and nodes == 2 * N[5]
+    for rr, bb in zip(recs, before):
+        for d in range(6):
+            x, y = rr.coefficients(d)
+            assert np.array_equal(x, bb[d][0]) and np.array_equal(y, bb[d][1]), d
+    with pytest.raises(EprError, match="unknown operation"):
+        r.time_op("bogus", 3)
+    with pytest.raises(EprError, match="below the depth"):
+        r.time_op("prolong", 5)
+    with pytest.raises(EprError, match="no dense context"):
+        time_batch_sweeps(recs, 6)
+    for rr in recs:
+        rr.close()
