@@ -448,51 +448,69 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False, fp32=0)
 
 def measure_learned(args):
     """The learned initializer (4D Hopf torus, depth 6): time-to-energy of the
-    plain solver and of the network's start, in the same harness
-    (learned.time_to_energy), best of three each; and the network's forward
-    pass on the device (MFMA convolutions) as algorithmic BF16 TFLOP/s."""
+    plain solver (the reference's cascade from its random start at level 0)
+    and of the network's start at level 4, in the headline precision (FP32
+    sweeps with each level's last iteration FP64) and in FP64.  E_ref is the
+    plain FP64 solve's final energy; the target E_ref (1 + 1e-6).  The learned
+    form's schedule of levels 4 and 5 is searched (counts 5, 10, 20 each;
+    learned.search_schedule); the plain form is timed on the reference's
+    schedule, and, for contrast, on that schedule with every coarse count
+    scaled down (1/2 .. 1/8) as far as it still reaches the target.  Also the
+    network's forward pass on the device (MFMA convolutions) as algorithmic
+    BF16 TFLOP/s."""
     from exteriorpoissonrecon_amd import Reconstructor, learned, samples
     if not os.path.exists(learned.WEIGHTS_4D):
         return None
     net = learned.UNet.load(learned.WEIGHTS_4D)
     pos, skew = samples.hopf_torus(256, lobes=3, amp=0.3, latitude="sin")
     log(f"[bench] hopf4d_d6_learned: {len(pos)} samples, dim 4, depth 6")
-    rec = Reconstructor(pos, skew, depth=6, aRadius=-1).build()
-    E_ref = float(rec.solve().sum())
-    solves = []
-    for _ in range(3):
-        rec.solve()
-        solves.append(rec.seconds("solve"))
-    solve_s = min(solves)
-    target = E_ref * (1 + 1e-6)
-    best = lambda runs: min(runs, key=lambda r: r[0] if r[0] is not None else 1e9)
-    plain = best([learned.time_to_energy(rec, target, None) for _ in range(3)])
-    lrn = best([learned.time_to_energy(rec, target, net, max_finest=40) for _ in range(3)])
-    X = learned.features(rec, 4)
-    net.forward(X)
-    t0 = time.perf_counter()
-    for _ in range(5):
-        net.forward(X)
-    fwd_s = (time.perf_counter() - t0) / 5
-    t0 = time.perf_counter()
-    for _ in range(5):
-        learned.init_learned(rec, net, 4)
-    init_s = (time.perf_counter() - t0) / 5
-    flops = learned.forward_flops(4, 4, 6)
-    E_l = float(learned.solve_learned(rec, net).sum())
-    log(f"[bench] learned: plain {plain[0]:.3f} s ({plain[1]} finest iterations), learned {lrn[0]} s "
-        f"({lrn[1]}), E_ref {E_ref:.12g}, forward {fwd_s * 1e3:.2f} ms ({flops / fwd_s / 1e12:.2f} TFLOP/s bf16), "
-        f"learned start {init_s * 1e3:.2f} ms")
-    rec.close(); net.close()
-    return {"workload": "hopf4d_d6 (Hopf torus of theta = pi/2 + 0.3 sin 3s, 65536 samples, depth 6, dense)",
-            "reference_energy": E_ref, "target": target, "plain_solve_s": solve_s,
-            "plain_time_to_energy_s": plain[0], "plain_finest_iterations": plain[1],
-            "learned_time_to_energy_s": lrn[0], "learned_finest_iterations": lrn[1],
-            "learned_final_energy": lrn[2], "speedup": (plain[0] / lrn[0]) if lrn[0] else None,
-            "solve_learned_energy": E_l, "learned_start_ms": init_s * 1e3,
-            "unet_forward_ms_host_io": fwd_s * 1e3, "unet_forward_flops": flops,
-            "unet_forward_tflops_bf16": flops / fwd_s / 1e12, "bf16_dense_peak_tflops": 2500.0,
-            "harness": "reference schedule below the finest level, then the finest iterations; the count that reaches the target is found with an energy check after each iteration, then that schedule alone is timed"}
+    out = {"workload": "hopf4d_d6 (Hopf torus of theta = pi/2 + 0.3 sin 3s, 65536 samples, depth 6, dense)"}
+    E_ref = None
+    for fp32 in (1, 0):
+        rec = Reconstructor(pos, skew, depth=6, aRadius=-1, fp32Sweeps=fp32).build()
+        if E_ref is None:
+            r64 = Reconstructor(pos, skew, depth=6, aRadius=-1).build()
+            E_ref = float(r64.solve().sum())
+            r64.close()
+        target = E_ref * (1 + 1e-6)
+        plain = learned.time_to_energy(rec, target, None, reps=3)
+        scaled, _ = learned.search_schedule(rec, target=target, net=None)
+        lrn, trials = learned.search_schedule(rec, net, target, grid=(5, 10, 20))
+        name = "fp32" if fp32 else "fp64"
+        res = {"plain_time_to_energy_s": plain[0], "plain_finest_iterations": plain[1], "plain_final_energy": plain[2],
+               "plain_schedule": plain[3],
+               "learned_time_to_energy_s": lrn[0] if lrn else None, "learned_finest_iterations": lrn[1] if lrn else None,
+               "learned_final_energy": lrn[2] if lrn else None, "learned_schedule": lrn[3] if lrn else None,
+               "speedup_vs_reference_schedule": (plain[0] / lrn[0]) if lrn and plain[0] else None,
+               "plain_scaled_time_to_energy_s": scaled[0] if scaled else None,
+               "plain_scaled_schedule": scaled[3] if scaled else None,
+               "speedup_vs_scaled_plain": (scaled[0] / lrn[0]) if lrn and scaled else None,
+               "learned_schedules_tried": [{"schedule": t[3], "s": t[0], "finest": t[1]} for t in trials]}
+        out[name] = res
+        log(f"[bench] learned ({name}): plain {plain[0]:.3f} s (reference schedule, {plain[1]} finest iterations); "
+            f"plain scaled {scaled[0] if scaled else None} s {scaled[3] if scaled else ''}; "
+            f"learned {lrn[0] if lrn else None} s {lrn[3] if lrn else ''} -> x{res['speedup_vs_reference_schedule']}")
+        if fp32:
+            X = learned.features(rec, 4)
+            net.forward(X)
+            t0 = time.perf_counter()
+            for _ in range(5):
+                net.forward(X)
+            fwd_s = (time.perf_counter() - t0) / 5
+            t0 = time.perf_counter()
+            for _ in range(5):
+                learned.init_learned(rec, net, 4)
+            init_s = (time.perf_counter() - t0) / 5
+            flops = learned.forward_flops(4, 4, 6)
+            out.update({"learned_start_ms": init_s * 1e3, "unet_forward_ms_host_io": fwd_s * 1e3,
+                        "unet_forward_flops": flops, "unet_forward_tflops_bf16": flops / fwd_s / 1e12,
+                        "bf16_dense_peak_tflops": 2500.0})
+        rec.close()
+    net.close()
+    out.update({"reference_energy": E_ref, "target": E_ref * (1 + 1e-6),
+                "speedup": out["fp32"]["speedup_vs_reference_schedule"],
+                "harness": "learned.time_to_energy: the finest iteration count that reaches the target is found with an FP64 energy check after each iteration, then the whole schedule runs as one epr_solve_schedule call in the stated precision (best of several); the learned form's level-4/5 counts are searched, the plain form runs the reference's schedule"})
+    return out
 
 
 def main():

@@ -27,6 +27,7 @@ EXPORTS = [
     "epr_unet_set_params", "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients",
     "epr_unet_set_dataset", "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned", "epr_time_gs_sweeps_fp32",
     "epr_autotune", "epr_autotune_batch", "epr_tune_get", "epr_tune_set", "epr_time_op", "epr_time_batch_sweeps",
+    "epr_solve_schedule",
 ]
 
 
@@ -88,6 +89,7 @@ def load():
     L.epr_learned_features.argtypes = [vp, ci, fp]
     L.epr_init_learned.argtypes = [vp, vp, ci, dp]
     L.epr_solve_learned.argtypes = [vp, vp, dp]
+    L.epr_solve_schedule.argtypes = [vp, vp, ctypes.POINTER(ci), dp]
     L.epr_energies.argtypes = [vp, ci, dp]
     L.epr_solve.argtypes = [vp, dp]
     L.epr_num_functions.argtypes = [vp, ci]
@@ -168,7 +170,7 @@ def load():
                  "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients", "epr_unet_set_dataset",
                  "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned",
                  "epr_autotune", "epr_autotune_batch", "epr_tune_get", "epr_tune_set", "epr_time_op",
-                 "epr_time_batch_sweeps"]:
+                 "epr_time_batch_sweeps", "epr_solve_schedule"]:
         getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L

@@ -359,6 +359,7 @@ struct CtxBase
 	virtual void learnedFeatures( int level , float *out ) = 0;
 	virtual void learnedInit( int level , unet::Net *net , double info[3] ) = 0;
 	virtual void solveLearned( unet::Net *net , double out[2] ) = 0;
+	virtual void solveSchedule( unet::Net *net , const int *iters , double out[2] ) = 0;
 	virtual size_t numFunctions( int level ) const = 0;
 	virtual void getCoeffs( int level , double *x , double *y ) = 0;
 	virtual void setCoeffs( int level , const double *x , const double *y ) = 0;
@@ -1335,7 +1336,9 @@ struct Ctx : public CtxBase
 
 	// the cascadic schedule (ExteriorPoisson.h:316-409)
 	void solve( double out[2] ){ solveFrom( out , nullptr ); }
-	void solveFrom( double out[2] , unet::Net *net )
+	// iters (optional): the iteration count of every level d in [start, maxSolveDepth]
+	// at iters[d] instead of the reference's ceil( iMult^(D-d) gsIters )
+	void solveFrom( double out[2] , unet::Net *net , const int *sched=nullptr )
 	{
 		needBuilt();
 		auto t0 = std::chrono::steady_clock::now();
@@ -1354,12 +1357,13 @@ struct Ctx : public CtxBase
 		}
 		if( net ) learnedInit( lo , net , nullptr );
 		else init( lo , nullptr , nullptr );
-		auto Iters = [&]( int d ){ return (int)std::ceil( std::pow( prm.iMult , D-d ) * prm.gsIters ); };
+		auto Iters = [&]( int d ){ return sched ? sched[d] : (int)std::ceil( std::pow( prm.iMult , D-d ) * prm.gsIters ); };
+		if( sched ) for( int d=lo ; d<=hi ; d++ ) if( sched[d]<0 ) throw std::string( "negative iteration count" );
 		if( prm.singleLevel )
 		{
 			for( int d=lo ; d<hi ; d++ ){ prolong( d , levels[d].x.p , levels[d+1].x.p , 0 ) ; prolong( d , levels[d].y.p , levels[d+1].y.p , 0 ); }
 			for( int d=lo ; d<hi ; d++ ) levels[d].x.zero( stream ) , levels[d].y.zero( stream );
-			gs( hi , prm.gsIters );
+			gs( hi , sched ? sched[hi] : prm.gsIters );
 		}
 		else for( int d=lo ; d<=hi ; d++ )
 		{
@@ -1452,6 +1456,7 @@ struct Ctx : public CtxBase
 
 	// the cascade of solve() started at the network's level from its prediction
 	void solveLearned( unet::Net *net , double out[2] ){ solveFrom( out , net ); }
+	void solveSchedule( unet::Net *net , const int *iters , double out[2] ){ solveFrom( out , net , iters ); }
 
 	void getCoeffs( int l , double *x , double *y )
 	{
@@ -2662,6 +2667,11 @@ int epr_energies_noalloc( epr_ctx *ctx , int level , double out[2] ){ if( !out )
 int epr_solve( epr_ctx *ctx , double out_E[2] ){ IMPL( ctx->impl->solve( out_E ) ); }
 int epr_learned_features( epr_ctx *ctx , int level , float *out ){ if( !out ) return EPR_ERR_ARG ; IMPL( ctx->impl->learnedFeatures( level , out ) ); }
 int epr_init_learned( epr_ctx *ctx , epr_unet *net , int level , double info[3] ){ if( !EprUnetNet( net ) ) return EPR_ERR_ARG ; IMPL( ctx->impl->learnedInit( level , EprUnetNet( net ) , info ) ); }
+int epr_solve_schedule( epr_ctx *ctx , epr_unet *net , const int *iters , double out_E[2] )
+{
+	if( net && !EprUnetNet( net ) ) return EPR_ERR_ARG;
+	IMPL( ctx->impl->solveSchedule( net ? EprUnetNet( net ) : nullptr , iters , out_E ) );
+}
 int epr_solve_learned( epr_ctx *ctx , epr_unet *net , double out_E[2] ){ if( !EprUnetNet( net ) ) return EPR_ERR_ARG ; IMPL( ctx->impl->solveLearned( EprUnetNet( net ) , out_E ) ); }
 size_t epr_num_functions( const epr_ctx *ctx , int level ){ return ( ctx && ctx->impl && level>=0 && level<=ctx->prm.depth ) ? ctx->impl->numFunctions( level ) : 0; }
 int epr_get_coeffs( epr_ctx *ctx , int level , double *x , double *y ){ IMPL( ctx->impl->getCoeffs( level , x , y ) ); }

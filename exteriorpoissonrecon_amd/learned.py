@@ -289,52 +289,100 @@ def solve_learned(rec, net):
     return E
 
 
-def _run(rec, net, finest, check=None):
-    """The cascade from the random start at level 0 (net None) or from the
-    network's start at its level: the reference's schedule below the finest
-    level (iters(d) = ceil(iMult^(D-d) gsIters)), then `finest` single
-    finest-level iterations; with check, the energy after each, stopping at
-    the first <= check.  Returns (finest iterations run, final energy)."""
-    D = rec.depth
+def reference_iters(rec):
+    """The reference's iteration count of every level, ceil(iMult^(D-d) gsIters)."""
     p = rec.params
-    iters = lambda d: int(np.ceil(p.iMult ** (D - d) * p.gsIters))
+    return [int(np.ceil(p.iMult ** (rec.depth - d) * p.gsIters)) for d in range(rec.depth + 1)]
+
+
+def solve_schedule(rec, net, iters):
+    """The cascade (epr_solve_schedule) from the random start (net None) or the
+    network's start, with iters[d] Gauss-Seidel iterations at level d (all
+    depth+1 counts; the levels below the start are ignored).  With
+    fp32Sweeps a colour-launch level runs all its iterations but the last in
+    FP32.  Returns (E_fit, E_reg) at the finest level."""
+    import ctypes
+    E = np.zeros(2)
+    it = (ctypes.c_int * (rec.depth + 1))(*[int(v) for v in iters])
+    rec._check(load().epr_solve_schedule(rec._h, net._h if net is not None else None, it, _p(E)))
+    return E
+
+
+def _first_crossing(rec, net, coarse, max_finest, target, start_level=0):
+    """The first finest-iteration count whose energy is <= target: the start
+    (the network's at its level, or the random one at start_level), the
+    coarse levels with the given counts, then single FP64 finest iterations
+    with an energy check after each (None when max_finest do not reach it)."""
+    D = rec.depth
+    lo = net.level if net is not None else start_level
     if net is None:
-        lo = 0
         rec.init(lo)
     else:
-        lo = net.level
         init_learned(rec, net, lo)
     for d in range(lo, D):
-        rec.gauss_seidel(d, iters(d))
+        if coarse[d]:
+            rec.gauss_seidel(d, coarse[d])
         rec.prolong_add(d)
-    if check is None:
-        rec.gauss_seidel(D, finest)
-        return finest, rec.energies(D).sum()
-    E = None
-    for k in range(1, finest + 1):
+    for k in range(1, max_finest + 1):
         rec.gauss_seidel(D, 1)
-        E = rec.energies(D).sum()
-        if E <= check:
-            return k, E
-    return None, E
+        if rec.energies(D).sum() <= target:
+            return k
+    return None
 
 
-def time_to_energy(rec, target, net=None, max_finest=40):
-    """Time to reach the FP64 energy target from the random start (net None)
-    or the network's start.  A first run checks the finest level's energy
-    after every iteration to find the iteration count k that reaches the
-    target; a second run does exactly that schedule with k finest
-    iterations and one energy evaluation at the end, and is the one timed.
-    Returns (seconds, k, final energy); seconds is None when the target is
-    not reached within max_finest finest iterations."""
-    k, E = _run(rec, net, max_finest, check=target)
+def time_to_energy(rec, target, net=None, coarse=None, max_finest=40, reps=3):
+    """Time to reach the FP64 energy target from the random start at level 0
+    (net None) or the network's start at its level.  coarse: the iteration
+    counts of the levels below the finest (None: the reference's).  A first
+    run checks the energy after every finest iteration (FP64) to find the
+    count k that reaches the target; then the whole schedule with k finest
+    iterations runs as one epr_solve_schedule call (the precision of the
+    reconstruction: with fp32Sweeps all but the last finest iteration are
+    FP32), best of reps; if its final energy misses the target, k grows by one.
+    Returns (seconds, k, final energy, iteration counts); seconds is None when
+    max_finest iterations do not reach the target."""
+    coarse = list(reference_iters(rec) if coarse is None else coarse)
+    k = _first_crossing(rec, net, coarse, max_finest, target)
     if k is None:
-        return None, max_finest, E
-    t0 = time.perf_counter()
-    k2, E2 = _run(rec, net, k)
-    dt = time.perf_counter() - t0
-    assert E2 == E, (E2, E)
-    return dt, k, E2
+        return None, max_finest, None, coarse
+    while k <= max_finest:
+        iters = coarse[:rec.depth] + [k]
+        best, E = None, None
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            E = float(solve_schedule(rec, net, iters).sum())
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        if E <= target:
+            return best, k, E, iters
+        k += 1
+    return None, max_finest, E, coarse
+
+
+def search_schedule(rec, net, target, grid=(5, 10, 20, 40), max_finest=40, reps=2):
+    """The fastest schedule of the levels between the network's level and the
+    finest (every level's count from grid) with which the learned start
+    reaches the target, by time_to_energy on each; (net None: the reference's
+    schedule with every coarse count scaled by 1, 1/2, 1/4 or 1/8, for the
+    plain form).  Returns the best time_to_energy result and all trials."""
+    D = rec.depth
+    trials = []
+    if net is None:
+        ref = reference_iters(rec)
+        cands = [[max(1, int(np.ceil(v * f))) for v in ref[:D]] + [ref[D]] for f in (1, 0.5, 0.25, 0.125)]
+    else:
+        import itertools
+        L = net.level
+        cands = []
+        for combo in itertools.product(grid, repeat=D - L):
+            c = [0] * L + list(combo) + [0]
+            cands.append(c)
+    for c in cands:
+        r = time_to_energy(rec, target, net, c, max_finest, reps)
+        trials.append(r)
+    ok = [r for r in trials if r[0] is not None]
+    best = min(ok, key=lambda r: r[0]) if ok else None
+    return best, trials
 
 
 def main(argv=None):

@@ -458,6 +458,12 @@ int  epr_init_learned( epr_ctx *ctx , epr_unet *net , int level , double info[3]
 /* epr_solve with the cascade started at the network's level from its
  * prediction instead of the random start at minSolveDepth. */
 int  epr_solve_learned( epr_ctx *ctx , epr_unet *net , double out_E[2] );
+/* epr_solve (net NULL) or epr_solve_learned (net given) with the iteration
+ * count of every level d of the cascade at iters[d] (depth+1 entries; NULL:
+ * the reference's ceil( iMult^(depth-d) gsIters )).  No reference
+ * counterpart: the learned start needs far fewer coarse iterations than the
+ * reference's schedule gives its random start. */
+int  epr_solve_schedule( epr_ctx *ctx , epr_unet *net , const int *iters , double out_E[2] );
 
 /* typeid( MishaK::Hat::Hierarchical{Regular,Adapted}Indexer< dim > ).hash_code()
  * of a libstdc++ build of the reference: the value a .tree carries after its
