@@ -277,6 +277,25 @@ inline void SetLdsAttributes( void )
 	if( failed[dev] ) std::rethrow_exception( failed[dev] );
 }
 
+// The energy terms of a level: KEnergy's partial-sum rows (the window terms),
+// then KEnergyWedge's (the wedge term), summed in a fixed order into out[4]
+// (W, y^T B x, R-form, K-form).  EnergyRows: the rows partials must hold.
+template< int Dim > long long EnergyRows( long long N , int res )
+{
+	constexpr int C = DimC< Dim >::C , NP = Dim*( Dim-1 )/2;
+	long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= res;
+	return (long long)Blocks( N*C , 256 ) + (long long)Blocks( ne*NP , 256 );
+}
+template< int Dim > void LaunchEnergy( const LevelView< Dim > &v , double *partials , double *out , hipStream_t st )
+{
+	constexpr int C = DimC< Dim >::C , NP = Dim*( Dim-1 )/2;
+	long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= v.res;
+	const unsigned nbW = Blocks( v.N*C , 256 ) , nbE = Blocks( ne*NP , 256 );
+	KEnergy< Dim ><<< nbW , 256 , 0 , st >>>( v , partials );
+	KEnergyWedge< Dim ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
+	KSumRows<4><<< 1 , 256 , 0 , st >>>( partials , (long long)nbW + nbE , out );
+}
+
 // ---- tuned sweep plans (epr_autotune, epr_autotune_batch)
 // Per device, dimension, level resolution, single/batched solve and precision:
 // the sweep kernel of the level (kind, one of Ctx::GS_*; -1 keeps the built-in
@@ -1170,10 +1189,9 @@ struct Ctx : public CtxBase
 		}
 		else if( op=="energy" )
 		{
-			const unsigned int nb = Blocks( L.N*C , 256 );
-			dPartials.ensure( (size_t)nb*4 ) , dTmp.ensure( 8 );
+			dPartials.ensure( (size_t)EnergyRows< Dim >( L.N , L.res )*4 ) , dTmp.ensure( 8 );
 			const LevelView< Dim > v = view( l );
-			f = [&]{ KEnergy< Dim ><<< nb , 256 , 0 , stream >>>( v , dPartials.p ) ; KSumRows<4><<< 1 , 256 , 0 , stream >>>( dPartials.p , nb , dTmp.p ); };
+			f = [&]{ LaunchEnergy< Dim >( v , dPartials.p , dTmp.p , stream ); };
 			b = 8.*( 2*L.N + 2*L.N*W );
 		}
 		else throw std::string( "unknown operation: " + op );
@@ -1189,12 +1207,9 @@ struct Ctx : public CtxBase
 	void energyTerms( int l , double t[4] )
 	{
 		LevelView< Dim > v = view( l );
-		long long threads = levels[l].N*C;
-		const unsigned int nb = Blocks( threads , 256 );
-		dPartials.ensure( (size_t)nb*4 );
+		dPartials.ensure( (size_t)EnergyRows< Dim >( levels[l].N , levels[l].res )*4 );
 		dTmp.ensure( 8 );
-		KEnergy< Dim ><<< nb , 256 , 0 , stream >>>( v , dPartials.p );
-		KSumRows<4><<< 1 , 256 , 0 , stream >>>( dPartials.p , nb , dTmp.p );
+		LaunchEnergy< Dim >( v , dPartials.p , dTmp.p , stream );
 		dTmp.download( t , 4 , stream );
 		HIPCHECK( hipStreamSynchronize( stream ) );
 	}
@@ -1216,13 +1231,11 @@ struct Ctx : public CtxBase
 	{
 		needBuilt() ; checkLevel( l );
 		LevelView< Dim > v = view( l );
-		const long long threads = levels[l].N*C;
-		const long long nb = ( threads + 255 )/256;
+		const long long nb = EnergyRows< Dim >( levels[l].N , levels[l].res );
 		if( nb>NoAllocMaxBlocks )
-			throw std::string( "level too large for epr_energies_noalloc: " + std::to_string( nb ) + " partial sums, the scratch holds " + std::to_string( NoAllocMaxBlocks ) + " (at most " + std::to_string( NoAllocMaxBlocks*256/C ) + " functions)" );
+			throw std::string( "level too large for epr_energies_noalloc: " + std::to_string( nb ) + " partial sums, the scratch holds " + std::to_string( NoAllocMaxBlocks ) );
 		double *sp = NoAllocScratch();
-		KEnergy< Dim ><<< (unsigned)nb , 256 , 0 , stream >>>( v , sp + 4 );
-		KSumRows<4><<< 1 , 256 , 0 , stream >>>( sp + 4 , nb , sp );
+		LaunchEnergy< Dim >( v , sp + 4 , sp , stream );
 		HIPCHECK( hipGetLastError() );
 		double t[4];
 		HIPCHECK( hipMemcpyAsync( t , sp , 4*sizeof(double) , hipMemcpyDeviceToHost , stream ) );
@@ -1955,7 +1968,7 @@ struct BatchGroup
 		partOff.resize( n );
 		for( size_t r=0 ; r<n ; r++ )
 		{
-			const long long nb = std::max( Blocks( lev( r , lo[r] ).N*C , 256 ) , Blocks( lev( r , hi[r] ).N*C , 256 ) );
+			const long long nb = std::max( EnergyRows< Dim >( lev( r , lo[r] ).N , lev( r , lo[r] ).res ) , EnergyRows< Dim >( lev( r , hi[r] ).N , lev( r , hi[r] ).res ) );
 			partOff[r] = H.partials , H.partials += (size_t)nb*4;
 		}
 
@@ -2138,10 +2151,7 @@ struct BatchGroup
 		{
 			for( size_t r=0 ; r<n ; r++ )
 			{
-				const unsigned int nb = Blocks( lev( r , level[r] ).N*C , 256 );
-				double *part = P.partials + partOff[r];
-				KEnergy< Dim ><<< nb , 256 , 0 , st >>>( cs[r]->view( level[r] ) , part );
-				KSumRows<4><<< 1 , 256 , 0 , st >>>( part , nb , P.terms + 4*ids[r] );
+				LaunchEnergy< Dim >( cs[r]->view( level[r] ) , P.partials + partOff[r] , P.terms + 4*ids[r] , st );
 			}
 		};
 		// ---- zero, random start, rescale s = ( c / W(x,y) )^(1/4)

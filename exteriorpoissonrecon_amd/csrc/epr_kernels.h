@@ -149,6 +149,90 @@ __device__ __forceinline__ void ElementWedge( const Real *xp , const Real *yp , 
 	q20 = a20 , q02 = a02 , q11 = Real( -2 )*a11 , q10 = Real( 2 )*b10 , q01 = Real( -2 )*b01 , zy = b10;
 }
 
+// The wedge energy of one element for one coordinate pair (i, j):
+// (1/2) int_e F_ij^2, F_ij = d_i x d_j y - d_j x d_i y, at unit resolution,
+// for multilinear x, y with corner values xc, yc whose index bit q stands for
+// coordinate order[q], order = (i, j, the other coordinates ascending).  The
+// element's wedge energy (the quartic form of the corner-0 tables, summed
+// over the corners) is the sum of these over the pairs: the reference's
+// 2-form inner product of grad x ^ grad y with itself is (1/2) sum_{i<j}
+// F_ij^2 (Include/Hat.inl:32-37, 358-370).  d_i x is multilinear in the other
+// coordinates, so F_ij = sum_{a,b,m} c[a,b,m] psi_a(t_i) psi_b(t_j) prod_q
+// e_{m_q}(t_q) in the hat basis psi_0 = 1-t, psi_1 = t and the products
+// e_0 = psi_0^2, e_1 = psi_0 psi_1, e_2 = psi_1^2 for the other coordinates;
+// int F^2 = c^T (G (x) G (x) H (x) ...) c with the 1D Gram matrices
+// G = [[1/3,1/6],[1/6,1/3]] and H_{mm'} = (m+m')!(4-m-m')!/5!, evaluated as
+// |U c|^2 with U the Kronecker product of their upper Cholesky factors,
+// applied in place along each axis.  About 4 3^(Dim-2) (8 + 2 Dim) flops per
+// pair instead of the corner-0 contractions of every corner.
+constexpr double UG00 = 0.5773502691896257 , UG01 = 0.28867513459481287 , UG11 = 0.5;
+constexpr double UH00 = 0.4472135954999579 , UH01 = 0.1118033988749895 , UH02 = 0.07453559924999299;
+constexpr double UH11 = 0.14433756729740643 , UH12 = 0.2886751345948129 , UH22 = 0.3333333333333333;
+template< int Dim >
+__device__ __forceinline__ double PairWedgeEnergy( const double *xc , const double *yc )
+{
+	constexpr int NO = Dim-2 , R = 1<<NO , M3 = NO==1 ? 3 : 9;
+	// differences along i (indexed by b_j and the other bits) and along j (by b_i and the others)
+	double dxi[2][R] , dyi[2][R] , dxj[2][R] , dyj[2][R];
+#pragma unroll
+	for( int b=0 ; b<2 ; b++ )
+#pragma unroll
+		for( int r=0 ; r<R ; r++ )
+		{
+			dxi[b][r] = xc[ 1 | (b<<1) | (r<<2) ] - xc[ (b<<1) | (r<<2) ];
+			dyi[b][r] = yc[ 1 | (b<<1) | (r<<2) ] - yc[ (b<<1) | (r<<2) ];
+			dxj[b][r] = xc[ b | 2 | (r<<2) ] - xc[ b | (r<<2) ];
+			dyj[b][r] = yc[ b | 2 | (r<<2) ] - yc[ b | (r<<2) ];
+		}
+	// c[(a + 2b) M3 + sum_q m_q 3^q]
+	double c[4*M3];
+#pragma unroll
+	for( int k=0 ; k<4*M3 ; k++ ) c[k] = 0;
+#pragma unroll
+	for( int r=0 ; r<R ; r++ )
+#pragma unroll
+		for( int rp=0 ; rp<R ; rp++ )
+		{
+			int m = 0 , p3 = 1;
+#pragma unroll
+			for( int q=0 ; q<NO ; q++ ){ m += ( ( (r>>q)&1 ) + ( (rp>>q)&1 ) )*p3 ; p3 *= 3; }
+#pragma unroll
+			for( int a=0 ; a<2 ; a++ )
+#pragma unroll
+				for( int b=0 ; b<2 ; b++ ) c[ (a+2*b)*M3 + m ] += dxi[b][r]*dyj[a][rp] - dxj[a][r]*dyi[b][rp];
+		}
+	// U along a (stride M3), b (stride 2 M3), then each other coordinate's digit
+#pragma unroll
+	for( int k=0 ; k<M3 ; k++ )
+#pragma unroll
+		for( int b=0 ; b<2 ; b++ )
+		{
+			double &v0 = c[ (2*b)*M3 + k ] , &v1 = c[ (1+2*b)*M3 + k ];
+			v0 = UG00*v0 + UG01*v1 , v1 = UG11*v1;
+		}
+#pragma unroll
+	for( int k=0 ; k<M3 ; k++ )
+#pragma unroll
+		for( int a=0 ; a<2 ; a++ )
+		{
+			double &v0 = c[ a*M3 + k ] , &v1 = c[ (a+2)*M3 + k ];
+			v0 = UG00*v0 + UG01*v1 , v1 = UG11*v1;
+		}
+#pragma unroll
+	for( int q=0 , p3=1 ; q<NO ; q++ , p3*=3 )
+#pragma unroll
+		for( int k=0 ; k<4*M3 ; k++ )
+			if( ( ( k%M3 )/p3 )%3==0 )
+			{
+				double &v0 = c[k] , &v1 = c[k+p3] , &v2 = c[k+2*p3];
+				v0 = UH00*v0 + UH01*v1 + UH02*v2 , v1 = UH11*v1 + UH12*v2 , v2 = UH22*v2;
+			}
+	double s = 0;
+#pragma unroll
+	for( int k=0 ; k<4*M3 ; k++ ) s += c[k]*c[k];
+	return 0.5*s;
+}
+
 // The omega=1/2 block-Newton step of ExteriorPoissonSolvers.h:173-181 with the
 // 2x2 inverse of Misha/Geometry.inl:319-331.
 template< typename Real >
@@ -246,7 +330,8 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , co
 		for( int m=0 ; m<NWL ; m++ ) EPR_LOAD_WINDOW( m )
 	}
 	const Real xi = L.x[i] , yi = L.y[i];
-	// ---- the incident element of this lane
+	// ---- the incident element of this lane (the energy's wedge term is KEnergyWedge's)
+	if constexpr( MODE!=MODE_ENERGY )
 	{
 		bool valid = true;
 		Idx sig[Dim];
@@ -272,7 +357,7 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , co
 			Real a20 , a02 , a11 , a10 , a01 , zy;
 			ElementWedge< Dim , Real >( xp , yp , a20 , a02 , a11 , a10 , a01 , zy );
 			q20 = a20*L.sW , q02 = a02*L.sW , q11 = a11*L.sW , q10 = a10*L.sW , q01 = a01*L.sW;
-			if constexpr( MODE==MODE_ENERGY ) eW = xp[0]*zy*L.sW;
+			(void)zy;
 		}
 	}
 #pragma unroll
@@ -439,22 +524,30 @@ __global__ void __launch_bounds__( 256 ) KQFit( LevelView< Dim > L , double *q )
 	if( lane==0 ) for( int k=0 ; k<5 ; k++ ) q[5*i+k] = a[k];
 }
 
-// Energy terms: per block partial sums of (W, y^T B x, x^T R x + y^T R y, x^T K x + y^T K y).
-// W = 1/2 sum_i x_i dW/dx_i (W is homogeneous of degree 2 in x).
+// Energy terms as per-block partial-sum rows (W, y^T B x, x^T R x + y^T R y,
+// x^T K x + y^T K y), summed in a fixed order by KSumRows< 4 >: KEnergy writes
+// the window terms' rows (W = 0), KEnergyWedge the wedge term's rows after them.
+// KEnergy: the nodes in the column order of R and B (colour by colour), so
+// the node groups of a wavefront read consecutive columns.
 template< int Dim >
 __global__ void __launch_bounds__( 256 ) KEnergy( LevelView< Dim > L , double *partials )
 {
 	constexpr int C = DimC< Dim >::C , G = 256/C;
 	__shared__ double sh[G][4];
 	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-	const long long i = g / C;
+	const long long k = g / C;   // column
 	const int lane = (int)( g % C );
 	double a[5] = { 0 , 0 , 0 , 0 , 0 };
-	if( i<L.N )
+	if( k<L.N )
 	{
+		int c = 0;
+#pragma unroll
+		for( int q=1 ; q<C ; q++ ) c += k>=L.lay.cstart[q] ? 1 : 0;
+		ColourSet< Dim > cs ; cs.set( L.res , c );
 		int F[Dim];
-		Unflatten< Dim >( i , L.res , F );
-		NodeGroup< Dim , MODE_ENERGY , Dim==3 >( L , F , i , L.lay.col( F ) , lane , a );
+		cs.node( k - L.lay.cstart[c] , F );
+		const long long i = FlatIndex< Dim >( F , L.stride );
+		NodeGroup< Dim , MODE_ENERGY , Dim==3 >( L , F , i , k , lane , a );
 	}
 	if( lane==0 ) for( int k=0 ; k<4 ; k++ ) sh[ threadIdx.x/C ][k] = a[k];
 	__syncthreads();
@@ -464,6 +557,57 @@ __global__ void __launch_bounds__( 256 ) KEnergy( LevelView< Dim > L , double *p
 		for( int gidx=0 ; gidx<G ; gidx++ ) for( int k=0 ; k<4 ; k++ ) s[k] += sh[gidx][k];
 		for( int k=0 ; k<4 ; k++ ) partials[ (long long)blockIdx.x*4 + k ] = s[k];
 	}
+}
+
+// W: every element's wedge energy, one (element, coordinate pair) per thread
+// (PairWedgeEnergy at unit resolution, scaled by the level's r^(4-Dim)); the
+// element index runs with the last coordinate fastest, the element's lowest
+// corner is its node of the same multi-index.  Rows (W, 0, 0, 0).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KEnergyWedge( LevelView< Dim > L , double *partials )
+{
+	constexpr int C = DimC< Dim >::C , NP = Dim*( Dim-1 )/2;
+	__shared__ double sh[256];
+	long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= L.res;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	const long long e = g / NP;
+	double w = 0;
+	if( e<ne )
+	{
+		int F[Dim];
+		{
+			long long t = e;
+#pragma unroll
+			for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = (int)( t % L.res ) ; t /= L.res; }
+		}
+		const long long i = FlatIndex< Dim >( F , L.stride );
+		long long st[Dim];
+		{
+			int p = (int)( g - e*NP ) , pi = 0 , pj = 1;
+#pragma unroll
+			for( int a=0 ; a<Dim ; a++ )
+#pragma unroll
+				for( int b=a+1 ; b<Dim ; b++ ){ if( p==0 ) pi = a , pj = b ; p--; }
+			st[0] = L.stride[pi] , st[1] = L.stride[pj];
+			int q = 2;
+#pragma unroll
+			for( int d=0 ; d<Dim ; d++ ) if( d!=pi && d!=pj ) st[q++] = L.stride[d];
+		}
+		double xc[C] , yc[C];
+#pragma unroll
+		for( int c=0 ; c<C ; c++ )
+		{
+			long long o = i;
+#pragma unroll
+			for( int q=0 ; q<Dim ; q++ ) if( (c>>q)&1 ) o += st[q];
+			xc[c] = L.x[o] , yc[c] = L.y[o];
+		}
+		w = PairWedgeEnergy< Dim >( xc , yc )*L.sW;
+	}
+	sh[threadIdx.x] = w;
+	__syncthreads();
+	for( int h=128 ; h>0 ; h>>=1 ){ if( threadIdx.x<h ) sh[threadIdx.x] += sh[threadIdx.x+h] ; __syncthreads(); }
+	if( threadIdx.x<4 ) partials[ (long long)blockIdx.x*4 + threadIdx.x ] = threadIdx.x==0 ? sh[0] : 0.;
 }
 
 // Deterministic sum of nb rows of nv values (one block).
