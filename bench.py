@@ -196,6 +196,7 @@ def measure_batch(args, steps, warmup, with_cpu, unbatched=True, tune=True):
     per_node_bytes = B_NODE[3] // 2 if ok else B_NODE[3]
     achieved = F_NODE[3] * nodes / 8 / per_launch / 1e12
     kname = f"KGsPhaseBatch<3, {'float' if ok else 'double'}, int>"
+    copy_gbs = copy_bandwidth()   # also the calibration kernel of the committed HBM byte counts
     traffic = measured_traffic("batch3d_256" + ("_fp32" if ok else ""))
     peak = FP32_PEAK_TFLOPS if ok else FP64_PEAK_TFLOPS
     log(f"[bench] {kname}: {per_launch * 1e6:.1f} us per launch over {len(deep)} inputs ({nodes / 8:.0f} nodes), "
@@ -264,7 +265,7 @@ def measure_batch(args, steps, warmup, with_cpu, unbatched=True, tune=True):
                      "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak, "traffic": traffic,
                      "kernel": f"{kname} (finest level of the {len(deep)} depth-{top} inputs, one colour grid over all of them)",
                      "us_per_launch": per_launch * 1e6, "nodes_per_launch": nodes / 8,
-                     "hbm_gbs_algorithmic": per_node_bytes * nodes / 8 / per_launch / 1e9,
+                     "hbm_gbs_algorithmic": per_node_bytes * nodes / 8 / per_launch / 1e9, "copy_kernel_gbs": copy_gbs,
                      "traffic_gbs": traffic / per_launch / 1e9 if traffic else None},
         "cpu_baseline": cpu,
     }

@@ -27,7 +27,7 @@ EXPORTS = [
     "epr_unet_set_params", "epr_unet_get_params", "epr_unet_forward", "epr_unet_loss_gradients",
     "epr_unet_set_dataset", "epr_unet_train", "epr_learned_features", "epr_init_learned", "epr_solve_learned", "epr_time_gs_sweeps_fp32",
     "epr_autotune", "epr_autotune_batch", "epr_tune_get", "epr_tune_set", "epr_time_op", "epr_time_batch_sweeps",
-    "epr_solve_schedule",
+    "epr_solve_schedule", "epr_sample_generate",
 ]
 
 
@@ -90,6 +90,9 @@ def load():
     L.epr_init_learned.argtypes = [vp, vp, ci, dp]
     L.epr_solve_learned.argtypes = [vp, vp, dp]
     L.epr_solve_schedule.argtypes = [vp, vp, ctypes.POINTER(ci), dp]
+    L.epr_sample_generate.argtypes = [ctypes.c_char_p, ctypes.c_uint, ci, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                      ctypes.POINTER(ci), ctypes.POINTER(sz), dp, dp, sz]
+    L.epr_sample_generate.restype = ctypes.c_int
     L.epr_energies.argtypes = [vp, ci, dp]
     L.epr_solve.argtypes = [vp, dp]
     L.epr_num_functions.argtypes = [vp, ci]

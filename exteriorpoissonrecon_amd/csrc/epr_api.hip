@@ -32,6 +32,7 @@
 #include "epr_dev.h"
 #include "epr_hat.h"
 #include "epr_kernels.h"
+#include "epr_misha.h"
 #include "epr_rand.h"
 #include "epr_unet.h"
 
@@ -83,17 +84,27 @@ struct SampleCells
 			}
 			for( int a=0 ; a<Dim ; a++ ) pos[i*Dim+a] /= scl;
 		}
-		for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) unitToWorld[i][j] = ( i==j ? 1. : 0. );
-		for( int d=0 ; d<Dim ; d++ ){ unitToWorld[d][d] = 1./M[d][d] ; unitToWorld[Dim][d] = -M[Dim][d]/M[d][d]; }
+		// the unit-cube-to-world map: the reference's Gauss-Jordan inverse of the
+		// world-to-unit-cube map (ExteriorPoissonRecon.cpp:171), bit for bit
+		sample::Mat< Dim+1 > W , U2W;
+		memcpy( W.c , M , sizeof(M) );
+		U2W = W.inverse();
+		memcpy( unitToWorld , U2W.c , sizeof(unitToWorld) );
 		// --jitter (ExteriorPoissonRecon.cpp:174-180): a random translation of the
 		// unit-cube positions drawn before the seed is set, i.e. from glibc's
-		// unseeded sequence (srand(1)); the unit-cube-to-world map absorbs its inverse
+		// unseeded sequence (srand(1)); the unit-cube-to-world map absorbs its
+		// (Gauss-Jordan) inverse
 		if( jitter>0 )
 		{
-			double T[Dim+1][Dim+1] , Ti[Dim+1][Dim+1] , U[Dim+1][Dim+1];
-			for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) T[i][j] = Ti[i][j] = ( i==j ? 1. : 0. );
+			double T[Dim+1][Dim+1] , U[Dim+1][Dim+1];
+			for( int i=0 ; i<=Dim ; i++ ) for( int j=0 ; j<=Dim ; j++ ) T[i][j] = ( i==j ? 1. : 0. );
 			GlibcRand g( 1 );
-			for( int d=0 ; d<Dim ; d++ ) T[Dim][d] = ( g.random01() * 2. - 1. ) * jitter , Ti[Dim][d] = -T[Dim][d];
+			for( int d=0 ; d<Dim ; d++ ) T[Dim][d] = ( g.random01() * 2. - 1. ) * jitter;
+			sample::Mat< Dim+1 > Tm;
+			memcpy( Tm.c , T , sizeof(T) );
+			const sample::Mat< Dim+1 > TiM = Tm.inverse();
+			double Ti[Dim+1][Dim+1];
+			memcpy( Ti , TiM.c , sizeof(Ti) );
 			for( size_t i=0 ; i<n ; i++ )
 			{
 				double q[Dim] , scl = 1;
@@ -279,12 +290,15 @@ inline void SetLdsAttributes( void )
 
 // The energy terms of a level: KEnergy's partial-sum rows (the window terms),
 // then KEnergyWedge's (the wedge term), summed in a fixed order into out[4]
-// (W, y^T B x, R-form, K-form).  EnergyRows: the rows partials must hold.
+// (W, y^T B x, R-form, K-form): EnergySumBlocks contiguous row ranges first
+// (KSumRowsPart, their sums after the rows), then those sums.  EnergyRows:
+// the rows partials must hold.
+constexpr int EnergySumBlocks = 256;
 template< int Dim > long long EnergyRows( long long N , int res )
 {
 	constexpr int C = DimC< Dim >::C , NP = Dim*( Dim-1 )/2;
 	long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= res;
-	return (long long)Blocks( N*C , 256 ) + (long long)Blocks( ne*NP , 256 );
+	return (long long)Blocks( N*C , 256 ) + (long long)Blocks( ne*NP , 256 ) + EnergySumBlocks;
 }
 template< int Dim > void LaunchEnergy( const LevelView< Dim > &v , double *partials , double *out , hipStream_t st )
 {
@@ -293,7 +307,9 @@ template< int Dim > void LaunchEnergy( const LevelView< Dim > &v , double *parti
 	const unsigned nbW = Blocks( v.N*C , 256 ) , nbE = Blocks( ne*NP , 256 );
 	KEnergy< Dim ><<< nbW , 256 , 0 , st >>>( v , partials );
 	KEnergyWedge< Dim ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
-	KSumRows<4><<< 1 , 256 , 0 , st >>>( partials , (long long)nbW + nbE , out );
+	double *sums = partials + 4*( (size_t)nbW + nbE );
+	KSumRowsPart<4><<< EnergySumBlocks , 256 , 0 , st >>>( partials , (long long)nbW + nbE , sums );
+	KSumRows<4><<< 1 , 256 , 0 , st >>>( sums , EnergySumBlocks , out );
 }
 
 // ---- tuned sweep plans (epr_autotune, epr_autotune_batch)

@@ -628,6 +628,29 @@ __global__ void __launch_bounds__( 256 ) KSumRows( const double *in , long long 
 	if( threadIdx.x==0 ) for( int k=0 ; k<NV ; k++ ) out[k] = sh[0][k];
 }
 
+// First pass of a fixed-order sum of nb rows of NV values over gridDim.x
+// blocks: block b sums the contiguous rows [b per, (b+1) per) (per =
+// ceil(nb / gridDim.x)) into out row b; KSumRows over the gridDim.x rows
+// finishes it.
+template< int NV >
+__global__ void __launch_bounds__( 256 ) KSumRowsPart( const double *in , long long nb , double *out )
+{
+	__shared__ double sh[256][NV];
+	const long long per = ( nb + gridDim.x - 1 ) / gridDim.x;
+	const long long r0 = (long long)blockIdx.x*per , r1 = r0+per<nb ? r0+per : nb;
+	double s[NV];
+	for( int k=0 ; k<NV ; k++ ) s[k] = 0;
+	for( long long r=r0+threadIdx.x ; r<r1 ; r+=blockDim.x ) for( int k=0 ; k<NV ; k++ ) s[k] += in[ r*NV + k ];
+	for( int k=0 ; k<NV ; k++ ) sh[threadIdx.x][k] = s[k];
+	__syncthreads();
+	for( int h=128 ; h>0 ; h>>=1 )
+	{
+		if( threadIdx.x<h ) for( int k=0 ; k<NV ; k++ ) sh[threadIdx.x][k] += sh[threadIdx.x+h][k];
+		__syncthreads();
+	}
+	if( threadIdx.x==0 ) for( int k=0 ; k<NV ; k++ ) out[ (long long)blockIdx.x*NV + k ] = sh[0][k];
+}
+
 // Block partial sums of a strided array (first pass of a deterministic reduction).
 __global__ void __launch_bounds__( 256 ) KPartialSum( const double *in , long long n , double *partials )
 {

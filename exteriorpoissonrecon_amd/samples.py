@@ -227,6 +227,36 @@ def random_rotation(dim, rng):
     return q
 
 
+def reference_sample(sample_type, res=1024, regular=False, a_noise=0.0, p_noise=0.0, s_noise=0.0):
+    """The reference's Sample program (Sample/Sample.cpp, Include/Samples.h)
+    restated in the library (epr_sample_generate): `sample_type` as its
+    --type (torus_knot:<p>:<q>, spiral:<r>, borromean_rings, spherical:1:<n>:<l>
+    curves in 3D; clifford_torus, hopf_torus:<n>:<l> surfaces in 4D), res
+    samples (res^2 for surfaces) at random parameters from
+    std::default_random_engine (or the regular (i+0.5)/res grid), mapped to the
+    unit cube, with --aNoise (degrees), --pNoise and --sNoise noise, and mapped
+    back.  Returns (positions n x dim, frames n x dim x dim) equal bit for bit
+    to what the reference writes, so write_ply gives its bytes."""
+    import ctypes
+    from ._lib import EprError, load
+    L = load()
+    dim, n = ctypes.c_int(), ctypes.c_size_t()
+    args = (sample_type.encode(), int(res), 1 if regular else 0, float(a_noise), float(p_noise), float(s_noise))
+    rc = L.epr_sample_generate(*args, ctypes.byref(dim), ctypes.byref(n), None, None, 0)
+    if rc != 0:
+        msg = L.epr_last_error(None)
+        raise EprError(f"epr error {rc}: {msg.decode() if msg else ''}")
+    pos = np.zeros((n.value, dim.value))
+    skew = np.zeros((n.value, dim.value, dim.value))
+    pp = pos.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    sp = skew.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    rc = L.epr_sample_generate(*args, ctypes.byref(dim), ctypes.byref(n), pp, sp, n.value)
+    if rc != 0:
+        msg = L.epr_last_error(None)
+        raise EprError(f"epr error {rc}: {msg.decode() if msg else ''}")
+    return pos, skew
+
+
 def write_ply(path, pos, skew):
     """Binary little-endian PLY with x,y,z[,w] and skew_c_r (double): the
     bytes the reference's Sample writes for the same values

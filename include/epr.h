@@ -309,6 +309,23 @@ void epr_mg_destroy( epr_mg *mg );
  * of known traffic and gives the achievable copy bandwidth. */
 int  epr_time_copy( size_t bytes , int reps , double *ms );
 
+/* The reference's synthetic input generator, Sample/Sample.cpp over
+ * Include/Samples.h (host only, no device needed): `type` as Sample's --type
+ * (curves in 3D: torus_knot:<p>:<q>, spiral:<r>, borromean_rings,
+ * spherical:1:<n>:<l>; surfaces in 4D: clifford_torus, hopf_torus:<n>:<l>),
+ * res samples of a curve or res^2 of a surface, at the parameters Samples.h's
+ * GetSamples draws (random from std::default_random_engine unless regular),
+ * mapped to the unit cube (Sample.cpp:87-118), with positional (pNoise),
+ * angular (aNoise, degrees) and spot (sNoise) noise (Sample.cpp:120-201),
+ * mapped back.  *dim and *count receive the dimension and the sample count;
+ * when pos and skew are given and cap >= *count, pos receives count x dim
+ * positions and skew count x dim x dim frame matrices (entry (c, r) at
+ * [i][c][r], the reference PLY's skew_c_r), bit for bit the values the
+ * reference writes.  Other types are an error ("Could not parse sample
+ * type"). */
+int  epr_sample_generate( const char *type , unsigned int res , int regular , double aNoise , double pNoise , double sNoise ,
+                          int *dim , size_t *count , double *pos , double *skew , size_t cap );
+
 /* Measurement hooks (no reference counterpart), for the HBM roofline of the
  * transfer and energy kernels and the batched sweep.
  *

@@ -42,7 +42,8 @@ def test_gpu_adapted_matches_reference(path, tmp_path):
     for k in ("node_depth", "node_offset", "node_has_children", "node_function", "node_element",
               "active_sizes", "num_functions", "num_elements"):
         assert np.array_equal(getattr(t, k), g[k]), k
-    assert np.abs(t.voxel_to_world - g["voxel_to_world"]).max() <= 1e-15 * np.abs(g["voxel_to_world"]).max()
+    # the voxel-to-world map: the reference's bits (signed zeros included)
+    assert np.asarray(t.voxel_to_world, "<f8").tobytes() == np.asarray(g["voxel_to_world"], "<f8").tobytes()
     for d in range(len(t.x)):
         for mine, ref in ((t.x[d], g[f"x{d}"]), (t.y[d], g[f"y{d}"])):
             assert np.abs(mine - ref).max() <= TOL * max(np.abs(ref).max(), 1e-300), d

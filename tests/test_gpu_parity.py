@@ -43,6 +43,9 @@ def test_build_products(built):
     name, g, r = built
     np.testing.assert_array_equal(r.array("unit_pos"), g["unit_pos"])
     np.testing.assert_array_equal(r.array("cells"), g["cells"])
+    # the unit-cube-to-world map (the Gauss-Jordan inverse of ExteriorPoissonRecon.cpp:171), bit for bit incl. signed zeros
+    if "unit_to_world" in g:
+        assert r.array("unit_to_world").tobytes() == np.asarray(g["unit_to_world"], "<f8").tobytes()
     assert rel_err(r.array("measure"), g["measure"]) < 1e-12
     assert rel_err(r.array("sample_depth"), g["sample_depth"]) < 1e-12
     assert rel_err(r.array("noise"), g["noise"]) < 1e-9

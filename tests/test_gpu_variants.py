@@ -44,7 +44,8 @@ def test_gpu_variant_matches_reference(path, tmp_path):
         for mine, ref in ((t.dense(d, t.x[d]), g[f"x{d}"]), (t.dense(d, t.y[d]), g[f"y{d}"])):
             assert np.abs(mine - ref).max() <= 1e-9 * max(np.abs(ref).max(), 1e-300), d
     assert abs(E.sum() - g["printed_E"][0]) <= 1e-5 * g["printed_E"][0]
-    assert np.abs(t.voxel_to_world - g["voxel_to_world"]).max() <= 1e-15 * np.abs(g["voxel_to_world"]).max()
+    # the voxel-to-world map: the reference's bits (signed zeros included)
+    assert np.asarray(t.voxel_to_world, "<f8").tobytes() == np.asarray(g["voxel_to_world"], "<f8").tobytes()
 
 
 @pytest.mark.parametrize("name", ["variant_knot3_d4_separate", "variant_knot3_d4_jitter"])

@@ -223,6 +223,62 @@ def test_ply_writer_matches_reference_sample():
             assert open(q, "rb").read() == ref, name
 
 
+def _sample_args(text):
+    """Sample's command line (--type, --res, --aNoise, --pNoise, --sNoise, --regular) as keywords."""
+    a = text.split()
+    kw = {"res": 1024, "regular": False, "a_noise": 0.0, "p_noise": 0.0, "s_noise": 0.0}
+    i = 0
+    while i < len(a):
+        if a[i] == "--regular":
+            kw["regular"] = True
+            i += 1
+            continue
+        key, val = a[i], a[i + 1]
+        i += 2
+        if key == "--type":
+            kw["sample_type"] = val
+        elif key == "--res":
+            kw["res"] = int(val)
+        else:
+            kw[{"--aNoise": "a_noise", "--pNoise": "p_noise", "--sNoise": "s_noise"}[key]] = float(val)
+    return kw
+
+
+@pytest.mark.parametrize("fixture", ["sample_plys", "sample_extra"])
+def test_sample_generator_writes_the_reference_bytes(fixture, tmp_path):
+    """The restated Sample program (epr_sample_generate: Samples.h's curve and
+    surface samplers with their AutoDiff derivative arithmetic, the unit-cube
+    map and AddNoise of Sample.cpp) regenerates every PLY the reference's own
+    Sample wrote (tests/golden/make_sample_golden.py: torus knots incl. a gcd-2
+    knot, Borromean rings, spirals, the spherical curve, Clifford and Hopf tori,
+    random and regular parameters, positional, angular and spot noise) byte
+    for byte, through samples.reference_sample + write_ply and through the
+    bin/Sample command line."""
+    g = np.load(os.path.join(ROOT, "tests", "golden", fixture + ".npz"))
+    names = [k for k in g.files if not k.endswith("_args")]
+    assert len(names) >= 5
+    exe = os.path.join(ROOT, "exteriorpoissonrecon_amd", "bin", "Sample")
+    for name in names:
+        ref = g[name].tobytes()
+        args = str(g[name + "_args"])
+        pos, skew = samples.reference_sample(**_sample_args(args))
+        p = str(tmp_path / f"{name}.ply")
+        samples.write_ply(p, pos, skew)
+        assert open(p, "rb").read() == ref, name
+        if os.path.exists(exe):
+            q = str(tmp_path / f"{name}_cli.ply")
+            subprocess.run([exe, *args.split(), "--out", q], check=True, capture_output=True, timeout=120)
+            assert open(q, "rb").read() == ref, name
+
+
+def test_sample_generator_errors():
+    from exteriorpoissonrecon_amd import EprError
+    with pytest.raises(EprError, match="Could not parse sample type"):
+        samples.reference_sample("mobius_strip", 16)
+    with pytest.raises(EprError, match="Could not read"):
+        samples.reference_sample("torus_knot:2", 16)
+
+
 def test_bench_refuses_more_than_one_gpu():
     """bench.py measures one GPU: --gpus above 1 exits with a message, before touching a device."""
     res = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True, timeout=120)

@@ -98,3 +98,95 @@ def test_bspline_closed_form():
     # closed form [(1-x)^3, 3x^3-6x^2+4, -3x^3+3x^2+3x+1, x^3]/6 sums to one
     w = np.stack([(1 - x) ** 3, 3 * x ** 3 - 6 * x ** 2 + 4, -3 * x ** 3 + 3 * x ** 2 + 3 * x + 1, x ** 3]) / 6
     np.testing.assert_allclose(w.sum(0), 1, atol=1e-15)
+
+
+def test_pair_form_of_the_element_wedge_energy():
+    """KEnergyWedge evaluates an element's wedge energy as (1/2) sum over
+    coordinate pairs (i, j) of int F_ij^2, F_ij = d_i x d_j y - d_j x d_i y,
+    through the 1D Gram matrices of the hat basis and of products of two hats
+    (their upper Cholesky factors, the constants of epr_kernels.h).  Pinned
+    against the quartic form of the exact rational element tensor
+    M(a,b,c,d) = int <grad phi_a ^ grad phi_b, grad phi_c ^ grad phi_d> that
+    csrc/gen_wedge_tables.py derives, on random corner values in 3D and 4D."""
+    import itertools
+    import os
+    import sys
+    from fractions import Fraction
+    sys.path.insert(0, os.path.join(os.path.dirname(__file__), "..", "exteriorpoissonrecon_amd", "csrc"))
+    import gen_wedge_tables as g
+
+    UG = np.array([[0.5773502691896257, 0.28867513459481287], [0.0, 0.5]])
+    UH = np.array([[0.4472135954999579, 0.1118033988749895, 0.07453559924999299],
+                   [0.0, 0.14433756729740643, 0.2886751345948129], [0.0, 0.0, 0.3333333333333333]])
+    # the factors are those of G = [[1/3,1/6],[1/6,1/3]] and H_mm' = (m+m')!(4-m-m')!/5!
+    assert np.allclose(UG.T @ UG, [[1 / 3, 1 / 6], [1 / 6, 1 / 3]], rtol=0, atol=1e-16)
+    assert np.allclose(UH.T @ UH, [[1 / 5, 1 / 20, 1 / 30], [1 / 20, 1 / 30, 1 / 20], [1 / 30, 1 / 20, 1 / 5]], rtol=0, atol=1e-16)
+
+    def tensor(dim):
+        C = 1 << dim
+
+        def bit(c, d):
+            return (c >> (dim - 1 - d)) & 1
+
+        def grad(c, a):
+            e = [(0, 0) if d == a else ((1, 0) if bit(c, d) else (0, 1)) for d in range(dim)]
+            return (1 if bit(c, a) else -1), e
+
+        def integral(terms):
+            s, tot = 1, [[0, 0] for _ in range(dim)]
+            for si, e in terms:
+                s *= si
+                for d in range(dim):
+                    tot[d][0] += e[d][0]
+                    tot[d][1] += e[d][1]
+            v = Fraction(s)
+            for d in range(dim):
+                v *= g.integral_1d(*tot[d])
+            return v
+
+        def comp(a, b, i, j):
+            return [(Fraction(1, 2), [grad(a, j), grad(b, i)]), (Fraction(-1, 2), [grad(a, i), grad(b, j)])]
+
+        kl = [(k, l) for k in range(C) for l in range(k + 1, C)]
+        pairs = [(i, j) for i in range(dim) for j in range(i)]
+        M = np.array([[float(sum(2 * c1 * c2 * integral(t1 + t2) for (i, j) in pairs for c1, t1 in comp(a, b, i, j)
+                                 for c2, t2 in comp(c, d, i, j))) for (c, d) in kl] for (a, b) in kl])
+        return kl, M, bit
+
+    def pair_form(dim, val_x, val_y, i, j):
+        others = [d for d in range(dim) if d not in (i, j)]
+        no = dim - 2
+
+        def at(v, bi, bj, r):
+            F = [0] * dim
+            F[i], F[j] = bi, bj
+            for q, d in enumerate(others):
+                F[d] = r[q]
+            return v[tuple(F)]
+        c = np.zeros((2, 2) + (3,) * no)
+        for a in (0, 1):
+            for b in (0, 1):
+                for r in itertools.product((0, 1), repeat=no):
+                    for rp in itertools.product((0, 1), repeat=no):
+                        m = tuple(u + w for u, w in zip(r, rp))
+                        dxi = at(val_x, 1, b, r) - at(val_x, 0, b, r)
+                        dyi = at(val_y, 1, b, rp) - at(val_y, 0, b, rp)
+                        dxj = at(val_x, a, 1, r) - at(val_x, a, 0, r)
+                        dyj = at(val_y, a, 1, rp) - at(val_y, a, 0, rp)
+                        c[(a, b) + m] += dxi * dyj - dxj * dyi
+        for ax in range(dim):
+            U = UG if ax < 2 else UH
+            c = np.moveaxis(np.tensordot(U, np.moveaxis(c, ax, 0), axes=(1, 0)), 0, ax)
+        return 0.5 * float((c * c).sum())
+
+    rng = np.random.default_rng(7)
+    for dim in (3, 4):
+        kl, M, bit = tensor(dim)
+        C = 1 << dim
+        for _ in range(3):
+            xs, ys = rng.standard_normal(C), rng.standard_normal(C)
+            w = np.array([xs[k] * ys[l] - xs[l] * ys[k] for k, l in kl])
+            vx = {tuple(bit(c, d) for d in range(dim)): xs[c] for c in range(C)}
+            vy = {tuple(bit(c, d) for d in range(dim)): ys[c] for c in range(C)}
+            W = sum(pair_form(dim, vx, vy, i, j) for i in range(dim) for j in range(i + 1, dim))
+            assert abs(W - w @ M @ w) <= 1e-14 * abs(w @ M @ w), (dim, W, w @ M @ w)
