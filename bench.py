@@ -346,6 +346,15 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False, fp32=0)
     dim, depth = wl["dim"], wl["depth"]
     log(f"[bench] {name}: {pos.shape[0]} samples, dim {dim}, depth {depth}{' --fp32' if fp32 else ''}")
     rec = Reconstructor(pos, skew, depth=depth, aRadius=args.aRadius, fp32Sweeps=fp32)
+    # the sweep plans of every level tuned on this GPU (epr_autotune), before the timed steps
+    tuned = None
+    if not args.no_tune:
+        rec.build()
+        t0 = time.perf_counter()
+        trials = rec.autotune(fp32=bool(fp32))
+        tuned = {"seconds": time.perf_counter() - t0,
+                 "picks": [f"level {t[0]}: {t[3]} {t[4]} {t[5]} threads, {t[6]:.1f} us/iteration" for t in trials if t[-1]]}
+        log(f"[bench] autotune ({'fp32' if fp32 else 'fp64'}): {tuned['seconds']:.2f} s; " + "; ".join(tuned["picks"]))
 
     E = None
     for i in range(warmup):
@@ -423,7 +432,8 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False, fp32=0)
                    "set_samples_s": float(np.mean(sets)), "build_s": float(np.mean(builds)),
                    "solve_s": float(np.mean(solves)), "final_energy": float(E.sum()),
                    "energy_rel_err_vs_reference": rel,
-                   "reference_energy_source": REFERENCE_SOURCE.get(name) if Eref is not None else None},
+                   "reference_energy_source": REFERENCE_SOURCE.get(name) if Eref is not None else None,
+                   "autotune": tuned},
         # the sweep runs on the FP64 vector ALU (v_fma_f64, no matrix
         # instructions); MI355X's FP64 vector peak equals its FP64 matrix peak
         "roofline": {"bound": "mfma",
@@ -529,6 +539,7 @@ def main():
                     help="default workload: skip the nested learned-initializer measurement (4D Hopf torus)")
     ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
     ap.add_argument("--aRadius", type=int, default=-1, help="-1: dense grid; 0..5: adapted hierarchy of that radius")
+    ap.add_argument("--no-tune", action="store_true", help="keep the built-in sweep plans (no epr_autotune before the timed steps)")
     ap.add_argument("--no-batch", action="store_true",
                     help="default workload: skip the nested batched-workload (batch3d_256) measurement")
     ap.add_argument("--steps-batch", type=int, default=5, help="timed steps of the nested batched measurement")
@@ -568,7 +579,7 @@ def main():
         cfg = dict(top["config"])
         cfg["precision"] = "fp32 sweeps, each level's last iteration and the energies fp64" if ok else "fp64"
         cfg["fp64"] = {"ms_per_step": r64["value"] * 1e3, **{k: r64["config"][k] for k in (
-            "set_samples_s", "build_s", "solve_s", "final_energy", "energy_rel_err_vs_reference")},
+            "set_samples_s", "build_s", "solve_s", "final_energy", "energy_rel_err_vs_reference", "autotune")},
             "roofline": r64["roofline"]}
         if "hbm_kernels" in r64:
             cfg["hbm_kernels"] = r64["hbm_kernels"]

@@ -305,8 +305,10 @@ template< int Dim > void LaunchEnergy( const LevelView< Dim > &v , double *parti
 	constexpr int C = DimC< Dim >::C , NP = Dim*( Dim-1 )/2;
 	long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= v.res;
 	const unsigned nbW = Blocks( v.N*C , 256 ) , nbE = Blocks( ne*NP , 256 );
-	KEnergy< Dim ><<< nbW , 256 , 0 , st >>>( v , partials );
-	KEnergyWedge< Dim ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
+	if( v.N*DimC< Dim >::W<( 1LL<<31 ) ) KEnergy< Dim , int ><<< nbW , 256 , 0 , st >>>( v , partials );
+	else KEnergy< Dim ><<< nbW , 256 , 0 , st >>>( v , partials );
+	if( ne*NP<( 1LL<<31 ) && v.N<( 1LL<<31 ) ) KEnergyWedge< Dim , int ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
+	else KEnergyWedge< Dim ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
 	double *sums = partials + 4*( (size_t)nbW + nbE );
 	KSumRowsPart<4><<< EnergySumBlocks , 256 , 0 , st >>>( partials , (long long)nbW + nbE , sums );
 	KSumRows<4><<< 1 , 256 , 0 , st >>>( sums , EnergySumBlocks , out );
@@ -626,7 +628,15 @@ struct Ctx : public CtxBase
 	void galerkinT( const double *Mf , const Level &Fl )
 	{
 		ColourSet< Dim > cs0 ; cs0.set( Fl.res , 0 );   // class 0 (all coordinates even) is the largest
-		KGalerkinT< Dim ><<< dim3( Blocks( cs0.total , 64 ) , C ) , 64 , 0 , stream >>>( Mf , dGalT.p , Fl.res/2 , Fl.lay );
+		KGalerkinT< Dim ><<< dim3( Blocks( cs0.total , 64 ) , C ) , 64*( W/27 ) , 0 , stream >>>( Mf , dGalT.p , Fl.N , Fl.res/2 , Fl.lay );
+	}
+	// the coarser level's M from T (galerkinT of the level above it)
+	void galerkinC( double *Mc , const Level &Cl , const Level &Fl )
+	{
+		if( Fl.N>=( 1LL<<31 ) ) throw std::string( "Galerkin restriction: more than 2^31 fine functions" );
+		// small coarse levels: one thread per entry (more threads in flight)
+		const bool perEntry = Cl.N*W<( 1LL<<20 );
+		KGalerkinC< Dim ><<< perEntry ? Blocks( Cl.N*W , 256 ) : Blocks( Cl.N , 64 ) , 256 , 0 , stream >>>( dGalT.p , Mc , Cl.res , Cl.lay , Fl.lay , (int)Fl.N , Cl.fm.p , perEntry ? 1 : 0 );
 	}
 
 	template< int KR > void launchEstSplat( TreeGrids< Dim > T )
@@ -807,10 +817,8 @@ struct Ctx : public CtxBase
 			{
 				Level &Cl = levels[l] , &Fl = levels[l+1];
 				Cl.R.ensure( (size_t)Cl.N*W ) , Cl.B.ensure( (size_t)Cl.N*W );
-				galerkinT( Fl.R.p , Fl );
-				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.R.p , Cl.res , Cl.lay , Cl.fm.p );
-				galerkinT( Fl.B.p , Fl );
-				KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , Cl.B.p , Cl.res , Cl.lay , Cl.fm.p );
+				galerkinT( Fl.R.p , Fl ) , galerkinC( Cl.R.p , Cl , Fl );
+				galerkinT( Fl.B.p , Fl ) , galerkinC( Cl.B.p , Cl , Fl );
 			}
 			HIPCHECK( hipGetLastError() );
 		}
@@ -1151,13 +1159,21 @@ struct Ctx : public CtxBase
 		long long nf = levels[l+1].N;
 		KProlong< Dim ><<< Blocks( nf , 256 ) , 256 , 0 , stream >>>( in , out , levels[l].res , mode , regular ? nullptr : levels[l+1].fm.p );
 	}
+	// the same for x and y in one launch (32-bit indices while the fine level
+	// has fewer than 2^31 functions, else one launch each)
+	void prolong2( int l , const double *inX , const double *inY , double *outX , double *outY , int mode , bool regular=false )
+	{
+		long long nf = levels[l+1].N;
+		if( nf>=( 1LL<<31 ) ){ prolong( l , inX , outX , mode , regular ) ; prolong( l , inY , outY , mode , regular ) ; return; }
+		FastDiv n1 ; n1.set( (unsigned int)( 2*levels[l].res+1 ) );
+		KProlong2< Dim ><<< Blocks( nf , 256 ) , 256 , 0 , stream >>>( inX , inY , outX , outY , levels[l].res , mode , regular ? nullptr : levels[l+1].fm.p , n1 );
+	}
 
 	void prolongAdd( int l )
 	{
 		needBuilt();
 		if( l<0 || l>=D ) throw std::string( "level out of range" );
-		prolong( l , levels[l].x.p , levels[l+1].x.p , 1 );
-		prolong( l , levels[l].y.p , levels[l+1].y.p , 1 );
+		prolong2( l , levels[l].x.p , levels[l].y.p , levels[l+1].x.p , levels[l+1].y.p , 1 );
 		HIPCHECK( hipGetLastError() );
 		HIPCHECK( hipStreamSynchronize( stream ) );
 	}
@@ -1166,7 +1182,7 @@ struct Ctx : public CtxBase
 	// one of them at level l after a warm-up, on scratch outputs (the level's
 	// data is left as it is); ms per launch and the launch's algorithmic HBM
 	// bytes (each input and output value once):
-	//   "prolong"     KProlong, level l -> l+1, out += P x: N_l + 2 N_{l+1} doubles
+	//   "prolong"     KProlong2, level l -> l+1, x and y: out += P in, 2 (N_l + 2 N_{l+1}) doubles
 	//   "galerkin_t"  KGalerkinT, T = P^T R of level l (> 0): 2 N_l 3^Dim doubles
 	//   "galerkin_c"  KGalerkinC, R_{l-1} = T P from level l's T: (N_l + N_{l-1}) 3^Dim doubles
 	//   "energy"      KEnergy + KSumRows at level l: x, y, R, B (2 N_l + 2 N_l 3^Dim doubles)
@@ -1181,10 +1197,11 @@ struct Ctx : public CtxBase
 		{
 			if( l>=D ) throw std::string( "prolong: level must be below the depth" );
 			Level &Fl = levels[l+1];
-			dTmp.ensure( (size_t)Fl.N );
+			dTmp.ensure( (size_t)Fl.N ) , dTmp2.ensure( (size_t)Fl.N );
 			HIPCHECK( hipMemsetAsync( dTmp.p , 0 , (size_t)Fl.N*sizeof(double) , stream ) );
-			f = [&]{ prolong( l , L.x.p , dTmp.p , 1 ); };
-			b = 8.*( L.N + 2*Fl.N );
+			HIPCHECK( hipMemsetAsync( dTmp2.p , 0 , (size_t)Fl.N*sizeof(double) , stream ) );
+			f = [&]{ prolong2( l , L.x.p , L.y.p , dTmp.p , dTmp2.p , 1 ); };
+			b = 16.*( L.N + 2*Fl.N );
 		}
 		else if( op=="galerkin_t" )
 		{
@@ -1200,7 +1217,7 @@ struct Ctx : public CtxBase
 			dGalT.ensure( (size_t)L.N*W );
 			galerkinT( L.R.p , L );
 			dTmp2.ensure( (size_t)Cl.N*W );
-			f = [&]{ KGalerkinC< Dim ><<< Blocks( Cl.N*W , 256 ) , 256 , 0 , stream >>>( dGalT.p , dTmp2.p , Cl.res , Cl.lay , Cl.fm.p ); };
+			f = [&]{ galerkinC( dTmp2.p , Cl , L ); };
 			b = 8.*( L.N + Cl.N )*W;
 		}
 		else if( op=="energy" )
@@ -1390,7 +1407,7 @@ struct Ctx : public CtxBase
 		if( sched ) for( int d=lo ; d<=hi ; d++ ) if( sched[d]<0 ) throw std::string( "negative iteration count" );
 		if( prm.singleLevel )
 		{
-			for( int d=lo ; d<hi ; d++ ){ prolong( d , levels[d].x.p , levels[d+1].x.p , 0 ) ; prolong( d , levels[d].y.p , levels[d+1].y.p , 0 ); }
+			for( int d=lo ; d<hi ; d++ ) prolong2( d , levels[d].x.p , levels[d].y.p , levels[d+1].x.p , levels[d+1].y.p , 0 );
 			for( int d=lo ; d<hi ; d++ ) levels[d].x.zero( stream ) , levels[d].y.zero( stream );
 			gs( hi , sched ? sched[hi] : prm.gsIters );
 		}
@@ -1405,16 +1422,15 @@ struct Ctx : public CtxBase
 			}
 			if( d<hi )
 			{
-				prolong( d , levels[d].x.p , levels[d+1].x.p , 1 );
-				prolong( d , levels[d].y.p , levels[d+1].y.p , 1 );
+				prolong2( d , levels[d].x.p , levels[d].y.p , levels[d+1].x.p , levels[d+1].y.p , 1 );
 			}
 		}
 		double e[2];
 		energies( hi , e );
 		if( out ) out[0] = e[0] , out[1] = e[1];
 		if( prm.verbose ) printf( "Error: %g -> %g = %g + %g => %g\n" , c_ , e[0]+e[1] , e[0] , e[1] , std::sqrt( ( e[0]+e[1] )/c_ ) );
-		for( int d=hi ; d<D ; d++ ){ prolong( d , levels[d].x.p , levels[d+1].x.p , 0 ) ; prolong( d , levels[d].y.p , levels[d+1].y.p , 0 ); }
-		for( int d=D ; d>0 ; d-- ){ prolong( d-1 , levels[d-1].x.p , levels[d].x.p , 2 ) ; prolong( d-1 , levels[d-1].y.p , levels[d].y.p , 2 ); }
+		for( int d=hi ; d<D ; d++ ) prolong2( d , levels[d].x.p , levels[d].y.p , levels[d+1].x.p , levels[d+1].y.p , 0 );
+		for( int d=D ; d>0 ; d-- ) prolong2( d-1 , levels[d-1].x.p , levels[d-1].y.p , levels[d].x.p , levels[d].y.p , 2 );
 		HIPCHECK( hipGetLastError() );
 		HIPCHECK( hipStreamSynchronize( stream ) );
 		solved = true;
@@ -1517,10 +1533,11 @@ struct Ctx : public CtxBase
 		if( which=='P' )
 		{
 			if( l>=D ) throw std::string( "no finer level" );
-			DevBuf< double > a , b;
+			// through the paired launch the solve uses, x in both slots
+			DevBuf< double > a , b , b2;
 			a.upload( x , levels[l].N , stream );
-			b.alloc( levels[l+1].N );
-			prolong( l , a.p , b.p , 0 );
+			b.alloc( levels[l+1].N ) , b2.alloc( levels[l+1].N );
+			prolong2( l , a.p , a.p , b.p , b2.p , 0 );
 			b.download( y , levels[l+1].N , stream );
 			HIPCHECK( hipStreamSynchronize( stream ) );
 			return;
@@ -1643,7 +1660,7 @@ struct Ctx : public CtxBase
 		HIPCHECK( hipMemcpyAsync( ay.p , levels[0].y.p , levels[0].N*sizeof(double) , hipMemcpyDeviceToDevice , stream ) );
 		for( int d=1 ; d<=D ; d++ )
 		{
-			prolong( d-1 , ax.p , bx.p , 0 , true ) , prolong( d-1 , ay.p , by.p , 0 , true );
+			prolong2( d-1 , ax.p , ay.p , bx.p , by.p , 0 , true );
 			const long long N = levels[d].N;
 			KAddInto<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( levels[d].x.p , bx.p , ax.p , N );
 			KAddInto<<< Blocks( N , 256 ) , 256 , 0 , stream >>>( levels[d].y.p , by.p , ay.p , N );

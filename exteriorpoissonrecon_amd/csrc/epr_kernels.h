@@ -529,13 +529,14 @@ __global__ void __launch_bounds__( 256 ) KQFit( LevelView< Dim > L , double *q )
 // the window terms' rows (W = 0), KEnergyWedge the wedge term's rows after them.
 // KEnergy: the nodes in the column order of R and B (colour by colour), so
 // the node groups of a wavefront read consecutive columns.
-template< int Dim >
+// Idx: the index type, int where N 3^Dim < 2^31.
+template< int Dim , typename Idx = long long >
 __global__ void __launch_bounds__( 256 ) KEnergy( LevelView< Dim > L , double *partials )
 {
 	constexpr int C = DimC< Dim >::C , G = 256/C;
 	__shared__ double sh[G][4];
-	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-	const long long k = g / C;   // column
+	const Idx g = (Idx)blockIdx.x*blockDim.x + threadIdx.x;
+	const Idx k = g / C;   // column
 	const int lane = (int)( g % C );
 	double a[5] = { 0 , 0 , 0 , 0 , 0 };
 	if( k<L.N )
@@ -546,8 +547,8 @@ __global__ void __launch_bounds__( 256 ) KEnergy( LevelView< Dim > L , double *p
 		ColourSet< Dim > cs ; cs.set( L.res , c );
 		int F[Dim];
 		cs.node( k - L.lay.cstart[c] , F );
-		const long long i = FlatIndex< Dim >( F , L.stride );
-		NodeGroup< Dim , MODE_ENERGY , Dim==3 >( L , F , i , k , lane , a );
+		const Idx i = (Idx)FlatIndex< Dim >( F , L.stride );
+		NodeGroup< Dim , MODE_ENERGY , Dim==3 , double , Idx >( L , F , i , k , lane , a );
 	}
 	if( lane==0 ) for( int k=0 ; k<4 ; k++ ) sh[ threadIdx.x/C ][k] = a[k];
 	__syncthreads();
@@ -562,26 +563,27 @@ __global__ void __launch_bounds__( 256 ) KEnergy( LevelView< Dim > L , double *p
 // W: every element's wedge energy, one (element, coordinate pair) per thread
 // (PairWedgeEnergy at unit resolution, scaled by the level's r^(4-Dim)); the
 // element index runs with the last coordinate fastest, the element's lowest
-// corner is its node of the same multi-index.  Rows (W, 0, 0, 0).
-template< int Dim >
+// corner is its node of the same multi-index.  Rows (W, 0, 0, 0).  Idx: int
+// where the (element, pair) count and N are below 2^31.
+template< int Dim , typename Idx = long long >
 __global__ void __launch_bounds__( 256 ) KEnergyWedge( LevelView< Dim > L , double *partials )
 {
 	constexpr int C = DimC< Dim >::C , NP = Dim*( Dim-1 )/2;
 	__shared__ double sh[256];
-	long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= L.res;
-	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-	const long long e = g / NP;
+	Idx ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= L.res;
+	const Idx g = (Idx)blockIdx.x*blockDim.x + threadIdx.x;
+	const Idx e = g / NP;
 	double w = 0;
 	if( e<ne )
 	{
 		int F[Dim];
 		{
-			long long t = e;
+			Idx t = e;
 #pragma unroll
-			for( int d=Dim-1 ; d>=0 ; d-- ){ F[d] = (int)( t % L.res ) ; t /= L.res; }
+			for( int d=Dim-1 ; d>=0 ; d-- ){ const Idx q = t / L.res ; F[d] = (int)( t - q*L.res ) ; t = q; }
 		}
-		const long long i = FlatIndex< Dim >( F , L.stride );
-		long long st[Dim];
+		const Idx i = (Idx)FlatIndex< Dim >( F , L.stride );
+		Idx st[Dim];
 		{
 			int p = (int)( g - e*NP ) , pi = 0 , pj = 1;
 #pragma unroll
@@ -597,7 +599,7 @@ __global__ void __launch_bounds__( 256 ) KEnergyWedge( LevelView< Dim > L , doub
 #pragma unroll
 		for( int c=0 ; c<C ; c++ )
 		{
-			long long o = i;
+			Idx o = i;
 #pragma unroll
 			for( int q=0 ; q<Dim ; q++ ) if( (c>>q)&1 ) o += st[q];
 			xc[c] = L.x[o] , yc[c] = L.y[o];
@@ -700,6 +702,58 @@ __device__ __forceinline__ void Prolong( const double *xc , double *xf , int cre
 }
 template< int Dim >
 __global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf , int cres , int mode , const unsigned char *fmaskF ){ Prolong< Dim >( xc , xf , cres , mode , fmaskF , (long long)blockIdx.x*blockDim.x + threadIdx.x ); }
+
+// Unsigned division by a fixed divisor d >= 1 of numbers below 2^31: the
+// high half of n * ceil(2^32/d), one down when that overshoots (exact).
+struct FastDiv
+{
+	unsigned int d , m;
+	__host__ void set( unsigned int _d ){ d = _d , m = (unsigned int)( ( 0xFFFFFFFFull )/_d + 1 ); }
+	__device__ __forceinline__ unsigned int div( unsigned int n ) const { unsigned int q = d==1 ? n : __umulhi( n , m ) ; if( q*d>n ) q-- ; return q; }
+};
+// Prolong of x and y together (out (=, +=, -=) P in) for levels whose
+// fine functions number below 2^31: 32-bit indices, the fine multi-index by
+// one fixed-divisor division per coordinate, the coarse index as a base plus
+// the coarse strides of the odd coordinates; the same terms in the same order
+// as Prolong (the weight of every term is 1/2 per odd coordinate), so the
+// same bits.
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KProlong2( const double *xc , const double *yc , double *xf , double *yf , int cres , int mode , const unsigned char *fmaskF , FastDiv n1 )
+{
+	const unsigned int fres = 2*cres;
+	unsigned int nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= fres+1;
+	const unsigned int f = blockIdx.x*blockDim.x + threadIdx.x;
+	if( f>=nf ) return;
+	if( fmaskF && !fmaskF[f] ){ if( mode==0 ) xf[f] = 0 , yf[f] = 0 ; return; }
+	int F[Dim];
+	{
+		unsigned int t = f;
+#pragma unroll
+		for( int d=Dim-1 ; d>=0 ; d-- ){ const unsigned int q = n1.div( t ) ; F[d] = (int)( t - q*n1.d ) ; t = q; }
+	}
+	unsigned int cs[Dim] , c0 = 0;
+	{
+		unsigned int s = 1;
+#pragma unroll
+		for( int d=Dim-1 ; d>=0 ; d-- ){ cs[d] = s ; c0 += (unsigned int)( F[d]>>1 )*s ; s *= cres+1; }
+	}
+	int nodd = 0;
+	unsigned int oddStride[Dim];
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ) if( F[d]&1 ) oddStride[nodd++] = cs[d];
+	double w = 1;
+	for( int k=0 ; k<nodd ; k++ ) w *= 0.5;
+	double sx = 0 , sy = 0;
+	for( int m=0 ; m<(1<<nodd) ; m++ )
+	{
+		unsigned int c = c0;
+		for( int oi=0 ; oi<nodd ; oi++ ) if( ( m>>( nodd-1-oi ) ) & 1 ) c += oddStride[oi];
+		sx += w*xc[c] , sy += w*yc[c];
+	}
+	if( mode==0 ) xf[f] = sx , yf[f] = sy;
+	else if( mode==1 ) xf[f] = xf[f] + sx , yf[f] = yf[f] + sy;
+	else xf[f] = xf[f] - sx , yf[f] = yf[f] - sy;
+}
 
 // out = a + b
 // y = M x for a level operator stored by columns (column f holds M(f+o_w, f)):
@@ -1474,157 +1528,217 @@ template< int Dim > struct GalerkinShape
 	static constexpr double Weight( int cls , int wm ){ double w = 1 ; for( int d=0 ; d<Dim ; d++ ) if( Opts( cls , wm , d )==2 ) w *= 0.5 ; return w; }
 };
 
-template< int Dim , int CLS , int WM , int ... Cs >
+// Scatter of one column entry into the accumulators of the outputs whose
+// first window digit is WO0 (acc[wo % 27]).
+template< int Dim , int CLS , int WM , int WO0 , int ... Cs >
 __device__ __forceinline__ void GalerkinScatter( double mv , const int *K , int cres , double *acc , std::integer_sequence< int , Cs... > )
 {
 	typedef GalerkinShape< Dim > G;
 	constexpr double pmi = G::Weight( CLS , WM );
 	( [&]{
 		constexpr int wo = G::Wo( CLS , WM , Cs );
-		bool in = true;
-#pragma unroll
-		for( int d=0 ; d<Dim ; d++ )
+		if constexpr( wo/27==WO0 )
 		{
-			const int off = G::Off( CLS , WM , d , G::OptOf( CLS , WM , Cs , d ) );
-			const int i = ( K[d]>>1 ) + off - 1;
-			in = in && i>=0 && i<=cres;
+			bool in = true;
+#pragma unroll
+			for( int d=0 ; d<Dim ; d++ )
+			{
+				const int off = G::Off( CLS , WM , d , G::OptOf( CLS , WM , Cs , d ) );
+				const int i = ( K[d]>>1 ) + off - 1;
+				in = in && i>=0 && i<=cres;
+			}
+			if( in ) acc[wo%27] = __dadd_rn( acc[wo%27] , __dmul_rn( pmi , mv ) );
 		}
-		if( in ) acc[wo] = __dadd_rn( acc[wo] , __dmul_rn( pmi , mv ) );
 	}() , ... );
 }
 
-template< int Dim , int CLS , int ... WMs >
-__device__ __forceinline__ void GalerkinColumn( const double *col , const int *K , int fres , int cres , double *acc , std::integer_sequence< int , WMs... > )
+// The column entries WM = 27 CH + i (i in Is) of one fine node, from LDS,
+// scattered into the accumulators of window digit WO0 (entries outside the
+// grid are skipped; a zero entry adds zero, which leaves every accumulator
+// as it is: they start at +0 and never become -0).
+template< int Dim , int CLS , int CH , int WO0 , int ... Is >
+__device__ __forceinline__ void GalerkinChunk( const double *col , const int *K , int fres , int cres , double *acc , std::integer_sequence< int , Is... > )
 {
 	typedef GalerkinShape< Dim > G;
 	( [&]{
+		constexpr int WM = 27*CH + Is;
 		bool in = true;
 #pragma unroll
-		for( int d=0 ; d<Dim ; d++ ){ const int m = K[d] + G::Digit( WMs , d ) - 1 ; in = in && m>=0 && m<=fres; }
-		if( !in ) return;
-		const double mv = col[WMs];
-		if( mv!=0 ) GalerkinScatter< Dim , CLS , WMs >( mv , K , cres , acc , std::make_integer_sequence< int , G::NumCombos( CLS , WMs ) >{} );
+		for( int d=0 ; d<Dim ; d++ ){ const int m = K[d] + G::Digit( WM , d ) - 1 ; in = in && m>=0 && m<=fres; }
+		if( in ) GalerkinScatter< Dim , CLS , WM , WO0 >( col[Is] , K , cres , acc , std::make_integer_sequence< int , G::NumCombos( CLS , WM ) >{} );
+	}() , ... );
+}
+template< int Dim , int CLS , int CH , int ... WOs >
+__device__ __forceinline__ void GalerkinChunkWave( int wave , const double *col , const int *K , int fres , int cres , double *acc , std::integer_sequence< int , WOs... > )
+{
+	( [&]{ if( wave==WOs ) GalerkinChunk< Dim , CLS , CH , WOs >( col , K , fres , cres , acc , std::make_integer_sequence< int , 27 >{} ); }() , ... );
+}
+
+// Chunk by chunk (27 column entries each): stage the block's columns in LDS,
+// then each wavefront scatters its lanes' columns into its outputs.
+template< int Dim , int CLS , int ... CHs >
+__device__ __forceinline__ void GalerkinTChunks( const double *Mf , long long col0 , int lim , bool have , int wave , const int *K , int fres , int cres , double *buf , double *acc , std::integer_sequence< int , CHs... > )
+{
+	constexpr int W = DimC< Dim >::W , NW = W/27;
+	const int lane = threadIdx.x & 63;
+	( [&]{
+		if( CHs ) __syncthreads();
+		for( int e=threadIdx.x ; e<64*27 ; e+=64*NW ) buf[e] = e<lim ? Mf[ ( col0 + e/27 )*W + 27*CHs + e%27 ] : 0.;
+		__syncthreads();
+		if( have ) GalerkinChunkWave< Dim , CLS , CHs >( wave , buf + lane*27 , K , fres , cres , acc , std::make_integer_sequence< int , NW >{} );
 	}() , ... );
 }
 
-// 64 fine nodes of parity class CLS per workgroup (block b of the class):
-// their columns are loaded into LDS with contiguous reads, scattered from LDS
-// into register accumulators, and the results leave through LDS with
-// contiguous writes (node-major T[k*W + wo]).
+// T is offset-major over the fine columns in colour-grouped order:
+// T[wo*Nf + col] (wo the coarse window offset), so a wavefront's stores here
+// and its loads in KGalerkinC are contiguous.  64 fine nodes of parity class
+// CLS per workgroup (block b of the class), one wavefront per first window
+// digit of the outputs (3D: one, 4D: three, 27 accumulators each): the
+// columns pass through LDS 27 entries at a time (contiguous loads), are
+// scattered from LDS into the accumulators, and leave one offset plane at a
+// time (contiguous stores).
 template< int Dim , int CLS >
-__device__ __forceinline__ void GalerkinTBlock( const double *Mf , double *T , int cres , const ColourSet< Dim > &cs , long long colBase , unsigned b , double *buf , long long *knode )
+__device__ __forceinline__ void GalerkinTBlock( const double *Mf , double *T , long long Nf , int cres , const ColourSet< Dim > &cs , long long colBase , unsigned b , double *buf )
 {
-	constexpr int W = DimC< Dim >::W;
-	const int lane = threadIdx.x , fres = 2*cres;
+	constexpr int W = DimC< Dim >::W , NCH = W/27;
+	const int lane = threadIdx.x & 63 , wave = threadIdx.x >> 6 , fres = 2*cres;
 	const long long g = (long long)b*64 + lane;
+	const bool have = g<cs.total;
 	int K[Dim];
-	long long k = -1;
-	if( g<cs.total )
-	{
-		cs.node( g , K );
-		k = 0;
-		for( int d=0 ; d<Dim ; d++ ) k = k*( fres+1 ) + K[d];
-	}
-	if constexpr( Dim==4 )
-	{
-		// 4D: 81 accumulators per thread already hold occupancy at two waves
-		// per SIMD; staging 64 columns in LDS (41 KB) would cut it below one
-		if( k<0 ) return;
-		double acc[W];
-#pragma unroll
-		for( int w=0 ; w<W ; w++ ) acc[w] = 0;
-		GalerkinColumn< Dim , CLS >( Mf + ( colBase + g )*W , K , fres , cres , acc , std::make_integer_sequence< int , W >{} );
-#pragma unroll
-		for( int w=0 ; w<W ; w++ ) T[ k*W + w ] = acc[w];
-		return;
-	}
-	knode[lane] = k;
-	__syncthreads();
-	// the block's columns are consecutive in the colour-grouped layout
+	if( have ) cs.node( g , K );
+	else for( int d=0 ; d<Dim ; d++ ) K[d] = 0;
 	const long long col0 = colBase + (long long)b*64;
-	for( int e=lane ; e<64*W ; e+=64 ){ const long long kn = knode[ e/W ] ; buf[e] = kn>=0 ? Mf[ col0*W + e ] : 0.; }
-	__syncthreads();
-	double acc[W];
+	const int lim = (int)min( (long long)64 , cs.total - (long long)b*64 )*27;   // LDS entries that belong to the class
+	double acc[27];
 #pragma unroll
-	for( int w=0 ; w<W ; w++ ) acc[w] = 0;
-	if( k>=0 ) GalerkinColumn< Dim , CLS >( buf + lane*W , K , fres , cres , acc , std::make_integer_sequence< int , W >{} );
-	__syncthreads();
+	for( int w=0 ; w<27 ; w++ ) acc[w] = 0;
+	GalerkinTChunks< Dim , CLS >( Mf , col0 , lim , have , wave , K , fres , cres , buf , acc , std::make_integer_sequence< int , NCH >{} );
+	if( !have ) return;
 #pragma unroll
-	for( int w=0 ; w<W ; w++ ) buf[ lane*W + w ] = acc[w];
-	__syncthreads();
-	for( int e=lane ; e<64*W ; e+=64 ){ const long long kn = knode[ e/W ] ; if( kn>=0 ) T[ kn*W + e%W ] = buf[e]; }
+	for( int w=0 ; w<27 ; w++ ) T[ ( 27*wave + w )*Nf + colBase + g ] = acc[w];
 }
 template< int Dim , int ... Cs >
-__device__ __forceinline__ void GalerkinTDispatch( int cls , const double *Mf , double *T , int cres , long long colBase , unsigned b , double *buf , long long *knode , std::integer_sequence< int , Cs... > )
+__device__ __forceinline__ void GalerkinTDispatch( int cls , const double *Mf , double *T , long long Nf , int cres , long long colBase , unsigned b , double *buf , std::integer_sequence< int , Cs... > )
 {
 	( [&]{
 		if( cls!=Cs ) return;
 		ColourSet< Dim > cs ; cs.set( 2*cres , Cs );
-		if( (long long)b*64<cs.total ) GalerkinTBlock< Dim , Cs >( Mf , T , cres , cs , colBase , b , buf , knode );   // block-uniform
+		if( (long long)b*64<cs.total ) GalerkinTBlock< Dim , Cs >( Mf , T , Nf , cres , cs , colBase , b , buf );   // block-uniform
 	}() , ... );
 }
-// T = P^T M of a fine level: blockIdx.y is the parity class of the fine nodes
+// T = P^T M of a fine level (Nf functions): blockIdx.y is the parity class of
+// the fine nodes; 64 x 3^(Dim-3) threads
 template< int Dim >
-__global__ void __launch_bounds__( 64 ) KGalerkinT( const double *Mf , double *T , int cres , ColLayout< Dim > lay )
+__global__ void __launch_bounds__( Dim==3 ? 64 : 192 ) KGalerkinT( const double *Mf , double *T , long long Nf , int cres , ColLayout< Dim > lay )
 {
-	__shared__ double buf[ Dim==4 ? 1 : 64*DimC< Dim >::W ];
-	__shared__ long long knode[64];
+	__shared__ double buf[64*27];
 	const int cls = blockIdx.y;
-	GalerkinTDispatch< Dim >( cls , Mf , T , cres , lay.cstart[cls] , blockIdx.x , buf , knode , std::make_integer_sequence< int , DimC< Dim >::C >{} );
+	GalerkinTDispatch< Dim >( cls , Mf , T , Nf , cres , lay.cstart[cls] , blockIdx.x , buf , std::make_integer_sequence< int , DimC< Dim >::C >{} );
+}
+
+// M_c = T P: entry (J, I = J + o) sums T[K][I] P[K][J] over the fine nodes
+// K = 2J + delta (|delta_d| <= 1 inside the grid; delta_d in {-1,0,1} where
+// o_d = 0, {0,1} where o_d = 1, {-1,0} where o_d = -1) in increasing K, the
+// first nonzero term taken as it is (the Eigen product order).  The deltas
+// are unrolled 27 at a time (delta_0 by delta_0 in 4D): a chunk's loads are
+// all issued before its terms are added in order.
+template< int Dim , int ... Ds >
+__device__ __forceinline__ void GalerkinCChunk( const double *T , int Nf , const long long *cstart , const int *J , const int *o , int cres , int D0 , double &acc , bool &any , std::integer_sequence< int , Ds... > )
+{
+	// Ds: the deltas of the last three dimensions (3D: all of them) in base 3, 0 -> -1
+	constexpr int R = Dim==3 ? 3 : 4;
+	const int fres = 2*cres;
+	double v[27];
+	( [&]{
+		int dl[R];
+		if constexpr( Dim==4 ) dl[0] = D0;
+		for( int q=0 ; q<3 ; q++ ) dl[R-3+q] = ( Ds / ( q==0 ? 9 : q==1 ? 3 : 1 ) ) % 3 - 1;
+		bool ok = true;
+		int cls = 0 , ci = 0 , wo = 0;
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ )
+		{
+			const int Kd = 2*J[d] + dl[d] , p = Kd&1;
+			ok = ok && ( o[d]!=1 || dl[d]>=0 ) && ( o[d]!=-1 || dl[d]<=0 ) && Kd>=0 && Kd<=fres;
+			cls |= p<<d;
+			ci = ci*( p ? cres : cres+1 ) + ( Kd>>1 );
+			wo = wo*3 + ( o[d] - ( dl[d]<0 ? -1 : 0 ) + 1 );
+		}
+		v[Ds] = ok ? T[ (long long)wo*Nf + cstart[cls] + ci ] : 0.;
+	}() , ... );
+	( [&]{
+		int nz = 0;
+		if constexpr( Dim==4 ) nz += D0!=0;
+		nz += ( Ds/9 )%3!=1 , nz += ( Ds/3 )%3!=1 , nz += Ds%3!=1;
+		const double pkj = nz==0 ? 1. : nz==1 ? 0.5 : nz==2 ? 0.25 : nz==3 ? 0.125 : 0.0625;
+		const double t = v[Ds];
+		if( t!=0 ){ const double pr = __dmul_rn( t , pkj ) ; acc = any ? __dadd_rn( acc , pr ) : pr ; any = true; }
+	}() , ... );
 }
 
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *Mc , int cres , ColLayout< Dim > lay , const unsigned char *fmaskC )
+__device__ __forceinline__ double GalerkinEntry( const double *T , int Nf , const long long *cstart , const int *J , int cres , int wi , const unsigned char *fmaskC , int j )
 {
-	constexpr int W = DimC< Dim >::W;
-	const int fres = 2*cres;
-	long long nc = 1 ; for( int d=0 ; d<Dim ; d++ ) nc *= cres+1;
-	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
-	if( g>=nc*W ) return;
-	const long long j = g / W;
-	const int wi = (int)( g % W );
-	int J[Dim] , I[Dim] , lo[Dim] , hi[Dim] , K[Dim];
-	Unflatten< Dim >( j , cres , J );
-	const long long out = lay.col( J )*W + wi;
+	int o[Dim];
+	bool in = true;
+	int ii = 0;
 	{
 		int t = wi;
-		for( int d=Dim-1 ; d>=0 ; d-- ){ I[d] = J[d] + t%3 - 1 ; t /= 3; }
+#pragma unroll
+		for( int d=Dim-1 ; d>=0 ; d-- ){ o[d] = t%3 - 1 ; t /= 3 ; const int Id = J[d] + o[d] ; in = in && Id>=0 && Id<=cres; }
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ ) ii = ii*( cres+1 ) + J[d] + o[d];
 	}
-	long long fstride[Dim];
-	{
-		long long s = 1;
-		for( int d=Dim-1 ; d>=0 ; d-- ){ fstride[d] = s ; s *= fres+1; }
-	}
-	if( fmaskC )   // adapted levels: only indexed coarse functions have rows and columns
-	{
-		bool in = fmaskC[j]!=0;
-		long long ii = 0;
-		for( int d=0 ; d<Dim ; d++ ){ in = in && I[d]>=0 && I[d]<=cres ; ii = ii*(cres+1) + I[d]; }
-		if( !in || !fmaskC[ii] ){ Mc[out] = 0 ; return; }
-	}
-	for( int d=0 ; d<Dim ; d++ )
-	{
-		if( I[d]<0 || I[d]>cres ){ Mc[out] = 0 ; return; }
-		lo[d] = max( max( 2*J[d]-1 , 2*I[d]-2 ) , 0 );
-		hi[d] = min( min( 2*J[d]+1 , 2*I[d]+2 ) , fres );
-		if( lo[d]>hi[d] ){ Mc[out] = 0 ; return; }
-		K[d] = lo[d];
-	}
+	// adapted levels: only indexed coarse functions have rows and columns
+	if( in && fmaskC ) in = fmaskC[j]!=0 && fmaskC[ii]!=0;
+	if( !in ) return 0.;
 	double acc = 0;
 	bool any = false;
-	while( true )
+	if constexpr( Dim==3 ) GalerkinCChunk< 3 >( T , Nf , cstart , J , o , cres , 0 , acc , any , std::make_integer_sequence< int , 27 >{} );
+	else
 	{
-		double pkj = 1;
-		long long k = 0;
-		int wo = 0;
-		for( int d=0 ; d<Dim ; d++ ){ if( K[d]!=2*J[d] ) pkj *= 0.5 ; k += K[d]*fstride[d] ; wo = wo*3 + ( I[d] - ( K[d]>>1 ) + 1 ); }
-		const double t = T[ k*W + wo ];
-		if( t!=0 ){ const double pr = __dmul_rn( t , pkj ) ; acc = any ? __dadd_rn( acc , pr ) : pr ; any = true; }
-		int d = Dim-1;
-		for( ; d>=0 ; d-- ){ if( ++K[d]<=hi[d] ) break ; K[d] = lo[d]; }
-		if( d<0 ) break;
+		// one chunk at a time (a chunk's 27 loads in flight, not 81), chunks
+		// outside the delta range of o_0 or the grid skipped
+#pragma unroll 1
+		for( int d0=-1 ; d0<=1 ; d0++ )
+		{
+			if( ( o[0]==1 && d0<0 ) || ( o[0]==-1 && d0>0 ) || 2*J[0]+d0<0 || 2*J[0]+d0>2*cres ) continue;
+			GalerkinCChunk< 4 >( T , Nf , cstart , J , o , cres , d0 , acc , any , std::make_integer_sequence< int , 27 >{} );
+		}
 	}
-	Mc[out] = acc;
+	return acc;
+}
+
+// Two thread maps (nc and Nf below 2^31).  perEntry: thread g is entry
+// (g / 3^Dim, g % 3^Dim), for small levels (latency).  Else a workgroup is
+// 64 consecutive coarse nodes (natural order, one per lane) by 4 window
+// offsets at a time, so for a fixed delta a wavefront reads consecutive
+// columns of one parity class of one offset plane of T.
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KGalerkinC( const double *T , double *Mc , int cres , ColLayout< Dim > lay , ColLayout< Dim > layF , int Nf , const unsigned char *fmaskC , int perEntry )
+{
+	constexpr int W = DimC< Dim >::W;
+	int nc = 1 ; for( int d=0 ; d<Dim ; d++ ) nc *= cres+1;
+	int j , wi0 , wstep;
+	if( perEntry )
+	{
+		const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+		if( g>=(long long)nc*W ) return;
+		j = (int)( g / W ) , wi0 = (int)( g % W ) , wstep = W;
+	}
+	else
+	{
+		j = (int)blockIdx.x*64 + ( threadIdx.x & 63 ) , wi0 = threadIdx.x >> 6 , wstep = 4;
+		if( j>=nc ) return;
+	}
+	int J[Dim];
+	{
+		int t = j;
+#pragma unroll
+		for( int d=Dim-1 ; d>=0 ; d-- ){ const int q = t/( cres+1 ) ; J[d] = t - q*( cres+1 ) ; t = q; }
+	}
+	const long long outBase = lay.col( J )*W;
+	for( int wi=wi0 ; wi<W ; wi+=wstep ) Mc[ outBase + wi ] = GalerkinEntry< Dim >( T , Nf , layF.cstart , J , cres , wi , fmaskC , j );
 }
 
 // All iterations of a small level in one workgroup with the coefficients (and,

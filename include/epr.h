@@ -331,7 +331,7 @@ int  epr_sample_generate( const char *type , unsigned int res , int regular , do
  *
  * epr_time_op: reps launches (after one warm-up) of one kernel of a built
  * context at `level`, on scratch outputs (the context's data is unchanged):
- * "prolong" (KProlong, level -> level+1, out += P x), "galerkin_t"
+ * "prolong" (KProlong2, level -> level+1, x and y: out += P in), "galerkin_t"
  * (KGalerkinT, T = P^T R of the level), "galerkin_c" (KGalerkinC, the coarser
  * level's R from T), "energy" (KEnergy + its fixed-order row sums).  *ms
  * receives milliseconds per launch, *bytes the launch's algorithmic HBM

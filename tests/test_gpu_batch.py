@@ -246,7 +246,7 @@ def test_gpu_measurement_hooks_leave_data():
     before = [[r.coefficients(d) for d in range(6)] for r in recs]
     r = recs[0]
     N = [r.num_functions(d) for d in range(6)]
-    for op, level, expect in (("prolong", 4, 8 * (N[4] + 2 * N[5])), ("galerkin_t", 5, 16 * N[5] * 27),
+    for op, level, expect in (("prolong", 4, 16 * (N[4] + 2 * N[5])), ("galerkin_t", 5, 16 * N[5] * 27),
                               ("galerkin_c", 5, 8 * (N[5] + N[4]) * 27), ("energy", 5, 8 * (2 * N[5] + 2 * N[5] * 27))):
         ms, b = r.time_op(op, level, reps=2)
         assert ms > 0 and b == expect, (op, ms, b, expect)
