@@ -506,7 +506,7 @@ struct Ctx : public CtxBase
 		{
 			bool same = graphX==x.p && graphR==R.p && graphF==fm.p && graphAct==act.p;
 			for( int c=0 ; c<=(1<<Dim) ; c++ ) same = same && graphActStart[c]==actStart[c];
-			if( graph && !same ){ (void)hipGraphExecDestroy( graph ) ; graph = nullptr; }
+			if( graph && !same ){ (void)hipGraphExecDestroy( graph ) ; graph = nullptr; }   // the caller has drained the stream
 		}
 	};
 	std::vector< Level > levels;
@@ -575,7 +575,8 @@ struct Ctx : public CtxBase
 			for( int d=Dim-1 ; d>=0 ; d-- ){ L.stride[d] = s ; s *= L.res+1; }
 		}
 	}
-	~Ctx(){ if( stream ) (void)hipStreamDestroy( stream ); }
+	// the levels' executable graphs are destroyed after this body: drain the stream first
+	~Ctx(){ if( stream ){ (void)hipStreamSynchronize( stream ) ; (void)hipStreamDestroy( stream ); } }
 
 	size_t numFunctions( int l ) const { return (size_t)levels[l].N; }
 	int minD( void ) const { return std::min( std::max( prm.minSolveDepth , 0 ) , D ); }
@@ -822,7 +823,9 @@ struct Ctx : public CtxBase
 			}
 			HIPCHECK( hipGetLastError() );
 		}
-		// ---- stencil case rows and coefficient storage
+		// ---- stencil case rows and coefficient storage (a captured graph that
+		// is dropped may still run from an earlier call: drain the stream first)
+		HIPCHECK( hipStreamSynchronize( stream ) );
 		for( int l=0 ; l<=D ; l++ )
 		{
 			Level &L = levels[l];
@@ -982,7 +985,8 @@ struct Ctx : public CtxBase
 			// of the same size)
 			if( !L.graph || L.graphIters!=iters || L.graphThreads!=pl.threads )
 			{
-				if( L.graph ){ (void)hipGraphExecDestroy( L.graph ) ; L.graph = nullptr; }
+				// an executable graph is destroyed only once its last launch has ended
+				if( L.graph ){ HIPCHECK( hipStreamSynchronize( stream ) ) ; (void)hipGraphExecDestroy( L.graph ) ; L.graph = nullptr; }
 				hipGraph_t g;
 				HIPCHECK( hipStreamBeginCapture( stream , hipStreamCaptureModeThreadLocal ) );
 				for( int it=0 ; it<iters ; it++ ) for( int c=0 ; c<C ; c++ )
@@ -1065,7 +1069,7 @@ struct Ctx : public CtxBase
 		const int threads = floatThreads( l );
 		if( !L.graph32 || L.graph32Iters!=iters || moved || L.graph32Threads!=threads )
 		{
-			if( L.graph32 ){ (void)hipGraphExecDestroy( L.graph32 ) ; L.graph32 = nullptr; }
+			if( L.graph32 ){ HIPCHECK( hipStreamSynchronize( stream ) ) ; (void)hipGraphExecDestroy( L.graph32 ) ; L.graph32 = nullptr; }
 			const LevelView< Dim > v = view( l );
 			LevelView< Dim , float > f;
 			f.res = v.res , f.N = v.N , f.x = L.fx.p , f.y = L.fy.p , f.R = L.fR.p , f.B = L.fB.p , f.K = L.fK.p;
@@ -2259,6 +2263,7 @@ struct BatchExec
 	std::mutex busy;
 	~BatchExec()
 	{
+		if( origin ) (void)hipStreamSynchronize( origin );   // the last replay has ended
 		if( graph ) (void)hipGraphExecDestroy( graph );
 		for( hipEvent_t e : events ) (void)hipEventDestroy( e );
 		for( hipStream_t s : streams ) (void)hipStreamDestroy( s );

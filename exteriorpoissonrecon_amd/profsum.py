@@ -67,13 +67,18 @@ def finest(d, kernel):
         raise SystemExit(f"no dispatches of {kernel} under {d}")
     gmax = max(g for g, _, _ in rows)
     big = [(g, t, r) for g, t, r in rows if g * 4 >= gmax]
-    ts = sorted(t for _, t, _ in big)
-    r0 = big[0][2]
-    return (f"{kernel}, largest-grid dispatches (grid sizes {sorted({g for g, _, _ in big})}): {len(ts)} dispatches, "
-            f"avg {sum(ts) / len(ts) / 1e3:.1f} us, median {ts[len(ts) // 2] / 1e3:.1f} us, "
-            f"min {ts[0] / 1e3:.1f} us, max {ts[-1] / 1e3:.1f} us; "
-            f"VGPR {r0['VGPR_Count']}, AGPR {r0['Accum_VGPR_Count']}, SGPR {r0['SGPR_Count']}, "
-            f"LDS {r0['LDS_Block_Size']} B, scratch {r0['Scratch_Size']} B, workgroup {r0['Workgroup_Size_X']}")
+    out = []
+    # one line per workgroup size (an autotuner's trial launches use other sizes than the picked plan)
+    for wg in sorted({r["Workgroup_Size_X"] for _, _, r in big}, key=int):
+        sel = [(g, t, r) for g, t, r in big if r["Workgroup_Size_X"] == wg]
+        ts = sorted(t for _, t, _ in sel)
+        r0 = sel[0][2]
+        out.append(f"{kernel}, largest-grid dispatches (grid sizes {sorted({g for g, _, _ in sel})}), workgroup {wg}: "
+                   f"{len(ts)} dispatches, avg {sum(ts) / len(ts) / 1e3:.1f} us, median {ts[len(ts) // 2] / 1e3:.1f} us, "
+                   f"min {ts[0] / 1e3:.1f} us, max {ts[-1] / 1e3:.1f} us; "
+                   f"VGPR {r0['VGPR_Count']}, AGPR {r0['Accum_VGPR_Count']}, SGPR {r0['SGPR_Count']}, "
+                   f"LDS {r0['LDS_Block_Size']} B, scratch {r0['Scratch_Size']} B")
+    return "\n\n".join(out)
 
 
 def _counters(d, counter):
