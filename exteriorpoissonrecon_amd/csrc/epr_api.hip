@@ -942,7 +942,13 @@ struct Ctx : public CtxBase
 	template< typename Real >
 	void launchPhase( const LevelView< Dim , Real > &v , const ColourSet< Dim > &cs , long long colBase , int threads=256 )
 	{
-		if( v.N*W < ( 1LL<<31 ) ) KGsPhase< Dim , Real , int ><<< Blocks( cs.total*C , threads ) , threads , 0 , stream >>>( v , cs , colBase );
+		// 32-bit byte offsets while every coefficient offset of the level fits
+		if( v.N*(long long)sizeof(Real) < ( 1LL<<32 ) && !v.fmask )
+		{
+			ColourDiv< Dim > cd ; cd.set( cs );
+			KGsPhase32< Dim , Real ><<< Blocks( cs.total*C , threads ) , threads , 0 , stream >>>( v , cd , colBase );
+		}
+		else if( v.N*W < ( 1LL<<31 ) ) KGsPhase< Dim , Real , int ><<< Blocks( cs.total*C , threads ) , threads , 0 , stream >>>( v , cs , colBase );
 		else KGsPhase< Dim , Real , long long ><<< Blocks( cs.total*C , threads ) , threads , 0 , stream >>>( v , cs , colBase );
 	}
 	void gs( int l , int iters )

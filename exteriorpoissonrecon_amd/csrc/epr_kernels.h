@@ -388,6 +388,15 @@ __device__ __forceinline__ void NodeGroup( const LevelView< Dim , Real > &L , co
 	acc[0] = q20 , acc[1] = q02 , acc[2] = q11 , acc[3] = q10 , acc[4] = q01;
 }
 
+// Unsigned division by a fixed divisor d >= 1 of numbers below 2^31: the
+// high half of n * ceil(2^32/d), one down when that overshoots (exact).
+struct FastDiv
+{
+	unsigned int d , m;
+	__host__ void set( unsigned int _d ){ d = _d , m = (unsigned int)( ( 0xFFFFFFFFull )/_d + 1 ); }
+	__device__ __forceinline__ unsigned int div( unsigned int n ) const { unsigned int q = d==1 ? n : __umulhi( n , m ) ; if( q*d>n ) q-- ; return q; }
+};
+
 // Workgroups are dispatched to the 8 XCDs round robin (workgroup b to XCD
 // b mod 8), each XCD with its own L2.  This maps the workgroups of one XCD to
 // one contiguous range of the work, so neighbouring rows and planes -- which
@@ -451,6 +460,144 @@ __device__ __forceinline__ void GsPhase( const LevelView< Dim , Real > &L , cons
 // Idx = int when N 3^Dim < 2^31 (every index of the level fits), else long long
 template< int Dim , typename Real = double , typename Idx = long long >
 __global__ void __launch_bounds__( 256 ) KGsPhase( LevelView< Dim , Real > L , ColourSet< Dim > cs , long long colBase ){ GsPhase< Dim , Real , Idx >( L , cs , colBase , XcdBlock( blockIdx.x , gridDim.x ) ); }
+
+// ---- The colour phase of a dense level with per-lane constants: the same
+// node groups and arithmetic as GsPhase, with the lane's window and element
+// offsets as precomputed byte offsets and boundary masks, and x, y read at
+// 32-bit byte offsets from the level's base pointers (exact while
+// N sizeof(Real) < 2^32: offsets relative to the node are added modulo 2^32).
+// Several nodes per thread (amortising the lane constants) measured slower.
+
+// A colour's node decomposition with fixed-divisor divisions.
+template< int Dim >
+struct ColourDiv
+{
+	FastDiv cnt[Dim];
+	int par[Dim];
+	unsigned int total;
+	__host__ void set( const ColourSet< Dim > &cs ){ for( int d=0 ; d<Dim ; d++ ) cnt[d].set( (unsigned int)cs.cnt[d] ) , par[d] = cs.par[d] ; total = (unsigned int)cs.total; }
+	__device__ __forceinline__ void node( unsigned int k , int *F ) const
+	{
+#pragma unroll
+		for( int d=Dim-1 ; d>=0 ; d-- ){ const unsigned int q = cnt[d].div( k ) ; F[d] = 2*(int)( k - q*cnt[d].d ) + par[d] ; k = q; }
+	}
+};
+
+// The lane's share of the 3^Dim window (entries lane + m C) and its incident
+// element: byte offsets relative to the node and the coordinates (bit d) in
+// which they step below / above it.
+template< int Dim , typename Real >
+struct GsLane
+{
+	static constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W , NWL = ( W + C - 1 )/C;
+	unsigned int wOff[NWL] , cOff[C];
+	int wNeg[NWL] , wPos[NWL] , eNeg , ePos;
+	bool lastIn;   // lane + (NWL-1) C < W (the earlier entries always are)
+	__device__ __forceinline__ void set( int lane , const int *strideI )
+	{
+#pragma unroll
+		for( int m=0 ; m<NWL ; m++ )
+		{
+			int t = lane + m*C , o = 0 , neg = 0 , pos = 0;
+#pragma unroll
+			for( int d=Dim-1 ; d>=0 ; d-- ){ const int od = t%3 - 1 ; t /= 3 ; o += od*strideI[d] ; if( od<0 ) neg |= 1<<d ; if( od>0 ) pos |= 1<<d; }
+			wOff[m] = (unsigned int)o*(unsigned int)sizeof(Real) , wNeg[m] = neg , wPos[m] = pos;
+		}
+		lastIn = lane + (NWL-1)*C < W;
+		int sig[Dim];
+		eNeg = ePos = 0;
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ ){ const int b = Bit< Dim >( lane , d ) ; sig[d] = b ? strideI[d] : -strideI[d] ; if( b ) ePos |= 1<<d ; else eNeg |= 1<<d; }
+#pragma unroll
+		for( int c=0 ; c<C ; c++ )
+		{
+			int o = 0;
+#pragma unroll
+			for( int d=0 ; d<Dim ; d++ ) if( Bit< Dim >( c , d ) ) o += sig[d];
+			cOff[c] = (unsigned int)o*(unsigned int)sizeof(Real);
+		}
+	}
+};
+template< typename Real > __device__ __forceinline__ Real LoadAt( const Real *base , unsigned int byteOff ){ return *(const Real *)( (const char *)base + byteOff ); }
+
+// NodeGroup< Dim , MODE_GS > of node F (flat index i, column col) with the
+// lane's constants.
+template< int Dim , typename Real >
+__device__ __forceinline__ void NodeGroupGs( const LevelView< Dim , Real > &L , const GsLane< Dim , Real > &ln , const int *F , unsigned int i , long long col , int lane , Real acc[5] )
+{
+	constexpr int C = DimC< Dim >::C , W = DimC< Dim >::W , NWL = GsLane< Dim , Real >::NWL;
+	constexpr bool PREFETCH = Dim==3;
+	Real q20 = 0 , q02 = 0 , q11 = 0 , q10 = 0 , q01 = 0;
+	int cse = 0 , low = 0 , high = 0;
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ){ cse = cse*3 + ( F[d]<=0 ? 0 : ( F[d]>=L.res ? 2 : 1 ) ) ; if( F[d]==0 ) low |= 1<<d ; if( F[d]==L.res ) high |= 1<<d; }
+	const unsigned int ib = i*(unsigned int)sizeof(Real);
+	const Real *Rn = L.R + col*W + lane , *Bn = L.B + col*W + lane , *Kn = L.K + cse*W + lane;
+	Real wx[NWL] , wy[NWL] , wR[NWL] , wB[NWL] , wK[NWL];
+#define EPR_LOAD_WINDOW_GS( m ) \
+	{ \
+		const bool in_ = ( (m)<NWL-1 || ln.lastIn ) && !( ln.wNeg[m] & low ) && !( ln.wPos[m] & high ); \
+		const unsigned int ob_ = ib + ln.wOff[m]; \
+		if( in_ ) wx[m] = LoadAt( L.x , ob_ ) , wy[m] = LoadAt( L.y , ob_ ) , wR[m] = Rn[(m)*C] , wB[m] = Bn[(m)*C] , wK[m] = Kn[(m)*C]; \
+		else      wx[m] = wy[m] = wR[m] = wB[m] = wK[m] = 0; \
+	}
+	if constexpr( PREFETCH )
+	{
+#pragma unroll
+		for( int m=0 ; m<NWL ; m++ ) EPR_LOAD_WINDOW_GS( m )
+	}
+	if( !( ln.eNeg & low ) && !( ln.ePos & high ) )
+	{
+		Real xp[C] , yp[C];
+#pragma unroll
+		for( int c=0 ; c<C ; c++ ){ const unsigned int ob = ib + ln.cOff[c] ; xp[c] = LoadAt( L.x , ob ) , yp[c] = LoadAt( L.y , ob ); }
+		Real a20 , a02 , a11 , a10 , a01 , zy;
+		ElementWedge< Dim , Real >( xp , yp , a20 , a02 , a11 , a10 , a01 , zy );
+		q20 = a20*L.sW , q02 = a02*L.sW , q11 = a11*L.sW , q10 = a10*L.sW , q01 = a01*L.sW;
+		(void)zy;
+	}
+#pragma unroll
+	for( int m=0 ; m<NWL ; m++ )
+	{
+		if constexpr( !PREFETCH ) EPR_LOAD_WINDOW_GS( m )
+		const int w = lane + m*C;
+		const Real xr = wx[m] , yr = wy[m] , Rv = wR[m] , Bv = wB[m] , Kv = wK[m];
+		const Real Dv = Kv*L.dW;
+		if( w==W/2 ){ q20 += Rv + Dv ; q02 += Rv + Dv; }
+		q10 += Real( 2 )*Rv*xr + Real( 2 )*Dv*xr;
+		q01 += Real( 2 )*Rv*yr + Real( 2 )*Dv*yr;
+		if( w!=W/2 ){ q10 -= Real( 2 )*Bv*yr ; q01 += Real( 2 )*Bv*xr; }
+	}
+#undef EPR_LOAD_WINDOW_GS
+	q20 = GroupSum< C >( q20 ) , q02 = GroupSum< C >( q02 ) , q11 = GroupSum< C >( q11 ) , q10 = GroupSum< C >( q10 ) , q01 = GroupSum< C >( q01 );
+	acc[0] = q20 , acc[1] = q02 , acc[2] = q11 , acc[3] = q10 , acc[4] = q01;
+}
+
+// One node group per C lanes (the lane's constants computed once per thread).
+template< int Dim , typename Real >
+__global__ void __launch_bounds__( 256 ) KGsPhase32( LevelView< Dim , Real > L , ColourDiv< Dim > cs , long long colBase )
+{
+	constexpr int C = DimC< Dim >::C;
+	const unsigned int k = ( XcdBlock( blockIdx.x , gridDim.x )*blockDim.x + threadIdx.x )/C;
+	if( k>=cs.total ) return;
+	const int lane = threadIdx.x & (C-1);
+	int strideI[Dim];
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ) strideI[d] = (int)L.stride[d];
+	GsLane< Dim , Real > ln ; ln.set( lane , strideI );
+	int F[Dim];
+	cs.node( k , F );
+	unsigned int i = 0;
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ) i += (unsigned int)F[d]*(unsigned int)strideI[d];
+	Real q[5];
+	NodeGroupGs< Dim , Real >( L , ln , F , i , colBase+k , lane , q );
+	if( lane==0 )
+	{
+		Real dx , dy;
+		if( LevelStep< Dim , Real >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
+	}
+}
 
 // One colour phase over a list of nodes (the active nodes of one colour of an
 // adapted level, dense indices in increasing order); block b of the phase.
@@ -703,14 +850,6 @@ __device__ __forceinline__ void Prolong( const double *xc , double *xf , int cre
 template< int Dim >
 __global__ void __launch_bounds__( 256 ) KProlong( const double *xc , double *xf , int cres , int mode , const unsigned char *fmaskF ){ Prolong< Dim >( xc , xf , cres , mode , fmaskF , (long long)blockIdx.x*blockDim.x + threadIdx.x ); }
 
-// Unsigned division by a fixed divisor d >= 1 of numbers below 2^31: the
-// high half of n * ceil(2^32/d), one down when that overshoots (exact).
-struct FastDiv
-{
-	unsigned int d , m;
-	__host__ void set( unsigned int _d ){ d = _d , m = (unsigned int)( ( 0xFFFFFFFFull )/_d + 1 ); }
-	__device__ __forceinline__ unsigned int div( unsigned int n ) const { unsigned int q = d==1 ? n : __umulhi( n , m ) ; if( q*d>n ) q-- ; return q; }
-};
 // Prolong of x and y together (out (=, +=, -=) P in) for levels whose
 // fine functions number below 2^31: 32-bit indices, the fine multi-index by
 // one fixed-divisor division per coordinate, the coarse index as a base plus
