@@ -195,7 +195,7 @@ def measure_batch(args, steps, warmup, with_cpu, unbatched=True, tune=True):
     per_launch = ms * 1e-3 / launches
     per_node_bytes = B_NODE[3] // 2 if ok else B_NODE[3]
     achieved = F_NODE[3] * nodes / 8 / per_launch / 1e12
-    kname = f"KGsPhaseBatch<3, {'float' if ok else 'double'}, int>"
+    kname = f"KGsPhaseBatch32<3, {'float' if ok else 'double'}>"
     copy_gbs = copy_bandwidth()   # also the calibration kernel of the committed HBM byte counts
     traffic = measured_traffic("batch3d_256" + ("_fp32" if ok else ""))
     peak = FP32_PEAK_TFLOPS if ok else FP64_PEAK_TFLOPS
@@ -441,7 +441,7 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False, fp32=0)
                      else "fp64 vector ALU (78.6 TFLOP/s, = the FP64 matrix peak)",
                      "achieved": achieved, "peak": FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / (FP32_PEAK_TFLOPS if fp32 else FP64_PEAK_TFLOPS), "traffic": traffic,
-                     "kernel": f"{'KGsPhase' if args.aRadius < 0 else 'KGsPhaseList'}<{dim}{', float' if fp32 else ''}> (finest level, one colour)",
+                     "kernel": f"{'KGsPhase32' if args.aRadius < 0 else 'KGsPhaseList'}<{dim}{', float' if fp32 else ''}> (finest level, one colour)",
                      "hbm_gbs_algorithmic": bytes_per_launch / per_launch_s / 1e9,
                      "flops_per_node": F_NODE[dim], "reference_equivalent_tflops": achieved_ref,
                      "us_per_launch": per_launch_s * 1e6, "copy_kernel_gbs": copy_gbs,

@@ -2116,7 +2116,7 @@ struct BatchGroup
 			}
 			for( size_t r : S.smallRs ) KGsSmall< Dim ><<< 1 , 512 , 0 , st >>>( cs[r]->view( d ) , iters( r , d ) );
 			const int th = S.plan.threads;
-			const bool idx32 = L0.N*DimC< Dim >::W < ( 1LL<<31 );
+			const bool idx32 = L0.N*DimC< Dim >::W < ( 1LL<<31 ) , b32 = L0.N*(long long)sizeof(double) < ( 1LL<<32 );
 			if( S.dense32.count )
 			{
 				// --fp32: x, y rounded, the float iterations, x, y widened (Ctx::gsFloat)
@@ -2135,7 +2135,8 @@ struct BatchGroup
 					{
 						ColourSet< Dim > cset ; cset.set( 1<<d , c );
 						const dim3 grid( Blocks( cset.total*C , S.floatThreads ) , (unsigned)k );
-						if( idx32 ) KGsPhaseBatch< Dim , float , int ><<< grid , S.floatThreads , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
+						if( b32 ){ ColourDiv< Dim > cdiv ; cdiv.set( cset ) ; KGsPhaseBatch32< Dim , float ><<< grid , S.floatThreads , 0 , st >>>( F0 , J , cdiv , L0.lay.cstart[c] ); }
+						else if( idx32 ) KGsPhaseBatch< Dim , float , int ><<< grid , S.floatThreads , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
 						else KGsPhaseBatch< Dim , float , long long ><<< grid , S.floatThreads , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
 					}
 				}
@@ -2152,7 +2153,8 @@ struct BatchGroup
 					{
 						ColourSet< Dim > cset ; cset.set( 1<<d , c );
 						const dim3 grid( Blocks( cset.total*C , th ) , (unsigned)k );
-						if( idx32 ) KGsPhaseBatch< Dim , double , int ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
+						if( b32 ){ ColourDiv< Dim > cdiv ; cdiv.set( cset ) ; KGsPhaseBatch32< Dim , double ><<< grid , th , 0 , st >>>( L0 , J , cdiv , L0.lay.cstart[c] ); }
+						else if( idx32 ) KGsPhaseBatch< Dim , double , int ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
 						else KGsPhaseBatch< Dim , double , long long ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
 					}
 				}
@@ -2590,7 +2592,7 @@ void TimeBatchPhases( const std::vector< Ctx< Dim > * > &all , int d , bool fp32
 		F0.sW = (float)L0.sW , F0.dW = 0 , F0.lay = L0.lay , F0.fmask = nullptr , F0.separate = 0;
 		for( int k=0 ; k<Dim ; k++ ) F0.stride[k] = L0.stride[k];
 		const int th = fp32 ? cs[0]->floatThreads( d , true ) : cs[0]->gsPlan( d , true ).threads;
-		const bool idx32 = N*W < ( 1LL<<31 );
+		const bool idx32 = N*W < ( 1LL<<31 ) , b32 = N*(long long)sizeof(double) < ( 1LL<<32 );
 		const unsigned k = (unsigned)cs.size();
 		auto run = [&]
 		{
@@ -2601,13 +2603,15 @@ void TimeBatchPhases( const std::vector< Ctx< Dim > * > &all , int d , bool fp32
 				if( fp32 )
 				{
 					const BatchLevelT< float > *J = (const BatchLevelT< float > *)table.p;
-					if( idx32 ) KGsPhaseBatch< Dim , float , int ><<< grid , th , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
+					if( b32 ){ ColourDiv< Dim > cdiv ; cdiv.set( cset ) ; KGsPhaseBatch32< Dim , float ><<< grid , th , 0 , st >>>( F0 , J , cdiv , L0.lay.cstart[c] ); }
+						else if( idx32 ) KGsPhaseBatch< Dim , float , int ><<< grid , th , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
 					else KGsPhaseBatch< Dim , float , long long ><<< grid , th , 0 , st >>>( F0 , J , cset , L0.lay.cstart[c] );
 				}
 				else
 				{
 					const BatchLevel *J = (const BatchLevel *)table.p;
-					if( idx32 ) KGsPhaseBatch< Dim , double , int ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
+					if( b32 ){ ColourDiv< Dim > cdiv ; cdiv.set( cset ) ; KGsPhaseBatch32< Dim , double ><<< grid , th , 0 , st >>>( L0 , J , cdiv , L0.lay.cstart[c] ); }
+						else if( idx32 ) KGsPhaseBatch< Dim , double , int ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
 					else KGsPhaseBatch< Dim , double , long long ><<< grid , th , 0 , st >>>( L0 , J , cset , L0.lay.cstart[c] );
 				}
 			}

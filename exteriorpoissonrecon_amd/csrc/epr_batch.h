@@ -67,6 +67,14 @@ __global__ void __launch_bounds__( 256 ) KGsPhaseBatch( LevelView< Dim , Real > 
 	const unsigned r = lin / gridDim.x , b = lin - r*gridDim.x;
 	GsPhase< Dim , Real , Idx >( BatchView( L0 , P[r] ) , cs , colBase , b );
 }
+// the same with GsPhase32 (dense levels whose coefficients sit below 4 GiB)
+template< int Dim , typename Real >
+__global__ void __launch_bounds__( 256 ) KGsPhaseBatch32( LevelView< Dim , Real > L0 , const BatchLevelT< Real > *P , ColourDiv< Dim > cs , long long colBase )
+{
+	const unsigned lin = XcdBlock( blockIdx.y*gridDim.x + blockIdx.x , gridDim.x*gridDim.y );
+	const unsigned r = lin / gridDim.x , b = lin - r*gridDim.x;
+	GsPhase32< Dim , Real >( BatchView( L0 , P[r] ) , cs , colBase , b );
+}
 // the adapted counterpart: colour c's active-node list of reconstruction r at lists[ r*2^Dim + c ]
 template< int Dim >
 __global__ void __launch_bounds__( 256 ) KGsPhaseListBatch( LevelView< Dim > L0 , const BatchLevel *P , const BatchList *lists , int c )

@@ -573,12 +573,13 @@ __device__ __forceinline__ void NodeGroupGs( const LevelView< Dim , Real > &L , 
 	acc[0] = q20 , acc[1] = q02 , acc[2] = q11 , acc[3] = q10 , acc[4] = q01;
 }
 
-// One node group per C lanes (the lane's constants computed once per thread).
+// One node group per C lanes (the lane's constants computed once per
+// thread); block b of the phase.  Dense levels only (no active-node mask).
 template< int Dim , typename Real >
-__global__ void __launch_bounds__( 256 ) KGsPhase32( LevelView< Dim , Real > L , ColourDiv< Dim > cs , long long colBase )
+__device__ __forceinline__ void GsPhase32( const LevelView< Dim , Real > &L , const ColourDiv< Dim > &cs , long long colBase , unsigned b )
 {
 	constexpr int C = DimC< Dim >::C;
-	const unsigned int k = ( XcdBlock( blockIdx.x , gridDim.x )*blockDim.x + threadIdx.x )/C;
+	const unsigned int k = ( b*blockDim.x + threadIdx.x )/C;
 	if( k>=cs.total ) return;
 	const int lane = threadIdx.x & (C-1);
 	int strideI[Dim];
@@ -598,6 +599,8 @@ __global__ void __launch_bounds__( 256 ) KGsPhase32( LevelView< Dim , Real > L ,
 		if( LevelStep< Dim , Real >( L , q , dx , dy ) ) L.x[i] += dx , L.y[i] += dy;
 	}
 }
+template< int Dim , typename Real >
+__global__ void __launch_bounds__( 256 ) KGsPhase32( LevelView< Dim , Real > L , ColourDiv< Dim > cs , long long colBase ){ GsPhase32< Dim , Real >( L , cs , colBase , XcdBlock( blockIdx.x , gridDim.x ) ); }
 
 // One colour phase over a list of nodes (the active nodes of one colour of an
 // adapted level, dense indices in increasing order); block b of the phase.

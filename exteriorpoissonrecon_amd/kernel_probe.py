@@ -41,7 +41,7 @@ def main(argv=None):
     top = max(d for _, _, d, _ in inputs)
     deep = [r for r, (_, _, d, _) in zip(recs, inputs) if d == top]
     t, launches, nodes = time_batch_sweeps(deep, top, fp32=bool(fp32), iters=a.iters)
-    print(f"[probe] KGsPhaseBatch<3, {'float' if fp32 else 'double'}, int>: {launches} launches over {len(deep)} inputs, "
+    print(f"[probe] KGsPhaseBatch32<3, {'float' if fp32 else 'double'}>: {launches} launches over {len(deep)} inputs, "
           f"{t * 1e3 / launches:.1f} us per launch ({nodes / 8:.0f} nodes)", flush=True)
     for r in recs:
         r.close()
