@@ -361,3 +361,25 @@ def test_hopf_sampler_matches_reference_sample(tmp_path):
     assert np.abs(pos - P).max() < 1e-12
     unit = lambda k: k / np.linalg.norm(k.reshape(len(k), -1), axis=1)[:, None, None]
     assert np.abs(unit(skew) - unit(K)).max() < 1e-12
+
+
+def test_fixed_divisor_division_is_exact():
+    """The multiply-high division the 32-bit index kernels use (FastDiv in
+    csrc/epr_kernels.h: q = hi32(n * ceil(2^32 / d)), one down when q d > n)
+    equals n // d for every divisor the kernels take (grid extents 2^r + 1 and
+    colour-class extents) and numerators up to 2^31 - 1, boundaries included."""
+    rng = np.random.default_rng(7)
+    divisors = sorted({1, 2, 3} | {(1 << r) + 1 for r in range(0, 14)} | {(1 << r) // 2 + 1 for r in range(1, 14)}
+                      | {(1 << r) // 2 for r in range(1, 14)} | set(rng.integers(1, 1 << 16, 200).tolist()))
+    for d in divisors:
+        m = np.uint64(((1 << 32) - 1) // d + 1)
+        k = np.arange(0, (1 << 31) // d + 2, max(1, ((1 << 31) // d) // 4000), dtype=np.uint64)
+        n = np.concatenate([k * np.uint64(d), k * np.uint64(d) - np.uint64(1), k * np.uint64(d) + np.uint64(d - 1),
+                            rng.integers(0, 1 << 31, 4000).astype(np.uint64), np.array([(1 << 31) - 1], np.uint64)])
+        n = n[(n < np.uint64(1 << 31))]
+        if d == 1:
+            q = n.copy()
+        else:
+            q = (n * m) >> np.uint64(32)
+            q = q - (q * np.uint64(d) > n).astype(np.uint64)
+        assert np.array_equal(q, n // np.uint64(d)), d
