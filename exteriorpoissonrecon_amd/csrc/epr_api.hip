@@ -575,6 +575,12 @@ struct Ctx : public CtxBase
 			for( int d=Dim-1 ; d>=0 ; d-- ){ L.stride[d] = s ; s *= L.res+1; }
 		}
 	}
+	// --fp32: the levels whose colour launches sweep in FP32 (all iterations but
+	// the last): dense levels of at least Fp32MinNodes functions.  The smaller
+	// levels are latency-bound (FP32 gains nothing there) and run the most
+	// iterations, so they stay FP64 like the one-workgroup levels.
+	static constexpr long long Fp32MinNodes = 32768;
+	bool floatLevel( int l ) const { return prm.fp32Sweeps && !adapted() && levels[l].N>=Fp32MinNodes; }
 	// the levels' executable graphs are destroyed after this body: drain the stream first
 	~Ctx(){ if( stream ){ (void)hipStreamSynchronize( stream ) ; (void)hipStreamDestroy( stream ); } }
 
@@ -833,7 +839,7 @@ struct Ctx : public CtxBase
 			L.checkGraph();
 		}
 		// --fp32: the float copies of the operators of the levels swept by colour launches
-		if( prm.fp32Sweeps && !adapted() ) for( int l=0 ; l<=D ; l++ ) if( gsPlan( l ).kind==GS_PHASES || gsPlan( l , true ).kind==GS_PHASES ) narrowOperators( l );
+		for( int l=0 ; l<=D ; l++ ) if( floatLevel( l ) && ( gsPlan( l ).kind==GS_PHASES || gsPlan( l , true ).kind==GS_PHASES ) ) narrowOperators( l );
 		{
 			double csum;
 			dTmp.download( &csum , 1 , stream );
@@ -979,7 +985,7 @@ struct Ctx : public CtxBase
 		else
 		{
 			Level &L = levels[l];
-			if( prm.fp32Sweeps && !L.fm.p && iters>1 )
+			if( floatLevel( l ) && iters>1 )
 			{
 				// --fp32: all iterations but the last in FP32, the last in FP64 from
 				// the widened iterate (the refinement), energies FP64
@@ -1363,7 +1369,7 @@ struct Ctx : public CtxBase
 				}
 				TuneTable::Get().set( Dim , L.res , false , false , TunePick{ bc.first , bc.second , best } );
 				for( size_t k=0 ; k<cand.size() ; k++ ) rep += lines[k] + ( cand[k]==bc ? " *\n" : "\n" );
-				if( ( flags & 1 ) && bc.first==GS_PHASES && !L.fm.p )
+				if( ( flags & 1 ) && bc.first==GS_PHASES && !L.fm.p && L.N>=Fp32MinNodes )
 				{
 					narrowOperators( l );
 					double bestF = 1e300;
@@ -1922,7 +1928,7 @@ struct BatchGroup
 	int floatIters( size_t r , int d ) const
 	{
 		const int it = iters( r , d );
-		return ( cs[r]->prm.fp32Sweeps && !cs[r]->adapted() && it>1 ) ? it-1 : 0;
+		return ( cs[r]->floatLevel( d ) && it>1 ) ? it-1 : 0;
 	}
 	Lv &lev( size_t r , int d ) const { return cs[r]->levels[d]; }
 
@@ -2395,7 +2401,7 @@ void SolveBatch( const std::vector< Ctx< Dim > * > &cs , double *energies )
 	for( Cx *c : cs ) if( c->prm.fp32Sweeps && !c->adapted() )
 	{
 		bool any = false;
-		for( int d=0 ; d<=c->D ; d++ ) if( c->gsPlan( d , true ).kind==Cx::GS_PHASES && !c->levels[d].narrowed ) c->narrowOperators( d ) , any = true;
+		for( int d=0 ; d<=c->D ; d++ ) if( c->floatLevel( d ) && c->gsPlan( d , true ).kind==Cx::GS_PHASES && !c->levels[d].narrowed ) c->narrowOperators( d ) , any = true;
 		if( any ) HIPCHECK( hipStreamSynchronize( c->stream ) );
 	}
 	static const bool useGraph = !( getenv( "EPR_BATCH_GRAPH" ) && atoi( getenv( "EPR_BATCH_GRAPH" ) )==0 );
@@ -2472,7 +2478,7 @@ std::string AutotuneBatch( const std::vector< Ctx< Dim > * > &cs , int flags )
 			// the level's sweeps of every group under the current table
 			auto trial = [&]( void )
 			{
-				for( Cx *c : with ) if( c->prm.fp32Sweeps && !c->adapted() && c->gsPlan( d , true ).kind==Cx::GS_PHASES && !c->levels[d].narrowed )
+				for( Cx *c : with ) if( c->floatLevel( d ) && c->gsPlan( d , true ).kind==Cx::GS_PHASES && !c->levels[d].narrowed )
 				{
 					c->narrowOperators( d );
 					HIPCHECK( hipStreamSynchronize( c->stream ) );
@@ -2508,7 +2514,7 @@ std::string AutotuneBatch( const std::vector< Ctx< Dim > * > &cs , int flags )
 			TuneTable::Get().set( Dim , res , true , false , TunePick{ bc.first , bc.second , best } );
 			for( size_t k=0 ; k<cand.size() ; k++ ) rep += lines[k] + ( cand[k]==bc ? " *\n" : "\n" );
 			bool anyF = false;
-			for( Cx *c : with ) anyF |= c->prm.fp32Sweeps && !c->adapted();
+			for( Cx *c : with ) anyF |= c->floatLevel( d );
 			if( ( flags & 1 ) && bc.first==Cx::GS_PHASES && anyF )
 			{
 				double bestF = 1e300;

@@ -54,10 +54,11 @@ typedef struct
 	                          instead of PairedGaussSeidelSolve */
 	double jitter;         /* --jitter 0: a random translation of the unit-cube samples
 	                          (ExteriorPoissonRecon.cpp:174-180) */
-	int    fp32Sweeps;     /* --fp32 (no reference counterpart, default 0): on dense levels swept by
-	                          colour launches, every iteration of a level but its last runs in
-	                          FP32 (coefficients and R, B, K rounded to float); the last runs in
-	                          FP64 from the widened iterate, and energies stay FP64 */
+	int    fp32Sweeps;     /* --fp32 (no reference counterpart, default 0): on dense levels of at
+	                          least 32768 functions swept by colour launches, every iteration of a
+	                          level but its last runs in FP32 (coefficients and R, B, K rounded to
+	                          float); the last runs in FP64 from the widened iterate; smaller levels
+	                          and the energies stay FP64 */
 } epr_params;
 
 /* Fills the reference defaults (ExteriorPoissonRecon.cpp:56-72). */
