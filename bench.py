@@ -457,6 +457,27 @@ def measure_workload(name, args, steps, warmup, with_cpu, extract=False, fp32=0)
     return res, rec
 
 
+def measure_adapted(args):
+    """The reference's default command line (the adapted hierarchy, --aRadius 3)
+    on both headline workloads, on the GPU only (the reference CLI takes 23 s
+    in 3D and about 15 minutes in 4D for them on 8 host cores): whole
+    reconstructions per step (FP64: --fp32 applies to dense levels) and the
+    Extract pipeline on the .tree each one writes."""
+    import copy
+    a = copy.copy(args)
+    a.aRadius = 3
+    out = {}
+    for name, steps in (("borromean3d_d7", max(3, args.steps // 2)), ("clifford4d_d6", max(2, args.steps_4d // 2))):
+        r, rec = measure_workload(name, a, steps, 1, False, extract=True)
+        rec.close()
+        c = r["config"]
+        out[f"{name}_r3"] = {"ms_per_reconstruction": r["value"] * 1e3, "steps": steps, "warmup": 1, "dtype": "f64",
+                             **{k: c[k] for k in ("workload", "dim", "depth", "aRadius", "grid_nodes", "active_nodes",
+                                                  "set_samples_s", "build_s", "solve_s", "final_energy", "autotune")},
+                             "roofline": r["roofline"], "hbm_kernels": r.get("hbm_kernels"), "extract": r.get("extract")}
+    return out
+
+
 def measure_learned(args):
     """The learned initializer (4D Hopf torus, depth 6): time-to-energy of the
     plain solver (the reference's cascade from its random start at level 0)
@@ -540,6 +561,7 @@ def main():
     ap.add_argument("--detail", action="store_true", help="per-level sweep timings to stderr")
     ap.add_argument("--aRadius", type=int, default=-1, help="-1: dense grid; 0..5: adapted hierarchy of that radius")
     ap.add_argument("--no-tune", action="store_true", help="keep the built-in sweep plans (no epr_autotune before the timed steps)")
+    ap.add_argument("--no-adapted", action="store_true", help="default workload: skip the nested adapted-hierarchy (--aRadius 3) runs with Extract")
     ap.add_argument("--no-batch", action="store_true",
                     help="default workload: skip the nested batched-workload (batch3d_256) measurement")
     ap.add_argument("--steps-batch", type=int, default=5, help="timed steps of the nested batched measurement")
@@ -618,6 +640,9 @@ def main():
             out["clifford4d_d6"]["levels"] = r4["levels"]
     if args.config == "borromean3d_d7" and not args.no_learned:
         out["hopf4d_d6_learned"] = measure_learned(args)
+    # the reference's default command line (adapted hierarchy, radius 3) and Extract, in the same run
+    if args.config == "borromean3d_d7" and not args.no_adapted and args.aRadius < 0:
+        out["adapted"] = measure_adapted(args)
     # the batched serving workload, in the same run
     if args.config == "borromean3d_d7" and not args.no_batch and args.aRadius < 0:
         out["batch3d_256"] = measure_batch(args, args.steps_batch, 1, not args.no_cpu)
