@@ -152,3 +152,31 @@ def test_batch_workload_256():
         assert np.abs(hy - o[f"hy{depth}"]).max() / scale < TOL_SOLVE
     for r in recs:
         r.close()
+
+
+def test_batch_workload_256_fp32_within_bar():
+    """The 256-input batch with --fp32 (FP32 sweeps on the levels of 32768+
+    functions, each level's last iteration FP64): every input's final energy
+    within the metric's 1e-6 of the FP64 batch on the same builds, under the
+    built-in plans and under the autotuner's picks for both precisions."""
+    from exteriorpoissonrecon_amd import Reconstructor, autotune_batch, build_batch, samples, set_tune_table, solve_batch
+    inputs = samples.batch_workload(256)
+    print("[gpu] batch fp32 against fp64", flush=True)
+    recs = [Reconstructor(p, s, depth=d, aRadius=-1, fp32Sweeps=1) for p, s, d, _ in inputs]
+    recs64 = [Reconstructor(p, s, depth=d, aRadius=-1) for p, s, d, _ in inputs]
+    build_batch(recs)
+    build_batch(recs64)
+    try:
+        for tuned in (False, True):
+            if tuned:
+                autotune_batch(recs64, fp32=False)
+                autotune_batch(recs, fp32=True)
+            E32 = solve_batch(recs)
+            E64 = solve_batch(recs64)
+            rel = np.abs(E32.sum(1) - E64.sum(1)) / np.abs(E64.sum(1))
+            print(f"[gpu]   tuned={tuned}: largest relative energy difference {rel.max():.2e} (input {int(rel.argmax())})", flush=True)
+            assert rel.max() < TOL_SOLVE, (tuned, rel.max(), int(rel.argmax()))
+    finally:
+        set_tune_table(None)
+        for r in recs + recs64:
+            r.close()
