@@ -53,6 +53,7 @@ struct SampleCells
 	void bin( size_t n , const double *wpos , double scale , int D , double jitter=0 )
 	{
 		if( !n ) throw std::string( "Expected some samples: 0" );
+		if( n>=( 1ULL<<32 ) ) throw std::string( "more than 2^32 - 1 samples" );
 		for( size_t i=0 ; i<n*Dim ; i++ ) if( !std::isfinite( wpos[i] ) ) throw std::string( "non-finite sample position" );
 		double bb0[Dim] , bb1[Dim];
 		for( int d=0 ; d<Dim ; d++ ) bb0[d] = bb1[d] = wpos[d];
@@ -122,23 +123,29 @@ struct SampleCells
 			memcpy( unitToWorld , U , sizeof(U) );
 		}
 		int res = 1<<D;
-		struct IS { size_t idx ; int I[Dim] ; bool operator<( const IS &o ) const { for( int d=0 ; d<Dim ; d++ ){ if( I[d]<o.I[d] ) return true ; if( I[d]>o.I[d] ) return false; } return false; } };
+		// the cell's multi-index packed into one key, coordinate 0 most significant
+		// (13 bits a coordinate: res <= 2^12), so a key comparison is the
+		// lexicographic comparison of OrderedSamples.inl and std::sort makes the
+		// same comparisons, hence the same permutation, as on the multi-indices
+		constexpr int KB = 13;
+		struct IS { unsigned long long key ; unsigned int idx ; bool operator<( const IS &o ) const { return key<o.key; } };
 		std::vector< IS > v( n );
 		for( size_t i=0 ; i<n ; i++ )
 		{
-			v[i].idx = i;
-			for( int d=0 ; d<Dim ; d++ ) v[i].I[d] = std::max< int >( 0 , std::min< int >( (int)floor( pos[i*Dim+d] * res ) , res-1 ) );
+			unsigned long long k = 0;
+			for( int d=0 ; d<Dim ; d++ ) k = ( k<<KB ) | (unsigned long long)std::max< int >( 0 , std::min< int >( (int)floor( pos[i*Dim+d] * res ) , res-1 ) );
+			v[i].key = k , v[i].idx = (unsigned int)i;
 		}
 		std::sort( v.begin() , v.end() );
 		cellIdx.clear() , cellStart.clear() , cellSamples.clear();
+		cellSamples.reserve( n );
 		size_t start = 0;
 		while( start<v.size() )
 		{
 			size_t end = start;
-			auto same = [&]( size_t a , size_t b ){ for( int d=0 ; d<Dim ; d++ ) if( v[a].I[d]!=v[b].I[d] ) return false ; return true; };
-			while( end<v.size() && same( end , start ) ) end++;
+			while( end<v.size() && v[end].key==v[start].key ) end++;
 			cellStart.push_back( (int)cellSamples.size() );
-			for( int d=0 ; d<Dim ; d++ ) cellIdx.push_back( v[start].I[d] );
+			for( int d=0 ; d<Dim ; d++ ) cellIdx.push_back( (int)( ( v[start].key>>( KB*( Dim-1-d ) ) ) & ( ( 1ULL<<KB )-1 ) ) );
 			for( size_t i=start ; i<end ; i++ ) cellSamples.push_back( (int)v[i].idx );
 			start = end;
 		}
