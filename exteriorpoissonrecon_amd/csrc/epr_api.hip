@@ -6,6 +6,7 @@
 // std::sort so cells and their sample order are identical), plus the O(1)
 // stencil tables.  Every grid-sized computation runs on the device.
 #include <hip/hip_runtime.h>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <chrono>
@@ -19,6 +20,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <numeric>
 #include <set>
 #include <string>
 #include <thread>
@@ -300,6 +302,16 @@ inline void SetLdsAttributes( void )
 // (W, y^T B x, R-form, K-form): EnergySumBlocks contiguous row ranges first
 // (KSumRowsPart, their sums after the rows), then those sums.  EnergyRows:
 // the rows partials must hold.
+// An odd multiplier near n / golden ratio that is coprime with n: k -> (k m) mod n
+// is then a permutation of [0, n) whose consecutive values lie far apart.
+inline unsigned int CoprimeStride( unsigned int n )
+{
+	if( n<=2 ) return 1;
+	unsigned int m = (unsigned int)( 0.6180339887*n ) | 1u;
+	while( std::gcd( m , n )!=1 ) m += 2;
+	return m;
+}
+
 constexpr int EnergySumBlocks = 256;
 template< int Dim > long long EnergyRows( long long N , int res )
 {
@@ -670,14 +682,34 @@ struct Ctx : public CtxBase
 		if( t<=D+1 )
 		{
 			long long tot = (long long)ncell*(D+2-t)*taps;
-			KEstSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p , t );
+			static const bool seg = !( getenv( "EPR_EST_SEG" ) && atoi( getenv( "EPR_EST_SEG" ) )==0 );   // 0: one atomic per cell (for timing)
+			if( seg && Dim*D<=64 )
+			{
+				mortonOrder();
+				KEstSplatSeg< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , dMKey[1].p , dMOrd[1].p , ncell , T , dDepthScale.p , t );
+			}
+			else KEstSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p , t , CoprimeStride( (unsigned int)ncell ) );
 		}
+	}
+	// the occupied cells sorted by Morton key: keys in dMKey[1], cell indices in dMOrd[1]
+	DevBuf< unsigned long long > dMKey[2];
+	DevBuf< int > dMOrd[2];
+	DevBuf< unsigned char > dSortTmp;
+	void mortonOrder( void )
+	{
+		for( int k=0 ; k<2 ; k++ ) dMKey[k].ensure( (size_t)ncell ) , dMOrd[k].ensure( (size_t)ncell );
+		KMortonKeys< Dim ><<< Blocks( ncell , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , D , dMKey[0].p , dMOrd[0].p );
+		HIPCHECK( hipGetLastError() );
+		size_t tb = 0;
+		HIPCHECK( rocprim::radix_sort_pairs( nullptr , tb , dMKey[0].p , dMKey[1].p , dMOrd[0].p , dMOrd[1].p , (unsigned int)ncell , 0 , Dim*D , stream ) );
+		dSortTmp.ensure( std::max< size_t >( tb , 1 ) );
+		HIPCHECK( rocprim::radix_sort_pairs( (void *)dSortTmp.p , tb , dMKey[0].p , dMKey[1].p , dMOrd[0].p , dMOrd[1].p , (unsigned int)ncell , 0 , Dim*D , stream ) );
 	}
 	template< int KR > void launchSkewSplat( TreeGrids< Dim > T )
 	{
 		int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= 2*KR+2;
 		long long tot = (long long)n*taps;
-		KSkewSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dPos.p , dSkewScaled.p , dDepth.p , (int)n , T );
+		KSkewSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dPos.p , dSkewScaled.p , dDepth.p , (int)n , T , CoprimeStride( (unsigned int)n ) );
 	}
 
 	// ---------------- build ----------------

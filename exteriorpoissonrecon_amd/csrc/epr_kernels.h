@@ -1120,8 +1120,11 @@ __global__ void __launch_bounds__( 256 ) KEstSplatLds( const double *cellP , con
 }
 
 // Depths tMin..maxDepth+1: one thread per (cell, depth, tap), FP64 atomics.
+// perm: a multiplier coprime with ncell; the k-th cell processed is cell
+// (k perm) mod ncell, so that neighbouring cells (which share their corners)
+// are not splatted at the same time (3D depth 7: 4.54 -> 4.33 ms per build).
 template< int Dim , int KR >
-__global__ void KEstSplat( const double *cellP , const double *cellData , int ncell , TreeGrids< Dim > T , const double *depthScale , int tMin )
+__global__ void KEstSplat( const double *cellP , const double *cellData , int ncell , TreeGrids< Dim > T , const double *depthScale , int tMin , unsigned int perm )
 {
 	constexpr int NV = DimC< Dim >::NDNI , NT = 2*KR+2;
 	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
@@ -1129,14 +1132,93 @@ __global__ void KEstSplat( const double *cellP , const double *cellData , int nc
 	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
 	if( g>=(long long)ncell*nt*taps ) return;
 	const int tap = (int)( g % taps );
-	const int t = (int)( ( g / taps ) % nt ) + tMin;
-	const int c = (int)( g / ( (long long)taps*nt ) );
+	const long long ct = g / taps;
+	const int t = (int)( ct % nt ) + tMin;
+	const int c = (int)( ( (unsigned long long)( ct / nt )*perm ) % (unsigned int)ncell );
 	int o[Dim];
 	double scl;
 	if( !EstTap< Dim , KR >( cellP + (long long)c*Dim , t , tap , o , scl ) ) return;
 	double *node = T.data[t] + CornerIndex< Dim >( t , o )*NV;
 	const double dw = depthScale[t-1];
 	for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
+}
+
+// Morton key of every occupied cell of depth D (the bits of its coordinates
+// interleaved, coordinate 0 first at each bit, the highest bit first) and its
+// index.  Sorted by key, the cells inside one cell of any coarser depth are
+// consecutive: the depth-t ancestor is the key shifted right by Dim (D+1-t).
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KMortonKeys( const int *cellIdx , int ncell , int D , unsigned long long *key , int *idx )
+{
+	const int c = blockIdx.x*blockDim.x + threadIdx.x;
+	if( c>=ncell ) return;
+	int F[Dim];
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ) F[d] = cellIdx[ (long long)c*Dim+d ];
+	unsigned long long k = 0;
+	for( int b=D-1 ; b>=0 ; b-- )
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ ) k = ( k<<1 ) | (unsigned long long)( ( F[d]>>b ) & 1 );
+	key[c] = k , idx[c] = c;
+}
+
+// KEstSplat over the cells in Morton order: the thread of the first cell of
+// each depth-t cell (and tap) adds the contributions of all the cells inside
+// that depth-t cell in registers and issues one atomic per value, instead of
+// one per cell (the other threads return).  A cell whose mean position lies
+// outside its own cell (rounding, or a position clamped into the grid) adds
+// its own contribution with atomics of its own.
+template< int Dim , int KR >
+__global__ void KEstSplatSeg( const double *cellP , const double *cellData , const unsigned long long *mkey , const int *morder , int ncell , TreeGrids< Dim > T , const double *depthScale , int tMin )
+{
+	constexpr int NV = DimC< Dim >::NDNI , NT = 2*KR+2;
+	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
+	const int D = T.maxDepth , nt = D+2-tMin;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=(long long)ncell*nt*taps ) return;
+	const int tap = (int)( g % taps );
+	const long long ct = g / taps;
+	const int t = (int)( ct % nt ) + tMin;
+	const int i = (int)( ct / nt );
+	const int shift = Dim*( D+1-t );
+	const unsigned long long a = mkey[i]>>shift;
+	if( i>0 && ( mkey[i-1]>>shift )==a ) return;
+	const double dw = depthScale[t-1];
+	double acc[NV];
+	int oh[Dim];
+	bool have = false;
+	for( int j=i ; j<ncell && ( mkey[j]>>shift )==a ; j++ )
+	{
+		const int c = morder[j];
+		int o[Dim];
+		double scl;
+		if( !EstTap< Dim , KR >( cellP + (long long)c*Dim , t , tap , o , scl ) ) continue;
+		if( !have )
+		{
+			have = true;
+#pragma unroll
+			for( int d=0 ; d<Dim ; d++ ) oh[d] = o[d];
+#pragma unroll
+			for( int k=0 ; k<NV ; k++ ) acc[k] = 0;
+		}
+		bool same = true;
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ ) same = same && o[d]==oh[d];
+		if( same )
+		{
+#pragma unroll
+			for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) acc[k] += ( cellData[ (long long)c*NV+k ]*dw )*scl;
+		}
+		else
+		{
+			double *node = T.data[t] + CornerIndex< Dim >( t , o )*NV;
+			for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
+		}
+	}
+	if( !have ) return;
+	double *node = T.data[t] + CornerIndex< Dim >( t , oh )*NV;
+#pragma unroll
+	for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) atomicAdd( node+k , acc[k] );
 }
 
 // Linear interpolation of a tree depth at p (GridSamples.inl:774-809).
@@ -1261,15 +1343,16 @@ __global__ void KEstQuery( const double *pos , const double *skewPacked , int n 
 }
 
 // Skew splat at the fractional sample depth (GridSamples.inl:393-491): one
-// thread per (sample, tap), up to two tree depths, FP64 atomics.
+// thread per (sample, tap), up to two tree depths, FP64 atomics.  perm: the
+// k-th sample processed is sample (k perm) mod n (perm coprime with n).
 template< int Dim , int KR >
-__global__ void KSkewSplat( const double *pos , const double *skewScaled , const double *depth , int n , TreeGrids< Dim > T )
+__global__ void KSkewSplat( const double *pos , const double *skewScaled , const double *depth , int n , TreeGrids< Dim > T , unsigned int perm )
 {
 	constexpr int CD = DimC< Dim >::CD , Deg = 2*KR+1 , NT = Deg+1;
 	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
 	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
 	if( g>=(long long)n*taps ) return;
-	const int s = (int)( g / taps ) , tap = (int)( g % taps );
+	const int s = (int)( ( (unsigned long long)( g / taps )*perm ) % (unsigned int)n ) , tap = (int)( g % taps );
 	const double *p = pos + (long long)s*Dim;
 	const int D = T.maxDepth;
 	const double dep = depth[s] + 1.;
