@@ -682,8 +682,7 @@ struct Ctx : public CtxBase
 		if( t<=D+1 )
 		{
 			long long tot = (long long)ncell*(D+2-t)*taps;
-			static const bool seg = !( getenv( "EPR_EST_SEG" ) && atoi( getenv( "EPR_EST_SEG" ) )==0 );   // 0: one atomic per cell (for timing)
-			if( seg && Dim*D<=64 )
+			if( segSplats( 1 ) )
 			{
 				mortonOrder();
 				KEstSplatSeg< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , dMKey[1].p , dMOrd[1].p , ncell , T , dDepthScale.p , t );
@@ -691,12 +690,27 @@ struct Ctx : public CtxBase
 			else KEstSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p , t , CoprimeStride( (unsigned int)ncell ) );
 		}
 	}
-	// the occupied cells sorted by Morton key: keys in dMKey[1], cell indices in dMOrd[1]
+	// the splats over Morton runs of cells (EPR_EST_SEG bit 0: the estimator
+	// splat, bit 1: the skew splat; a cleared bit: one atomic per cell or sample
+	// and value, for timing)
+	bool segSplats( int which ) const
+	{
+		static const int seg = getenv( "EPR_EST_SEG" ) ? atoi( getenv( "EPR_EST_SEG" ) ) : 3;
+		return ( seg & which ) && Dim*D+4<=64 && ( which!=2 || n<( 1LL<<30 ) );
+	}
+	// the occupied cells sorted by Morton key: keys in dMKey[1], cell indices in
+	// dMOrd[1] (once per build)
 	DevBuf< unsigned long long > dMKey[2];
 	DevBuf< int > dMOrd[2];
 	DevBuf< unsigned char > dSortTmp;
+	// the skew splat's (sample, depth) entries, sorted by (depth, depth-t cell) into [1]
+	DevBuf< unsigned long long > dSKey[2];
+	DevBuf< int > dSVal[2];
+	bool mortonDone = false;
 	void mortonOrder( void )
 	{
+		if( mortonDone ) return;
+		mortonDone = true;
 		for( int k=0 ; k<2 ; k++ ) dMKey[k].ensure( (size_t)ncell ) , dMOrd[k].ensure( (size_t)ncell );
 		KMortonKeys< Dim ><<< Blocks( ncell , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , D , dMKey[0].p , dMOrd[0].p );
 		HIPCHECK( hipGetLastError() );
@@ -708,8 +722,23 @@ struct Ctx : public CtxBase
 	template< int KR > void launchSkewSplat( TreeGrids< Dim > T )
 	{
 		int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= 2*KR+2;
-		long long tot = (long long)n*taps;
-		KSkewSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dPos.p , dSkewScaled.p , dDepth.p , (int)n , T , CoprimeStride( (unsigned int)n ) );
+		if( segSplats( 2 ) )
+		{
+			const long long nent = 2*(long long)n;
+			for( int k=0 ; k<2 ; k++ ) dSKey[k].ensure( (size_t)nent ) , dSVal[k].ensure( (size_t)nent );
+			KSkewEntries< Dim ><<< Blocks( ncell , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , dCellStart.p , dCellSamples.p , dDepth.p , ncell , D , dSKey[0].p , dSVal[0].p );
+			HIPCHECK( hipGetLastError() );
+			size_t tb = 0;
+			HIPCHECK( rocprim::radix_sort_pairs( nullptr , tb , dSKey[0].p , dSKey[1].p , dSVal[0].p , dSVal[1].p , (unsigned int)nent , 0 , Dim*D+4 , stream ) );
+			dSortTmp.ensure( std::max< size_t >( tb , 1 ) );
+			HIPCHECK( rocprim::radix_sort_pairs( (void *)dSortTmp.p , tb , dSKey[0].p , dSKey[1].p , dSVal[0].p , dSVal[1].p , (unsigned int)nent , 0 , Dim*D+4 , stream ) );
+			KSkewSplatRuns< Dim , KR ><<< Blocks( nent*taps , 256 ) , 256 , 0 , stream >>>( dPos.p , dSkewScaled.p , dDepth.p , dSKey[1].p , dSVal[1].p , nent , T );
+		}
+		else
+		{
+			const long long tot = (long long)n*taps;
+			KSkewSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dPos.p , dSkewScaled.p , dDepth.p , (int)n , T , CoprimeStride( (unsigned int)n ) );
+		}
 	}
 
 	// ---------------- build ----------------
@@ -717,6 +746,7 @@ struct Ctx : public CtxBase
 	{
 		if( !n ) throw std::string( "no samples" );
 		for( Level &L : levels ) L.narrowed = false;   // the float copies of R, B follow the new operators
+		mortonDone = false;   // the cells may have changed
 		auto t0 = std::chrono::steady_clock::now();
 		const int res = 1<<D;
 		dPos.upload( pos.data() , pos.size() , stream );

@@ -1167,7 +1167,18 @@ __global__ void __launch_bounds__( 256 ) KMortonKeys( const int *cellIdx , int n
 // that depth-t cell in registers and issues one atomic per value, instead of
 // one per cell (the other threads return).  A cell whose mean position lies
 // outside its own cell (rounding, or a position clamped into the grid) adds
-// its own contribution with atomics of its own.
+// its own contribution with atomics of its own.  A run of cells also starts
+// at every multiple of SplatRun Morton positions, which bounds a thread's
+// serial loop for the coarse depths of large inputs.
+constexpr int SplatRun = 64;
+__device__ __forceinline__ bool SplatRunHead( const unsigned long long *mkey , int i , int shift )
+{
+	return i==0 || ( i % SplatRun )==0 || ( mkey[i-1]>>shift )!=( mkey[i]>>shift );
+}
+__device__ __forceinline__ bool SplatRunNext( const unsigned long long *mkey , int i , int j , int ncell , int shift )
+{
+	return j<ncell && ( j==i || ( j % SplatRun )!=0 ) && ( mkey[j]>>shift )==( mkey[i]>>shift );
+}
 template< int Dim , int KR >
 __global__ void KEstSplatSeg( const double *cellP , const double *cellData , const unsigned long long *mkey , const int *morder , int ncell , TreeGrids< Dim > T , const double *depthScale , int tMin )
 {
@@ -1181,13 +1192,12 @@ __global__ void KEstSplatSeg( const double *cellP , const double *cellData , con
 	const int t = (int)( ct % nt ) + tMin;
 	const int i = (int)( ct / nt );
 	const int shift = Dim*( D+1-t );
-	const unsigned long long a = mkey[i]>>shift;
-	if( i>0 && ( mkey[i-1]>>shift )==a ) return;
+	if( !SplatRunHead( mkey , i , shift ) ) return;
 	const double dw = depthScale[t-1];
 	double acc[NV];
 	int oh[Dim];
 	bool have = false;
-	for( int j=i ; j<ncell && ( mkey[j]>>shift )==a ; j++ )
+	for( int j=i ; SplatRunNext( mkey , i , j , ncell , shift ) ; j++ )
 	{
 		const int c = morder[j];
 		int o[Dim];
@@ -1342,30 +1352,61 @@ __global__ void KEstQuery( const double *pos , const double *skewPacked , int n 
 	}
 }
 
+// The tree depths of a sample's skew splat (GridSamples.inl:393-491): t[0]
+// (and t[1]) with the weights fw (-1: the whole value at one depth).
+template< int Dim >
+__device__ __forceinline__ int SkewDepths( double depth , int D , int ts[2] , double fw[2] )
+{
+	const double dep = depth + 1.;
+	if( !( dep<D+1 ) ){ ts[0] = D+1 , fw[0] = -1 ; return 1; }    // also catches NaN
+	if( dep<=1 ){ ts[0] = 1 , fw[0] = -1 ; return 1; }
+	const int d1 = min( max( (int)floor( dep ) , 1 ) , D );
+	const double dd = dep-d1;
+	ts[0] = d1 , fw[0] = 1.-dd , ts[1] = d1+1 , fw[1] = dd;
+	return 2;
+}
+// tap ii of p at depth t: its corner o and B-spline weight; false outside the grid
+template< int Dim , int KR >
+__device__ __forceinline__ bool SkewTap( const double *p , int t , const int *ii , int *o , double &scl )
+{
+	constexpr int Deg = 2*KR+1 , NT = Deg+1;
+	const int res = 1<<(t-1);
+	scl = 1.;
+	bool in = true;
+	for( int d=0 ; d<Dim ; d++ )
+	{
+		const int E = (int)floor( p[d]*res );
+		double v[NT];
+		BSpline< Deg >( p[d]*res - E , v );
+		scl = scl * v[ ii[d] ];
+		o[d] = E - KR + ii[d];
+		in = in && o[d]>=0 && o[d]<=res;
+	}
+	return in;
+}
+// value c of sample s's contribution at depth t with weight fw, before the tap's weight
+template< int Dim >
+__device__ __forceinline__ double SkewValue( const double *skewScaled , int s , int c , int t , double fw )
+{
+	constexpr int CD = DimC< Dim >::CD;
+	const double v = skewScaled[ (long long)s*CD + c ] , dw = (double)( 1ull<<( (t-1)*Dim ) );
+	return fw<0 ? v*dw : ( v*fw )*dw;
+}
+
 // Skew splat at the fractional sample depth (GridSamples.inl:393-491): one
 // thread per (sample, tap), up to two tree depths, FP64 atomics.  perm: the
 // k-th sample processed is sample (k perm) mod n (perm coprime with n).
 template< int Dim , int KR >
 __global__ void KSkewSplat( const double *pos , const double *skewScaled , const double *depth , int n , TreeGrids< Dim > T , unsigned int perm )
 {
-	constexpr int CD = DimC< Dim >::CD , Deg = 2*KR+1 , NT = Deg+1;
+	constexpr int CD = DimC< Dim >::CD , NT = 2*KR+2;
 	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
 	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
 	if( g>=(long long)n*taps ) return;
 	const int s = (int)( ( (unsigned long long)( g / taps )*perm ) % (unsigned int)n ) , tap = (int)( g % taps );
 	const double *p = pos + (long long)s*Dim;
-	const int D = T.maxDepth;
-	const double dep = depth[s] + 1.;
-	int ts[2] ; double fw[2] ; int nts;
-	auto DW = [&]( int d ){ return (double)( 1<<( d*Dim ) ); };
-	if( !( dep<D+1 ) ){ ts[0] = D+1 , fw[0] = -1 , nts = 1; }    // also catches NaN
-	else if( dep<=1 ){ ts[0] = 1 , fw[0] = -1 , nts = 1; }
-	else
-	{
-		const int d1 = min( max( (int)floor( dep ) , 1 ) , D );
-		const double dd = dep-d1;
-		ts[0] = d1 , fw[0] = 1.-dd , ts[1] = d1+1 , fw[1] = dd , nts = 2;
-	}
+	int ts[2] ; double fw[2];
+	const int nts = SkewDepths< Dim >( depth[s] , T.maxDepth , ts , fw );
 	int ii[Dim];
 	{
 		int tt = tap;
@@ -1373,28 +1414,109 @@ __global__ void KSkewSplat( const double *pos , const double *skewScaled , const
 	}
 	for( int k=0 ; k<nts ; k++ )
 	{
-		const int t = ts[k] , res = 1<<(t-1);
+		const int t = ts[k];
 		int o[Dim];
-		double scl = 1.;
-		bool in = true;
-		for( int d=0 ; d<Dim ; d++ )
-		{
-			const int E = (int)floor( p[d]*res );
-			double v[NT];
-			BSpline< Deg >( p[d]*res - E , v );
-			scl = scl * v[ ii[d] ];
-			o[d] = E - KR + ii[d];
-			in = in && o[d]>=0 && o[d]<=res;
-		}
-		if( !in ) continue;
+		double scl;
+		if( !SkewTap< Dim , KR >( p , t , ii , o , scl ) ) continue;
 		double *node = T.data[t] + CornerIndex< Dim >( t , o )*CD;
-		for( int c=0 ; c<CD ; c++ )
+		for( int c=0 ; c<CD ; c++ ) atomicAdd( node+c , SkewValue< Dim >( skewScaled , s , c , t , fw[k] )*scl );
+	}
+}
+
+// The skew splat's (sample, depth) entries keyed by (depth t, the Morton key
+// of the depth-t cell holding the sample): thread per occupied cell, over its
+// samples; entry 2q+k is the k-th depth of the q-th sample of cellSamples,
+// value 2s+k.  A sample with one depth gets the key SkewNoEntry( Dim , D )
+// (every bit of the sorted range set, above every real key) for k = 1.
+__host__ __device__ inline unsigned long long SkewNoEntry( int Dim , int D ){ return ( 1ull<<( Dim*D+4 ) ) - 1; }
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KSkewEntries( const int *cellIdx , const int *cellStart , const int *cellSamples , const double *depth , int ncell , int D , unsigned long long *key , int *val )
+{
+	const int c = blockIdx.x*blockDim.x + threadIdx.x;
+	if( c>=ncell ) return;
+	int F[Dim];
+#pragma unroll
+	for( int d=0 ; d<Dim ; d++ ) F[d] = cellIdx[ (long long)c*Dim+d ];
+	unsigned long long m = 0;
+	for( int b=D-1 ; b>=0 ; b-- )
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ ) m = ( m<<1 ) | (unsigned long long)( ( F[d]>>b ) & 1 );
+	for( int q=cellStart[c] ; q<cellStart[c+1] ; q++ )
+	{
+		const int s = cellSamples[q];
+		int ts[2] ; double fw[2];
+		const int nts = SkewDepths< Dim >( depth[s] , D , ts , fw );
+#pragma unroll
+		for( int k=0 ; k<2 ; k++ )
 		{
-			const double v = skewScaled[ (long long)s*CD + c ];
-			const double dv = fw[k]<0 ? v*DW( t-1 ) : ( v*fw[k] )*DW( t-1 );
-			atomicAdd( node+c , dv*scl );
+			key[ 2*(long long)q+k ] = k<nts ? ( (unsigned long long)ts[k]<<( Dim*D ) ) | ( m>>( Dim*( D+1-ts[k] ) ) ) : SkewNoEntry( Dim , D );
+			val[ 2*(long long)q+k ] = 2*s+k;
 		}
 	}
+}
+
+// KSkewSplat over the sorted entries: one thread per (entry, tap); the thread
+// of the first entry of a run of equal keys (a run also starts at every
+// multiple of SplatRun entries) adds the run's contributions to the depth-t
+// cell in registers and issues one atomic per value (a sample whose position
+// lies outside its own cell adds its own share by atomics), the others return.
+template< int Dim , int KR >
+__global__ void KSkewSplatRuns( const double *pos , const double *skewScaled , const double *depth , const unsigned long long *key , const int *val , long long nent , TreeGrids< Dim > T )
+{
+	constexpr int CD = DimC< Dim >::CD , NT = 2*KR+2;
+	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
+	const int D = T.maxDepth;
+	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	if( g>=nent*taps ) return;
+	const int tap = (int)( g % taps );
+	const long long e = g / taps;
+	const unsigned long long ke = key[e];
+	if( ke==SkewNoEntry( Dim , D ) ) return;
+	if( e>0 && ( e % SplatRun )!=0 && key[e-1]==ke ) return;
+	int ii[Dim];
+	{
+		int tt = tap;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ ii[d] = tt%NT ; tt /= NT; }
+	}
+	const int t = (int)( ke>>( Dim*D ) );
+	double acc[CD];
+	int oh[Dim];
+	bool have = false;
+	for( long long f=e ; f<nent && ( f==e || ( f % SplatRun )!=0 ) && key[f]==ke ; f++ )
+	{
+		const int s = val[f]>>1 , k = val[f] & 1;
+		int ts[2] ; double fw[2];
+		SkewDepths< Dim >( depth[s] , D , ts , fw );
+		const double w = fw[k];
+		int o[Dim];
+		double scl;
+		if( !SkewTap< Dim , KR >( pos + (long long)s*Dim , t , ii , o , scl ) ) continue;
+		if( !have )
+		{
+			have = true;
+#pragma unroll
+			for( int d=0 ; d<Dim ; d++ ) oh[d] = o[d];
+#pragma unroll
+			for( int q=0 ; q<CD ; q++ ) acc[q] = 0;
+		}
+		bool same = true;
+#pragma unroll
+		for( int d=0 ; d<Dim ; d++ ) same = same && o[d]==oh[d];
+		if( same )
+		{
+#pragma unroll
+			for( int q=0 ; q<CD ; q++ ) acc[q] += SkewValue< Dim >( skewScaled , s , q , t , w )*scl;
+		}
+		else
+		{
+			double *node = T.data[t] + CornerIndex< Dim >( t , o )*CD;
+			for( int q=0 ; q<CD ; q++ ) atomicAdd( node+q , SkewValue< Dim >( skewScaled , s , q , t , w )*scl );
+		}
+	}
+	if( !have ) return;
+	double *node = T.data[t] + CornerIndex< Dim >( t , oh )*CD;
+#pragma unroll
+	for( int q=0 ; q<CD ; q++ ) atomicAdd( node+q , acc[q] );
 }
 
 // cmap[e] = c for every occupied cell c (element index e with coordinate 0 fastest)
