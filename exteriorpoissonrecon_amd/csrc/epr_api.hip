@@ -865,7 +865,9 @@ struct Ctx : public CtxBase
 			Level &F = levels[D];
 			F.R.ensure( (size_t)F.N*W ) , F.B.ensure( (size_t)F.N*W );
 			// B
-			KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay , F.em.p , F.fm.p );
+			static const bool bcols = !( getenv( "EPR_BCOLS" ) && atoi( getenv( "EPR_BCOLS" ) )==0 );   // 0: one thread per node in grid order (for timing)
+			if( bcols && !F.em.p && !F.fm.p ) KAssembleBCols< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay );
+			else KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay , F.em.p , F.fm.p );
 			// R
 			if( prm.sWeight>0 )
 			{

@@ -1811,6 +1811,72 @@ __device__ __forceinline__ void BOutputs( const double (&S)[1<<Dim][DimC< Dim >:
 	}() , ... );
 }
 
+// outputs W0 + ws of a node, to out[ws] (stride 1)
+template< int Dim , int W0 , int ... ws >
+__device__ __forceinline__ void BOutputsChunk( const double (&S)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , double *out , std::integer_sequence< int , ws... > )
+{
+	( [&]{
+		constexpr int w = W0 + ws;
+		bool in = true;
+		for( int d=0 ; d<Dim ; d++ ){ const int g = F[d] + WOff< Dim >( w , d ) ; in = in && g>=0 && g<=res; }
+		out[ws] = in ? __dmul_rn( BOutput< Dim , w >( S , std::make_integer_sequence< int , 1<<Dim >{} ) , scale ) : 0.;
+	}() , ... );
+}
+template< int Dim , int Q , int CH >
+__device__ __forceinline__ void BColumnsChunk( const double (&Sv)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , bool live , double *stage , double *B , long long k0 , long long N )
+{
+	constexpr int W = DimC< Dim >::W;
+	const int lane = threadIdx.x & 63;
+	if( live ) BOutputsChunk< Dim , Q*CH >( Sv , F , res , scale , stage + lane*CH , std::make_integer_sequence< int , CH >{} );
+	__syncthreads();
+	for( int i=lane ; i<64*CH ; i+=64 )
+	{
+		const int cl = i / CH , wl = i - cl*CH;
+		if( k0+cl<N ) B[ ( k0+cl )*W + Q*CH + wl ] = stage[i];
+	}
+	__syncthreads();
+}
+template< int Dim , int CH , int ... qs >
+__device__ __forceinline__ void BColumns( const double (&Sv)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , bool live , double *stage , double *B , long long k0 , long long N , std::integer_sequence< int , qs... > )
+{
+	( BColumnsChunk< Dim , qs , CH >( Sv , F , res , scale , live , stage , B , k0 , N ) , ... );
+}
+
+// KAssembleBNode for dense levels, one thread per column of B (the nodes in
+// the column order, colour by colour): each wavefront stages 27 outputs of
+// its 64 columns in LDS at a time and writes them as consecutive runs of
+// 27 doubles, instead of one strided double per lane and output.  The values
+// are KAssembleBNode's, bit for bit.
+template< int Dim >
+__global__ void __launch_bounds__( 256 ) KAssembleBCols( const double *S , int res , double scale , double *B , ColLayout< Dim > lay )
+{
+	constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD , CH = 27;
+	__shared__ double stage[4][64*CH];
+	const long long N = lay.cstart[C];
+	const long long k = (long long)blockIdx.x*blockDim.x + threadIdx.x;
+	const long long k0 = k - ( threadIdx.x & 63 );
+	const bool live = k<N;
+	int F[Dim];
+	double Sv[C][CD];
+	{
+		int c = 0;
+#pragma unroll
+		for( int q=1 ; q<C ; q++ ) c += k>=lay.cstart[q] ? 1 : 0;
+		ColourSet< Dim > cs ; cs.set( res , c );
+		cs.node( live ? k - lay.cstart[c] : 0 , F );
+	}
+#pragma unroll
+	for( int b=0 ; b<C ; b++ )
+	{
+		bool in = live;
+		long long e = 0;
+		for( int d=Dim-1 ; d>=0 ; d-- ){ const int E = F[d]-1+Bit< Dim >( b , d ) ; in = in && E>=0 && E<res ; e = e*res + E; }
+#pragma unroll
+		for( int c=0 ; c<CD ; c++ ) Sv[b][c] = in ? S[ e*CD + c ] : 0.;
+	}
+	BColumns< Dim , CH >( Sv , F , res , scale , live , stage[ threadIdx.x>>6 ] , B , k0 , N , std::make_integer_sequence< int , W/CH >{} );
+}
+
 template< int Dim >
 __global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int res , double scale , double *B , ColLayout< Dim > lay , const unsigned char *emask , const unsigned char *fmask )
 {
