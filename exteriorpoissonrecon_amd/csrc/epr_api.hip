@@ -685,7 +685,9 @@ struct Ctx : public CtxBase
 			if( segSplats( 1 ) )
 			{
 				mortonOrder();
-				KEstSplatSeg< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , dMKey[1].p , dMOrd[1].p , ncell , T , dDepthScale.p , t );
+				static const bool wide = !( getenv( "EPR_EST_WIDE" ) && atoi( getenv( "EPR_EST_WIDE" ) )==0 );   // 0: per-lane atomics (for timing)
+				if( wide && taps%64==0 ) KEstSplatSeg< Dim , KR , true ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , dMKey[1].p , dMOrd[1].p , ncell , T , dDepthScale.p , t );
+				else KEstSplatSeg< Dim , KR , false ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , dMKey[1].p , dMOrd[1].p , ncell , T , dDepthScale.p , t );
 			}
 			else KEstSplat< Dim , KR ><<< Blocks( tot , 256 ) , 256 , 0 , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p , t , CoprimeStride( (unsigned int)ncell ) );
 		}
@@ -732,7 +734,9 @@ struct Ctx : public CtxBase
 			HIPCHECK( rocprim::radix_sort_pairs( nullptr , tb , dSKey[0].p , dSKey[1].p , dSVal[0].p , dSVal[1].p , (unsigned int)nent , 0 , Dim*D+4 , stream ) );
 			dSortTmp.ensure( std::max< size_t >( tb , 1 ) );
 			HIPCHECK( rocprim::radix_sort_pairs( (void *)dSortTmp.p , tb , dSKey[0].p , dSKey[1].p , dSVal[0].p , dSVal[1].p , (unsigned int)nent , 0 , Dim*D+4 , stream ) );
-			KSkewSplatRuns< Dim , KR ><<< Blocks( nent*taps , 256 ) , 256 , 0 , stream >>>( dPos.p , dSkewScaled.p , dDepth.p , dSKey[1].p , dSVal[1].p , nent , T );
+			static const bool wide = !( getenv( "EPR_EST_WIDE" ) && atoi( getenv( "EPR_EST_WIDE" ) )==0 );
+			if( wide && taps%64==0 ) KSkewSplatRuns< Dim , KR , true ><<< Blocks( nent*taps , 256 ) , 256 , 0 , stream >>>( dPos.p , dSkewScaled.p , dDepth.p , dSKey[1].p , dSVal[1].p , nent , T );
+			else KSkewSplatRuns< Dim , KR , false ><<< Blocks( nent*taps , 256 ) , 256 , 0 , stream >>>( dPos.p , dSkewScaled.p , dDepth.p , dSKey[1].p , dSVal[1].p , nent , T );
 		}
 		else
 		{
@@ -866,7 +870,7 @@ struct Ctx : public CtxBase
 			F.R.ensure( (size_t)F.N*W ) , F.B.ensure( (size_t)F.N*W );
 			// B
 			static const bool bcols = !( getenv( "EPR_BCOLS" ) && atoi( getenv( "EPR_BCOLS" ) )==0 );   // 0: one thread per node in grid order (for timing)
-			if( bcols && !F.em.p && !F.fm.p ) KAssembleBCols< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay );
+			if( bcols && Dim==3 && !F.em.p && !F.fm.p ) KAssembleBCols< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay );
 			else KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay , F.em.p , F.fm.p );
 			// R
 			if( prm.sWeight>0 )

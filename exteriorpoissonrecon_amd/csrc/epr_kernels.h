@@ -1143,6 +1143,15 @@ __global__ void KEstSplat( const double *cellP , const double *cellData , int nc
 	for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
 }
 
+// Orders a wavefront's LDS accesses on either side (a wave's own LDS
+// operations complete in order, so this only constrains the compiler).
+__device__ __forceinline__ void WaveSync( void )
+{
+	__builtin_amdgcn_fence( __ATOMIC_RELEASE , "wavefront" );
+	__builtin_amdgcn_wave_barrier();
+	__builtin_amdgcn_fence( __ATOMIC_ACQUIRE , "wavefront" );
+}
+
 // Morton key of every occupied cell of depth D (the bits of its coordinates
 // interleaved, coordinate 0 first at each bit, the highest bit first) and its
 // index.  Sorted by key, the cells inside one cell of any coarser depth are
@@ -1179,8 +1188,12 @@ __device__ __forceinline__ bool SplatRunNext( const unsigned long long *mkey , i
 {
 	return j<ncell && ( j==i || ( j % SplatRun )!=0 ) && ( mkey[j]>>shift )==( mkey[i]>>shift );
 }
-template< int Dim , int KR >
-__global__ void KEstSplatSeg( const double *cellP , const double *cellData , const unsigned long long *mkey , const int *morder , int ncell , TreeGrids< Dim > T , const double *depthScale , int tMin )
+// WIDE (taps a multiple of 64, so that a wavefront holds one cell and depth):
+// the run's sums go through LDS and each atomic instruction covers
+// consecutive values of consecutive nodes (the taps of a wavefront run along
+// the last coordinate) instead of one value of 64 scattered nodes.
+template< int Dim , int KR , bool WIDE >
+__global__ void __launch_bounds__( 256 ) KEstSplatSeg( const double *cellP , const double *cellData , const unsigned long long *mkey , const int *morder , int ncell , TreeGrids< Dim > T , const double *depthScale , int tMin )
 {
 	constexpr int NV = DimC< Dim >::NDNI , NT = 2*KR+2;
 	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
@@ -1192,11 +1205,16 @@ __global__ void KEstSplatSeg( const double *cellP , const double *cellData , con
 	const int t = (int)( ct % nt ) + tMin;
 	const int i = (int)( ct / nt );
 	const int shift = Dim*( D+1-t );
-	if( !SplatRunHead( mkey , i , shift ) ) return;
+	if( !SplatRunHead( mkey , i , shift ) ) return;   // uniform over a wavefront when WIDE
 	const double dw = depthScale[t-1];
 	double acc[NV];
 	int oh[Dim];
 	bool have = false;
+	if constexpr( WIDE )
+	{
+#pragma unroll
+		for( int k=0 ; k<NV ; k++ ) acc[k] = 0;
+	}
 	for( int j=i ; SplatRunNext( mkey , i , j , ncell , shift ) ; j++ )
 	{
 		const int c = morder[j];
@@ -1225,10 +1243,30 @@ __global__ void KEstSplatSeg( const double *cellP , const double *cellData , con
 			for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) atomicAdd( node+k , ( cellData[ (long long)c*NV+k ]*dw )*scl );
 		}
 	}
-	if( !have ) return;
-	double *node = T.data[t] + CornerIndex< Dim >( t , oh )*NV;
+	if constexpr( WIDE )
+	{
+		__shared__ double sv[4][64*NV];
+		__shared__ long long sn[4][64];
+		const int lane = threadIdx.x & 63 , wv = threadIdx.x>>6;
 #pragma unroll
-	for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) atomicAdd( node+k , acc[k] );
+		for( int k=0 ; k<NV ; k++ ) sv[wv][ lane*NV+k ] = acc[k];
+		sn[wv][lane] = have ? CornerIndex< Dim >( t , oh )*NV : -1;
+		WaveSync();
+		for( int q=lane ; q<64*NV ; q+=64 )
+		{
+			const int src = q / NV , k = q - src*NV;
+			const long long n0 = sn[wv][src];
+			const double v = sv[wv][q];
+			if( n0>=0 && !EstLower< Dim >( k ) && v!=0 ) atomicAdd( T.data[t] + n0 + k , v );
+		}
+	}
+	else
+	{
+		if( !have ) return;
+		double *node = T.data[t] + CornerIndex< Dim >( t , oh )*NV;
+#pragma unroll
+		for( int k=0 ; k<NV ; k++ ) if( !EstLower< Dim >( k ) ) atomicAdd( node+k , acc[k] );
+	}
 }
 
 // Linear interpolation of a tree depth at p (GridSamples.inl:774-809).
@@ -1460,8 +1498,10 @@ __global__ void __launch_bounds__( 256 ) KSkewEntries( const int *cellIdx , cons
 // multiple of SplatRun entries) adds the run's contributions to the depth-t
 // cell in registers and issues one atomic per value (a sample whose position
 // lies outside its own cell adds its own share by atomics), the others return.
-template< int Dim , int KR >
-__global__ void KSkewSplatRuns( const double *pos , const double *skewScaled , const double *depth , const unsigned long long *key , const int *val , long long nent , TreeGrids< Dim > T )
+// WIDE: taps a multiple of 64, the run's sums flushed through LDS as in
+// KEstSplatSeg.
+template< int Dim , int KR , bool WIDE >
+__global__ void __launch_bounds__( 256 ) KSkewSplatRuns( const double *pos , const double *skewScaled , const double *depth , const unsigned long long *key , const int *val , long long nent , TreeGrids< Dim > T )
 {
 	constexpr int CD = DimC< Dim >::CD , NT = 2*KR+2;
 	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
@@ -1482,6 +1522,11 @@ __global__ void KSkewSplatRuns( const double *pos , const double *skewScaled , c
 	double acc[CD];
 	int oh[Dim];
 	bool have = false;
+	if constexpr( WIDE )
+	{
+#pragma unroll
+		for( int q=0 ; q<CD ; q++ ) acc[q] = 0;
+	}
 	for( long long f=e ; f<nent && ( f==e || ( f % SplatRun )!=0 ) && key[f]==ke ; f++ )
 	{
 		const int s = val[f]>>1 , k = val[f] & 1;
@@ -1513,10 +1558,30 @@ __global__ void KSkewSplatRuns( const double *pos , const double *skewScaled , c
 			for( int q=0 ; q<CD ; q++ ) atomicAdd( node+q , SkewValue< Dim >( skewScaled , s , q , t , w )*scl );
 		}
 	}
-	if( !have ) return;
-	double *node = T.data[t] + CornerIndex< Dim >( t , oh )*CD;
+	if constexpr( WIDE )
+	{
+		__shared__ double sv[4][64*CD];
+		__shared__ long long sn[4][64];
+		const int lane = threadIdx.x & 63 , wv = threadIdx.x>>6;
 #pragma unroll
-	for( int q=0 ; q<CD ; q++ ) atomicAdd( node+q , acc[q] );
+		for( int q=0 ; q<CD ; q++ ) sv[wv][ lane*CD+q ] = acc[q];
+		sn[wv][lane] = have ? CornerIndex< Dim >( t , oh )*CD : -1;
+		WaveSync();
+		for( int r=lane ; r<64*CD ; r+=64 )
+		{
+			const int src = r / CD , q = r - src*CD;
+			const long long n0 = sn[wv][src];
+			const double v = sv[wv][r];
+			if( n0>=0 && v!=0 ) atomicAdd( T.data[t] + n0 + q , v );
+		}
+	}
+	else
+	{
+		if( !have ) return;
+		double *node = T.data[t] + CornerIndex< Dim >( t , oh )*CD;
+#pragma unroll
+		for( int q=0 ; q<CD ; q++ ) atomicAdd( node+q , acc[q] );
+	}
 }
 
 // cmap[e] = c for every occupied cell c (element index e with coordinate 0 fastest)
@@ -2249,14 +2314,6 @@ __device__ __forceinline__ double SwapAdd( double v )
 		l0 = l[0] , l1 = l[1] , h0 = h[0] , h1 = h[1];
 	}
 	return __longlong_as_double( (long long)( ( (unsigned long long)h0<<32 ) | l0 ) ) + __longlong_as_double( (long long)( ( (unsigned long long)h1<<32 ) | l1 ) );
-}
-// Orders a wavefront's LDS accesses on either side (a wave's own LDS
-// operations complete in order, so this only constrains the compiler).
-__device__ __forceinline__ void WaveSync( void )
-{
-	__builtin_amdgcn_fence( __ATOMIC_RELEASE , "wavefront" );
-	__builtin_amdgcn_wave_barrier();
-	__builtin_amdgcn_fence( __ATOMIC_ACQUIRE , "wavefront" );
 }
 
 // The step of NodeStep with the reciprocal of det from v_rcp_f64 refined by
