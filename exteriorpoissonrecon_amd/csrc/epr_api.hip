@@ -668,17 +668,28 @@ struct Ctx : public CtxBase
 	template< int KR > void launchEstSplat( TreeGrids< Dim > T )
 	{
 		int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= 2*KR+2;
-		// depths whose corner values fit in LDS are summed per workgroup first
-		int t = 1;
+		// depths whose corner values fit in LDS are summed per workgroup first:
+		// those that fit twice into a CU's LDS in one launch (blockIdx.y = depth
+		// - tLo, every workgroup sized for the largest), a larger one on its own
+		// (so that it does not limit the smaller depths' workgroups to one a CU)
+		const long long items = (long long)ncell*taps;
+		const unsigned blocks = (unsigned)std::max< long long >( 1 , std::min< long long >( 256 , items/4096 ) );
+		int t = 1 , tLo = 1;
+		size_t lds = 0;
+		auto flush = [&]( int tEnd )
+		{
+			if( tEnd>tLo ) KEstSplatLds< Dim , KR ><<< dim3( blocks , tEnd-tLo ) , 256 , lds , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p , tLo );
+			tLo = tEnd , lds = 0;
+		};
 		for( ; t<=D+1 ; t++ )
 		{
 			long long corners = 1 ; for( int d=0 ; d<Dim ; d++ ) corners *= ( 1LL<<(t-1) ) + 1;
-			const size_t lds = (size_t)corners*NV*sizeof(double);
-			if( lds>LdsLimit ) break;
-			const long long items = (long long)ncell*taps;
-			const unsigned blocks = (unsigned)std::max< long long >( 1 , std::min< long long >( 256 , items/4096 ) );
-			KEstSplatLds< Dim , KR ><<< blocks , 256 , lds , stream >>>( dCellP.p , dCellData.p , ncell , T , dDepthScale.p , t );
+			const size_t b = (size_t)corners*NV*sizeof(double);
+			if( b>LdsLimit ) break;
+			if( b>LdsLimit/2 ){ flush( t ) ; lds = b ; flush( t+1 ); }
+			else lds = b;
 		}
+		flush( t );
 		if( t<=D+1 )
 		{
 			long long tot = (long long)ncell*(D+2-t)*taps;

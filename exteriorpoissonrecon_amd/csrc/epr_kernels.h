@@ -1095,9 +1095,11 @@ template< int Dim > __host__ __device__ inline void EstMirror( double *v )
 	for( int i=1 ; i<Dim ; i++ ) for( int j=0 ; j<i ; j++ ) v[ 1+Dim+i*Dim+j ] = v[ 1+Dim+j*Dim+i ];
 }
 
+// Depth t = tLo + blockIdx.y (several depths in one launch).
 template< int Dim , int KR >
-__global__ void __launch_bounds__( 256 ) KEstSplatLds( const double *cellP , const double *cellData , int ncell , TreeGrids< Dim > T , const double *depthScale , int t )
+__global__ void __launch_bounds__( 256 ) KEstSplatLds( const double *cellP , const double *cellData , int ncell , TreeGrids< Dim > T , const double *depthScale , int tLo )
 {
+	const int t = tLo + (int)blockIdx.y;
 	constexpr int NV = DimC< Dim >::NDNI , NT = 2*KR+2;
 	extern __shared__ double acc[];
 	int taps = 1 ; for( int d=0 ; d<Dim ; d++ ) taps *= NT;
