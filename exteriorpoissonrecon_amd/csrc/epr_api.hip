@@ -739,7 +739,7 @@ struct Ctx : public CtxBase
 		{
 			const long long nent = 2*(long long)n;
 			for( int k=0 ; k<2 ; k++ ) dSKey[k].ensure( (size_t)nent ) , dSVal[k].ensure( (size_t)nent );
-			KSkewEntries< Dim ><<< Blocks( ncell , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , dCellStart.p , dCellSamples.p , dDepth.p , ncell , D , dSKey[0].p , dSVal[0].p );
+			KSkewEntries< Dim ><<< Blocks( (long long)n , 256 ) , 256 , 0 , stream >>>( dPos.p , dDepth.p , (int)n , D , dSKey[0].p , dSVal[0].p );
 			HIPCHECK( hipGetLastError() );
 			size_t tb = 0;
 			HIPCHECK( rocprim::radix_sort_pairs( nullptr , tb , dSKey[0].p , dSKey[1].p , dSVal[0].p , dSVal[1].p , (unsigned int)nent , 0 , Dim*D+4 , stream ) );

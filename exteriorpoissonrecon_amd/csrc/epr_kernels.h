@@ -1464,34 +1464,32 @@ __global__ void KSkewSplat( const double *pos , const double *skewScaled , const
 }
 
 // The skew splat's (sample, depth) entries keyed by (depth t, the Morton key
-// of the depth-t cell holding the sample): thread per occupied cell, over its
-// samples; entry 2q+k is the k-th depth of the q-th sample of cellSamples,
-// value 2s+k.  A sample with one depth gets the key SkewNoEntry( Dim , D )
-// (every bit of the sorted range set, above every real key) for k = 1.
+// of the depth-t cell holding the sample): one thread per sample, whose cell
+// of depth D is its clamped floor at resolution 2^D, as the host's binning
+// takes it; entry 2s+k is the k-th depth of sample s, value 2s+k.  A sample
+// with one depth gets the key SkewNoEntry( Dim , D ) (every bit of the sorted
+// range set, above every real key) for k = 1.
 __host__ __device__ inline unsigned long long SkewNoEntry( int Dim , int D ){ return ( 1ull<<( Dim*D+4 ) ) - 1; }
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KSkewEntries( const int *cellIdx , const int *cellStart , const int *cellSamples , const double *depth , int ncell , int D , unsigned long long *key , int *val )
+__global__ void __launch_bounds__( 256 ) KSkewEntries( const double *pos , const double *depth , int n , int D , unsigned long long *key , int *val )
 {
-	const int c = blockIdx.x*blockDim.x + threadIdx.x;
-	if( c>=ncell ) return;
+	const int s = blockIdx.x*blockDim.x + threadIdx.x;
+	if( s>=n ) return;
+	const int res = 1<<D;
 	int F[Dim];
 #pragma unroll
-	for( int d=0 ; d<Dim ; d++ ) F[d] = cellIdx[ (long long)c*Dim+d ];
+	for( int d=0 ; d<Dim ; d++ ) F[d] = max( 0 , min( (int)floor( pos[ (long long)s*Dim+d ]*res ) , res-1 ) );
 	unsigned long long m = 0;
 	for( int b=D-1 ; b>=0 ; b-- )
 #pragma unroll
 		for( int d=0 ; d<Dim ; d++ ) m = ( m<<1 ) | (unsigned long long)( ( F[d]>>b ) & 1 );
-	for( int q=cellStart[c] ; q<cellStart[c+1] ; q++ )
-	{
-		const int s = cellSamples[q];
-		int ts[2] ; double fw[2];
-		const int nts = SkewDepths< Dim >( depth[s] , D , ts , fw );
+	int ts[2] ; double fw[2];
+	const int nts = SkewDepths< Dim >( depth[s] , D , ts , fw );
 #pragma unroll
-		for( int k=0 ; k<2 ; k++ )
-		{
-			key[ 2*(long long)q+k ] = k<nts ? ( (unsigned long long)ts[k]<<( Dim*D ) ) | ( m>>( Dim*( D+1-ts[k] ) ) ) : SkewNoEntry( Dim , D );
-			val[ 2*(long long)q+k ] = 2*s+k;
-		}
+	for( int k=0 ; k<2 ; k++ )
+	{
+		key[ 2*(long long)s+k ] = k<nts ? ( (unsigned long long)ts[k]<<( Dim*D ) ) | ( m>>( Dim*( D+1-ts[k] ) ) ) : SkewNoEntry( Dim , D );
+		val[ 2*(long long)s+k ] = 2*s+k;
 	}
 }
 
