@@ -323,10 +323,16 @@ template< int Dim > void LaunchEnergy( const LevelView< Dim > &v , double *parti
 {
 	constexpr int C = DimC< Dim >::C , NP = Dim*( Dim-1 )/2;
 	long long ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= v.res;
-	const unsigned nbW = Blocks( v.N*C , 256 ) , nbE = Blocks( ne*NP , 256 );
+	static const bool perElement = !( getenv( "EPR_WEDGE_ELEM" ) && atoi( getenv( "EPR_WEDGE_ELEM" ) )==0 );   // 0: one thread per (element, pair) (for timing)
+	const unsigned nbW = Blocks( v.N*C , 256 ) , nbE = perElement ? Blocks( ne , 256 ) : Blocks( ne*NP , 256 );
 	if( v.N*DimC< Dim >::W<( 1LL<<31 ) ) KEnergy< Dim , int ><<< nbW , 256 , 0 , st >>>( v , partials );
 	else KEnergy< Dim ><<< nbW , 256 , 0 , st >>>( v , partials );
-	if( ne*NP<( 1LL<<31 ) && v.N<( 1LL<<31 ) ) KEnergyWedge< Dim , int ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
+	if( perElement )
+	{
+		if( ne<( 1LL<<31 ) && v.N<( 1LL<<31 ) ) KEnergyWedgeElem< Dim , int ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
+		else KEnergyWedgeElem< Dim ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
+	}
+	else if( ne*NP<( 1LL<<31 ) && v.N<( 1LL<<31 ) ) KEnergyWedge< Dim , int ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
 	else KEnergyWedge< Dim ><<< nbE , 256 , 0 , st >>>( v , partials + 4*(size_t)nbW );
 	double *sums = partials + 4*( (size_t)nbW + nbE );
 	KSumRowsPart<4><<< EnergySumBlocks , 256 , 0 , st >>>( partials , (long long)nbW + nbE , sums );

@@ -765,6 +765,77 @@ __global__ void __launch_bounds__( 256 ) KEnergyWedge( LevelView< Dim > L , doub
 	if( threadIdx.x<4 ) partials[ (long long)blockIdx.x*4 + threadIdx.x ] = threadIdx.x==0 ? sh[0] : 0.;
 }
 
+// Pair p (in (i, j) order, i < j) of Dim coordinates; corner c' of the
+// pair's order (bit q: coordinate order[q], order = (i, j, the others
+// ascending)) as a corner of the element's own order (bit d: coordinate d).
+template< int Dim > __host__ __device__ constexpr int PairFirst( int p ){ for( int a=0 ; a<Dim ; a++ ) for( int b=a+1 ; b<Dim ; b++ ){ if( p==0 ) return a ; p--; } return -1; }
+template< int Dim > __host__ __device__ constexpr int PairSecond( int p ){ for( int a=0 ; a<Dim ; a++ ) for( int b=a+1 ; b<Dim ; b++ ){ if( p==0 ) return b ; p--; } return -1; }
+template< int Dim > __host__ __device__ constexpr int PairCorner( int p , int cp )
+{
+	const int a = PairFirst< Dim >( p ) , b = PairSecond< Dim >( p );
+	int ord[Dim] = {};
+	ord[0] = a , ord[1] = b;
+	int q = 2;
+	for( int d=0 ; d<Dim ; d++ ) if( d!=a && d!=b ) ord[q++] = d;
+	int c = 0;
+	for( int k=0 ; k<Dim ; k++ ) if( ( cp>>k )&1 ) c |= 1<<ord[k];
+	return c;
+}
+template< int Dim , int p , int ... cps >
+__device__ __forceinline__ double ElementPairEnergy( const double *xs , const double *ys , std::integer_sequence< int , cps... > )
+{
+	constexpr int C = DimC< Dim >::C;
+	const double xc[C] = { xs[ PairCorner< Dim >( p , cps ) ]... } , yc[C] = { ys[ PairCorner< Dim >( p , cps ) ]... };
+	return PairWedgeEnergy< Dim >( xc , yc );
+}
+template< int Dim , int ... ps >
+__device__ __forceinline__ double ElementWedgeEnergy( const double *xs , const double *ys , double sW , std::integer_sequence< int , ps... > )
+{
+	constexpr int C = DimC< Dim >::C;
+	double w = 0;
+	( ( w += ElementPairEnergy< Dim , ps >( xs , ys , std::make_integer_sequence< int , C >{} )*sW ) , ... );
+	return w;
+}
+
+// KEnergyWedge with one thread per element and all its coordinate pairs: the
+// element's 2^Dim corner pairs are loaded once instead of once per pair (the
+// per-thread sum adds the pairs in order, then the block sums as KEnergyWedge
+// does).  Rows (W, 0, 0, 0).  Idx: int where the element count and N are
+// below 2^31.
+template< int Dim , typename Idx = long long >
+__global__ void __launch_bounds__( 256 , 2 ) KEnergyWedgeElem( LevelView< Dim > L , double *partials )
+{
+	constexpr int C = DimC< Dim >::C , NP = Dim*( Dim-1 )/2;
+	__shared__ double sh[256];
+	Idx ne = 1 ; for( int d=0 ; d<Dim ; d++ ) ne *= L.res;
+	const Idx e = (Idx)blockIdx.x*blockDim.x + threadIdx.x;
+	double w = 0;
+	if( e<ne )
+	{
+		int F[Dim];
+		{
+			Idx t = e;
+#pragma unroll
+			for( int d=Dim-1 ; d>=0 ; d-- ){ const Idx q = t / L.res ; F[d] = (int)( t - q*L.res ) ; t = q; }
+		}
+		const Idx i = (Idx)FlatIndex< Dim >( F , L.stride );
+		double xs[C] , ys[C];
+#pragma unroll
+		for( int c=0 ; c<C ; c++ )
+		{
+			Idx o = i;
+#pragma unroll
+			for( int d=0 ; d<Dim ; d++ ) if( (c>>d)&1 ) o += (Idx)L.stride[d];
+			xs[c] = L.x[o] , ys[c] = L.y[o];
+		}
+		w = ElementWedgeEnergy< Dim >( xs , ys , L.sW , std::make_integer_sequence< int , NP >{} );
+	}
+	sh[threadIdx.x] = w;
+	__syncthreads();
+	for( int h=128 ; h>0 ; h>>=1 ){ if( threadIdx.x<h ) sh[threadIdx.x] += sh[threadIdx.x+h] ; __syncthreads(); }
+	if( threadIdx.x<4 ) partials[ (long long)blockIdx.x*4 + threadIdx.x ] = threadIdx.x==0 ? sh[0] : 0.;
+}
+
 // Deterministic sum of nb rows of nv values (one block).
 template< int NV >
 __global__ void __launch_bounds__( 256 ) KSumRows( const double *in , long long nb , double *out )
