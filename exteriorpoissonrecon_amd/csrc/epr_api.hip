@@ -507,6 +507,7 @@ struct Ctx : public CtxBase
 		// and the captured float launches
 		DevBuf< float > fx , fy , fR , fB , fK;
 		bool narrowed = false;
+		bool fusedR = false , fusedB = false;   // fR, fB written by this build's assembly
 		hipGraphExec_t graph32 = nullptr;
 		int graph32Iters = 0 , graph32Threads = 0;
 		const float *graph32P[5] = { nullptr , nullptr , nullptr , nullptr , nullptr };
@@ -766,7 +767,7 @@ struct Ctx : public CtxBase
 	void build( void )
 	{
 		if( !n ) throw std::string( "no samples" );
-		for( Level &L : levels ) L.narrowed = false;   // the float copies of R, B follow the new operators
+		for( Level &L : levels ) L.narrowed = false , L.fusedR = L.fusedB = false;   // the float copies of R, B follow the new operators
 		mortonDone = false;   // the cells may have changed
 		auto t0 = std::chrono::steady_clock::now();
 		const int res = 1<<D;
@@ -885,9 +886,17 @@ struct Ctx : public CtxBase
 		{
 			Level &F = levels[D];
 			F.R.ensure( (size_t)F.N*W ) , F.B.ensure( (size_t)F.N*W );
+			// the FP32 sweeps' copies of the finest R and B written by the assembly
+			// itself where it can (instead of a narrowing pass that reads them back)
+			const bool f32 = floatLevel( D );
+			if( f32 ) F.fR.ensure( (size_t)F.N*W ) , F.fB.ensure( (size_t)F.N*W );
 			// B
 			static const bool bcols = !( getenv( "EPR_BCOLS" ) && atoi( getenv( "EPR_BCOLS" ) )==0 );   // 0: one thread per node in grid order (for timing)
-			if( bcols && Dim==3 && !F.em.p && !F.fm.p ) KAssembleBCols< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay );
+			if( bcols && Dim==3 && !F.em.p && !F.fm.p )
+			{
+				KAssembleBCols< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay , f32 ? F.fB.p : nullptr );
+				F.fusedB = f32;
+			}
 			else KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay , F.em.p , F.fm.p );
 			// R
 			if( prm.sWeight>0 )
@@ -903,11 +912,13 @@ struct Ctx : public CtxBase
 				KMeanMeasure<<< 1 , 256 , 0 , stream >>>( dPartials.p , nb , (long long)n , prm.sWeight , dScale.p );
 				// R is zero but in the columns of nodes next to occupied cells
 				dNodeFlag.ensure( (size_t)F.N ) , dNodeList.ensure( std::min< size_t >( (size_t)F.N , (size_t)ncell<<Dim ) ) , dNodeCount.ensure( 1 );
-				zeroSpans( { { (void *)F.R.p , (size_t)F.N*W*sizeof(double) } , { (void *)dNodeFlag.p , (size_t)F.N*sizeof(int) } , { (void *)dNodeCount.p , sizeof(unsigned int) } } );
+				if( f32 ) zeroSpans( { { (void *)F.R.p , (size_t)F.N*W*sizeof(double) } , { (void *)F.fR.p , (size_t)F.N*W*sizeof(float) } , { (void *)dNodeFlag.p , (size_t)F.N*sizeof(int) } , { (void *)dNodeCount.p , sizeof(unsigned int) } } );
+				else zeroSpans( { { (void *)F.R.p , (size_t)F.N*W*sizeof(double) } , { (void *)dNodeFlag.p , (size_t)F.N*sizeof(int) } , { (void *)dNodeCount.p , sizeof(unsigned int) } } );
 				KOccupiedNodes< Dim ><<< Blocks( (long long)ncell<<Dim , 256 ) , 256 , 0 , stream >>>( dCellIdx.p , ncell , res , dNodeFlag.p , dNodeList.p , dNodeCount.p );
 				dPosS.ensure( n*Dim ) , dRWS.ensure( n );
 				KCellOrder< Dim ><<< Blocks( (long long)n , 256 ) , 256 , 0 , stream >>>( dPos.p , dRW.p , dCellSamples.p , (int)n , dPosS.p , dRWS.p );
-				KAssembleR< Dim ><<< Blocks( std::min< long long >( F.N , (long long)ncell<<Dim )*W , 256 ) , 256 , 0 , stream >>>( dPosS.p , dCellMap.p , dCellStart.p , dRWS.p , res , dScale.p , F.R.p , F.lay , dNodeList.p , dNodeCount.p );
+				KAssembleR< Dim ><<< Blocks( std::min< long long >( F.N , (long long)ncell<<Dim )*W , 256 ) , 256 , 0 , stream >>>( dPosS.p , dCellMap.p , dCellStart.p , dRWS.p , res , dScale.p , F.R.p , F.lay , dNodeList.p , dNodeCount.p , f32 ? F.fR.p : nullptr );
+				F.fusedR = f32;
 			}
 			else F.R.zero( stream );
 			HIPCHECK( hipGetLastError() );
@@ -1150,8 +1161,8 @@ struct Ctx : public CtxBase
 		const long long N = L.N;
 		if( L.narrowed ) return;
 		L.fx.ensure( N ) , L.fy.ensure( N ) , L.fR.ensure( (size_t)N*W ) , L.fB.ensure( (size_t)N*W ) , L.fK.ensure( (size_t)W*W );
-		KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.R.p , L.fR.p , N*W );
-		KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.B.p , L.fB.p , N*W );
+		if( !L.fusedR ) KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.R.p , L.fR.p , N*W );
+		if( !L.fusedB ) KNarrow<<< Blocks( N*W , 256 ) , 256 , 0 , stream >>>( L.B.p , L.fB.p , N*W );
 		KNarrow<<< Blocks( (long long)W*W , 256 ) , 256 , 0 , stream >>>( L.K.p , L.fK.p , (long long)W*W );
 		HIPCHECK( hipGetLastError() );
 		L.narrowed = true;

@@ -1872,7 +1872,7 @@ __global__ void __launch_bounds__( 256 ) KOccupiedNodes( const int *cellIdx , in
 template< int Dim >
 __global__ void __launch_bounds__( 256 ) KAssembleR( const double *posS , const int *cellMap , const int *cellStart , const double *rwS ,
                                                       int res , const double *scale , double *R , ColLayout< Dim > lay ,
-                                                      const unsigned int *list , const unsigned int *count )
+                                                      const unsigned int *list , const unsigned int *count , float *fR )
 {
 	constexpr int W = DimC< Dim >::W;
 	const long long g = (long long)blockIdx.x*blockDim.x + threadIdx.x;
@@ -1882,7 +1882,10 @@ __global__ void __launch_bounds__( 256 ) KAssembleR( const double *posS , const 
 	Unflatten< Dim >( list[ g/W ] , res , F );
 	bool any;
 	const double v = REntry< Dim >( posS , cellMap , cellStart , rwS , res , F , w , any );
-	R[ lay.col( F )*W + w ] = any ? v * scale[0] * scale[1] : 0;
+	const double r = any ? v * scale[0] * scale[1] : 0;
+	const long long o = lay.col( F )*W + w;
+	R[o] = r;
+	if( fR ) fR[o] = (float)r;   // the FP32 sweeps' copy (optional)
 }
 
 // ---- B assembly, one thread per finest node f1 and all 3^Dim outputs of its
@@ -1959,7 +1962,7 @@ __device__ __forceinline__ void BOutputsChunk( const double (&S)[1<<Dim][DimC< D
 	}() , ... );
 }
 template< int Dim , int Q , int CH >
-__device__ __forceinline__ void BColumnsChunk( const double (&Sv)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , bool live , double *stage , double *B , long long k0 , long long N )
+__device__ __forceinline__ void BColumnsChunk( const double (&Sv)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , bool live , double *stage , double *B , float *fB , long long k0 , long long N )
 {
 	constexpr int W = DimC< Dim >::W;
 	const int lane = threadIdx.x & 63;
@@ -1968,23 +1971,29 @@ __device__ __forceinline__ void BColumnsChunk( const double (&Sv)[1<<Dim][DimC< 
 	for( int i=lane ; i<64*CH ; i+=64 )
 	{
 		const int cl = i / CH , wl = i - cl*CH;
-		if( k0+cl<N ) B[ ( k0+cl )*W + Q*CH + wl ] = stage[i];
+		if( k0+cl<N )
+		{
+			const double v = stage[i];
+			B[ ( k0+cl )*W + Q*CH + wl ] = v;
+			if( fB ) fB[ ( k0+cl )*W + Q*CH + wl ] = (float)v;
+		}
 	}
 	__syncthreads();
 }
 template< int Dim , int CH , int ... qs >
-__device__ __forceinline__ void BColumns( const double (&Sv)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , bool live , double *stage , double *B , long long k0 , long long N , std::integer_sequence< int , qs... > )
+__device__ __forceinline__ void BColumns( const double (&Sv)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , bool live , double *stage , double *B , float *fB , long long k0 , long long N , std::integer_sequence< int , qs... > )
 {
-	( BColumnsChunk< Dim , qs , CH >( Sv , F , res , scale , live , stage , B , k0 , N ) , ... );
+	( BColumnsChunk< Dim , qs , CH >( Sv , F , res , scale , live , stage , B , fB , k0 , N ) , ... );
 }
 
 // KAssembleBNode for dense levels, one thread per column of B (the nodes in
 // the column order, colour by colour): each wavefront stages 27 outputs of
 // its 64 columns in LDS at a time and writes them as consecutive runs of
 // 27 doubles, instead of one strided double per lane and output.  The values
-// are KAssembleBNode's, bit for bit.
+// are KAssembleBNode's, bit for bit.  fB (optional): the same rounded to float
+// (the FP32 sweeps' copy, written here instead of by a narrowing pass).
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KAssembleBCols( const double *S , int res , double scale , double *B , ColLayout< Dim > lay )
+__global__ void __launch_bounds__( 256 ) KAssembleBCols( const double *S , int res , double scale , double *B , ColLayout< Dim > lay , float *fB )
 {
 	constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD , CH = 27;
 	__shared__ double stage[4][64*CH];
@@ -2010,7 +2019,7 @@ __global__ void __launch_bounds__( 256 ) KAssembleBCols( const double *S , int r
 #pragma unroll
 		for( int c=0 ; c<CD ; c++ ) Sv[b][c] = in ? S[ e*CD + c ] : 0.;
 	}
-	BColumns< Dim , CH >( Sv , F , res , scale , live , stage[ threadIdx.x>>6 ] , B , k0 , N , std::make_integer_sequence< int , W/CH >{} );
+	BColumns< Dim , CH >( Sv , F , res , scale , live , stage[ threadIdx.x>>6 ] , B , fB , k0 , N , std::make_integer_sequence< int , W/CH >{} );
 }
 
 template< int Dim >
