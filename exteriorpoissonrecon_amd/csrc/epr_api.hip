@@ -888,8 +888,9 @@ struct Ctx : public CtxBase
 			F.R.ensure( (size_t)F.N*W ) , F.B.ensure( (size_t)F.N*W );
 			// the FP32 sweeps' copies of the finest R and B written by the assembly
 			// itself where it can (instead of a narrowing pass that reads them back)
-			const bool f32 = floatLevel( D );
-			if( f32 ) F.fR.ensure( (size_t)F.N*W ) , F.fB.ensure( (size_t)F.N*W );
+			static const bool fuse = !( getenv( "EPR_FUSE_NARROW" ) && atoi( getenv( "EPR_FUSE_NARROW" ) )==0 );   // 0: a narrowing pass (for timing)
+			const bool f32 = fuse && floatLevel( D );
+			if( floatLevel( D ) ) F.fR.ensure( (size_t)F.N*W ) , F.fB.ensure( (size_t)F.N*W );
 			// B
 			static const bool bcols = !( getenv( "EPR_BCOLS" ) && atoi( getenv( "EPR_BCOLS" ) )==0 );   // 0: one thread per node in grid order (for timing)
 			if( bcols && Dim==3 && !F.em.p && !F.fm.p )
@@ -897,7 +898,9 @@ struct Ctx : public CtxBase
 				KAssembleBCols< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay , f32 ? F.fB.p : nullptr );
 				F.fusedB = f32;
 			}
-			else KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay , F.em.p , F.fm.p );
+			// (4D: a float copy written here, 81 strided stores a thread, made the
+			// FP32 build slower, 51.5 -> 53.4 ms, than the narrowing pass)
+			else KAssembleBNode< Dim ><<< Blocks( F.N , 256 ) , 256 , 0 , stream >>>( dS.p , res , std::pow( (double)res , 2.-Dim ) , F.B.p , F.lay , F.em.p , F.fm.p , nullptr );
 			// R
 			if( prm.sWeight>0 )
 			{

@@ -1941,12 +1941,14 @@ __device__ __forceinline__ double BOutput( const double (&S)[1<<Dim][DimC< Dim >
 	return dot;
 }
 template< int Dim , int ... ws >
-__device__ __forceinline__ void BOutputs( const double (&S)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , double *col , std::integer_sequence< int , ws... > )
+__device__ __forceinline__ void BOutputs( const double (&S)[1<<Dim][DimC< Dim >::CD] , const int *F , int res , double scale , double *col , float *fcol , std::integer_sequence< int , ws... > )
 {
 	( [&]{
 		bool in = true;
 		for( int d=0 ; d<Dim ; d++ ){ const int g = F[d] + WOff< Dim >( ws , d ) ; in = in && g>=0 && g<=res; }
-		col[ws] = in ? __dmul_rn( BOutput< Dim , ws >( S , std::make_integer_sequence< int , 1<<Dim >{} ) , scale ) : 0.;
+		const double v = in ? __dmul_rn( BOutput< Dim , ws >( S , std::make_integer_sequence< int , 1<<Dim >{} ) , scale ) : 0.;
+		col[ws] = v;
+		if( fcol ) fcol[ws] = (float)v;
 	}() , ... );
 }
 
@@ -2022,8 +2024,10 @@ __global__ void __launch_bounds__( 256 ) KAssembleBCols( const double *S , int r
 	BColumns< Dim , CH >( Sv , F , res , scale , live , stage[ threadIdx.x>>6 ] , B , fB , k0 , N , std::make_integer_sequence< int , W/CH >{} );
 }
 
+// fB (optional, dense levels): the values rounded to float as well (the FP32
+// sweeps' copy).
 template< int Dim >
-__global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int res , double scale , double *B , ColLayout< Dim > lay , const unsigned char *emask , const unsigned char *fmask )
+__global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int res , double scale , double *B , ColLayout< Dim > lay , const unsigned char *emask , const unsigned char *fmask , float *fB )
 {
 	constexpr int W = DimC< Dim >::W , C = DimC< Dim >::C , CD = DimC< Dim >::CD;
 	long long nf = 1 ; for( int d=0 ; d<Dim ; d++ ) nf *= res+1;
@@ -2043,7 +2047,7 @@ __global__ void __launch_bounds__( 256 ) KAssembleBNode( const double *S , int r
 		for( int c=0 ; c<CD ; c++ ) Sv[b][c] = in ? S[ e*CD + c ] : 0.;
 	}
 	double *col = B + lay.col( F )*W;
-	BOutputs< Dim >( Sv , F , res , scale , col , std::make_integer_sequence< int , W >{} );
+	BOutputs< Dim >( Sv , F , res , scale , col , fB && !fmask ? fB + lay.col( F )*W : nullptr , std::make_integer_sequence< int , W >{} );
 	if( fmask )   // adapted levels: entries only between indexed functions
 	{
 		for( int w=0 ; w<W ; w++ )
