@@ -658,10 +658,10 @@ struct Ctx : public CtxBase
 	}
 
 	// T = P^T M of a fine level into dGalT, all parity classes in one launch
-	void galerkinT( const double *Mf , const Level &Fl )
+	void galerkinT( const double *Mf , const Level &Fl , float *fMf=nullptr )
 	{
 		ColourSet< Dim > cs0 ; cs0.set( Fl.res , 0 );   // class 0 (all coordinates even) is the largest
-		KGalerkinT< Dim ><<< dim3( Blocks( cs0.total , 64 ) , C ) , 64*( W/27 ) , 0 , stream >>>( Mf , dGalT.p , Fl.N , Fl.res/2 , Fl.lay );
+		KGalerkinT< Dim ><<< dim3( Blocks( cs0.total , 64 ) , C ) , 64*( W/27 ) , 0 , stream >>>( Mf , dGalT.p , Fl.N , Fl.res/2 , Fl.lay , fMf );
 	}
 	// the coarser level's M from T (galerkinT of the level above it)
 	void galerkinC( double *Mc , const Level &Cl , const Level &Fl )
@@ -930,8 +930,13 @@ struct Ctx : public CtxBase
 			{
 				Level &Cl = levels[l] , &Fl = levels[l+1];
 				Cl.R.ensure( (size_t)Cl.N*W ) , Cl.B.ensure( (size_t)Cl.N*W );
-				galerkinT( Fl.R.p , Fl ) , galerkinC( Cl.R.p , Cl , Fl );
-				galerkinT( Fl.B.p , Fl ) , galerkinC( Cl.B.p , Cl , Fl );
+				// the FP32 sweeps' copies of a float level's R and B written as
+				// the restriction reads them (those its assembly has not written)
+				const bool f32 = fuse && floatLevel( l+1 );
+				if( f32 ) Fl.fR.ensure( (size_t)Fl.N*W ) , Fl.fB.ensure( (size_t)Fl.N*W );
+				galerkinT( Fl.R.p , Fl , f32 && !Fl.fusedR ? Fl.fR.p : nullptr ) , galerkinC( Cl.R.p , Cl , Fl );
+				galerkinT( Fl.B.p , Fl , f32 && !Fl.fusedB ? Fl.fB.p : nullptr ) , galerkinC( Cl.B.p , Cl , Fl );
+				if( f32 ) Fl.fusedR = Fl.fusedB = true;
 			}
 			HIPCHECK( hipGetLastError() );
 		}

@@ -2142,13 +2142,19 @@ __device__ __forceinline__ void GalerkinChunkWave( int wave , const double *col 
 // Chunk by chunk (27 column entries each): stage the block's columns in LDS,
 // then each wavefront scatters its lanes' columns into its outputs.
 template< int Dim , int CLS , int ... CHs >
-__device__ __forceinline__ void GalerkinTChunks( const double *Mf , long long col0 , int lim , bool have , int wave , const int *K , int fres , int cres , double *buf , double *acc , std::integer_sequence< int , CHs... > )
+__device__ __forceinline__ void GalerkinTChunks( const double *Mf , float *fMf , long long col0 , int lim , bool have , int wave , const int *K , int fres , int cres , double *buf , double *acc , std::integer_sequence< int , CHs... > )
 {
 	constexpr int W = DimC< Dim >::W , NW = W/27;
 	const int lane = threadIdx.x & 63;
 	( [&]{
 		if( CHs ) __syncthreads();
-		for( int e=threadIdx.x ; e<64*27 ; e+=64*NW ) buf[e] = e<lim ? Mf[ ( col0 + e/27 )*W + 27*CHs + e%27 ] : 0.;
+		for( int e=threadIdx.x ; e<64*27 ; e+=64*NW )
+		{
+			const long long o = ( col0 + e/27 )*W + 27*CHs + e%27;
+			const double v = e<lim ? Mf[o] : 0.;
+			buf[e] = v;
+			if( fMf && e<lim ) fMf[o] = (float)v;   // every entry of Mf is loaded once: its float copy
+		}
 		__syncthreads();
 		if( have ) GalerkinChunkWave< Dim , CLS , CHs >( wave , buf + lane*27 , K , fres , cres , acc , std::make_integer_sequence< int , NW >{} );
 	}() , ... );
@@ -2163,7 +2169,7 @@ __device__ __forceinline__ void GalerkinTChunks( const double *Mf , long long co
 // scattered from LDS into the accumulators, and leave one offset plane at a
 // time (contiguous stores).
 template< int Dim , int CLS >
-__device__ __forceinline__ void GalerkinTBlock( const double *Mf , double *T , long long Nf , int cres , const ColourSet< Dim > &cs , long long colBase , unsigned b , double *buf )
+__device__ __forceinline__ void GalerkinTBlock( const double *Mf , float *fMf , double *T , long long Nf , int cres , const ColourSet< Dim > &cs , long long colBase , unsigned b , double *buf )
 {
 	constexpr int W = DimC< Dim >::W , NCH = W/27;
 	const int lane = threadIdx.x & 63 , wave = threadIdx.x >> 6 , fres = 2*cres;
@@ -2177,28 +2183,29 @@ __device__ __forceinline__ void GalerkinTBlock( const double *Mf , double *T , l
 	double acc[27];
 #pragma unroll
 	for( int w=0 ; w<27 ; w++ ) acc[w] = 0;
-	GalerkinTChunks< Dim , CLS >( Mf , col0 , lim , have , wave , K , fres , cres , buf , acc , std::make_integer_sequence< int , NCH >{} );
+	GalerkinTChunks< Dim , CLS >( Mf , fMf , col0 , lim , have , wave , K , fres , cres , buf , acc , std::make_integer_sequence< int , NCH >{} );
 	if( !have ) return;
 #pragma unroll
 	for( int w=0 ; w<27 ; w++ ) T[ ( 27*wave + w )*Nf + colBase + g ] = acc[w];
 }
 template< int Dim , int ... Cs >
-__device__ __forceinline__ void GalerkinTDispatch( int cls , const double *Mf , double *T , long long Nf , int cres , long long colBase , unsigned b , double *buf , std::integer_sequence< int , Cs... > )
+__device__ __forceinline__ void GalerkinTDispatch( int cls , const double *Mf , float *fMf , double *T , long long Nf , int cres , long long colBase , unsigned b , double *buf , std::integer_sequence< int , Cs... > )
 {
 	( [&]{
 		if( cls!=Cs ) return;
 		ColourSet< Dim > cs ; cs.set( 2*cres , Cs );
-		if( (long long)b*64<cs.total ) GalerkinTBlock< Dim , Cs >( Mf , T , Nf , cres , cs , colBase , b , buf );   // block-uniform
+		if( (long long)b*64<cs.total ) GalerkinTBlock< Dim , Cs >( Mf , fMf , T , Nf , cres , cs , colBase , b , buf );   // block-uniform
 	}() , ... );
 }
 // T = P^T M of a fine level (Nf functions): blockIdx.y is the parity class of
-// the fine nodes; 64 x 3^(Dim-3) threads
+// the fine nodes; 64 x 3^(Dim-3) threads.  fMf (optional): M rounded to float
+// (the FP32 sweeps' copy, written as M passes through)
 template< int Dim >
-__global__ void __launch_bounds__( Dim==3 ? 64 : 192 ) KGalerkinT( const double *Mf , double *T , long long Nf , int cres , ColLayout< Dim > lay )
+__global__ void __launch_bounds__( Dim==3 ? 64 : 192 ) KGalerkinT( const double *Mf , double *T , long long Nf , int cres , ColLayout< Dim > lay , float *fMf )
 {
 	__shared__ double buf[64*27];
 	const int cls = blockIdx.y;
-	GalerkinTDispatch< Dim >( cls , Mf , T , Nf , cres , lay.cstart[cls] , blockIdx.x , buf , std::make_integer_sequence< int , DimC< Dim >::C >{} );
+	GalerkinTDispatch< Dim >( cls , Mf , fMf , T , Nf , cres , lay.cstart[cls] , blockIdx.x , buf , std::make_integer_sequence< int , DimC< Dim >::C >{} );
 }
 
 // M_c = T P: entry (J, I = J + o) sums T[K][I] P[K][J] over the fine nodes
